@@ -1,0 +1,549 @@
+"""fortfem_amd -- MI355X (gfx950) drop-in for fortfem's sparse-assembly +
+Jacobi-PCG path.
+
+The compute lives in ``libfortfem_hip.so`` (hand-written HIP kernels behind the
+C-ABI of ``include/fortfem_hip.h``).  This module is the Python host side: it
+mirrors the reference's procedure names, argument meaning and status
+behaviour so that callers (and the parity tests) read like fortfem code:
+
+=====================================  ==============================================
+reference (lazy-fortran/fortfem)        here
+=====================================  ==============================================
+generate_structured_tetra_box_mesh      :func:`generate_structured_tetra_box_mesh`
+  (src/mesh/structured_tetra_box_mesh.f90:12-70)
+unit_square_mesh (fortfem_api_mesh.f90)  :func:`unit_square_mesh`
+assemble_tetra_lagrange_stiffness_csc   :func:`assemble_tetra_lagrange_stiffness_csc`
+  (assembly_tetra_lagrange_arbitrary_order_3d.f90:276-334)
+assemble_tetra_lagrange_scalar_load     :func:`assemble_tetra_lagrange_scalar_load`
+  (:705-774) / _load_samples (:473-528)  :func:`assemble_tetra_lagrange_scalar_load_samples`
+solver_options / solver_options_t       :func:`solver_options`
+  (advanced_solvers.f90:32-44,1430-1465)
+solve_sparse (advanced_solvers.f90:135)  :func:`solve_sparse`
+spmv (fortfem_sparse_matrix.f90:46-58)   :func:`spmv`
+solve_tetra_lagrange_poisson            :func:`solve_tetra_lagrange_poisson`
+  (fortfem_tetra_lagrange_solver_3d.f90:50-129)
+=====================================  ==============================================
+
+Failures are reported like the reference: assembly routines return a
+``FortsparseStatus`` whose ``code`` is nonzero on failure (the reference presets
+FORTSPARSE_INVALID_MATRIX and clears it on success), and programmer errors
+(``error stop`` in the reference, e.g. size mismatches in ``spmv`` and
+``solve_sparse``) raise.  A missing native library or GPU raises: there is no
+CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _capi as capi
+from ._capi import FFHError
+
+FORTSPARSE_INVALID_MATRIX = 2  # status preset of the reference assemblers
+
+
+@dataclass
+class FortsparseStatus:
+    code: int = 0
+    msg: str = ""
+
+
+@dataclass
+class CSC:
+    """fortsparse csc_t: 1-based col_ptr(n+1), row_idx(nnz), val(nnz)."""
+
+    nrow: int
+    ncol: int
+    col_ptr: np.ndarray
+    row_idx: np.ndarray
+    val: np.ndarray
+
+    @property
+    def nnz(self) -> int:
+        return int(self.row_idx.shape[0])
+
+    def to_dense(self) -> np.ndarray:
+        a = np.zeros((self.nrow, self.ncol))
+        cols = np.repeat(np.arange(self.ncol), np.diff(self.col_ptr))
+        np.add.at(a, (self.row_idx - 1, cols), self.val)
+        return a
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _i32(a):
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+def _ptr(a):
+    return a.ctypes.data_as(C.c_void_p) if a is not None else None
+
+
+class Context:
+    """One device + stream (ffh_context)."""
+
+    def __init__(self, device: int = 0):
+        self._lib = capi.load()
+        h = C.c_void_p()
+        capi.check(self._lib.ffh_init(device, C.byref(h)))
+        self.handle = h
+
+    def close(self):
+        if getattr(self, "handle", None):
+            self._lib.ffh_finalize(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def check(self, code):
+        capi.check(code, self.handle)
+
+    def set_profiling(self, enable: bool):
+        self.check(self._lib.ffh_set_profiling(self.handle, 1 if enable else 0))
+
+    def last_solve_profile(self, which: int = 0):
+        cnt = C.c_int(0)
+        ms = C.c_double(0.0)
+        self.check(self._lib.ffh_last_solve_profile(self.handle, which, C.byref(cnt), C.byref(ms)))
+        return cnt.value, ms.value
+
+
+_DEFAULT: Context | None = None
+
+
+def default_context() -> Context:
+    global _DEFAULT
+    if _DEFAULT is None:
+        _DEFAULT = Context(0)
+    return _DEFAULT
+
+
+# ------------------------------------------------------------------- options
+@dataclass
+class SolverOptionsT:
+    """solver_options_t (advanced_solvers.f90:32-44) plus this library's fields."""
+
+    method: str = "auto"
+    tolerance: float = 1.0e-6
+    max_iterations: int = 1000
+    preconditioner: str = "none"
+    tolerance_type: str = "relative"
+    precision: str = "fp64"          # "fp64" | "mixed"
+    breakdown: str = "absolute"      # "absolute" (reference) | "relative"
+    inner_tolerance: float = 1.0e-4
+    max_refinements: int = 50
+    check_interval: int = 0
+    use_graph: bool = False
+
+    def to_c(self) -> capi.SolverOptions:
+        o = capi.SolverOptions()
+        capi.load().ffh_solver_options_default(C.byref(o))
+        m = self.method.strip().lower()
+        if m not in ("auto", "cg", "pcg"):
+            m = "pcg"  # solve_sparse default branch (:184-187)
+        o.method = 0 if m == "cg" else 1
+        o.preconditioner = 1 if self.preconditioner.strip().lower() == "jacobi" else 0
+        o.tolerance_type = 1 if self.tolerance_type.strip().lower() == "absolute" else 0
+        o.max_iterations = int(self.max_iterations)
+        o.tolerance = float(self.tolerance)
+        o.precision = capi.MIXED if self.precision == "mixed" else capi.FP64
+        o.breakdown_mode = 1 if self.breakdown == "relative" else 0
+        o.inner_tolerance = float(self.inner_tolerance)
+        o.max_refinements = int(self.max_refinements)
+        o.check_interval = int(self.check_interval)
+        o.use_graph = 1 if self.use_graph else 0
+        return o
+
+
+def solver_options(method=None, tolerance=None, max_iterations=None, preconditioner=None,
+                   tolerance_type=None, **extra) -> SolverOptionsT:
+    """solver_options() constructor (advanced_solvers.f90:1430-1465)."""
+    o = SolverOptionsT()
+    if method is not None:
+        o.method = method
+    if tolerance is not None:
+        o.tolerance = tolerance
+    if max_iterations is not None:
+        o.max_iterations = max_iterations
+    if preconditioner is not None:
+        o.preconditioner = preconditioner
+    if tolerance_type is not None:
+        o.tolerance_type = tolerance_type
+    for k, v in extra.items():
+        if not hasattr(o, k):
+            raise TypeError(f"unknown solver option {k}")
+        setattr(o, k, v)
+    return o
+
+
+@dataclass
+class SolverStatsT:
+    """solver_stats_t (advanced_solvers.f90:47-57) plus this library's fields."""
+
+    converged: bool = False
+    iterations: int = 0
+    final_residual: float = 0.0
+    solve_time: float = 0.0
+    method_used: str = ""
+    refinements: int = 0
+    true_residual: float = 0.0
+    initial_residual: float = 0.0
+
+    @classmethod
+    def from_c(cls, s: capi.SolverStats, method: str):
+        return cls(bool(s.converged), int(s.iterations), float(s.final_residual),
+                   float(s.solve_time), method, int(s.refinements), float(s.true_residual),
+                   float(s.initial_residual))
+
+
+# -------------------------------------------------------------------- meshes
+def generate_structured_tetra_box_mesh(bounds, counts):
+    """bounds = [[xlo, ylo, zlo], [xhi, yhi, zhi]] (bounds(3,2)); returns
+    (vertices (nv,3), tetrahedra (T,4) 1-based, status)."""
+    lib = capi.load()
+    b = _f64(np.asarray(bounds, dtype=np.float64).reshape(2, 3).reshape(-1))
+    c = _i32(np.asarray(counts).reshape(3))
+    nv = C.c_int(0)
+    nt = C.c_int(0)
+    st = lib.ffh_box_mesh_size(_ptr(c), C.byref(nv), C.byref(nt))
+    if st != 0:
+        return np.zeros((0, 3)), np.zeros((0, 4), dtype=np.int32), 1
+    v = np.zeros((nv.value, 3))
+    t = np.zeros((nt.value, 4), dtype=np.int32)
+    st = lib.ffh_box_mesh(_ptr(b), _ptr(c), _ptr(v), _ptr(t))
+    return v, t, (0 if st == 0 else 1)
+
+
+def unit_box_mesh(n):
+    v, t, st = generate_structured_tetra_box_mesh([[0, 0, 0], [1, 1, 1]], [n, n, n])
+    if st:
+        raise ValueError("invalid box mesh")
+    return v, t
+
+
+def unit_square_mesh(n):
+    """unit_square_mesh(n): (vertices (n^2,2), triangles (2(n-1)^2,3) 1-based, is_boundary)."""
+    lib = capi.load()
+    v = np.zeros((n * n, 2))
+    t = np.zeros((2 * (n - 1) ** 2, 3), dtype=np.int32)
+    b = np.zeros(n * n, dtype=np.int32)
+    capi.check(lib.ffh_unit_square_mesh(n, _ptr(v), _ptr(t), _ptr(b)))
+    return v, t, b.astype(bool)
+
+
+# ------------------------------------------------------------------- problem
+class Problem:
+    """A mesh + discrete space + operator resident in HBM (ffh_problem)."""
+
+    def __init__(self, vertices, cells, ctx: Context | None = None):
+        self.ctx = ctx or default_context()
+        self._lib = capi.load()
+        v = _f64(vertices)
+        c = _i32(cells)
+        self.dim = v.shape[1]
+        h = C.c_void_p()
+        self.ctx.check(self._lib.ffh_problem_create(self.ctx.handle, self.dim, v.shape[0], _ptr(v),
+                                                    c.shape[0], _ptr(c), C.byref(h)))
+        self.handle = h
+        self.ncell = c.shape[0]
+
+    def close(self):
+        if getattr(self, "handle", None):
+            self._lib.ffh_problem_free(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, code):
+        self.ctx.check(code)
+
+    def pattern_build(self, family: int, order: int):
+        n = C.c_int(0)
+        nnz = C.c_int64(0)
+        self._check(self._lib.ffh_pattern_build(self.handle, family, order, C.byref(n), C.byref(nnz)))
+        return n.value, nnz.value
+
+    def sizes(self):
+        vals = [C.c_int(0), C.c_int64(0), C.c_int(0), C.c_int64(0), C.c_int(0), C.c_int(0)]
+        self._check(self._lib.ffh_problem_sizes(self.handle, *(C.byref(x) for x in vals)))
+        keys = ("n_global", "nnz", "n_free", "nnz_free", "n_local", "n_colors")
+        return {k: x.value for k, x in zip(keys, vals)}
+
+    def global_dofs(self, base=1):
+        s = self.sizes()
+        g = np.zeros((self.ncell, s["n_local"]), dtype=np.int32)
+        self._check(self._lib.ffh_global_dofs_download(self.handle, base, _ptr(g)))
+        return g
+
+    def assemble(self, quadrature_degree, kappa=1.0, mass=0.0, cell_kappa=None, precision="fp64",
+                 reuse_coefficients=False):
+        prec = capi.FP32 if precision == "fp32" else capi.FP64
+        if cell_kappa is not None or reuse_coefficients:
+            ck = None if reuse_coefficients else _f64(cell_kappa)
+            self._check(self._lib.ffh_assemble(self.handle, quadrature_degree, capi.COEFF_CELL,
+                                               _ptr(ck), float(mass), prec))
+        else:
+            k = _f64([kappa])
+            self._check(self._lib.ffh_assemble(self.handle, quadrature_degree, capi.COEFF_SCALAR,
+                                               _ptr(k), float(mass), prec))
+
+    def load(self, quadrature_degree, kind, params):
+        p = _f64(np.atleast_1d(params))
+        self._check(self._lib.ffh_load(self.handle, quadrature_degree, kind, _ptr(p), p.size))
+
+    def dirichlet(self, mode=0, values=None):
+        vv = None if values is None else _f64(values)
+        self._check(self._lib.ffh_dirichlet(self.handle, mode, _ptr(vv)))
+
+    def constrained(self):
+        s = self.sizes()
+        c = np.zeros(s["n_global"], dtype=np.int32)
+        self._check(self._lib.ffh_constrained_download(self.handle, _ptr(c)))
+        return c.astype(bool)
+
+    def set_rhs(self, rhs, which=capi.REDUCED):
+        r = _f64(rhs)
+        self._check(self._lib.ffh_set_rhs(self.handle, which, _ptr(r)))
+
+    def rhs(self, which=capi.FULL):
+        s = self.sizes()
+        n = s["n_global"] if which == capi.FULL else s["n_free"]
+        r = np.zeros(n)
+        self._check(self._lib.ffh_rhs_download(self.handle, which, _ptr(r)))
+        return r
+
+    def matrix(self, which=capi.FULL) -> "Matrix":
+        h = C.c_void_p()
+        self._check(self._lib.ffh_problem_matrix(self.handle, which, C.byref(h)))
+        return Matrix(h, self.ctx, borrowed=True, owner=self)
+
+    def solve(self, opts: SolverOptionsT, download=True):
+        s = self.sizes()
+        x = np.zeros(s["n_global"]) if download else None
+        st = capi.SolverStats()
+        self._check(self._lib.ffh_problem_solve(self.handle, C.byref(opts.to_c()), _ptr(x), C.byref(st)))
+        method = "cg" if opts.method.strip().lower() == "cg" else "pcg"
+        return x, SolverStatsT.from_c(st, method)
+
+
+class Matrix:
+    """A CSC matrix resident in HBM (ffh_matrix)."""
+
+    def __init__(self, handle, ctx: Context, borrowed=False, owner=None):
+        self.handle = handle
+        self.ctx = ctx
+        self.borrowed = borrowed
+        self._owner = owner
+        self._lib = capi.load()
+
+    @classmethod
+    def upload(cls, a: CSC, ctx: Context | None = None, base: int = 1):
+        ctx = ctx or default_context()
+        lib = capi.load()
+        h = C.c_void_p()
+        cp, ri, va = _i32(a.col_ptr), _i32(a.row_idx), _f64(a.val)
+        ctx.check(lib.ffh_matrix_upload(ctx.handle, a.ncol, a.nnz, base, _ptr(cp), _ptr(ri), _ptr(va),
+                                        C.byref(h)))
+        return cls(h, ctx)
+
+    def close(self):
+        if getattr(self, "handle", None) and not self.borrowed:
+            self._lib.ffh_matrix_free(self.handle)
+        self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def size(self):
+        n = C.c_int(0)
+        nnz = C.c_int64(0)
+        code = self._lib.ffh_matrix_download(self.handle, 1, 0, None, None, None, C.byref(n), C.byref(nnz))
+        if code not in (capi.FFH_OK, capi.FFH_ERR_CAPACITY):
+            self.ctx.check(code)
+        return n.value, nnz.value
+
+    def download(self, base=1) -> CSC:
+        n, nnz = self.size()
+        cp = np.zeros(n + 1, dtype=np.int32)
+        ri = np.zeros(nnz, dtype=np.int32)
+        va = np.zeros(nnz)
+        nn = C.c_int(0)
+        kk = C.c_int64(0)
+        self.ctx.check(self._lib.ffh_matrix_download(self.handle, base, nnz, _ptr(cp), _ptr(ri), _ptr(va),
+                                                     C.byref(nn), C.byref(kk)))
+        return CSC(n, n, cp, ri, va)
+
+    def spmv(self, x):
+        n, _ = self.size()
+        xx = _f64(x)
+        if xx.shape[0] != n:
+            raise ValueError("spmv: vector has incompatible size")
+        y = np.zeros(n)
+        self.ctx.check(self._lib.ffh_spmv(self.handle, _ptr(xx), _ptr(y)))
+        return y
+
+    def bench_spmv(self, precision="fp64", reps=50):
+        ms = C.c_double(0)
+        by = C.c_double(0)
+        prec = capi.FP32 if precision == "fp32" else capi.FP64
+        self.ctx.check(self._lib.ffh_bench_spmv(self.handle, prec, reps, C.byref(ms), C.byref(by)))
+        return ms.value, by.value
+
+
+# ----------------------------------------------------- reference-named calls
+def spmv(matrix: CSC, vector, ctx: Context | None = None):
+    """spmv (fortfem_sparse_matrix.f90:46-58): size mismatches raise (error stop)."""
+    v = _f64(vector)
+    if v.shape[0] != matrix.ncol:
+        raise ValueError("spmv: vector has incompatible size")
+    return Matrix.upload(matrix, ctx).spmv(v)
+
+
+def solve_sparse(a_sparse: CSC, b, x, opts: SolverOptionsT, ctx: Context | None = None):
+    """solve_sparse (advanced_solvers.f90:135-195).  x is updated in place and
+    the stats are returned.  Non-square or size mismatches raise (error stop)."""
+    if a_sparse.ncol != a_sparse.nrow:
+        raise ValueError("solve_sparse: only square matrices are supported")
+    bb = _f64(b)
+    if bb.shape[0] != a_sparse.nrow or np.asarray(x).shape[0] != a_sparse.nrow:
+        raise ValueError("solve_sparse: incompatible vector size")
+    m = Matrix.upload(a_sparse, ctx)
+    xx = _f64(x).copy()
+    st = capi.SolverStats()
+    m.ctx.check(m._lib.ffh_solve_sparse(m.handle, _ptr(bb), _ptr(xx), C.byref(opts.to_c()), C.byref(st)))
+    x[...] = xx
+    method = "cg" if opts.method.strip().lower() == "cg" else "pcg"
+    return SolverStatsT.from_c(st, method)
+
+
+def assemble_tetra_lagrange_stiffness_csc(mesh_vertices, tetrahedra, degree, quadrature_degree,
+                                          stiffness_coefficient=None, mass_coefficient=None,
+                                          cell_coefficient=None, ctx: Context | None = None):
+    """assemble_tetra_lagrange_stiffness_csc (:276-334) -> (CSC, status).
+    cell_coefficient: optional per-cell stiffness coefficient (new argument)."""
+    status = FortsparseStatus(FORTSPARSE_INVALID_MATRIX, "Tetrahedral H1 sparse assembly failed")
+    v = np.asarray(mesh_vertices)
+    t = np.asarray(tetrahedra)
+    if v.ndim != 2 or v.shape[1] != 3 or t.ndim != 2 or t.shape[1] != 4 or t.shape[0] < 1:
+        return None, status
+    if t.min() < 1 or t.max() > v.shape[0]:
+        return None, status
+    try:
+        p = Problem(v, t, ctx)
+        p.pattern_build(capi.FAMILY_H1, degree)
+        p.assemble(quadrature_degree, kappa=1.0 if stiffness_coefficient is None else stiffness_coefficient,
+                   mass=0.0 if mass_coefficient is None else mass_coefficient, cell_kappa=cell_coefficient)
+        a = p.matrix(capi.FULL).download()
+    except FFHError as e:
+        if e.code == capi.FFH_ERR_DEVICE:
+            raise
+        return None, status
+    return a, FortsparseStatus(0, "")
+
+
+def _physical_points(vertices, tets, quadrature_degree):
+    from ._quadrature import tetra_duffy
+
+    x, y, z, _ = tetra_duffy(quadrature_degree)
+    v = np.asarray(vertices, dtype=np.float64)
+    t = np.asarray(tets) - 1
+    v0 = v[t[:, 0]]
+    J = np.stack([v[t[:, 1]] - v0, v[t[:, 2]] - v0, v[t[:, 3]] - v0], axis=2)  # (T,3,3) cols
+    xi = np.stack([x, y, z], axis=0)  # (3,nq)
+    return v0[:, None, :] + np.einsum("tij,jq->tqi", J, xi)
+
+
+def assemble_tetra_lagrange_scalar_load_samples(mesh_vertices, tetrahedra, degree, quadrature_degree,
+                                                source_values, ctx: Context | None = None):
+    """assemble_tetra_lagrange_scalar_load_samples (:473-528);
+    source_values(nq, T) column-major == array (T, nq) here. -> (rhs, status)."""
+    status = FortsparseStatus(FORTSPARSE_INVALID_MATRIX, "Sampled tetrahedral H1 scalar load failed")
+    try:
+        p = Problem(mesh_vertices, tetrahedra, ctx)
+        p.pattern_build(capi.FAMILY_H1, degree)
+        p.load(quadrature_degree, capi.SOURCE_SAMPLES, _f64(source_values).reshape(-1))
+        return p.rhs(capi.FULL), FortsparseStatus(0, "")
+    except FFHError as e:
+        if e.code == capi.FFH_ERR_DEVICE:
+            raise
+        return None, status
+
+
+def assemble_tetra_lagrange_scalar_load(mesh_vertices, tetrahedra, degree, quadrature_degree, source,
+                                        ctx: Context | None = None):
+    """assemble_tetra_lagrange_scalar_load (:705-774) with a vectorised
+    source(x, y, z); the callback is sampled at the mapped Duffy points."""
+    pts = _physical_points(mesh_vertices, tetrahedra, quadrature_degree)
+    samples = source(pts[..., 0], pts[..., 1], pts[..., 2])
+    return assemble_tetra_lagrange_scalar_load_samples(mesh_vertices, tetrahedra, degree,
+                                                       quadrature_degree, samples, ctx)
+
+
+def solve_tetra_lagrange_poisson(vertices, tetrahedra, degree, source, boundary_value=None,
+                                 opts: SolverOptionsT | None = None, ctx: Context | None = None,
+                                 cell_coefficient=None):
+    """solve_tetra_lagrange_poisson (fortfem_tetra_lagrange_solver_3d.f90:50-62)
+    with the sparse direct solve replaced by Jacobi-PCG.  boundary_value:
+    callable g(x, y, z) evaluated at the constrained nodes, or None for g = 0.
+    -> (solution, status, stats)."""
+    status = FortsparseStatus(FORTSPARSE_INVALID_MATRIX, "Tetrahedral H1 diffusion-reaction solve failed")
+    if degree < 1 or degree > 4:
+        return None, status, None
+    q = 2 * degree + 4
+    p = Problem(vertices, tetrahedra, ctx)
+    try:
+        p.pattern_build(capi.FAMILY_H1, degree)
+        p.assemble(q, kappa=1.0, cell_kappa=cell_coefficient)
+        pts = _physical_points(vertices, tetrahedra, q)
+        p.load(q, capi.SOURCE_SAMPLES, _f64(source(pts[..., 0], pts[..., 1], pts[..., 2])).reshape(-1))
+        values = None
+        if boundary_value is not None:
+            values = _node_values(vertices, tetrahedra, degree, p.global_dofs(base=0), boundary_value)
+        p.dirichlet(0, values)
+        o = opts or solver_options(method="pcg", preconditioner="jacobi", tolerance=1e-10,
+                                   max_iterations=100000, breakdown="relative")
+        x, st = p.solve(o)
+    except FFHError as e:
+        if e.code == capi.FFH_ERR_DEVICE:
+            raise
+        return None, status, None
+    return x, FortsparseStatus(0, ""), st
+
+
+def _node_values(vertices, tets, degree, gd0, g):
+    """g at the Lagrange node of every dof (v_1 + J node, as build_dirichlet_data)."""
+    from ._quadrature import lagrange_nodes
+
+    nodes = lagrange_nodes(degree)  # (n,3)
+    v = np.asarray(vertices, dtype=np.float64)
+    t = np.asarray(tets) - 1
+    v0 = v[t[:, 0]]
+    J = np.stack([v[t[:, 1]] - v0, v[t[:, 2]] - v0, v[t[:, 3]] - v0], axis=2)
+    pts = v0[:, None, :] + np.einsum("tij,nj->tni", J, nodes)
+    out = np.zeros(int(gd0.max()) + 1)
+    out[gd0.reshape(-1)] = g(pts[..., 0], pts[..., 1], pts[..., 2]).reshape(-1)
+    return out
+
+
+__all__ = [
+    "CSC", "Context", "FFHError", "FortsparseStatus", "Matrix", "Problem", "SolverOptionsT", "SolverStatsT",
+    "assemble_tetra_lagrange_scalar_load", "assemble_tetra_lagrange_scalar_load_samples",
+    "assemble_tetra_lagrange_stiffness_csc", "capi", "default_context", "generate_structured_tetra_box_mesh",
+    "solve_sparse", "solve_tetra_lagrange_poisson", "solver_options", "spmv", "unit_box_mesh",
+    "unit_square_mesh",
+]

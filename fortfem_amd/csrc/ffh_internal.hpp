@@ -1,0 +1,226 @@
+// ffh_internal.hpp -- internal types of libfortfem_hip.so (not installed).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <array>
+#include <cstdint>
+#include <functional>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/fortfem_hip.h"
+
+namespace ffh {
+
+// ---------------------------------------------------------------- errors --
+struct Error {
+    int code;
+    std::string msg;
+};
+
+#define FFH_HIP(call)                                                        \
+    do {                                                                     \
+        hipError_t e_ = (call);                                              \
+        if (e_ != hipSuccess)                                                \
+            throw ::ffh::Error{FFH_ERR_DEVICE,                               \
+                               std::string(#call) + ": " +                   \
+                                   hipGetErrorString(e_)};                   \
+    } while (0)
+
+#define FFH_REQUIRE(cond, code, message)                                     \
+    do {                                                                     \
+        if (!(cond)) throw ::ffh::Error{(code), (message)};                  \
+    } while (0)
+
+// Device buffer owned by RAII; hipFree on destruction.
+template <typename T>
+struct DBuf {
+    T *p = nullptr;
+    size_t n = 0;
+    DBuf() = default;
+    DBuf(const DBuf &) = delete;
+    DBuf &operator=(const DBuf &) = delete;
+    ~DBuf() { reset(); }
+    void reset()
+    {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    void alloc(size_t count)
+    {
+        if (count == n && p) return;
+        reset();
+        if (count == 0) return;
+        FFH_HIP(hipMalloc(&p, count * sizeof(T)));
+        n = count;
+    }
+    void upload(const T *h, size_t count, hipStream_t s)
+    {
+        alloc(count);
+        if (count)
+            FFH_HIP(hipMemcpyAsync(p, h, count * sizeof(T),
+                                   hipMemcpyHostToDevice, s));
+    }
+    void upload(const std::vector<T> &h, hipStream_t s)
+    {
+        upload(h.data(), h.size(), s);
+    }
+    void download(T *h, size_t count, hipStream_t s) const
+    {
+        if (count)
+            FFH_HIP(hipMemcpyAsync(h, p, count * sizeof(T),
+                                   hipMemcpyDeviceToHost, s));
+    }
+};
+
+// Control block of one Krylov solve, resident in device memory.  Written
+// only by the last block of each reduction kernel.
+struct KrylovCtl {
+    double rz_old, pap, alpha, beta, rr, rz;
+    double residual_norm, initial_norm, tol;
+    int iter, max_iter, converged, done, breakdown_hit;
+    int breakdown_mode, tol_type, pad0;
+    double tolerance;
+    unsigned int counter[4];
+    int error;
+    int pad1;
+};
+
+// ---------------------------------------------------------------- matrix --
+// A CSC matrix in HBM (0-based).  Products are row gathers: for an exactly
+// symmetric matrix the CSC arrays are read as CSR; otherwise g_* hold the
+// transposed copy.
+struct DevMatrix {
+    int n = 0;
+    int64_t nnz = 0;
+    DBuf<int> col_ptr, row_idx;
+    DBuf<double> val;
+    DBuf<float> val32;
+    bool symmetric = true;
+    DBuf<int> t_ptr, t_idx;
+    DBuf<double> t_val;
+    DBuf<float> t_val32;
+    const int *gptr() const { return symmetric ? col_ptr.p : t_ptr.p; }
+    const int *gidx() const { return symmetric ? row_idx.p : t_idx.p; }
+    const double *gval() const { return symmetric ? val.p : t_val.p; }
+    const float *gval32() const { return symmetric ? val32.p : t_val32.p; }
+    bool val32_valid = false;
+    double avg_row = 0.0;
+};
+
+// Solver workspace attached to a matrix (kept resident between solves).
+struct Workspace {
+    int n = 0;
+    DBuf<double> r, p, ap, x, b, dinv;
+    DBuf<float> r32, p32, ap32, x32, dinv32;
+    DBuf<double> partials;
+    DBuf<KrylovCtl> ctl;
+    KrylovCtl *host_ctl = nullptr; // pinned mailbox
+    hipGraphExec_t graph[2] = {nullptr, nullptr};
+    int graph_chunk[2] = {0, 0};
+    ~Workspace();
+};
+
+struct Context;
+
+struct Matrix {
+    Context *ctx = nullptr;
+    DevMatrix m;
+    Workspace ws;
+    bool borrowed = false; // owned by a problem
+};
+
+struct Context {
+    int device = 0;
+    int cu_count = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    hipEvent_t ev_start = nullptr, ev_stop = nullptr;
+    bool profiling = false;
+    std::vector<hipEvent_t> prof_ev[2];
+    int prof_used[2] = {0, 0};
+    double prof_avg[2] = {0.0, 0.0};
+    int prof_count[2] = {0, 0};
+    void prof_begin();
+    void prof_mark(int which, bool start);
+    void prof_end();
+};
+
+// --------------------------------------------------------------- problem --
+struct Problem {
+    Context *ctx = nullptr;
+    int dim = 3;
+    int nvert = 0, ncell = 0;
+    std::vector<double> vertices; // 3*nv (or 2*nv)
+    std::vector<int> cells;       // 4*T (or 3*T), 1-based
+    DBuf<double> d_vertices;
+    int family = 0, order = 0;
+    int n_local = 0;
+    int n_global = 0;
+    std::vector<int> global_dofs; // n_local*T, 0-based
+    // colouring
+    int n_colors = 0;
+    std::vector<int> color_start; // n_colors+1 positions
+    std::vector<int> perm;        // pos -> cell
+    DBuf<int> d_perm;
+    DBuf<int> d_cells_c;   // colour-sorted cell vertex ids (0-based), nv_cell*T
+    DBuf<int> d_gdofs_c;   // [l][pos], 0-based dof ids
+    DBuf<int> d_slots;     // [r*n+c][pos] value slots
+    DBuf<signed char> d_signs; // Nedelec k=1 edge orientation [l][pos]
+    // full matrix + reduced matrix
+    Matrix full, reduced;
+    DBuf<double> rhs_full, rhs_red;
+    bool have_dirichlet = false;
+    std::vector<int> constrained;  // n_global flags
+    std::vector<int> free_dofs;    // 0-based
+    DBuf<int> d_free_dofs;
+    DBuf<int> d_red_orig;           // reduced entry -> full entry
+    DBuf<int> d_lift_ptr, d_lift_k, d_lift_c; // per free row lift entries
+    DBuf<double> d_bvals;           // boundary values (n_global)
+    int dirichlet_mode = 0;
+    // element tables
+    DBuf<double> d_tab64;
+    DBuf<float> d_tab32;
+    int tab_qdeg = -1, tab_nq = 0;
+    DBuf<double> d_load_tab; // per-point xi (3) + w*phi (n_local)
+    int load_qdeg = -1, load_nq = 0;
+    DBuf<int> d_err;
+    DBuf<double> d_coeff;
+    int coeff_kind = -1;
+    DBuf<double> x_red, x_full, d_samples;
+    bool red_dirty = true;
+    bool assembled = false, loaded = false;
+    // P1_2D symmetric elimination (mode 1)
+    std::vector<int> boundary2d;
+};
+
+// host-side symbolic routines (ffh_host.cpp)
+void box_mesh(const double bounds[6], const int counts[3], double *vertices,
+              int *tets);
+void unit_square_mesh(int n, double *vertices, int *tris, int *boundary);
+void gauss_legendre01(int n, double *x, double *w);
+int tetra_duffy(int p, std::vector<double> &x, std::vector<double> &y,
+                std::vector<double> &z, std::vector<double> &w);
+int lagrange_dof_count(int d);
+void lagrange_indices(int d, std::vector<int> &bary, std::vector<double> &nodes);
+void lagrange_eval(int d, const std::vector<int> &bary, const double *pt,
+                   double *values, double *grads);
+int lagrange_dof_map(int d, int ntet, const int *tets, std::vector<int> &gd,
+                     int &count);
+void build_pattern(int n_global, int ncell, int nloc,
+                   const std::vector<int> &gd, std::vector<int> &col_ptr,
+                   std::vector<int> &row_idx);
+int color_cells(int n_global, int ncell, int nloc, const std::vector<int> &gd,
+                std::vector<int> &color_start, std::vector<int> &perm);
+void build_slots(int ncell, int nloc, const std::vector<int> &gd,
+                 const std::vector<int> &perm, const std::vector<int> &col_ptr,
+                 const std::vector<int> &row_idx, std::vector<int> &slots);
+void lagrange_boundary(int d, int ntet, const int *tets,
+                       const std::vector<int> &gd, int n_global,
+                       std::vector<int> &constrained);
+void parallel_for(int64_t n, const std::function<void(int64_t, int64_t)> &f);
+
+} // namespace ffh

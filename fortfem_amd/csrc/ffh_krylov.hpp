@@ -1,0 +1,31 @@
+// ffh_krylov.hpp -- solver entry points shared by the C-ABI translation units.
+#pragma once
+#include "ffh_internal.hpp"
+
+namespace ffh {
+void ensure_workspace(Context *ctx, Matrix &M);
+void ensure_val32(Context *ctx, DevMatrix &A);
+void invalidate_graphs(Workspace &w);
+int spmv_group(const DevMatrix &A);
+void spmv64(Context *ctx, const DevMatrix &A, const double *x, double *y);
+void spmv32(Context *ctx, DevMatrix &A, const float *x, float *y);
+void solve_fp64(Context *ctx, Matrix &M, const double *b, double *x,
+                const ffh_solver_options &o, ffh_solver_stats &st);
+void solve_mixed(Context *ctx, Matrix &M, const double *b, double *x,
+                 const ffh_solver_options &o, ffh_solver_stats &st);
+double bench_spmv(Context *ctx, Matrix &M, int precision, int reps,
+                  double *bytes);
+} // namespace ffh
+
+namespace ffh {
+// problems (ffh_problem.hip)
+void problem_pattern(Problem *P, int family, int order);
+void problem_assemble(Problem *P, int qdeg, int coeff_kind, const double *coeff,
+                      double mass, int precision);
+void problem_load(Problem *P, int qdeg, int kind, const double *params,
+                  int64_t np);
+void problem_dirichlet(Problem *P, int mode, const double *values);
+void problem_reduce(Problem *P);
+void problem_solve(Problem *P, const ffh_solver_options &o, double *solution,
+                   ffh_solver_stats &st);
+} // namespace ffh

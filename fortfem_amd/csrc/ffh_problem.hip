@@ -1,0 +1,831 @@
+// ffh_problem.hip -- problems resident in HBM: symbolic stage, element
+// kernels with the coloured atomic-free CSC scatter, loads, Dirichlet
+// reduction and the constrained solve.
+//
+// Reference: src/assembly/assembly_tetra_lagrange_arbitrary_order_3d.f90
+// (element :40-105, CSC :276-334, load :473-528 / :705-774),
+// src/solvers/fortfem_tetra_lagrange_solver_3d.f90 (:64-129, boundary
+// :551-610), src/solvers/fortfem_sparse_direct.f90 (:264-344).
+//
+// Element kernels use the reference-tensor form of the element integral:
+// for affine cells the reference's quadrature sum
+//   K_rc = sum_q det w_q kappa (J^-T g_r(q)).(J^-T g_c(q))
+// equals kappa det sum_ab (J^-1 J^-T)_ab S^ab_rc with
+// S^ab_rc = sum_q w_q d_a phi_r(q) d_b phi_c(q), tabulated once in FP64 with
+// the same Duffy rule.  Only the six distinct entries of the symmetric
+// G = J^-1 J^-T enter, so an element costs 6 n(n+1)/2 multiply-adds instead of
+// the reference's n_q n^2 (3x3 matvecs).  Jacobian, determinant and inverse
+// are always FP64.  The upper triangle is computed once and written to both
+// (r,c) and (c,r) slots, so the assembled matrix is exactly symmetric and
+// products can read the CSC arrays as CSR.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+#include <unordered_map>
+
+#include "ffh_internal.hpp"
+#include "ffh_krylov.hpp"
+
+namespace ffh {
+
+constexpr int kBlockA = 128;
+
+// ---------------------------------------------------------------- kernels --
+__device__ __forceinline__ bool tet_geometry(const double *__restrict__ verts, int v0,
+                                             int v1, int v2, int v3, double jac[9],
+                                             double inv[9], double &det)
+{
+    const int id[4] = {v0, v1, v2, v3};
+    double x0 = verts[3 * (size_t)v0], y0 = verts[3 * (size_t)v0 + 1],
+           z0 = verts[3 * (size_t)v0 + 2];
+    for (int c = 0; c < 3; ++c) {
+        const double *v = verts + 3 * (size_t)id[c + 1];
+        jac[0 + 3 * c] = v[0] - x0;
+        jac[1 + 3 * c] = v[1] - y0;
+        jac[2 + 3 * c] = v[2] - z0;
+    }
+#define J(r, c) jac[(r) + 3 * (c)]
+    det = J(0, 0) * (J(1, 1) * J(2, 2) - J(1, 2) * J(2, 1)) -
+          J(0, 1) * (J(1, 0) * J(2, 2) - J(1, 2) * J(2, 0)) +
+          J(0, 2) * (J(1, 0) * J(2, 1) - J(1, 1) * J(2, 0));
+    double mx = 0.0;
+    for (int i = 0; i < 9; ++i) mx = fmax(mx, fabs(jac[i]));
+    double s = fmax(1.0, mx);
+    // degenerate guard of the reference element (:65-66)
+    if (!(det > 64.0 * 2.220446049250313e-16 * s * s * s)) return false;
+    double ri = 1.0 / det;
+    inv[0 + 3 * 0] = (J(1, 1) * J(2, 2) - J(1, 2) * J(2, 1)) * ri;
+    inv[0 + 3 * 1] = (J(0, 2) * J(2, 1) - J(0, 1) * J(2, 2)) * ri;
+    inv[0 + 3 * 2] = (J(0, 1) * J(1, 2) - J(0, 2) * J(1, 1)) * ri;
+    inv[1 + 3 * 0] = (J(1, 2) * J(2, 0) - J(1, 0) * J(2, 2)) * ri;
+    inv[1 + 3 * 1] = (J(0, 0) * J(2, 2) - J(0, 2) * J(2, 0)) * ri;
+    inv[1 + 3 * 2] = (J(0, 2) * J(1, 0) - J(0, 0) * J(1, 2)) * ri;
+    inv[2 + 3 * 0] = (J(1, 0) * J(2, 1) - J(1, 1) * J(2, 0)) * ri;
+    inv[2 + 3 * 1] = (J(0, 1) * J(2, 0) - J(0, 0) * J(2, 1)) * ri;
+    inv[2 + 3 * 2] = (J(0, 0) * J(1, 1) - J(0, 1) * J(1, 0)) * ri;
+#undef J
+    return true;
+}
+
+// One thread per cell of one colour.  tab = 7 tables of NL x NL:
+// S00, S11, S22, S01+S10, S02+S20, S12+S21, mass.
+template <int NL, typename CT, typename OT>
+__global__ __launch_bounds__(kBlockA) void k_h1_cells(
+    int pos0, int pos1, int ncell, const int *__restrict__ cells_c,
+    const double *__restrict__ verts, const int *__restrict__ perm,
+    const double *__restrict__ kappa_cell, double kappa, double mu,
+    const CT *__restrict__ tab, const int *__restrict__ slots, OT *__restrict__ val,
+    int *__restrict__ err)
+{
+    const int pos = pos0 + blockIdx.x * kBlockA + threadIdx.x;
+    if (pos >= pos1) return;
+    const int4 c = reinterpret_cast<const int4 *>(cells_c)[pos];
+    double jac[9], inv[9], det;
+    if (!tet_geometry(verts, c.x, c.y, c.z, c.w, jac, inv, det)) {
+        atomicOr(err, 1);
+        return;
+    }
+    // G = J^-1 J^-T (symmetric)
+    double g[6];
+    {
+        const int a_[6] = {0, 1, 2, 0, 0, 1}, b_[6] = {0, 1, 2, 1, 2, 2};
+#pragma unroll
+        for (int k = 0; k < 6; ++k) {
+            const int a = a_[k], b = b_[k];
+            g[k] = inv[a + 0] * inv[b + 0] + inv[a + 3] * inv[b + 3] +
+                   inv[a + 6] * inv[b + 6];
+        }
+    }
+    const double ke = kappa_cell ? kappa_cell[perm[pos]] : kappa;
+    const CT kd = (CT)(ke * det), md = (CT)(mu * det);
+    CT gg[6];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) gg[k] = (CT)g[k];
+    constexpr int NN = NL * NL;
+    const size_t T = (size_t)ncell;
+    for (int r = 0; r < NL; ++r) {
+#pragma unroll 2
+        for (int cc = r; cc < NL; ++cc) {
+            const int rc = r * NL + cc;
+            CT s = gg[0] * tab[rc] + gg[1] * tab[NN + rc] + gg[2] * tab[2 * NN + rc] +
+                   gg[3] * tab[3 * NN + rc] + gg[4] * tab[4 * NN + rc] +
+                   gg[5] * tab[5 * NN + rc];
+            CT v = kd * s;
+            if (mu != 0.0) v += md * tab[6 * NN + rc];
+            const int s0 = slots[(size_t)rc * T + pos];
+            val[s0] += (OT)v;
+            if (cc != r) {
+                const int s1 = slots[(size_t)(cc * NL + r) * T + pos];
+                val[s1] += (OT)v;
+            }
+        }
+    }
+}
+
+// 2D P1 (assemble_scalar_p1_matrix, src/fortfem_forms_simple.f90:1678-1728):
+// local = kappa area G^T G + mu area (1 + I)/12.
+__global__ __launch_bounds__(kBlockA) void k_p1_2d_cells(
+    int pos0, int pos1, int ncell, const int *__restrict__ cells_c,
+    const double *__restrict__ verts, const int *__restrict__ perm,
+    const double *__restrict__ kappa_cell, double kappa, double mu,
+    const int *__restrict__ slots, double *__restrict__ val, int *__restrict__ err)
+{
+    const int pos = pos0 + blockIdx.x * kBlockA + threadIdx.x;
+    if (pos >= pos1) return;
+    const int *c = cells_c + 3 * (size_t)pos;
+    double x[3], y[3];
+    for (int k = 0; k < 3; ++k) {
+        x[k] = verts[2 * (size_t)c[k]];
+        y[k] = verts[2 * (size_t)c[k] + 1];
+    }
+    const double det = (x[1] - x[0]) * (y[2] - y[0]) - (x[2] - x[0]) * (y[1] - y[0]);
+    if (!(fabs(det) > 64.0 * 2.220446049250313e-16)) {
+        atomicOr(err, 1);
+        return;
+    }
+    const double area = 0.5 * fabs(det);
+    const double gx[3] = {(y[1] - y[2]) / det, (y[2] - y[0]) / det, (y[0] - y[1]) / det};
+    const double gy[3] = {(x[2] - x[1]) / det, (x[0] - x[2]) / det, (x[1] - x[0]) / det};
+    const double ke = kappa_cell ? kappa_cell[perm[pos]] : kappa;
+    const size_t T = (size_t)ncell;
+    for (int a = 0; a < 3; ++a)
+        for (int b = a; b < 3; ++b) {
+            double v = ke * area * (gx[a] * gx[b] + gy[a] * gy[b]) +
+                       mu * area * (a == b ? 2.0 : 1.0) / 12.0;
+            val[slots[(size_t)(a * 3 + b) * T + pos]] += v;
+            if (a != b) val[slots[(size_t)(b * 3 + a) * T + pos]] += v;
+        }
+}
+
+// Load: b_g += det sum_q w_q f(x_q) phi_l(q); tab per point = xi(3), w phi(NL).
+template <int NL>
+__global__ __launch_bounds__(kBlockA) void k_h1_load(
+    int pos0, int pos1, int ncell, const int *__restrict__ cells_c,
+    const double *__restrict__ verts, const int *__restrict__ perm,
+    const int *__restrict__ gdofs_c, const double *__restrict__ tab, int nq,
+    int kind, double p0, double p1, double p2, double p3,
+    const double *__restrict__ samples, double *__restrict__ rhs, int *__restrict__ err)
+{
+    const int pos = pos0 + blockIdx.x * kBlockA + threadIdx.x;
+    if (pos >= pos1) return;
+    const int4 c = reinterpret_cast<const int4 *>(cells_c)[pos];
+    double jac[9], inv[9], det;
+    if (!tet_geometry(verts, c.x, c.y, c.z, c.w, jac, inv, det)) {
+        atomicOr(err, 1);
+        return;
+    }
+    const double *v0 = verts + 3 * (size_t)c.x;
+    double acc[NL];
+#pragma unroll
+    for (int l = 0; l < NL; ++l) acc[l] = 0.0;
+    const int e = perm[pos];
+    for (int q = 0; q < nq; ++q) {
+        const double *t = tab + (size_t)q * (3 + NL);
+        double f;
+        if (kind == FFH_SOURCE_CONSTANT) {
+            f = p0;
+        } else if (kind == FFH_SOURCE_SIN_PRODUCT) {
+            double x = v0[0] + (jac[0] * t[0] + jac[3] * t[1] + jac[6] * t[2]);
+            double y = v0[1] + (jac[1] * t[0] + jac[4] * t[1] + jac[7] * t[2]);
+            double z = v0[2] + (jac[2] * t[0] + jac[5] * t[1] + jac[8] * t[2]);
+            f = p0 * sin(p1 * x) * sin(p2 * y) * sin(p3 * z);
+        } else {
+            f = samples[(size_t)nq * e + q];
+        }
+#pragma unroll
+        for (int l = 0; l < NL; ++l) acc[l] += f * t[3 + l];
+    }
+    const size_t T = (size_t)ncell;
+#pragma unroll
+    for (int l = 0; l < NL; ++l) rhs[gdofs_c[(size_t)l * T + pos]] += det * acc[l];
+}
+
+__global__ void k_widen(int64_t n, const float *__restrict__ a, double *__restrict__ b)
+{
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        b[i] = (double)a[i];
+}
+
+__global__ void k_gather_vals(int64_t n, const int *__restrict__ orig,
+                              const double *__restrict__ src, double *__restrict__ dst)
+{
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        dst[i] = src[orig[i]];
+}
+
+// reduced_rhs(f) = rhs(free(f)) - sum_c A(free(f), c) g(c), c ascending
+// (sparse_direct_solve_constrained_real, :327-337)
+__global__ void k_reduce_rhs(int nf, const int *__restrict__ free_dofs,
+                             const double *__restrict__ rhs,
+                             const int *__restrict__ lift_ptr,
+                             const int *__restrict__ lift_k,
+                             const int *__restrict__ lift_c,
+                             const double *__restrict__ val,
+                             const double *__restrict__ g, double *__restrict__ out)
+{
+    for (int f = blockIdx.x * blockDim.x + threadIdx.x; f < nf;
+         f += gridDim.x * blockDim.x) {
+        double b = rhs[free_dofs[f]];
+        if (g)
+            for (int k = lift_ptr[f]; k < lift_ptr[f + 1]; ++k)
+                b = b - val[lift_k[k]] * g[lift_c[k]];
+        out[f] = b;
+    }
+}
+
+__global__ void k_scatter_solution(int n, int nf, const int *__restrict__ free_dofs,
+                                   const double *__restrict__ xr,
+                                   const double *__restrict__ g, double *__restrict__ x)
+{
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += gridDim.x * blockDim.x)
+        x[i] = g ? g[i] : 0.0;
+}
+
+__global__ void k_scatter_free(int nf, const int *__restrict__ free_dofs,
+                               const double *__restrict__ xr, double *__restrict__ x)
+{
+    for (int f = blockIdx.x * blockDim.x + threadIdx.x; f < nf;
+         f += gridDim.x * blockDim.x)
+        x[free_dofs[f]] = xr[f];
+}
+
+// symmetric elimination of solve_compiled_p1_problem
+// (src/solvers/fortfem_solvers_laplacian.f90:88-97), value part: rows and
+// columns of boundary dofs become identity, rhs lifted by the column.
+__global__ void k_eliminate(int n, const int *__restrict__ ptr,
+                            const int *__restrict__ idx, double *__restrict__ val,
+                            const int *__restrict__ bnd, double g,
+                            const double *__restrict__ rhs_in, double *__restrict__ rhs)
+{
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += gridDim.x * blockDim.x) {
+        // row i: b_i - sum over boundary columns d (ascending) of a_id g
+        double b = rhs_in[i];
+        if (bnd[i]) {
+            b = g;
+        } else {
+            for (int k = ptr[i]; k < ptr[i + 1]; ++k)
+                if (bnd[idx[k]]) b = b - val[k] * g;
+        }
+        rhs[i] = b;
+    }
+}
+
+__global__ void k_eliminate_vals(int n, const int *__restrict__ ptr,
+                                 const int *__restrict__ idx, double *__restrict__ val,
+                                 const int *__restrict__ bnd)
+{
+    for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < n;
+         j += gridDim.x * blockDim.x)
+        for (int k = ptr[j]; k < ptr[j + 1]; ++k) {
+            int i = idx[k];
+            if (bnd[i] || bnd[j]) val[k] = (i == j) ? 1.0 : 0.0;
+        }
+}
+
+// ---------------------------------------------------------------- helpers --
+static int grid1(const Context *ctx, int64_t n, int block = 256)
+{
+    int64_t need = (n + block - 1) / block;
+    return (int)std::max<int64_t>(1, std::min<int64_t>(need, (int64_t)ctx->cu_count * 16));
+}
+
+// S tables of the reference-tensor form with the reference's Duffy rule
+static void h1_tables(int d, int qdeg, std::vector<double> &tab)
+{
+    std::vector<double> x, y, z, w;
+    tetra_duffy(qdeg, x, y, z, w);
+    std::vector<int> bary;
+    std::vector<double> nodes;
+    lagrange_indices(d, bary, nodes);
+    const int n = lagrange_dof_count(d), NN = n * n;
+    tab.assign(7 * NN, 0.0);
+    std::vector<double> val(n), grad(3 * n);
+    const int pa[6] = {0, 1, 2, 0, 0, 1}, pb[6] = {0, 1, 2, 1, 2, 2};
+    for (size_t q = 0; q < w.size(); ++q) {
+        double pt[3] = {x[q], y[q], z[q]};
+        lagrange_eval(d, bary, pt, val.data(), grad.data());
+        for (int r = 0; r < n; ++r)
+            for (int c = 0; c < n; ++c) {
+                const int rc = r * n + c;
+                for (int k = 0; k < 6; ++k) {
+                    const int a = pa[k], b = pb[k];
+                    double s = grad[3 * r + a] * grad[3 * c + b];
+                    if (a != b) s += grad[3 * r + b] * grad[3 * c + a];
+                    tab[k * NN + rc] += w[q] * s;
+                }
+                tab[6 * NN + rc] += w[q] * val[r] * val[c];
+            }
+    }
+}
+
+static void h1_load_table(int d, int qdeg, std::vector<double> &tab, int &nq)
+{
+    std::vector<double> x, y, z, w;
+    tetra_duffy(qdeg, x, y, z, w);
+    std::vector<int> bary;
+    std::vector<double> nodes;
+    lagrange_indices(d, bary, nodes);
+    const int n = lagrange_dof_count(d);
+    nq = (int)w.size();
+    tab.assign((size_t)nq * (3 + n), 0.0);
+    std::vector<double> val(n), grad(3 * n);
+    for (int q = 0; q < nq; ++q) {
+        double pt[3] = {x[q], y[q], z[q]};
+        lagrange_eval(d, bary, pt, val.data(), grad.data());
+        double *t = tab.data() + (size_t)q * (3 + n);
+        t[0] = x[q], t[1] = y[q], t[2] = z[q];
+        for (int l = 0; l < n; ++l) t[3 + l] = w[q] * val[l];
+    }
+}
+
+#define FFH_NL_DISPATCH(NLv, ...)                                             \
+    switch (NLv) {                                                           \
+    case 4: { constexpr int NL = 4; __VA_ARGS__; } break;                     \
+    case 10: { constexpr int NL = 10; __VA_ARGS__; } break;                   \
+    case 20: { constexpr int NL = 20; __VA_ARGS__; } break;                   \
+    case 35: { constexpr int NL = 35; __VA_ARGS__; } break;                   \
+    default: throw Error{FFH_ERR_INVALID, "unsupported local dof count"};    \
+    }
+
+static void check_err(Problem *P, const char *what)
+{
+    int h = 0;
+    P->d_err.download(&h, 1, P->ctx->stream);
+    FFH_HIP(hipStreamSynchronize(P->ctx->stream));
+    if (h) {
+        FFH_HIP(hipMemsetAsync(P->d_err.p, 0, sizeof(int), P->ctx->stream));
+        throw Error{FFH_ERR_INVALID, std::string(what) + ": degenerate cell"};
+    }
+}
+
+// -------------------------------------------------------------- symbolic --
+void problem_pattern(Problem *P, int family, int order)
+{
+    Context *ctx = P->ctx;
+    const int T = P->ncell;
+    std::vector<int> gd;
+    int G = 0;
+    int nl = 0;
+    if (family == FFH_FAMILY_H1) {
+        FFH_REQUIRE(P->dim == 3, FFH_ERR_INVALID, "H1 family needs a tetrahedral mesh");
+        FFH_REQUIRE(order >= 1 && order <= 4, FFH_ERR_INVALID, "H1 degree must be 1..4");
+        for (size_t i = 0; i < P->cells.size(); ++i)
+            FFH_REQUIRE(P->cells[i] >= 1 && P->cells[i] <= P->nvert, FFH_ERR_INVALID,
+                        "cell vertex id out of range");
+        FFH_REQUIRE(lagrange_dof_map(order, T, P->cells.data(), gd, G) == 0,
+                    FFH_ERR_INVALID, "invalid tetrahedra for the dof map");
+        nl = lagrange_dof_count(order);
+    } else if (family == FFH_FAMILY_P1_2D) {
+        FFH_REQUIRE(P->dim == 2, FFH_ERR_INVALID, "P1_2D family needs a triangle mesh");
+        nl = 3;
+        gd.resize((size_t)3 * T);
+        for (size_t i = 0; i < gd.size(); ++i) {
+            FFH_REQUIRE(P->cells[i] >= 1 && P->cells[i] <= P->nvert, FFH_ERR_INVALID,
+                        "triangle vertex id out of range");
+            gd[i] = P->cells[i] - 1;
+        }
+        G = P->nvert; // dof = vertex index (mesh_2d)
+    } else {
+        throw Error{FFH_ERR_INVALID, "unsupported element family"};
+    }
+    P->family = family;
+    P->order = order;
+    P->n_local = nl;
+    P->n_global = G;
+    std::vector<int> col_ptr, row_idx;
+    build_pattern(G, T, nl, gd, col_ptr, row_idx);
+    std::vector<int> cstart, perm;
+    P->n_colors = color_cells(G, T, nl, gd, cstart, perm);
+    P->color_start = cstart;
+    P->perm = perm;
+    std::vector<int> slots;
+    build_slots(T, nl, gd, perm, col_ptr, row_idx, slots);
+    const int vpc = P->dim == 3 ? 4 : 3;
+    std::vector<int> cells_c((size_t)vpc * T), gdofs_c((size_t)nl * T);
+    parallel_for(T, [&](int64_t lo, int64_t hi) {
+        for (int64_t pos = lo; pos < hi; ++pos) {
+            const int e = perm[pos];
+            for (int k = 0; k < vpc; ++k)
+                cells_c[(size_t)vpc * pos + k] = P->cells[(size_t)vpc * e + k] - 1;
+            for (int l = 0; l < nl; ++l)
+                gdofs_c[(size_t)l * T + pos] = gd[(size_t)nl * e + l];
+        }
+    });
+    hipStream_t s = ctx->stream;
+    P->d_perm.upload(perm, s);
+    P->d_cells_c.upload(cells_c, s);
+    P->d_gdofs_c.upload(gdofs_c, s);
+    P->d_slots.upload(slots, s);
+    DevMatrix &A = P->full.m;
+    A.n = G;
+    A.nnz = (int64_t)row_idx.size();
+    A.col_ptr.upload(col_ptr, s);
+    A.row_idx.upload(row_idx, s);
+    A.val.alloc(A.nnz);
+    FFH_HIP(hipMemsetAsync(A.val.p, 0, sizeof(double) * A.nnz, s));
+    A.symmetric = true;
+    A.val32_valid = false;
+    A.val32.reset();
+    P->full.ctx = ctx;
+    P->full.borrowed = true;
+    invalidate_graphs(P->full.ws);
+    P->rhs_full.alloc(G);
+    FFH_HIP(hipMemsetAsync(P->rhs_full.p, 0, sizeof(double) * G, s));
+    P->global_dofs.swap(gd);
+    P->d_err.alloc(1);
+    FFH_HIP(hipMemsetAsync(P->d_err.p, 0, sizeof(int), s));
+    P->tab_qdeg = -1;
+    P->load_qdeg = -1;
+    P->have_dirichlet = false;
+    P->red_dirty = true;
+    FFH_HIP(hipStreamSynchronize(s));
+}
+
+// ---------------------------------------------------------------- numeric --
+void problem_assemble(Problem *P, int qdeg, int coeff_kind, const double *coeff,
+                      double mass, int precision)
+{
+    Context *ctx = P->ctx;
+    hipStream_t s = ctx->stream;
+    FFH_REQUIRE(P->n_global > 0, FFH_ERR_INVALID, "pattern_build must come first");
+    const int T = P->ncell;
+    double kappa = 1.0;
+    const double *dk = nullptr;
+    if (coeff_kind == FFH_COEFF_SCALAR) {
+        if (coeff) kappa = coeff[0];
+    } else if (coeff_kind == FFH_COEFF_CELL) {
+        if (coeff) {
+            P->d_coeff.upload(coeff, (size_t)T, s);
+            P->coeff_kind = FFH_COEFF_CELL;
+        }
+        FFH_REQUIRE(P->coeff_kind == FFH_COEFF_CELL && P->d_coeff.p, FFH_ERR_INVALID,
+                    "per-cell coefficients were never uploaded");
+        dk = P->d_coeff.p;
+    } else {
+        throw Error{FFH_ERR_INVALID, "unsupported coefficient kind for this family"};
+    }
+    DevMatrix &A = P->full.m;
+    const int nc = P->n_colors;
+    if (P->family == FFH_FAMILY_P1_2D) {
+        FFH_HIP(hipMemsetAsync(A.val.p, 0, sizeof(double) * A.nnz, s));
+        for (int c = 0; c < nc; ++c) {
+            int p0 = P->color_start[c], p1 = P->color_start[c + 1];
+            int grid = (p1 - p0 + kBlockA - 1) / kBlockA;
+            if (grid == 0) continue;
+            hipLaunchKernelGGL(k_p1_2d_cells, dim3(grid), dim3(kBlockA), 0, s, p0, p1, T,
+                               P->d_cells_c.p, P->d_vertices.p, P->d_perm.p, dk, kappa,
+                               mass, P->d_slots.p, A.val.p, P->d_err.p);
+        }
+        FFH_HIP(hipGetLastError());
+        check_err(P, "2D P1 assembly");
+    } else {
+        FFH_REQUIRE(qdeg >= 0, FFH_ERR_INVALID, "quadrature degree must be >= 0");
+        if (P->tab_qdeg != qdeg) {
+            std::vector<double> tab;
+            h1_tables(P->order, qdeg, tab);
+            P->d_tab64.upload(tab, s);
+            std::vector<float> t32(tab.begin(), tab.end());
+            P->d_tab32.upload(t32, s);
+            P->tab_qdeg = qdeg;
+        }
+        const bool f32 = precision == FFH_FP32;
+        if (f32) {
+            A.val32.alloc(A.nnz);
+            FFH_HIP(hipMemsetAsync(A.val32.p, 0, sizeof(float) * A.nnz, s));
+        } else {
+            FFH_HIP(hipMemsetAsync(A.val.p, 0, sizeof(double) * A.nnz, s));
+        }
+        FFH_NL_DISPATCH(P->n_local, {
+            for (int c = 0; c < nc; ++c) {
+                int p0 = P->color_start[c], p1 = P->color_start[c + 1];
+                int grid = (p1 - p0 + kBlockA - 1) / kBlockA;
+                if (grid == 0) continue;
+                if (f32)
+                    hipLaunchKernelGGL((k_h1_cells<NL, float, float>), dim3(grid),
+                                       dim3(kBlockA), 0, s, p0, p1, T, P->d_cells_c.p,
+                                       P->d_vertices.p, P->d_perm.p, dk, kappa, mass,
+                                       P->d_tab32.p, P->d_slots.p, A.val32.p, P->d_err.p);
+                else
+                    hipLaunchKernelGGL((k_h1_cells<NL, double, double>), dim3(grid),
+                                       dim3(kBlockA), 0, s, p0, p1, T, P->d_cells_c.p,
+                                       P->d_vertices.p, P->d_perm.p, dk, kappa, mass,
+                                       P->d_tab64.p, P->d_slots.p, A.val.p, P->d_err.p);
+            }
+        });
+        FFH_HIP(hipGetLastError());
+        check_err(P, "tetrahedral H1 assembly");
+        if (f32) {
+            // the FP64 view of an FP32-contracted matrix, for export and checks
+            hipLaunchKernelGGL(k_widen, dim3(grid1(ctx, A.nnz)), dim3(256), 0, s, A.nnz,
+                               A.val32.p, A.val.p);
+            A.val32_valid = true;
+        } else {
+            A.val32_valid = false;
+        }
+    }
+    P->assembled = true;
+    P->red_dirty = true;
+}
+
+void problem_load(Problem *P, int qdeg, int kind, const double *params, int64_t np)
+{
+    Context *ctx = P->ctx;
+    hipStream_t s = ctx->stream;
+    FFH_REQUIRE(P->n_global > 0, FFH_ERR_INVALID, "pattern_build must come first");
+    const int T = P->ncell, G = P->n_global;
+    if (kind == FFH_SOURCE_DOF_VECTOR) {
+        FFH_REQUIRE(params && np == G, FFH_ERR_INVALID, "rhs vector must have n_global entries");
+        P->rhs_full.upload(params, G, s);
+        P->loaded = true;
+        P->red_dirty = true;
+        return;
+    }
+    FFH_REQUIRE(P->family == FFH_FAMILY_H1, FFH_ERR_INVALID,
+                "closed-form loads are implemented for the H1 family");
+    FFH_REQUIRE(qdeg >= 0, FFH_ERR_INVALID, "quadrature degree must be >= 0");
+    if (P->load_qdeg != qdeg) {
+        std::vector<double> tab;
+        int nq = 0;
+        h1_load_table(P->order, qdeg, tab, nq);
+        P->d_load_tab.upload(tab, s);
+        P->load_qdeg = qdeg;
+        P->load_nq = nq;
+    }
+    double p[4] = {0, 0, 0, 0};
+    const double *samples = nullptr;
+    if (kind == FFH_SOURCE_CONSTANT) {
+        FFH_REQUIRE(params && np >= 1, FFH_ERR_INVALID, "constant load needs params[0]");
+        p[0] = params[0];
+    } else if (kind == FFH_SOURCE_SIN_PRODUCT) {
+        FFH_REQUIRE(params && np >= 4, FFH_ERR_INVALID, "sin-product load needs 4 params");
+        for (int i = 0; i < 4; ++i) p[i] = params[i];
+    } else if (kind == FFH_SOURCE_SAMPLES) {
+        FFH_REQUIRE(params && np == (int64_t)P->load_nq * T, FFH_ERR_INVALID,
+                    "samples must be (nq, ncell)");
+        P->d_samples.upload(params, (size_t)np, s);
+        samples = P->d_samples.p;
+    } else {
+        throw Error{FFH_ERR_INVALID, "unknown source kind"};
+    }
+    FFH_HIP(hipMemsetAsync(P->rhs_full.p, 0, sizeof(double) * G, s));
+    FFH_NL_DISPATCH(P->n_local, {
+        for (int c = 0; c < P->n_colors; ++c) {
+            int p0 = P->color_start[c], p1 = P->color_start[c + 1];
+            int grid = (p1 - p0 + kBlockA - 1) / kBlockA;
+            if (grid == 0) continue;
+            hipLaunchKernelGGL((k_h1_load<NL>), dim3(grid), dim3(kBlockA), 0, s, p0, p1, T,
+                               P->d_cells_c.p, P->d_vertices.p, P->d_perm.p,
+                               P->d_gdofs_c.p, P->d_load_tab.p, P->load_nq, kind, p[0],
+                               p[1], p[2], p[3], samples, P->rhs_full.p, P->d_err.p);
+        }
+    });
+    FFH_HIP(hipGetLastError());
+    check_err(P, "tetrahedral H1 load");
+    P->loaded = true;
+    P->red_dirty = true;
+}
+
+// Dirichlet: symbolic reduction (host, once) + numeric (device)
+void problem_dirichlet(Problem *P, int mode, const double *values)
+{
+    Context *ctx = P->ctx;
+    hipStream_t s = ctx->stream;
+    FFH_REQUIRE(P->n_global > 0, FFH_ERR_INVALID, "pattern_build must come first");
+    const int G = P->n_global;
+    std::vector<int> cons;
+    if (P->family == FFH_FAMILY_H1) {
+        lagrange_boundary(P->order, P->ncell, P->cells.data(), P->global_dofs, G, cons);
+    } else if (P->family == FFH_FAMILY_P1_2D) {
+        // boundary vertices: endpoints of edges owned by one triangle
+        // (find_boundary, src/mesh/mesh_2d.f90:553-603)
+        std::unordered_map<uint64_t, int> edges;
+        edges.reserve((size_t)P->ncell * 3);
+        for (int t = 0; t < P->ncell; ++t)
+            for (int k = 0; k < 3; ++k) {
+                int a = P->cells[3 * t + k], b = P->cells[3 * t + (k + 1) % 3];
+                if (a > b) std::swap(a, b);
+                edges[((uint64_t)a << 32) | (uint32_t)b]++;
+            }
+        cons.assign(G, 0);
+        for (auto &kv : edges)
+            if (kv.second == 1) {
+                cons[(int)(kv.first >> 32) - 1] = 1;
+                cons[(int)(kv.first & 0xffffffffu) - 1] = 1;
+            }
+    } else {
+        throw Error{FFH_ERR_INVALID, "Dirichlet data not available for this family"};
+    }
+    P->constrained = cons;
+    P->dirichlet_mode = mode;
+    if (values) {
+        P->d_bvals.upload(values, G, s);
+    } else {
+        P->d_bvals.reset();
+    }
+    if (mode == 1) {
+        FFH_REQUIRE(P->family == FFH_FAMILY_P1_2D, FFH_ERR_INVALID,
+                    "symmetric elimination (mode 1) is the 2D P1 path");
+        P->have_dirichlet = true;
+        P->red_dirty = true;
+        return;
+    }
+    // free dofs ascending, reduced pattern keeps the original row order
+    std::vector<int> free_dofs, free_index(G, -1);
+    for (int i = 0; i < G; ++i)
+        if (!cons[i]) {
+            free_index[i] = (int)free_dofs.size();
+            free_dofs.push_back(i);
+        }
+    const int nf = (int)free_dofs.size();
+    std::vector<int> hcp(G + 1), hri(P->full.m.nnz);
+    P->full.m.col_ptr.download(hcp.data(), G + 1, s);
+    P->full.m.row_idx.download(hri.data(), hri.size(), s);
+    FFH_HIP(hipStreamSynchronize(s));
+    std::vector<int> cnt(nf, 0), lcnt(nf, 0);
+    parallel_for(nf, [&](int64_t lo, int64_t hi) {
+        for (int64_t f = lo; f < hi; ++f) {
+            int j = free_dofs[f], a = 0, l = 0;
+            for (int k = hcp[j]; k < hcp[j + 1]; ++k) {
+                if (cons[hri[k]]) ++l;
+                else ++a;
+            }
+            cnt[f] = a;
+            lcnt[f] = l;
+        }
+    });
+    std::vector<int> rcp(nf + 1, 0), lptr(nf + 1, 0);
+    for (int f = 0; f < nf; ++f) {
+        rcp[f + 1] = rcp[f] + cnt[f];
+        lptr[f + 1] = lptr[f] + lcnt[f];
+    }
+    std::vector<int> rri(rcp[nf]), orig(rcp[nf]), lk(lptr[nf]), lc(lptr[nf]);
+    parallel_for(nf, [&](int64_t lo, int64_t hi) {
+        for (int64_t f = lo; f < hi; ++f) {
+            int j = free_dofs[f], o = rcp[f], l = lptr[f];
+            for (int k = hcp[j]; k < hcp[j + 1]; ++k) {
+                int r = hri[k];
+                if (cons[r]) {
+                    // A(j, r) = A(r, j) (exactly symmetric assembly)
+                    lk[l] = k;
+                    lc[l] = r;
+                    ++l;
+                } else {
+                    rri[o] = free_index[r];
+                    orig[o] = k;
+                    ++o;
+                }
+            }
+        }
+    });
+    DevMatrix &R = P->reduced.m;
+    R.n = nf;
+    R.nnz = rcp[nf];
+    R.col_ptr.upload(rcp, s);
+    R.row_idx.upload(rri, s);
+    R.val.alloc(R.nnz);
+    R.symmetric = true;
+    R.val32_valid = false;
+    P->reduced.ctx = ctx;
+    P->reduced.borrowed = true;
+    invalidate_graphs(P->reduced.ws);
+    P->d_red_orig.upload(orig, s);
+    P->d_free_dofs.upload(free_dofs, s);
+    P->d_lift_ptr.upload(lptr, s);
+    P->d_lift_k.upload(lk, s);
+    P->d_lift_c.upload(lc, s);
+    P->rhs_red.alloc(nf);
+    P->free_dofs = free_dofs;
+    P->have_dirichlet = true;
+    P->red_dirty = true;
+    FFH_HIP(hipStreamSynchronize(s));
+}
+
+// numeric reduction of the current full matrix / rhs
+void problem_reduce(Problem *P)
+{
+    if (!P->have_dirichlet || !P->red_dirty) return;
+    Context *ctx = P->ctx;
+    hipStream_t s = ctx->stream;
+    if (P->dirichlet_mode == 1) {
+        // P1_2D symmetric elimination + drop |a| <= 1e-14 (sparse_from_dense,
+        // src/solvers/fortfem_sparse_matrix.f90:16-44)
+        DevMatrix &A = P->full.m;
+        const int n = A.n;
+        DBuf<int> bnd;
+        bnd.upload(P->constrained, s);
+        DBuf<double> vals;
+        vals.alloc(A.nnz);
+        FFH_HIP(hipMemcpyAsync(vals.p, A.val.p, sizeof(double) * A.nnz,
+                               hipMemcpyDeviceToDevice, s));
+        P->rhs_red.alloc(n);
+        double g = 0.0;
+        if (P->d_bvals.p) {
+            // constant boundary value of solve_compiled_p1_problem (bc%value)
+            std::vector<double> hv(n);
+            P->d_bvals.download(hv.data(), n, s);
+            FFH_HIP(hipStreamSynchronize(s));
+            for (int i = 0; i < n; ++i)
+                if (P->constrained[i]) {
+                    g = hv[i];
+                    break;
+                }
+        }
+        hipLaunchKernelGGL(k_eliminate, dim3(grid1(ctx, n)), dim3(256), 0, s, n,
+                           A.col_ptr.p, A.row_idx.p, vals.p, bnd.p, g, P->rhs_full.p,
+                           P->rhs_red.p);
+        hipLaunchKernelGGL(k_eliminate_vals, dim3(grid1(ctx, n)), dim3(256), 0, s, n,
+                           A.col_ptr.p, A.row_idx.p, vals.p, bnd.p);
+        FFH_HIP(hipGetLastError());
+        // drop pattern (value dependent): compacted on the host side of the
+        // symbolic stage from the device values
+        std::vector<int> cp(n + 1), ri(A.nnz);
+        std::vector<double> hv(A.nnz);
+        A.col_ptr.download(cp.data(), n + 1, s);
+        A.row_idx.download(ri.data(), A.nnz, s);
+        vals.download(hv.data(), A.nnz, s);
+        FFH_HIP(hipStreamSynchronize(s));
+        std::vector<int> rcp(n + 1, 0), rri, orig;
+        for (int j = 0; j < n; ++j) {
+            for (int k = cp[j]; k < cp[j + 1]; ++k)
+                if (std::fabs(hv[k]) > 1.0e-14) {
+                    rri.push_back(ri[k]);
+                    orig.push_back(k);
+                }
+            rcp[j + 1] = (int)rri.size();
+        }
+        DevMatrix &R = P->reduced.m;
+        R.n = n;
+        R.nnz = (int64_t)rri.size();
+        R.col_ptr.upload(rcp, s);
+        R.row_idx.upload(rri, s);
+        R.val.alloc(R.nnz);
+        R.symmetric = true;
+        R.val32_valid = false;
+        P->reduced.ctx = ctx;
+        P->reduced.borrowed = true;
+        invalidate_graphs(P->reduced.ws);
+        P->d_red_orig.upload(orig, s);
+        hipLaunchKernelGGL(k_gather_vals, dim3(grid1(ctx, R.nnz)), dim3(256), 0, s, R.nnz,
+                           P->d_red_orig.p, vals.p, R.val.p);
+        FFH_HIP(hipGetLastError());
+        FFH_HIP(hipStreamSynchronize(s));
+        P->red_dirty = false;
+        return;
+    }
+    DevMatrix &R = P->reduced.m;
+    hipLaunchKernelGGL(k_gather_vals, dim3(grid1(ctx, R.nnz)), dim3(256), 0, s, R.nnz,
+                       P->d_red_orig.p, P->full.m.val.p, R.val.p);
+    hipLaunchKernelGGL(k_reduce_rhs, dim3(grid1(ctx, R.n)), dim3(256), 0, s, R.n,
+                       P->d_free_dofs.p, P->rhs_full.p, P->d_lift_ptr.p, P->d_lift_k.p,
+                       P->d_lift_c.p, P->full.m.val.p, P->d_bvals.p, P->rhs_red.p);
+    FFH_HIP(hipGetLastError());
+    R.val32_valid = false;
+    P->red_dirty = false;
+}
+
+void problem_solve(Problem *P, const ffh_solver_options &o, double *solution,
+                   ffh_solver_stats &st)
+{
+    Context *ctx = P->ctx;
+    hipStream_t s = ctx->stream;
+    FFH_REQUIRE(P->assembled, FFH_ERR_INVALID, "assemble before solving");
+    const int G = P->n_global;
+    Matrix *M;
+    const double *b;
+    if (P->have_dirichlet) {
+        problem_reduce(P);
+        M = &P->reduced;
+        b = P->rhs_red.p;
+    } else {
+        M = &P->full;
+        b = P->rhs_full.p;
+    }
+    const int n = M->m.n;
+    P->x_red.alloc(std::max(n, 1));
+    FFH_HIP(hipMemsetAsync(P->x_red.p, 0, sizeof(double) * n, s));
+    if (o.precision == FFH_MIXED)
+        solve_mixed(ctx, *M, b, P->x_red.p, o, st);
+    else
+        solve_fp64(ctx, *M, b, P->x_red.p, o, st);
+    P->x_full.alloc(G);
+    if (P->have_dirichlet && P->dirichlet_mode == 0) {
+        hipLaunchKernelGGL(k_scatter_solution, dim3(grid1(ctx, G)), dim3(256), 0, s, G,
+                           (int)P->free_dofs.size(), P->d_free_dofs.p, P->x_red.p,
+                           P->d_bvals.p, P->x_full.p);
+        hipLaunchKernelGGL(k_scatter_free, dim3(grid1(ctx, n)), dim3(256), 0, s, n,
+                           P->d_free_dofs.p, P->x_red.p, P->x_full.p);
+        FFH_HIP(hipGetLastError());
+    } else {
+        FFH_HIP(hipMemcpyAsync(P->x_full.p, P->x_red.p, sizeof(double) * G,
+                               hipMemcpyDeviceToDevice, s));
+    }
+    if (solution) P->x_full.download(solution, G, s);
+    FFH_HIP(hipStreamSynchronize(s));
+}
+
+} // namespace ffh

@@ -1,0 +1,221 @@
+/*
+ * fortfem_hip.h -- C-ABI of libfortfem_hip.so, the MI355X (gfx950) drop-in
+ * for fortfem's sparse-assembly + Jacobi-PCG path.
+ *
+ * Plain C types only (int, int64_t, double, opaque handles), so Fortran binds
+ * it with ISO_C_BINDING and Python with ctypes.  Every entry point names the
+ * reference interface it replaces (lazy-fortran/fortfem, path:line).
+ *
+ * Status codes follow the reference C-API (include/fortfem.h:22-23,141-143):
+ *    0 ok, -1 invalid input, -2 capacity too small (sizes still returned),
+ *   -3 bad or exhausted handle; this library adds -4 device (HIP) error and
+ *   -5 admission rejected (serving).  ffh_last_error() gives the message.
+ *
+ * Index base: arrays that cross the boundary carry an explicit `base`
+ * argument (1 = Fortran/fortsparse, 0 = C), as in the reference's
+ * zero-based C export (src/bindings/fortfem_capi_mesh.f90:560-567).
+ *
+ * Threading: one ffh_context per host thread; every handle belongs to the
+ * context that created it and all device work of a context is ordered on its
+ * one HIP stream.
+ */
+#ifndef FORTFEM_HIP_H
+#define FORTFEM_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FFH_OK 0
+#define FFH_ERR_INVALID (-1)
+#define FFH_ERR_CAPACITY (-2)
+#define FFH_ERR_HANDLE (-3)
+#define FFH_ERR_DEVICE (-4)
+#define FFH_ERR_ADMISSION (-5)
+
+/* element families */
+#define FFH_FAMILY_H1 1      /* tetrahedral Lagrange, degree 1..4        */
+#define FFH_FAMILY_NEDELEC 2 /* tetrahedral first-kind Nedelec, k 1..6    */
+#define FFH_FAMILY_P1_2D 3   /* triangular P1 on a 2D mesh               */
+
+/* coefficient kinds for ffh_assemble */
+#define FFH_COEFF_SCALAR 0      /* coeff[0]                              */
+#define FFH_COEFF_CELL 1        /* coeff[T], one scalar per cell          */
+#define FFH_COEFF_CELL_TENSOR 2 /* coeff[9*T], column-major 3x3 per cell  */
+
+/* load kinds for ffh_load */
+#define FFH_SOURCE_CONSTANT 0    /* params[0] = f                          */
+#define FFH_SOURCE_SIN_PRODUCT 1 /* f = p0 sin(p1 x) sin(p2 y) sin(p3 z)    */
+#define FFH_SOURCE_SAMPLES 2     /* params = f(nq, T) at the Duffy points   */
+#define FFH_SOURCE_DOF_VECTOR 3  /* params = rhs(n_global) directly         */
+
+/* precision */
+#define FFH_FP64 0
+#define FFH_MIXED 1 /* FP32 matrix/vectors inside FP64 iterative refinement */
+#define FFH_FP32 1  /* element contractions in FP32 (ffh_assemble)        */
+
+/* which matrix / vector of a problem */
+#define FFH_FULL 0
+#define FFH_REDUCED 1
+
+typedef struct ffh_context ffh_context;
+typedef struct ffh_problem ffh_problem;
+typedef struct ffh_matrix ffh_matrix;
+
+/* Mirrors solver_options_t (src/solvers/advanced_solvers.f90:32-44) and the
+ * solver_options() defaults (:1430-1465): method 'auto'->'pcg', tolerance
+ * 1e-6, max_iterations 1000, preconditioner 'none', tolerance_type
+ * 'relative'.  Fields after max_iterations are additions of this library. */
+typedef struct ffh_solver_options {
+    int method;          /* 0 cg, 1 pcg ('auto' is pcg, :159-161)        */
+    int preconditioner;  /* 0 none, 1 jacobi                             */
+    int tolerance_type;  /* 0 relative (tol*||b-Ax0||), 1 absolute       */
+    int max_iterations;
+    double tolerance;
+    int precision;       /* FFH_FP64 or FFH_MIXED                          */
+    int breakdown_mode;  /* 0 reference |pAp| < 1e-14 (:897),
+                            1 scale-invariant |pAp| <= 1e-14 |rz|         */
+    double inner_tolerance; /* MIXED: relative tolerance per FP32 solve   */
+    int max_refinements;    /* MIXED: outer FP64 refinement steps cap    */
+    int check_interval;     /* iterations per host convergence poll, 0 auto */
+    int use_graph;          /* replay iteration chunks from a HIP graph   */
+} ffh_solver_options;
+
+/* Mirrors solver_stats_t (:47-57): converged, iterations, final_residual,
+ * solve_time; the rest are additions. */
+typedef struct ffh_solver_stats {
+    int converged;
+    int iterations;        /* Krylov iterations (MIXED: summed inner)    */
+    double final_residual; /* recursive ||r|| (FP64) / true ||b-Ax|| (MIXED) */
+    double solve_time;     /* seconds, HIP events on the context stream   */
+    int refinements;       /* MIXED: outer steps                          */
+    double true_residual;  /* ||b - A x||_2 recomputed in FP64 at exit    */
+    double initial_residual;
+} ffh_solver_stats;
+
+/* ---- context ------------------------------------------------------------ */
+/* No reference counterpart (the reference is host-only); owns the device,
+ * stream and events that every later call uses. */
+int ffh_init(int device, ffh_context **ctx);
+int ffh_finalize(ffh_context *ctx);
+const char *ffh_last_error(const ffh_context *ctx);
+/* fills the defaults of solver_options() (advanced_solvers.f90:1430-1465) */
+int ffh_solver_options_default(ffh_solver_options *opts);
+
+/* ---- meshes -------------------------------------------------------------- */
+/* generate_structured_tetra_box_mesh (src/mesh/structured_tetra_box_mesh.f90:
+ * 12-70): bounds(3,2) column-major, counts(3); vertices(3,(n+1)^3), 1-based
+ * tetrahedra(4, 6 n^3).  ffh_box_mesh_size gives the sizes. */
+int ffh_box_mesh_size(const int counts[3], int *nvert, int *ntet);
+int ffh_box_mesh(const double bounds[6], const int counts[3],
+                 double *vertices, int *tetrahedra);
+/* unit_square_mesh (src/fortfem_api_mesh.f90:128-139; mesh_2d.f90:90-148,
+ * 553-603): vertices(2,n^2), triangles(3,2(n-1)^2), boundary flags(n^2). */
+int ffh_unit_square_mesh(int n, double *vertices, int *triangles,
+                         int *is_boundary_vertex);
+
+/* ---- problems: mesh + space + operator resident in HBM -------------------- */
+/* Takes the (mesh_vertices, tetrahedra) arguments of
+ * assemble_tetra_lagrange_stiffness_csc (src/assembly/
+ * assembly_tetra_lagrange_arbitrary_order_3d.f90:276-285) and
+ * assemble_tetra_nedelec_curl_mass_csc (assembly_tetra_nedelec_3d.f90:
+ * 2042-2049); tetrahedra are 1-based.  dim = 3 (tets, 4 ids per cell) or
+ * 2 (triangles, 3 ids per cell, family FFH_FAMILY_P1_2D). */
+int ffh_problem_create(ffh_context *ctx, int dim, int nvert,
+                       const double *vertices, int ncell, const int *cells,
+                       ffh_problem **out);
+int ffh_problem_free(ffh_problem *prob);
+
+/* Symbolic stage shared by the CSC assemblers: global numbering of
+ * build_tetra_lagrange_dof_map (src/elements/tetra_lagrange_global_dof_map.f90:
+ * 13-79) or build_tetra_nedelec_dof_map (tetra_nedelec_global_dof_map.f90:
+ * 15-105), the CSC pattern that csc_from_triplet would produce (ascending
+ * rows, structural zeros kept), an element -> value-slot map and an element
+ * colouring in which no two cells of one colour share a dof. */
+int ffh_pattern_build(ffh_problem *prob, int family, int order,
+                      int *n_global, int64_t *nnz);
+int ffh_problem_sizes(const ffh_problem *prob, int *n_global, int64_t *nnz,
+                      int *n_free, int64_t *nnz_free, int *n_local,
+                      int *n_colors);
+/* global_dofs(n_local, ncell) as the reference dof map returns it */
+int ffh_global_dofs_download(ffh_problem *prob, int base, int *global_dofs);
+
+/* Numeric assembly into the fixed pattern: the element loop + triplets +
+ * csc_from_triplet of assemble_tetra_lagrange_stiffness_csc (:276-334, per
+ * cell kappa as the stiffness_coefficient of :47-48),
+ * assemble_tetra_nedelec_curl_mass_csc (:2042-2112; coeff = curl alpha,
+ * mass = beta) and assemble_tetra_nedelec_weighted_csc (:1870-2000; coeff =
+ * tensor K).  precision FFH_FP32 runs the element contraction in FP32. */
+int ffh_assemble(ffh_problem *prob, int quadrature_degree, int coeff_kind,
+                 const double *coeff, double mass_coefficient, int precision);
+
+/* assemble_tetra_lagrange_scalar_load (:705-774) / _load_samples (:473-528)
+ * with the source given as a closed form, samples or a dof vector. */
+int ffh_load(ffh_problem *prob, int quadrature_degree, int source_kind,
+             const double *params, int64_t nparams);
+
+/* build_dirichlet_data (src/solvers/fortfem_tetra_lagrange_solver_3d.f90:
+ * 551-610) for H1, the boundary-vertex set of find_boundary (mesh_2d.f90:
+ * 553-603) for P1_2D, then the free-dof reduction of
+ * sparse_direct_solve_constrained_real (src/solvers/fortfem_sparse_direct.f90:
+ * 264-337).  values: NULL for g = 0, else values(n_global) (only constrained
+ * entries are read).  For P1_2D, mode 1 applies the symmetric elimination of
+ * solve_compiled_p1_problem (fortfem_solvers_laplacian.f90:88-97) to the
+ * full matrix instead (identity rows, no reduction). */
+int ffh_dirichlet(ffh_problem *prob, int mode, const double *values);
+/* constrained flags (n_global), 1 = constrained */
+int ffh_constrained_download(ffh_problem *prob, int *constrained);
+/* rhs of the reduced system directly (free-dof order), e.g. random RHS */
+int ffh_set_rhs(ffh_problem *prob, int which, const double *rhs);
+int ffh_rhs_download(ffh_problem *prob, int which, double *rhs);
+
+/* borrowed handle of the problem's full or reduced matrix */
+int ffh_problem_matrix(ffh_problem *prob, int which, ffh_matrix **mat);
+
+/* Solve the reduced system and scatter back (sparse_direct_solve_constrained
+ * with sparse_solve_once replaced by solve_sparse's CG/PCG, :264-344):
+ * solution(n_global), constrained entries = boundary values. */
+int ffh_problem_solve(ffh_problem *prob, const ffh_solver_options *opts,
+                      double *solution, ffh_solver_stats *stats);
+
+/* ---- CSC matrices (fortsparse csc_t) -------------------------------------- */
+/* csc_t with col_ptr(n+1), row_idx(nnz), val(nnz) of the given base.  A
+ * matrix that is not exactly symmetric also gets a transposed copy so that
+ * products stay row-gathers. */
+int ffh_matrix_upload(ffh_context *ctx, int n, int64_t nnz, int base,
+                      const int *col_ptr, const int *row_idx,
+                      const double *val, ffh_matrix **out);
+int ffh_matrix_free(ffh_matrix *mat);
+/* export with the capacity protocol of export_real_csc
+ * (src/bindings/fortfem_capi_mesh.f90:545-571): -2 when capacity < nnz,
+ * n and nnz are still returned. */
+int ffh_matrix_download(ffh_matrix *mat, int base, int64_t capacity,
+                        int *col_ptr, int *row_idx, double *val, int *n,
+                        int64_t *nnz);
+/* spmv (src/solvers/fortfem_sparse_matrix.f90:46-58): y = A x, host arrays */
+int ffh_spmv(ffh_matrix *mat, const double *x, double *y);
+/* solve_sparse (src/solvers/advanced_solvers.f90:135-195): x is the initial
+ * guess on entry and the solution on exit (host arrays). */
+int ffh_solve_sparse(ffh_matrix *mat, const double *b, double *x,
+                     const ffh_solver_options *opts, ffh_solver_stats *stats);
+
+/* ---- measurement ---------------------------------------------------------- */
+/* Times `reps` back-to-back SpMV launches on the context stream with HIP
+ * events; precision FFH_FP64 or FFH_FP32.  avg_ms = mean per launch,
+ * bytes = algorithmic bytes of one launch. */
+int ffh_bench_spmv(ffh_matrix *mat, int precision, int reps, double *avg_ms,
+                   double *bytes);
+/* Kernel timing inside solves: when enabled, every SpMV launch of a solve
+ * (which 0 = FP64 SpMV+dot, 1 = FP32 SpMV+dot) is bracketed by HIP events on
+ * the context stream; ffh_last_solve_profile returns the launch count and
+ * mean duration (ms) of the last solve. */
+int ffh_set_profiling(ffh_context *ctx, int enable);
+int ffh_last_solve_profile(ffh_context *ctx, int which, int *count,
+                           double *avg_ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,396 @@
+"""FP64 CPU oracle for the fortfem sparse-assembly + Jacobi-PCG path.
+
+TEST INFRASTRUCTURE ONLY.  This package wraps ``oracle/libffo.so`` (a plain-C
+restatement of the reference algorithms; every C function cites the reference
+file:line it follows).  Only ``tests/``, ``__graft_entry__.smoke()`` and the
+``cpu_baseline`` leg of ``bench.py`` may import it, and only as the checker or
+the timed CPU baseline.  The product package ``fortfem_amd`` never imports it.
+
+Array conventions mirror the reference's Fortran storage: ``vertices`` is
+(nv, 3) C-order == vertices(3, nv) column-major, ``tetrahedra`` is (T, 4) with
+1-based ids, ``global_dofs`` is (T, n_loc) with 1-based ids, CSC arrays are
+1-based like fortsparse's ``csc_t``.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+_dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+_i64p = C.POINTER(C.c_int64)
+
+
+def lib_path() -> str:
+    return os.path.join(_HERE, "libffo.so")
+
+
+def build() -> None:
+    """Compile the oracle with its Makefile (gcc)."""
+    import subprocess
+
+    subprocess.run(["make", "-s", "-C", _HERE, "libffo.so"], check=True)
+
+
+def _lib():
+    global _LIB
+    if _LIB is None:
+        path = lib_path()
+        if not os.path.exists(path):
+            build()
+        _LIB = C.CDLL(path)
+        L = _LIB
+        L.ffo_det3.restype = C.c_double
+        L.ffo_det3.argtypes = [_dp]
+    return _LIB
+
+
+def _f(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _i(a):
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+def _check(status, what):
+    if status != 0:
+        raise ValueError(f"oracle {what} failed with status {status}")
+
+
+# ---------------------------------------------------------------- meshes ---
+def box_mesh(bounds, counts):
+    """generate_structured_tetra_box_mesh; bounds = [[lo..], [hi..]]."""
+    b = _f(np.asarray(bounds, dtype=np.float64).reshape(2, 3).reshape(-1))
+    c = _i(counts)
+    nv = int(np.prod(c + 1))
+    nt = 6 * int(np.prod(c))
+    v = np.zeros((nv, 3))
+    t = np.zeros((nt, 4), dtype=np.int32)
+    _check(_lib().ffo_box_mesh(b.ctypes.data_as(C.c_void_p), c.ctypes.data_as(C.c_void_p),
+                               v.ctypes.data_as(C.c_void_p), t.ctypes.data_as(C.c_void_p)),
+           "box_mesh")
+    return v, t
+
+
+def unit_box_mesh(n):
+    return box_mesh([[0.0, 0.0, 0.0], [1.0, 1.0, 1.0]], [n, n, n])
+
+
+def unit_square_mesh(n):
+    v = np.zeros((n * n, 2))
+    t = np.zeros((2 * (n - 1) ** 2, 3), dtype=np.int32)
+    bnd = np.zeros(n * n, dtype=np.int32)
+    _check(_lib().ffo_unit_square_mesh(C.c_int(n), v.ctypes.data_as(C.c_void_p),
+                                       t.ctypes.data_as(C.c_void_p),
+                                       bnd.ctypes.data_as(C.c_void_p)), "unit_square_mesh")
+    return v, t, bnd.astype(bool)
+
+
+# ------------------------------------------------------------ quadrature ---
+def gauss_legendre_ab(n, a, b):
+    x = np.zeros(n)
+    w = np.zeros(n)
+    _check(_lib().ffo_gauss_legendre_ab(C.c_int(n), C.c_double(a), C.c_double(b),
+                                        x.ctypes.data_as(C.c_void_p),
+                                        w.ctypes.data_as(C.c_void_p)), "gauss_legendre")
+    return x, w
+
+
+def tetra_duffy(p):
+    nq = _lib().ffo_tetra_duffy_count(C.c_int(p))
+    if nq <= 0:
+        raise ValueError("negative degree")
+    x, y, z, w = (np.zeros(nq) for _ in range(4))
+    _check(_lib().ffo_tetra_duffy(C.c_int(p), *(a.ctypes.data_as(C.c_void_p) for a in (x, y, z, w))),
+           "tetra_duffy")
+    return x, y, z, w
+
+
+def triangle_duffy(p):
+    nq = _lib().ffo_triangle_duffy_count(C.c_int(p))
+    if nq <= 0:
+        raise ValueError("negative degree")
+    x, y, w = (np.zeros(nq) for _ in range(3))
+    _check(_lib().ffo_triangle_duffy(C.c_int(p), *(a.ctypes.data_as(C.c_void_p) for a in (x, y, w))),
+           "triangle_duffy")
+    return x, y, w
+
+
+# ------------------------------------------------------------- Lagrange ---
+def lagrange_dof_count(d):
+    return _lib().ffo_lagrange_dof_count(C.c_int(d))
+
+
+def lagrange_init(d):
+    n = lagrange_dof_count(d)
+    bary = np.zeros((n, 4), dtype=np.int32)
+    nodes = np.zeros((n, 3))
+    _check(_lib().ffo_lagrange_init(C.c_int(d), bary.ctypes.data_as(C.c_void_p),
+                                    nodes.ctypes.data_as(C.c_void_p)), "lagrange_init")
+    return bary, nodes
+
+
+def lagrange_eval(d, point):
+    n = lagrange_dof_count(d)
+    p = _f(point)
+    vals = np.zeros(n)
+    grads = np.zeros((n, 3))
+    st = _lib().ffo_lagrange_eval(C.c_int(d), p.ctypes.data_as(C.c_void_p),
+                                  vals.ctypes.data_as(C.c_void_p), grads.ctypes.data_as(C.c_void_p))
+    return st, vals, grads
+
+
+def lagrange_dof_map(d, tets, search=False):
+    t = _i(tets)
+    n = max(lagrange_dof_count(d), 1)
+    g = np.zeros((t.shape[0], n), dtype=np.int32)
+    cnt = C.c_int(0)
+    fn = _lib().ffo_lagrange_dof_map_search if search else _lib().ffo_lagrange_dof_map
+    st = fn(C.c_int(d), C.c_int(t.shape[0]), t.ctypes.data_as(C.c_void_p),
+            g.ctypes.data_as(C.c_void_p), C.byref(cnt))
+    return st, g, cnt.value
+
+
+def lagrange_stiffness_element(vertices4, d, q, kappa=1.0, mu=0.0):
+    v = _f(vertices4)
+    n = lagrange_dof_count(d)
+    m = np.zeros((n, n))  # m[c, r] in C order == matrix(r, c) column-major
+    st = _lib().ffo_lagrange_stiffness_element(v.ctypes.data_as(C.c_void_p), C.c_int(d), C.c_int(q),
+                                               C.c_double(kappa), C.c_double(mu),
+                                               m.ctypes.data_as(C.c_void_p))
+    return st, m.T.copy()
+
+
+@dataclass
+class CSC:
+    n_rows: int
+    n_cols: int
+    col_ptr: np.ndarray  # 1-based int32
+    row_idx: np.ndarray  # 1-based int32
+    val: np.ndarray
+
+    @property
+    def nnz(self):
+        return int(self.row_idx.shape[0])
+
+    def to_dense(self):
+        a = np.zeros((self.n_rows, self.n_cols))
+        for c in range(self.n_cols):
+            lo, hi = self.col_ptr[c] - 1, self.col_ptr[c + 1] - 1
+            a[self.row_idx[lo:hi] - 1, c] += self.val[lo:hi]
+        return a
+
+    def matvec(self, x):
+        return csc_matvec(self, x)
+
+
+def csc_from_triplet(nrow, ncol, rows, cols, vals):
+    r, c, v = _i(rows), _i(cols), _f(vals)
+    nt = r.shape[0]
+    cp = np.zeros(ncol + 1, dtype=np.int32)
+    ri = np.zeros(max(nt, 1), dtype=np.int32)
+    va = np.zeros(max(nt, 1))
+    nnz = C.c_int64(0)
+    _check(_lib().ffo_csc_from_triplet(C.c_int(nrow), C.c_int(ncol), C.c_int64(nt),
+                                       r.ctypes.data_as(C.c_void_p), c.ctypes.data_as(C.c_void_p),
+                                       v.ctypes.data_as(C.c_void_p), cp.ctypes.data_as(C.c_void_p),
+                                       ri.ctypes.data_as(C.c_void_p), va.ctypes.data_as(C.c_void_p),
+                                       C.byref(nnz)), "csc_from_triplet")
+    k = nnz.value
+    return CSC(nrow, ncol, cp, ri[:k].copy(), va[:k].copy())
+
+
+def csc_matvec(a: CSC, x):
+    xx = _f(x)
+    y = np.zeros(a.n_rows)
+    _lib().ffo_csc_matvec(C.c_int(a.n_rows), C.c_int(a.n_cols), a.col_ptr.ctypes.data_as(C.c_void_p),
+                          a.row_idx.ctypes.data_as(C.c_void_p), _f(a.val).ctypes.data_as(C.c_void_p),
+                          xx.ctypes.data_as(C.c_void_p), y.ctypes.data_as(C.c_void_p))
+    return y
+
+
+def sparse_from_dense(dense):
+    d = np.asfortranarray(dense, dtype=np.float64)
+    n = d.shape[0]
+    flat = np.ascontiguousarray(d.reshape(-1, order="F"))
+    nnz_max = int(np.count_nonzero(np.abs(d) > 1e-14))
+    cp = np.zeros(n + 1, dtype=np.int32)
+    ri = np.zeros(max(nnz_max, 1), dtype=np.int32)
+    va = np.zeros(max(nnz_max, 1))
+    nnz = C.c_int64(0)
+    _lib().ffo_sparse_from_dense(C.c_int(n), flat.ctypes.data_as(C.c_void_p), cp.ctypes.data_as(C.c_void_p),
+                                 ri.ctypes.data_as(C.c_void_p), va.ctypes.data_as(C.c_void_p), C.byref(nnz))
+    k = nnz.value
+    return CSC(n, n, cp, ri[:k].copy(), va[:k].copy())
+
+
+def lagrange_stiffness_csc(vertices, tets, d, q, kappa=1.0, mu=0.0, cell_kappa=None):
+    """assemble_tetra_lagrange_stiffness_csc (+ per-cell kappa variant)."""
+    v, t = _f(vertices), _i(tets)
+    st, g, G = lagrange_dof_map(d, t)
+    _check(st, "lagrange_dof_map")
+    n = g.shape[1]
+    nt = t.shape[0] * n * n
+    rows = np.zeros(nt, dtype=np.int32)
+    cols = np.zeros(nt, dtype=np.int32)
+    vals = np.zeros(nt)
+    ck = None if cell_kappa is None else _f(cell_kappa)
+    _check(_lib().ffo_lagrange_stiffness_triplets(
+        C.c_int(v.shape[0]), v.ctypes.data_as(C.c_void_p), C.c_int(t.shape[0]),
+        t.ctypes.data_as(C.c_void_p), C.c_int(d), C.c_int(q),
+        None if ck is None else ck.ctypes.data_as(C.c_void_p), C.c_double(kappa), C.c_double(mu),
+        g.ctypes.data_as(C.c_void_p), rows.ctypes.data_as(C.c_void_p), cols.ctypes.data_as(C.c_void_p),
+        vals.ctypes.data_as(C.c_void_p)), "stiffness_triplets")
+    return csc_from_triplet(G, G, rows, cols, vals), g, G
+
+
+def tetra_physical_points(vertices, tets, q):
+    v, t = _f(vertices), _i(tets)
+    nq = _lib().ffo_tetra_duffy_count(C.c_int(q))
+    pts = np.zeros((t.shape[0], nq, 3))
+    _check(_lib().ffo_tetra_physical_points(C.c_int(v.shape[0]), v.ctypes.data_as(C.c_void_p),
+                                            C.c_int(t.shape[0]), t.ctypes.data_as(C.c_void_p),
+                                            C.c_int(q), pts.ctypes.data_as(C.c_void_p)), "physical_points")
+    return pts
+
+
+def lagrange_load_samples(vertices, tets, d, q, source_values, global_dofs, G):
+    """assemble_tetra_lagrange_scalar_load_samples; source_values (T, nq)."""
+    v, t, s, g = _f(vertices), _i(tets), _f(source_values), _i(global_dofs)
+    rhs = np.zeros(G)
+    _check(_lib().ffo_lagrange_load_samples(C.c_int(v.shape[0]), v.ctypes.data_as(C.c_void_p),
+                                            C.c_int(t.shape[0]), t.ctypes.data_as(C.c_void_p),
+                                            C.c_int(d), C.c_int(q), s.ctypes.data_as(C.c_void_p),
+                                            g.ctypes.data_as(C.c_void_p), C.c_int(G),
+                                            rhs.ctypes.data_as(C.c_void_p)), "load_samples")
+    return rhs
+
+
+def lagrange_load(vertices, tets, d, q, source, global_dofs, G):
+    """assemble_tetra_lagrange_scalar_load with a vectorised source(x, y, z)."""
+    pts = tetra_physical_points(vertices, tets, q)
+    samples = source(pts[..., 0], pts[..., 1], pts[..., 2])
+    return lagrange_load_samples(vertices, tets, d, q, samples, global_dofs, G)
+
+
+def lagrange_dirichlet(vertices, tets, d, global_dofs, G, search=False):
+    v, t, g = _f(vertices), _i(tets), _i(global_dofs)
+    cons = np.zeros(G, dtype=np.int32)
+    pts = np.zeros((G, 3))
+    if search:
+        _check(_lib().ffo_lagrange_dirichlet_search(C.c_int(t.shape[0]), t.ctypes.data_as(C.c_void_p),
+                                                    C.c_int(d), g.ctypes.data_as(C.c_void_p), C.c_int(G),
+                                                    cons.ctypes.data_as(C.c_void_p)), "dirichlet_search")
+    else:
+        _check(_lib().ffo_lagrange_dirichlet(C.c_int(v.shape[0]), v.ctypes.data_as(C.c_void_p),
+                                             C.c_int(t.shape[0]), t.ctypes.data_as(C.c_void_p), C.c_int(d),
+                                             g.ctypes.data_as(C.c_void_p), C.c_int(G),
+                                             cons.ctypes.data_as(C.c_void_p),
+                                             pts.ctypes.data_as(C.c_void_p)), "dirichlet")
+    return cons.astype(bool), pts
+
+
+def constrained_reduction(a: CSC, constrained, cvalues, rhs):
+    n = a.n_cols
+    cons = _i(np.asarray(constrained, dtype=np.int32))
+    cv = _f(cvalues)
+    b = _f(rhs)
+    free = np.zeros(n, dtype=np.int32)
+    cp = np.zeros(n + 1, dtype=np.int32)
+    ri = np.zeros(max(a.nnz, 1), dtype=np.int32)
+    va = np.zeros(max(a.nnz, 1))
+    rb = np.zeros(n)
+    nf = C.c_int(0)
+    nnz = C.c_int64(0)
+    _check(_lib().ffo_constrained_reduction(C.c_int(n), a.col_ptr.ctypes.data_as(C.c_void_p),
+                                            a.row_idx.ctypes.data_as(C.c_void_p),
+                                            _f(a.val).ctypes.data_as(C.c_void_p),
+                                            cons.ctypes.data_as(C.c_void_p), cv.ctypes.data_as(C.c_void_p),
+                                            b.ctypes.data_as(C.c_void_p), C.byref(nf),
+                                            free.ctypes.data_as(C.c_void_p), cp.ctypes.data_as(C.c_void_p),
+                                            ri.ctypes.data_as(C.c_void_p), va.ctypes.data_as(C.c_void_p),
+                                            C.byref(nnz), rb.ctypes.data_as(C.c_void_p)), "reduction")
+    f = nf.value
+    k = nnz.value
+    return CSC(f, f, cp[: f + 1].copy(), ri[:k].copy(), va[:k].copy()), free[:f].copy(), rb[:f].copy()
+
+
+# --------------------------------------------------------------- Krylov ---
+class _Stats(C.Structure):
+    _fields_ = [("converged", C.c_int), ("iterations", C.c_int), ("final_residual", C.c_double)]
+
+
+@dataclass
+class SolverStats:
+    converged: bool
+    iterations: int
+    final_residual: float
+
+
+METHODS = {"cg": 0, "pcg": 1, "auto": 1}
+PRECONDS = {"none": 0, "jacobi": 1}
+TOL_TYPES = {"relative": 0, "absolute": 1}
+BREAKDOWN = {"absolute": 0, "relative": 1}
+
+
+def solve_sparse(a: CSC, b, x0=None, method="pcg", preconditioner="jacobi", tolerance=1e-6,
+                 tolerance_type="relative", max_iterations=1000, breakdown="absolute"):
+    """solve_sparse restated (advanced_solvers.f90:135-195)."""
+    n = a.n_cols
+    x = np.zeros(n) if x0 is None else _f(x0).copy()
+    bb = _f(b)
+    st = _Stats()
+    _lib().ffo_solve_sparse(C.c_int(n), a.col_ptr.ctypes.data_as(C.c_void_p),
+                            a.row_idx.ctypes.data_as(C.c_void_p), _f(a.val).ctypes.data_as(C.c_void_p),
+                            bb.ctypes.data_as(C.c_void_p), x.ctypes.data_as(C.c_void_p),
+                            C.c_int(METHODS[method]), C.c_int(PRECONDS[preconditioner]),
+                            C.c_double(tolerance), C.c_int(TOL_TYPES[tolerance_type]),
+                            C.c_int(max_iterations), C.c_int(BREAKDOWN[breakdown]), C.byref(st))
+    return x, SolverStats(bool(st.converged), int(st.iterations), float(st.final_residual))
+
+
+# ------------------------------------------------------------------ 2D ---
+def p1_2d_dense(vertices, tris, mass=0.0, stiffness=1.0):
+    v, t = _f(vertices), _i(tris)
+    n = v.shape[0]
+    m = np.zeros((n, n))  # C-order m[c, r] == matrix(r, c)
+    _check(_lib().ffo_p1_2d_assemble_dense(C.c_int(n), v.ctypes.data_as(C.c_void_p), C.c_int(t.shape[0]),
+                                           t.ctypes.data_as(C.c_void_p), C.c_double(mass),
+                                           C.c_double(stiffness), m.ctypes.data_as(C.c_void_p)), "p1_2d")
+    return np.asfortranarray(m.T)
+
+
+def p1_2d_load(vertices, tris, f=1.0):
+    v, t = _f(vertices), _i(tris)
+    b = np.zeros(v.shape[0])
+    _check(_lib().ffo_p1_2d_load(C.c_int(v.shape[0]), v.ctypes.data_as(C.c_void_p), C.c_int(t.shape[0]),
+                                 t.ctypes.data_as(C.c_void_p), C.c_double(f), b.ctypes.data_as(C.c_void_p)),
+           "p1_2d_load")
+    return b
+
+
+def p1_2d_symmetric_dirichlet(dense, rhs, is_boundary, value=0.0):
+    n = dense.shape[0]
+    m = np.ascontiguousarray(np.asfortranarray(dense).reshape(-1, order="F"))
+    b = _f(rhs).copy()
+    bnd = _i(np.asarray(is_boundary, dtype=np.int32))
+    _lib().ffo_p1_2d_symmetric_dirichlet(C.c_int(n), m.ctypes.data_as(C.c_void_p), b.ctypes.data_as(C.c_void_p),
+                                         bnd.ctypes.data_as(C.c_void_p), C.c_double(value))
+    return m.reshape((n, n), order="F"), b
+
+
+def p1_2d_poisson_system(n=65, f=1.0, g=0.0):
+    """Configuration 1: unit_square_mesh(n), f, symmetric elimination, then
+    sparse_from_dense -- the CSC the reference's Jacobi-PCG sees."""
+    v, t, bnd = unit_square_mesh(n)
+    a = p1_2d_dense(v, t, 0.0, 1.0)
+    b = p1_2d_load(v, t, f)
+    a, b = p1_2d_symmetric_dirichlet(a, b, bnd, g)
+    return sparse_from_dense(a), b, (v, t, bnd)

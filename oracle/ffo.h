@@ -1,0 +1,129 @@
+/*
+ * ffo.h -- FP64 CPU oracle for the fortfem sparse-assembly + Jacobi-PCG path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is a from-scratch C restatement of
+ * the reference algorithms (lazy-fortran/fortfem), used as the checker for the
+ * HIP library and as the CPU baseline leg of bench.py.  Only tests/,
+ * __graft_entry__.smoke() and bench.py's cpu_baseline may load it.  The
+ * product path (fortfem_amd, libfortfem_hip.so) never links or calls it.
+ *
+ * Conventions follow the reference's Fortran storage so that arrays compare
+ * bit-for-bit:
+ *   vertices(3, nv)   -> double v[3*i + c]      (column-major, c = x,y,z)
+ *   tetrahedra(4, T)  -> int    t[4*e + k]      (1-based vertex ids)
+ *   global_dofs(n, T) -> int    g[n*e + k]      (1-based dof ids)
+ *   CSC               -> col_ptr/row_idx 1-based (fortsparse csc_t)
+ *
+ * Every function cites the reference file:line it restates; see each .c file.
+ */
+#ifndef FFO_H
+#define FFO_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- math helpers (fortnum det3 / inv3 restated from the cofactor formula) */
+double ffo_det3(const double a[9]);                 /* a column-major 3x3 */
+int ffo_inv3(const double a[9], double inv[9]);
+
+/* ---- meshes (A1, A2) ---------------------------------------------------- */
+int ffo_box_mesh(const double bounds[6], const int counts[3],
+                 double *vertices, int *tetrahedra);
+int ffo_unit_square_mesh(int n, double *vertices, int *triangles,
+                         int *is_boundary_vertex);
+
+/* ---- quadrature (A3) ---------------------------------------------------- */
+int ffo_gauss_legendre_ab(int n, double a, double b, double *nodes,
+                          double *weights);
+int ffo_tetra_duffy_count(int polynomial_degree);
+int ffo_tetra_duffy(int polynomial_degree, double *x, double *y, double *z,
+                    double *w);
+int ffo_triangle_duffy_count(int polynomial_degree);
+int ffo_triangle_duffy(int polynomial_degree, double *x, double *y, double *w);
+
+/* ---- tetrahedral Lagrange H1 (A4-A9) ------------------------------------ */
+int ffo_lagrange_dof_count(int degree);
+int ffo_lagrange_init(int degree, int *barycentric_indices, double *nodes);
+int ffo_lagrange_eval(int degree, const double point[3], double *values,
+                      double *gradients);
+int ffo_lagrange_dof_map(int degree, int ntet, const int *tetrahedra,
+                         int *global_dofs, int *global_count);
+int ffo_lagrange_dof_map_search(int degree, int ntet, const int *tetrahedra,
+                                int *global_dofs, int *global_count);
+int ffo_lagrange_stiffness_element(const double vertices[12], int degree,
+                                   int quadrature_degree, double kappa,
+                                   double mu, double *matrix);
+/* triplets are column-outer / row-inner per tet, 1-based */
+int ffo_lagrange_stiffness_triplets(int nvert, const double *vertices,
+                                    int ntet, const int *tetrahedra,
+                                    int degree, int quadrature_degree,
+                                    const double *cell_kappa, double kappa,
+                                    double mu, const int *global_dofs,
+                                    int *rows, int *cols, double *vals);
+int ffo_lagrange_load_samples(int nvert, const double *vertices, int ntet,
+                              const int *tetrahedra, int degree,
+                              int quadrature_degree,
+                              const double *source_values,
+                              const int *global_dofs, int global_count,
+                              double *rhs);
+int ffo_tetra_physical_points(int nvert, const double *vertices, int ntet,
+                              const int *tetrahedra, int quadrature_degree,
+                              double *points);
+int ffo_lagrange_dirichlet(int nvert, const double *vertices, int ntet,
+                           const int *tetrahedra, int degree,
+                           const int *global_dofs, int global_count,
+                           int *constrained, double *node_points);
+int ffo_lagrange_dirichlet_search(int ntet, const int *tetrahedra,
+                                  int degree, const int *global_dofs,
+                                  int global_count, int *constrained);
+
+/* ---- CSC (A10, A22, A23) ------------------------------------------------ */
+int ffo_csc_from_triplet(int nrow, int ncol, int64_t ntrip, const int *rows,
+                         const int *cols, const double *vals, int *col_ptr,
+                         int *row_idx, double *val, int64_t *nnz);
+int ffo_csc_matvec(int nrow, int ncol, const int *col_ptr, const int *row_idx,
+                   const double *val, const double *x, double *y);
+int ffo_sparse_from_dense(int n, const double *dense, int *col_ptr,
+                          int *row_idx, double *val, int64_t *nnz);
+int ffo_constrained_reduction(int n, const int *col_ptr, const int *row_idx,
+                              const double *val, const int *constrained,
+                              const double *constrained_values,
+                              const double *rhs, int *free_count,
+                              int *free_dofs, int *red_col_ptr,
+                              int *red_row_idx, double *red_val,
+                              int64_t *red_nnz, double *red_rhs);
+
+/* ---- Krylov (A24-A28) --------------------------------------------------- */
+typedef struct ffo_solver_stats {
+    int converged;
+    int iterations;
+    double final_residual;
+} ffo_solver_stats;
+
+/* method: 0 = cg, 1 = pcg;  precond: 0 = none, 1 = jacobi;
+ * tol_type: 0 = relative, 1 = absolute;
+ * breakdown_mode: 0 = reference |pAp| < 1e-14, 1 = |pAp| < 1e-14 |rz| */
+int ffo_solve_sparse(int n, const int *col_ptr, const int *row_idx,
+                     const double *val, const double *b, double *x,
+                     int method, int precond, double tolerance, int tol_type,
+                     int max_iterations, int breakdown_mode,
+                     ffo_solver_stats *stats);
+
+/* ---- 2D P1 (A29) -------------------------------------------------------- */
+int ffo_p1_2d_assemble_dense(int nvert, const double *vertices, int ntri,
+                             const int *triangles, double mass_coefficient,
+                             double stiffness_coefficient, double *matrix);
+int ffo_p1_2d_load(int nvert, const double *vertices, int ntri,
+                   const int *triangles, double load_coefficient,
+                   double *vector);
+int ffo_p1_2d_symmetric_dirichlet(int n, double *matrix, double *rhs,
+                                  const int *is_boundary, double value);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,245 @@
+"""GPU parity of the tetrahedral H1 path (HIP through the C-ABI) against the oracle.
+
+Tolerances (north star): index arrays bit-exact; FP64 values and solutions
+within 1e-12 relative; mixed precision within 1e-8 of the FP64 solution.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+SIN = [1.0, np.pi, np.pi, np.pi]
+
+
+def sin_f(x, y, z):
+    return np.sin(np.pi * x) * np.sin(np.pi * y) * np.sin(np.pi * z)
+
+
+def rel_entry_err(a, b):
+    """max |a - b| / max |b| (entry-wise, scaled by the largest entry)."""
+    return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
+
+
+@pytest.fixture(scope="module")
+def ff():
+    import fortfem_amd
+
+    return fortfem_amd
+
+
+def test_box_mesh_bit_exact(ff):
+    for bounds, counts in [([[-1, 2, 4], [1, 5, 10]], [2, 1, 3]), ([[0, 0, 0], [1, 1, 1]], [5, 4, 3])]:
+        v, t, st = ff.generate_structured_tetra_box_mesh(bounds, counts)
+        vo, to = O.box_mesh(bounds, counts)
+        assert st == 0
+        assert np.array_equal(t, to)
+        assert np.array_equal(v, vo)
+
+
+@pytest.mark.parametrize("degree", [1, 2, 3, 4])
+def test_dof_map_matches_oracle(ff, degree):
+    v, t = ff.unit_box_mesh(3)
+    p = ff.Problem(v, t)
+    n, nnz = p.pattern_build(ff.capi.FAMILY_H1, degree)
+    st, g, G = O.lagrange_dof_map(degree, t)
+    assert n == G
+    assert np.array_equal(p.global_dofs(base=1), g)
+
+
+@pytest.mark.parametrize("degree,N", [(1, 5), (2, 4), (3, 2), (4, 2)])
+def test_stiffness_csc_parity(ff, degree, N):
+    from fortfem_amd import workloads
+
+    v, t = ff.unit_box_mesh(N)
+    kappa = workloads.lognormal_kappa(t.shape[0], 2)
+    q = 2 * degree + 4
+    a_ref, g, G = O.lagrange_stiffness_csc(v, t, degree, q, cell_kappa=kappa, mu=0.5)
+    p = ff.Problem(v, t)
+    p.pattern_build(ff.capi.FAMILY_H1, degree)
+    p.assemble(q, cell_kappa=kappa, mass=0.5)
+    a = p.matrix(ff.capi.FULL).download()
+    assert np.array_equal(a.col_ptr, a_ref.col_ptr)
+    assert np.array_equal(a.row_idx, a_ref.row_idx)
+    assert rel_entry_err(a.val, a_ref.val) <= 1e-12
+
+
+def test_reference_api_stiffness_and_energy(ff):
+    """Energy of a linear field on the 2-tet mesh is 1/3
+    (test/test_tetra_lagrange_arbitrary_order_assembly.f90:37-66)."""
+    mv = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0], [0, 0, 1], [0, 0, -1.0]])
+    mt = np.array([[1, 2, 3, 4], [1, 3, 2, 5]], dtype=np.int32)
+    a, status = ff.assemble_tetra_lagrange_stiffness_csc(mv, mt, 1, 4)
+    assert status.code == 0
+    p = ff.Problem(mv, mt)
+    p.pattern_build(ff.capi.FAMILY_H1, 1)
+    g = p.global_dofs(base=1)
+    coef = np.zeros(a.ncol)
+    for e in range(2):
+        for l, vert in enumerate([3, 2, 1, 0]):  # P1 local order v4, v3, v2, v1
+            coef[g[e, l] - 1] = mv[mt[e, vert] - 1, 0]
+    y = ff.spmv(a, coef)
+    assert abs(coef @ y - 1.0 / 3.0) < 2e-12
+
+
+def test_degenerate_cell_reports_status(ff):
+    mv = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0], [1, 1, 0.0]])
+    mt = np.array([[1, 2, 3, 4]], dtype=np.int32)
+    a, status = ff.assemble_tetra_lagrange_stiffness_csc(mv, mt, 1, 4)
+    assert a is None and status.code != 0
+
+
+@pytest.mark.parametrize("degree", [1, 2])
+def test_load_parity(ff, degree):
+    v, t = ff.unit_box_mesh(4)
+    q = 2 * degree + 4
+    st, g, G = O.lagrange_dof_map(degree, t)
+    p = ff.Problem(v, t)
+    p.pattern_build(ff.capi.FAMILY_H1, degree)
+    b_ref = O.lagrange_load(v, t, degree, q, sin_f, g, G)
+    p.load(q, ff.capi.SOURCE_SIN_PRODUCT, SIN)
+    assert rel_entry_err(p.rhs(), b_ref) <= 1e-12
+    p.load(q, ff.capi.SOURCE_CONSTANT, [2.5])
+    b_ref = O.lagrange_load(v, t, degree, q, lambda x, y, z: 2.5 + 0 * x, g, G)
+    assert rel_entry_err(p.rhs(), b_ref) <= 1e-12
+    rhs, status = ff.assemble_tetra_lagrange_scalar_load(v, t, degree, q, sin_f)
+    assert status.code == 0
+    assert rel_entry_err(rhs, O.lagrange_load(v, t, degree, q, sin_f, g, G)) <= 1e-12
+
+
+def test_dirichlet_reduction_parity(ff):
+    from fortfem_amd import workloads
+
+    v, t = ff.unit_box_mesh(4)
+    kappa = workloads.lognormal_kappa(t.shape[0], 2)
+    a_ref, g, G = O.lagrange_stiffness_csc(v, t, 2, 8, cell_kappa=kappa)
+    b_ref = O.lagrange_load(v, t, 2, 8, sin_f, g, G)
+    cons, _ = O.lagrange_dirichlet(v, t, 2, g, G)
+    bvals = np.where(cons, 0.3, 0.0)
+    red, free, rb = O.constrained_reduction(a_ref, cons, bvals, b_ref)
+    p = ff.Problem(v, t)
+    p.pattern_build(ff.capi.FAMILY_H1, 2)
+    p.assemble(8, cell_kappa=kappa)
+    p.load(8, ff.capi.SOURCE_SIN_PRODUCT, SIN)
+    p.dirichlet(0, bvals)
+    assert np.array_equal(p.constrained(), cons)
+    r = p.matrix(ff.capi.REDUCED).download()
+    assert np.array_equal(r.col_ptr, red.col_ptr)
+    assert np.array_equal(r.row_idx, red.row_idx)
+    assert rel_entry_err(r.val, red.val) <= 1e-12
+    assert rel_entry_err(p.rhs(ff.capi.REDUCED), rb) <= 1e-12
+
+
+def _reference_solution(v, t, degree, kappa, tol=1e-10):
+    q = 2 * degree + 4
+    a_ref, g, G = O.lagrange_stiffness_csc(v, t, degree, q, cell_kappa=kappa)
+    b_ref = O.lagrange_load(v, t, degree, q, sin_f, g, G)
+    cons, _ = O.lagrange_dirichlet(v, t, degree, g, G)
+    red, free, rb = O.constrained_reduction(a_ref, cons, np.zeros(G), b_ref)
+    xr, st = O.solve_sparse(red, rb, method="pcg", preconditioner="jacobi", tolerance=tol,
+                            max_iterations=20000, breakdown="relative")
+    x = np.zeros(G)
+    x[free - 1] = xr
+    return x, st
+
+
+@pytest.mark.parametrize("degree,N", [(1, 8), (2, 6)])
+def test_poisson_pcg_parity(ff, degree, N):
+    from fortfem_amd import workloads
+
+    v, t = ff.unit_box_mesh(N)
+    kappa = workloads.lognormal_kappa(t.shape[0], 2)
+    q = 2 * degree + 4
+    x_ref, st_ref = _reference_solution(v, t, degree, kappa)
+    p = ff.Problem(v, t)
+    p.pattern_build(ff.capi.FAMILY_H1, degree)
+    p.assemble(q, cell_kappa=kappa)
+    p.load(q, ff.capi.SOURCE_SIN_PRODUCT, SIN)
+    p.dirichlet(0)
+    for graph in (False, True):
+        opts = ff.solver_options(method="pcg", preconditioner="jacobi", tolerance=1e-10,
+                                 max_iterations=20000, breakdown="relative", use_graph=graph)
+        x, st = p.solve(opts)
+        assert st.converged
+        assert abs(st.iterations - st_ref.iterations) <= 1
+        assert np.linalg.norm(x - x_ref) / np.linalg.norm(x_ref) <= 1e-9
+    # mixed precision: FP32 inner PCG inside FP64 refinement
+    opts = ff.solver_options(method="pcg", preconditioner="jacobi", tolerance=1e-10,
+                             max_iterations=20000, breakdown="relative", precision="mixed")
+    xm, stm = p.solve(opts)
+    assert stm.converged and stm.true_residual <= 1e-10 * stm.initial_residual
+    assert np.linalg.norm(xm - x_ref) / np.linalg.norm(x_ref) <= 1e-8
+
+
+def test_solve_sparse_tridiagonal_kat(ff):
+    """CG to 1e-10 on the reference's SPD tridiagonal matrix
+    (test/test_advanced_solvers.f90:39-84,663-690)."""
+    n = 100
+    A = np.zeros((n, n))
+    for i in range(n):
+        A[i, i] = 4.1
+        if i > 0:
+            A[i, i - 1] = A[i - 1, i] = -1.0
+    x_exact = np.arange(1, n + 1, dtype=float)
+    b = A @ x_exact
+    a = O.sparse_from_dense(A)
+    csc = ff.CSC(n, n, a.col_ptr, a.row_idx, a.val)
+    x = np.zeros(n)
+    st = ff.solve_sparse(csc, b, x, ff.solver_options(method="cg", tolerance=1e-10, max_iterations=n))
+    xo, sto = O.solve_sparse(a, b, method="cg", preconditioner="none", tolerance=1e-10, max_iterations=n)
+    assert st.converged and st.iterations < n // 2
+    assert st.iterations == sto.iterations
+    assert np.linalg.norm(A @ x - b) < 1e-7 and np.linalg.norm(x - x_exact) < 1e-6
+    assert np.max(np.abs(x - xo)) <= 1e-12 * np.max(np.abs(xo))
+    # iteration cap leaves converged = false (:342-348)
+    x = np.zeros(n)
+    st = ff.solve_sparse(csc, b, x, ff.solver_options(method="pcg", tolerance=1e-12, max_iterations=5))
+    assert not st.converged and st.iterations == 5
+    # spmv = matmul (:742-765)
+    y = ff.spmv(csc, x_exact)
+    assert np.all(np.abs(y - A @ x_exact) < 1e-12)
+
+
+def test_solve_sparse_nonsymmetric_operator(ff):
+    """Row-only Dirichlet (assemble_laplacian_system, fortfem_solvers_laplacian.f90:
+    130-136) leaves A nonsymmetric; products must still be A x."""
+    n = 12
+    A = np.zeros((n, n))
+    for i in range(n):
+        A[i, i] = 2.5
+        if i > 0:
+            A[i, i - 1] = -1.0
+        if i < n - 1:
+            A[i, i + 1] = -1.0
+    A[0, :] = 0.0
+    A[0, 0] = 1.0
+    a = O.sparse_from_dense(A)
+    csc = ff.CSC(n, n, a.col_ptr, a.row_idx, a.val)
+    xv = np.linspace(-1, 2, n)
+    assert np.max(np.abs(ff.spmv(csc, xv) - A @ xv)) < 1e-13
+    b = np.ones(n)
+    x = np.zeros(n)
+    st = ff.solve_sparse(csc, b, x, ff.solver_options(method="pcg", preconditioner="jacobi",
+                                                      tolerance=1e-10, max_iterations=200))
+    xo, sto = O.solve_sparse(a, b, method="pcg", preconditioner="jacobi", tolerance=1e-10, max_iterations=200)
+    assert st.iterations == sto.iterations and st.converged == sto.converged
+    assert np.max(np.abs(x - xo)) <= 1e-10 * np.max(np.abs(xo))
+
+
+def test_reference_breakdown_semantics(ff):
+    """The reference's absolute |pAp| < 1e-14 breakdown (advanced_solvers.f90:897)
+    stops early on a small-h mesh; the GPU path reproduces the same stop."""
+    v, t = ff.unit_box_mesh(6)
+    a_ref, g, G = O.lagrange_stiffness_csc(v, t, 2, 8)
+    b_ref = O.lagrange_load(v, t, 2, 8, sin_f, g, G)
+    cons, _ = O.lagrange_dirichlet(v, t, 2, g, G)
+    red, free, rb = O.constrained_reduction(a_ref, cons, np.zeros(G), b_ref)
+    xo, sto = O.solve_sparse(red, rb, tolerance=1e-10, max_iterations=5000, preconditioner="jacobi")
+    csc = ff.CSC(red.n_rows, red.n_cols, red.col_ptr, red.row_idx, red.val)
+    x = np.zeros(red.n_rows)
+    st = ff.solve_sparse(csc, rb, x, ff.solver_options(method="pcg", preconditioner="jacobi",
+                                                       tolerance=1e-10, max_iterations=5000))
+    assert not sto.converged
+    assert st.converged == sto.converged
+    assert abs(st.iterations - sto.iterations) <= 1
