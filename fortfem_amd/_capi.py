@@ -41,7 +41,7 @@ REDUCED = 1
 # every symbol include/fortfem_hip.h declares
 EXPORTS = (
     "ffh_init", "ffh_finalize", "ffh_last_error", "ffh_solver_options_default",
-    "ffh_box_mesh_size", "ffh_box_mesh", "ffh_unit_square_mesh",
+    "ffh_box_mesh_size", "ffh_box_mesh", "ffh_unit_square_mesh", "ffh_tetra_duffy",
     "ffh_problem_create", "ffh_problem_free", "ffh_pattern_build", "ffh_problem_sizes",
     "ffh_global_dofs_download", "ffh_assemble", "ffh_load", "ffh_dirichlet",
     "ffh_constrained_download", "ffh_set_rhs", "ffh_rhs_download", "ffh_problem_matrix",
@@ -109,6 +109,7 @@ def load() -> C.CDLL:
         "ffh_box_mesh_size": (C.c_int, [vp, ip, ip]),
         "ffh_box_mesh": (C.c_int, [vp, vp, vp, vp]),
         "ffh_unit_square_mesh": (C.c_int, [C.c_int, vp, vp, vp]),
+        "ffh_tetra_duffy": (C.c_int, [C.c_int, ip, vp, vp, vp, vp]),
         "ffh_problem_create": (C.c_int, [vp, C.c_int, C.c_int, vp, C.c_int, vp, C.POINTER(vp)]),
         "ffh_problem_free": (C.c_int, [vp]),
         "ffh_pattern_build": (C.c_int, [vp, C.c_int, C.c_int, ip, i64p]),

@@ -33,7 +33,7 @@ def build_fortran() -> str | None:
     fc = flang()
     if fc is None:
         return None
-    subprocess.run(["make", "-s", "-C", fdir, f"FC={fc}"], check=True)
+    subprocess.run(["make", "-s", "-C", fdir, f"FLANG={fc}"], check=True)
     return os.path.join(fdir, "libfortfem_hip_f.so")
 
 

@@ -93,6 +93,7 @@ void load_matrix(Context *ctx, DevMatrix &A, int n, int64_t nnz, int base,
         A.t_idx.upload(ti, s);
         A.t_val.upload(tv, s);
     }
+    build_rowblocks(ctx, A, sym ? cp : tp);
     FFH_HIP(hipStreamSynchronize(s));
 }
 
@@ -235,6 +236,23 @@ int ffh_box_mesh(const double bounds[6], const int counts[3], double *vertices,
     for (int c = 0; c < 3; ++c)
         if (!(bounds[3 + c] > bounds[c])) return FFH_ERR_INVALID;
     return guarded(nullptr, [&] { box_mesh(bounds, counts, vertices, tetrahedra); });
+}
+
+int ffh_tetra_duffy(int p, int *nq, double *x, double *y, double *z, double *w)
+{
+    if (p < 0 || !nq) return FFH_ERR_INVALID;
+    const int m = (p + 4) / 2;
+    *nq = m * m * m;
+    if (!x) return FFH_OK;
+    if (!y || !z || !w) return FFH_ERR_INVALID;
+    return guarded(nullptr, [&] {
+        std::vector<double> a, b, c, d;
+        tetra_duffy(p, a, b, c, d);
+        std::copy(a.begin(), a.end(), x);
+        std::copy(b.begin(), b.end(), y);
+        std::copy(c.begin(), c.end(), z);
+        std::copy(d.begin(), d.end(), w);
+    });
 }
 
 int ffh_unit_square_mesh(int n, double *vertices, int *triangles, int *is_boundary_vertex)

@@ -79,7 +79,7 @@ struct DBuf {
 // Control block of one Krylov solve, resident in device memory.  Written
 // only by the last block of each reduction kernel.
 struct KrylovCtl {
-    double rz_old, pap, alpha, beta, rr, rz;
+    double rzs[2], pap, alpha, beta, rr, rz;
     double residual_norm, initial_norm, tol;
     int iter, max_iter, converged, done, breakdown_hit;
     int breakdown_mode, tol_type, pad0;
@@ -108,7 +108,8 @@ struct DevMatrix {
     const double *gval() const { return symmetric ? val.p : t_val.p; }
     const float *gval32() const { return symmetric ? val32.p : t_val32.p; }
     bool val32_valid = false;
-    double avg_row = 0.0;
+    DBuf<int> rowblocks; // CSR-stream row blocks of the gather arrays
+    int nrb = 0;
 };
 
 // Solver workspace attached to a matrix (kept resident between solves).
@@ -186,6 +187,7 @@ struct Problem {
     DBuf<float> d_tab32;
     int tab_qdeg = -1, tab_nq = 0;
     DBuf<double> d_load_tab; // per-point xi (3) + w*phi (n_local)
+    DBuf<double> d_elem;     // element matrices / vectors, [entry][pos]
     int load_qdeg = -1, load_nq = 0;
     DBuf<int> d_err;
     DBuf<double> d_coeff;

@@ -9,12 +9,15 @@
 //   (1) Ap = A p fused with the p.Ap reduction -> alpha     (HBM: matrix)
 //   (2) x += alpha p, r -= alpha Ap, r.r and r.(D r)        -> beta, stop test
 //   (3) p = D r + beta p
-// Every reduction is block partials + a last-arriving-block finish in a fixed
-// order, so results are deterministic run to run.  All scalars live in a
-// device control block; the host polls it once per chunk of iterations and a
-// chunk may be replayed from a HIP graph.  Once the control block says done,
-// later launches of the chunk return immediately, so the stopping iteration
-// is exactly the reference's.
+// Every reduction is per-block partials; the next kernel's blocks each re-sum
+// the previous kernel's partials in block order (no atomics, deterministic)
+// and derive alpha / beta / the stop test identically.  Scalars that a kernel
+// both reads and updates (r.z) live in two slots indexed by iteration parity,
+// so the one block that writes never races the blocks that read.  The host
+// polls the control block once per chunk of iterations; a chunk may be
+// replayed from a HIP graph.  Once the control block says done, later
+// launches of the chunk return immediately, so the stopping iteration is
+// exactly the reference's.
 #include <algorithm>
 #include <cmath>
 
@@ -23,7 +26,9 @@
 
 namespace ffh {
 
-constexpr int kBlock = 256;
+constexpr int kBlock = 256;       // elementwise and reduction kernels
+constexpr int kStreamBlock = 256; // CSR-stream SpMV
+constexpr int kStreamNnz = 2048;  // nonzeros staged in LDS per row block
 
 __device__ __forceinline__ double wave_sum(double v)
 {
@@ -32,10 +37,11 @@ __device__ __forceinline__ double wave_sum(double v)
     return v;
 }
 
-// Sum over the block; the result is valid in thread 0.
+// Sum over a block of BS threads; the result is valid in thread 0.
+template <int BS>
 __device__ __forceinline__ double block_sum(double v)
 {
-    __shared__ double sh[kBlock / 64];
+    __shared__ double sh[BS / 64];
     v = wave_sum(v);
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     __syncthreads();
@@ -43,33 +49,23 @@ __device__ __forceinline__ double block_sum(double v)
     __syncthreads();
     double t = 0.0;
     if (threadIdx.x == 0)
-        for (int i = 0; i < kBlock / 64; ++i) t += sh[i];
+        for (int i = 0; i < BS / 64; ++i) t += sh[i];
     return t;
 }
 
-// Arrive at a reduction; true in every thread of the last block to arrive.
-__device__ __forceinline__ bool last_block(unsigned int *counter)
+// Total of partials[0 .. nb) in block order, broadcast to every thread.
+template <int BS>
+__device__ __forceinline__ double sum_partials(const double *__restrict__ partials, int nb)
 {
-    __shared__ int is_last;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        unsigned int prev = __hip_atomic_fetch_add(
-            counter, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-        is_last = (prev == gridDim.x - 1);
-    }
-    __syncthreads();
-    if (is_last) __threadfence();
-    return is_last;
-}
-
-// Sum of partials[0 .. gridDim.x) in a fixed order; valid in thread 0.
-__device__ __forceinline__ double sum_partials(const double *partials,
-                                               int stride, int offset)
-{
+    __shared__ double tot;
     double t = 0.0;
-    for (int i = threadIdx.x; i < (int)gridDim.x; i += kBlock)
-        t += partials[(size_t)i * stride + offset];
-    return block_sum(t);
+    for (int i = threadIdx.x; i < nb; i += BS) t += partials[i];
+    t = block_sum<BS>(t);
+    if (threadIdx.x == 0) tot = t;
+    __syncthreads();
+    const double r = tot;
+    __syncthreads();
+    return r;
 }
 
 __device__ __forceinline__ bool is_breakdown(double pap, double rz, int mode)
@@ -78,132 +74,201 @@ __device__ __forceinline__ bool is_breakdown(double pap, double rz, int mode)
     return fabs(pap) < 1.0e-14;
 }
 
-// Row gather of one row by a group of G lanes: sum_k a_k x[col_k].
-template <int G, typename MT, typename VT>
-__device__ __forceinline__ VT row_dot(int row, int lane, const int *__restrict__ ptr,
-                                      const int *__restrict__ idx,
-                                      const MT *__restrict__ val,
-                                      const VT *__restrict__ x)
+// ------------------------------------------------------------ CSR stream --
+// A row block is a run of consecutive rows whose nonzeros fit in LDS.  The
+// block streams the run's values and column indices with fully coalesced
+// loads (each lane takes every BS-th nonzero), multiplies by the gathered x
+// into LDS, then one thread per row adds its row's products in column order.
+// A row longer than the LDS stage is summed by the whole block.
+// Epilogue E receives (row, A x row value) and accumulates up to two partial
+// sums.
+template <typename MT, typename VT, typename E>
+__device__ __forceinline__ void csr_stream(const int *__restrict__ ptr,
+                                           const int *__restrict__ idx,
+                                           const MT *__restrict__ val,
+                                           const VT *__restrict__ x,
+                                           const int *__restrict__ rowblocks, int nrb,
+                                           E &epi, double &d0, double &d1)
 {
-    const int b = ptr[row], e = ptr[row + 1];
-    VT s = VT(0);
-    for (int k = b + lane; k < e; k += G)
-        s += (VT)val[k] * x[__builtin_nontemporal_load(idx + k)];
+    constexpr int BS = kStreamBlock, NZ = kStreamNnz, U = NZ / BS;
+    __shared__ VT prod[NZ];
+    for (int blk = blockIdx.x; blk < nrb; blk += gridDim.x) {
+        const int r0 = rowblocks[blk], r1 = rowblocks[blk + 1];
+        const int k0 = ptr[r0], k1 = ptr[r1];
+        const int cnt = k1 - k0;
+        if (cnt <= NZ) {
+            int c[U];
+            MT v[U];
 #pragma unroll
-    for (int off = G / 2; off > 0; off >>= 1) s += __shfl_xor(s, off, G);
-    return s;
+            for (int u = 0; u < U; ++u) {
+                const int k = threadIdx.x + u * BS;
+                if (k < cnt) {
+                    c[u] = __builtin_nontemporal_load(idx + k0 + k);
+                    v[u] = __builtin_nontemporal_load(val + k0 + k);
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int k = threadIdx.x + u * BS;
+                if (k < cnt) prod[k] = (VT)v[u] * x[c[u]];
+            }
+            __syncthreads();
+            for (int r = r0 + threadIdx.x; r < r1; r += BS) {
+                const int b = ptr[r] - k0, e = ptr[r + 1] - k0;
+                VT s = VT(0);
+                for (int k = b; k < e; ++k) s += prod[k];
+                epi(r, s, d0, d1);
+            }
+            __syncthreads();
+        } else {
+            // one long row (r1 == r0 + 1)
+            VT s = VT(0);
+            for (int k = k0 + threadIdx.x; k < k1; k += BS) s += (VT)val[k] * x[idx[k]];
+            double t = block_sum<BS>((double)s);
+            __shared__ double tot;
+            if (threadIdx.x == 0) tot = t;
+            __syncthreads();
+            if (threadIdx.x == 0) epi(r0, (VT)tot, d0, d1);
+            __syncthreads();
+        }
+    }
 }
+
+template <typename VT>
+struct EpiStore {
+    VT *y;
+    __device__ void operator()(int r, VT s, double &, double &) const { y[r] = s; }
+};
+
+template <typename VT>
+struct EpiAp { // ap = A p ; p.Ap
+    const VT *p;
+    VT *ap;
+    __device__ void operator()(int r, VT s, double &d0, double &) const
+    {
+        ap[r] = s;
+        d0 += (double)p[r] * (double)s;
+    }
+};
+
+template <typename VT>
+struct EpiInit { // r = b - A x ; z = D r ; p = z ; r.r, r.z
+    const VT *b, *dinv;
+    VT *r, *p;
+    __device__ void operator()(int i, VT s, double &d0, double &d1) const
+    {
+        VT ri = b[i] - s;
+        VT zi = dinv[i] * ri;
+        r[i] = ri;
+        p[i] = zi;
+        d0 += (double)ri * (double)ri;
+        d1 += (double)ri * (double)zi;
+    }
+};
+
+struct EpiResidual { // r = b - A x ; r.r
+    const double *b;
+    double *r;
+    __device__ void operator()(int i, double s, double &d0, double &) const
+    {
+        double ri = b[i] - s;
+        r[i] = ri;
+        d0 += ri * ri;
+    }
+};
 
 // --------------------------------------------------------------- kernels --
-template <int G, typename MT, typename VT>
-__global__ __launch_bounds__(kBlock) void k_spmv(int n, const int *__restrict__ ptr,
-                                                 const int *__restrict__ idx,
-                                                 const MT *__restrict__ val,
-                                                 const VT *__restrict__ x,
-                                                 VT *__restrict__ y)
+template <typename MT, typename VT>
+__global__ __launch_bounds__(kStreamBlock) void k_spmv(const int *__restrict__ ptr,
+                                                       const int *__restrict__ idx,
+                                                       const MT *__restrict__ val,
+                                                       const VT *__restrict__ x,
+                                                       VT *__restrict__ y,
+                                                       const int *__restrict__ rowblocks,
+                                                       int nrb)
 {
-    const int lane = threadIdx.x % G;
-    const int ngrp = gridDim.x * (kBlock / G);
-    for (int row = blockIdx.x * (kBlock / G) + threadIdx.x / G; row < n;
-         row += ngrp) {
-        VT s = row_dot<G>(row, lane, ptr, idx, val, x);
-        if (lane == 0) y[row] = s;
-    }
+    EpiStore<VT> epi{y};
+    double d0 = 0.0, d1 = 0.0;
+    csr_stream<MT, VT>(ptr, idx, val, x, rowblocks, nrb, epi, d0, d1);
 }
 
-// r = b - A x ; z = D r ; p = z ; partial r.r, r.z -> initial state
-template <int G, typename MT, typename VT>
-__global__ __launch_bounds__(kBlock) void k_pcg_init(
-    int n, const int *__restrict__ ptr, const int *__restrict__ idx,
-    const MT *__restrict__ val, const VT *__restrict__ x,
-    const VT *__restrict__ b, VT *__restrict__ r, VT *__restrict__ p,
-    const VT *__restrict__ dinv, double *__restrict__ partials, KrylovCtl *ctl)
+// r = b - A x ; z = D r ; p = z ; partials of r.r and r.z
+template <typename MT, typename VT>
+__global__ __launch_bounds__(kStreamBlock) void k_pcg_init(
+    const int *__restrict__ ptr, const int *__restrict__ idx, const MT *__restrict__ val,
+    const VT *__restrict__ x, const VT *__restrict__ b, VT *__restrict__ r,
+    VT *__restrict__ p, const VT *__restrict__ dinv, const int *__restrict__ rowblocks,
+    int nrb, double *__restrict__ partials, int stride)
 {
-    const int lane = threadIdx.x % G;
-    const int ngrp = gridDim.x * (kBlock / G);
-    double rr = 0.0, rz = 0.0;
-    for (int row = blockIdx.x * (kBlock / G) + threadIdx.x / G; row < n;
-         row += ngrp) {
-        VT ax = row_dot<G>(row, lane, ptr, idx, val, x);
-        if (lane == 0) {
-            VT ri = b[row] - ax;
-            VT zi = dinv[row] * ri;
-            r[row] = ri;
-            p[row] = zi;
-            rr += (double)ri * (double)ri;
-            rz += (double)ri * (double)zi;
-        }
-    }
-    double s0 = block_sum(rr);
-    double s1 = block_sum(rz);
+    EpiInit<VT> epi{b, dinv, r, p};
+    double d0 = 0.0, d1 = 0.0;
+    csr_stream<MT, VT>(ptr, idx, val, x, rowblocks, nrb, epi, d0, d1);
+    const double s0 = block_sum<kStreamBlock>(d0);
+    const double s1 = block_sum<kStreamBlock>(d1);
     if (threadIdx.x == 0) {
-        partials[2 * blockIdx.x] = s0;
-        partials[2 * blockIdx.x + 1] = s1;
-    }
-    if (last_block(&ctl->counter[0])) {
-        double trr = sum_partials(partials, 2, 0);
-        double trz = sum_partials(partials, 2, 1);
-        if (threadIdx.x == 0) {
-            double init = sqrt(trr);
-            ctl->rr = trr;
-            ctl->initial_norm = init;
-            ctl->residual_norm = init;
-            ctl->tol = ctl->tol_type == 1 ? ctl->tolerance : ctl->tolerance * init;
-            ctl->rz_old = trz;
-            ctl->iter = 0;
-            ctl->done = ctl->max_iter <= 0 ? 1 : 0;
-            __hip_atomic_store(&ctl->counter[0], 0u, __ATOMIC_RELAXED,
-                               __HIP_MEMORY_SCOPE_AGENT);
-        }
+        partials[stride + blockIdx.x] = s0;
+        partials[2 * stride + blockIdx.x] = s1;
     }
 }
 
-// Ap = A p ; p.Ap -> alpha (or breakdown)
-template <int G, typename MT, typename VT>
-__global__ __launch_bounds__(kBlock) void k_pcg_spmv(
-    int n, const int *__restrict__ ptr, const int *__restrict__ idx,
-    const MT *__restrict__ val, const VT *__restrict__ p, VT *__restrict__ ap,
-    double *__restrict__ partials, KrylovCtl *ctl)
+// initial norm, tolerance (compute_solver_tolerance, :1063-1073), r.z
+__global__ __launch_bounds__(kBlock) void k_pcg_init_finish(const double *__restrict__ partials,
+                                                            int stride, int nb, KrylovCtl *ctl)
+{
+    const double trr = sum_partials<kBlock>(partials + stride, nb);
+    const double trz = sum_partials<kBlock>(partials + 2 * stride, nb);
+    if (threadIdx.x == 0) {
+        const double init = sqrt(trr);
+        ctl->rr = trr;
+        ctl->initial_norm = init;
+        ctl->residual_norm = init;
+        ctl->tol = ctl->tol_type == 1 ? ctl->tolerance : ctl->tolerance * init;
+        ctl->rzs[0] = trz;
+        ctl->iter = 0;
+        ctl->done = ctl->max_iter <= 0 ? 1 : 0;
+    }
+}
+
+// Ap = A p ; partials of p.Ap
+template <typename MT, typename VT>
+__global__ __launch_bounds__(kStreamBlock) void k_pcg_spmv(
+    const int *__restrict__ ptr, const int *__restrict__ idx, const MT *__restrict__ val,
+    const VT *__restrict__ p, VT *__restrict__ ap, const int *__restrict__ rowblocks,
+    int nrb, double *__restrict__ partials, const KrylovCtl *ctl)
 {
     if (ctl->done) return;
-    const int lane = threadIdx.x % G;
-    const int ngrp = gridDim.x * (kBlock / G);
-    double dot = 0.0;
-    for (int row = blockIdx.x * (kBlock / G) + threadIdx.x / G; row < n;
-         row += ngrp) {
-        VT y = row_dot<G>(row, lane, ptr, idx, val, p);
-        if (lane == 0) {
-            ap[row] = y;
-            dot += (double)p[row] * (double)y;
-        }
-    }
-    double s = block_sum(dot);
-    if (threadIdx.x == 0) partials[2 * blockIdx.x] = s;
-    if (last_block(&ctl->counter[1])) {
-        double pap = sum_partials(partials, 2, 0);
-        if (threadIdx.x == 0) {
-            ctl->pap = pap;
-            if (is_breakdown(pap, ctl->rz_old, ctl->breakdown_mode)) {
-                ctl->done = 1;
-                ctl->breakdown_hit = 1;
-            } else {
-                ctl->alpha = ctl->rz_old / pap;
-            }
-            __hip_atomic_store(&ctl->counter[1], 0u, __ATOMIC_RELAXED,
-                               __HIP_MEMORY_SCOPE_AGENT);
-        }
-    }
+    EpiAp<VT> epi{p, ap};
+    double d0 = 0.0, d1 = 0.0;
+    csr_stream<MT, VT>(ptr, idx, val, p, rowblocks, nrb, epi, d0, d1);
+    const double s0 = block_sum<kStreamBlock>(d0);
+    if (threadIdx.x == 0) partials[blockIdx.x] = s0;
 }
 
-// x += alpha p ; r -= alpha Ap ; r.r, r.(D r) -> stop test, beta
+// p.Ap -> alpha (or breakdown); x += alpha p ; r -= alpha Ap ; partials of
+// r.r and r.(D r)
 template <typename VT>
 __global__ __launch_bounds__(kBlock) void k_pcg_update(
     int n, VT *__restrict__ x, VT *__restrict__ r, const VT *__restrict__ p,
     const VT *__restrict__ ap, const VT *__restrict__ dinv,
-    double *__restrict__ partials, KrylovCtl *ctl)
+    double *__restrict__ partials, int stride, int nb_spmv, int parity, KrylovCtl *ctl)
 {
     if (ctl->done) return;
-    const VT alpha = (VT)ctl->alpha;
+    const double pap = sum_partials<kBlock>(partials, nb_spmv);
+    const double rz_old = ctl->rzs[parity];
+    if (is_breakdown(pap, rz_old, ctl->breakdown_mode)) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) {
+            ctl->pap = pap;
+            ctl->done = 1;
+            ctl->breakdown_hit = 1;
+        }
+        return;
+    }
+    const double alpha_d = rz_old / pap;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        ctl->pap = pap;
+        ctl->alpha = alpha_d;
+    }
+    const VT alpha = (VT)alpha_d;
     double rr = 0.0, rz = 0.0;
     for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
         x[i] = x[i] + alpha * p[i];
@@ -213,44 +278,45 @@ __global__ __launch_bounds__(kBlock) void k_pcg_update(
         rr += (double)ri * (double)ri;
         rz += (double)ri * (double)zi;
     }
-    double s0 = block_sum(rr);
-    double s1 = block_sum(rz);
+    const double s0 = block_sum<kBlock>(rr);
+    const double s1 = block_sum<kBlock>(rz);
     if (threadIdx.x == 0) {
-        partials[2 * blockIdx.x] = s0;
-        partials[2 * blockIdx.x + 1] = s1;
-    }
-    if (last_block(&ctl->counter[2])) {
-        double trr = sum_partials(partials, 2, 0);
-        double trz = sum_partials(partials, 2, 1);
-        if (threadIdx.x == 0) {
-            ctl->iter += 1;
-            ctl->rr = trr;
-            double rn = sqrt(trr);
-            ctl->residual_norm = rn;
-            if (rn <= ctl->tol) {
-                ctl->converged = 1;
-                ctl->done = 1;
-            } else {
-                ctl->beta = trz / ctl->rz_old;
-                ctl->rz_old = trz;
-                ctl->rz = trz;
-                if (ctl->iter >= ctl->max_iter) ctl->done = 1;
-            }
-            __hip_atomic_store(&ctl->counter[2], 0u, __ATOMIC_RELAXED,
-                               __HIP_MEMORY_SCOPE_AGENT);
-        }
+        partials[stride + blockIdx.x] = s0;
+        partials[2 * stride + blockIdx.x] = s1;
     }
 }
 
-// p = D r + beta p
+// stop test on ||r|| ; beta = rz_new / rz_old ; p = D r + beta p
 template <typename VT>
 __global__ __launch_bounds__(kBlock) void k_pcg_dir(int n, VT *__restrict__ p,
                                                     const VT *__restrict__ r,
                                                     const VT *__restrict__ dinv,
-                                                    const KrylovCtl *ctl)
+                                                    const double *__restrict__ partials,
+                                                    int stride, int nb_update, int parity,
+                                                    KrylovCtl *ctl)
 {
     if (ctl->done) return;
-    const VT beta = (VT)ctl->beta;
+    const double trr = sum_partials<kBlock>(partials + stride, nb_update);
+    const double trz = sum_partials<kBlock>(partials + 2 * stride, nb_update);
+    const double rn = sqrt(trr);
+    const bool conv = rn <= ctl->tol;
+    const double rz_old = ctl->rzs[parity];
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        const int it = ctl->iter + 1;
+        ctl->iter = it;
+        ctl->rr = trr;
+        ctl->residual_norm = rn;
+        if (conv) {
+            ctl->converged = 1;
+            ctl->done = 1;
+        } else {
+            ctl->rzs[parity ^ 1] = trz;
+            ctl->beta = trz / rz_old;
+            if (it >= ctl->max_iter) ctl->done = 1;
+        }
+    }
+    if (conv) return;
+    const VT beta = (VT)(trz / rz_old);
     for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock)
         p[i] = dinv[i] * r[i] + beta * p[i];
 }
@@ -274,37 +340,26 @@ __global__ __launch_bounds__(kBlock) void k_jacobi(int n, const int *__restrict_
     }
 }
 
-// r = b - A x (FP64) with ||r||^2 into ctl->rr -- true residual of the
-// refinement loop.
-template <int G>
-__global__ __launch_bounds__(kBlock) void k_residual(
-    int n, const int *__restrict__ ptr, const int *__restrict__ idx,
+// r = b - A x (FP64) and partials of ||r||^2 -- the true residual of the
+// refinement loop and of the final FP64 report.
+__global__ __launch_bounds__(kStreamBlock) void k_residual(
+    const int *__restrict__ ptr, const int *__restrict__ idx,
     const double *__restrict__ val, const double *__restrict__ x,
     const double *__restrict__ b, double *__restrict__ r,
-    double *__restrict__ partials, KrylovCtl *ctl)
+    const int *__restrict__ rowblocks, int nrb, double *__restrict__ partials)
 {
-    const int lane = threadIdx.x % G;
-    const int ngrp = gridDim.x * (kBlock / G);
-    double rr = 0.0;
-    for (int row = blockIdx.x * (kBlock / G) + threadIdx.x / G; row < n;
-         row += ngrp) {
-        double ax = row_dot<G>(row, lane, ptr, idx, val, x);
-        if (lane == 0) {
-            double ri = b[row] - ax;
-            r[row] = ri;
-            rr += ri * ri;
-        }
-    }
-    double s = block_sum(rr);
-    if (threadIdx.x == 0) partials[2 * blockIdx.x] = s;
-    if (last_block(&ctl->counter[3])) {
-        double t = sum_partials(partials, 2, 0);
-        if (threadIdx.x == 0) {
-            ctl->rr = t;
-            __hip_atomic_store(&ctl->counter[3], 0u, __ATOMIC_RELAXED,
-                               __HIP_MEMORY_SCOPE_AGENT);
-        }
-    }
+    EpiResidual epi{b, r};
+    double d0 = 0.0, d1 = 0.0;
+    csr_stream<double, double>(ptr, idx, val, x, rowblocks, nrb, epi, d0, d1);
+    const double s0 = block_sum<kStreamBlock>(d0);
+    if (threadIdx.x == 0) partials[blockIdx.x] = s0;
+}
+
+__global__ __launch_bounds__(kBlock) void k_residual_finish(const double *__restrict__ partials,
+                                                            int nb, KrylovCtl *ctl)
+{
+    const double t = sum_partials<kBlock>(partials, nb);
+    if (threadIdx.x == 0) ctl->rr = t;
 }
 
 __global__ __launch_bounds__(kBlock) void k_scale_f32(int n, const double *__restrict__ r,
@@ -334,30 +389,39 @@ __global__ __launch_bounds__(kBlock) void k_to_f32(int64_t n, const double *__re
 }
 
 // ---------------------------------------------------------------- driver --
-static int pick_group(double avg_row)
-{
-    if (avg_row <= 6) return 4;
-    if (avg_row <= 20) return 8;
-    if (avg_row <= 48) return 16;
-    if (avg_row <= 128) return 32;
-    return 64;
-}
+static int grid_cap(const Context *ctx) { return ctx->cu_count * 4; }
 
 static int grid_for(const Context *ctx, int64_t items, int per_block)
 {
     int64_t need = (items + per_block - 1) / per_block;
-    int64_t cap = (int64_t)ctx->cu_count * 8;
-    return (int)std::max<int64_t>(1, std::min(need, cap));
+    return (int)std::max<int64_t>(1, std::min<int64_t>(need, grid_cap(ctx)));
 }
 
-#define FFH_GROUP_DISPATCH(G, ...)                                            \
-    switch (G) {                                                             \
-    case 4: { constexpr int GG = 4; __VA_ARGS__; } break;                    \
-    case 8: { constexpr int GG = 8; __VA_ARGS__; } break;                    \
-    case 16: { constexpr int GG = 16; __VA_ARGS__; } break;                  \
-    case 32: { constexpr int GG = 32; __VA_ARGS__; } break;                  \
-    default: { constexpr int GG = 64; __VA_ARGS__; } break;                  \
+static int stream_grid(const Context *ctx, const DevMatrix &A)
+{
+    return std::max(1, std::min(A.nrb, grid_cap(ctx)));
+}
+
+// Row blocks of the CSR-stream kernels: consecutive rows whose nonzeros fit
+// the LDS stage; a longer row gets a block of its own.
+void build_rowblocks(Context *ctx, DevMatrix &A, const std::vector<int> &ptr)
+{
+    std::vector<int> rb;
+    rb.reserve(ptr.size() / 16 + 2);
+    const int n = (int)ptr.size() - 1;
+    int r = 0;
+    rb.push_back(0);
+    while (r < n) {
+        int r1 = r + 1;
+        while (r1 < n && ptr[r1 + 1] - ptr[r] <= kStreamNnz) ++r1;
+        rb.push_back(r1);
+        r = r1;
     }
+    if (n == 0) rb.push_back(0);
+    A.nrb = (int)rb.size() - 1;
+    A.rowblocks.upload(rb, ctx->stream);
+    FFH_HIP(hipStreamSynchronize(ctx->stream));
+}
 
 Workspace::~Workspace()
 {
@@ -366,21 +430,20 @@ Workspace::~Workspace()
     if (host_ctl) (void)hipHostFree(host_ctl);
 }
 
+static int partial_stride(const Context *ctx) { return grid_cap(ctx); }
+
 void ensure_workspace(Context *ctx, Matrix &M)
 {
     Workspace &w = M.ws;
     const int n = M.m.n;
-    if (w.n != n) {
-        for (auto &g : w.graph)
-            if (g) {
-                (void)hipGraphExecDestroy(g);
-                g = nullptr;
-            }
-    }
+    if (w.n != n) invalidate_graphs(w);
     w.n = n;
     w.r.alloc(n), w.p.alloc(n), w.ap.alloc(n), w.x.alloc(n), w.b.alloc(n),
         w.dinv.alloc(n);
-    w.partials.alloc(2 * (size_t)ctx->cu_count * 8 + 64);
+    if (!w.partials.p) {
+        w.partials.alloc(3 * (size_t)partial_stride(ctx));
+        FFH_HIP(hipMemsetAsync(w.partials.p, 0, sizeof(double) * w.partials.n, ctx->stream));
+    }
     w.ctl.alloc(1);
     if (!w.host_ctl)
         FFH_HIP(hipHostMalloc((void **)&w.host_ctl, sizeof(KrylovCtl),
@@ -412,31 +475,20 @@ void ensure_val32(Context *ctx, DevMatrix &A)
     A.val32_valid = true;
 }
 
-int spmv_group(const DevMatrix &A)
-{
-    return pick_group(A.n ? (double)A.nnz / A.n : 1.0);
-}
-
 void spmv64(Context *ctx, const DevMatrix &A, const double *x, double *y)
 {
-    const int G = spmv_group(A);
-    FFH_GROUP_DISPATCH(G, {
-        int grid = grid_for(ctx, A.n, kBlock / GG);
-        hipLaunchKernelGGL((k_spmv<GG, double, double>), dim3(grid), dim3(kBlock), 0,
-                           ctx->stream, A.n, A.gptr(), A.gidx(), A.gval(), x, y);
-    });
+    hipLaunchKernelGGL((k_spmv<double, double>), dim3(stream_grid(ctx, A)),
+                       dim3(kStreamBlock), 0, ctx->stream, A.gptr(), A.gidx(), A.gval(), x,
+                       y, A.rowblocks.p, A.nrb);
     FFH_HIP(hipGetLastError());
 }
 
 void spmv32(Context *ctx, DevMatrix &A, const float *x, float *y)
 {
     ensure_val32(ctx, A);
-    const int G = spmv_group(A);
-    FFH_GROUP_DISPATCH(G, {
-        int grid = grid_for(ctx, A.n, kBlock / GG);
-        hipLaunchKernelGGL((k_spmv<GG, float, float>), dim3(grid), dim3(kBlock), 0,
-                           ctx->stream, A.n, A.gptr(), A.gidx(), A.gval32(), x, y);
-    });
+    hipLaunchKernelGGL((k_spmv<float, float>), dim3(stream_grid(ctx, A)),
+                       dim3(kStreamBlock), 0, ctx->stream, A.gptr(), A.gidx(), A.gval32(),
+                       x, y, A.rowblocks.p, A.nrb);
     FFH_HIP(hipGetLastError());
 }
 
@@ -449,23 +501,22 @@ static void launch_iterations(Context *ctx, Matrix &M, int iters, VT *x, VT *r,
     const DevMatrix &A = M.m;
     Workspace &w = M.ws;
     const int n = A.n;
-    const int G = spmv_group(A);
     const int gridv = grid_for(ctx, n, kBlock);
-    FFH_GROUP_DISPATCH(G, {
-        const int grids = grid_for(ctx, n, kBlock / GG);
-        for (int it = 0; it < iters; ++it) {
-            if (prof_slot >= 0) ctx->prof_mark(prof_slot, true);
-            hipLaunchKernelGGL((k_pcg_spmv<GG, MT, VT>), dim3(grids), dim3(kBlock), 0,
-                               ctx->stream, n, A.gptr(), A.gidx(), val, p, ap,
-                               w.partials.p, w.ctl.p);
-            if (prof_slot >= 0) ctx->prof_mark(prof_slot, false);
-            hipLaunchKernelGGL((k_pcg_update<VT>), dim3(gridv), dim3(kBlock), 0,
-                               ctx->stream, n, x, r, p, ap, dinv, w.partials.p,
-                               w.ctl.p);
-            hipLaunchKernelGGL((k_pcg_dir<VT>), dim3(gridv), dim3(kBlock), 0,
-                               ctx->stream, n, p, r, dinv, w.ctl.p);
-        }
-    });
+    const int grids = stream_grid(ctx, A);
+    const int stride = partial_stride(ctx);
+    for (int it = 0; it < iters; ++it) {
+        const int parity = it & 1; // chunks are even, so this is the global parity
+        if (prof_slot >= 0) ctx->prof_mark(prof_slot, true);
+        hipLaunchKernelGGL((k_pcg_spmv<MT, VT>), dim3(grids), dim3(kStreamBlock), 0,
+                           ctx->stream, A.gptr(), A.gidx(), val, p, ap, A.rowblocks.p,
+                           A.nrb, w.partials.p, w.ctl.p);
+        if (prof_slot >= 0) ctx->prof_mark(prof_slot, false);
+        hipLaunchKernelGGL((k_pcg_update<VT>), dim3(gridv), dim3(kBlock), 0, ctx->stream,
+                           n, x, r, p, ap, dinv, w.partials.p, stride, grids, parity,
+                           w.ctl.p);
+        hipLaunchKernelGGL((k_pcg_dir<VT>), dim3(gridv), dim3(kBlock), 0, ctx->stream, n,
+                           p, r, dinv, w.partials.p, stride, gridv, parity, w.ctl.p);
+    }
     FFH_HIP(hipGetLastError());
 }
 
@@ -481,12 +532,9 @@ static void pcg_device(Context *ctx, Matrix &M, const MT *val, const VT *b, VT *
     Workspace &w = M.ws;
     const int n = A.n;
     // Jacobi diagonal ('none' and method cg use the identity)
-    {
-        int grid = grid_for(ctx, n, kBlock);
-        hipLaunchKernelGGL((k_jacobi<VT>), dim3(grid), dim3(kBlock), 0, ctx->stream, n,
-                           A.gptr(), A.gidx(), A.gval(),
-                           (method == 1 && precond == 1) ? 1 : 0, dinv);
-    }
+    hipLaunchKernelGGL((k_jacobi<VT>), dim3(grid_for(ctx, n, kBlock)), dim3(kBlock), 0,
+                       ctx->stream, n, A.gptr(), A.gidx(), A.gval(),
+                       (method == 1 && precond == 1) ? 1 : 0, dinv);
     KrylovCtl init{};
     init.max_iter = max_iter;
     init.breakdown_mode = breakdown_mode;
@@ -495,15 +543,15 @@ static void pcg_device(Context *ctx, Matrix &M, const MT *val, const VT *b, VT *
     *w.host_ctl = init;
     FFH_HIP(hipMemcpyAsync(w.ctl.p, w.host_ctl, sizeof(KrylovCtl),
                            hipMemcpyHostToDevice, ctx->stream));
-    const int G = spmv_group(A);
-    FFH_GROUP_DISPATCH(G, {
-        int grid = grid_for(ctx, n, kBlock / GG);
-        hipLaunchKernelGGL((k_pcg_init<GG, MT, VT>), dim3(grid), dim3(kBlock), 0,
-                           ctx->stream, n, A.gptr(), A.gidx(), val, x, b, r, p,
-                           dinv, w.partials.p, w.ctl.p);
-    });
+    const int gi = stream_grid(ctx, A);
+    hipLaunchKernelGGL((k_pcg_init<MT, VT>), dim3(gi), dim3(kStreamBlock), 0, ctx->stream,
+                       A.gptr(), A.gidx(), val, x, b, r, p, dinv, A.rowblocks.p, A.nrb,
+                       w.partials.p, partial_stride(ctx));
+    hipLaunchKernelGGL(k_pcg_init_finish, dim3(1), dim3(kBlock), 0, ctx->stream, w.partials.p,
+                       partial_stride(ctx), gi, w.ctl.p);
     FFH_HIP(hipGetLastError());
-    int chunk = check_interval > 0 ? check_interval : 16;
+    // iteration chunks are even so that the parity slots line up
+    int chunk = check_interval > 0 ? std::max(2, (check_interval + 1) & ~1) : 16;
     const bool profiling = ctx->profiling;
     if (profiling) use_graph = 0;
     const int gslot = sizeof(VT) == 8 ? 0 : 1;
@@ -521,24 +569,32 @@ static void pcg_device(Context *ctx, Matrix &M, const MT *val, const VT *b, VT *
         w.graph_chunk[gslot] = chunk;
     }
     // graphs bake in the vector pointers; they are the workspace's own
-    int done_iters = 0;
     for (;;) {
-        int todo = chunk;
         if (use_graph)
             FFH_HIP(hipGraphLaunch(w.graph[gslot], ctx->stream));
         else
-            launch_iterations<MT, VT>(ctx, M, todo, x, r, p, ap, dinv, val,
+            launch_iterations<MT, VT>(ctx, M, chunk, x, r, p, ap, dinv, val,
                                       profiling ? slot : -1);
-        done_iters += todo;
         FFH_HIP(hipMemcpyAsync(w.host_ctl, w.ctl.p, sizeof(KrylovCtl),
                                hipMemcpyDeviceToHost, ctx->stream));
         FFH_HIP(hipStreamSynchronize(ctx->stream));
         if (w.host_ctl->done) break;
-        // grow the chunk towards the expected remaining work
         if (!use_graph && chunk < 64) chunk *= 2;
-        (void)done_iters;
     }
     out = *w.host_ctl;
+}
+
+static void launch_residual(Context *ctx, Matrix &M)
+{
+    DevMatrix &A = M.m;
+    Workspace &w = M.ws;
+    const int g = stream_grid(ctx, A);
+    hipLaunchKernelGGL(k_residual, dim3(g), dim3(kStreamBlock), 0, ctx->stream, A.gptr(),
+                       A.gidx(), A.gval(), w.x.p, w.b.p, w.r.p, A.rowblocks.p, A.nrb,
+                       w.partials.p);
+    hipLaunchKernelGGL(k_residual_finish, dim3(1), dim3(kBlock), 0, ctx->stream, w.partials.p,
+                       g, w.ctl.p);
+    FFH_HIP(hipGetLastError());
 }
 
 // Public FP64 solve on device vectors.
@@ -562,17 +618,7 @@ void solve_fp64(Context *ctx, Matrix &M, const double *b, double *x,
                                out);
     FFH_HIP(hipMemcpyAsync(x, w.x.p, sizeof(double) * M.m.n,
                            hipMemcpyDeviceToDevice, ctx->stream));
-    // true residual in FP64
-    {
-        const int G = spmv_group(M.m);
-        FFH_GROUP_DISPATCH(G, {
-            int grid = grid_for(ctx, M.m.n, kBlock / GG);
-            hipLaunchKernelGGL((k_residual<GG>), dim3(grid), dim3(kBlock), 0,
-                               ctx->stream, M.m.n, M.m.gptr(), M.m.gidx(), M.m.gval(),
-                               w.x.p, w.b.p, w.r.p, w.partials.p, w.ctl.p);
-        });
-        FFH_HIP(hipGetLastError());
-    }
+    launch_residual(ctx, M); // true residual in FP64
     FFH_HIP(hipEventRecord(ctx->ev_stop, ctx->stream));
     FFH_HIP(hipMemcpyAsync(w.host_ctl, w.ctl.p, sizeof(KrylovCtl),
                            hipMemcpyDeviceToHost, ctx->stream));
@@ -610,17 +656,10 @@ void solve_mixed(Context *ctx, Matrix &M, const double *b, double *x,
                            ctx->stream));
     FFH_HIP(hipMemcpyAsync(w.x.p, x, sizeof(double) * n, hipMemcpyDeviceToDevice,
                            ctx->stream));
-    const int G = spmv_group(A);
     auto true_residual = [&]() -> double {
         if (ctx->profiling) ctx->prof_mark(0, true);
-        FFH_GROUP_DISPATCH(G, {
-            int grid = grid_for(ctx, n, kBlock / GG);
-            hipLaunchKernelGGL((k_residual<GG>), dim3(grid), dim3(kBlock), 0,
-                               ctx->stream, n, A.gptr(), A.gidx(), A.gval(), w.x.p,
-                               w.b.p, w.r.p, w.partials.p, w.ctl.p);
-        });
+        launch_residual(ctx, M);
         if (ctx->profiling) ctx->prof_mark(0, false);
-        FFH_HIP(hipGetLastError());
         FFH_HIP(hipMemcpyAsync(w.host_ctl, w.ctl.p, sizeof(KrylovCtl),
                                hipMemcpyDeviceToHost, ctx->stream));
         FFH_HIP(hipStreamSynchronize(ctx->stream));
@@ -637,19 +676,15 @@ void solve_mixed(Context *ctx, Matrix &M, const double *b, double *x,
     int total_iters = 0, outer = 0, converged = rnorm <= tol;
     const int max_ref = o.max_refinements > 0 ? o.max_refinements : 50;
     const double inner_tol = o.inner_tolerance > 0 ? o.inner_tolerance : 1e-4;
-    int gridv = grid_for(ctx, n, kBlock);
+    const int gridv = grid_for(ctx, n, kBlock);
     while (!converged && outer < max_ref && total_iters < o.max_iterations) {
         const double s = rnorm > 0 ? 1.0 / rnorm : 1.0;
+        // inner right-hand side r / ||r|| in FP32, held in ap32 until the
+        // init kernel has consumed it; the correction accumulates in x32
         hipLaunchKernelGGL(k_scale_f32, dim3(gridv), dim3(kBlock), 0, ctx->stream, n,
-                           w.r.p, s, w.r32.p + 0, w.x32.p);
-        // inner solve A d = r / ||r|| in FP32 (b32 held in ap32 scratch)
-        FFH_HIP(hipMemcpyAsync(w.ap32.p, w.r32.p, sizeof(float) * n,
-                               hipMemcpyDeviceToDevice, ctx->stream));
+                           w.r.p, s, w.ap32.p, w.x32.p);
         KrylovCtl out{};
-        // pcg_device reads b (ap32 copy) only in the init kernel, before any
-        // write to ap32 by the iteration kernels
-        float *b32 = w.ap32.p;
-        pcg_device<float, float>(ctx, M, A.gval32(), b32, w.x32.p, w.r32.p, w.p32.p,
+        pcg_device<float, float>(ctx, M, A.gval32(), w.ap32.p, w.x32.p, w.r32.p, w.p32.p,
                                  w.ap32.p, w.dinv32.p, o.method, o.preconditioner,
                                  inner_tol, 0, o.max_iterations - total_iters, 1,
                                  o.check_interval, o.use_graph, 1, out);
@@ -658,7 +693,7 @@ void solve_mixed(Context *ctx, Matrix &M, const double *b, double *x,
                            w.x.p, w.x32.p, rnorm);
         FFH_HIP(hipGetLastError());
         ++outer;
-        double prev = rnorm;
+        const double prev = rnorm;
         rnorm = true_residual();
         converged = rnorm <= tol;
         if (out.iter == 0 || !(rnorm < prev)) break; // stagnation

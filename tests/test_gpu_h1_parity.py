@@ -243,3 +243,28 @@ def test_reference_breakdown_semantics(ff):
     assert not sto.converged
     assert st.converged == sto.converged
     assert abs(st.iterations - sto.iterations) <= 1
+
+
+def test_quartic_reproduced_at_degree_four(ff):
+    """test_tetra_lagrange_poisson_p_convergence.f90:26-40: P4 reproduces
+    u = xyz(1-x-y-z) (f = 2(xy+xz+yz), u = 0 on the boundary) at every node."""
+    from fortfem_amd._quadrature import lagrange_nodes
+
+    mv = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0], [0, 0, 1], [0.25, 0.25, 0.25]])
+    mt = np.array([[1, 2, 3, 5], [1, 4, 2, 5], [1, 3, 4, 5], [2, 4, 3, 5]], dtype=np.int32)
+    opts = ff.solver_options(method="pcg", preconditioner="jacobi", tolerance=1e-14,
+                             max_iterations=10000, breakdown="relative")
+    x, status, st = ff.solve_tetra_lagrange_poisson(mv, mt, 4, lambda x, y, z: 2 * (x * y + x * z + y * z),
+                                                    opts=opts)
+    assert status.code == 0 and st.converged
+    p = ff.Problem(mv, mt)
+    p.pattern_build(ff.capi.FAMILY_H1, 4)
+    g = p.global_dofs(base=0)
+    nodes = lagrange_nodes(4)
+    pts = np.zeros((x.shape[0], 3))
+    for e in range(4):
+        vv = mv[mt[e] - 1]
+        J = np.stack([vv[1] - vv[0], vv[2] - vv[0], vv[3] - vv[0]], axis=1)
+        pts[g[e]] = vv[0] + nodes @ J.T
+    u = pts[:, 0] * pts[:, 1] * pts[:, 2] * (1 - pts.sum(1))
+    assert np.max(np.abs(x - u)) < 1e-12
