@@ -165,15 +165,17 @@ contains
         integer, intent(out) :: status
 
         integer(c_int) :: nq
-        real(dp) :: dummy(1)
+        integer :: m
 
         status = 1
         if (polynomial_degree < 0) return
-        if (ffh_tetra_duffy(int(polynomial_degree, c_int), nq, dummy, dummy, &
-            dummy, dummy) /= FFH_OK) return
-        allocate(x(nq), y(nq), z(nq), weights(nq))
+        ! (p + 4)/2 points per direction (tetra_duffy_quadrature.f90:25); the
+        ! C routine fills all nq^3 entries, so the arrays are sized first
+        m = (polynomial_degree + 4)/2
+        allocate(x(m**3), y(m**3), z(m**3), weights(m**3))
         if (ffh_tetra_duffy(int(polynomial_degree, c_int), nq, x, y, z, &
             weights) /= FFH_OK) return
+        if (nq /= m**3) return
         status = 0
     end subroutine tetra_duffy_quadrature
 

@@ -394,3 +394,111 @@ def p1_2d_poisson_system(n=65, f=1.0, g=0.0):
     b = p1_2d_load(v, t, f)
     a, b = p1_2d_symmetric_dirichlet(a, b, bnd, g)
     return sparse_from_dense(a), b, (v, t, bnd)
+
+
+# ------------------------------------------------------------- Nedelec ---
+def nedelec_dof_count(k):
+    return _lib().ffo_nedelec_dof_count(C.c_int(k))
+
+
+def nedelec_coefficients(k):
+    n = nedelec_dof_count(k)
+    c = np.zeros((n, n))
+    _check(_lib().ffo_nedelec_coefficients(C.c_int(k), c.ctypes.data_as(C.c_void_p)), "nedelec_coefficients")
+    return c
+
+
+def nedelec_eval(k, point, coeff=None):
+    """reference basis values/curls (n, 3); k = 1 gives the Whitney functions"""
+    n = nedelec_dof_count(k)
+    p = _f(point)
+    v = np.zeros((n, 3))
+    c = np.zeros((n, 3))
+    if k == 1:
+        _lib().ffo_whitney(p.ctypes.data_as(C.c_void_p), v.ctypes.data_as(C.c_void_p), c.ctypes.data_as(C.c_void_p))
+    else:
+        cf = nedelec_coefficients(k) if coeff is None else _f(coeff)
+        _lib().ffo_nedelec_eval(C.c_int(k), cf.ctypes.data_as(C.c_void_p), p.ctypes.data_as(C.c_void_p),
+                                v.ctypes.data_as(C.c_void_p), c.ctypes.data_as(C.c_void_p))
+    return v, c
+
+
+def face_basis_to_local(k, perm):
+    s = k * (k - 1)
+    T = np.zeros((s, s))
+    pp = _i(perm)
+    _check(_lib().ffo_face_basis_to_local(C.c_int(k), pp.ctypes.data_as(C.c_void_p), T.ctypes.data_as(C.c_void_p)),
+           "face_basis_to_local")
+    return T
+
+
+def edge_dof_map(tets, search=False):
+    t = _i(tets)
+    T = t.shape[0]
+    edges = np.zeros((6 * T, 2), dtype=np.int32)
+    gd = np.zeros((T, 6), dtype=np.int32)
+    orient = np.zeros((T, 6), dtype=np.int32)
+    ne = C.c_int(0)
+    fn = _lib().ffo_edge_dof_map_search if search else _lib().ffo_edge_dof_map
+    st = fn(C.c_int(T), t.ctypes.data_as(C.c_void_p), edges.ctypes.data_as(C.c_void_p), gd.ctypes.data_as(C.c_void_p),
+            orient.ctypes.data_as(C.c_void_p), C.byref(ne))
+    return st, edges[:ne.value].copy(), gd, orient
+
+
+def nedelec_dof_map(k, tets):
+    t = _i(tets)
+    T = t.shape[0]
+    n = nedelec_dof_count(k)
+    gd = np.zeros((T, n), dtype=np.int32)
+    eo = np.zeros((T, 6), dtype=np.int32)
+    fp = np.zeros((T, 4, 3), dtype=np.int32)
+    nd, ne, nf = C.c_int(0), C.c_int(0), C.c_int(0)
+    st = _lib().ffo_nedelec_dof_map(C.c_int(k), C.c_int(T), t.ctypes.data_as(C.c_void_p), gd.ctypes.data_as(C.c_void_p),
+                                    eo.ctypes.data_as(C.c_void_p), fp.ctypes.data_as(C.c_void_p), C.byref(nd),
+                                    C.byref(ne), C.byref(nf))
+    _check(st, "nedelec_dof_map")
+    return gd, eo, fp, nd.value, ne.value, nf.value
+
+
+def nedelec_basis_transform(k, edge_orient, face_perm):
+    n = nedelec_dof_count(k)
+    T = np.zeros((n, n))
+    eo = _i(edge_orient)
+    fp = _i(face_perm)
+    _check(_lib().ffo_nedelec_basis_transform(C.c_int(k), eo.ctypes.data_as(C.c_void_p), fp.ctypes.data_as(C.c_void_p),
+                                              T.ctypes.data_as(C.c_void_p)), "basis_transform")
+    return T
+
+
+def nedelec_element(k, vertices4, qdeg=None, alpha=1.0, beta=1.0, tensor=None):
+    n = nedelec_dof_count(k)
+    cf = nedelec_coefficients(k) if k >= 2 else np.zeros(1)
+    if qdeg is None:
+        qdeg = 2 * k + 2 if tensor is not None else 2 * k
+    m = np.zeros((n, n))  # C order m[c, r]
+    K = None if tensor is None else np.ascontiguousarray(np.asarray(tensor, dtype=np.float64).T)  # column-major
+    st = _lib().ffo_nedelec_element(C.c_int(k), cf.ctypes.data_as(C.c_void_p), _f(vertices4).ctypes.data_as(C.c_void_p),
+                                    C.c_int(qdeg), C.c_double(alpha), None if K is None else K.ctypes.data_as(C.c_void_p),
+                                    C.c_double(beta), m.ctypes.data_as(C.c_void_p))
+    return st, m.T.copy()
+
+
+def nedelec_csc(k, vertices, tets, alpha=1.0, beta=1.0, cell_tensor=None):
+    """assemble_tetra_nedelec_curl_mass_csc / _weighted_csc restated.
+    cell_tensor: (T, 3, 3) per-cell tensors (weighted variant)."""
+    v, t = _f(vertices), _i(tets)
+    n = nedelec_dof_count(k)
+    nt = t.shape[0] * n * n
+    rows = np.zeros(nt, dtype=np.int32)
+    cols = np.zeros(nt, dtype=np.int32)
+    vals = np.zeros(nt)
+    nd = C.c_int(0)
+    K = None
+    if cell_tensor is not None:
+        K = np.ascontiguousarray(np.transpose(np.asarray(cell_tensor, dtype=np.float64), (0, 2, 1)).reshape(-1))
+    _check(_lib().ffo_nedelec_triplets(C.c_int(k), C.c_int(v.shape[0]), v.ctypes.data_as(C.c_void_p),
+                                       C.c_int(t.shape[0]), t.ctypes.data_as(C.c_void_p), C.c_double(alpha),
+                                       C.c_double(beta), None if K is None else K.ctypes.data_as(C.c_void_p),
+                                       rows.ctypes.data_as(C.c_void_p), cols.ctypes.data_as(C.c_void_p),
+                                       vals.ctypes.data_as(C.c_void_p), C.byref(nd)), "nedelec_triplets")
+    return csc_from_triplet(nd.value, nd.value, rows, cols, vals)

@@ -113,6 +113,30 @@ int ffo_solve_sparse(int n, const int *col_ptr, const int *row_idx,
                      int max_iterations, int breakdown_mode,
                      ffo_solver_stats *stats);
 
+/* ---- tetrahedral first-kind Nedelec (A11-A20) ---------------------------- */
+int ffo_nedelec_dof_count(int k);
+int ffo_nedelec_coefficients(int k, double *coeff);
+int ffo_nedelec_eval(int k, const double *coeff, const double p[3], double *vals,
+                     double *curls);
+int ffo_whitney(const double p[3], double *vals, double *curls);
+int ffo_face_basis_to_local(int k, const int perm[3], double *T);
+int ffo_edge_dof_map(int ntet, const int *tets, int *edges, int *gd, int *orient,
+                     int *nedges);
+int ffo_edge_dof_map_search(int ntet, const int *tets, int *edges, int *gd,
+                            int *orient, int *nedges);
+int ffo_nedelec_dof_map(int k, int ntet, const int *tets, int *gd,
+                        int *edge_orient, int *face_perm, int *ndofs,
+                        int *nedges, int *nfaces);
+int ffo_nedelec_basis_transform(int k, const int edge_orient[6],
+                                const int face_perm[12], double *T);
+int ffo_nedelec_element(int k, const double *coeff, const double v[12],
+                        int qdeg, double alpha, const double *tensor,
+                        double beta, double *m);
+int ffo_nedelec_triplets(int k, int nvert, const double *vertices, int ntet,
+                         const int *tets, double alpha, double beta,
+                         const double *cell_tensor, int *rows, int *cols,
+                         double *vals, int *ndofs);
+
 /* ---- 2D P1 (A29) -------------------------------------------------------- */
 int ffo_p1_2d_assemble_dense(int nvert, const double *vertices, int ntri,
                              const int *triangles, double mass_coefficient,

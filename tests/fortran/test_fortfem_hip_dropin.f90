@@ -58,6 +58,7 @@ program test_fortfem_hip_dropin
 contains
 
     subroutine record(condition, description)
+        use, intrinsic :: iso_fortran_env, only: output_unit
         logical, intent(in) :: condition
         character(len=*), intent(in) :: description
 
@@ -67,6 +68,7 @@ contains
             write (*, '(2a)') "FAIL ", description
             all_passed = .false.
         end if
+        flush (output_unit)
     end subroutine record
 
     real(dp) function det3(a)

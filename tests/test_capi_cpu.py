@@ -96,3 +96,17 @@ def test_workload_rng_is_reproducible():
     assert u.min() >= 0 and u.max() < 1
     # SplitMix64 reference value: first output of seed 0
     assert int(workloads.splitmix64(0, 1)[0]) == 0xE220A8397B1DCDAF
+
+
+def test_fortran_host_dropin_program():
+    """Host-only Fortran drop-in checks (mesh, Duffy rule through the C-ABI);
+    also the regression test for the Duffy size query, which once wrote all
+    points into a one-element array."""
+    import subprocess
+
+    exe = os.path.join(ROOT, "tests", "fortran", "test_fortfem_hip_host")
+    if not os.path.exists(exe):
+        pytest.skip("Fortran drop-in not built (no flang)")
+    out = subprocess.run([exe], capture_output=True, text=True, timeout=60)
+    assert out.returncode == 0, out.stdout + out.stderr
+    assert "host Fortran checks passed" in out.stdout
