@@ -404,6 +404,27 @@ class Matrix:
         self.ctx.check(self._lib.ffh_bench_spmv(self.handle, prec, reps, C.byref(ms), C.byref(by)))
         return ms.value, by.value
 
+    def solve(self, b, x, opts: SolverOptionsT):
+        """solve_sparse on the resident matrix: x (initial guess) is updated in place."""
+        n, _ = self.size()
+        bb = _f64(b)
+        if bb.shape[0] != n or np.asarray(x).shape[0] != n:
+            raise ValueError("solve_sparse: incompatible vector size")
+        xx = _f64(x).copy()
+        st = capi.SolverStats()
+        self.ctx.check(self._lib.ffh_solve_sparse(self.handle, _ptr(bb), _ptr(xx), C.byref(opts.to_c()),
+                                                  C.byref(st)))
+        x[...] = xx
+        method = "cg" if opts.method.strip().lower() == "cg" else "pcg"
+        return SolverStatsT.from_c(st, method)
+
+    def spmv_variant(self, precision="fp64", request=-1):
+        """The SpMV variant in use (autotuned on first use); request >= 0 forces one."""
+        out = C.c_int(0)
+        prec = capi.FP32 if precision == "fp32" else capi.FP64
+        self.ctx.check(self._lib.ffh_matrix_spmv_variant(self.handle, prec, request, C.byref(out)))
+        return out.value
+
 
 # ----------------------------------------------------- reference-named calls
 def spmv(matrix: CSC, vector, ctx: Context | None = None):
@@ -422,13 +443,7 @@ def solve_sparse(a_sparse: CSC, b, x, opts: SolverOptionsT, ctx: Context | None 
     bb = _f64(b)
     if bb.shape[0] != a_sparse.nrow or np.asarray(x).shape[0] != a_sparse.nrow:
         raise ValueError("solve_sparse: incompatible vector size")
-    m = Matrix.upload(a_sparse, ctx)
-    xx = _f64(x).copy()
-    st = capi.SolverStats()
-    m.ctx.check(m._lib.ffh_solve_sparse(m.handle, _ptr(bb), _ptr(xx), C.byref(opts.to_c()), C.byref(st)))
-    x[...] = xx
-    method = "cg" if opts.method.strip().lower() == "cg" else "pcg"
-    return SolverStatsT.from_c(st, method)
+    return Matrix.upload(a_sparse, ctx).solve(bb, x, opts)
 
 
 def assemble_tetra_lagrange_stiffness_csc(mesh_vertices, tetrahedra, degree, quadrature_degree,

@@ -525,6 +525,18 @@ int ffh_bench_spmv(ffh_matrix *M, int precision, int reps, double *avg_ms, doubl
     return guarded(M->ctx, [&] { *avg_ms = bench_spmv(M->ctx, *M, precision, reps, bytes); });
 }
 
+int ffh_matrix_spmv_variant(ffh_matrix *M, int precision, int request, int *chosen)
+{
+    if (!M) return FFH_ERR_HANDLE;
+    if (!chosen || (precision != FFH_FP64 && precision != FFH_FP32) ||
+        request >= FFH_SPMV_VARIANTS)
+        return FFH_ERR_INVALID;
+    return guarded(M->ctx, [&] {
+        ensure_workspace(M->ctx, *M);
+        *chosen = select_spmv_variant(M->ctx, *M, precision == FFH_FP32, request);
+    });
+}
+
 int ffh_set_profiling(ffh_context *ctx, int enable)
 {
     if (!ctx) return FFH_ERR_HANDLE;

@@ -110,6 +110,7 @@ struct DevMatrix {
     bool val32_valid = false;
     DBuf<int> rowblocks; // CSR-stream row blocks of the gather arrays
     int nrb = 0;
+    int variant[2] = {-1, -1}; // autotuned SpMV variant (FP64, FP32)
 };
 
 // Solver workspace attached to a matrix (kept resident between solves).

@@ -20,6 +20,7 @@
 // exactly the reference's.
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 
 #include "ffh_internal.hpp"
 #include "ffh_krylov.hpp"
@@ -134,6 +135,54 @@ __device__ __forceinline__ void csr_stream(const int *__restrict__ ptr,
     }
 }
 
+// Vector CSR: a group of G lanes per row (grid-stride over rows); each lane
+// issues two independent (index, value) loads before its gathers, then the
+// group reduces by shuffles.  Used when it beats the stream kernel for a
+// matrix (chosen by the autotuner below).
+template <int G, typename MT, typename VT, typename E>
+__device__ __forceinline__ void csr_vector(int n, const int *__restrict__ ptr,
+                                           const int *__restrict__ idx,
+                                           const MT *__restrict__ val,
+                                           const VT *__restrict__ x, E &epi, double &d0,
+                                           double &d1)
+{
+    constexpr int RPB = kStreamBlock / G;
+    const int lane = threadIdx.x % G;
+    const int ngrp = gridDim.x * RPB;
+    for (int row = blockIdx.x * RPB + threadIdx.x / G; row < n; row += ngrp) {
+        const int b = ptr[row], e = ptr[row + 1];
+        VT s = VT(0);
+        int k = b + lane;
+        for (; k + G < e; k += 2 * G) {
+            const int c0 = __builtin_nontemporal_load(idx + k);
+            const int c1 = __builtin_nontemporal_load(idx + k + G);
+            const MT v0 = __builtin_nontemporal_load(val + k);
+            const MT v1 = __builtin_nontemporal_load(val + k + G);
+            s += (VT)v0 * x[c0];
+            s += (VT)v1 * x[c1];
+        }
+        if (k < e) s += (VT)val[k] * x[idx[k]];
+#pragma unroll
+        for (int off = G / 2; off > 0; off >>= 1) s += __shfl_xor(s, off, G);
+        if (lane == 0) epi(row, s, d0, d1);
+    }
+}
+
+// Row engine of a launch: V = 0 stream, otherwise vector CSR with G = V lanes.
+template <int V, typename MT, typename VT, typename E>
+__device__ __forceinline__ void row_engine(int n, const int *__restrict__ ptr,
+                                           const int *__restrict__ idx,
+                                           const MT *__restrict__ val,
+                                           const VT *__restrict__ x,
+                                           const int *__restrict__ rowblocks, int nrb, E &epi,
+                                           double &d0, double &d1)
+{
+    if constexpr (V == 0)
+        csr_stream<MT, VT>(ptr, idx, val, x, rowblocks, nrb, epi, d0, d1);
+    else
+        csr_vector<V, MT, VT>(n, ptr, idx, val, x, epi, d0, d1);
+}
+
 template <typename VT>
 struct EpiStore {
     VT *y;
@@ -178,8 +227,8 @@ struct EpiResidual { // r = b - A x ; r.r
 };
 
 // --------------------------------------------------------------- kernels --
-template <typename MT, typename VT>
-__global__ __launch_bounds__(kStreamBlock) void k_spmv(const int *__restrict__ ptr,
+template <int V, typename MT, typename VT>
+__global__ __launch_bounds__(kStreamBlock) void k_spmv(int n, const int *__restrict__ ptr,
                                                        const int *__restrict__ idx,
                                                        const MT *__restrict__ val,
                                                        const VT *__restrict__ x,
@@ -189,20 +238,20 @@ __global__ __launch_bounds__(kStreamBlock) void k_spmv(const int *__restrict__ p
 {
     EpiStore<VT> epi{y};
     double d0 = 0.0, d1 = 0.0;
-    csr_stream<MT, VT>(ptr, idx, val, x, rowblocks, nrb, epi, d0, d1);
+    row_engine<V, MT, VT>(n, ptr, idx, val, x, rowblocks, nrb, epi, d0, d1);
 }
 
 // r = b - A x ; z = D r ; p = z ; partials of r.r and r.z
-template <typename MT, typename VT>
+template <int V, typename MT, typename VT>
 __global__ __launch_bounds__(kStreamBlock) void k_pcg_init(
-    const int *__restrict__ ptr, const int *__restrict__ idx, const MT *__restrict__ val,
+    int n, const int *__restrict__ ptr, const int *__restrict__ idx, const MT *__restrict__ val,
     const VT *__restrict__ x, const VT *__restrict__ b, VT *__restrict__ r,
     VT *__restrict__ p, const VT *__restrict__ dinv, const int *__restrict__ rowblocks,
     int nrb, double *__restrict__ partials, int stride)
 {
     EpiInit<VT> epi{b, dinv, r, p};
     double d0 = 0.0, d1 = 0.0;
-    csr_stream<MT, VT>(ptr, idx, val, x, rowblocks, nrb, epi, d0, d1);
+    row_engine<V, MT, VT>(n, ptr, idx, val, x, rowblocks, nrb, epi, d0, d1);
     const double s0 = block_sum<kStreamBlock>(d0);
     const double s1 = block_sum<kStreamBlock>(d1);
     if (threadIdx.x == 0) {
@@ -230,16 +279,16 @@ __global__ __launch_bounds__(kBlock) void k_pcg_init_finish(const double *__rest
 }
 
 // Ap = A p ; partials of p.Ap
-template <typename MT, typename VT>
+template <int V, typename MT, typename VT>
 __global__ __launch_bounds__(kStreamBlock) void k_pcg_spmv(
-    const int *__restrict__ ptr, const int *__restrict__ idx, const MT *__restrict__ val,
+    int n, const int *__restrict__ ptr, const int *__restrict__ idx, const MT *__restrict__ val,
     const VT *__restrict__ p, VT *__restrict__ ap, const int *__restrict__ rowblocks,
     int nrb, double *__restrict__ partials, const KrylovCtl *ctl)
 {
     if (ctl->done) return;
     EpiAp<VT> epi{p, ap};
     double d0 = 0.0, d1 = 0.0;
-    csr_stream<MT, VT>(ptr, idx, val, p, rowblocks, nrb, epi, d0, d1);
+    row_engine<V, MT, VT>(n, ptr, idx, val, p, rowblocks, nrb, epi, d0, d1);
     const double s0 = block_sum<kStreamBlock>(d0);
     if (threadIdx.x == 0) partials[blockIdx.x] = s0;
 }
@@ -342,15 +391,16 @@ __global__ __launch_bounds__(kBlock) void k_jacobi(int n, const int *__restrict_
 
 // r = b - A x (FP64) and partials of ||r||^2 -- the true residual of the
 // refinement loop and of the final FP64 report.
+template <int V>
 __global__ __launch_bounds__(kStreamBlock) void k_residual(
-    const int *__restrict__ ptr, const int *__restrict__ idx,
+    int n, const int *__restrict__ ptr, const int *__restrict__ idx,
     const double *__restrict__ val, const double *__restrict__ x,
     const double *__restrict__ b, double *__restrict__ r,
     const int *__restrict__ rowblocks, int nrb, double *__restrict__ partials)
 {
     EpiResidual epi{b, r};
     double d0 = 0.0, d1 = 0.0;
-    csr_stream<double, double>(ptr, idx, val, x, rowblocks, nrb, epi, d0, d1);
+    row_engine<V, double, double>(n, ptr, idx, val, x, rowblocks, nrb, epi, d0, d1);
     const double s0 = block_sum<kStreamBlock>(d0);
     if (threadIdx.x == 0) partials[blockIdx.x] = s0;
 }
@@ -397,10 +447,31 @@ static int grid_for(const Context *ctx, int64_t items, int per_block)
     return (int)std::max<int64_t>(1, std::min<int64_t>(need, grid_cap(ctx)));
 }
 
-static int stream_grid(const Context *ctx, const DevMatrix &A)
+// SpMV variants: engine (0 = CSR stream, G = vector CSR with G lanes per row)
+// and a grid multiplier over the base cap of 4 blocks per CU.
+struct SpmvVariant {
+    int engine, gmul;
+};
+static const SpmvVariant kVariants[] = {{0, 1}, {0, 2}, {4, 1}, {8, 1}, {8, 2},
+                                        {16, 1}, {16, 2}, {32, 1}, {32, 2}};
+constexpr int kNumVariants = sizeof(kVariants) / sizeof(kVariants[0]);
+
+static int variant_grid(const Context *ctx, const DevMatrix &A, int vi)
 {
-    return std::max(1, std::min(A.nrb, grid_cap(ctx)));
+    const SpmvVariant v = kVariants[vi];
+    const int64_t cap = (int64_t)grid_cap(ctx) * v.gmul;
+    const int64_t need = v.engine == 0 ? A.nrb : ((int64_t)A.n * v.engine + kStreamBlock - 1) / kStreamBlock;
+    return (int)std::max<int64_t>(1, std::min(need, cap));
 }
+
+#define FFH_ENGINE(eng, ...)                                                  \
+    switch (eng) {                                                           \
+    case 0: { constexpr int VV = 0; __VA_ARGS__; } break;                     \
+    case 4: { constexpr int VV = 4; __VA_ARGS__; } break;                     \
+    case 8: { constexpr int VV = 8; __VA_ARGS__; } break;                     \
+    case 16: { constexpr int VV = 16; __VA_ARGS__; } break;                   \
+    default: { constexpr int VV = 32; __VA_ARGS__; } break;                   \
+    }
 
 // Row blocks of the CSR-stream kernels: consecutive rows whose nonzeros fit
 // the LDS stage; a longer row gets a block of its own.
@@ -420,6 +491,7 @@ void build_rowblocks(Context *ctx, DevMatrix &A, const std::vector<int> &ptr)
     if (n == 0) rb.push_back(0);
     A.nrb = (int)rb.size() - 1;
     A.rowblocks.upload(rb, ctx->stream);
+    A.variant[0] = A.variant[1] = -1;
     FFH_HIP(hipStreamSynchronize(ctx->stream));
 }
 
@@ -430,7 +502,7 @@ Workspace::~Workspace()
     if (host_ctl) (void)hipHostFree(host_ctl);
 }
 
-static int partial_stride(const Context *ctx) { return grid_cap(ctx); }
+static int partial_stride(const Context *ctx) { return grid_cap(ctx) * 2; }
 
 void ensure_workspace(Context *ctx, Matrix &M)
 {
@@ -475,41 +547,109 @@ void ensure_val32(Context *ctx, DevMatrix &A)
     A.val32_valid = true;
 }
 
+template <typename MT, typename VT>
+static void launch_spmv(Context *ctx, const DevMatrix &A, const MT *val, const VT *x, VT *y,
+                        int vi)
+{
+    const int grid = variant_grid(ctx, A, vi);
+    FFH_ENGINE(kVariants[vi].engine, {
+        hipLaunchKernelGGL((k_spmv<VV, MT, VT>), dim3(grid), dim3(kStreamBlock), 0,
+                           ctx->stream, A.n, A.gptr(), A.gidx(), val, x, y, A.rowblocks.p,
+                           A.nrb);
+    });
+    FFH_HIP(hipGetLastError());
+}
+
+// Autotuner: the fastest SpMV variant for this matrix and precision, timed
+// with HIP events on the context stream (10 launches each after a warm-up).
+// FFH_SPMV_VARIANT=<index> forces a variant.
+template <typename MT, typename VT>
+static int tune_spmv(Context *ctx, DevMatrix &A, const MT *val, const VT *x, VT *y)
+{
+    if (const char *env = std::getenv("FFH_SPMV_VARIANT")) {
+        int v = std::atoi(env);
+        if (v >= 0 && v < kNumVariants) return v;
+    }
+    if (A.nnz == 0) return 0;
+    int best = 0;
+    float best_ms = 1e30f;
+    for (int vi = 0; vi < kNumVariants; ++vi) {
+        if (kVariants[vi].engine == 0 && kVariants[vi].gmul > 1 &&
+            A.nrb <= grid_cap(ctx))
+            continue; // no more row blocks than the base grid
+        launch_spmv<MT, VT>(ctx, A, val, x, y, vi);
+        FFH_HIP(hipEventRecord(ctx->ev_start, ctx->stream));
+        for (int r = 0; r < 10; ++r) launch_spmv<MT, VT>(ctx, A, val, x, y, vi);
+        FFH_HIP(hipEventRecord(ctx->ev_stop, ctx->stream));
+        FFH_HIP(hipEventSynchronize(ctx->ev_stop));
+        float ms = 0.0f;
+        FFH_HIP(hipEventElapsedTime(&ms, ctx->ev_start, ctx->ev_stop));
+        if (ms < best_ms) {
+            best_ms = ms;
+            best = vi;
+        }
+    }
+    return best;
+}
+
+static int spmv_variant(Context *ctx, Matrix &M, bool f32)
+{
+    DevMatrix &A = M.m;
+    int &v = A.variant[f32 ? 1 : 0];
+    if (v < 0) {
+        if (f32)
+            v = tune_spmv<float, float>(ctx, A, A.gval32(), M.ws.p32.p, M.ws.ap32.p);
+        else
+            v = tune_spmv<double, double>(ctx, A, A.gval(), M.ws.p.p, M.ws.ap.p);
+    }
+    return v;
+}
+
+int select_spmv_variant(Context *ctx, Matrix &M, bool f32, int request)
+{
+    static_assert(kNumVariants == FFH_SPMV_VARIANTS, "variant table and header disagree");
+    if (f32) ensure_val32(ctx, M.m);
+    if (request >= 0) {
+        M.m.variant[f32 ? 1 : 0] = request;
+        invalidate_graphs(M.ws);
+        return request;
+    }
+    return spmv_variant(ctx, M, f32);
+}
+
 void spmv64(Context *ctx, const DevMatrix &A, const double *x, double *y)
 {
-    hipLaunchKernelGGL((k_spmv<double, double>), dim3(stream_grid(ctx, A)),
-                       dim3(kStreamBlock), 0, ctx->stream, A.gptr(), A.gidx(), A.gval(), x,
-                       y, A.rowblocks.p, A.nrb);
-    FFH_HIP(hipGetLastError());
+    const int vi = A.variant[0] >= 0 ? A.variant[0] : 0;
+    launch_spmv<double, double>(ctx, A, A.gval(), x, y, vi);
 }
 
 void spmv32(Context *ctx, DevMatrix &A, const float *x, float *y)
 {
     ensure_val32(ctx, A);
-    hipLaunchKernelGGL((k_spmv<float, float>), dim3(stream_grid(ctx, A)),
-                       dim3(kStreamBlock), 0, ctx->stream, A.gptr(), A.gidx(), A.gval32(),
-                       x, y, A.rowblocks.p, A.nrb);
-    FFH_HIP(hipGetLastError());
+    const int vi = A.variant[1] >= 0 ? A.variant[1] : 0;
+    launch_spmv<float, float>(ctx, A, A.gval32(), x, y, vi);
 }
 
 // Launch `iters` PCG iterations (3 kernels each) of precision VT.
 template <typename MT, typename VT>
 static void launch_iterations(Context *ctx, Matrix &M, int iters, VT *x, VT *r,
-                              VT *p, VT *ap, const VT *dinv, const MT *val,
+                              VT *p, VT *ap, const VT *dinv, const MT *val, int vi,
                               int prof_slot)
 {
     const DevMatrix &A = M.m;
     Workspace &w = M.ws;
     const int n = A.n;
     const int gridv = grid_for(ctx, n, kBlock);
-    const int grids = stream_grid(ctx, A);
+    const int grids = variant_grid(ctx, A, vi);
     const int stride = partial_stride(ctx);
     for (int it = 0; it < iters; ++it) {
         const int parity = it & 1; // chunks are even, so this is the global parity
         if (prof_slot >= 0) ctx->prof_mark(prof_slot, true);
-        hipLaunchKernelGGL((k_pcg_spmv<MT, VT>), dim3(grids), dim3(kStreamBlock), 0,
-                           ctx->stream, A.gptr(), A.gidx(), val, p, ap, A.rowblocks.p,
-                           A.nrb, w.partials.p, w.ctl.p);
+        FFH_ENGINE(kVariants[vi].engine, {
+            hipLaunchKernelGGL((k_pcg_spmv<VV, MT, VT>), dim3(grids), dim3(kStreamBlock), 0,
+                               ctx->stream, n, A.gptr(), A.gidx(), val, p, ap, A.rowblocks.p,
+                               A.nrb, w.partials.p, w.ctl.p);
+        });
         if (prof_slot >= 0) ctx->prof_mark(prof_slot, false);
         hipLaunchKernelGGL((k_pcg_update<VT>), dim3(gridv), dim3(kBlock), 0, ctx->stream,
                            n, x, r, p, ap, dinv, w.partials.p, stride, grids, parity,
@@ -543,10 +683,13 @@ static void pcg_device(Context *ctx, Matrix &M, const MT *val, const VT *b, VT *
     *w.host_ctl = init;
     FFH_HIP(hipMemcpyAsync(w.ctl.p, w.host_ctl, sizeof(KrylovCtl),
                            hipMemcpyHostToDevice, ctx->stream));
-    const int gi = stream_grid(ctx, A);
-    hipLaunchKernelGGL((k_pcg_init<MT, VT>), dim3(gi), dim3(kStreamBlock), 0, ctx->stream,
-                       A.gptr(), A.gidx(), val, x, b, r, p, dinv, A.rowblocks.p, A.nrb,
-                       w.partials.p, partial_stride(ctx));
+    const int vi = spmv_variant(ctx, M, sizeof(VT) == 4);
+    const int gi = variant_grid(ctx, A, vi);
+    FFH_ENGINE(kVariants[vi].engine, {
+        hipLaunchKernelGGL((k_pcg_init<VV, MT, VT>), dim3(gi), dim3(kStreamBlock), 0,
+                           ctx->stream, n, A.gptr(), A.gidx(), val, x, b, r, p, dinv,
+                           A.rowblocks.p, A.nrb, w.partials.p, partial_stride(ctx));
+    });
     hipLaunchKernelGGL(k_pcg_init_finish, dim3(1), dim3(kBlock), 0, ctx->stream, w.partials.p,
                        partial_stride(ctx), gi, w.ctl.p);
     FFH_HIP(hipGetLastError());
@@ -555,25 +698,25 @@ static void pcg_device(Context *ctx, Matrix &M, const MT *val, const VT *b, VT *
     const bool profiling = ctx->profiling;
     if (profiling) use_graph = 0;
     const int gslot = sizeof(VT) == 8 ? 0 : 1;
-    if (use_graph && (w.graph[gslot] == nullptr || w.graph_chunk[gslot] != chunk)) {
+    if (use_graph && (w.graph[gslot] == nullptr || w.graph_chunk[gslot] != chunk * 64 + vi)) {
         if (w.graph[gslot]) {
             (void)hipGraphExecDestroy(w.graph[gslot]);
             w.graph[gslot] = nullptr;
         }
         hipGraph_t g;
         FFH_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-        launch_iterations<MT, VT>(ctx, M, chunk, x, r, p, ap, dinv, val, -1);
+        launch_iterations<MT, VT>(ctx, M, chunk, x, r, p, ap, dinv, val, vi, -1);
         FFH_HIP(hipStreamEndCapture(ctx->stream, &g));
         FFH_HIP(hipGraphInstantiate(&w.graph[gslot], g, nullptr, nullptr, 0));
         (void)hipGraphDestroy(g);
-        w.graph_chunk[gslot] = chunk;
+        w.graph_chunk[gslot] = chunk * 64 + vi;
     }
     // graphs bake in the vector pointers; they are the workspace's own
     for (;;) {
         if (use_graph)
             FFH_HIP(hipGraphLaunch(w.graph[gslot], ctx->stream));
         else
-            launch_iterations<MT, VT>(ctx, M, chunk, x, r, p, ap, dinv, val,
+            launch_iterations<MT, VT>(ctx, M, chunk, x, r, p, ap, dinv, val, vi,
                                       profiling ? slot : -1);
         FFH_HIP(hipMemcpyAsync(w.host_ctl, w.ctl.p, sizeof(KrylovCtl),
                                hipMemcpyDeviceToHost, ctx->stream));
@@ -588,10 +731,13 @@ static void launch_residual(Context *ctx, Matrix &M)
 {
     DevMatrix &A = M.m;
     Workspace &w = M.ws;
-    const int g = stream_grid(ctx, A);
-    hipLaunchKernelGGL(k_residual, dim3(g), dim3(kStreamBlock), 0, ctx->stream, A.gptr(),
-                       A.gidx(), A.gval(), w.x.p, w.b.p, w.r.p, A.rowblocks.p, A.nrb,
-                       w.partials.p);
+    const int vi = spmv_variant(ctx, M, false);
+    const int g = variant_grid(ctx, A, vi);
+    FFH_ENGINE(kVariants[vi].engine, {
+        hipLaunchKernelGGL((k_residual<VV>), dim3(g), dim3(kStreamBlock), 0, ctx->stream, A.n,
+                           A.gptr(), A.gidx(), A.gval(), w.x.p, w.b.p, w.r.p, A.rowblocks.p,
+                           A.nrb, w.partials.p);
+    });
     hipLaunchKernelGGL(k_residual_finish, dim3(1), dim3(kBlock), 0, ctx->stream, w.partials.p,
                        g, w.ctl.p);
     FFH_HIP(hipGetLastError());
@@ -731,6 +877,7 @@ double bench_spmv(Context *ctx, Matrix &M, int precision, int reps, double *byte
     const int sv = f32 ? 4 : 8;
     // algorithmic bytes: values + column indices + row pointers + x + y
     *bytes = (double)A.nnz * (sv + 4) + 4.0 * (n + 1) + (double)n * 2 * sv;
+    spmv_variant(ctx, M, f32);
     auto once = [&]() {
         if (f32) spmv32(ctx, A, w.p32.p, w.ap32.p);
         else spmv64(ctx, A, w.p.p, w.ap.p);

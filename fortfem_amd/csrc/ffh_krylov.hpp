@@ -13,6 +13,7 @@ void solve_fp64(Context *ctx, Matrix &M, const double *b, double *x,
                 const ffh_solver_options &o, ffh_solver_stats &st);
 void solve_mixed(Context *ctx, Matrix &M, const double *b, double *x,
                  const ffh_solver_options &o, ffh_solver_stats &st);
+int select_spmv_variant(Context *ctx, Matrix &M, bool f32, int request);
 double bench_spmv(Context *ctx, Matrix &M, int precision, int reps,
                   double *bytes);
 } // namespace ffh

@@ -214,6 +214,13 @@ int ffh_solve_sparse(ffh_matrix *mat, const double *b, double *x,
  * bytes = algorithmic bytes of one launch. */
 int ffh_bench_spmv(ffh_matrix *mat, int precision, int reps, double *avg_ms,
                    double *bytes);
+/* SpMV variant of a matrix (no reference counterpart: a tuning control).
+ * request >= 0 forces variant `request` (0 .. FFH_SPMV_VARIANTS-1) for the
+ * precision; request < 0 keeps the current choice, running the autotuner
+ * (HIP-event timing of every variant) if none is made yet.  *chosen receives
+ * the variant in use. */
+#define FFH_SPMV_VARIANTS 9
+int ffh_matrix_spmv_variant(ffh_matrix *mat, int precision, int request, int *chosen);
 /* Kernel timing inside solves: when enabled, every SpMV launch of a solve
  * (which 0 = FP64 SpMV+dot, 1 = FP32 SpMV+dot) is bracketed by HIP events on
  * the context stream; ffh_last_solve_profile returns the launch count and
