@@ -561,46 +561,66 @@ static void launch_spmv(Context *ctx, const DevMatrix &A, const MT *val, const V
 }
 
 // Autotuner: the fastest SpMV variant for this matrix and precision, timed
-// with HIP events on the context stream (10 launches each after a warm-up).
-// FFH_SPMV_VARIANT=<index> forces a variant.
+// with events of its own (10 launches each after a warm-up) on the context
+// stream, into scratch vectors of its own, so that no solver vector or timing
+// event is touched.  FFH_SPMV_VARIANT=<index> forces a variant.
 template <typename MT, typename VT>
-static int tune_spmv(Context *ctx, DevMatrix &A, const MT *val, const VT *x, VT *y)
+static int tune_spmv(Context *ctx, DevMatrix &A, const MT *val)
 {
     if (const char *env = std::getenv("FFH_SPMV_VARIANT")) {
         int v = std::atoi(env);
         if (v >= 0 && v < kNumVariants) return v;
     }
-    if (A.nnz == 0) return 0;
+    if (A.nnz == 0 || A.n == 0) return 0;
+    DBuf<VT> tx, ty;
+    tx.alloc(A.n);
+    ty.alloc(A.n);
+    FFH_HIP(hipMemsetAsync(tx.p, 0, sizeof(VT) * A.n, ctx->stream));
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    FFH_HIP(hipEventCreate(&e0));
     int best = 0;
-    float best_ms = 1e30f;
-    for (int vi = 0; vi < kNumVariants; ++vi) {
-        if (kVariants[vi].engine == 0 && kVariants[vi].gmul > 1 &&
-            A.nrb <= grid_cap(ctx))
-            continue; // no more row blocks than the base grid
-        launch_spmv<MT, VT>(ctx, A, val, x, y, vi);
-        FFH_HIP(hipEventRecord(ctx->ev_start, ctx->stream));
-        for (int r = 0; r < 10; ++r) launch_spmv<MT, VT>(ctx, A, val, x, y, vi);
-        FFH_HIP(hipEventRecord(ctx->ev_stop, ctx->stream));
-        FFH_HIP(hipEventSynchronize(ctx->ev_stop));
-        float ms = 0.0f;
-        FFH_HIP(hipEventElapsedTime(&ms, ctx->ev_start, ctx->ev_stop));
-        if (ms < best_ms) {
-            best_ms = ms;
-            best = vi;
+    try {
+        FFH_HIP(hipEventCreate(&e1));
+        float best_ms = 1e30f;
+        for (int vi = 0; vi < kNumVariants; ++vi) {
+            if (kVariants[vi].engine == 0 && kVariants[vi].gmul > 1 &&
+                A.nrb <= grid_cap(ctx))
+                continue; // no more row blocks than the base grid
+            launch_spmv<MT, VT>(ctx, A, val, tx.p, ty.p, vi);
+            FFH_HIP(hipEventRecord(e0, ctx->stream));
+            for (int r = 0; r < 10; ++r) launch_spmv<MT, VT>(ctx, A, val, tx.p, ty.p, vi);
+            FFH_HIP(hipEventRecord(e1, ctx->stream));
+            FFH_HIP(hipEventSynchronize(e1));
+            float ms = 0.0f;
+            FFH_HIP(hipEventElapsedTime(&ms, e0, e1));
+            if (ms < best_ms) {
+                best_ms = ms;
+                best = vi;
+            }
         }
+    } catch (...) {
+        (void)hipEventDestroy(e0);
+        if (e1) (void)hipEventDestroy(e1);
+        throw;
     }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
     return best;
 }
 
+// The variant of a matrix for a precision, tuned on first use.  Call outside
+// graph capture and before a solve's timed region.
 static int spmv_variant(Context *ctx, Matrix &M, bool f32)
 {
     DevMatrix &A = M.m;
     int &v = A.variant[f32 ? 1 : 0];
     if (v < 0) {
-        if (f32)
-            v = tune_spmv<float, float>(ctx, A, A.gval32(), M.ws.p32.p, M.ws.ap32.p);
-        else
-            v = tune_spmv<double, double>(ctx, A, A.gval(), M.ws.p.p, M.ws.ap.p);
+        if (f32) {
+            ensure_val32(ctx, A);
+            v = tune_spmv<float, float>(ctx, A, A.gval32());
+        } else {
+            v = tune_spmv<double, double>(ctx, A, A.gval());
+        }
     }
     return v;
 }
@@ -748,6 +768,7 @@ void solve_fp64(Context *ctx, Matrix &M, const double *b, double *x,
                 const ffh_solver_options &o, ffh_solver_stats &st)
 {
     ensure_workspace(ctx, M);
+    spmv_variant(ctx, M, false);
     Workspace &w = M.ws;
     KrylovCtl out{};
     if (ctx->profiling) ctx->prof_begin();
@@ -794,6 +815,8 @@ void solve_mixed(Context *ctx, Matrix &M, const double *b, double *x,
     ensure_workspace32(ctx, M);
     DevMatrix &A = M.m;
     ensure_val32(ctx, A);
+    spmv_variant(ctx, M, false);
+    spmv_variant(ctx, M, true);
     Workspace &w = M.ws;
     const int n = A.n;
     if (ctx->profiling) ctx->prof_begin();
