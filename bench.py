@@ -30,6 +30,13 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 
 
+# HBM bytes per launch of the roofline kernel, from rocprofv3 --pmc FETCH_SIZE
+# and WRITE_SIZE passes over this script (FETCH_SIZE scaled by 2.0, WRITE_SIZE
+# by 1.0, the factors a 256 MiB device copy reads), CSR-stream engine with the
+# x2 grid.  Re-measure when the kernel changes.
+MEASURED_TRAFFIC = {"c2": 66.1e6}
+
+
 def spmv_bytes(nnz: int, n: int, sv: int) -> float:
     """Algorithmic bytes of one CSR-gather SpMV: values + int32 column indices
     + int32 row pointers + x once + y once."""
@@ -207,7 +214,7 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": None,
+            "traffic": MEASURED_TRAFFIC.get(args.workload),
             "avg_launch_ms": avg_ms,
             "launches": cnt,
             "bytes_per_launch": b32,

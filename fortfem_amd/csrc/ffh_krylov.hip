@@ -114,11 +114,28 @@ __device__ __forceinline__ void csr_stream(const int *__restrict__ ptr,
                 if (k < cnt) prod[k] = (VT)v[u] * x[c[u]];
             }
             __syncthreads();
-            for (int r = r0 + threadIdx.x; r < r1; r += BS) {
-                const int b = ptr[r] - k0, e = ptr[r + 1] - k0;
+            const int nrows = r1 - r0;
+            if (2 * nrows > BS) {
+                for (int r = r0 + threadIdx.x; r < r1; r += BS) {
+                    const int b = ptr[r] - k0, e = ptr[r + 1] - k0;
+                    VT s = VT(0);
+                    for (int k = b; k < e; ++k) s += prod[k];
+                    epi(r, s, d0, d1);
+                }
+            } else {
+                // gs lanes per row (a power of two, 2..64, gs * nrows <= BS):
+                // every row of the block is summed in one pass
+                int gs = 2;
+                while (gs < 64 && 2 * gs * nrows <= BS) gs *= 2;
+                const int lane = threadIdx.x & (gs - 1);
+                const int r = r0 + threadIdx.x / gs;
                 VT s = VT(0);
-                for (int k = b; k < e; ++k) s += prod[k];
-                epi(r, s, d0, d1);
+                if (r < r1) {
+                    const int b = ptr[r] - k0, e = ptr[r + 1] - k0;
+                    for (int k = b + lane; k < e; k += gs) s += prod[k];
+                }
+                for (int off = gs >> 1; off > 0; off >>= 1) s += __shfl_xor(s, off, gs);
+                if (lane == 0 && r < r1) epi(r, s, d0, d1);
             }
             __syncthreads();
         } else {
@@ -452,8 +469,8 @@ static int grid_for(const Context *ctx, int64_t items, int per_block)
 struct SpmvVariant {
     int engine, gmul;
 };
-static const SpmvVariant kVariants[] = {{0, 1}, {0, 2}, {4, 1}, {8, 1}, {8, 2},
-                                        {16, 1}, {16, 2}, {32, 1}, {32, 2}};
+static const SpmvVariant kVariants[] = {{0, 1}, {0, 2}, {0, 4}, {4, 2}, {8, 1},
+                                        {8, 2}, {16, 2}, {32, 2}, {32, 4}};
 constexpr int kNumVariants = sizeof(kVariants) / sizeof(kVariants[0]);
 
 static int variant_grid(const Context *ctx, const DevMatrix &A, int vi)
@@ -502,7 +519,7 @@ Workspace::~Workspace()
     if (host_ctl) (void)hipHostFree(host_ctl);
 }
 
-static int partial_stride(const Context *ctx) { return grid_cap(ctx) * 2; }
+static int partial_stride(const Context *ctx) { return grid_cap(ctx) * 4; }
 
 void ensure_workspace(Context *ctx, Matrix &M)
 {
@@ -584,8 +601,8 @@ static int tune_spmv(Context *ctx, DevMatrix &A, const MT *val)
         float best_ms = 1e30f;
         for (int vi = 0; vi < kNumVariants; ++vi) {
             if (kVariants[vi].engine == 0 && kVariants[vi].gmul > 1 &&
-                A.nrb <= grid_cap(ctx))
-                continue; // no more row blocks than the base grid
+                A.nrb <= grid_cap(ctx) * (kVariants[vi].gmul / 2))
+                continue; // same grid as the variant before it
             launch_spmv<MT, VT>(ctx, A, val, tx.p, ty.p, vi);
             FFH_HIP(hipEventRecord(e0, ctx->stream));
             for (int r = 0; r < 10; ++r) launch_spmv<MT, VT>(ctx, A, val, tx.p, ty.p, vi);
