@@ -288,9 +288,16 @@ class Problem:
         return g
 
     def assemble(self, quadrature_degree, kappa=1.0, mass=0.0, cell_kappa=None, precision="fp64",
-                 reuse_coefficients=False):
+                 reuse_coefficients=False, cell_tensor=None):
+        """Assemble the family's operator.  H1: kappa (stiffness) and mass;
+        Nedelec: kappa = curl coefficient alpha, mass = beta, or cell_tensor
+        (T, 3, 3) symmetric tensors on the curl term (weighted variant)."""
         prec = capi.FP32 if precision == "fp32" else capi.FP64
-        if cell_kappa is not None or reuse_coefficients:
+        if cell_tensor is not None:
+            K = np.ascontiguousarray(np.asarray(cell_tensor, dtype=np.float64).transpose(0, 2, 1))
+            self._check(self._lib.ffh_assemble(self.handle, quadrature_degree, capi.COEFF_CELL_TENSOR,
+                                               _ptr(K.reshape(-1)), float(mass), prec))
+        elif cell_kappa is not None or reuse_coefficients:
             ck = None if reuse_coefficients else _f64(cell_kappa)
             self._check(self._lib.ffh_assemble(self.handle, quadrature_degree, capi.COEFF_CELL,
                                                _ptr(ck), float(mass), prec))
@@ -463,6 +470,100 @@ def assemble_tetra_lagrange_stiffness_csc(mesh_vertices, tetrahedra, degree, qua
         p.pattern_build(capi.FAMILY_H1, degree)
         p.assemble(quadrature_degree, kappa=1.0 if stiffness_coefficient is None else stiffness_coefficient,
                    mass=0.0 if mass_coefficient is None else mass_coefficient, cell_kappa=cell_coefficient)
+        a = p.matrix(capi.FULL).download()
+    except FFHError as e:
+        if e.code == capi.FFH_ERR_DEVICE:
+            raise
+        return None, status
+    return a, FortsparseStatus(0, "")
+
+
+def nedelec_dof_count(k):
+    return k * (k + 2) * (k + 3) // 2
+
+
+def build_tetra_nedelec_dof_map(order, tetrahedra):
+    """build_tetra_nedelec_dof_map (tetra_nedelec_global_dof_map.f90:15-105),
+    host routine of the library -> (global_dofs (T, n) 1-based,
+    edge_orientations (T, 6), face_permutations (T, 4, 3), ndofs)."""
+    t = _i32(np.asarray(tetrahedra).reshape(-1, 4))
+    T = t.shape[0]
+    n = nedelec_dof_count(order)
+    gd = np.zeros((T, n), dtype=np.int32)
+    eo = np.zeros((T, 6), dtype=np.int32)
+    fp = np.zeros((T, 4, 3), dtype=np.int32)
+    nd = C.c_int(0)
+    code = capi.load().ffh_nedelec_dof_map(order, T, _ptr(t), 1, _ptr(gd), _ptr(eo), _ptr(fp), C.byref(nd))
+    capi.check(code)
+    return gd, eo, fp, nd.value
+
+
+def evaluate_tetra_nedelec_first_kind(order, points):
+    """evaluate_tetra_nedelec_first_kind (tetra_nedelec_arbitrary_order.f90:75-106)
+    at reference points (npts, 3) -> (values, curls), each (npts, n, 3)."""
+    p = _f64(np.atleast_2d(points))
+    n = nedelec_dof_count(order)
+    vals = np.zeros((p.shape[0], n, 3))
+    curls = np.zeros((p.shape[0], n, 3))
+    capi.check(capi.load().ffh_nedelec_basis(order, p.shape[0], _ptr(p), _ptr(vals), _ptr(curls)))
+    return vals, curls
+
+
+def assemble_tetra_nedelec_curl_mass_csc(mesh_vertices, tetrahedra, curl_coefficient=None,
+                                         mass_coefficient=None, order=None, cell_coefficient=None,
+                                         precision="fp64", ctx: Context | None = None):
+    """assemble_tetra_nedelec_curl_mass_csc (assembly_tetra_nedelec_3d.f90:
+    2042-2112): alpha (curl) and beta (mass) default to 1, order to 1.
+    cell_coefficient: optional per-cell curl coefficient (new argument).
+    -> (CSC, status)."""
+    status = FortsparseStatus(FORTSPARSE_INVALID_MATRIX, "Tetrahedral Nedelec sparse assembly failed")
+    k = 1 if order is None else int(order)
+    v = np.asarray(mesh_vertices)
+    t = np.asarray(tetrahedra)
+    if k < 1 or v.ndim != 2 or v.shape[1] != 3 or t.ndim != 2 or t.shape[1] != 4 or t.shape[0] < 1:
+        return None, status
+    try:
+        p = Problem(v, t, ctx)
+        p.pattern_build(capi.FAMILY_NEDELEC, k)
+        p.assemble(-1, kappa=1.0 if curl_coefficient is None else curl_coefficient,
+                   mass=1.0 if mass_coefficient is None else mass_coefficient,
+                   cell_kappa=cell_coefficient, precision=precision)
+        a = p.matrix(capi.FULL).download()
+    except FFHError as e:
+        if e.code == capi.FFH_ERR_DEVICE:
+            raise
+        return None, status
+    return a, FortsparseStatus(0, "")
+
+
+def assemble_tetra_nedelec_weighted_csc(mesh_vertices, tetrahedra, coefficient, mass_coefficient,
+                                        order=None, precision="fp64", ctx: Context | None = None):
+    """assemble_tetra_nedelec_weighted_csc (:1870-2000).  coefficient: array
+    (T, 3, 3) of symmetric per-cell tensors, or a callable K(x, y, z) -> (3, 3)
+    that must be constant on every cell (it is sampled at the four vertices
+    and the centroid; a field varying inside a cell is rejected, because the
+    device element uses the reference-tensor form for cell-constant K).
+    -> (CSC, status)."""
+    status = FortsparseStatus(FORTSPARSE_INVALID_MATRIX, "Tetrahedral weighted Nedelec assembly failed")
+    k = 1 if order is None else int(order)
+    v = np.asarray(mesh_vertices, dtype=np.float64)
+    t = np.asarray(tetrahedra)
+    if k < 1 or v.ndim != 2 or v.shape[1] != 3 or t.ndim != 2 or t.shape[1] != 4 or t.shape[0] < 1:
+        return None, status
+    if callable(coefficient):
+        pts = v[t - 1]  # (T, 4, 3)
+        cen = pts.mean(axis=1)
+        K = np.array([np.asarray(coefficient(*c), dtype=np.float64) for c in cen])
+        for j in range(4):
+            Kj = np.array([np.asarray(coefficient(*c), dtype=np.float64) for c in pts[:, j]])
+            if np.max(np.abs(Kj - K)) > 1e-14 * max(1.0, np.max(np.abs(K))):
+                raise NotImplementedError("coefficient tensors must be constant on each cell")
+    else:
+        K = np.asarray(coefficient, dtype=np.float64).reshape(t.shape[0], 3, 3)
+    try:
+        p = Problem(v, t, ctx)
+        p.pattern_build(capi.FAMILY_NEDELEC, k)
+        p.assemble(-1, mass=float(mass_coefficient), cell_tensor=K, precision=precision)
         a = p.matrix(capi.FULL).download()
     except FFHError as e:
         if e.code == capi.FFH_ERR_DEVICE:

@@ -47,7 +47,7 @@ EXPORTS = (
     "ffh_constrained_download", "ffh_set_rhs", "ffh_rhs_download", "ffh_problem_matrix",
     "ffh_problem_solve", "ffh_matrix_upload", "ffh_matrix_free", "ffh_matrix_download",
     "ffh_spmv", "ffh_solve_sparse", "ffh_bench_spmv", "ffh_set_profiling",
-    "ffh_last_solve_profile", "ffh_matrix_spmv_variant",
+    "ffh_last_solve_profile", "ffh_matrix_spmv_variant", "ffh_nedelec_dof_map", "ffh_nedelec_basis",
 )
 
 
@@ -132,6 +132,8 @@ def load() -> C.CDLL:
         "ffh_set_profiling": (C.c_int, [vp, C.c_int]),
         "ffh_last_solve_profile": (C.c_int, [vp, C.c_int, ip, dp]),
         "ffh_matrix_spmv_variant": (C.c_int, [vp, C.c_int, C.c_int, ip]),
+        "ffh_nedelec_dof_map": (C.c_int, [C.c_int, C.c_int, vp, C.c_int, vp, vp, vp, ip]),
+        "ffh_nedelec_basis": (C.c_int, [C.c_int, C.c_int, vp, vp, vp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)

@@ -31,6 +31,10 @@ int guarded(Context *ctx, F &&f)
         if (ctx) ctx->err = "host allocation failed";
         g_err = "host allocation failed";
         return FFH_ERR_DEVICE;
+    } catch (const std::exception &e) {
+        if (ctx) ctx->err = e.what();
+        g_err = e.what();
+        return FFH_ERR_INVALID;
     } catch (...) {
         if (ctx) ctx->err = "unexpected error";
         g_err = "unexpected error";
@@ -260,6 +264,40 @@ int ffh_unit_square_mesh(int n, double *vertices, int *triangles, int *is_bounda
     if (n < 2 || !vertices || !triangles) return FFH_ERR_INVALID;
     return guarded(nullptr,
                    [&] { unit_square_mesh(n, vertices, triangles, is_boundary_vertex); });
+}
+
+int ffh_nedelec_dof_map(int k, int ntet, const int *tets, int base, int *global_dofs,
+                        int *edge_orient, int *face_perm, int *ndofs)
+{
+    if (k < 1 || k > 6 || ntet < 1 || !tets || (base != 0 && base != 1)) return FFH_ERR_INVALID;
+    return guarded(nullptr, [&] {
+        std::vector<int> gd;
+        std::vector<unsigned char> cls;
+        int nd = 0, ne = 0, nf = 0;
+        FFH_REQUIRE(nedelec_dof_map(k, ntet, tets, gd, cls, nd, ne, nf) == 0, FFH_ERR_INVALID,
+                    "invalid tetrahedra for the Nedelec dof map");
+        if (global_dofs)
+            for (size_t i = 0; i < gd.size(); ++i) global_dofs[i] = gd[i] + base;
+        for (int64_t e = 0; e < ntet; ++e) {
+            int eo[6], fp[12];
+            nedelec_class_orientation(cls[e], eo, fp);
+            if (edge_orient) std::copy(eo, eo + 6, edge_orient + 6 * e);
+            if (face_perm) std::copy(fp, fp + 12, face_perm + 12 * e);
+        }
+        if (ndofs) *ndofs = nd;
+    });
+}
+
+int ffh_nedelec_basis(int k, int npts, const double *points, double *values, double *curls)
+{
+    if (k < 1 || k > 6 || npts < 0 || !points || !values || !curls) return FFH_ERR_INVALID;
+    return guarded(nullptr, [&] {
+        const int n = nedelec_dof_count(k);
+        for (int q = 0; q < npts; ++q)
+            FFH_REQUIRE(nedelec_basis_eval(k, points + 3 * q, values + (size_t)3 * n * q,
+                                           curls + (size_t)3 * n * q) == 0,
+                        FFH_ERR_INVALID, "Nedelec basis construction failed");
+    });
 }
 
 int ffh_problem_create(ffh_context *ctx, int dim, int nvert, const double *vertices,

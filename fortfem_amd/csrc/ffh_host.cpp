@@ -5,6 +5,7 @@
 #include <cmath>
 #include <cstring>
 #include <functional>
+#include <exception>
 #include <thread>
 #include <unordered_map>
 
@@ -12,22 +13,32 @@
 
 namespace ffh {
 
-void parallel_for(int64_t n, const std::function<void(int64_t, int64_t)> &f)
+void parallel_for(int64_t n, const std::function<void(int64_t, int64_t)> &f, int64_t grain)
 {
     unsigned hw = std::thread::hardware_concurrency();
     int nt = (int)std::max(1u, std::min(hw ? hw : 1u, 16u));
-    if (n < 4096 || nt == 1) {
+    if (n < grain || nt == 1) {
         f(0, n);
         return;
     }
+    nt = (int)std::min<int64_t>(nt, n);
     std::vector<std::thread> th;
+    std::vector<std::exception_ptr> errs(nt);
     int64_t chunk = (n + nt - 1) / nt;
     for (int t = 0; t < nt; ++t) {
         int64_t lo = t * chunk, hi = std::min<int64_t>(n, lo + chunk);
         if (lo >= hi) break;
-        th.emplace_back([&f, lo, hi] { f(lo, hi); });
+        th.emplace_back([&f, &errs, t, lo, hi] {
+            try {
+                f(lo, hi);
+            } catch (...) {
+                errs[t] = std::current_exception();
+            }
+        });
     }
     for (auto &x : th) x.join();
+    for (auto &e : errs)
+        if (e) std::rethrow_exception(e);
 }
 
 // ------------------------------------------------------------------ meshes
@@ -484,4 +495,37 @@ void lagrange_boundary(int d, int ntet, const int *tets,
         }
 }
 
+// Cell-major slots of the packed upper pairs (r <= c): for colour position
+// pos and pair u, the value slots of (row g[r], col g[c]) and of its mirror
+// (row g[c], col g[r]); equal on the diagonal.
+void build_pair_slots(int ncell, int nloc, const std::vector<int> &gd,
+                      const std::vector<int> &perm, const std::vector<int> &col_ptr,
+                      const std::vector<int> &row_idx, std::vector<int2> &slots)
+{
+    const size_t np = (size_t)nloc * (nloc + 1) / 2;
+    slots.resize(np * ncell);
+    auto find = [&](int row, int col) {
+        const int *b = row_idx.data() + col_ptr[col];
+        const int *e = row_idx.data() + col_ptr[col + 1];
+        const int *f = std::lower_bound(b, e, row);
+        FFH_REQUIRE(f != e && *f == row, FFH_ERR_INVALID, "pattern misses an element entry");
+        return (int)(f - row_idx.data());
+    };
+    parallel_for(ncell, [&](int64_t lo, int64_t hi) {
+        for (int64_t pos = lo; pos < hi; ++pos) {
+            const int *g = gd.data() + (size_t)nloc * perm[pos];
+            int2 *o = slots.data() + (size_t)pos * np;
+            size_t u = 0;
+            for (int r = 0; r < nloc; ++r)
+                for (int c = r; c < nloc; ++c, ++u) {
+                    int2 sl;
+                    sl.x = find(g[r], g[c]);
+                    sl.y = r == c ? sl.x : find(g[c], g[r]);
+                    o[u] = sl;
+                }
+        }
+    });
+}
+
 } // namespace ffh
+

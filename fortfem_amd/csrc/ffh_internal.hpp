@@ -10,6 +10,8 @@
 #include <string>
 #include <vector>
 
+#include "ffh_nedelec.hpp"
+
 #include "../../include/fortfem_hip.h"
 
 namespace ffh {
@@ -171,7 +173,18 @@ struct Problem {
     DBuf<int> d_cells_c;   // colour-sorted cell vertex ids (0-based), nv_cell*T
     DBuf<int> d_gdofs_c;   // [l][pos], 0-based dof ids
     DBuf<int> d_slots;     // [r*n+c][pos] value slots
-    DBuf<signed char> d_signs; // Nedelec k=1 edge orientation [l][pos]
+    // Nedelec: orientation class per cell, classes present (slot order),
+    // class slot per colour position, pair slots [pos][u] = (upper, mirror),
+    // class tables (slot x 12 terms x packed pairs), 12 terms per cell
+    std::vector<unsigned char> cell_class;
+    std::vector<int> class_list;
+    DBuf<unsigned char> d_cls_c;
+    DBuf<int2> d_slots2;
+    DBuf<double> d_ned64;
+    DBuf<float> d_ned32;
+    int ned_rule = -1;
+    DBuf<double> d_gterm;
+    int n_edges = 0, n_faces = 0;
     // full matrix + reduced matrix
     Matrix full, reduced;
     DBuf<double> rhs_full, rhs_red;
@@ -218,12 +231,14 @@ void build_pattern(int n_global, int ncell, int nloc,
                    std::vector<int> &row_idx);
 int color_cells(int n_global, int ncell, int nloc, const std::vector<int> &gd,
                 std::vector<int> &color_start, std::vector<int> &perm);
+void build_pair_slots(int ncell, int nloc, const std::vector<int> &gd,
+                      const std::vector<int> &perm, const std::vector<int> &col_ptr,
+                      const std::vector<int> &row_idx, std::vector<int2> &slots);
 void build_slots(int ncell, int nloc, const std::vector<int> &gd,
                  const std::vector<int> &perm, const std::vector<int> &col_ptr,
                  const std::vector<int> &row_idx, std::vector<int> &slots);
 void lagrange_boundary(int d, int ntet, const int *tets,
                        const std::vector<int> &gd, int n_global,
                        std::vector<int> &constrained);
-void parallel_for(int64_t n, const std::function<void(int64_t, int64_t)> &f);
 
 } // namespace ffh

@@ -193,6 +193,85 @@ __global__ __launch_bounds__(kBlockA) void k_p1_2d_cells(
         }
 }
 
+// Nedelec terms of one cell (colour order): the 12 coefficients of the
+// reference tensors.  Curl part J^T K J / det with K = alpha I (scalar or per
+// cell) or the cell's tensor; value part beta det J^-1 J^-T.  Terms follow the
+// table order (00, 11, 22, 01, 02, 12), curl then value.
+__global__ __launch_bounds__(kBlockA) void k_ned_terms(
+    int ncell, const int *__restrict__ cells_c, const double *__restrict__ verts,
+    const int *__restrict__ perm, const double *__restrict__ alpha_cell,
+    const double *__restrict__ tensor, double alpha, double beta,
+    double *__restrict__ gterm, int *__restrict__ err)
+{
+    const int pos = blockIdx.x * kBlockA + threadIdx.x;
+    if (pos >= ncell) return;
+    const int4 c = reinterpret_cast<const int4 *>(cells_c)[pos];
+    double jac[9], inv[9], det;
+    if (!tet_geometry(verts, c.x, c.y, c.z, c.w, jac, inv, det)) {
+        atomicOr(err, 1);
+        return;
+    }
+    const int e = perm[pos];
+    double K[9];
+    if (tensor) {
+        for (int i = 0; i < 9; ++i) K[i] = tensor[9 * (size_t)e + i];
+    } else {
+        const double a = alpha_cell ? alpha_cell[e] : alpha;
+        for (int i = 0; i < 9; ++i) K[i] = (i % 4 == 0) ? a : 0.0;
+    }
+    const int pa[6] = {0, 1, 2, 0, 0, 1}, pb[6] = {0, 1, 2, 1, 2, 2};
+    const size_t T = (size_t)ncell;
+    const double rdet = 1.0 / det;
+#pragma unroll
+    for (int s = 0; s < 6; ++s) {
+        const int a = pa[s], b = pb[s];
+        // (J^T K J)_ab = sum_tu J_ta K_tu J_ub (column-major J, K)
+        double cm = 0.0;
+        for (int t = 0; t < 3; ++t) {
+            double kj = K[t + 0] * jac[0 + 3 * b] + K[t + 3] * jac[1 + 3 * b] + K[t + 6] * jac[2 + 3 * b];
+            cm += jac[t + 3 * a] * kj;
+        }
+        const double mm = inv[a + 0] * inv[b + 0] + inv[a + 3] * inv[b + 3] + inv[a + 6] * inv[b + 6];
+        gterm[(size_t)s * T + pos] = cm * rdet;
+        gterm[(size_t)(6 + s) * T + pos] = beta * det * mm;
+    }
+}
+
+// Nedelec assembly of one colour: lanes run over packed pairs u of a chunk,
+// each thread walks kNedCells consecutive cells of the colour (sorted by
+// orientation class, so the 12 table entries stay in registers across cells)
+// and adds the entry to its two value slots.  Cells of a colour share no dof:
+// every slot has one writer.
+constexpr int kNedCells = 16;
+template <typename CT, typename OT>
+__global__ __launch_bounds__(256) void k_ned_assemble(
+    int pos0, int cnt, int ncell, int np, const unsigned char *__restrict__ cls_c,
+    const double *__restrict__ gterm, const CT *__restrict__ tab,
+    const int2 *__restrict__ slots2, OT *__restrict__ val)
+{
+    const int u = blockIdx.y * 256 + threadIdx.x;
+    if (u >= np) return;
+    const int p0 = pos0 + blockIdx.x * kNedCells;
+    const int p1 = min(pos0 + cnt, p0 + kNedCells);
+    const size_t T = (size_t)ncell;
+    int cur = -1;
+    CT r[12];
+    for (int pos = p0; pos < p1; ++pos) {
+        const int c = cls_c[pos];
+        if (c != cur) {
+            cur = c;
+#pragma unroll
+            for (int s = 0; s < 12; ++s) r[s] = tab[((size_t)c * 12 + s) * np + u];
+        }
+        CT e = CT(0);
+#pragma unroll
+        for (int s = 0; s < 12; ++s) e += (CT)gterm[(size_t)s * T + pos] * r[s];
+        const int2 sl = slots2[(size_t)pos * np + u];
+        val[sl.x] += (OT)e;
+        if (sl.y != sl.x) val[sl.y] += (OT)e;
+    }
+}
+
 // Load phase 1: element vectors det sum_q w_q f(x_q) phi_l(q) into
 // EL[l][pos]; tab per point = xi(3), w phi(NL).
 template <int NL>
@@ -415,6 +494,16 @@ void problem_pattern(Problem *P, int family, int order)
         FFH_REQUIRE(lagrange_dof_map(order, T, P->cells.data(), gd, G) == 0,
                     FFH_ERR_INVALID, "invalid tetrahedra for the dof map");
         nl = lagrange_dof_count(order);
+    } else if (family == FFH_FAMILY_NEDELEC) {
+        FFH_REQUIRE(P->dim == 3, FFH_ERR_INVALID, "Nedelec family needs a tetrahedral mesh");
+        FFH_REQUIRE(order >= 1 && order <= 6, FFH_ERR_INVALID, "Nedelec order must be 1..6");
+        for (size_t i = 0; i < P->cells.size(); ++i)
+            FFH_REQUIRE(P->cells[i] >= 1 && P->cells[i] <= P->nvert, FFH_ERR_INVALID,
+                        "cell vertex id out of range");
+        FFH_REQUIRE(nedelec_dof_map(order, T, P->cells.data(), gd, P->cell_class, G,
+                                    P->n_edges, P->n_faces) == 0,
+                    FFH_ERR_INVALID, "invalid tetrahedra for the Nedelec dof map");
+        nl = nedelec_dof_count(order);
     } else if (family == FFH_FAMILY_P1_2D) {
         FFH_REQUIRE(P->dim == 2, FFH_ERR_INVALID, "P1_2D family needs a triangle mesh");
         nl = 3;
@@ -436,10 +525,32 @@ void problem_pattern(Problem *P, int family, int order)
     build_pattern(G, T, nl, gd, col_ptr, row_idx);
     std::vector<int> cstart, perm;
     P->n_colors = color_cells(G, T, nl, gd, cstart, perm);
+    std::vector<int> slots;
+    std::vector<int2> slots2;
+    std::vector<unsigned char> cls_c;
+    if (family == FFH_FAMILY_NEDELEC) {
+        // inside a colour, cells sorted by orientation class
+        for (int c = 0; c < P->n_colors; ++c)
+            std::stable_sort(perm.begin() + cstart[c], perm.begin() + cstart[c + 1],
+                             [&](int a, int b) { return P->cell_class[a] < P->cell_class[b]; });
+        int slot_of[kNedelecClasses];
+        std::fill(slot_of, slot_of + kNedelecClasses, -1);
+        P->class_list.clear();
+        cls_c.resize(T);
+        for (int pos = 0; pos < T; ++pos) {
+            const int c = P->cell_class[perm[pos]];
+            if (slot_of[c] < 0) {
+                slot_of[c] = (int)P->class_list.size();
+                P->class_list.push_back(c);
+            }
+            cls_c[pos] = (unsigned char)slot_of[c];
+        }
+        build_pair_slots(T, nl, gd, perm, col_ptr, row_idx, slots2);
+    } else {
+        build_slots(T, nl, gd, perm, col_ptr, row_idx, slots);
+    }
     P->color_start = cstart;
     P->perm = perm;
-    std::vector<int> slots;
-    build_slots(T, nl, gd, perm, col_ptr, row_idx, slots);
     const int vpc = P->dim == 3 ? 4 : 3;
     std::vector<int> cells_c((size_t)vpc * T), gdofs_c((size_t)nl * T);
     parallel_for(T, [&](int64_t lo, int64_t hi) {
@@ -455,7 +566,15 @@ void problem_pattern(Problem *P, int family, int order)
     P->d_perm.upload(perm, s);
     P->d_cells_c.upload(cells_c, s);
     P->d_gdofs_c.upload(gdofs_c, s);
-    P->d_slots.upload(slots, s);
+    if (family == FFH_FAMILY_NEDELEC) {
+        P->d_slots.reset();
+        P->d_slots2.upload(slots2, s);
+        P->d_cls_c.upload(cls_c, s);
+        P->ned_rule = -1;
+    } else {
+        P->d_slots.upload(slots, s);
+        P->d_slots2.reset();
+    }
     DevMatrix &A = P->full.m;
     A.n = G;
     A.nnz = (int64_t)row_idx.size();
@@ -483,12 +602,115 @@ void problem_pattern(Problem *P, int family, int order)
 }
 
 // ---------------------------------------------------------------- numeric --
+// Nedelec curl-mass assembly (assemble_tetra_nedelec_curl_mass_csc,
+// assembly_tetra_nedelec_3d.f90:2042-2112 / 2289-2355: coefficient = curl
+// coefficient alpha, mass = beta) and the tensor-weighted variant
+// (assemble_tetra_nedelec_weighted_csc, :1870-2000: a symmetric 3x3 tensor per
+// cell on the curl term).  Each cell's matrix is T^T M T of the reference, so
+// the assembled values match its triplet sums.
+static void assemble_nedelec(Problem *P, int coeff_kind, const double *coeff, double mass,
+                             int precision)
+{
+    Context *ctx = P->ctx;
+    hipStream_t s = ctx->stream;
+    const int T = P->ncell, k = P->order, nl = P->n_local;
+    const int np = nl * (nl + 1) / 2;
+    double alpha = 1.0;
+    const double *dalpha = nullptr, *dtensor = nullptr;
+    bool weighted = false;
+    if (coeff_kind == FFH_COEFF_SCALAR) {
+        if (coeff) alpha = coeff[0];
+    } else if (coeff_kind == FFH_COEFF_CELL) {
+        if (coeff) {
+            P->d_coeff.upload(coeff, (size_t)T, s);
+            P->coeff_kind = FFH_COEFF_CELL;
+        }
+        FFH_REQUIRE(P->coeff_kind == FFH_COEFF_CELL && P->d_coeff.p, FFH_ERR_INVALID,
+                    "per-cell coefficients were never uploaded");
+        dalpha = P->d_coeff.p;
+    } else if (coeff_kind == FFH_COEFF_CELL_TENSOR) {
+        if (coeff) {
+            for (int64_t e = 0; e < T; ++e) {
+                const double *K = coeff + 9 * e;
+                double mx = 0.0;
+                for (int i = 0; i < 9; ++i) mx = std::max(mx, std::fabs(K[i]));
+                FFH_REQUIRE(std::fabs(K[1] - K[3]) <= 1e-12 * mx &&
+                                std::fabs(K[2] - K[6]) <= 1e-12 * mx &&
+                                std::fabs(K[5] - K[7]) <= 1e-12 * mx,
+                            FFH_ERR_INVALID, "cell tensors must be symmetric");
+            }
+            P->d_coeff.upload(coeff, (size_t)9 * T, s);
+            P->coeff_kind = FFH_COEFF_CELL_TENSOR;
+        }
+        FFH_REQUIRE(P->coeff_kind == FFH_COEFF_CELL_TENSOR && P->d_coeff.p, FFH_ERR_INVALID,
+                    "cell tensors were never uploaded");
+        dtensor = P->d_coeff.p;
+        weighted = true;
+    } else {
+        throw Error{FFH_ERR_INVALID, "unknown coefficient kind"};
+    }
+    // class tables for the element rule of the variant (k = 1: 4 points)
+    const int rule = (k == 1) ? 0 : (weighted ? 2 : 1);
+    if (P->ned_rule != rule) {
+        std::vector<double> tab;
+        nedelec_class_tables(k, weighted, P->class_list, tab);
+        P->d_ned64.upload(tab, s);
+        std::vector<float> t32(tab.begin(), tab.end());
+        P->d_ned32.upload(t32, s);
+        FFH_HIP(hipStreamSynchronize(s));
+        P->ned_rule = rule;
+    }
+    P->d_gterm.alloc((size_t)12 * T);
+    hipLaunchKernelGGL(k_ned_terms, dim3((T + kBlockA - 1) / kBlockA), dim3(kBlockA), 0, s, T,
+                       P->d_cells_c.p, P->d_vertices.p, P->d_perm.p, dalpha, dtensor, alpha,
+                       mass, P->d_gterm.p, P->d_err.p);
+    FFH_HIP(hipGetLastError());
+    check_err(P, "Nedelec assembly");
+    DevMatrix &A = P->full.m;
+    const bool f32 = precision == FFH_FP32;
+    if (f32) {
+        A.val32.alloc(A.nnz);
+        FFH_HIP(hipMemsetAsync(A.val32.p, 0, sizeof(float) * A.nnz, s));
+    } else {
+        FFH_HIP(hipMemsetAsync(A.val.p, 0, sizeof(double) * A.nnz, s));
+    }
+    const int gy = (np + 255) / 256;
+    for (int c = 0; c < P->n_colors; ++c) {
+        const int p0 = P->color_start[c], cnt = P->color_start[c + 1] - p0;
+        if (cnt == 0) continue;
+        const dim3 grid((cnt + kNedCells - 1) / kNedCells, gy);
+        if (f32)
+            hipLaunchKernelGGL((k_ned_assemble<float, float>), grid, dim3(256), 0, s, p0, cnt, T,
+                               np, P->d_cls_c.p, P->d_gterm.p, P->d_ned32.p, P->d_slots2.p,
+                               A.val32.p);
+        else
+            hipLaunchKernelGGL((k_ned_assemble<double, double>), grid, dim3(256), 0, s, p0, cnt,
+                               T, np, P->d_cls_c.p, P->d_gterm.p, P->d_ned64.p, P->d_slots2.p,
+                               A.val.p);
+    }
+    FFH_HIP(hipGetLastError());
+    if (f32) {
+        hipLaunchKernelGGL(k_widen, dim3(grid1(ctx, A.nnz)), dim3(256), 0, s, A.nnz, A.val32.p,
+                           A.val.p);
+        A.val32_valid = true;
+    } else {
+        A.val32_valid = false;
+    }
+    FFH_HIP(hipStreamSynchronize(s));
+    P->assembled = true;
+    P->red_dirty = true;
+}
+
 void problem_assemble(Problem *P, int qdeg, int coeff_kind, const double *coeff,
                       double mass, int precision)
 {
     Context *ctx = P->ctx;
     hipStream_t s = ctx->stream;
     FFH_REQUIRE(P->n_global > 0, FFH_ERR_INVALID, "pattern_build must come first");
+    if (P->family == FFH_FAMILY_NEDELEC) {
+        assemble_nedelec(P, coeff_kind, coeff, mass, precision);
+        return;
+    }
     const int T = P->ncell;
     double kappa = 1.0;
     const double *dk = nullptr;
@@ -651,6 +873,13 @@ void problem_dirichlet(Problem *P, int mode, const double *values)
     std::vector<int> cons;
     if (P->family == FFH_FAMILY_H1) {
         lagrange_boundary(P->order, P->ncell, P->cells.data(), P->global_dofs, G, cons);
+    } else if (P->family == FFH_FAMILY_NEDELEC) {
+        // zero tangential trace (build_zero_tangential_mask,
+        // src/solvers/fortfem_tetra_nedelec_solver_3d.f90:431-470)
+        std::vector<unsigned char> mask;
+        nedelec_boundary_dofs(P->order, P->ncell, P->cells.data(), P->global_dofs, mask);
+        cons.assign(G, 0);
+        for (int i = 0; i < G && i < (int)mask.size(); ++i) cons[i] = mask[i];
     } else if (P->family == FFH_FAMILY_P1_2D) {
         // boundary vertices: endpoints of edges owned by one triangle
         // (find_boundary, src/mesh/mesh_2d.f90:553-603)

@@ -123,6 +123,21 @@ int ffh_unit_square_mesh(int n, double *vertices, int *triangles,
 int ffh_tetra_duffy(int p, int *nq, double *x, double *y, double *z,
                     double *w);
 
+/* ---- Nedelec host routines ------------------------------------------------ */
+/* build_tetra_nedelec_dof_map (src/elements/tetra_nedelec_global_dof_map.f90:
+ * 15-105): tets (4 x ntet, 1-based ids); global_dofs (n x ntet,
+ * n = k(k+2)(k+3)/2) in base 0 or 1; edge_orient (6 x ntet, +1/-1);
+ * face_perm (3 x 4 x ntet, 1..3); any output may be NULL.  *ndofs receives
+ * the global count.  Host routine. */
+int ffh_nedelec_dof_map(int k, int ntet, const int *tets, int base, int *global_dofs,
+                        int *edge_orient, int *face_perm, int *ndofs);
+/* evaluate_tetra_nedelec_first_kind (src/elements/tetra_nedelec_arbitrary_order.f90:
+ * 75-106) and evaluate_tetra_nedelec_first_order (tetra_nedelec_first_order.f90:
+ * 11-51) at npts reference points (3 x npts): values and curls
+ * (3 x n x npts).  Host routine. */
+int ffh_nedelec_basis(int k, int npts, const double *points, double *values,
+                      double *curls);
+
 /* ---- problems: mesh + space + operator resident in HBM -------------------- */
 /* Takes the (mesh_vertices, tetrahedra) arguments of
  * assemble_tetra_lagrange_stiffness_csc (src/assembly/
