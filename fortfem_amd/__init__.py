@@ -288,12 +288,15 @@ class Problem:
         return g
 
     def assemble(self, quadrature_degree, kappa=1.0, mass=0.0, cell_kappa=None, precision="fp64",
-                 reuse_coefficients=False, cell_tensor=None):
+                 reuse_coefficients=False, cell_tensor=None, reuse_tensor=False):
         """Assemble the family's operator.  H1: kappa (stiffness) and mass;
         Nedelec: kappa = curl coefficient alpha, mass = beta, or cell_tensor
         (T, 3, 3) symmetric tensors on the curl term (weighted variant)."""
         prec = capi.FP32 if precision == "fp32" else capi.FP64
-        if cell_tensor is not None:
+        if reuse_tensor:
+            self._check(self._lib.ffh_assemble(self.handle, quadrature_degree, capi.COEFF_CELL_TENSOR,
+                                               None, float(mass), prec))
+        elif cell_tensor is not None:
             K = np.ascontiguousarray(np.asarray(cell_tensor, dtype=np.float64).transpose(0, 2, 1))
             self._check(self._lib.ffh_assemble(self.handle, quadrature_degree, capi.COEFF_CELL_TENSOR,
                                                _ptr(K.reshape(-1)), float(mass), prec))

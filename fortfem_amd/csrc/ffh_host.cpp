@@ -495,36 +495,32 @@ void lagrange_boundary(int d, int ntet, const int *tets,
         }
 }
 
-// Cell-major slots of the packed upper pairs (r <= c): for colour position
-// pos and pair u, the value slots of (row g[r], col g[c]) and of its mirror
-// (row g[c], col g[r]); equal on the diagonal.
-void build_pair_slots(int ncell, int nloc, const std::vector<int> &gd,
-                      const std::vector<int> &perm, const std::vector<int> &col_ptr,
-                      const std::vector<int> &row_idx, std::vector<int2> &slots)
+// Cell-major slots of every element entry: for colour position pos, entry
+// t = c * nloc + r (column-major in the element) holds the value slot of
+// (row g[r], col g[c]).  Lanes running over r then write rows of one column,
+// which the CSC stores next to each other for dofs of one edge / face / cell.
+void build_entry_slots(int ncell, int nloc, const std::vector<int> &gd,
+                       const std::vector<int> &perm, const std::vector<int> &col_ptr,
+                       const std::vector<int> &row_idx, std::vector<int> &slots)
 {
-    const size_t np = (size_t)nloc * (nloc + 1) / 2;
-    slots.resize(np * ncell);
-    auto find = [&](int row, int col) {
-        const int *b = row_idx.data() + col_ptr[col];
-        const int *e = row_idx.data() + col_ptr[col + 1];
-        const int *f = std::lower_bound(b, e, row);
-        FFH_REQUIRE(f != e && *f == row, FFH_ERR_INVALID, "pattern misses an element entry");
-        return (int)(f - row_idx.data());
-    };
+    const size_t nn = (size_t)nloc * nloc;
+    slots.resize(nn * ncell);
     parallel_for(ncell, [&](int64_t lo, int64_t hi) {
         for (int64_t pos = lo; pos < hi; ++pos) {
             const int *g = gd.data() + (size_t)nloc * perm[pos];
-            int2 *o = slots.data() + (size_t)pos * np;
-            size_t u = 0;
-            for (int r = 0; r < nloc; ++r)
-                for (int c = r; c < nloc; ++c, ++u) {
-                    int2 sl;
-                    sl.x = find(g[r], g[c]);
-                    sl.y = r == c ? sl.x : find(g[c], g[r]);
-                    o[u] = sl;
+            int *o = slots.data() + (size_t)pos * nn;
+            for (int c = 0; c < nloc; ++c) {
+                const int *b = row_idx.data() + col_ptr[g[c]];
+                const int *e = row_idx.data() + col_ptr[g[c] + 1];
+                for (int r = 0; r < nloc; ++r) {
+                    const int *f = std::lower_bound(b, e, g[r]);
+                    FFH_REQUIRE(f != e && *f == g[r], FFH_ERR_INVALID,
+                                "pattern misses an element entry");
+                    o[(size_t)c * nloc + r] = (int)(f - row_idx.data());
                 }
+            }
         }
-    });
+    }, 64);
 }
 
 } // namespace ffh

@@ -174,12 +174,12 @@ struct Problem {
     DBuf<int> d_gdofs_c;   // [l][pos], 0-based dof ids
     DBuf<int> d_slots;     // [r*n+c][pos] value slots
     // Nedelec: orientation class per cell, classes present (slot order),
-    // class slot per colour position, pair slots [pos][u] = (upper, mirror),
+    // class slot per colour position, entry slots [pos][c * n + r],
     // class tables (slot x 12 terms x packed pairs), 12 terms per cell
     std::vector<unsigned char> cell_class;
     std::vector<int> class_list;
     DBuf<unsigned char> d_cls_c;
-    DBuf<int2> d_slots2;
+    DBuf<int> d_slots_cm;
     DBuf<double> d_ned64;
     DBuf<float> d_ned32;
     int ned_rule = -1;
@@ -231,9 +231,9 @@ void build_pattern(int n_global, int ncell, int nloc,
                    std::vector<int> &row_idx);
 int color_cells(int n_global, int ncell, int nloc, const std::vector<int> &gd,
                 std::vector<int> &color_start, std::vector<int> &perm);
-void build_pair_slots(int ncell, int nloc, const std::vector<int> &gd,
-                      const std::vector<int> &perm, const std::vector<int> &col_ptr,
-                      const std::vector<int> &row_idx, std::vector<int2> &slots);
+void build_entry_slots(int ncell, int nloc, const std::vector<int> &gd,
+                       const std::vector<int> &perm, const std::vector<int> &col_ptr,
+                       const std::vector<int> &row_idx, std::vector<int> &slots);
 void build_slots(int ncell, int nloc, const std::vector<int> &gd,
                  const std::vector<int> &perm, const std::vector<int> &col_ptr,
                  const std::vector<int> &row_idx, std::vector<int> &slots);

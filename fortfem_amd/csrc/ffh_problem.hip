@@ -237,38 +237,43 @@ __global__ __launch_bounds__(kBlockA) void k_ned_terms(
     }
 }
 
-// Nedelec assembly of one colour: lanes run over packed pairs u of a chunk,
-// each thread walks kNedCells consecutive cells of the colour (sorted by
-// orientation class, so the 12 table entries stay in registers across cells)
-// and adds the entry to its two value slots.  Cells of a colour share no dof:
-// every slot has one writer.
+// Nedelec assembly of one colour: lanes run over element entries
+// t = c * n + r of a chunk (rows of one element column side by side), each
+// thread walks kNedCells consecutive cells of the colour (sorted by orientation
+// class, so the 12 table entries of (min(r,c), max(r,c)) stay in registers
+// across cells) and adds the entry to its value slot.  Cells of a colour share
+// no dof: every slot has one writer.  Both triangles come from the same packed
+// table entry, so the matrix is exactly symmetric.
 constexpr int kNedCells = 16;
 template <typename CT, typename OT>
 __global__ __launch_bounds__(256) void k_ned_assemble(
-    int pos0, int cnt, int ncell, int np, const unsigned char *__restrict__ cls_c,
+    int pos0, int cnt, int ncell, int n, const unsigned char *__restrict__ cls_c,
     const double *__restrict__ gterm, const CT *__restrict__ tab,
-    const int2 *__restrict__ slots2, OT *__restrict__ val)
+    const int *__restrict__ slots_cm, OT *__restrict__ val)
 {
-    const int u = blockIdx.y * 256 + threadIdx.x;
-    if (u >= np) return;
+    const int nn = n * n;
+    const int t = blockIdx.y * 256 + threadIdx.x;
+    if (t >= nn) return;
+    const int c = t / n, r = t - c * n;
+    const int lo = min(r, c), hi = max(r, c);
+    const int u = lo * n - (lo * (lo - 1)) / 2 + (hi - lo);
+    const int np = n * (n + 1) / 2;
     const int p0 = pos0 + blockIdx.x * kNedCells;
     const int p1 = min(pos0 + cnt, p0 + kNedCells);
     const size_t T = (size_t)ncell;
     int cur = -1;
-    CT r[12];
+    CT rt[12];
     for (int pos = p0; pos < p1; ++pos) {
-        const int c = cls_c[pos];
-        if (c != cur) {
-            cur = c;
+        const int cl = cls_c[pos];
+        if (cl != cur) {
+            cur = cl;
 #pragma unroll
-            for (int s = 0; s < 12; ++s) r[s] = tab[((size_t)c * 12 + s) * np + u];
+            for (int s = 0; s < 12; ++s) rt[s] = tab[((size_t)cl * 12 + s) * np + u];
         }
         CT e = CT(0);
 #pragma unroll
-        for (int s = 0; s < 12; ++s) e += (CT)gterm[(size_t)s * T + pos] * r[s];
-        const int2 sl = slots2[(size_t)pos * np + u];
-        val[sl.x] += (OT)e;
-        if (sl.y != sl.x) val[sl.y] += (OT)e;
+        for (int s = 0; s < 12; ++s) e += (CT)gterm[(size_t)s * T + pos] * rt[s];
+        val[slots_cm[(size_t)pos * nn + t]] += (OT)e;
     }
 }
 
@@ -526,7 +531,7 @@ void problem_pattern(Problem *P, int family, int order)
     std::vector<int> cstart, perm;
     P->n_colors = color_cells(G, T, nl, gd, cstart, perm);
     std::vector<int> slots;
-    std::vector<int2> slots2;
+    std::vector<int> slots_cm;
     std::vector<unsigned char> cls_c;
     if (family == FFH_FAMILY_NEDELEC) {
         // inside a colour, cells sorted by orientation class
@@ -545,7 +550,7 @@ void problem_pattern(Problem *P, int family, int order)
             }
             cls_c[pos] = (unsigned char)slot_of[c];
         }
-        build_pair_slots(T, nl, gd, perm, col_ptr, row_idx, slots2);
+        build_entry_slots(T, nl, gd, perm, col_ptr, row_idx, slots_cm);
     } else {
         build_slots(T, nl, gd, perm, col_ptr, row_idx, slots);
     }
@@ -568,12 +573,12 @@ void problem_pattern(Problem *P, int family, int order)
     P->d_gdofs_c.upload(gdofs_c, s);
     if (family == FFH_FAMILY_NEDELEC) {
         P->d_slots.reset();
-        P->d_slots2.upload(slots2, s);
+        P->d_slots_cm.upload(slots_cm, s);
         P->d_cls_c.upload(cls_c, s);
         P->ned_rule = -1;
     } else {
         P->d_slots.upload(slots, s);
-        P->d_slots2.reset();
+        P->d_slots_cm.reset();
     }
     DevMatrix &A = P->full.m;
     A.n = G;
@@ -674,18 +679,18 @@ static void assemble_nedelec(Problem *P, int coeff_kind, const double *coeff, do
     } else {
         FFH_HIP(hipMemsetAsync(A.val.p, 0, sizeof(double) * A.nnz, s));
     }
-    const int gy = (np + 255) / 256;
+    const int gy = (nl * nl + 255) / 256;
     for (int c = 0; c < P->n_colors; ++c) {
         const int p0 = P->color_start[c], cnt = P->color_start[c + 1] - p0;
         if (cnt == 0) continue;
         const dim3 grid((cnt + kNedCells - 1) / kNedCells, gy);
         if (f32)
             hipLaunchKernelGGL((k_ned_assemble<float, float>), grid, dim3(256), 0, s, p0, cnt, T,
-                               np, P->d_cls_c.p, P->d_gterm.p, P->d_ned32.p, P->d_slots2.p,
+                               nl, P->d_cls_c.p, P->d_gterm.p, P->d_ned32.p, P->d_slots_cm.p,
                                A.val32.p);
         else
             hipLaunchKernelGGL((k_ned_assemble<double, double>), grid, dim3(256), 0, s, p0, cnt,
-                               T, np, P->d_cls_c.p, P->d_gterm.p, P->d_ned64.p, P->d_slots2.p,
+                               T, nl, P->d_cls_c.p, P->d_gterm.p, P->d_ned64.p, P->d_slots_cm.p,
                                A.val.p);
     }
     FFH_HIP(hipGetLastError());
