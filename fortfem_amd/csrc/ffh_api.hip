@@ -507,6 +507,7 @@ int ffh_matrix_download(ffh_matrix *M, int base, int64_t capacity, int *col_ptr,
     if (capacity < M->m.nnz) return FFH_ERR_CAPACITY;
     return guarded(M->ctx, [&] {
         hipStream_t s = M->ctx->stream;
+        ensure_val64(M->ctx, M->m);
         if (col_ptr) M->m.col_ptr.download(col_ptr, M->m.n + 1, s);
         if (row_idx) M->m.row_idx.download(row_idx, M->m.nnz, s);
         if (val) M->m.val.download(val, M->m.nnz, s);
@@ -527,6 +528,7 @@ int ffh_spmv(ffh_matrix *M, const double *x, double *y)
     return guarded(M->ctx, [&] {
         Context *ctx = M->ctx;
         ensure_workspace(ctx, *M);
+        ensure_val64(ctx, M->m);
         M->ws.p.upload(x, M->m.n, ctx->stream);
         spmv64(ctx, M->m, M->ws.p.p, M->ws.ap.p);
         M->ws.ap.download(y, M->m.n, ctx->stream);

@@ -2,6 +2,7 @@
 // quadrature tables, the reference's global numbering with O(N) hashing, the
 // CSC pattern, the element colouring and the element -> value-slot map.
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstring>
 #include <functional>
@@ -499,9 +500,13 @@ void lagrange_boundary(int d, int ntet, const int *tets,
 // t = c * nloc + r (column-major in the element) holds the value slot of
 // (row g[r], col g[c]).  Lanes running over r then write rows of one column,
 // which the CSC stores next to each other for dofs of one edge / face / cell.
+// The first cell in colour order that reaches a slot stores it as ~slot
+// (negative): it writes the value instead of adding, so the values need no
+// clearing before an assembly.
 void build_entry_slots(int ncell, int nloc, const std::vector<int> &gd,
-                       const std::vector<int> &perm, const std::vector<int> &col_ptr,
-                       const std::vector<int> &row_idx, std::vector<int> &slots)
+                       const std::vector<int> &perm, const std::vector<int> &color_start,
+                       const std::vector<int> &col_ptr, const std::vector<int> &row_idx,
+                       std::vector<int> &slots)
 {
     const size_t nn = (size_t)nloc * nloc;
     slots.resize(nn * ncell);
@@ -521,6 +526,24 @@ void build_entry_slots(int ncell, int nloc, const std::vector<int> &gd,
             }
         }
     }, 64);
+    // first touch: colours in order; cells of one colour own disjoint slots
+    const size_t nnz = row_idx.size();
+    std::vector<std::atomic<uint64_t>> seen((nnz + 63) / 64);
+    for (auto &w : seen) w.store(0, std::memory_order_relaxed);
+    const int ncol = (int)color_start.size() - 1;
+    for (int c = 0; c < ncol; ++c) {
+        const int p0 = color_start[c], p1 = color_start[c + 1];
+        parallel_for(p1 - p0, [&](int64_t lo, int64_t hi) {
+            for (int64_t pos = p0 + lo; pos < p0 + hi; ++pos) {
+                int *o = slots.data() + (size_t)pos * nn;
+                for (size_t t = 0; t < nn; ++t) {
+                    const int sl = o[t];
+                    const uint64_t bit = 1ull << (sl & 63);
+                    if (!(seen[sl >> 6].fetch_or(bit, std::memory_order_relaxed) & bit)) o[t] = ~sl;
+                }
+            }
+        }, 64);
+    }
 }
 
 } // namespace ffh

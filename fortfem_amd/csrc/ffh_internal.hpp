@@ -110,6 +110,7 @@ struct DevMatrix {
     const double *gval() const { return symmetric ? val.p : t_val.p; }
     const float *gval32() const { return symmetric ? val32.p : t_val32.p; }
     bool val32_valid = false;
+    bool val64_stale = false; // FP32 assembly: val holds no FP64 view yet
     DBuf<int> rowblocks; // CSR-stream row blocks of the gather arrays
     int nrb = 0;
     int variant[2] = {-1, -1}; // autotuned SpMV variant (FP64, FP32)
@@ -231,9 +232,12 @@ void build_pattern(int n_global, int ncell, int nloc,
                    std::vector<int> &row_idx);
 int color_cells(int n_global, int ncell, int nloc, const std::vector<int> &gd,
                 std::vector<int> &color_start, std::vector<int> &perm);
+// FP64 view of FP32-assembled values, made on first use
+void ensure_val64(Context *ctx, DevMatrix &A);
 void build_entry_slots(int ncell, int nloc, const std::vector<int> &gd,
-                       const std::vector<int> &perm, const std::vector<int> &col_ptr,
-                       const std::vector<int> &row_idx, std::vector<int> &slots);
+                       const std::vector<int> &perm, const std::vector<int> &color_start,
+                       const std::vector<int> &col_ptr, const std::vector<int> &row_idx,
+                       std::vector<int> &slots);
 void build_slots(int ncell, int nloc, const std::vector<int> &gd,
                  const std::vector<int> &perm, const std::vector<int> &col_ptr,
                  const std::vector<int> &row_idx, std::vector<int> &slots);

@@ -645,6 +645,7 @@ static int spmv_variant(Context *ctx, Matrix &M, bool f32)
 int select_spmv_variant(Context *ctx, Matrix &M, bool f32, int request)
 {
     static_assert(kNumVariants == FFH_SPMV_VARIANTS, "variant table and header disagree");
+    ensure_val64(ctx, M.m);
     if (f32) ensure_val32(ctx, M.m);
     if (request >= 0) {
         M.m.variant[f32 ? 1 : 0] = request;
@@ -785,6 +786,7 @@ void solve_fp64(Context *ctx, Matrix &M, const double *b, double *x,
                 const ffh_solver_options &o, ffh_solver_stats &st)
 {
     ensure_workspace(ctx, M);
+    ensure_val64(ctx, M.m);
     spmv_variant(ctx, M, false);
     Workspace &w = M.ws;
     KrylovCtl out{};
@@ -831,6 +833,7 @@ void solve_mixed(Context *ctx, Matrix &M, const double *b, double *x,
     ensure_workspace(ctx, M);
     ensure_workspace32(ctx, M);
     DevMatrix &A = M.m;
+    ensure_val64(ctx, A);
     ensure_val32(ctx, A);
     spmv_variant(ctx, M, false);
     spmv_variant(ctx, M, true);
@@ -903,6 +906,7 @@ void solve_mixed(Context *ctx, Matrix &M, const double *b, double *x,
 double bench_spmv(Context *ctx, Matrix &M, int precision, int reps, double *bytes)
 {
     ensure_workspace(ctx, M);
+    ensure_val64(ctx, M.m);
     DevMatrix &A = M.m;
     Workspace &w = M.ws;
     const int n = A.n;

@@ -241,8 +241,9 @@ __global__ __launch_bounds__(kBlockA) void k_ned_terms(
 // t = c * n + r of a chunk (rows of one element column side by side), each
 // thread walks kNedCells consecutive cells of the colour (sorted by orientation
 // class, so the 12 table entries of (min(r,c), max(r,c)) stay in registers
-// across cells) and adds the entry to its value slot.  Cells of a colour share
-// no dof: every slot has one writer.  Both triangles come from the same packed
+// across cells) and stores (first contribution, negative slot) or adds the
+// entry to its value slot.  Cells of a colour share no dof: every slot has one
+// writer.  Both triangles come from the same packed
 // table entry, so the matrix is exactly symmetric.
 constexpr int kNedCells = 16;
 template <typename CT, typename OT>
@@ -273,7 +274,11 @@ __global__ __launch_bounds__(256) void k_ned_assemble(
         CT e = CT(0);
 #pragma unroll
         for (int s = 0; s < 12; ++s) e += (CT)gterm[(size_t)s * T + pos] * rt[s];
-        val[slots_cm[(size_t)pos * nn + t]] += (OT)e;
+        const int sl = slots_cm[(size_t)pos * nn + t];
+        if (sl < 0)
+            val[~sl] = (OT)e; // first contribution to this slot
+        else
+            val[sl] += (OT)e;
     }
 }
 
@@ -413,6 +418,15 @@ static int grid1(const Context *ctx, int64_t n, int block = 256)
     return (int)std::max<int64_t>(1, std::min<int64_t>(need, (int64_t)ctx->cu_count * 16));
 }
 
+void ensure_val64(Context *ctx, DevMatrix &A)
+{
+    if (!A.val64_stale) return;
+    hipLaunchKernelGGL(k_widen, dim3(grid1(ctx, A.nnz)), dim3(256), 0, ctx->stream, A.nnz,
+                       A.val32.p, A.val.p);
+    FFH_HIP(hipGetLastError());
+    A.val64_stale = false;
+}
+
 // S tables of the reference-tensor form with the reference's Duffy rule
 static void h1_tables(int d, int qdeg, std::vector<double> &tab)
 {
@@ -550,7 +564,7 @@ void problem_pattern(Problem *P, int family, int order)
             }
             cls_c[pos] = (unsigned char)slot_of[c];
         }
-        build_entry_slots(T, nl, gd, perm, col_ptr, row_idx, slots_cm);
+        build_entry_slots(T, nl, gd, perm, cstart, col_ptr, row_idx, slots_cm);
     } else {
         build_slots(T, nl, gd, perm, col_ptr, row_idx, slots);
     }
@@ -673,12 +687,7 @@ static void assemble_nedelec(Problem *P, int coeff_kind, const double *coeff, do
     check_err(P, "Nedelec assembly");
     DevMatrix &A = P->full.m;
     const bool f32 = precision == FFH_FP32;
-    if (f32) {
-        A.val32.alloc(A.nnz);
-        FFH_HIP(hipMemsetAsync(A.val32.p, 0, sizeof(float) * A.nnz, s));
-    } else {
-        FFH_HIP(hipMemsetAsync(A.val.p, 0, sizeof(double) * A.nnz, s));
-    }
+    if (f32) A.val32.alloc(A.nnz); // every slot is written (first touch stores)
     const int gy = (nl * nl + 255) / 256;
     for (int c = 0; c < P->n_colors; ++c) {
         const int p0 = P->color_start[c], cnt = P->color_start[c + 1] - p0;
@@ -694,13 +703,8 @@ static void assemble_nedelec(Problem *P, int coeff_kind, const double *coeff, do
                                A.val.p);
     }
     FFH_HIP(hipGetLastError());
-    if (f32) {
-        hipLaunchKernelGGL(k_widen, dim3(grid1(ctx, A.nnz)), dim3(256), 0, s, A.nnz, A.val32.p,
-                           A.val.p);
-        A.val32_valid = true;
-    } else {
-        A.val32_valid = false;
-    }
+    A.val32_valid = f32;
+    A.val64_stale = f32;
     FFH_HIP(hipStreamSynchronize(s));
     P->assembled = true;
     P->red_dirty = true;
@@ -735,6 +739,8 @@ void problem_assemble(Problem *P, int qdeg, int coeff_kind, const double *coeff,
     DevMatrix &A = P->full.m;
     const int nc = P->n_colors;
     if (P->family == FFH_FAMILY_P1_2D) {
+        A.val64_stale = false;
+        A.val32_valid = false;
         FFH_HIP(hipMemsetAsync(A.val.p, 0, sizeof(double) * A.nnz, s));
         for (int c = 0; c < nc; ++c) {
             int p0 = P->color_start[c], p1 = P->color_start[c + 1];
@@ -793,14 +799,10 @@ void problem_assemble(Problem *P, int qdeg, int coeff_kind, const double *coeff,
         }
         FFH_HIP(hipGetLastError());
         check_err(P, "tetrahedral H1 assembly");
-        if (f32) {
-            // the FP64 view of an FP32-contracted matrix, for export and checks
-            hipLaunchKernelGGL(k_widen, dim3(grid1(ctx, A.nnz)), dim3(256), 0, s, A.nnz,
-                               A.val32.p, A.val.p);
-            A.val32_valid = true;
-        } else {
-            A.val32_valid = false;
-        }
+        // FP32 contraction: the FP64 view (export, FP64 solves) is made on
+        // first use
+        A.val32_valid = f32;
+        A.val64_stale = f32;
     }
     P->assembled = true;
     P->red_dirty = true;
@@ -997,6 +999,7 @@ void problem_reduce(Problem *P)
     if (!P->have_dirichlet || !P->red_dirty) return;
     Context *ctx = P->ctx;
     hipStream_t s = ctx->stream;
+    ensure_val64(ctx, P->full.m);
     if (P->dirichlet_mode == 1) {
         // P1_2D symmetric elimination + drop |a| <= 1e-14 (sparse_from_dense,
         // src/solvers/fortfem_sparse_matrix.f90:16-44)
