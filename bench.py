@@ -120,7 +120,10 @@ def run_c3(args):
                 ts.append(time.perf_counter() - t1)
             ms = float(np.median(ts)) * 1e3
             sv = 8 if prec == "fp64" else 4
-            by = T * (4.0 * nl * nl + 2.0 * sv * nl * nl)  # slot + value read-modify-write
+            # per element entry: int32 slot + value store (first touch of a slot,
+            # nnz of them) or read-modify-write (the rest); 12 FP64 terms per cell
+            ent = float(T) * nl * nl
+            by = 4.0 * ent + sv * nnz + 2.0 * sv * (ent - nnz) + 96.0 * T
             row[prec + "_ms"] = round(ms, 3)
             row[prec + "_gbs"] = round(by / (ms * 1e-3) / 1e9, 1)
             row[prec + "_gflops"] = round(24.0 * nl * nl * T / (ms * 1e-3) / 1e9, 1)
