@@ -282,6 +282,31 @@ __global__ __launch_bounds__(256) void k_ned_assemble(
     }
 }
 
+// 2D P1 load (assemble_scalar_p1_load, src/fortfem_forms_simple.f90:1730-1761):
+// f area / 3 to each vertex, one colour per launch.
+__global__ __launch_bounds__(kBlockA) void k_p1_2d_load(int pos0, int pos1,
+                                                         const int *__restrict__ cells_c,
+                                                         const double *__restrict__ verts,
+                                                         double f, double *__restrict__ rhs,
+                                                         int *__restrict__ err)
+{
+    const int pos = pos0 + blockIdx.x * kBlockA + threadIdx.x;
+    if (pos >= pos1) return;
+    const int *c = cells_c + 3 * (size_t)pos;
+    double x[3], y[3];
+    for (int k = 0; k < 3; ++k) {
+        x[k] = verts[2 * (size_t)c[k]];
+        y[k] = verts[2 * (size_t)c[k] + 1];
+    }
+    const double det = (x[1] - x[0]) * (y[2] - y[0]) - (x[2] - x[0]) * (y[1] - y[0]);
+    if (!(fabs(det) > 64.0 * 2.220446049250313e-16)) {
+        atomicOr(err, 1);
+        return;
+    }
+    const double v = f * 0.5 * fabs(det) / 3.0;
+    for (int k = 0; k < 3; ++k) rhs[c[k]] += v;
+}
+
 // Load phase 1: element vectors det sum_q w_q f(x_q) phi_l(q) into
 // EL[l][pos]; tab per point = xi(3), w phi(NL).
 template <int NL>
@@ -821,8 +846,26 @@ void problem_load(Problem *P, int qdeg, int kind, const double *params, int64_t 
         P->red_dirty = true;
         return;
     }
+    if (P->family == FFH_FAMILY_P1_2D) {
+        FFH_REQUIRE(kind == FFH_SOURCE_CONSTANT && params && np >= 1, FFH_ERR_INVALID,
+                    "the 2D P1 load takes a constant coefficient");
+        FFH_HIP(hipMemsetAsync(P->rhs_full.p, 0, sizeof(double) * G, s));
+        for (int c = 0; c < P->n_colors; ++c) {
+            const int p0 = P->color_start[c], p1 = P->color_start[c + 1];
+            const int grid = (p1 - p0 + kBlockA - 1) / kBlockA;
+            if (grid == 0) continue;
+            hipLaunchKernelGGL(k_p1_2d_load, dim3(grid), dim3(kBlockA), 0, s, p0, p1,
+                               P->d_cells_c.p, P->d_vertices.p, params[0], P->rhs_full.p,
+                               P->d_err.p);
+        }
+        FFH_HIP(hipGetLastError());
+        check_err(P, "2D P1 load");
+        P->loaded = true;
+        P->red_dirty = true;
+        return;
+    }
     FFH_REQUIRE(P->family == FFH_FAMILY_H1, FFH_ERR_INVALID,
-                "closed-form loads are implemented for the H1 family");
+                "closed-form loads are implemented for the H1 and 2D P1 families");
     FFH_REQUIRE(qdeg >= 0, FFH_ERR_INVALID, "quadrature degree must be >= 0");
     if (P->load_qdeg != qdeg) {
         std::vector<double> tab;

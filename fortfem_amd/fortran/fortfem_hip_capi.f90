@@ -60,6 +60,7 @@ module fortfem_hip_capi
 
     public :: ffh_init, ffh_finalize, ffh_last_error, ffh_solver_options_default
     public :: ffh_box_mesh_size, ffh_box_mesh, ffh_unit_square_mesh, ffh_tetra_duffy
+    public :: ffh_nedelec_dof_map, ffh_nedelec_basis
     public :: ffh_problem_create, ffh_problem_free, ffh_pattern_build
     public :: ffh_problem_sizes, ffh_global_dofs_download, ffh_assemble, ffh_load
     public :: ffh_dirichlet, ffh_constrained_download, ffh_set_rhs, ffh_rhs_download
@@ -121,6 +122,24 @@ module fortfem_hip_capi
             integer(c_int), intent(out) :: nq
             real(c_double), intent(out) :: x(*), y(*), z(*), w(*)
         end function ffh_tetra_duffy
+
+        integer(c_int) function ffh_nedelec_dof_map(k, ntet, tets, base, &
+                global_dofs, edge_orient, face_perm, ndofs) &
+                bind(C, name="ffh_nedelec_dof_map")
+            import :: c_int
+            integer(c_int), value :: k, ntet, base
+            integer(c_int), intent(in) :: tets(*)
+            integer(c_int), intent(out) :: global_dofs(*), edge_orient(*), face_perm(*)
+            integer(c_int), intent(out) :: ndofs
+        end function ffh_nedelec_dof_map
+
+        integer(c_int) function ffh_nedelec_basis(k, npts, points, values, curls) &
+                bind(C, name="ffh_nedelec_basis")
+            import :: c_int, c_double
+            integer(c_int), value :: k, npts
+            real(c_double), intent(in) :: points(*)
+            real(c_double), intent(out) :: values(*), curls(*)
+        end function ffh_nedelec_basis
 
         integer(c_int) function ffh_problem_create(ctx, dim, nvert, vertices, &
                 ncell, cells, prob) bind(C, name="ffh_problem_create")

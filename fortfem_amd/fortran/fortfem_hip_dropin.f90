@@ -374,6 +374,172 @@ contains
 
 end module fortfem_assembly_tetra_lagrange_arbitrary_order_3d
 
+! src/assembly/assembly_tetra_nedelec_3d.f90: curl-mass and tensor-weighted
+! first-kind Nedelec CSC assembly, orders 1..6, on the GPU.
+module fortfem_assembly_tetra_nedelec_3d
+    use, intrinsic :: iso_c_binding, only: c_int, c_int64_t, c_double, c_ptr
+    use fortfem_hip_types, only: dp, csc_t, fortsparse_status_t, status_set, &
+        FORTSPARSE_INVALID_MATRIX
+    use fortfem_hip_capi
+    use fortfem_hip_runtime, only: hip_context
+    use fortfem_assembly_tetra_lagrange_arbitrary_order_3d, only: download_csc
+    implicit none
+    private
+
+    public :: assemble_tetra_nedelec_curl_mass_csc
+    public :: assemble_tetra_nedelec_weighted_csc
+    public :: assemble_tetra_nedelec_weighted_csc_cells
+    public :: tensor_coefficient_3d
+
+    abstract interface
+        pure subroutine tensor_coefficient_3d(x, y, z, value)
+            import :: dp
+            real(dp), intent(in) :: x, y, z
+            real(dp), intent(out) :: value(3, 3)
+        end subroutine tensor_coefficient_3d
+    end interface
+
+contains
+
+    ! Shared driver: pattern + assembly of one Nedelec problem, downloaded.
+    subroutine nedelec_csc(mesh_vertices, tetrahedra, k, kind, coeff, mass, &
+            matrix, ok)
+        real(dp), intent(in) :: mesh_vertices(:, :)
+        integer, intent(in) :: tetrahedra(:, :)
+        integer, intent(in) :: k
+        integer(c_int), intent(in) :: kind
+        real(c_double), intent(in) :: coeff(*)
+        real(dp), intent(in) :: mass
+        type(csc_t), intent(out) :: matrix
+        logical, intent(out) :: ok
+        type(c_ptr) :: prob, mat
+        integer(c_int) :: n_global, code
+        integer(c_int64_t) :: nnz
+
+        ok = .false.
+        if (size(mesh_vertices, 1) /= 3) return
+        if (size(tetrahedra, 1) /= 4 .or. size(tetrahedra, 2) < 1) return
+        if (any(tetrahedra < 1) .or. &
+            any(tetrahedra > size(mesh_vertices, 2))) return
+        code = ffh_problem_create(hip_context(), 3_c_int, &
+            int(size(mesh_vertices, 2), c_int), mesh_vertices, &
+            int(size(tetrahedra, 2), c_int), tetrahedra, prob)
+        if (code /= FFH_OK) return
+        code = ffh_pattern_build(prob, FFH_FAMILY_NEDELEC, int(k, c_int), &
+            n_global, nnz)
+        if (code == FFH_OK) code = ffh_assemble(prob, -1_c_int, kind, coeff, &
+            real(mass, c_double), FFH_FP64)
+        if (code == FFH_OK) code = ffh_problem_matrix(prob, FFH_FULL, mat)
+        if (code == FFH_OK) call download_csc(mat, matrix, ok)
+        code = ffh_problem_free(prob)
+    end subroutine nedelec_csc
+
+    ! :2042-2112.  curl_coefficient and mass_coefficient default to 1, order
+    ! to 1.  cell_coefficient (optional, new): per-cell curl coefficient.
+    subroutine assemble_tetra_nedelec_curl_mass_csc(mesh_vertices, tetrahedra, &
+            matrix, status, curl_coefficient, mass_coefficient, order, &
+            cell_coefficient)
+        real(dp), intent(in) :: mesh_vertices(:, :)
+        integer, intent(in) :: tetrahedra(:, :)
+        type(csc_t), intent(out) :: matrix
+        type(fortsparse_status_t), intent(out) :: status
+        real(dp), intent(in), optional :: curl_coefficient, mass_coefficient
+        integer, intent(in), optional :: order
+        real(dp), intent(in), optional :: cell_coefficient(:)
+        real(c_double) :: alpha(1)
+        real(dp) :: beta
+        integer :: k
+        logical :: ok
+
+        call status_set(status, FORTSPARSE_INVALID_MATRIX, &
+            "Tetrahedral Nedelec sparse assembly failed")
+        alpha = 1.0_dp
+        beta = 1.0_dp
+        k = 1
+        if (present(curl_coefficient)) alpha = curl_coefficient
+        if (present(mass_coefficient)) beta = mass_coefficient
+        if (present(order)) k = order
+        if (k < 1 .or. k > 6) return
+        if (present(cell_coefficient)) then
+            if (size(cell_coefficient) /= size(tetrahedra, 2)) return
+            call nedelec_csc(mesh_vertices, tetrahedra, k, FFH_COEFF_CELL, &
+                cell_coefficient, beta, matrix, ok)
+        else
+            call nedelec_csc(mesh_vertices, tetrahedra, k, FFH_COEFF_SCALAR, &
+                alpha, beta, matrix, ok)
+        end if
+        if (ok) call status_set(status, 0, "")
+    end subroutine assemble_tetra_nedelec_curl_mass_csc
+
+    ! :1870-2000.  The device element takes one symmetric tensor per cell:
+    ! the callback is sampled at each cell's centroid and four vertices and
+    ! must return the same tensor at all five points (a tensor varying inside
+    ! a cell fails with a status message instead of being approximated).
+    subroutine assemble_tetra_nedelec_weighted_csc(mesh_vertices, tetrahedra, &
+            coefficient, mass_coefficient, matrix, status, order)
+        real(dp), intent(in) :: mesh_vertices(:, :)
+        integer, intent(in) :: tetrahedra(:, :)
+        procedure(tensor_coefficient_3d) :: coefficient
+        real(dp), intent(in) :: mass_coefficient
+        type(csc_t), intent(out) :: matrix
+        type(fortsparse_status_t), intent(out) :: status
+        integer, intent(in), optional :: order
+        real(dp), allocatable :: tensors(:, :, :)
+        real(dp) :: centre(3), kv(3, 3), scale
+        integer :: cell, corner
+
+        call status_set(status, FORTSPARSE_INVALID_MATRIX, &
+            "Tetrahedral weighted Nedelec assembly failed")
+        if (size(mesh_vertices, 1) /= 3 .or. size(tetrahedra, 1) /= 4) return
+        if (any(tetrahedra < 1) .or. &
+            any(tetrahedra > size(mesh_vertices, 2))) return
+        allocate(tensors(3, 3, size(tetrahedra, 2)))
+        do cell = 1, size(tetrahedra, 2)
+            centre = 0.25_dp*sum(mesh_vertices(:, tetrahedra(:, cell)), dim=2)
+            call coefficient(centre(1), centre(2), centre(3), tensors(:, :, cell))
+            scale = max(1.0_dp, maxval(abs(tensors(:, :, cell))))
+            do corner = 1, 4
+                associate (x => mesh_vertices(:, tetrahedra(corner, cell)))
+                    call coefficient(x(1), x(2), x(3), kv)
+                end associate
+                if (maxval(abs(kv - tensors(:, :, cell))) > 1.0e-14_dp*scale) then
+                    call status_set(status, FORTSPARSE_INVALID_MATRIX, &
+                        "weighted Nedelec assembly needs a tensor constant on each cell")
+                    return
+                end if
+            end do
+        end do
+        call assemble_tetra_nedelec_weighted_csc_cells(mesh_vertices, &
+            tetrahedra, tensors, mass_coefficient, matrix, status, order)
+    end subroutine assemble_tetra_nedelec_weighted_csc
+
+    ! New: one symmetric tensor K(3,3,cell) per cell.
+    subroutine assemble_tetra_nedelec_weighted_csc_cells(mesh_vertices, &
+            tetrahedra, cell_tensor, mass_coefficient, matrix, status, order)
+        real(dp), intent(in) :: mesh_vertices(:, :)
+        integer, intent(in) :: tetrahedra(:, :)
+        real(dp), intent(in) :: cell_tensor(:, :, :)
+        real(dp), intent(in) :: mass_coefficient
+        type(csc_t), intent(out) :: matrix
+        type(fortsparse_status_t), intent(out) :: status
+        integer, intent(in), optional :: order
+        integer :: k
+        logical :: ok
+
+        call status_set(status, FORTSPARSE_INVALID_MATRIX, &
+            "Tetrahedral weighted Nedelec assembly failed")
+        k = 1
+        if (present(order)) k = order
+        if (k < 1 .or. k > 6) return
+        if (size(cell_tensor, 1) /= 3 .or. size(cell_tensor, 2) /= 3) return
+        if (size(cell_tensor, 3) /= size(tetrahedra, 2)) return
+        call nedelec_csc(mesh_vertices, tetrahedra, k, FFH_COEFF_CELL_TENSOR, &
+            cell_tensor, mass_coefficient, matrix, ok)
+        if (ok) call status_set(status, 0, "")
+    end subroutine assemble_tetra_nedelec_weighted_csc_cells
+
+end module fortfem_assembly_tetra_nedelec_3d
+
 module fortfem_sparse_matrix
     use, intrinsic :: iso_c_binding, only: c_int, c_int64_t, c_ptr
     use fortfem_hip_types, only: dp, sparse_matrix_t => csc_t

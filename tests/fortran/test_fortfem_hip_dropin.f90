@@ -36,6 +36,7 @@ program test_fortfem_hip_dropin
     use fortfem_structured_tetra_box_mesh, only: generate_structured_tetra_box_mesh
     use fortfem_assembly_tetra_lagrange_arbitrary_order_3d, only: &
         assemble_tetra_lagrange_stiffness_csc
+    use fortfem_assembly_tetra_nedelec_3d, only: assemble_tetra_nedelec_curl_mass_csc
     use fortfem_sparse_matrix, only: spmv
     use fortfem_advanced_solvers, only: solver_options_t, solver_stats_t, &
         solver_options, solve_sparse
@@ -51,6 +52,7 @@ program test_fortfem_hip_dropin
     call check_cg_tridiagonal()
     call check_quartic_p4()
     call check_p2_box_poisson()
+    call check_nedelec_rotational_energy()
     call fortfem_hip_finalize()
     if (.not. all_passed) error stop 1
     write (*, '(a)') "drop-in Fortran checks passed"
@@ -108,6 +110,55 @@ contains
         call record(abs(volume - product(bounds(:, 2) - bounds(:, 1))) < 1.0e-12_dp, &
             "exact volume partition")
     end subroutine check_box_mesh
+
+    ! test_tetra_nedelec_sparse_assembly_slow.f90: the Whitney interpolant of
+    ! the rotational field 0.5 c x x (curl = c) on the two-tet mesh has
+    ! alpha |c|^2 |Omega| = 2 * 14 / 3 = 28/3 curl energy.
+    subroutine check_nedelec_rotational_energy()
+        real(dp) :: mesh_vertices(3, 5), c(3), mid(3), tangent(3), field(3)
+        real(dp), allocatable :: dofs(:), product(:)
+        integer :: tetrahedra(4, 2), edges(2, 12), cell, l, a, b, e, nedges
+        integer, parameter :: local(2, 6) = reshape([1, 2, 1, 3, 1, 4, 2, 3, 2, 4, 3, 4], [2, 6])
+        logical :: found
+        type(csc_t) :: matrix
+        type(fortsparse_status_t) :: status
+
+        mesh_vertices = reshape([0.0_dp, 0.0_dp, 0.0_dp, 1.0_dp, 0.0_dp, 0.0_dp, &
+            0.0_dp, 1.0_dp, 0.0_dp, 0.0_dp, 0.0_dp, 1.0_dp, 0.0_dp, 0.0_dp, -1.0_dp], [3, 5])
+        tetrahedra(:, 1) = [1, 2, 3, 4]
+        tetrahedra(:, 2) = [1, 3, 2, 5]
+        call assemble_tetra_nedelec_curl_mass_csc(mesh_vertices, tetrahedra, matrix, &
+            status, curl_coefficient=2.0_dp, mass_coefficient=0.0_dp, order=1)
+        call record(status%code == 0 .and. matrix%nrow == 9, "Nedelec k=1 CSC assembly succeeds")
+        ! edges by first appearance, (min, max)
+        nedges = 0
+        do cell = 1, 2
+            do l = 1, 6
+                a = minval(tetrahedra(local(:, l), cell))
+                b = maxval(tetrahedra(local(:, l), cell))
+                found = .false.
+                do e = 1, nedges
+                    if (edges(1, e) == a .and. edges(2, e) == b) found = .true.
+                end do
+                if (.not. found) then
+                    nedges = nedges + 1
+                    edges(:, nedges) = [a, b]
+                end if
+            end do
+        end do
+        allocate(dofs(nedges), product(nedges))
+        c = [1.0_dp, -2.0_dp, 3.0_dp]
+        do e = 1, nedges
+            mid = 0.5_dp*(mesh_vertices(:, edges(1, e)) + mesh_vertices(:, edges(2, e)))
+            tangent = mesh_vertices(:, edges(2, e)) - mesh_vertices(:, edges(1, e))
+            field = 0.5_dp*[c(2)*mid(3) - c(3)*mid(2), c(3)*mid(1) - c(1)*mid(3), &
+                c(1)*mid(2) - c(2)*mid(1)]
+            dofs(e) = dot_product(field, tangent)
+        end do
+        call spmv(matrix, dofs, product)
+        call record(abs(dot_product(dofs, product) - 28.0_dp/3.0_dp) < 3.0e-12_dp, &
+            "Nedelec rotational-field energy is 28/3")
+    end subroutine check_nedelec_rotational_energy
 
     subroutine check_linear_energy()
         real(dp) :: mesh_vertices(3, 5), coefficients(5), product(5)
