@@ -666,3 +666,92 @@ __all__ = [
     "solve_sparse", "solve_tetra_lagrange_poisson", "solver_options", "spmv", "unit_box_mesh",
     "unit_square_mesh",
 ]
+
+
+# ------------------------------------------------------------------ serving
+class Server:
+    """Continuous-batching solve server (ffh_server_*): requests on prepared
+    meshes are admitted by device-memory footprint and advanced together."""
+
+    def __init__(self, budget_bytes=8 << 30, max_systems=64, chunk=16, ctx: Context | None = None):
+        self.ctx = ctx or default_context()
+        self._lib = capi.load()
+        h = C.c_void_p()
+        self.ctx.check(self._lib.ffh_server_create(self.ctx.handle, int(budget_bytes), max_systems, chunk,
+                                                   C.byref(h)))
+        self.handle = h
+        self._n = {}
+
+    def close(self):
+        if getattr(self, "handle", None):
+            self._lib.ffh_server_free(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def prepare(self, mesh_key, family, order, vertices, tets):
+        v = _f64(vertices)
+        t = _i32(tets)
+        n = C.c_int(0)
+        nnz = C.c_int64(0)
+        self.ctx.check(self._lib.ffh_server_prepare(self.handle, mesh_key, family, order, v.shape[0], _ptr(v),
+                                                    t.shape[0], _ptr(t), C.byref(n), C.byref(nnz)))
+        self._n[(mesh_key, family, order)] = (n.value, t.shape[0])
+        return n.value, nnz.value
+
+    def submit(self, mesh_key, family, order, cell_coeff, rhs, mass=0.0, tol=1e-10, max_iterations=100000):
+        """rhs: (n_free,) or (nrhs, n_free) array.  Returns a ticket."""
+        b = _f64(np.atleast_2d(rhs))
+        c = _f64(cell_coeff)
+        tk = C.c_int64(0)
+        self.ctx.check(self._lib.ffh_server_submit(self.handle, mesh_key, family, order, _ptr(c), float(mass),
+                                                   b.shape[0], _ptr(b), float(tol), int(max_iterations),
+                                                   C.byref(tk)))
+        return tk.value, b.shape[0], b.shape[1]
+
+    def step(self):
+        a, q, f = C.c_int(0), C.c_int(0), C.c_int(0)
+        self.ctx.check(self._lib.ffh_server_step(self.handle, C.byref(a), C.byref(q), C.byref(f)))
+        return a.value, q.value, f.value
+
+    def poll(self, ticket, nrhs, n):
+        """None while pending, else (solutions (nrhs, n), iterations, converged, rel_residual)."""
+        x = np.zeros((nrhs, n))
+        it = np.zeros(nrhs, dtype=np.int32)
+        cv = np.zeros(nrhs, dtype=np.int32)
+        rr = np.zeros(nrhs)
+        code = self._lib.ffh_server_poll(self.handle, ticket, _ptr(x), _ptr(it), _ptr(cv), _ptr(rr))
+        if code == 1:
+            return None
+        self.ctx.check(code)
+        return x, it, cv.astype(bool), rr
+
+    def stats(self):
+        u, i = C.c_int64(0), C.c_int64(0)
+        self.ctx.check(self._lib.ffh_server_stats(self.handle, C.byref(u), C.byref(i)))
+        return {"used_bytes": u.value, "iterations_run": i.value}
+
+    def run_until_done(self, tickets, max_steps=1000000):
+        """Steps until every (ticket, nrhs, n) has finished; returns {ticket: result}."""
+        out = {}
+        pending = list(tickets)
+        for _ in range(max_steps):
+            if not pending:
+                break
+            self.step()
+            rest = []
+            for (tk, nrhs, n) in pending:
+                r = self.poll(tk, nrhs, n)
+                if r is None:
+                    rest.append((tk, nrhs, n))
+                else:
+                    out[tk] = r
+            pending = rest
+        if pending:
+            raise RuntimeError("server did not finish the requests")
+        return out
+

@@ -10,7 +10,6 @@
 
 using namespace ffh;
 
-struct ffh_context : Context {};
 struct ffh_problem : Problem {};
 struct ffh_matrix : Matrix {};
 

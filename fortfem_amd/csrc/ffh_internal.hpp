@@ -246,3 +246,6 @@ void lagrange_boundary(int d, int ntet, const int *tets,
                        std::vector<int> &constrained);
 
 } // namespace ffh
+
+// opaque handle of the C-ABI (include/fortfem_hip.h)
+struct ffh_context : ffh::Context {};

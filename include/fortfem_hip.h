@@ -63,6 +63,7 @@ extern "C" {
 typedef struct ffh_context ffh_context;
 typedef struct ffh_problem ffh_problem;
 typedef struct ffh_matrix ffh_matrix;
+typedef struct ffh_server ffh_server;
 
 /* Mirrors solver_options_t (src/solvers/advanced_solvers.f90:32-44) and the
  * solver_options() defaults (:1430-1465): method 'auto'->'pcg', tolerance
@@ -222,6 +223,41 @@ int ffh_spmv(ffh_matrix *mat, const double *x, double *y);
  * guess on entry and the solution on exit (host arrays). */
 int ffh_solve_sparse(ffh_matrix *mat, const double *b, double *x,
                      const ffh_solver_options *opts, ffh_solver_stats *stats);
+
+/* ---- served solves (continuous batching) ----------------------------------
+ * No single reference routine: the request loop serves what the reference
+ * does one call at a time (assemble_* + solve_sparse per right-hand side,
+ * advanced_solvers.f90:135-195, 831-925).  A server keeps symbolic problems
+ * per (mesh_key, family, order) and, per admitted request, its assembled
+ * reduced operator and PCG state in HBM.  All active systems and their
+ * right-hand sides advance together, one launch per PCG step; requests join
+ * and leave between steps; admission is by device-memory footprint.
+ * Solutions are over the free dofs (zero Dirichlet data). */
+/* budget_bytes: device memory for admitted requests; max_systems: batch
+ * slots; chunk: PCG iterations per step between FP64 residual refreshes. */
+int ffh_server_create(ffh_context *ctx, int64_t budget_bytes, int max_systems, int chunk,
+                      ffh_server **server);
+int ffh_server_free(ffh_server *server);
+/* Builds (once per key) the pattern, colouring and Dirichlet reduction of a
+ * tetrahedral mesh for FFH_FAMILY_H1 or FFH_FAMILY_NEDELEC; returns the
+ * free-dof count a request's right-hand sides must have. */
+int ffh_server_prepare(ffh_server *server, int mesh_key, int family, int order, int nvert,
+                       const double *vertices, int ncell, const int *cells, int *n_free,
+                       int64_t *nnz_free);
+/* Queues a request: per-cell coefficient (H1 stiffness kappa / Nedelec curl
+ * alpha), mass coefficient, nrhs (1..8) right-hand sides (n_free x nrhs,
+ * column-major), relative tolerance and iteration cap per right-hand side.
+ * FFH_ERR_ADMISSION if the request can never fit the budget. */
+int ffh_server_submit(ffh_server *server, int mesh_key, int family, int order,
+                      const double *cell_coeff, double mass, int nrhs, const double *rhs,
+                      double tol, int max_iterations, int64_t *ticket);
+/* Admits queued requests that fit, runs one step (chunk iterations + FP64
+ * refresh) of every active system, retires finished requests. */
+int ffh_server_step(ffh_server *server, int *n_active, int *n_queued, int *n_finished);
+/* 0 and the results (then the ticket is released) if finished, 1 if not yet. */
+int ffh_server_poll(ffh_server *server, int64_t ticket, double *solution, int *iterations,
+                    int *converged, double *rel_residual);
+int ffh_server_stats(const ffh_server *server, int64_t *used_bytes, int64_t *iterations_run);
 
 /* ---- measurement ---------------------------------------------------------- */
 /* Times `reps` back-to-back SpMV launches on the context stream with HIP

@@ -1,0 +1,133 @@
+"""GPU tests of the served (continuously batched) solve path.
+
+* every served solution is within 1e-8 (relative) of the
+  oracle's FP64 Jacobi-PCG solution of the same reduced system;
+* a request's result is bit-for-bit the same alone and in a mix (each system's
+  reductions run over its own row chunks in a fixed order);
+* admission respects the device-memory budget and rejects requests that can
+  never fit.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ff():
+    import fortfem_amd
+
+    return fortfem_amd
+
+
+def tangential_mask(k, t):
+    from test_gpu_nedelec_parity import _tangential_mask
+
+    return _tangential_mask(k, None, t)
+
+
+def oracle_system(ff, family, k, N, coeff):
+    """The reduced FP64 system a served request solves (zero Dirichlet data)."""
+    v, t = O.unit_box_mesh(N)
+    if family == ff.capi.FAMILY_H1:
+        q = 2 * k + 4
+        a, g, G = O.lagrange_stiffness_csc(v, t, k, q, cell_kappa=coeff)
+        cons, _ = O.lagrange_dirichlet(v, t, k, g, G)
+    else:
+        K = coeff[:, None, None] * np.eye(3)[None]
+        a = O.nedelec_csc(k, v, t, beta=1.0, cell_tensor=K)
+        G = a.n_rows
+        cons = tangential_mask(k, t).astype(np.int32)
+    red, free, _ = O.constrained_reduction(a, cons, np.zeros(G), np.zeros(G))
+    return v, t, red
+
+
+CASES = [(1, 1, 5, 3), (1, 2, 3, 2), (2, 1, 3, 1), (2, 2, 2, 4)]  # family, order, N, nrhs
+
+
+def make_requests(ff, server, seed=0):
+    rng = np.random.default_rng(seed)
+    reqs = []
+    for key, (fam, k, N, nrhs) in enumerate(CASES):
+        v, t, red = oracle_system(ff, fam, k, N, np.ones(6 * N ** 3))
+        n, _ = server.prepare(key, fam, k, v, t)
+        assert n == red.n_rows
+        coeff = np.exp(rng.normal(size=t.shape[0]))
+        b = rng.uniform(-1, 1, size=(nrhs, n))
+        mass = 0.0 if fam == ff.capi.FAMILY_H1 else 1.0
+        reqs.append(dict(key=key, fam=fam, k=k, N=N, coeff=coeff, b=b, mass=mass, n=n))
+    return reqs
+
+
+def test_served_solutions_match_oracle(ff):
+    srv = ff.Server(budget_bytes=1 << 30, max_systems=16, chunk=8)
+    reqs = make_requests(ff, srv)
+    tickets = []
+    for r in reqs:
+        tk, nrhs, n = srv.submit(r["key"], r["fam"], r["k"], r["coeff"], r["b"], mass=r["mass"], tol=1e-10)
+        r["ticket"] = tk
+        tickets.append((tk, nrhs, n))
+    res = srv.run_until_done(tickets)
+    for r in reqs:
+        x, it, conv, rel = res[r["ticket"]]
+        assert conv.all() and (rel <= 1e-10).all(), (r["fam"], r["k"], rel)
+        _, _, red = oracle_system(ff, r["fam"], r["k"], r["N"], r["coeff"])
+        for j in range(r["b"].shape[0]):
+            xo, st = O.solve_sparse(red, r["b"][j], method="pcg", preconditioner="jacobi", tolerance=1e-13,
+                                    max_iterations=100000, breakdown="relative")
+            err = np.linalg.norm(x[j] - xo) / np.linalg.norm(xo)
+            assert err <= 1e-8, (r["fam"], r["k"], j, err)
+
+
+def test_alone_equals_in_mix(ff):
+    alone = ff.Server(budget_bytes=1 << 30, max_systems=16, chunk=8)
+    reqs = make_requests(ff, alone, seed=3)
+    target = reqs[1]
+    tk, nrhs, n = alone.submit(target["key"], target["fam"], target["k"], target["coeff"], target["b"],
+                               mass=target["mass"])
+    xa = alone.run_until_done([(tk, nrhs, n)])[tk][0]
+    mix = ff.Server(budget_bytes=1 << 30, max_systems=16, chunk=8)
+    reqs2 = make_requests(ff, mix, seed=3)
+    order = [reqs2[3], reqs2[0], reqs2[1], reqs2[2]]
+    tickets = []
+    for i, r in enumerate(order):
+        tk2 = mix.submit(r["key"], r["fam"], r["k"], r["coeff"], r["b"], mass=r["mass"])
+        tickets.append(tk2)
+        if i == 1:
+            mix.step()  # the target joins while others are running
+    out = mix.run_until_done(tickets)
+    xb = out[tickets[2][0]][0]
+    assert np.array_equal(xa, xb)
+
+
+def test_admission_by_memory(ff):
+    v, t = O.unit_box_mesh(4)
+    srv = ff.Server(budget_bytes=1 << 30, max_systems=8, chunk=8)
+    n, nnz = srv.prepare(0, ff.capi.FAMILY_H1, 1, v, t)
+    one = nnz * 12 + n * 8 + n * 4 * 28 + 4096
+    small = ff.Server(budget_bytes=int(one * 1.5), max_systems=8, chunk=8)
+    small.prepare(0, ff.capi.FAMILY_H1, 1, v, t)
+    rng = np.random.default_rng(1)
+    tickets = [small.submit(0, ff.capi.FAMILY_H1, 1, np.ones(t.shape[0]), rng.uniform(-1, 1, (4, n)))
+               for _ in range(3)]
+    seen = 0
+    done = {}
+    for _ in range(10000):
+        a, q, f = small.step()
+        assert a <= 1
+        seen = max(seen, a)
+        for tk in tickets:
+            if tk[0] not in done:
+                r = small.poll(*tk)
+                if r is not None:
+                    done[tk[0]] = r
+        if len(done) == 3:
+            break
+    assert len(done) == 3 and seen == 1
+    tiny = ff.Server(budget_bytes=1024, max_systems=8, chunk=8)
+    tiny.prepare(0, ff.capi.FAMILY_H1, 1, v, t)
+    with pytest.raises(ff.FFHError) as e:
+        tiny.submit(0, ff.capi.FAMILY_H1, 1, np.ones(t.shape[0]), np.ones(n))
+    assert e.value.code == ff.capi.FFH_ERR_ADMISSION
