@@ -33,7 +33,9 @@ CPU fallback.
 """
 from __future__ import annotations
 
+import atexit
 import ctypes as C
+import weakref
 from dataclasses import dataclass, field
 
 import numpy as np
@@ -83,6 +85,20 @@ def _ptr(a):
     return a.ctypes.data_as(C.c_void_p) if a is not None else None
 
 
+_OWNED = weakref.WeakSet()  # problems, matrices and servers, closed before their context
+
+
+@atexit.register
+def _shutdown():
+    for obj in list(_OWNED):
+        try:
+            obj.close()
+        except Exception:
+            pass
+    if _DEFAULT is not None:
+        _DEFAULT.close()
+
+
 class Context:
     """One device + stream (ffh_context)."""
 
@@ -94,6 +110,9 @@ class Context:
 
     def close(self):
         if getattr(self, "handle", None):
+            for obj in list(_OWNED):
+                if getattr(obj, "ctx", None) is self:
+                    obj.close()
             self._lib.ffh_finalize(self.handle)
             self.handle = None
 
@@ -254,6 +273,7 @@ class Problem:
                                                     c.shape[0], _ptr(c), C.byref(h)))
         self.handle = h
         self.ncell = c.shape[0]
+        _OWNED.add(self)
 
     def close(self):
         if getattr(self, "handle", None):
@@ -357,6 +377,8 @@ class Matrix:
         self.borrowed = borrowed
         self._owner = owner
         self._lib = capi.load()
+        if not borrowed:
+            _OWNED.add(self)
 
     @classmethod
     def upload(cls, a: CSC, ctx: Context | None = None, base: int = 1):
@@ -681,6 +703,7 @@ class Server:
                                                    C.byref(h)))
         self.handle = h
         self._n = {}
+        _OWNED.add(self)
 
     def close(self):
         if getattr(self, "handle", None):
@@ -729,6 +752,13 @@ class Server:
             return None
         self.ctx.check(code)
         return x, it, cv.astype(bool), rr
+
+    def progress(self, ticket, nrhs):
+        st = np.zeros(nrhs, dtype=np.int32)
+        it = np.zeros(nrhs, dtype=np.int32)
+        rr = np.zeros(nrhs)
+        self.ctx.check(self._lib.ffh_server_progress(self.handle, ticket, _ptr(st), _ptr(it), _ptr(rr)))
+        return st, it, rr
 
     def stats(self):
         u, i = C.c_int64(0), C.c_int64(0)

@@ -156,6 +156,7 @@ __global__ __launch_bounds__(256) void k_b_alpha(const BatchSys *__restrict__ sy
                 // relative breakdown test; the FP64 refresh then decides
                 if (!(fabs(t) > 1.0e-14 * fabs(q.rz))) {
                     q.state = kCandidate;
+                    q.restart = 2; // breakdown: restart the direction after the refresh
                     q.alpha = 0.0;
                 } else {
                     q.alpha = q.rz / t;
@@ -226,7 +227,10 @@ __global__ __launch_bounds__(256) void k_b_beta(const BatchSys *__restrict__ sys
                 q.rr = trr;
                 q.beta = q.rz != 0.0 ? trz / q.rz : 0.0;
                 q.rz = trz;
-                if (trr <= q.tol2) q.state = kCandidate;
+                // pause for a refresh once converged by the recurrence, or
+                // once the FP32 residual has dropped by 1e3 since the last
+                // FP64 residual (beyond that the FP32 recurrence drifts)
+                if (trr <= q.tol2 || trr <= 1.0e-6 * q.true_rr) q.state = kCandidate;
                 else if (q.iters >= q.max_iter) q.state = kStopped;
             }
         }
@@ -262,7 +266,8 @@ __global__ __launch_bounds__(256) void k_b_resid(const int2 *__restrict__ work,
                                                  double *__restrict__ res_scratch,
                                                  double *__restrict__ part_rr,
                                                  double *__restrict__ part_rz,
-                                                 double *__restrict__ part_bb)
+                                                 double *__restrict__ part_bb,
+                                                 double *__restrict__ part_dd)
 {
     __shared__ double sh[256];
     const int2 wk = work[blockIdx.x];
@@ -281,7 +286,7 @@ __global__ __launch_bounds__(256) void k_b_resid(const int2 *__restrict__ work,
     __syncthreads();
     const int r = threadIdx.x % R;
     const BatchScal q = sc[s.slot * kMaxRhs + r];
-    double crr = 0.0, crz = 0.0, cbb = 0.0;
+    double crr = 0.0, crz = 0.0, cbb = 0.0, cdd = 0.0;
     const bool live = q.state == kActive || q.state == kCandidate || q.state == kFresh;
     for (int e = threadIdx.x; e < (row1 - row0) * R; e += 256) {
         const size_t i = (size_t)row0 * R + e;
@@ -289,7 +294,11 @@ __global__ __launch_bounds__(256) void k_b_resid(const int2 *__restrict__ work,
         const double bi = s.b[i];
         if (q.state == kFresh) s.x[i] = 0.0;
         const double ri = q.state == kFresh ? bi : bi - ax[i];
-        if (live) s.r[i] = (float)ri;
+        if (live) {
+            const double drift = (double)s.r[i] - ri; // recurrence vs FP64 residual
+            cdd += drift * drift;
+            s.r[i] = (float)ri;
+        }
         crr += ri * ri;
         crz += ri * s.dinv[row] * ri;
         cbb += bi * bi;
@@ -297,35 +306,41 @@ __global__ __launch_bounds__(256) void k_b_resid(const int2 *__restrict__ work,
     const double trr = rhs_sum(crr, R, sh);
     const double trz = rhs_sum(crz, R, sh);
     const double tbb = rhs_sum(cbb, R, sh);
+    const double tdd = rhs_sum(cdd, R, sh);
     if (threadIdx.x < R) {
         part_rr[(size_t)blockIdx.x * kMaxRhs + threadIdx.x] = trr;
         part_rz[(size_t)blockIdx.x * kMaxRhs + threadIdx.x] = trz;
         part_bb[(size_t)blockIdx.x * kMaxRhs + threadIdx.x] = tbb;
+        part_dd[(size_t)blockIdx.x * kMaxRhs + threadIdx.x] = tdd;
     }
 }
 
 // per system: true residual -> converged, or continue (reliable update keeps
-// p; a candidate that has not converged, or a fresh system, restarts p = D r)
+// p; a breakdown, or a fresh system, restarts p = D r)
 __global__ __launch_bounds__(256) void k_b_refresh(const BatchSys *__restrict__ sys,
                                                    BatchScal *__restrict__ sc,
                                                    const double *__restrict__ part_rr,
                                                    const double *__restrict__ part_rz,
-                                                   const double *__restrict__ part_bb)
+                                                   const double *__restrict__ part_bb,
+                                                   const double *__restrict__ part_dd)
 {
     __shared__ double sh[4];
     const BatchSys s = sys[blockIdx.x];
     for (int r = 0; r < s.R; ++r) {
-        double trr = 0.0, trz = 0.0, tbb = 0.0;
+        double trr = 0.0, trz = 0.0, tbb = 0.0, tdd = 0.0;
         for (int w = s.w0 + threadIdx.x; w < s.w1; w += 256) {
             trr += part_rr[(size_t)w * kMaxRhs + r];
             trz += part_rz[(size_t)w * kMaxRhs + r];
             tbb += part_bb[(size_t)w * kMaxRhs + r];
+            tdd += part_dd[(size_t)w * kMaxRhs + r];
         }
         trr = bsum256(trr, sh);
         trz = bsum256(trz, sh);
         tbb = bsum256(tbb, sh);
+        tdd = bsum256(tdd, sh);
         if (threadIdx.x == 0) {
             BatchScal &q = sc[s.slot * kMaxRhs + r];
+            const bool broke = q.restart == 2;
             q.restart = 0;
             if (q.state == kFresh) {
                 q.bb = tbb;
@@ -344,7 +359,11 @@ __global__ __launch_bounds__(256) void k_b_refresh(const BatchSys *__restrict__ 
                 } else if (q.iters >= q.max_iter) {
                     q.state = kStopped;
                 } else {
-                    q.restart = q.state == kCandidate;
+                    // reliable update: r is now the FP64 residual.  The
+                    // direction is kept while the recurrence tracked it
+                    // (drift under 10 %); after a breakdown or a larger
+                    // drift (the FP32 operator's own limit) CG restarts.
+                    q.restart = broke || tdd > 0.01 * trr;
                     q.state = kActive;
                     q.rz = trz;
                 }
@@ -425,7 +444,7 @@ struct ffh_server {
     ffh::DBuf<ffh::BatchSys> d_sys;
     ffh::DBuf<int2> d_work;
     ffh::DBuf<ffh::BatchScal> d_scal;
-    ffh::DBuf<double> d_p0, d_p1, d_p2, d_scratch;
+    ffh::DBuf<double> d_p0, d_p1, d_p2, d_p3, d_scratch;
     std::vector<ffh::BatchScal> h_scal;
     int nwork = 0;
     int64_t iterations_run = 0;
@@ -551,6 +570,7 @@ static void rebuild_batch(ffh_server *S)
         S->d_p0.alloc(np);
         S->d_p1.alloc(np);
         S->d_p2.alloc(np);
+        S->d_p3.alloc(np);
     }
     if (S->d_scratch.n < scratch) S->d_scratch.alloc(std::max<size_t>(scratch, 1));
     S->nwork = (int)work.size();
@@ -577,9 +597,9 @@ static void run_chunk(ffh_server *S)
         hipLaunchKernelGGL(k_b_dir, dim3(W), dim3(256), 0, s, S->d_work.p, S->d_sys.p, S->d_scal.p, 0);
     }
     hipLaunchKernelGGL(k_b_resid, dim3(W), dim3(256), 0, s, S->d_work.p, S->d_sys.p, S->d_scal.p,
-                       S->d_scratch.p, S->d_p0.p, S->d_p1.p, S->d_p2.p);
+                       S->d_scratch.p, S->d_p0.p, S->d_p1.p, S->d_p2.p, S->d_p3.p);
     hipLaunchKernelGGL(k_b_refresh, dim3(NS), dim3(256), 0, s, S->d_sys.p, S->d_scal.p, S->d_p0.p,
-                       S->d_p1.p, S->d_p2.p);
+                       S->d_p1.p, S->d_p2.p, S->d_p3.p);
     hipLaunchKernelGGL(k_b_dir, dim3(W), dim3(256), 0, s, S->d_work.p, S->d_sys.p, S->d_scal.p, 1);
     FFH_HIP(hipGetLastError());
     S->iterations_run += S->chunk;
@@ -786,6 +806,30 @@ int ffh_server_poll(ffh_server *S, int64_t ticket, double *solution, int *iterat
         if (rel_residual) rel_residual[r] = q->resid[r];
     }
     S->reqs.erase(it);
+    return FFH_OK;
+}
+
+int ffh_server_progress(const ffh_server *S, int64_t ticket, int *state, int *iterations,
+                        double *rel_residual)
+{
+    if (!S) return FFH_ERR_HANDLE;
+    auto it = S->reqs.find(ticket);
+    if (it == S->reqs.end()) return FFH_ERR_HANDLE;
+    const ffh::ServedRequest *q = it->second.get();
+    for (int r = 0; r < q->nrhs; ++r) {
+        int st = q->done ? (q->converged[r] ? ffh::kConverged : ffh::kStopped) : -1;
+        int itn = q->done ? q->iters[r] : 0;
+        double rel = q->done ? q->resid[r] : -1.0;
+        if (!q->done && q->slot >= 0) {
+            const ffh::BatchScal &c = S->h_scal[(size_t)q->slot * ffh::kMaxRhs + r];
+            st = c.state;
+            itn = c.iters;
+            rel = c.bb > 0 ? std::sqrt(c.true_rr / c.bb) : 0.0;
+        }
+        if (state) state[r] = st;
+        if (iterations) iterations[r] = itn;
+        if (rel_residual) rel_residual[r] = rel;
+    }
     return FFH_OK;
 }
 

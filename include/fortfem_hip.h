@@ -258,6 +258,12 @@ int ffh_server_step(ffh_server *server, int *n_active, int *n_queued, int *n_fin
 int ffh_server_poll(ffh_server *server, int64_t ticket, double *solution, int *iterations,
                     int *converged, double *rel_residual);
 int ffh_server_stats(const ffh_server *server, int64_t *used_bytes, int64_t *iterations_run);
+/* Progress of a request as of the last step: per right-hand side the state
+ * (-1 queued, 0 iterating, 1 converged, 2 stopped, 4 waiting for the FP64
+ * refresh, 5 admitted, not started), iterations and the relative FP64
+ * residual of the last refresh. */
+int ffh_server_progress(const ffh_server *server, int64_t ticket, int *state, int *iterations,
+                        double *rel_residual);
 
 /* ---- measurement ---------------------------------------------------------- */
 /* Times `reps` back-to-back SpMV launches on the context stream with HIP
