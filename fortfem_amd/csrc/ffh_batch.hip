@@ -227,11 +227,9 @@ __global__ __launch_bounds__(256) void k_b_beta(const BatchSys *__restrict__ sys
                 q.rr = trr;
                 q.beta = q.rz != 0.0 ? trz / q.rz : 0.0;
                 q.rz = trz;
-                // pause for a refresh once converged by the recurrence, or
-                // once the FP32 residual has dropped by 1e3 since the last
-                // FP64 residual (beyond that the FP32 recurrence drifts)
-                if (trr <= q.tol2 || trr <= 1.0e-6 * q.true_rr) q.state = kCandidate;
-                else if (q.iters >= q.max_iter) q.state = kStopped;
+                // convergence is decided on the FP64 residual at the end of
+                // the step; the recurrence only counts iterations here
+                if (q.iters >= q.max_iter) q.state = kStopped;
             }
         }
     }
@@ -360,10 +358,11 @@ __global__ __launch_bounds__(256) void k_b_refresh(const BatchSys *__restrict__ 
                     q.state = kStopped;
                 } else {
                     // reliable update: r is now the FP64 residual.  The
-                    // direction is kept while the recurrence tracked it
-                    // (drift under 10 %); after a breakdown or a larger
-                    // drift (the FP32 operator's own limit) CG restarts.
-                    q.restart = broke || tdd > 0.01 * trr;
+                    // direction is kept while the recurrence still tracks
+                    // it (drift under half the residual); after a breakdown
+                    // or once the recurrence has run past what the FP32
+                    // operator resolves, CG restarts from r.
+                    q.restart = broke || tdd > 0.25 * trr;
                     q.state = kActive;
                     q.rz = trz;
                 }

@@ -69,7 +69,7 @@ def test_served_solutions_match_oracle(ff):
         tk, nrhs, n = srv.submit(r["key"], r["fam"], r["k"], r["coeff"], r["b"], mass=r["mass"], tol=1e-10)
         r["ticket"] = tk
         tickets.append((tk, nrhs, n))
-    res = srv.run_until_done(tickets)
+    res = srv.run_until_done(tickets, max_steps=20000)
     for r in reqs:
         x, it, conv, rel = res[r["ticket"]]
         assert conv.all() and (rel <= 1e-10).all(), (r["fam"], r["k"], rel)
@@ -87,7 +87,7 @@ def test_alone_equals_in_mix(ff):
     target = reqs[1]
     tk, nrhs, n = alone.submit(target["key"], target["fam"], target["k"], target["coeff"], target["b"],
                                mass=target["mass"])
-    xa = alone.run_until_done([(tk, nrhs, n)])[tk][0]
+    xa = alone.run_until_done([(tk, nrhs, n)], max_steps=20000)[tk][0]
     mix = ff.Server(budget_bytes=1 << 30, max_systems=16, chunk=8)
     reqs2 = make_requests(ff, mix, seed=3)
     order = [reqs2[3], reqs2[0], reqs2[1], reqs2[2]]
@@ -97,7 +97,7 @@ def test_alone_equals_in_mix(ff):
         tickets.append(tk2)
         if i == 1:
             mix.step()  # the target joins while others are running
-    out = mix.run_until_done(tickets)
+    out = mix.run_until_done(tickets, max_steps=20000)
     xb = out[tickets[2][0]][0]
     assert np.array_equal(xa, xb)
 
