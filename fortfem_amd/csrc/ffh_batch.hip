@@ -248,11 +248,13 @@ __global__ __launch_bounds__(256) void k_b_dir(const int2 *__restrict__ work,
     const int r = threadIdx.x % R;
     const BatchScal q = sc[s.slot * kMaxRhs + r];
     if (restart_only ? (q.restart == 0 || q.state != kActive) : q.state != kActive) return;
-    const float bt = restart_only ? 0.0f : (float)q.beta;
+    const float bt = (float)q.beta;
     for (int e = threadIdx.x; e < (row1 - row0) * R; e += 256) {
         const size_t i = (size_t)row0 * R + e;
         const int row = row0 + e / R;
-        s.p[i] = (float)(s.dinv[row] * (double)s.r[i]) + bt * s.p[i];
+        const float z = (float)(s.dinv[row] * (double)s.r[i]);
+        // a restart never reads the old direction (it may hold anything)
+        s.p[i] = restart_only ? z : z + bt * s.p[i];
     }
 }
 
@@ -509,11 +511,12 @@ static void admit(ffh_server *S, ServedRequest *q)
                        q->nrhs, q->R, tmp.p, q->b.p);
     q->x.alloc(nr);
     q->r.alloc(nr);
-    FFH_HIP(hipMemsetAsync(q->r.p, 0, sizeof(float) * nr, s));
     q->p.alloc(nr);
     q->ap.alloc(nr);
     FFH_HIP(hipGetLastError());
     FFH_HIP(hipMemsetAsync(q->x.p, 0, sizeof(double) * nr, s));
+    FFH_HIP(hipMemsetAsync(q->r.p, 0, sizeof(float) * nr, s));
+    FFH_HIP(hipMemsetAsync(q->p.p, 0, sizeof(float) * nr, s));
     FFH_HIP(hipStreamSynchronize(s)); // tmp and the cached problem's values are reused next
     q->slot = S->free_slots.back();
     S->free_slots.pop_back();
