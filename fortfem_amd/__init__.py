@@ -359,6 +359,12 @@ class Problem:
         self._check(self._lib.ffh_problem_matrix(self.handle, which, C.byref(h)))
         return Matrix(h, self.ctx, borrowed=True, owner=self)
 
+    def solution(self):
+        """The last solve's solution over all n_global dofs (Dirichlet values included)."""
+        x = np.zeros(self.sizes()["n_global"])
+        self._check(self._lib.ffh_problem_solution(self.handle, _ptr(x)))
+        return x
+
     def solve(self, opts: SolverOptionsT, download=True):
         s = self.sizes()
         x = np.zeros(s["n_global"]) if download else None

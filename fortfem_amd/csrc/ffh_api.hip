@@ -470,6 +470,18 @@ int ffh_problem_solve(ffh_problem *P, const ffh_solver_options *o, double *solut
     });
 }
 
+int ffh_problem_solution(ffh_problem *P, double *solution)
+{
+    if (!P) return FFH_ERR_HANDLE;
+    if (!solution) return FFH_ERR_INVALID;
+    return guarded(P->ctx, [&] {
+        FFH_REQUIRE(P->x_full.p && (int)P->x_full.n >= P->n_global, FFH_ERR_INVALID,
+                    "no solution yet: solve first");
+        P->x_full.download(solution, P->n_global, P->ctx->stream);
+        FFH_HIP(hipStreamSynchronize(P->ctx->stream));
+    });
+}
+
 int ffh_matrix_upload(ffh_context *ctx, int n, int64_t nnz, int base, const int *col_ptr,
                       const int *row_idx, const double *val, ffh_matrix **out)
 {

@@ -52,3 +52,35 @@ CONFIGS = {
     "c2_p2_tet": dict(N=32, degree=2),
     "c4_p1_tet": dict(N=128, degree=1),
 }
+
+
+def served_stream(count: int, config: int = 5):
+    """The served request stream (configuration 5): request i draws, from its
+    own seed BASE_SEED + config + 1000 i, a type (P1 35 %, P2 35 %, Nedelec
+    k=1 15 %, k=2 15 %), the cube size N in 8..48 and 1..8 right-hand sides.
+    Returns a list of dicts (family code 1 = H1, 2 = Nedelec)."""
+    out = []
+    for i in range(count):
+        u = uniform(BASE_SEED + config + 1000 * i, 3)
+        t = u[0]
+        if t < 0.35:
+            fam, k = 1, 1
+        elif t < 0.70:
+            fam, k = 1, 2
+        elif t < 0.85:
+            fam, k = 2, 1
+        else:
+            fam, k = 2, 2
+        N = 8 + int(u[1] * 41)
+        nrhs = 1 + int(u[2] * 8)
+        out.append({"index": i, "family": fam, "order": k, "N": N, "nrhs": nrhs,
+                    "seed": BASE_SEED + config + 1000 * i})
+    return out
+
+
+def request_data(req: dict, ncell: int, n_free: int):
+    """Per-cell lognormal coefficient and U[-1, 1] right-hand sides (nrhs, n_free)."""
+    coeff = np.exp(normal(req["seed"] + 1, ncell))
+    rhs = -1.0 + 2.0 * uniform(req["seed"] + 2, req["nrhs"] * n_free).reshape(req["nrhs"], n_free)
+    return coeff, rhs
+

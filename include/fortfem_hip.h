@@ -202,6 +202,9 @@ int ffh_problem_matrix(ffh_problem *prob, int which, ffh_matrix **mat);
  * solution(n_global), constrained entries = boundary values. */
 int ffh_problem_solve(ffh_problem *prob, const ffh_solver_options *opts,
                       double *solution, ffh_solver_stats *stats);
+/* The last solve's solution (n_global values, Dirichlet values included),
+ * downloaded without solving again. */
+int ffh_problem_solution(ffh_problem *prob, double *solution);
 
 /* ---- CSC matrices (fortsparse csc_t) -------------------------------------- */
 /* csc_t with col_ptr(n+1), row_idx(nnz), val(nnz) of the given base.  A
