@@ -159,6 +159,7 @@ def run_c5(args):
         if r["N"] not in meshes:
             meshes[r["N"]] = ff.unit_box_mesh(r["N"])
     srv = ff.Server(budget_bytes=int(args.budget_gb * (1 << 30)), max_systems=args.slots, chunk=args.chunk)
+    srv.set_precision(args.precision)
     probs = {}
     t0 = time.perf_counter()
     for r in stream:
@@ -238,6 +239,7 @@ def run_c5(args):
            "arrival": "all requests queued at t = 0 (saturated)",
            "max_rel_err_vs_fp64": max(errs), "all_converged": conv,
            "iterations_served": iters_srv, "iterations_fp64": iters64, "prepare_s": round(t_prep, 2),
+           "precision_policy": args.precision, "switched_to_fp64": srv.set_precision(-1),
            "served_s": round(t_served, 3), "fp64_s": round(t_base, 3)}
     print(json.dumps(out))
 
@@ -305,6 +307,8 @@ def main():
     ap.add_argument("--chunk", type=int, default=16, help="c5: iterations per server step")
     ap.add_argument("--budget-gb", type=float, default=64.0, help="c5: device memory for admitted requests")
     ap.add_argument("--max-iter", type=int, default=200000, help="c5: iteration cap per right-hand side")
+    ap.add_argument("--precision", type=int, default=0,
+                    help="c5: 0 FP32 iterations (FP64 on stagnation), 1 FP64, 2 FP64 for Nedelec")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "

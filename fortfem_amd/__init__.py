@@ -759,6 +759,13 @@ class Server:
         self.ctx.check(code)
         return x, it, cv.astype(bool), rr
 
+    def set_precision(self, policy=-1):
+        """0 FP32 iterations (switch to FP64 on stagnation), 1 FP64, 2 FP64 for
+        Nedelec; returns the number of requests switched so far."""
+        sw = C.c_int64(0)
+        self.ctx.check(self._lib.ffh_server_set_precision(self.handle, policy, C.byref(sw)))
+        return sw.value
+
     def progress(self, ticket, nrhs):
         st = np.zeros(nrhs, dtype=np.int32)
         it = np.zeros(nrhs, dtype=np.int32)

@@ -49,7 +49,7 @@ EXPORTS = (
     "ffh_spmv", "ffh_solve_sparse", "ffh_bench_spmv", "ffh_set_profiling",
     "ffh_last_solve_profile", "ffh_matrix_spmv_variant", "ffh_nedelec_dof_map", "ffh_nedelec_basis",
     "ffh_server_create", "ffh_server_free", "ffh_server_prepare", "ffh_server_submit", "ffh_server_step",
-    "ffh_server_poll", "ffh_server_stats", "ffh_server_progress", "ffh_problem_solution",
+    "ffh_server_poll", "ffh_server_stats", "ffh_server_progress", "ffh_problem_solution", "ffh_server_set_precision",
 )
 
 
@@ -146,6 +146,7 @@ def load() -> C.CDLL:
         "ffh_server_stats": (C.c_int, [vp, i64p, i64p]),
         "ffh_server_progress": (C.c_int, [vp, C.c_int64, vp, vp, vp]),
         "ffh_problem_solution": (C.c_int, [vp, vp]),
+        "ffh_server_set_precision": (C.c_int, [vp, C.c_int, i64p]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)

@@ -41,8 +41,9 @@ struct BatchSys {
     const double *dinv;
     const double *b; // n x R
     double *x;       // n x R
-    float *r, *p, *ap;
-    int n, R, w0, w1, slot, pad;
+    float *r, *p, *ap;       // FP32 iteration
+    double *r64, *p64, *ap64; // FP64 iteration (prec = 1)
+    int n, R, w0, w1, slot, prec;
 };
 
 // per (system slot, right-hand side)
@@ -113,6 +114,20 @@ __device__ __forceinline__ void chunk_spmv(const int *__restrict__ ptr, const in
     }
 }
 
+template <typename VT>
+__device__ __forceinline__ double spmv_body(const BatchSys &s, const VT *val, const VT *p, VT *ap,
+                                            int row0, int row1)
+{
+    double dot = 0.0;
+    switch (s.R) {
+    case 1: chunk_spmv<1, VT, VT, VT>(s.ptr, s.idx, val, p, row0, row1, ap, dot, p); break;
+    case 2: chunk_spmv<2, VT, VT, VT>(s.ptr, s.idx, val, p, row0, row1, ap, dot, p); break;
+    case 4: chunk_spmv<4, VT, VT, VT>(s.ptr, s.idx, val, p, row0, row1, ap, dot, p); break;
+    default: chunk_spmv<8, VT, VT, VT>(s.ptr, s.idx, val, p, row0, row1, ap, dot, p); break;
+    }
+    return dot;
+}
+
 __global__ __launch_bounds__(256) void k_b_spmv(const int2 *__restrict__ work,
                                                 const BatchSys *__restrict__ sys,
                                                 const BatchScal *__restrict__ sc,
@@ -122,13 +137,8 @@ __global__ __launch_bounds__(256) void k_b_spmv(const int2 *__restrict__ work,
     const int2 wk = work[blockIdx.x];
     const BatchSys s = sys[wk.x];
     const int row0 = wk.y, row1 = min(wk.y + kRowsPerItem, s.n);
-    double dot = 0.0;
-    switch (s.R) {
-    case 1: chunk_spmv<1, float, float, float>(s.ptr, s.idx, s.val32, s.p, row0, row1, s.ap, dot, s.p); break;
-    case 2: chunk_spmv<2, float, float, float>(s.ptr, s.idx, s.val32, s.p, row0, row1, s.ap, dot, s.p); break;
-    case 4: chunk_spmv<4, float, float, float>(s.ptr, s.idx, s.val32, s.p, row0, row1, s.ap, dot, s.p); break;
-    default: chunk_spmv<8, float, float, float>(s.ptr, s.idx, s.val32, s.p, row0, row1, s.ap, dot, s.p); break;
-    }
+    const double dot = s.prec ? spmv_body<double>(s, s.val64, s.p64, s.ap64, row0, row1)
+                              : spmv_body<float>(s, s.val32, s.p, s.ap, row0, row1);
     // lane r of each 8-lane group holds right-hand side r
     sh[threadIdx.x] = dot;
     __syncthreads();
@@ -168,6 +178,24 @@ __global__ __launch_bounds__(256) void k_b_alpha(const BatchSys *__restrict__ sy
     }
 }
 
+template <typename VT>
+__device__ __forceinline__ void update_body(const BatchSys &s, VT *r, const VT *p, const VT *ap,
+                                            double a, int row0, int row1, double &crr,
+                                            double &crz)
+{
+    const int R = s.R;
+    for (int e = threadIdx.x; e < (row1 - row0) * R; e += 256) {
+        const size_t i = (size_t)row0 * R + e;
+        const int row = row0 + e / R;
+        s.x[i] += a * (double)p[i];
+        const VT ri = r[i] - (VT)a * ap[i];
+        r[i] = ri;
+        const double zi = s.dinv[row] * (double)ri;
+        crr += (double)ri * (double)ri;
+        crz += (double)ri * zi;
+    }
+}
+
 // x += alpha p, r -= alpha Ap, partials of r.r and r.(D r)
 __global__ __launch_bounds__(256) void k_b_update(const int2 *__restrict__ work,
                                                   const BatchSys *__restrict__ sys,
@@ -184,17 +212,10 @@ __global__ __launch_bounds__(256) void k_b_update(const int2 *__restrict__ work,
     const BatchScal q = sc[s.slot * kMaxRhs + rr_];
     double crr = 0.0, crz = 0.0;
     if (q.state == kActive) {
-        const double a = q.alpha;
-        for (int e = threadIdx.x; e < (row1 - row0) * R; e += 256) {
-            const size_t i = (size_t)row0 * R + e;
-            const int row = row0 + e / R;
-            s.x[i] += a * (double)s.p[i];
-            const float ri = s.r[i] - (float)a * s.ap[i];
-            s.r[i] = ri;
-            const double zi = s.dinv[row] * (double)ri;
-            crr += (double)ri * (double)ri;
-            crz += (double)ri * zi;
-        }
+        if (s.prec)
+            update_body<double>(s, s.r64, s.p64, s.ap64, q.alpha, row0, row1, crr, crz);
+        else
+            update_body<float>(s, s.r, s.p, s.ap, q.alpha, row0, row1, crr, crz);
     }
     const double trr = rhs_sum(crr, R, sh);
     const double trz = rhs_sum(crz, R, sh);
@@ -236,6 +257,21 @@ __global__ __launch_bounds__(256) void k_b_beta(const BatchSys *__restrict__ sys
 }
 
 // p = D r + beta p
+template <typename VT>
+__device__ __forceinline__ void dir_body(const BatchSys &s, const VT *r, VT *p, double beta,
+                                         bool restart, int row0, int row1)
+{
+    const int R = s.R;
+    const VT bt = (VT)beta;
+    for (int e = threadIdx.x; e < (row1 - row0) * R; e += 256) {
+        const size_t i = (size_t)row0 * R + e;
+        const int row = row0 + e / R;
+        const VT z = (VT)(s.dinv[row] * (double)r[i]);
+        // a restart never reads the old direction (it may hold anything)
+        p[i] = restart ? z : z + bt * p[i];
+    }
+}
+
 // (restart_only: p = D r where the refresh asked for a restart)
 __global__ __launch_bounds__(256) void k_b_dir(const int2 *__restrict__ work,
                                                const BatchSys *__restrict__ sys,
@@ -248,17 +284,14 @@ __global__ __launch_bounds__(256) void k_b_dir(const int2 *__restrict__ work,
     const int r = threadIdx.x % R;
     const BatchScal q = sc[s.slot * kMaxRhs + r];
     if (restart_only ? (q.restart == 0 || q.state != kActive) : q.state != kActive) return;
-    const float bt = (float)q.beta;
-    for (int e = threadIdx.x; e < (row1 - row0) * R; e += 256) {
-        const size_t i = (size_t)row0 * R + e;
-        const int row = row0 + e / R;
-        const float z = (float)(s.dinv[row] * (double)s.r[i]);
-        // a restart never reads the old direction (it may hold anything)
-        s.p[i] = restart_only ? z : z + bt * s.p[i];
-    }
+    if (s.prec)
+        dir_body<double>(s, s.r64, s.p64, q.beta, restart_only != 0, row0, row1);
+    else
+        dir_body<float>(s, s.r, s.p, q.beta, restart_only != 0, row0, row1);
 }
 
-// FP64 residual b - A x (reliable update): r <- residual (FP32 copy),
+// FP64 residual b - A x (reliable update): r <- residual (FP32 copy, or
+// FP64 for a system iterating in FP64),
 // partials of its r.r and r.(D r).  Fresh systems start here with x = 0.
 __global__ __launch_bounds__(256) void k_b_resid(const int2 *__restrict__ work,
                                                  const BatchSys *__restrict__ sys,
@@ -295,9 +328,13 @@ __global__ __launch_bounds__(256) void k_b_resid(const int2 *__restrict__ work,
         if (q.state == kFresh) s.x[i] = 0.0;
         const double ri = q.state == kFresh ? bi : bi - ax[i];
         if (live) {
-            const double drift = (double)s.r[i] - ri; // recurrence vs FP64 residual
+            // recurrence vs FP64 residual
+            const double drift = (s.prec ? s.r64[i] : (double)s.r[i]) - ri;
             cdd += drift * drift;
-            s.r[i] = (float)ri;
+            if (s.prec)
+                s.r64[i] = ri;
+            else
+                s.r[i] = (float)ri;
         }
         crr += ri * ri;
         crz += ri * s.dinv[row] * ri;
@@ -412,7 +449,11 @@ struct ServedRequest {
     int max_iter = 100000;
     size_t bytes = 0;
     DBuf<float> val32, r, p, ap;
-    DBuf<double> val64, dinv, b, x;
+    DBuf<double> val64, dinv, b, x, r64, p64, ap64;
+    int family = 0;
+    int prec = 0;                 // 0 FP32 iteration, 1 FP64 iteration
+    std::vector<double> best_rr;  // best FP64 residual seen, per right-hand side
+    std::vector<int> stall;       // refreshes without halving it
     int slot = -1;
     bool done = false;
     std::vector<double> sol;
@@ -434,6 +475,8 @@ struct ffh_server {
     ffh::Context *ctx = nullptr;
     size_t budget = 0, used = 0;
     int max_sys = 64, chunk = 16;
+    int policy = 0; // 0 FP32 iteration + switch on stagnation, 1 FP64, 2 FP64 for Nedelec
+    int64_t switches = 0;
     std::map<ffh::ServerKey, std::unique_ptr<ffh::Problem>> cache;
     std::map<int64_t, std::unique_ptr<ffh::ServedRequest>> reqs;
     std::deque<ffh::ServedRequest *> queue;
@@ -453,9 +496,21 @@ struct ffh_server {
 
 namespace ffh {
 
+// worst-case device bytes of a request: both value copies, b and x, and the
+// FP32 and FP64 iteration vectors (an FP32 request may switch to FP64)
 static size_t request_bytes(int n, int64_t nnz, int R)
 {
-    return (size_t)nnz * 12 + (size_t)n * 8 + (size_t)n * R * (8 + 8 + 12) + 4096;
+    return (size_t)nnz * 12 + (size_t)n * 8 + (size_t)n * R * (8 + 8 + 12 + 24) + 4096;
+}
+
+static void alloc_fp64_vectors(ServedRequest *q, hipStream_t s)
+{
+    const size_t nr = (size_t)q->n * q->R;
+    q->r64.alloc(nr);
+    q->p64.alloc(nr);
+    q->ap64.alloc(nr);
+    FFH_HIP(hipMemsetAsync(q->r64.p, 0, sizeof(double) * nr, s));
+    FFH_HIP(hipMemsetAsync(q->p64.p, 0, sizeof(double) * nr, s));
 }
 
 static int rhs_stride(int nrhs) { return nrhs <= 1 ? 1 : nrhs <= 2 ? 2 : nrhs <= 4 ? 4 : 8; }
@@ -510,13 +565,20 @@ static void admit(ffh_server *S, ServedRequest *q)
     hipLaunchKernelGGL(k_b_interleave, dim3((unsigned)((nr + 255) / 256)), dim3(256), 0, s, n,
                        q->nrhs, q->R, tmp.p, q->b.p);
     q->x.alloc(nr);
-    q->r.alloc(nr);
-    q->p.alloc(nr);
-    q->ap.alloc(nr);
     FFH_HIP(hipGetLastError());
     FFH_HIP(hipMemsetAsync(q->x.p, 0, sizeof(double) * nr, s));
-    FFH_HIP(hipMemsetAsync(q->r.p, 0, sizeof(float) * nr, s));
-    FFH_HIP(hipMemsetAsync(q->p.p, 0, sizeof(float) * nr, s));
+    q->prec = S->policy == 1 || (S->policy == 2 && q->family == FFH_FAMILY_NEDELEC) ? 1 : 0;
+    if (q->prec) {
+        alloc_fp64_vectors(q, s);
+    } else {
+        q->r.alloc(nr);
+        q->p.alloc(nr);
+        q->ap.alloc(nr);
+        FFH_HIP(hipMemsetAsync(q->r.p, 0, sizeof(float) * nr, s));
+        FFH_HIP(hipMemsetAsync(q->p.p, 0, sizeof(float) * nr, s));
+    }
+    q->best_rr.assign(q->nrhs, 1e300);
+    q->stall.assign(q->nrhs, 0);
     FFH_HIP(hipStreamSynchronize(s)); // tmp and the cached problem's values are reused next
     q->slot = S->free_slots.back();
     S->free_slots.pop_back();
@@ -553,6 +615,10 @@ static void rebuild_batch(ffh_server *S)
         b.r = q->r.p;
         b.p = q->p.p;
         b.ap = q->ap.p;
+        b.r64 = q->r64.p;
+        b.p64 = q->p64.p;
+        b.ap64 = q->ap64.p;
+        b.prec = q->prec;
         b.n = q->n;
         b.R = q->R;
         b.slot = q->slot;
@@ -607,6 +673,42 @@ static void run_chunk(ffh_server *S)
     S->iterations_run += S->chunk;
 }
 
+// An FP32-iterating request whose FP64 residual has not halved over 16
+// refreshes (the FP32 operator cannot resolve further progress) continues
+// in FP64 from its current x: its directions restart after the next refresh.
+static void watch_stagnation(ffh_server *S, ServedRequest *q)
+{
+    bool stuck = false;
+    for (int r = 0; r < q->nrhs; ++r) {
+        const BatchScal &c = S->h_scal[(size_t)q->slot * kMaxRhs + r];
+        if (c.state != kActive) continue;
+        if (c.true_rr < 0.25 * q->best_rr[r]) {
+            q->best_rr[r] = c.true_rr;
+            q->stall[r] = 0;
+        } else if (++q->stall[r] >= 16) {
+            stuck = true;
+        }
+    }
+    if (!stuck) return;
+    hipStream_t s = S->ctx->stream;
+    alloc_fp64_vectors(q, s);
+    q->prec = 1;
+    for (int r = 0; r < q->nrhs; ++r) {
+        BatchScal &c = S->h_scal[(size_t)q->slot * kMaxRhs + r];
+        if (c.state == kActive || c.state == kCandidate) {
+            c.state = kCandidate;
+            c.restart = 2;
+        }
+    }
+    FFH_HIP(hipMemcpyAsync(S->d_scal.p + (size_t)q->slot * kMaxRhs,
+                           S->h_scal.data() + (size_t)q->slot * kMaxRhs, sizeof(BatchScal) * kMaxRhs,
+                           hipMemcpyHostToDevice, s));
+    FFH_HIP(hipStreamSynchronize(s));
+    q->r.reset(), q->p.reset(), q->ap.reset();
+    S->changed = true;
+    ++S->switches;
+}
+
 static void retire_finished(ffh_server *S, int *finished)
 {
     hipStream_t s = S->ctx->stream;
@@ -622,6 +724,7 @@ static void retire_finished(ffh_server *S, int *finished)
         }
         if (!all) {
             keep.push_back(q);
+            if (q->prec == 0) watch_stagnation(S, q);
             continue;
         }
         const size_t nr = (size_t)q->n * q->R;
@@ -642,6 +745,7 @@ static void retire_finished(ffh_server *S, int *finished)
         }
         q->val32.reset(), q->r.reset(), q->p.reset(), q->ap.reset();
         q->val64.reset(), q->dinv.reset(), q->b.reset(), q->x.reset();
+        q->r64.reset(), q->p64.reset(), q->ap64.reset();
         q->rhs.clear();
         q->rhs.shrink_to_fit();
         S->free_slots.push_back(q->slot);
@@ -685,6 +789,7 @@ int64_t server_submit(ffh_server *S, int mesh_key, int family, int order, const 
     auto q = std::make_unique<ServedRequest>();
     q->ticket = S->next_ticket++;
     q->sym = P;
+    q->family = family;
     q->n = P->reduced.m.n;
     q->nnz = P->reduced.m.nnz;
     q->nrhs = nrhs;
@@ -840,6 +945,15 @@ int ffh_server_stats(const ffh_server *S, int64_t *used_bytes, int64_t *iteratio
     if (!S) return FFH_ERR_HANDLE;
     if (used_bytes) *used_bytes = (int64_t)S->used;
     if (iterations_run) *iterations_run = S->iterations_run;
+    return FFH_OK;
+}
+
+int ffh_server_set_precision(ffh_server *S, int policy, int64_t *switches)
+{
+    if (!S) return FFH_ERR_HANDLE;
+    if (policy < -1 || policy > 2) return FFH_ERR_INVALID;
+    if (policy >= 0) S->policy = policy;
+    if (switches) *switches = S->switches;
     return FFH_OK;
 }
 

@@ -61,8 +61,10 @@ def make_requests(ff, server, seed=0):
     return reqs
 
 
-def test_served_solutions_match_oracle(ff):
+@pytest.mark.parametrize("policy", [0, 1, 2])
+def test_served_solutions_match_oracle(ff, policy):
     srv = ff.Server(budget_bytes=1 << 30, max_systems=16, chunk=8)
+    srv.set_precision(policy)
     reqs = make_requests(ff, srv)
     tickets = []
     for r in reqs:
