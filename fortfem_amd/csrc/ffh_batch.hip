@@ -248,9 +248,16 @@ __global__ __launch_bounds__(256) void k_b_beta(const BatchSys *__restrict__ sys
                 q.rr = trr;
                 q.beta = q.rz != 0.0 ? trz / q.rz : 0.0;
                 q.rz = trz;
-                // convergence is decided on the FP64 residual at the end of
-                // the step; the recurrence only counts iterations here
-                if (q.iters >= q.max_iter) q.state = kStopped;
+                // FP64 iteration: the reference's stop on the recurrence
+                // residual (pcg_iteration_loop :880-925), finalised at the
+                // refresh (which records the true residual).  FP32 iteration:
+                // convergence is decided on the FP64 residual at the refresh.
+                if (s.prec && trr <= q.tol2) {
+                    q.state = kCandidate;
+                    q.restart = 3;
+                } else if (q.iters >= q.max_iter) {
+                    q.state = kStopped;
+                }
             }
         }
     }
@@ -328,13 +335,14 @@ __global__ __launch_bounds__(256) void k_b_resid(const int2 *__restrict__ work,
         if (q.state == kFresh) s.x[i] = 0.0;
         const double ri = q.state == kFresh ? bi : bi - ax[i];
         if (live) {
-            // recurrence vs FP64 residual
+            // recurrence vs FP64 residual; an FP64 recurrence is kept as it
+            // is (the reference's iteration), only started from b
             const double drift = (s.prec ? s.r64[i] : (double)s.r[i]) - ri;
             cdd += drift * drift;
-            if (s.prec)
-                s.r64[i] = ri;
-            else
+            if (!s.prec)
                 s.r[i] = (float)ri;
+            else if (q.state == kFresh || q.restart == 2)
+                s.r64[i] = ri;
         }
         crr += ri * ri;
         crz += ri * s.dinv[row] * ri;
@@ -378,6 +386,7 @@ __global__ __launch_bounds__(256) void k_b_refresh(const BatchSys *__restrict__ 
         if (threadIdx.x == 0) {
             BatchScal &q = sc[s.slot * kMaxRhs + r];
             const bool broke = q.restart == 2;
+            const bool done_by_recurrence = q.state == kCandidate && q.restart == 3;
             q.restart = 0;
             if (q.state == kFresh) {
                 q.bb = tbb;
@@ -391,17 +400,24 @@ __global__ __launch_bounds__(256) void k_b_refresh(const BatchSys *__restrict__ 
             } else if (q.state == kActive || q.state == kCandidate) {
                 q.true_rr = trr;
                 q.rr = trr;
-                if (trr <= q.tol2) {
+                if (trr <= q.tol2 || done_by_recurrence) {
                     q.state = kConverged;
                 } else if (q.iters >= q.max_iter) {
                     q.state = kStopped;
+                } else if (s.prec) {
+                    // FP64 iteration: the recurrence continues untouched
+                    // (after a breakdown it restarts from the FP64 residual)
+                    q.state = kActive;
+                    if (broke) {
+                        q.restart = 1;
+                        q.rz = trz;
+                    }
                 } else {
-                    // reliable update: r is now the FP64 residual.  The
-                    // direction is kept while the recurrence still tracks
-                    // it (drift under half the residual); after a breakdown
-                    // or once the recurrence has run past what the FP32
-                    // operator resolves, CG restarts from r.
-                    q.restart = broke || tdd > 0.25 * trr;
+                    // FP32 iteration, reliable update: r is now the FP64
+                    // residual.  The direction is kept while the recurrence
+                    // tracked it (drift under 1 % of the residual); after a
+                    // breakdown or a larger drift CG restarts from r.
+                    q.restart = broke || tdd > 1.0e-4 * trr;
                     q.state = kActive;
                     q.rz = trz;
                 }
