@@ -90,7 +90,7 @@ void load_matrix(Context *ctx, DevMatrix &A, int n, int64_t nnz, int base,
     A.row_idx.upload(ri, s);
     A.val.upload(val, nnz, s);
     A.symmetric = sym;
-    A.val32_valid = false;
+    A.values_changed();
     if (!sym) {
         A.t_ptr.upload(tp, s);
         A.t_idx.upload(ti, s);

@@ -1,23 +1,27 @@
 // ffh_batch.hip -- served solves: many resident systems with several
-// right-hand sides each, advanced together by one launch per PCG step.
+// right-hand sides each, advanced together by one launch per PCG phase.
 //
 // Reference loop: src/solvers/advanced_solvers.f90 pcg_iteration_loop
 // (:880-925), compute_solver_tolerance (:1063-1073), Jacobi diagonal
 // (:1124-1155).  Each (system, right-hand side) runs its own Jacobi-PCG with
 // x0 = 0 and the relative stop ||r|| <= tol ||b||.
 //
-// Design: the active systems form one block-diagonal operator.  A work table
-// lists row chunks (kRowsPerItem rows of one system); every PCG launch covers
-// the whole table, and the right-hand sides of a system are interleaved
-// (row-major n x R, R in {1, 2, 4, 8}) so one pass over the matrix serves all
-// of them.  Dot products are per-chunk partials, summed per system in chunk
-// order by one block per system (deterministic: a system's result does not
-// depend on what else is in the batch).  The inner iteration runs in FP32
-// (matrix, r, p, Ap) with FP64 x and FP64 reductions; at the end of every
-// chunk of iterations the residual is recomputed in FP64 from the FP64
-// matrix (reliable update: r is replaced, p kept), which decides convergence.
-// Systems join and leave only at chunk boundaries; admission is by the
-// device-memory footprint of the request.
+// Design: the active systems form one block-diagonal operator.  Each system's
+// matrix is held in SELL-64 form (slices of 64 rows, one lane per row, every
+// index/value load of a wave 64 consecutive words); the structure belongs to
+// the cached pattern, the values to the request.  A work table lists items of
+// four slices (256 rows, one block) of one system; every launch covers the
+// whole table.  The right-hand sides of a system are interleaved (row-major
+// n x R, R in {1, 2, 4, 8}) so one pass over the matrix serves all of them
+// and each lane moves its row's R values with vector loads.  Dot products
+// are per-item partials, summed per system in item order by one block per
+// system (deterministic: a system's result does not depend on what else is
+// in the batch).  The inner iteration runs in FP32 (matrix, r, p, Ap, D^-1)
+// with FP64 x and FP64 reductions; at the end of every chunk of iterations
+// the residual is recomputed in FP64 from the FP64 matrix (reliable update:
+// r is replaced, p kept while the recurrence tracked the true residual),
+// which decides convergence.  Systems join and leave only at chunk
+// boundaries; admission is by the device-memory footprint of the request.
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -31,122 +35,220 @@
 
 namespace ffh {
 
-constexpr int kRowsPerItem = 256;
+constexpr int kRowsPerItem = 256; // four SELL-64 slices, one block
 constexpr int kMaxRhs = 8;
 
 struct BatchSys {
-    const int *ptr, *idx;
-    const float *val32;
+    const int *off, *sidx; // SELL-64 structure of the cached reduced pattern
+    const float *val32;    // SELL values of the request
     const double *val64;
-    const double *dinv;
+    const float *dinv32;
+    const double *dinv64;
     const double *b; // n x R
     double *x;       // n x R
-    float *r, *p, *ap;       // FP32 iteration
+    float *r, *p, *ap;        // FP32 iteration
     double *r64, *p64, *ap64; // FP64 iteration (prec = 1)
-    int n, R, w0, w1, slot, prec;
+    int n, R, w0, w1, slot, prec, nslice;
+    int x_zero; // x is zero when the system starts (its first residual is b)
 };
 
 // per (system slot, right-hand side)
 enum BatchState : int { kActive = 0, kConverged = 1, kStopped = 2, kUnused = 3, kCandidate = 4,
                         kFresh = 5 };
 struct BatchScal {
-    double rz, alpha, beta, rr, bb, tol2, true_rr;
+    double rz, alpha, beta, rr, bb, tol2, true_rr, init_rr;
     int iters, state, max_iter, restart;
+    int flags; // kAbsTol: stop at ||r|| <= tol instead of tol ||r0||
+    int pad;
+};
+constexpr int kAbsTol = 1;
+
+// R values of one row, moved with one vector access
+template <typename T, int R>
+struct alignas(sizeof(T) * R) Pack {
+    T v[R];
 };
 
+template <typename T, int R>
+__device__ __forceinline__ Pack<T, R> ld(const T *base, size_t row)
+{
+    return *reinterpret_cast<const Pack<T, R> *>(base + row * R);
+}
+
+template <typename T, int R>
+__device__ __forceinline__ void st(T *base, size_t row, const Pack<T, R> &v)
+{
+    *reinterpret_cast<Pack<T, R> *>(base + row * R) = v;
+}
+
 // ---------------------------------------------------------------- kernels --
-// Block-wide total of per-thread values (256 threads), valid in thread 0.
-__device__ __forceinline__ double bsum256(double v, double *sh)
+// Per-right-hand-side totals over a 256-thread block: wave shuffles, then the
+// four waves in order; total r is written by thread r to out[r].
+template <int R>
+__device__ __forceinline__ void block_sums(double (&d)[R], double *sh, double *out)
 {
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        double v = d[r];
+        for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+        if (lane == 0) sh[wv * R + r] = v;
+    }
     __syncthreads();
-    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
+    if (threadIdx.x < R) {
+        const int r = threadIdx.x;
+        out[r] = sh[r] + sh[R + r] + sh[2 * R + r] + sh[3 * R + r];
+    }
     __syncthreads();
-    double t = 0.0;
-    if (threadIdx.x == 0) t = sh[0] + sh[1] + sh[2] + sh[3];
-    __syncthreads();
-    return t;
 }
 
-// Sum of the per-thread value of each right-hand side r = t % R over the
-// block, in thread order; result for r in thread r.
-__device__ __forceinline__ double rhs_sum(double v, int R, double *sh)
+// the R scalar states of a system, broadcast through LDS
+struct ScalView {
+    double alpha[kMaxRhs], beta[kMaxRhs];
+    int state[kMaxRhs], restart[kMaxRhs];
+    int any_active;
+};
+
+__device__ __forceinline__ void load_scal(const BatchScal *__restrict__ sc, int slot, int R,
+                                          ScalView &v)
 {
-    sh[threadIdx.x] = v;
+    if (threadIdx.x < R) {
+        const BatchScal q = sc[slot * kMaxRhs + threadIdx.x];
+        v.alpha[threadIdx.x] = q.alpha;
+        v.beta[threadIdx.x] = q.beta;
+        v.state[threadIdx.x] = q.state;
+        v.restart[threadIdx.x] = q.restart;
+    }
     __syncthreads();
-    double t = 0.0;
-    if (threadIdx.x < R)
-        for (int j = threadIdx.x; j < 256; j += R) t += sh[j];
+    if (threadIdx.x == 0) {
+        int a = 0;
+        for (int r = 0; r < R; ++r) a |= v.state[r] == kActive;
+        v.any_active = a;
+    }
     __syncthreads();
-    return t;
 }
 
-// Ap = A p over one chunk (8 lanes per row, R interleaved right-hand sides),
-// partials of p.Ap
-template <int R, typename MT, typename VT, typename AT>
-__device__ __forceinline__ void chunk_spmv(const int *__restrict__ ptr, const int *__restrict__ idx,
-                                           const MT *__restrict__ val, const VT *__restrict__ xin,
-                                           int row0, int row1, AT *acc_out, double &dot,
-                                           const VT *__restrict__ dotv)
+// Ap = A p over the item's four slices (wave = slice, lane = row) and the
+// item's partials of p.Ap per right-hand side.  Entries go in steps of U:
+// the step's index/value loads are issued together, then its gathers of the
+// R-wide rows of p.
+template <int R, typename MT, typename VT>
+__device__ __forceinline__ void item_spmv(const BatchSys &s, int slice0, const MT *__restrict__ val,
+                                          const VT *__restrict__ p, VT *__restrict__ ap,
+                                          double (&d)[R])
 {
-    const int lane = threadIdx.x & 7, grp = threadIdx.x >> 3;
-    for (int row = row0 + grp; row < row1; row += 32) {
-        AT acc[R];
+    constexpr int U = R >= 4 ? 4 : 8;
+    const int lane = threadIdx.x & 63;
+    const int sl = slice0 + (threadIdx.x >> 6);
 #pragma unroll
-        for (int r = 0; r < R; ++r) acc[r] = AT(0);
-        const int b = ptr[row], e = ptr[row + 1];
-        for (int k = b + lane; k < e; k += 8) {
-            const int c = idx[k];
-            const AT v = (AT)val[k];
-            const VT *xc = xin + (size_t)c * R;
+    for (int r = 0; r < R; ++r) d[r] = 0.0;
+    if (sl >= s.nslice) return;
+    const int o = s.off[sl], w = (s.off[sl + 1] - o) >> 6;
+    const int *ip = s.sidx + o + lane;
+    const MT *vp = val + o + lane;
+    VT acc[R];
 #pragma unroll
-            for (int r = 0; r < R; ++r) acc[r] += v * (AT)xc[r];
+    for (int r = 0; r < R; ++r) acc[r] = VT(0);
+    for (int j = 0; j < w; j += U) {
+        int c[U];
+        VT v[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const bool in = j + u < w;
+            c[u] = in ? __builtin_nontemporal_load(ip + 64 * u) : 0;
+            v[u] = in ? (VT)__builtin_nontemporal_load(vp + 64 * u) : VT(0);
         }
+        Pack<VT, R> pc[U];
 #pragma unroll
-        for (int r = 0; r < R; ++r)
-            for (int off = 4; off > 0; off >>= 1) acc[r] += __shfl_xor(acc[r], off, 8);
+        for (int u = 0; u < U; ++u)
+            if (j + u < w) pc[u] = ld<VT, R>(p, c[u]);
 #pragma unroll
-        for (int r = 0; r < R; ++r)
-            if (lane == r) {
-                acc_out[(size_t)row * R + r] = acc[r];
-                if (dotv) dot += (double)dotv[(size_t)row * R + r] * (double)acc[r];
-            }
+        for (int u = 0; u < U; ++u)
+            if (j + u < w)
+#pragma unroll
+                for (int r = 0; r < R; ++r) acc[r] += v[u] * pc[u].v[r];
+        ip += 64 * U;
+        vp += 64 * U;
+    }
+    const int row = sl * 64 + lane;
+    if (row < s.n) {
+        const Pack<VT, R> own = ld<VT, R>(p, row);
+        Pack<VT, R> out;
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            out.v[r] = acc[r];
+            d[r] = (double)own.v[r] * (double)acc[r];
+        }
+        st<VT, R>(ap, row, out);
     }
 }
 
-template <typename VT>
-__device__ __forceinline__ double spmv_body(const BatchSys &s, const VT *val, const VT *p, VT *ap,
-                                            int row0, int row1)
+template <int R>
+__device__ __forceinline__ void spmv_item(const BatchSys &s, int slice0, double *sh, double *out)
 {
-    double dot = 0.0;
-    switch (s.R) {
-    case 1: chunk_spmv<1, VT, VT, VT>(s.ptr, s.idx, val, p, row0, row1, ap, dot, p); break;
-    case 2: chunk_spmv<2, VT, VT, VT>(s.ptr, s.idx, val, p, row0, row1, ap, dot, p); break;
-    case 4: chunk_spmv<4, VT, VT, VT>(s.ptr, s.idx, val, p, row0, row1, ap, dot, p); break;
-    default: chunk_spmv<8, VT, VT, VT>(s.ptr, s.idx, val, p, row0, row1, ap, dot, p); break;
-    }
-    return dot;
+    double d[R];
+    if (s.prec)
+        item_spmv<R, double, double>(s, slice0, s.val64, s.p64, s.ap64, d);
+    else
+        item_spmv<R, float, float>(s, slice0, s.val32, s.p, s.ap, d);
+    block_sums<R>(d, sh, out);
 }
 
+// Ap = A p of every item with an active right-hand side; partials of p.Ap
 __global__ __launch_bounds__(256) void k_b_spmv(const int2 *__restrict__ work,
                                                 const BatchSys *__restrict__ sys,
                                                 const BatchScal *__restrict__ sc,
                                                 double *__restrict__ part)
 {
-    __shared__ double sh[256];
+    __shared__ double sh[4 * kMaxRhs];
+    __shared__ ScalView sv;
     const int2 wk = work[blockIdx.x];
     const BatchSys s = sys[wk.x];
-    const int row0 = wk.y, row1 = min(wk.y + kRowsPerItem, s.n);
-    const double dot = s.prec ? spmv_body<double>(s, s.val64, s.p64, s.ap64, row0, row1)
-                              : spmv_body<float>(s, s.val32, s.p, s.ap, row0, row1);
-    // lane r of each 8-lane group holds right-hand side r
-    sh[threadIdx.x] = dot;
-    __syncthreads();
-    if (threadIdx.x < s.R) {
-        double t = 0.0;
-        for (int g = 0; g < 32; ++g) t += sh[g * 8 + threadIdx.x];
-        part[(size_t)blockIdx.x * kMaxRhs + threadIdx.x] = t;
+    load_scal(sc, s.slot, s.R, sv);
+    if (!sv.any_active) return;
+    double *out = part + (size_t)blockIdx.x * kMaxRhs;
+    const int slice0 = wk.y / 64;
+    switch (s.R) {
+    case 1: spmv_item<1>(s, slice0, sh, out); break;
+    case 2: spmv_item<2>(s, slice0, sh, out); break;
+    case 4: spmv_item<4>(s, slice0, sh, out); break;
+    default: spmv_item<8>(s, slice0, sh, out); break;
     }
+}
+
+// Totals of a system's per-item partials (items w0 .. w1 in order) for each
+// right-hand side; thread r holds total r after the call.
+template <int NQ>
+__device__ __forceinline__ void system_totals(const BatchSys &s, const double *const (&part)[NQ],
+                                              double (&tot)[NQ], double *sh)
+{
+    double acc[NQ][kMaxRhs];
+#pragma unroll
+    for (int q = 0; q < NQ; ++q)
+#pragma unroll
+        for (int r = 0; r < kMaxRhs; ++r) acc[q][r] = 0.0;
+    for (int w = s.w0 + threadIdx.x; w < s.w1; w += 256)
+#pragma unroll
+        for (int q = 0; q < NQ; ++q)
+#pragma unroll
+            for (int r = 0; r < kMaxRhs; ++r)
+                if (r < s.R) acc[q][r] += part[q][(size_t)w * kMaxRhs + r];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+    for (int q = 0; q < NQ; ++q)
+#pragma unroll
+        for (int r = 0; r < kMaxRhs; ++r) {
+            double v = acc[q][r];
+            for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+            if (lane == 0) sh[(q * kMaxRhs + r) * 4 + wv] = v;
+        }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+        const double *t = sh + (q * kMaxRhs + (threadIdx.x & (kMaxRhs - 1))) * 4;
+        tot[q] = t[0] + t[1] + t[2] + t[3];
+    }
+    __syncthreads();
 }
 
 // per system: p.Ap -> alpha (one block per system)
@@ -154,74 +256,91 @@ __global__ __launch_bounds__(256) void k_b_alpha(const BatchSys *__restrict__ sy
                                                  BatchScal *__restrict__ sc,
                                                  const double *__restrict__ part)
 {
-    __shared__ double sh[4];
+    __shared__ double sh[kMaxRhs * 4];
     const BatchSys s = sys[blockIdx.x];
-    for (int r = 0; r < s.R; ++r) {
-        double t = 0.0;
-        for (int w = s.w0 + threadIdx.x; w < s.w1; w += 256) t += part[(size_t)w * kMaxRhs + r];
-        t = bsum256(t, sh);
-        if (threadIdx.x == 0) {
-            BatchScal &q = sc[s.slot * kMaxRhs + r];
-            if (q.state == kActive) {
-                // relative breakdown test; the FP64 refresh then decides
-                if (!(fabs(t) > 1.0e-14 * fabs(q.rz))) {
-                    q.state = kCandidate;
-                    q.restart = 2; // breakdown: restart the direction after the refresh
-                    q.alpha = 0.0;
-                } else {
-                    q.alpha = q.rz / t;
-                }
-            } else {
+    const double *const parts[1] = {part};
+    double tot[1];
+    system_totals<1>(s, parts, tot, sh);
+    if (threadIdx.x < s.R) {
+        BatchScal &q = sc[s.slot * kMaxRhs + threadIdx.x];
+        const double t = tot[0];
+        if (q.state == kActive) {
+            // relative breakdown test; the FP64 refresh then decides
+            if (!(fabs(t) > 1.0e-14 * fabs(q.rz))) {
+                q.state = kCandidate;
+                q.restart = 2; // breakdown: restart the direction after the refresh
                 q.alpha = 0.0;
+            } else {
+                q.alpha = q.rz / t;
             }
+        } else {
+            q.alpha = 0.0;
         }
     }
 }
 
-template <typename VT>
-__device__ __forceinline__ void update_body(const BatchSys &s, VT *r, const VT *p, const VT *ap,
-                                            double a, int row0, int row1, double &crr,
-                                            double &crz)
+// x += alpha p, r -= alpha Ap (active right-hand sides), partials of r.r and
+// r.(D r); thread = row, R values per thread
+template <int R, typename VT>
+__device__ __forceinline__ void update_item(const BatchSys &s, int row, const ScalView &sv,
+                                            VT *__restrict__ rv, const VT *__restrict__ pv,
+                                            const VT *__restrict__ apv, const VT *__restrict__ dinv,
+                                            double (&crr)[R], double (&crz)[R])
 {
-    const int R = s.R;
-    for (int e = threadIdx.x; e < (row1 - row0) * R; e += 256) {
-        const size_t i = (size_t)row0 * R + e;
-        const int row = row0 + e / R;
-        s.x[i] += a * (double)p[i];
-        const VT ri = r[i] - (VT)a * ap[i];
-        r[i] = ri;
-        const double zi = s.dinv[row] * (double)ri;
-        crr += (double)ri * (double)ri;
-        crz += (double)ri * zi;
+#pragma unroll
+    for (int r = 0; r < R; ++r) crr[r] = crz[r] = 0.0;
+    if (row >= s.n) return;
+    Pack<double, R> x = ld<double, R>(s.x, row);
+    Pack<VT, R> ri = ld<VT, R>(rv, row);
+    const Pack<VT, R> pi = ld<VT, R>(pv, row), ai = ld<VT, R>(apv, row);
+    const double di = (double)dinv[row];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        if (sv.state[r] != kActive) continue;
+        const double a = sv.alpha[r];
+        x.v[r] += a * (double)pi.v[r];
+        ri.v[r] = ri.v[r] - (VT)a * ai.v[r];
+        const double rr = (double)ri.v[r];
+        crr[r] = rr * rr;
+        crz[r] = rr * (di * rr);
     }
+    st<double, R>(s.x, row, x);
+    st<VT, R>(rv, row, ri);
 }
 
-// x += alpha p, r -= alpha Ap, partials of r.r and r.(D r)
+template <int R>
+__device__ __forceinline__ void update_rows(const BatchSys &s, int row0, const ScalView &sv,
+                                            double *sh, double *o_rr, double *o_rz)
+{
+    double crr[R], crz[R];
+    const int row = row0 + threadIdx.x;
+    if (s.prec)
+        update_item<R, double>(s, row, sv, s.r64, s.p64, s.ap64, s.dinv64, crr, crz);
+    else
+        update_item<R, float>(s, row, sv, s.r, s.p, s.ap, s.dinv32, crr, crz);
+    block_sums<R>(crr, sh, o_rr);
+    block_sums<R>(crz, sh, o_rz);
+}
+
 __global__ __launch_bounds__(256) void k_b_update(const int2 *__restrict__ work,
                                                   const BatchSys *__restrict__ sys,
                                                   const BatchScal *__restrict__ sc,
                                                   double *__restrict__ part_rr,
                                                   double *__restrict__ part_rz)
 {
-    __shared__ double sh[256];
+    __shared__ double sh[4 * kMaxRhs];
+    __shared__ ScalView sv;
     const int2 wk = work[blockIdx.x];
     const BatchSys s = sys[wk.x];
-    const int R = s.R;
-    const int row0 = wk.y, row1 = min(wk.y + kRowsPerItem, s.n);
-    const int rr_ = threadIdx.x % R;
-    const BatchScal q = sc[s.slot * kMaxRhs + rr_];
-    double crr = 0.0, crz = 0.0;
-    if (q.state == kActive) {
-        if (s.prec)
-            update_body<double>(s, s.r64, s.p64, s.ap64, q.alpha, row0, row1, crr, crz);
-        else
-            update_body<float>(s, s.r, s.p, s.ap, q.alpha, row0, row1, crr, crz);
-    }
-    const double trr = rhs_sum(crr, R, sh);
-    const double trz = rhs_sum(crz, R, sh);
-    if (threadIdx.x < R) {
-        part_rr[(size_t)blockIdx.x * kMaxRhs + threadIdx.x] = trr;
-        part_rz[(size_t)blockIdx.x * kMaxRhs + threadIdx.x] = trz;
+    load_scal(sc, s.slot, s.R, sv);
+    if (!sv.any_active) return;
+    double *o_rr = part_rr + (size_t)blockIdx.x * kMaxRhs;
+    double *o_rz = part_rz + (size_t)blockIdx.x * kMaxRhs;
+    switch (s.R) {
+    case 1: update_rows<1>(s, wk.y, sv, sh, o_rr, o_rz); break;
+    case 2: update_rows<2>(s, wk.y, sv, sh, o_rr, o_rz); break;
+    case 4: update_rows<4>(s, wk.y, sv, sh, o_rr, o_rz); break;
+    default: update_rows<8>(s, wk.y, sv, sh, o_rr, o_rz); break;
     }
 }
 
@@ -231,52 +350,63 @@ __global__ __launch_bounds__(256) void k_b_beta(const BatchSys *__restrict__ sys
                                                 const double *__restrict__ part_rr,
                                                 const double *__restrict__ part_rz)
 {
-    __shared__ double sh[4];
+    __shared__ double sh[2 * kMaxRhs * 4];
     const BatchSys s = sys[blockIdx.x];
-    for (int r = 0; r < s.R; ++r) {
-        double trr = 0.0, trz = 0.0;
-        for (int w = s.w0 + threadIdx.x; w < s.w1; w += 256) {
-            trr += part_rr[(size_t)w * kMaxRhs + r];
-            trz += part_rz[(size_t)w * kMaxRhs + r];
-        }
-        trr = bsum256(trr, sh);
-        trz = bsum256(trz, sh);
-        if (threadIdx.x == 0) {
-            BatchScal &q = sc[s.slot * kMaxRhs + r];
-            if (q.state == kActive) {
-                q.iters += 1;
-                q.rr = trr;
-                q.beta = q.rz != 0.0 ? trz / q.rz : 0.0;
-                q.rz = trz;
-                // FP64 iteration: the reference's stop on the recurrence
-                // residual (pcg_iteration_loop :880-925), finalised at the
-                // refresh (which records the true residual).  FP32 iteration:
-                // convergence is decided on the FP64 residual at the refresh.
-                if (s.prec && trr <= q.tol2) {
-                    q.state = kCandidate;
-                    q.restart = 3;
-                } else if (q.iters >= q.max_iter) {
-                    q.state = kStopped;
-                }
+    const double *const parts[2] = {part_rr, part_rz};
+    double tot[2];
+    system_totals<2>(s, parts, tot, sh);
+    if (threadIdx.x < s.R) {
+        BatchScal &q = sc[s.slot * kMaxRhs + threadIdx.x];
+        const double trr = tot[0], trz = tot[1];
+        if (q.state == kActive) {
+            q.iters += 1;
+            q.rr = trr;
+            q.beta = q.rz != 0.0 ? trz / q.rz : 0.0;
+            q.rz = trz;
+            // FP64 iteration: the reference's stop on the recurrence
+            // residual (pcg_iteration_loop :880-925), finalised at the
+            // refresh (which records the true residual).  FP32 iteration:
+            // convergence is decided on the FP64 residual at the refresh.
+            if (s.prec && trr <= q.tol2) {
+                q.state = kCandidate;
+                q.restart = 3;
+            } else if (q.iters >= q.max_iter) {
+                q.state = kStopped;
             }
         }
     }
 }
 
-// p = D r + beta p
-template <typename VT>
-__device__ __forceinline__ void dir_body(const BatchSys &s, const VT *r, VT *p, double beta,
-                                         bool restart, int row0, int row1)
+// p = D r + beta p (a restart, p = D r, never reads the old direction)
+template <int R, typename VT>
+__device__ __forceinline__ void dir_item(const BatchSys &s, int row, const ScalView &sv,
+                                         const VT *__restrict__ rv, VT *__restrict__ pv,
+                                         const VT *__restrict__ dinv, bool restart_only)
 {
-    const int R = s.R;
-    const VT bt = (VT)beta;
-    for (int e = threadIdx.x; e < (row1 - row0) * R; e += 256) {
-        const size_t i = (size_t)row0 * R + e;
-        const int row = row0 + e / R;
-        const VT z = (VT)(s.dinv[row] * (double)r[i]);
-        // a restart never reads the old direction (it may hold anything)
-        p[i] = restart ? z : z + bt * p[i];
+    if (row >= s.n) return;
+    const Pack<VT, R> ri = ld<VT, R>(rv, row);
+    Pack<VT, R> pi = ld<VT, R>(pv, row);
+    const VT di = dinv[row];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        if (sv.state[r] != kActive) continue;
+        const bool rs = restart_only ? sv.restart[r] != 0 : false;
+        if (restart_only && !rs) continue;
+        const VT z = di * ri.v[r];
+        pi.v[r] = rs ? z : z + (VT)sv.beta[r] * pi.v[r];
     }
+    st<VT, R>(pv, row, pi);
+}
+
+template <int R>
+__device__ __forceinline__ void dir_rows(const BatchSys &s, int row0, const ScalView &sv,
+                                         bool restart_only)
+{
+    const int row = row0 + threadIdx.x;
+    if (s.prec)
+        dir_item<R, double>(s, row, sv, s.r64, s.p64, s.dinv64, restart_only);
+    else
+        dir_item<R, float>(s, row, sv, s.r, s.p, s.dinv32, restart_only);
 }
 
 // (restart_only: p = D r where the refresh asked for a restart)
@@ -284,79 +414,130 @@ __global__ __launch_bounds__(256) void k_b_dir(const int2 *__restrict__ work,
                                                const BatchSys *__restrict__ sys,
                                                const BatchScal *__restrict__ sc, int restart_only)
 {
+    __shared__ ScalView sv;
     const int2 wk = work[blockIdx.x];
     const BatchSys s = sys[wk.x];
-    const int R = s.R;
-    const int row0 = wk.y, row1 = min(wk.y + kRowsPerItem, s.n);
-    const int r = threadIdx.x % R;
-    const BatchScal q = sc[s.slot * kMaxRhs + r];
-    if (restart_only ? (q.restart == 0 || q.state != kActive) : q.state != kActive) return;
-    if (s.prec)
-        dir_body<double>(s, s.r64, s.p64, q.beta, restart_only != 0, row0, row1);
-    else
-        dir_body<float>(s, s.r, s.p, q.beta, restart_only != 0, row0, row1);
+    load_scal(sc, s.slot, s.R, sv);
+    if (!sv.any_active) return;
+    switch (s.R) {
+    case 1: dir_rows<1>(s, wk.y, sv, restart_only != 0); break;
+    case 2: dir_rows<2>(s, wk.y, sv, restart_only != 0); break;
+    case 4: dir_rows<4>(s, wk.y, sv, restart_only != 0); break;
+    default: dir_rows<8>(s, wk.y, sv, restart_only != 0); break;
+    }
 }
 
-// FP64 residual b - A x (reliable update): r <- residual (FP32 copy, or
-// FP64 for a system iterating in FP64),
-// partials of its r.r and r.(D r).  Fresh systems start here with x = 0.
+// FP64 residual b - A x of the item's rows (reliable update): r <- residual
+// (FP32 copy, or FP64 for a system iterating in FP64), partials of r.r,
+// r.(D r), b.b and of the recurrence drift.  A fresh system whose x is zero
+// takes r = b without a product.
+template <int R>
+__device__ __forceinline__ void resid_item(const BatchSys &s, int slice0, const ScalView &sv,
+                                           bool fresh, double *sh, double *o_rr, double *o_rz,
+                                           double *o_bb, double *o_dd)
+{
+    const int lane = threadIdx.x & 63;
+    const int sl = slice0 + (threadIdx.x >> 6);
+    double crr[R], crz[R], cbb[R], cdd[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) crr[r] = crz[r] = cbb[r] = cdd[r] = 0.0;
+    if (sl < s.nslice) {
+        double acc[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) acc[r] = 0.0;
+        if (!fresh || !s.x_zero) {
+            const int o = s.off[sl], w = (s.off[sl + 1] - o) >> 6;
+            const int *ip = s.sidx + o + lane;
+            const double *vp = s.val64 + o + lane;
+            constexpr int U = 4;
+            for (int j = 0; j < w; j += U) {
+                int c[U];
+                double v[U];
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    const bool in = j + u < w;
+                    c[u] = in ? __builtin_nontemporal_load(ip + 64 * u) : 0;
+                    v[u] = in ? __builtin_nontemporal_load(vp + 64 * u) : 0.0;
+                }
+                Pack<double, R> xc[U];
+#pragma unroll
+                for (int u = 0; u < U; ++u)
+                    if (j + u < w) xc[u] = ld<double, R>(s.x, c[u]);
+#pragma unroll
+                for (int u = 0; u < U; ++u)
+                    if (j + u < w)
+#pragma unroll
+                        for (int r = 0; r < R; ++r) acc[r] += v[u] * xc[u].v[r];
+                ip += 64 * U;
+                vp += 64 * U;
+            }
+        }
+        const int row = sl * 64 + lane;
+        if (row < s.n) {
+            const Pack<double, R> bi = ld<double, R>(s.b, row);
+            const double di = s.dinv64[row];
+            Pack<double, R> r64;
+            Pack<float, R> r32;
+            if (s.prec) r64 = ld<double, R>(s.r64, row);
+            else r32 = ld<float, R>(s.r, row);
+            bool w64 = false;
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                const int stt = sv.state[r];
+                const double ri = bi.v[r] - acc[r];
+                if (stt == kActive || stt == kCandidate || stt == kFresh) {
+                    // recurrence vs FP64 residual; an FP64 recurrence is kept
+                    // as it is (the reference's iteration), only started from b
+                    const double dr = (s.prec ? r64.v[r] : (double)r32.v[r]) - ri;
+                    cdd[r] = dr * dr;
+                    if (!s.prec) {
+                        r32.v[r] = (float)ri;
+                    } else if (stt == kFresh || sv.restart[r] == 2) {
+                        r64.v[r] = ri;
+                        w64 = true;
+                    }
+                }
+                crr[r] = ri * ri;
+                crz[r] = ri * (di * ri);
+                cbb[r] = bi.v[r] * bi.v[r];
+            }
+            if (!s.prec) st<float, R>(s.r, row, r32);
+            else if (w64) st<double, R>(s.r64, row, r64);
+        }
+    }
+    block_sums<R>(crr, sh, o_rr);
+    block_sums<R>(crz, sh, o_rz);
+    block_sums<R>(cbb, sh, o_bb);
+    block_sums<R>(cdd, sh, o_dd);
+}
+
 __global__ __launch_bounds__(256) void k_b_resid(const int2 *__restrict__ work,
                                                  const BatchSys *__restrict__ sys,
                                                  const BatchScal *__restrict__ sc,
-                                                 double *__restrict__ res_scratch,
                                                  double *__restrict__ part_rr,
                                                  double *__restrict__ part_rz,
                                                  double *__restrict__ part_bb,
                                                  double *__restrict__ part_dd)
 {
-    __shared__ double sh[256];
+    __shared__ double sh[4 * kMaxRhs];
+    __shared__ ScalView sv;
     const int2 wk = work[blockIdx.x];
     const BatchSys s = sys[wk.x];
-    const int R = s.R;
-    const int row0 = wk.y, row1 = min(wk.y + kRowsPerItem, s.n);
-    // A x in FP64 into the system's own slice of the scratch (n x R)
-    double *ax = res_scratch + (size_t)s.w0 * kRowsPerItem * kMaxRhs;
-    double dummy = 0.0;
-    switch (R) {
-    case 1: chunk_spmv<1, double, double, double>(s.ptr, s.idx, s.val64, s.x, row0, row1, ax, dummy, (const double *)nullptr); break;
-    case 2: chunk_spmv<2, double, double, double>(s.ptr, s.idx, s.val64, s.x, row0, row1, ax, dummy, (const double *)nullptr); break;
-    case 4: chunk_spmv<4, double, double, double>(s.ptr, s.idx, s.val64, s.x, row0, row1, ax, dummy, (const double *)nullptr); break;
-    default: chunk_spmv<8, double, double, double>(s.ptr, s.idx, s.val64, s.x, row0, row1, ax, dummy, (const double *)nullptr); break;
+    load_scal(sc, s.slot, s.R, sv);
+    bool live = false, fresh = true;
+    for (int r = 0; r < s.R; ++r) {
+        const int stt = sv.state[r];
+        live = live || stt == kActive || stt == kCandidate || stt == kFresh;
+        fresh = fresh && (stt == kFresh || stt == kUnused);
     }
-    __syncthreads();
-    const int r = threadIdx.x % R;
-    const BatchScal q = sc[s.slot * kMaxRhs + r];
-    double crr = 0.0, crz = 0.0, cbb = 0.0, cdd = 0.0;
-    const bool live = q.state == kActive || q.state == kCandidate || q.state == kFresh;
-    for (int e = threadIdx.x; e < (row1 - row0) * R; e += 256) {
-        const size_t i = (size_t)row0 * R + e;
-        const int row = row0 + e / R;
-        const double bi = s.b[i];
-        if (q.state == kFresh) s.x[i] = 0.0;
-        const double ri = q.state == kFresh ? bi : bi - ax[i];
-        if (live) {
-            // recurrence vs FP64 residual; an FP64 recurrence is kept as it
-            // is (the reference's iteration), only started from b
-            const double drift = (s.prec ? s.r64[i] : (double)s.r[i]) - ri;
-            cdd += drift * drift;
-            if (!s.prec)
-                s.r[i] = (float)ri;
-            else if (q.state == kFresh || q.restart == 2)
-                s.r64[i] = ri;
-        }
-        crr += ri * ri;
-        crz += ri * s.dinv[row] * ri;
-        cbb += bi * bi;
-    }
-    const double trr = rhs_sum(crr, R, sh);
-    const double trz = rhs_sum(crz, R, sh);
-    const double tbb = rhs_sum(cbb, R, sh);
-    const double tdd = rhs_sum(cdd, R, sh);
-    if (threadIdx.x < R) {
-        part_rr[(size_t)blockIdx.x * kMaxRhs + threadIdx.x] = trr;
-        part_rz[(size_t)blockIdx.x * kMaxRhs + threadIdx.x] = trz;
-        part_bb[(size_t)blockIdx.x * kMaxRhs + threadIdx.x] = tbb;
-        part_dd[(size_t)blockIdx.x * kMaxRhs + threadIdx.x] = tdd;
+    if (!live) return;
+    const size_t o = (size_t)blockIdx.x * kMaxRhs;
+    const int slice0 = wk.y / 64;
+    switch (s.R) {
+    case 1: resid_item<1>(s, slice0, sv, fresh, sh, part_rr + o, part_rz + o, part_bb + o, part_dd + o); break;
+    case 2: resid_item<2>(s, slice0, sv, fresh, sh, part_rr + o, part_rz + o, part_bb + o, part_dd + o); break;
+    case 4: resid_item<4>(s, slice0, sv, fresh, sh, part_rr + o, part_rz + o, part_bb + o, part_dd + o); break;
+    default: resid_item<8>(s, slice0, sv, fresh, sh, part_rr + o, part_rz + o, part_bb + o, part_dd + o); break;
     }
 }
 
@@ -369,79 +550,74 @@ __global__ __launch_bounds__(256) void k_b_refresh(const BatchSys *__restrict__ 
                                                    const double *__restrict__ part_bb,
                                                    const double *__restrict__ part_dd)
 {
-    __shared__ double sh[4];
+    __shared__ double sh[4 * kMaxRhs * 4];
     const BatchSys s = sys[blockIdx.x];
-    for (int r = 0; r < s.R; ++r) {
-        double trr = 0.0, trz = 0.0, tbb = 0.0, tdd = 0.0;
-        for (int w = s.w0 + threadIdx.x; w < s.w1; w += 256) {
-            trr += part_rr[(size_t)w * kMaxRhs + r];
-            trz += part_rz[(size_t)w * kMaxRhs + r];
-            tbb += part_bb[(size_t)w * kMaxRhs + r];
-            tdd += part_dd[(size_t)w * kMaxRhs + r];
-        }
-        trr = bsum256(trr, sh);
-        trz = bsum256(trz, sh);
-        tbb = bsum256(tbb, sh);
-        tdd = bsum256(tdd, sh);
-        if (threadIdx.x == 0) {
-            BatchScal &q = sc[s.slot * kMaxRhs + r];
-            const bool broke = q.restart == 2;
-            const bool done_by_recurrence = q.state == kCandidate && q.restart == 3;
-            q.restart = 0;
-            if (q.state == kFresh) {
-                q.bb = tbb;
-                q.tol2 = q.tol2 * tbb; // tol2 holds tol^2 until here; ||r0|| = ||b|| (x0 = 0)
-                q.iters = 0;
-                q.state = tbb == 0.0 ? kConverged : kActive;
-                q.rz = trz;
-                q.rr = trr;
-                q.true_rr = trr;
-                q.restart = q.state == kActive;
-            } else if (q.state == kActive || q.state == kCandidate) {
-                q.true_rr = trr;
-                q.rr = trr;
-                if (trr <= q.tol2 || done_by_recurrence) {
-                    q.state = kConverged;
-                } else if (q.iters >= q.max_iter) {
-                    q.state = kStopped;
-                } else if (s.prec) {
-                    // FP64 iteration: the recurrence continues untouched
-                    // (after a breakdown it restarts from the FP64 residual)
-                    q.state = kActive;
-                    if (broke) {
-                        q.restart = 1;
-                        q.rz = trz;
-                    }
-                } else {
-                    // FP32 iteration, reliable update: r is now the FP64
-                    // residual.  The direction is kept while the recurrence
-                    // tracked it (drift under 1 % of the residual); after a
-                    // breakdown or a larger drift CG restarts from r.
-                    q.restart = broke || tdd > 1.0e-4 * trr;
-                    q.state = kActive;
+    const double *const parts[4] = {part_rr, part_rz, part_bb, part_dd};
+    double tot[4];
+    system_totals<4>(s, parts, tot, sh);
+    if (threadIdx.x < s.R) {
+        BatchScal &q = sc[s.slot * kMaxRhs + threadIdx.x];
+        const double trr = tot[0], trz = tot[1], tbb = tot[2], tdd = tot[3];
+        const bool broke = q.restart == 2;
+        const bool done_by_recurrence = q.state == kCandidate && q.restart == 3;
+        q.restart = 0;
+        if (q.state == kFresh) {
+            // compute_solver_tolerance (:1063-1073): tol2 holds tol^2 until
+            // here; relative means tol ||b - A x0||
+            q.bb = tbb;
+            q.init_rr = trr;
+            if (!(q.flags & kAbsTol)) q.tol2 = q.tol2 * trr;
+            q.iters = 0;
+            q.state = trr <= q.tol2 ? kConverged : kActive;
+            q.rz = trz;
+            q.rr = trr;
+            q.true_rr = trr;
+            q.restart = q.state == kActive;
+        } else if (q.state == kActive || q.state == kCandidate) {
+            q.true_rr = trr;
+            q.rr = trr;
+            if (trr <= q.tol2 || done_by_recurrence) {
+                q.state = kConverged;
+            } else if (q.iters >= q.max_iter) {
+                q.state = kStopped;
+            } else if (s.prec) {
+                // FP64 iteration: the recurrence continues untouched
+                // (after a breakdown it restarts from the FP64 residual)
+                q.state = kActive;
+                if (broke) {
+                    q.restart = 1;
                     q.rz = trz;
                 }
+            } else {
+                // FP32 iteration, reliable update: r is now the FP64
+                // residual.  The direction is kept while the recurrence
+                // tracked it (drift under 1 % of the residual); after a
+                // breakdown or a larger drift CG restarts from r.
+                q.restart = broke || tdd > 1.0e-4 * trr;
+                q.state = kActive;
+                q.rz = trz;
             }
         }
     }
 }
 
-// Jacobi diagonal in FP64 (|a_ii| > 1e-14, else 1: advanced_solvers.f90:1124-1155)
+// Jacobi diagonal in FP64 and FP32 (|a_ii| > 1e-14, else 1: advanced_solvers.f90:1124-1155);
+// use = 0 gives the identity (unpreconditioned CG)
 __global__ void k_b_jacobi(int n, const int *__restrict__ ptr, const int *__restrict__ idx,
-                           const double *__restrict__ val, double *__restrict__ dinv)
+                           const double *__restrict__ val, int use, double *__restrict__ dinv,
+                           float *__restrict__ dinv32)
 {
     const int i = blockIdx.x * 256 + threadIdx.x;
     if (i >= n) return;
     double d = 0.0;
     for (int k = ptr[i]; k < ptr[i + 1]; ++k)
-        if (idx[k] == i) d = val[k];
-    dinv[i] = fabs(d) > 1.0e-14 ? 1.0 / d : 1.0;
-}
-
-__global__ void k_b_to_f32(int64_t n, const double *__restrict__ a, float *__restrict__ b)
-{
-    const int64_t i = blockIdx.x * 256ll + threadIdx.x;
-    if (i < n) b[i] = (float)a[i];
+        if (idx[k] == i) {
+            d = val[k];
+            break;
+        }
+    const double di = use && fabs(d) > 1.0e-14 ? 1.0 / d : 1.0;
+    dinv[i] = di;
+    dinv32[i] = (float)di;
 }
 
 // b (n x nrhs, column-major host layout) -> n x R interleaved, zero padding
@@ -454,6 +630,138 @@ __global__ void k_b_interleave(int n, int nrhs, int R, const double *__restrict_
     dst[i] = r < nrhs ? src[(size_t)r * n + row] : 0.0;
 }
 
+// ----------------------------------------------------------------- engine --
+// Device state of one batch (the server's, or one mixed-precision solve's):
+// the systems, the work table, the per-(slot, right-hand side) scalars with
+// their host mirror, and the per-item partial sums.
+struct BatchEngine {
+    DBuf<BatchSys> d_sys;
+    DBuf<int2> d_work;
+    DBuf<BatchScal> d_scal;
+    DBuf<double> d_p0, d_p1, d_p2, d_p3;
+    std::vector<BatchScal> h_scal;
+    int nwork = 0, nsys = 0;
+    // a chunk recorded as a HIP graph (replayed while the batch is unchanged)
+    hipGraphExec_t graph = nullptr;
+    int graph_chunk = -1;
+
+    ~BatchEngine() { drop_graph(); }
+
+    void drop_graph()
+    {
+        if (graph) (void)hipGraphExecDestroy(graph);
+        graph = nullptr;
+        graph_chunk = -1;
+    }
+
+    void init(int slots)
+    {
+        h_scal.assign((size_t)slots * kMaxRhs, BatchScal{});
+        d_scal.alloc(h_scal.size());
+    }
+
+    // append a system's work items (kRowsPerItem rows each) and the system
+    static void append(BatchSys b, std::vector<BatchSys> &sys, std::vector<int2> &work)
+    {
+        b.w0 = (int)work.size();
+        for (int row = 0; row < b.n; row += kRowsPerItem)
+            work.push_back(make_int2((int)sys.size(), row));
+        b.w1 = (int)work.size();
+        sys.push_back(b);
+    }
+
+    std::vector<BatchSys> h_sys; // what d_sys / d_work hold
+    std::vector<int2> h_work;
+
+    void set(Context *ctx, const std::vector<BatchSys> &sys, const std::vector<int2> &work)
+    {
+        hipStream_t s = ctx->stream;
+        if (d_sys.p && sys.size() == h_sys.size() && work.size() == h_work.size() &&
+            std::memcmp(sys.data(), h_sys.data(), sizeof(BatchSys) * sys.size()) == 0 &&
+            std::memcmp(work.data(), h_work.data(), sizeof(int2) * work.size()) == 0)
+            return; // unchanged: the recorded graph stays valid
+        h_sys = sys;
+        h_work = work;
+        d_sys.upload(sys, s);
+        d_work.upload(work, s);
+        const size_t np = std::max<size_t>(1, work.size() * kMaxRhs);
+        if (d_p0.n < np) {
+            d_p0.alloc(np);
+            d_p1.alloc(np);
+            d_p2.alloc(np);
+            d_p3.alloc(np);
+        }
+        nwork = (int)work.size();
+        nsys = (int)sys.size();
+        drop_graph();
+        FFH_HIP(hipStreamSynchronize(s));
+    }
+
+    // run() from a HIP graph: recorded on first use after set() (or a new
+    // chunk length), then replayed; profiling runs launch directly
+    void run_graph(Context *ctx, int chunk)
+    {
+        if (ctx->profiling) {
+            run(ctx, chunk);
+            return;
+        }
+        if (!graph || graph_chunk != chunk) {
+            drop_graph();
+            hipGraph_t g;
+            FFH_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+            run(ctx, chunk);
+            FFH_HIP(hipStreamEndCapture(ctx->stream, &g));
+            const hipError_t e = hipGraphInstantiate(&graph, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            FFH_HIP(e);
+            graph_chunk = chunk;
+        }
+        FFH_HIP(hipGraphLaunch(graph, ctx->stream));
+    }
+
+    // `chunk` PCG iterations of every system with an active right-hand side,
+    // then the FP64 residual refresh (which also starts fresh systems)
+    void run(Context *ctx, int chunk)
+    {
+        hipStream_t s = ctx->stream;
+        const int W = nwork, NS = nsys;
+        if (W == 0 || NS == 0) return;
+        for (int it = 0; it < chunk; ++it) {
+            if (ctx->profiling) ctx->prof_mark(1, true);
+            hipLaunchKernelGGL(k_b_spmv, dim3(W), dim3(256), 0, s, d_work.p, d_sys.p, d_scal.p,
+                               d_p0.p);
+            if (ctx->profiling) ctx->prof_mark(1, false);
+            hipLaunchKernelGGL(k_b_alpha, dim3(NS), dim3(256), 0, s, d_sys.p, d_scal.p, d_p0.p);
+            hipLaunchKernelGGL(k_b_update, dim3(W), dim3(256), 0, s, d_work.p, d_sys.p, d_scal.p,
+                               d_p0.p, d_p1.p);
+            hipLaunchKernelGGL(k_b_beta, dim3(NS), dim3(256), 0, s, d_sys.p, d_scal.p, d_p0.p,
+                               d_p1.p);
+            hipLaunchKernelGGL(k_b_dir, dim3(W), dim3(256), 0, s, d_work.p, d_sys.p, d_scal.p, 0);
+        }
+        if (ctx->profiling) ctx->prof_mark(0, true);
+        hipLaunchKernelGGL(k_b_resid, dim3(W), dim3(256), 0, s, d_work.p, d_sys.p, d_scal.p,
+                           d_p0.p, d_p1.p, d_p2.p, d_p3.p);
+        if (ctx->profiling) ctx->prof_mark(0, false);
+        hipLaunchKernelGGL(k_b_refresh, dim3(NS), dim3(256), 0, s, d_sys.p, d_scal.p, d_p0.p,
+                           d_p1.p, d_p2.p, d_p3.p);
+        hipLaunchKernelGGL(k_b_dir, dim3(W), dim3(256), 0, s, d_work.p, d_sys.p, d_scal.p, 1);
+        FFH_HIP(hipGetLastError());
+    }
+
+    void pull(Context *ctx)
+    {
+        FFH_HIP(hipMemcpyAsync(h_scal.data(), d_scal.p, sizeof(BatchScal) * h_scal.size(),
+                               hipMemcpyDeviceToHost, ctx->stream));
+        FFH_HIP(hipStreamSynchronize(ctx->stream));
+    }
+
+    void push(Context *ctx, int slot)
+    {
+        FFH_HIP(hipMemcpyAsync(d_scal.p + (size_t)slot * kMaxRhs, h_scal.data() + (size_t)slot * kMaxRhs,
+                               sizeof(BatchScal) * kMaxRhs, hipMemcpyHostToDevice, ctx->stream));
+    }
+};
+
 // ----------------------------------------------------------------- server --
 struct ServedRequest {
     int64_t ticket = 0;
@@ -464,7 +772,7 @@ struct ServedRequest {
     double mass = 0.0, tol = 1e-10;
     int max_iter = 100000;
     size_t bytes = 0;
-    DBuf<float> val32, r, p, ap;
+    DBuf<float> val32, dinv32, r, p, ap; // SELL-64 values, FP32 iteration
     DBuf<double> val64, dinv, b, x, r64, p64, ap64;
     int family = 0;
     int prec = 0;                 // 0 FP32 iteration, 1 FP64 iteration
@@ -500,23 +808,18 @@ struct ffh_server {
     std::vector<int> free_slots;
     bool changed = false;
     int64_t next_ticket = 1;
-    // device batch state
-    ffh::DBuf<ffh::BatchSys> d_sys;
-    ffh::DBuf<int2> d_work;
-    ffh::DBuf<ffh::BatchScal> d_scal;
-    ffh::DBuf<double> d_p0, d_p1, d_p2, d_p3, d_scratch;
-    std::vector<ffh::BatchScal> h_scal;
-    int nwork = 0;
+    ffh::BatchEngine eng; // device batch state
     int64_t iterations_run = 0;
 };
 
 namespace ffh {
 
-// worst-case device bytes of a request: both value copies, b and x, and the
-// FP32 and FP64 iteration vectors (an FP32 request may switch to FP64)
-static size_t request_bytes(int n, int64_t nnz, int R)
+// worst-case device bytes of a request: both SELL value copies, both
+// diagonals, b and x, and the FP32 and FP64 iteration vectors (an FP32
+// request may switch to FP64)
+static size_t request_bytes(int n, int64_t sell_total, int R)
 {
-    return (size_t)nnz * 12 + (size_t)n * 8 + (size_t)n * R * (8 + 8 + 12 + 24) + 4096;
+    return (size_t)sell_total * 12 + (size_t)n * 12 + (size_t)n * R * (8 + 8 + 12 + 24) + 4096;
 }
 
 static void alloc_fp64_vectors(ServedRequest *q, hipStream_t s)
@@ -550,6 +853,8 @@ Problem *server_prepare(ffh_server *S, int mesh_key, int family, int order, int 
     P->d_vertices.upload(P->vertices, S->ctx->stream);
     problem_pattern(P.get(), family, order);
     problem_dirichlet(P.get(), 0, nullptr);
+    FFH_REQUIRE(sell_structure(S->ctx, P->reduced.m), FFH_ERR_INVALID,
+                "the reduced matrix has no SELL-64 form (rows too uneven)");
     Problem *raw = P.get();
     S->cache[key] = std::move(P);
     return raw;
@@ -566,14 +871,14 @@ static void admit(ffh_server *S, ServedRequest *q)
     problem_reduce(P);
     const DevMatrix &A = P->reduced.m;
     const int n = q->n;
-    q->val64.alloc(std::max<int64_t>(q->nnz, 1));
-    FFH_HIP(hipMemcpyAsync(q->val64.p, A.val.p, sizeof(double) * q->nnz, hipMemcpyDeviceToDevice, s));
-    q->val32.alloc(std::max<int64_t>(q->nnz, 1));
-    hipLaunchKernelGGL(k_b_to_f32, dim3((unsigned)((q->nnz + 255) / 256)), dim3(256), 0, s, q->nnz,
-                       q->val64.p, q->val32.p);
+    const size_t tot = (size_t)std::max<int64_t>(A.sell_total, 1);
+    q->val64.alloc(tot);
+    q->val32.alloc(tot);
+    sell_values(ctx, A, A.val.p, q->val64.p, q->val32.p);
     q->dinv.alloc(n);
+    q->dinv32.alloc(n);
     hipLaunchKernelGGL(k_b_jacobi, dim3((n + 255) / 256), dim3(256), 0, s, n, A.col_ptr.p,
-                       A.row_idx.p, q->val64.p, q->dinv.p);
+                       A.row_idx.p, A.val.p, 1, q->dinv.p, q->dinv32.p);
     const size_t nr = (size_t)n * q->R;
     DBuf<double> tmp;
     tmp.upload(q->rhs.data(), q->rhs.size(), s);
@@ -599,15 +904,13 @@ static void admit(ffh_server *S, ServedRequest *q)
     q->slot = S->free_slots.back();
     S->free_slots.pop_back();
     for (int r = 0; r < kMaxRhs; ++r) {
-        BatchScal &c = S->h_scal[(size_t)q->slot * kMaxRhs + r];
+        BatchScal &c = S->eng.h_scal[(size_t)q->slot * kMaxRhs + r];
         c = BatchScal{};
         c.state = r < q->nrhs ? kFresh : kUnused;
         c.max_iter = q->max_iter;
         c.tol2 = q->tol * q->tol;
     }
-    FFH_HIP(hipMemcpyAsync(S->d_scal.p + (size_t)q->slot * kMaxRhs,
-                           S->h_scal.data() + (size_t)q->slot * kMaxRhs,
-                           sizeof(BatchScal) * kMaxRhs, hipMemcpyHostToDevice, s));
+    S->eng.push(ctx, q->slot);
     S->used += q->bytes;
     S->active.push_back(q);
     S->changed = true;
@@ -617,15 +920,16 @@ static void rebuild_batch(ffh_server *S)
 {
     std::vector<BatchSys> sys;
     std::vector<int2> work;
-    size_t scratch = 0;
     for (ServedRequest *q : S->active) {
         const DevMatrix &A = q->sym->reduced.m;
         BatchSys b{};
-        b.ptr = A.col_ptr.p;
-        b.idx = A.row_idx.p;
+        b.off = A.sell_off.p;
+        b.sidx = A.sell_idx.p;
+        b.nslice = A.nslice;
         b.val32 = q->val32.p;
         b.val64 = q->val64.p;
-        b.dinv = q->dinv.p;
+        b.dinv32 = q->dinv32.p;
+        b.dinv64 = q->dinv.p;
         b.b = q->b.p;
         b.x = q->x.p;
         b.r = q->r.p;
@@ -638,27 +942,10 @@ static void rebuild_batch(ffh_server *S)
         b.n = q->n;
         b.R = q->R;
         b.slot = q->slot;
-        b.w0 = (int)work.size();
-        for (int row = 0; row < q->n; row += kRowsPerItem)
-            work.push_back(make_int2((int)sys.size(), row));
-        b.w1 = (int)work.size();
-        sys.push_back(b);
+        b.x_zero = 1;
+        BatchEngine::append(b, sys, work);
     }
-    // FP64 A x scratch: a system's slice starts at w0 * kRowsPerItem * kMaxRhs
-    scratch = work.size() * (size_t)kRowsPerItem * kMaxRhs;
-    hipStream_t s = S->ctx->stream;
-    S->d_sys.upload(sys, s);
-    S->d_work.upload(work, s);
-    const size_t np = std::max<size_t>(1, work.size() * kMaxRhs);
-    if (S->d_p0.n < np) {
-        S->d_p0.alloc(np);
-        S->d_p1.alloc(np);
-        S->d_p2.alloc(np);
-        S->d_p3.alloc(np);
-    }
-    if (S->d_scratch.n < scratch) S->d_scratch.alloc(std::max<size_t>(scratch, 1));
-    S->nwork = (int)work.size();
-    FFH_HIP(hipStreamSynchronize(s));
+    S->eng.set(S->ctx, sys, work);
     S->changed = false;
 }
 
@@ -667,25 +954,8 @@ static void rebuild_batch(ffh_server *S)
 // last step: they sit out the iterations of their first step).
 static void run_chunk(ffh_server *S)
 {
-    hipStream_t s = S->ctx->stream;
-    const int W = S->nwork, NS = (int)S->active.size();
-    if (W == 0 || NS == 0) return;
-    for (int it = 0; it < S->chunk; ++it) {
-        hipLaunchKernelGGL(k_b_spmv, dim3(W), dim3(256), 0, s, S->d_work.p, S->d_sys.p, S->d_scal.p,
-                           S->d_p0.p);
-        hipLaunchKernelGGL(k_b_alpha, dim3(NS), dim3(256), 0, s, S->d_sys.p, S->d_scal.p, S->d_p0.p);
-        hipLaunchKernelGGL(k_b_update, dim3(W), dim3(256), 0, s, S->d_work.p, S->d_sys.p, S->d_scal.p,
-                           S->d_p0.p, S->d_p1.p);
-        hipLaunchKernelGGL(k_b_beta, dim3(NS), dim3(256), 0, s, S->d_sys.p, S->d_scal.p, S->d_p0.p,
-                           S->d_p1.p);
-        hipLaunchKernelGGL(k_b_dir, dim3(W), dim3(256), 0, s, S->d_work.p, S->d_sys.p, S->d_scal.p, 0);
-    }
-    hipLaunchKernelGGL(k_b_resid, dim3(W), dim3(256), 0, s, S->d_work.p, S->d_sys.p, S->d_scal.p,
-                       S->d_scratch.p, S->d_p0.p, S->d_p1.p, S->d_p2.p, S->d_p3.p);
-    hipLaunchKernelGGL(k_b_refresh, dim3(NS), dim3(256), 0, s, S->d_sys.p, S->d_scal.p, S->d_p0.p,
-                       S->d_p1.p, S->d_p2.p, S->d_p3.p);
-    hipLaunchKernelGGL(k_b_dir, dim3(W), dim3(256), 0, s, S->d_work.p, S->d_sys.p, S->d_scal.p, 1);
-    FFH_HIP(hipGetLastError());
+    if (S->eng.nwork == 0 || S->active.empty()) return;
+    S->eng.run(S->ctx, S->chunk);
     S->iterations_run += S->chunk;
 }
 
@@ -696,7 +966,7 @@ static void watch_stagnation(ffh_server *S, ServedRequest *q)
 {
     bool stuck = false;
     for (int r = 0; r < q->nrhs; ++r) {
-        const BatchScal &c = S->h_scal[(size_t)q->slot * kMaxRhs + r];
+        const BatchScal &c = S->eng.h_scal[(size_t)q->slot * kMaxRhs + r];
         if (c.state != kActive) continue;
         if (c.true_rr < 0.25 * q->best_rr[r]) {
             q->best_rr[r] = c.true_rr;
@@ -710,15 +980,13 @@ static void watch_stagnation(ffh_server *S, ServedRequest *q)
     alloc_fp64_vectors(q, s);
     q->prec = 1;
     for (int r = 0; r < q->nrhs; ++r) {
-        BatchScal &c = S->h_scal[(size_t)q->slot * kMaxRhs + r];
+        BatchScal &c = S->eng.h_scal[(size_t)q->slot * kMaxRhs + r];
         if (c.state == kActive || c.state == kCandidate) {
             c.state = kCandidate;
             c.restart = 2;
         }
     }
-    FFH_HIP(hipMemcpyAsync(S->d_scal.p + (size_t)q->slot * kMaxRhs,
-                           S->h_scal.data() + (size_t)q->slot * kMaxRhs, sizeof(BatchScal) * kMaxRhs,
-                           hipMemcpyHostToDevice, s));
+    S->eng.push(S->ctx, q->slot);
     FFH_HIP(hipStreamSynchronize(s));
     q->r.reset(), q->p.reset(), q->ap.reset();
     S->changed = true;
@@ -728,14 +996,12 @@ static void watch_stagnation(ffh_server *S, ServedRequest *q)
 static void retire_finished(ffh_server *S, int *finished)
 {
     hipStream_t s = S->ctx->stream;
-    FFH_HIP(hipMemcpyAsync(S->h_scal.data(), S->d_scal.p, sizeof(BatchScal) * S->h_scal.size(),
-                           hipMemcpyDeviceToHost, s));
-    FFH_HIP(hipStreamSynchronize(s));
+    S->eng.pull(S->ctx);
     std::vector<ServedRequest *> keep;
     for (ServedRequest *q : S->active) {
         bool all = true;
         for (int r = 0; r < q->nrhs; ++r) {
-            const int st = S->h_scal[(size_t)q->slot * kMaxRhs + r].state;
+            const int st = S->eng.h_scal[(size_t)q->slot * kMaxRhs + r].state;
             all = all && (st == kConverged || st == kStopped);
         }
         if (!all) {
@@ -754,12 +1020,12 @@ static void retire_finished(ffh_server *S, int *finished)
         q->converged.resize(q->nrhs);
         q->resid.resize(q->nrhs);
         for (int r = 0; r < q->nrhs; ++r) {
-            const BatchScal &c = S->h_scal[(size_t)q->slot * kMaxRhs + r];
+            const BatchScal &c = S->eng.h_scal[(size_t)q->slot * kMaxRhs + r];
             q->iters[r] = c.iters;
             q->converged[r] = c.state == kConverged;
             q->resid[r] = c.bb > 0 ? std::sqrt(c.true_rr / c.bb) : 0.0;
         }
-        q->val32.reset(), q->r.reset(), q->p.reset(), q->ap.reset();
+        q->val32.reset(), q->dinv32.reset(), q->r.reset(), q->p.reset(), q->ap.reset();
         q->val64.reset(), q->dinv.reset(), q->b.reset(), q->x.reset();
         q->r64.reset(), q->p64.reset(), q->ap64.reset();
         q->rhs.clear();
@@ -815,13 +1081,143 @@ int64_t server_submit(ffh_server *S, int mesh_key, int family, int order, const 
     q->rhs.assign(rhs, rhs + (size_t)q->n * nrhs);
     q->tol = tol;
     q->max_iter = max_iter;
-    q->bytes = request_bytes(q->n, q->nnz, q->R);
+    q->bytes = request_bytes(q->n, P->reduced.m.sell_total, q->R);
     FFH_REQUIRE(q->bytes <= S->budget, FFH_ERR_ADMISSION,
                 "request needs more device memory than the server budget");
     const int64_t t = q->ticket;
     S->queue.push_back(q.get());
     S->reqs[t] = std::move(q);
     return t;
+}
+
+// ------------------------------------------------------ one mixed solve --
+// FFH_MIXED of solve_sparse / problem_solve: the batch engine with one system
+// and one right-hand side.  SELL FP32 matrix and FP32 r, p, Ap, D^-1; FP64 x
+// and reductions; after every chunk the residual b - A x is recomputed in
+// FP64 with the FP64 matrix and replaces r (reliable update), and it alone
+// decides convergence: ||b - A x|| <= tol ||b - A x0|| (relative) or <= tol
+// (absolute).  A solve whose FP64 residual stops improving (16 refreshes
+// without a 4x drop) continues with FP64 iterations from its current x.
+struct SoloState {
+    BatchEngine eng;
+    int n = -1;
+    DBuf<double> dinv, b, x, r64, p64, ap64;
+    DBuf<float> dinv32, r, p, ap;
+};
+
+void free_solo(SoloState *s) { delete s; }
+
+void solve_mixed(Context *ctx, Matrix &M, const double *b, double *x, const ffh_solver_options &o,
+                 ffh_solver_stats &st)
+{
+    DevMatrix &A = M.m;
+    ensure_val64(ctx, A);
+    FFH_REQUIRE(sell_structure(ctx, A), FFH_ERR_INVALID,
+                "mixed precision needs the SELL-64 form (matrix rows too uneven)");
+    ensure_sell(ctx, A, false);
+    ensure_sell(ctx, A, true);
+    Workspace &w = M.ws;
+    if (!w.solo) w.solo = new SoloState();
+    SoloState &S = *w.solo;
+    const int n = A.n;
+    hipStream_t s = ctx->stream;
+    if (S.n != n) {
+        S.dinv.alloc(n), S.b.alloc(n), S.x.alloc(n);
+        S.dinv32.alloc(n), S.r.alloc(n), S.p.alloc(n), S.ap.alloc(n);
+        S.r64.reset(), S.p64.reset(), S.ap64.reset();
+        S.eng.init(1);
+        S.n = n;
+    }
+    if (ctx->profiling) ctx->prof_begin();
+    FFH_HIP(hipEventRecord(ctx->ev_start, s));
+    const int use = o.method == 1 && o.preconditioner == 1;
+    hipLaunchKernelGGL(k_b_jacobi, dim3((n + 255) / 256), dim3(256), 0, s, n, A.gptr(), A.gidx(),
+                       A.gval(), use, S.dinv.p, S.dinv32.p);
+    FFH_HIP(hipGetLastError());
+    FFH_HIP(hipMemcpyAsync(S.b.p, b, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+    FFH_HIP(hipMemcpyAsync(S.x.p, x, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+    FFH_HIP(hipMemsetAsync(S.r.p, 0, sizeof(float) * n, s));
+    FFH_HIP(hipMemsetAsync(S.p.p, 0, sizeof(float) * n, s));
+    for (int r = 0; r < kMaxRhs; ++r) {
+        BatchScal &c = S.eng.h_scal[r];
+        c = BatchScal{};
+        c.state = r == 0 ? kFresh : kUnused;
+        c.max_iter = o.max_iterations;
+        c.tol2 = o.tolerance * o.tolerance;
+        c.flags = o.tolerance_type == 1 ? kAbsTol : 0;
+    }
+    S.eng.push(ctx, 0);
+    auto set_system = [&](int prec) {
+        BatchSys bs{};
+        bs.off = A.sell_off.p;
+        bs.sidx = A.sell_idx.p;
+        bs.nslice = A.nslice;
+        bs.val32 = A.sell_val32.p;
+        bs.val64 = A.sell_val.p;
+        bs.dinv32 = S.dinv32.p;
+        bs.dinv64 = S.dinv.p;
+        bs.b = S.b.p;
+        bs.x = S.x.p;
+        bs.r = S.r.p;
+        bs.p = S.p.p;
+        bs.ap = S.ap.p;
+        bs.r64 = S.r64.p;
+        bs.p64 = S.p64.p;
+        bs.ap64 = S.ap64.p;
+        bs.prec = prec;
+        bs.n = n;
+        bs.R = 1;
+        bs.slot = 0;
+        bs.x_zero = 0;
+        std::vector<BatchSys> sys;
+        std::vector<int2> work;
+        BatchEngine::append(bs, sys, work);
+        S.eng.set(ctx, sys, work);
+    };
+    set_system(0);
+    const int chunk = o.check_interval > 0 ? o.check_interval : 16;
+    int prec = 0, refreshes = 0, stall = 0;
+    double best = 1e300;
+    BatchScal q{};
+    for (;;) {
+        if (o.use_graph)
+            S.eng.run_graph(ctx, chunk);
+        else
+            S.eng.run(ctx, chunk);
+        ++refreshes;
+        S.eng.pull(ctx);
+        q = S.eng.h_scal[0];
+        if (q.state == kConverged || q.state == kStopped) break;
+        if (prec == 0 && q.state == kActive) {
+            if (q.true_rr < 0.25 * best) {
+                best = q.true_rr;
+                stall = 0;
+            } else if (++stall >= 16) {
+                // FP32 iterations stopped making progress: continue in FP64
+                S.r64.alloc(n), S.p64.alloc(n), S.ap64.alloc(n);
+                FFH_HIP(hipMemsetAsync(S.r64.p, 0, sizeof(double) * n, s));
+                FFH_HIP(hipMemsetAsync(S.p64.p, 0, sizeof(double) * n, s));
+                prec = 1;
+                S.eng.h_scal[0].state = kCandidate;
+                S.eng.h_scal[0].restart = 2;
+                S.eng.push(ctx, 0);
+                set_system(1);
+            }
+        }
+    }
+    FFH_HIP(hipMemcpyAsync(x, S.x.p, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+    FFH_HIP(hipEventRecord(ctx->ev_stop, s));
+    FFH_HIP(hipStreamSynchronize(s));
+    float ms = 0.0f;
+    FFH_HIP(hipEventElapsedTime(&ms, ctx->ev_start, ctx->ev_stop));
+    if (ctx->profiling) ctx->prof_end();
+    st.converged = q.state == kConverged;
+    st.iterations = q.iters;
+    st.final_residual = std::sqrt(q.true_rr);
+    st.true_residual = std::sqrt(q.true_rr);
+    st.initial_residual = std::sqrt(q.init_rr);
+    st.solve_time = ms * 1e-3;
+    st.refinements = refreshes;
 }
 
 } // namespace ffh
@@ -863,8 +1259,7 @@ int ffh_server_create(ffh_context *ctx, int64_t budget_bytes, int max_systems, i
         S->budget = (size_t)budget_bytes;
         S->max_sys = max_systems;
         S->chunk = chunk;
-        S->h_scal.assign((size_t)max_systems * ffh::kMaxRhs, ffh::BatchScal{});
-        S->d_scal.alloc(S->h_scal.size());
+        S->eng.init(max_systems);
         for (int i = max_systems - 1; i >= 0; --i) S->free_slots.push_back(i);
     });
     if (rc != FFH_OK) {
@@ -944,7 +1339,7 @@ int ffh_server_progress(const ffh_server *S, int64_t ticket, int *state, int *it
         int itn = q->done ? q->iters[r] : 0;
         double rel = q->done ? q->resid[r] : -1.0;
         if (!q->done && q->slot >= 0) {
-            const ffh::BatchScal &c = S->h_scal[(size_t)q->slot * ffh::kMaxRhs + r];
+            const ffh::BatchScal &c = S->eng.h_scal[(size_t)q->slot * ffh::kMaxRhs + r];
             st = c.state;
             itn = c.iters;
             rel = c.bb > 0 ? std::sqrt(c.true_rr / c.bb) : 0.0;

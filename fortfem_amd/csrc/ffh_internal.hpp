@@ -114,7 +114,27 @@ struct DevMatrix {
     DBuf<int> rowblocks; // CSR-stream row blocks of the gather arrays
     int nrb = 0;
     int variant[2] = {-1, -1}; // autotuned SpMV variant (FP64, FP32)
+    // SELL-64 copy of the gather arrays, built on first use: slices of 64
+    // consecutive rows, entry j of the slice's row l at sell_off[s] + 64 j + l,
+    // each row padded to its slice's longest row with zero values on its own
+    // column.  sell_src gives the gather-array position of each entry (-1 for
+    // padding); the value copies are refreshed when the values change.
+    bool sell_built = false, sell_ok = false;
+    bool sell64_valid = false, sell32_valid = false;
+    int nslice = 0;
+    int64_t sell_total = 0;
+    DBuf<int> sell_off, sell_idx, sell_src;
+    DBuf<double> sell_val;
+    DBuf<float> sell_val32;
+    // the values (FP64, FP32 or both) were rewritten
+    void values_changed()
+    {
+        val32_valid = false;
+        sell64_valid = sell32_valid = false;
+    }
 };
+
+struct SoloState; // batch-engine state of the mixed-precision solve (ffh_batch.hip)
 
 // Solver workspace attached to a matrix (kept resident between solves).
 struct Workspace {
@@ -126,6 +146,7 @@ struct Workspace {
     KrylovCtl *host_ctl = nullptr; // pinned mailbox
     hipGraphExec_t graph[2] = {nullptr, nullptr};
     int graph_chunk[2] = {0, 0};
+    SoloState *solo = nullptr;
     ~Workspace();
 };
 

@@ -185,7 +185,70 @@ __device__ __forceinline__ void csr_vector(int n, const int *__restrict__ ptr,
     }
 }
 
-// Row engine of a launch: V = 0 stream, otherwise vector CSR with G = V lanes.
+// SELL-64: one wave per slice of 64 rows, one lane per row.  Every index
+// and value load of a wave reads 64 consecutive words; each lane adds its
+// row's products in column order (the CSR order).  Entries go in steps of
+// eight: the sixteen index/value loads of a step are issued together, then
+// the eight gathers, so a wave has many loads in flight.  Here ptr is the
+// slice offset array, idx/val the SELL entries and nrb the slice count.
+template <typename MT, typename VT, typename E>
+__device__ __forceinline__ void sell_rows(int n, const int *__restrict__ off,
+                                          const int *__restrict__ sidx,
+                                          const MT *__restrict__ sval,
+                                          const VT *__restrict__ x, int nslice, E &epi,
+                                          double &d0, double &d1)
+{
+    constexpr int WPB = kStreamBlock / 64, U = 8;
+    const int lane = threadIdx.x & 63;
+    for (int sl = blockIdx.x * WPB + (int)(threadIdx.x >> 6); sl < nslice;
+         sl += gridDim.x * WPB) {
+        const int o = off[sl];
+        const int w = (off[sl + 1] - o) >> 6;
+        const int *ip = sidx + o + lane;
+        const MT *vp = sval + o + lane;
+        VT s = VT(0);
+        int j = 0;
+        for (; j + U <= w; j += U) {
+            int c[U];
+            MT v[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                c[u] = __builtin_nontemporal_load(ip + 64 * u);
+                v[u] = __builtin_nontemporal_load(vp + 64 * u);
+            }
+            VT xv[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) xv[u] = x[c[u]];
+#pragma unroll
+            for (int u = 0; u < U; ++u) s += (VT)v[u] * xv[u];
+            ip += 64 * U;
+            vp += 64 * U;
+        }
+        if (j < w) {
+            // the last w - j (< U) entries, loads first
+            int c[U];
+            MT v[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u)
+                if (j + u < w) {
+                    c[u] = __builtin_nontemporal_load(ip + 64 * u);
+                    v[u] = __builtin_nontemporal_load(vp + 64 * u);
+                }
+            VT xv[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u)
+                if (j + u < w) xv[u] = x[c[u]];
+#pragma unroll
+            for (int u = 0; u < U; ++u)
+                if (j + u < w) s += (VT)v[u] * xv[u];
+        }
+        const int row = sl * 64 + lane;
+        if (row < n) epi(row, s, d0, d1);
+    }
+}
+
+// Row engine of a launch: V = 0 stream, V = 1 SELL-64, otherwise vector CSR
+// with G = V lanes.
 template <int V, typename MT, typename VT, typename E>
 __device__ __forceinline__ void row_engine(int n, const int *__restrict__ ptr,
                                            const int *__restrict__ idx,
@@ -196,6 +259,8 @@ __device__ __forceinline__ void row_engine(int n, const int *__restrict__ ptr,
 {
     if constexpr (V == 0)
         csr_stream<MT, VT>(ptr, idx, val, x, rowblocks, nrb, epi, d0, d1);
+    else if constexpr (V == 1)
+        sell_rows<MT, VT>(n, ptr, idx, val, x, nrb, epi, d0, d1);
     else
         csr_vector<V, MT, VT>(n, ptr, idx, val, x, epi, d0, d1);
 }
@@ -429,30 +494,62 @@ __global__ __launch_bounds__(kBlock) void k_residual_finish(const double *__rest
     if (threadIdx.x == 0) ctl->rr = t;
 }
 
-__global__ __launch_bounds__(kBlock) void k_scale_f32(int n, const double *__restrict__ r,
-                                                      double s, float *__restrict__ b32,
-                                                      float *__restrict__ x32)
-{
-    for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
-        b32[i] = (float)(r[i] * s);
-        x32[i] = 0.0f;
-    }
-}
-
-__global__ __launch_bounds__(kBlock) void k_refine(int n, double *__restrict__ x,
-                                                   const float *__restrict__ d32,
-                                                   double s)
-{
-    for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock)
-        x[i] = x[i] + s * (double)d32[i];
-}
-
 __global__ __launch_bounds__(kBlock) void k_to_f32(int64_t n, const double *__restrict__ a,
                                                    float *__restrict__ b)
 {
     for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n;
          i += (int64_t)gridDim.x * kBlock)
         b[i] = (float)a[i];
+}
+
+// SELL-64 construction: the longest row of every slice, then the entries
+// (padding: zero value on the row's own column, source position -1).
+__global__ __launch_bounds__(kBlock) void k_sell_width(int n, const int *__restrict__ ptr,
+                                                       int nslice, int *__restrict__ width)
+{
+    const int sl = blockIdx.x * kBlock + threadIdx.x;
+    if (sl >= nslice) return;
+    const int r0 = sl * 64, r1 = min(n, r0 + 64);
+    int w = 0;
+    for (int r = r0; r < r1; ++r) w = max(w, ptr[r + 1] - ptr[r]);
+    width[sl] = w;
+}
+
+__global__ __launch_bounds__(kBlock) void k_sell_fill(int n, const int *__restrict__ ptr,
+                                                      const int *__restrict__ idx, int nslice,
+                                                      const int *__restrict__ off,
+                                                      int *__restrict__ sidx,
+                                                      int *__restrict__ src)
+{
+    const int64_t t = blockIdx.x * (int64_t)kBlock + threadIdx.x;
+    if (t >= (int64_t)nslice * 64) return;
+    const int row = (int)t, sl = row >> 6, lane = row & 63;
+    const int o = off[sl], w = (off[sl + 1] - o) >> 6;
+    const int b = row < n ? ptr[row] : 0;
+    const int len = row < n ? ptr[row + 1] - b : 0;
+    const int pad = row < n ? row : 0;
+    for (int j = 0; j < w; ++j) {
+        const int e = o + 64 * j + lane;
+        if (j < len) {
+            sidx[e] = idx[b + j];
+            src[e] = b + j;
+        } else {
+            sidx[e] = pad;
+            src[e] = -1;
+        }
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_sell_values(int64_t total, const int *__restrict__ src,
+                                                        const T *__restrict__ val,
+                                                        T *__restrict__ sval)
+{
+    for (int64_t e = blockIdx.x * (int64_t)kBlock + threadIdx.x; e < total;
+         e += (int64_t)gridDim.x * kBlock) {
+        const int k = src[e];
+        sval[e] = k >= 0 ? val[k] : T(0);
+    }
 }
 
 // ---------------------------------------------------------------- driver --
@@ -470,20 +567,53 @@ struct SpmvVariant {
     int engine, gmul;
 };
 static const SpmvVariant kVariants[] = {{0, 1}, {0, 2}, {0, 4}, {4, 2}, {8, 1},
-                                        {8, 2}, {16, 2}, {32, 2}, {32, 4}};
+                                        {8, 2}, {16, 2}, {32, 2}, {32, 4}, {1, 1},
+                                        {1, 2}};
 constexpr int kNumVariants = sizeof(kVariants) / sizeof(kVariants[0]);
 
 static int variant_grid(const Context *ctx, const DevMatrix &A, int vi)
 {
     const SpmvVariant v = kVariants[vi];
     const int64_t cap = (int64_t)grid_cap(ctx) * v.gmul;
-    const int64_t need = v.engine == 0 ? A.nrb : ((int64_t)A.n * v.engine + kStreamBlock - 1) / kStreamBlock;
+    int64_t need;
+    if (v.engine == 0)
+        need = A.nrb;
+    else if (v.engine == 1)
+        need = ((int64_t)A.nslice + kStreamBlock / 64 - 1) / (kStreamBlock / 64);
+    else
+        need = ((int64_t)A.n * v.engine + kStreamBlock - 1) / kStreamBlock;
     return (int)std::max<int64_t>(1, std::min(need, cap));
+}
+
+// Gather arrays of a launch: the CSR (CSC read as CSR) arrays with the row
+// blocks of the stream engine, or the SELL-64 arrays with the slice count.
+template <typename MT>
+struct GArgs {
+    const int *ptr, *idx;
+    const MT *val;
+    const int *rb;
+    int nrb;
+};
+
+template <typename MT>
+static GArgs<MT> gargs(const DevMatrix &A, int vi)
+{
+    if (kVariants[vi].engine == 1) {
+        if constexpr (sizeof(MT) == 8)
+            return {A.sell_off.p, A.sell_idx.p, A.sell_val.p, nullptr, A.nslice};
+        else
+            return {A.sell_off.p, A.sell_idx.p, A.sell_val32.p, nullptr, A.nslice};
+    }
+    if constexpr (sizeof(MT) == 8)
+        return {A.gptr(), A.gidx(), A.gval(), A.rowblocks.p, A.nrb};
+    else
+        return {A.gptr(), A.gidx(), A.gval32(), A.rowblocks.p, A.nrb};
 }
 
 #define FFH_ENGINE(eng, ...)                                                  \
     switch (eng) {                                                           \
     case 0: { constexpr int VV = 0; __VA_ARGS__; } break;                     \
+    case 1: { constexpr int VV = 1; __VA_ARGS__; } break;                     \
     case 4: { constexpr int VV = 4; __VA_ARGS__; } break;                     \
     case 8: { constexpr int VV = 8; __VA_ARGS__; } break;                     \
     case 16: { constexpr int VV = 16; __VA_ARGS__; } break;                   \
@@ -509,11 +639,14 @@ void build_rowblocks(Context *ctx, DevMatrix &A, const std::vector<int> &ptr)
     A.nrb = (int)rb.size() - 1;
     A.rowblocks.upload(rb, ctx->stream);
     A.variant[0] = A.variant[1] = -1;
+    A.sell_built = A.sell_ok = false;
+    A.sell64_valid = A.sell32_valid = false;
     FFH_HIP(hipStreamSynchronize(ctx->stream));
 }
 
 Workspace::~Workspace()
 {
+    free_solo(solo);
     for (auto &g : graph)
         if (g) (void)hipGraphExecDestroy(g);
     if (host_ctl) (void)hipHostFree(host_ctl);
@@ -564,15 +697,120 @@ void ensure_val32(Context *ctx, DevMatrix &A)
     A.val32_valid = true;
 }
 
+// SELL-64 structure of the gather arrays.  Not built (sell_ok false, the
+// SELL variants fall back to the stream engine) when the padding would more
+// than triple the stored entries or the entry count exceeds int32.
+static void ensure_sell_structure(Context *ctx, DevMatrix &A)
+{
+    if (A.sell_built) return;
+    A.sell_built = true;
+    A.sell_ok = false;
+    const int n = A.n;
+    if (n == 0 || A.nnz == 0) return;
+    const int nslice = (n + 63) / 64;
+    hipStream_t s = ctx->stream;
+    DBuf<int> width;
+    width.alloc(nslice);
+    hipLaunchKernelGGL(k_sell_width, dim3((nslice + kBlock - 1) / kBlock), dim3(kBlock), 0, s, n,
+                       A.gptr(), nslice, width.p);
+    FFH_HIP(hipGetLastError());
+    std::vector<int> w(nslice);
+    width.download(w.data(), nslice, s);
+    FFH_HIP(hipStreamSynchronize(s));
+    std::vector<int> off(nslice + 1, 0);
+    int64_t total = 0;
+    for (int i = 0; i < nslice; ++i) {
+        total += 64ll * w[i];
+        if (total > INT32_MAX) return;
+        off[i + 1] = (int)total;
+    }
+    if (total > 3 * A.nnz + (1 << 20)) return;
+    A.sell_off.upload(off, s);
+    A.sell_idx.alloc(std::max<int64_t>(total, 1));
+    A.sell_src.alloc(std::max<int64_t>(total, 1));
+    hipLaunchKernelGGL(k_sell_fill, dim3((unsigned)(((int64_t)nslice * 64 + kBlock - 1) / kBlock)),
+                       dim3(kBlock), 0, s, n, A.gptr(), A.gidx(), nslice, A.sell_off.p,
+                       A.sell_idx.p, A.sell_src.p);
+    FFH_HIP(hipGetLastError());
+    FFH_HIP(hipStreamSynchronize(s));
+    A.nslice = nslice;
+    A.sell_total = total;
+    A.sell_ok = true;
+    A.sell64_valid = A.sell32_valid = false;
+}
+
+// SELL values of one precision, refreshed after the values change.
+static void ensure_sell_values(Context *ctx, DevMatrix &A, bool f32)
+{
+    ensure_sell_structure(ctx, A);
+    if (!A.sell_ok) return;
+    const int grid = grid_for(ctx, A.sell_total, kBlock);
+    if (f32) {
+        if (A.sell32_valid) return;
+        ensure_val32(ctx, A);
+        A.sell_val32.alloc(A.sell_total);
+        hipLaunchKernelGGL(k_sell_values<float>, dim3(grid), dim3(kBlock), 0, ctx->stream,
+                           A.sell_total, A.sell_src.p, A.gval32(), A.sell_val32.p);
+        A.sell32_valid = true;
+    } else {
+        if (A.sell64_valid) return;
+        ensure_val64(ctx, A);
+        A.sell_val.alloc(A.sell_total);
+        hipLaunchKernelGGL(k_sell_values<double>, dim3(grid), dim3(kBlock), 0, ctx->stream,
+                           A.sell_total, A.sell_src.p, A.gval(), A.sell_val.p);
+        A.sell64_valid = true;
+    }
+    FFH_HIP(hipGetLastError());
+}
+
+void ensure_sell(Context *ctx, DevMatrix &A, bool f32) { ensure_sell_values(ctx, A, f32); }
+
+bool sell_structure(Context *ctx, DevMatrix &A)
+{
+    ensure_sell_structure(ctx, A);
+    return A.sell_ok;
+}
+
+__global__ __launch_bounds__(kBlock) void k_sell_values2(int64_t total, const int *__restrict__ src,
+                                                         const double *__restrict__ val,
+                                                         double *__restrict__ o64,
+                                                         float *__restrict__ o32)
+{
+    for (int64_t e = blockIdx.x * (int64_t)kBlock + threadIdx.x; e < total;
+         e += (int64_t)gridDim.x * kBlock) {
+        const int k = src[e];
+        const double v = k >= 0 ? val[k] : 0.0;
+        if (o64) o64[e] = v;
+        if (o32) o32[e] = (float)v;
+    }
+}
+
+void sell_values(Context *ctx, const DevMatrix &A, const double *val, double *out64, float *out32)
+{
+    FFH_REQUIRE(A.sell_ok, FFH_ERR_INVALID, "matrix has no SELL-64 copy");
+    hipLaunchKernelGGL(k_sell_values2, dim3(grid_for(ctx, A.sell_total, kBlock)),
+                       dim3(kBlock), 0, ctx->stream, A.sell_total, A.sell_src.p, val, out64, out32);
+    FFH_HIP(hipGetLastError());
+}
+
+// The variant to launch: vi itself with its arrays current, or the stream
+// engine when vi is a SELL variant and the matrix has no SELL copy.
+static int ready_variant(Context *ctx, DevMatrix &A, int vi, bool f32)
+{
+    if (kVariants[vi].engine != 1) return vi;
+    ensure_sell_values(ctx, A, f32);
+    return A.sell_ok ? vi : 0;
+}
+
+// y = A x with variant vi (its arrays current: see ready_variant)
 template <typename MT, typename VT>
-static void launch_spmv(Context *ctx, const DevMatrix &A, const MT *val, const VT *x, VT *y,
-                        int vi)
+static void launch_spmv(Context *ctx, const DevMatrix &A, const VT *x, VT *y, int vi)
 {
     const int grid = variant_grid(ctx, A, vi);
+    const GArgs<MT> g = gargs<MT>(A, vi);
     FFH_ENGINE(kVariants[vi].engine, {
         hipLaunchKernelGGL((k_spmv<VV, MT, VT>), dim3(grid), dim3(kStreamBlock), 0,
-                           ctx->stream, A.n, A.gptr(), A.gidx(), val, x, y, A.rowblocks.p,
-                           A.nrb);
+                           ctx->stream, A.n, g.ptr, g.idx, g.val, x, y, g.rb, g.nrb);
     });
     FFH_HIP(hipGetLastError());
 }
@@ -582,11 +820,12 @@ static void launch_spmv(Context *ctx, const DevMatrix &A, const MT *val, const V
 // stream, into scratch vectors of its own, so that no solver vector or timing
 // event is touched.  FFH_SPMV_VARIANT=<index> forces a variant.
 template <typename MT, typename VT>
-static int tune_spmv(Context *ctx, DevMatrix &A, const MT *val)
+static int tune_spmv(Context *ctx, DevMatrix &A)
 {
+    constexpr bool f32 = sizeof(MT) == 4;
     if (const char *env = std::getenv("FFH_SPMV_VARIANT")) {
         int v = std::atoi(env);
-        if (v >= 0 && v < kNumVariants) return v;
+        if (v >= 0 && v < kNumVariants) return ready_variant(ctx, A, v, f32);
     }
     if (A.nnz == 0 || A.n == 0) return 0;
     DBuf<VT> tx, ty;
@@ -603,9 +842,11 @@ static int tune_spmv(Context *ctx, DevMatrix &A, const MT *val)
             if (kVariants[vi].engine == 0 && kVariants[vi].gmul > 1 &&
                 A.nrb <= grid_cap(ctx) * (kVariants[vi].gmul / 2))
                 continue; // same grid as the variant before it
-            launch_spmv<MT, VT>(ctx, A, val, tx.p, ty.p, vi);
+            if (kVariants[vi].engine == 1 && ready_variant(ctx, A, vi, f32) != vi)
+                continue; // no SELL copy of this matrix
+            launch_spmv<MT, VT>(ctx, A, tx.p, ty.p, vi);
             FFH_HIP(hipEventRecord(e0, ctx->stream));
-            for (int r = 0; r < 10; ++r) launch_spmv<MT, VT>(ctx, A, val, tx.p, ty.p, vi);
+            for (int r = 0; r < 10; ++r) launch_spmv<MT, VT>(ctx, A, tx.p, ty.p, vi);
             FFH_HIP(hipEventRecord(e1, ctx->stream));
             FFH_HIP(hipEventSynchronize(e1));
             float ms = 0.0f;
@@ -634,11 +875,12 @@ static int spmv_variant(Context *ctx, Matrix &M, bool f32)
     if (v < 0) {
         if (f32) {
             ensure_val32(ctx, A);
-            v = tune_spmv<float, float>(ctx, A, A.gval32());
+            v = tune_spmv<float, float>(ctx, A);
         } else {
-            v = tune_spmv<double, double>(ctx, A, A.gval());
+            v = tune_spmv<double, double>(ctx, A);
         }
     }
+    v = ready_variant(ctx, A, v, f32);
     return v;
 }
 
@@ -648,6 +890,7 @@ int select_spmv_variant(Context *ctx, Matrix &M, bool f32, int request)
     ensure_val64(ctx, M.m);
     if (f32) ensure_val32(ctx, M.m);
     if (request >= 0) {
+        request = ready_variant(ctx, M.m, request, f32);
         M.m.variant[f32 ? 1 : 0] = request;
         invalidate_graphs(M.ws);
         return request;
@@ -655,24 +898,24 @@ int select_spmv_variant(Context *ctx, Matrix &M, bool f32, int request)
     return spmv_variant(ctx, M, f32);
 }
 
-void spmv64(Context *ctx, const DevMatrix &A, const double *x, double *y)
+void spmv64(Context *ctx, DevMatrix &A, const double *x, double *y)
 {
-    const int vi = A.variant[0] >= 0 ? A.variant[0] : 0;
-    launch_spmv<double, double>(ctx, A, A.gval(), x, y, vi);
+    ensure_val64(ctx, A);
+    const int vi = ready_variant(ctx, A, A.variant[0] >= 0 ? A.variant[0] : 0, false);
+    launch_spmv<double, double>(ctx, A, x, y, vi);
 }
 
 void spmv32(Context *ctx, DevMatrix &A, const float *x, float *y)
 {
     ensure_val32(ctx, A);
-    const int vi = A.variant[1] >= 0 ? A.variant[1] : 0;
-    launch_spmv<float, float>(ctx, A, A.gval32(), x, y, vi);
+    const int vi = ready_variant(ctx, A, A.variant[1] >= 0 ? A.variant[1] : 0, true);
+    launch_spmv<float, float>(ctx, A, x, y, vi);
 }
 
 // Launch `iters` PCG iterations (3 kernels each) of precision VT.
 template <typename MT, typename VT>
 static void launch_iterations(Context *ctx, Matrix &M, int iters, VT *x, VT *r,
-                              VT *p, VT *ap, const VT *dinv, const MT *val, int vi,
-                              int prof_slot)
+                              VT *p, VT *ap, const VT *dinv, int vi, int prof_slot)
 {
     const DevMatrix &A = M.m;
     Workspace &w = M.ws;
@@ -680,13 +923,14 @@ static void launch_iterations(Context *ctx, Matrix &M, int iters, VT *x, VT *r,
     const int gridv = grid_for(ctx, n, kBlock);
     const int grids = variant_grid(ctx, A, vi);
     const int stride = partial_stride(ctx);
+    const GArgs<MT> g = gargs<MT>(A, vi);
     for (int it = 0; it < iters; ++it) {
         const int parity = it & 1; // chunks are even, so this is the global parity
         if (prof_slot >= 0) ctx->prof_mark(prof_slot, true);
         FFH_ENGINE(kVariants[vi].engine, {
             hipLaunchKernelGGL((k_pcg_spmv<VV, MT, VT>), dim3(grids), dim3(kStreamBlock), 0,
-                               ctx->stream, n, A.gptr(), A.gidx(), val, p, ap, A.rowblocks.p,
-                               A.nrb, w.partials.p, w.ctl.p);
+                               ctx->stream, n, g.ptr, g.idx, g.val, p, ap, g.rb, g.nrb,
+                               w.partials.p, w.ctl.p);
         });
         if (prof_slot >= 0) ctx->prof_mark(prof_slot, false);
         hipLaunchKernelGGL((k_pcg_update<VT>), dim3(gridv), dim3(kBlock), 0, ctx->stream,
@@ -700,7 +944,7 @@ static void launch_iterations(Context *ctx, Matrix &M, int iters, VT *x, VT *r,
 
 // One PCG solve on device vectors b, x (x = initial guess, overwritten).
 template <typename MT, typename VT>
-static void pcg_device(Context *ctx, Matrix &M, const MT *val, const VT *b, VT *x,
+static void pcg_device(Context *ctx, Matrix &M, const VT *b, VT *x,
                        VT *r, VT *p, VT *ap, VT *dinv, int method, int precond,
                        double tolerance, int tol_type, int max_iter,
                        int breakdown_mode, int check_interval, int use_graph,
@@ -723,10 +967,11 @@ static void pcg_device(Context *ctx, Matrix &M, const MT *val, const VT *b, VT *
                            hipMemcpyHostToDevice, ctx->stream));
     const int vi = spmv_variant(ctx, M, sizeof(VT) == 4);
     const int gi = variant_grid(ctx, A, vi);
+    const GArgs<MT> g = gargs<MT>(A, vi);
     FFH_ENGINE(kVariants[vi].engine, {
         hipLaunchKernelGGL((k_pcg_init<VV, MT, VT>), dim3(gi), dim3(kStreamBlock), 0,
-                           ctx->stream, n, A.gptr(), A.gidx(), val, x, b, r, p, dinv,
-                           A.rowblocks.p, A.nrb, w.partials.p, partial_stride(ctx));
+                           ctx->stream, n, g.ptr, g.idx, g.val, x, b, r, p, dinv, g.rb, g.nrb,
+                           w.partials.p, partial_stride(ctx));
     });
     hipLaunchKernelGGL(k_pcg_init_finish, dim3(1), dim3(kBlock), 0, ctx->stream, w.partials.p,
                        partial_stride(ctx), gi, w.ctl.p);
@@ -743,7 +988,7 @@ static void pcg_device(Context *ctx, Matrix &M, const MT *val, const VT *b, VT *
         }
         hipGraph_t g;
         FFH_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-        launch_iterations<MT, VT>(ctx, M, chunk, x, r, p, ap, dinv, val, vi, -1);
+        launch_iterations<MT, VT>(ctx, M, chunk, x, r, p, ap, dinv, vi, -1);
         FFH_HIP(hipStreamEndCapture(ctx->stream, &g));
         FFH_HIP(hipGraphInstantiate(&w.graph[gslot], g, nullptr, nullptr, 0));
         (void)hipGraphDestroy(g);
@@ -754,7 +999,7 @@ static void pcg_device(Context *ctx, Matrix &M, const MT *val, const VT *b, VT *
         if (use_graph)
             FFH_HIP(hipGraphLaunch(w.graph[gslot], ctx->stream));
         else
-            launch_iterations<MT, VT>(ctx, M, chunk, x, r, p, ap, dinv, val, vi,
+            launch_iterations<MT, VT>(ctx, M, chunk, x, r, p, ap, dinv, vi,
                                       profiling ? slot : -1);
         FFH_HIP(hipMemcpyAsync(w.host_ctl, w.ctl.p, sizeof(KrylovCtl),
                                hipMemcpyDeviceToHost, ctx->stream));
@@ -770,14 +1015,15 @@ static void launch_residual(Context *ctx, Matrix &M)
     DevMatrix &A = M.m;
     Workspace &w = M.ws;
     const int vi = spmv_variant(ctx, M, false);
-    const int g = variant_grid(ctx, A, vi);
+    const int gr = variant_grid(ctx, A, vi);
+    const GArgs<double> g = gargs<double>(A, vi);
     FFH_ENGINE(kVariants[vi].engine, {
-        hipLaunchKernelGGL((k_residual<VV>), dim3(g), dim3(kStreamBlock), 0, ctx->stream, A.n,
-                           A.gptr(), A.gidx(), A.gval(), w.x.p, w.b.p, w.r.p, A.rowblocks.p,
-                           A.nrb, w.partials.p);
+        hipLaunchKernelGGL((k_residual<VV>), dim3(gr), dim3(kStreamBlock), 0, ctx->stream, A.n,
+                           g.ptr, g.idx, g.val, w.x.p, w.b.p, w.r.p, g.rb, g.nrb,
+                           w.partials.p);
     });
     hipLaunchKernelGGL(k_residual_finish, dim3(1), dim3(kBlock), 0, ctx->stream, w.partials.p,
-                       g, w.ctl.p);
+                       gr, w.ctl.p);
     FFH_HIP(hipGetLastError());
 }
 
@@ -797,7 +1043,7 @@ void solve_fp64(Context *ctx, Matrix &M, const double *b, double *x,
                            hipMemcpyDeviceToDevice, ctx->stream));
     FFH_HIP(hipMemcpyAsync(w.x.p, x, sizeof(double) * M.m.n,
                            hipMemcpyDeviceToDevice, ctx->stream));
-    pcg_device<double, double>(ctx, M, M.m.gval(), w.b.p, w.x.p, w.r.p, w.p.p,
+    pcg_device<double, double>(ctx, M, w.b.p, w.x.p, w.r.p, w.p.p,
                                w.ap.p, w.dinv.p, o.method, o.preconditioner,
                                o.tolerance, o.tolerance_type, o.max_iterations,
                                o.breakdown_mode, o.check_interval, o.use_graph, 0,
@@ -821,86 +1067,6 @@ void solve_fp64(Context *ctx, Matrix &M, const double *b, double *x,
     st.refinements = 0;
     st.true_residual = std::sqrt(w.host_ctl->rr);
     st.initial_residual = out.initial_norm;
-}
-
-// Mixed precision: FP32 matrix, vectors and SpMV accumulation in the inner
-// Jacobi-PCG (reductions in FP64), inside an FP64 iterative-refinement loop
-// whose residual b - A x is recomputed in FP64 with the FP64 matrix.  Stops
-// when ||b - A x|| <= tol * ||b - A x0|| (relative) or <= tol (absolute).
-void solve_mixed(Context *ctx, Matrix &M, const double *b, double *x,
-                 const ffh_solver_options &o, ffh_solver_stats &st)
-{
-    ensure_workspace(ctx, M);
-    ensure_workspace32(ctx, M);
-    DevMatrix &A = M.m;
-    ensure_val64(ctx, A);
-    ensure_val32(ctx, A);
-    spmv_variant(ctx, M, false);
-    spmv_variant(ctx, M, true);
-    Workspace &w = M.ws;
-    const int n = A.n;
-    if (ctx->profiling) ctx->prof_begin();
-    FFH_HIP(hipEventRecord(ctx->ev_start, ctx->stream));
-    FFH_HIP(hipMemcpyAsync(w.b.p, b, sizeof(double) * n, hipMemcpyDeviceToDevice,
-                           ctx->stream));
-    FFH_HIP(hipMemcpyAsync(w.x.p, x, sizeof(double) * n, hipMemcpyDeviceToDevice,
-                           ctx->stream));
-    auto true_residual = [&]() -> double {
-        if (ctx->profiling) ctx->prof_mark(0, true);
-        launch_residual(ctx, M);
-        if (ctx->profiling) ctx->prof_mark(0, false);
-        FFH_HIP(hipMemcpyAsync(w.host_ctl, w.ctl.p, sizeof(KrylovCtl),
-                               hipMemcpyDeviceToHost, ctx->stream));
-        FFH_HIP(hipStreamSynchronize(ctx->stream));
-        return std::sqrt(w.host_ctl->rr);
-    };
-    // the control block counters must start at zero
-    KrylovCtl zero{};
-    *w.host_ctl = zero;
-    FFH_HIP(hipMemcpyAsync(w.ctl.p, w.host_ctl, sizeof(KrylovCtl),
-                           hipMemcpyHostToDevice, ctx->stream));
-    double rnorm = true_residual();
-    const double r0 = rnorm;
-    const double tol = o.tolerance_type == 1 ? o.tolerance : o.tolerance * r0;
-    int total_iters = 0, outer = 0, converged = rnorm <= tol;
-    const int max_ref = o.max_refinements > 0 ? o.max_refinements : 50;
-    const double inner_tol = o.inner_tolerance > 0 ? o.inner_tolerance : 1e-4;
-    const int gridv = grid_for(ctx, n, kBlock);
-    while (!converged && outer < max_ref && total_iters < o.max_iterations) {
-        const double s = rnorm > 0 ? 1.0 / rnorm : 1.0;
-        // inner right-hand side r / ||r|| in FP32, held in ap32 until the
-        // init kernel has consumed it; the correction accumulates in x32
-        hipLaunchKernelGGL(k_scale_f32, dim3(gridv), dim3(kBlock), 0, ctx->stream, n,
-                           w.r.p, s, w.ap32.p, w.x32.p);
-        KrylovCtl out{};
-        pcg_device<float, float>(ctx, M, A.gval32(), w.ap32.p, w.x32.p, w.r32.p, w.p32.p,
-                                 w.ap32.p, w.dinv32.p, o.method, o.preconditioner,
-                                 inner_tol, 0, o.max_iterations - total_iters, 1,
-                                 o.check_interval, o.use_graph, 1, out);
-        total_iters += out.iter;
-        hipLaunchKernelGGL(k_refine, dim3(gridv), dim3(kBlock), 0, ctx->stream, n,
-                           w.x.p, w.x32.p, rnorm);
-        FFH_HIP(hipGetLastError());
-        ++outer;
-        const double prev = rnorm;
-        rnorm = true_residual();
-        converged = rnorm <= tol;
-        if (out.iter == 0 || !(rnorm < prev)) break; // stagnation
-    }
-    FFH_HIP(hipMemcpyAsync(x, w.x.p, sizeof(double) * n, hipMemcpyDeviceToDevice,
-                           ctx->stream));
-    FFH_HIP(hipEventRecord(ctx->ev_stop, ctx->stream));
-    FFH_HIP(hipStreamSynchronize(ctx->stream));
-    float ms = 0.0f;
-    FFH_HIP(hipEventElapsedTime(&ms, ctx->ev_start, ctx->ev_stop));
-    if (ctx->profiling) ctx->prof_end();
-    st.converged = converged;
-    st.iterations = total_iters;
-    st.final_residual = rnorm;
-    st.solve_time = ms * 1e-3;
-    st.refinements = outer;
-    st.true_residual = rnorm;
-    st.initial_residual = r0;
 }
 
 double bench_spmv(Context *ctx, Matrix &M, int precision, int reps, double *bytes)

@@ -7,7 +7,7 @@ void ensure_workspace(Context *ctx, Matrix &M);
 void ensure_val32(Context *ctx, DevMatrix &A);
 void invalidate_graphs(Workspace &w);
 void build_rowblocks(Context *ctx, DevMatrix &A, const std::vector<int> &ptr);
-void spmv64(Context *ctx, const DevMatrix &A, const double *x, double *y);
+void spmv64(Context *ctx, DevMatrix &A, const double *x, double *y);
 void spmv32(Context *ctx, DevMatrix &A, const float *x, float *y);
 void solve_fp64(Context *ctx, Matrix &M, const double *b, double *x,
                 const ffh_solver_options &o, ffh_solver_stats &st);
@@ -29,4 +29,16 @@ void problem_dirichlet(Problem *P, int mode, const double *values);
 void problem_reduce(Problem *P);
 void problem_solve(Problem *P, const ffh_solver_options &o, double *solution,
                    ffh_solver_stats &st);
+} // namespace ffh
+
+namespace ffh {
+// SELL-64 copy of a matrix's gather arrays (ffh_krylov.hip): the structure
+// (false when the matrix is too uneven for it) and a value permutation into
+// caller buffers (either output may be null).
+bool sell_structure(Context *ctx, DevMatrix &A);
+// the matrix's own SELL value copy of one precision, made current
+void ensure_sell(Context *ctx, DevMatrix &A, bool f32);
+void free_solo(SoloState *s);
+void sell_values(Context *ctx, const DevMatrix &A, const double *val, double *out64,
+                 float *out32);
 } // namespace ffh

@@ -627,7 +627,7 @@ void problem_pattern(Problem *P, int family, int order)
     A.val.alloc(A.nnz);
     FFH_HIP(hipMemsetAsync(A.val.p, 0, sizeof(double) * A.nnz, s));
     A.symmetric = true;
-    A.val32_valid = false;
+    A.values_changed();
     A.val32.reset();
     build_rowblocks(ctx, A, col_ptr);
     P->full.ctx = ctx;
@@ -728,6 +728,7 @@ static void assemble_nedelec(Problem *P, int coeff_kind, const double *coeff, do
                                A.val.p);
     }
     FFH_HIP(hipGetLastError());
+    A.values_changed();
     A.val32_valid = f32;
     A.val64_stale = f32;
     FFH_HIP(hipStreamSynchronize(s));
@@ -765,7 +766,7 @@ void problem_assemble(Problem *P, int qdeg, int coeff_kind, const double *coeff,
     const int nc = P->n_colors;
     if (P->family == FFH_FAMILY_P1_2D) {
         A.val64_stale = false;
-        A.val32_valid = false;
+        A.values_changed();
         FFH_HIP(hipMemsetAsync(A.val.p, 0, sizeof(double) * A.nnz, s));
         for (int c = 0; c < nc; ++c) {
             int p0 = P->color_start[c], p1 = P->color_start[c + 1];
@@ -826,6 +827,7 @@ void problem_assemble(Problem *P, int qdeg, int coeff_kind, const double *coeff,
         check_err(P, "tetrahedral H1 assembly");
         // FP32 contraction: the FP64 view (export, FP64 solves) is made on
         // first use
+        A.values_changed();
         A.val32_valid = f32;
         A.val64_stale = f32;
     }
@@ -1019,7 +1021,7 @@ void problem_dirichlet(Problem *P, int mode, const double *values)
     R.row_idx.upload(rri, s);
     R.val.alloc(R.nnz);
     R.symmetric = true;
-    R.val32_valid = false;
+    R.values_changed();
     build_rowblocks(ctx, R, rcp);
     P->reduced.ctx = ctx;
     P->reduced.borrowed = true;
@@ -1097,7 +1099,7 @@ void problem_reduce(Problem *P)
         R.row_idx.upload(rri, s);
         R.val.alloc(R.nnz);
         R.symmetric = true;
-        R.val32_valid = false;
+        R.values_changed();
         build_rowblocks(ctx, R, rcp);
         P->reduced.ctx = ctx;
         P->reduced.borrowed = true;
@@ -1117,7 +1119,7 @@ void problem_reduce(Problem *P)
                        P->d_free_dofs.p, P->rhs_full.p, P->d_lift_ptr.p, P->d_lift_k.p,
                        P->d_lift_c.p, P->full.m.val.p, P->d_bvals.p, P->rhs_red.p);
     FFH_HIP(hipGetLastError());
-    R.val32_valid = false;
+    R.values_changed();
     P->red_dirty = false;
 }
 

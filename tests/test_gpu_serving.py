@@ -108,7 +108,10 @@ def test_admission_by_memory(ff):
     v, t = O.unit_box_mesh(4)
     srv = ff.Server(budget_bytes=1 << 30, max_systems=8, chunk=8)
     n, nnz = srv.prepare(0, ff.capi.FAMILY_H1, 1, v, t)
-    one = nnz * 12 + n * 8 + n * 4 * 28 + 4096
+    srv.submit(0, ff.capi.FAMILY_H1, 1, np.ones(t.shape[0]), np.ones((4, n)))
+    srv.step()
+    one = srv.stats()["used_bytes"]  # footprint of one admitted request
+    assert one > nnz * 12 + n * 4 * 8
     small = ff.Server(budget_bytes=int(one * 1.5), max_systems=8, chunk=8)
     small.prepare(0, ff.capi.FAMILY_H1, 1, v, t)
     rng = np.random.default_rng(1)
