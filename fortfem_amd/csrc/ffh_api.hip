@@ -540,8 +540,10 @@ int ffh_spmv(ffh_matrix *M, const double *x, double *y)
         Context *ctx = M->ctx;
         ensure_workspace(ctx, *M);
         ensure_val64(ctx, M->m);
-        M->ws.p.upload(x, M->m.n, ctx->stream);
-        spmv64(ctx, M->m, M->ws.p.p, M->ws.ap.p);
+        // the solver's b and Ap vectors serve as scratch (sizes unchanged)
+        FFH_HIP(hipMemcpyAsync(M->ws.b.p, x, sizeof(double) * M->m.n, hipMemcpyHostToDevice,
+                               ctx->stream));
+        spmv64(ctx, M->m, M->ws.b.p, M->ws.ap.p);
         M->ws.ap.download(y, M->m.n, ctx->stream);
         FFH_HIP(hipStreamSynchronize(ctx->stream));
     });

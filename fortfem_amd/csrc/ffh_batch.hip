@@ -12,7 +12,7 @@
 // the cached pattern, the values to the request.  A work table lists items of
 // four slices (256 rows, one block) of one system; every launch covers the
 // whole table.  The right-hand sides of a system are interleaved (row-major
-// n x R, R in {1, 2, 4, 8}) so one pass over the matrix serves all of them
+// n x R, R in {1, 2, 4, 6, 8}) so one pass over the matrix serves all of them
 // and each lane moves its row's R values with vector loads.  Dot products
 // are per-item partials, summed per system in item order by one block per
 // system (deterministic: a system's result does not depend on what else is
@@ -63,9 +63,9 @@ struct BatchScal {
 };
 constexpr int kAbsTol = 1;
 
-// R values of one row, moved with one vector access
+// R values of one row, moved with vector accesses (R = 6: three pairs)
 template <typename T, int R>
-struct alignas(sizeof(T) * R) Pack {
+struct alignas(sizeof(T) * ((R & (R - 1)) == 0 ? R : 2)) Pack {
     T v[R];
 };
 
@@ -149,26 +149,33 @@ __device__ __forceinline__ void item_spmv(const BatchSys &s, int slice0, const M
     VT acc[R];
 #pragma unroll
     for (int r = 0; r < R; ++r) acc[r] = VT(0);
-    for (int j = 0; j < w; j += U) {
+    int j = 0;
+    for (; j + U <= w; j += U) {
         int c[U];
         VT v[U];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
-            const bool in = j + u < w;
-            c[u] = in ? __builtin_nontemporal_load(ip + 64 * u) : 0;
-            v[u] = in ? (VT)__builtin_nontemporal_load(vp + 64 * u) : VT(0);
+            c[u] = __builtin_nontemporal_load(ip + 64 * u);
+            v[u] = (VT)__builtin_nontemporal_load(vp + 64 * u);
         }
         Pack<VT, R> pc[U];
 #pragma unroll
-        for (int u = 0; u < U; ++u)
-            if (j + u < w) pc[u] = ld<VT, R>(p, c[u]);
+        for (int u = 0; u < U; ++u) pc[u] = ld<VT, R>(p, c[u]);
 #pragma unroll
         for (int u = 0; u < U; ++u)
-            if (j + u < w)
 #pragma unroll
-                for (int r = 0; r < R; ++r) acc[r] += v[u] * pc[u].v[r];
+            for (int r = 0; r < R; ++r) acc[r] += v[u] * pc[u].v[r];
         ip += 64 * U;
         vp += 64 * U;
+    }
+    for (; j < w; ++j) {
+        const int c = __builtin_nontemporal_load(ip);
+        const VT v = (VT)__builtin_nontemporal_load(vp);
+        const Pack<VT, R> pc = ld<VT, R>(p, c);
+#pragma unroll
+        for (int r = 0; r < R; ++r) acc[r] += v * pc.v[r];
+        ip += 64;
+        vp += 64;
     }
     const int row = sl * 64 + lane;
     if (row < s.n) {
@@ -212,6 +219,7 @@ __global__ __launch_bounds__(256) void k_b_spmv(const int2 *__restrict__ work,
     case 1: spmv_item<1>(s, slice0, sh, out); break;
     case 2: spmv_item<2>(s, slice0, sh, out); break;
     case 4: spmv_item<4>(s, slice0, sh, out); break;
+    case 6: spmv_item<6>(s, slice0, sh, out); break;
     default: spmv_item<8>(s, slice0, sh, out); break;
     }
 }
@@ -340,6 +348,7 @@ __global__ __launch_bounds__(256) void k_b_update(const int2 *__restrict__ work,
     case 1: update_rows<1>(s, wk.y, sv, sh, o_rr, o_rz); break;
     case 2: update_rows<2>(s, wk.y, sv, sh, o_rr, o_rz); break;
     case 4: update_rows<4>(s, wk.y, sv, sh, o_rr, o_rz); break;
+    case 6: update_rows<6>(s, wk.y, sv, sh, o_rr, o_rz); break;
     default: update_rows<8>(s, wk.y, sv, sh, o_rr, o_rz); break;
     }
 }
@@ -423,6 +432,7 @@ __global__ __launch_bounds__(256) void k_b_dir(const int2 *__restrict__ work,
     case 1: dir_rows<1>(s, wk.y, sv, restart_only != 0); break;
     case 2: dir_rows<2>(s, wk.y, sv, restart_only != 0); break;
     case 4: dir_rows<4>(s, wk.y, sv, restart_only != 0); break;
+    case 6: dir_rows<6>(s, wk.y, sv, restart_only != 0); break;
     default: dir_rows<8>(s, wk.y, sv, restart_only != 0); break;
     }
 }
@@ -537,6 +547,7 @@ __global__ __launch_bounds__(256) void k_b_resid(const int2 *__restrict__ work,
     case 1: resid_item<1>(s, slice0, sv, fresh, sh, part_rr + o, part_rz + o, part_bb + o, part_dd + o); break;
     case 2: resid_item<2>(s, slice0, sv, fresh, sh, part_rr + o, part_rz + o, part_bb + o, part_dd + o); break;
     case 4: resid_item<4>(s, slice0, sv, fresh, sh, part_rr + o, part_rz + o, part_bb + o, part_dd + o); break;
+    case 6: resid_item<6>(s, slice0, sv, fresh, sh, part_rr + o, part_rz + o, part_bb + o, part_dd + o); break;
     default: resid_item<8>(s, slice0, sv, fresh, sh, part_rr + o, part_rz + o, part_bb + o, part_dd + o); break;
     }
 }
@@ -832,7 +843,10 @@ static void alloc_fp64_vectors(ServedRequest *q, hipStream_t s)
     FFH_HIP(hipMemsetAsync(q->p64.p, 0, sizeof(double) * nr, s));
 }
 
-static int rhs_stride(int nrhs) { return nrhs <= 1 ? 1 : nrhs <= 2 ? 2 : nrhs <= 4 ? 4 : 8; }
+static int rhs_stride(int nrhs)
+{
+    return nrhs <= 1 ? 1 : nrhs <= 2 ? 2 : nrhs <= 4 ? 4 : nrhs <= 6 ? 6 : 8;
+}
 
 Problem *server_prepare(ffh_server *S, int mesh_key, int family, int order, int dim, int nvert,
                         const double *verts, int ncell, const int *cells)
@@ -1088,136 +1102,6 @@ int64_t server_submit(ffh_server *S, int mesh_key, int family, int order, const 
     S->queue.push_back(q.get());
     S->reqs[t] = std::move(q);
     return t;
-}
-
-// ------------------------------------------------------ one mixed solve --
-// FFH_MIXED of solve_sparse / problem_solve: the batch engine with one system
-// and one right-hand side.  SELL FP32 matrix and FP32 r, p, Ap, D^-1; FP64 x
-// and reductions; after every chunk the residual b - A x is recomputed in
-// FP64 with the FP64 matrix and replaces r (reliable update), and it alone
-// decides convergence: ||b - A x|| <= tol ||b - A x0|| (relative) or <= tol
-// (absolute).  A solve whose FP64 residual stops improving (16 refreshes
-// without a 4x drop) continues with FP64 iterations from its current x.
-struct SoloState {
-    BatchEngine eng;
-    int n = -1;
-    DBuf<double> dinv, b, x, r64, p64, ap64;
-    DBuf<float> dinv32, r, p, ap;
-};
-
-void free_solo(SoloState *s) { delete s; }
-
-void solve_mixed(Context *ctx, Matrix &M, const double *b, double *x, const ffh_solver_options &o,
-                 ffh_solver_stats &st)
-{
-    DevMatrix &A = M.m;
-    ensure_val64(ctx, A);
-    FFH_REQUIRE(sell_structure(ctx, A), FFH_ERR_INVALID,
-                "mixed precision needs the SELL-64 form (matrix rows too uneven)");
-    ensure_sell(ctx, A, false);
-    ensure_sell(ctx, A, true);
-    Workspace &w = M.ws;
-    if (!w.solo) w.solo = new SoloState();
-    SoloState &S = *w.solo;
-    const int n = A.n;
-    hipStream_t s = ctx->stream;
-    if (S.n != n) {
-        S.dinv.alloc(n), S.b.alloc(n), S.x.alloc(n);
-        S.dinv32.alloc(n), S.r.alloc(n), S.p.alloc(n), S.ap.alloc(n);
-        S.r64.reset(), S.p64.reset(), S.ap64.reset();
-        S.eng.init(1);
-        S.n = n;
-    }
-    if (ctx->profiling) ctx->prof_begin();
-    FFH_HIP(hipEventRecord(ctx->ev_start, s));
-    const int use = o.method == 1 && o.preconditioner == 1;
-    hipLaunchKernelGGL(k_b_jacobi, dim3((n + 255) / 256), dim3(256), 0, s, n, A.gptr(), A.gidx(),
-                       A.gval(), use, S.dinv.p, S.dinv32.p);
-    FFH_HIP(hipGetLastError());
-    FFH_HIP(hipMemcpyAsync(S.b.p, b, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
-    FFH_HIP(hipMemcpyAsync(S.x.p, x, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
-    FFH_HIP(hipMemsetAsync(S.r.p, 0, sizeof(float) * n, s));
-    FFH_HIP(hipMemsetAsync(S.p.p, 0, sizeof(float) * n, s));
-    for (int r = 0; r < kMaxRhs; ++r) {
-        BatchScal &c = S.eng.h_scal[r];
-        c = BatchScal{};
-        c.state = r == 0 ? kFresh : kUnused;
-        c.max_iter = o.max_iterations;
-        c.tol2 = o.tolerance * o.tolerance;
-        c.flags = o.tolerance_type == 1 ? kAbsTol : 0;
-    }
-    S.eng.push(ctx, 0);
-    auto set_system = [&](int prec) {
-        BatchSys bs{};
-        bs.off = A.sell_off.p;
-        bs.sidx = A.sell_idx.p;
-        bs.nslice = A.nslice;
-        bs.val32 = A.sell_val32.p;
-        bs.val64 = A.sell_val.p;
-        bs.dinv32 = S.dinv32.p;
-        bs.dinv64 = S.dinv.p;
-        bs.b = S.b.p;
-        bs.x = S.x.p;
-        bs.r = S.r.p;
-        bs.p = S.p.p;
-        bs.ap = S.ap.p;
-        bs.r64 = S.r64.p;
-        bs.p64 = S.p64.p;
-        bs.ap64 = S.ap64.p;
-        bs.prec = prec;
-        bs.n = n;
-        bs.R = 1;
-        bs.slot = 0;
-        bs.x_zero = 0;
-        std::vector<BatchSys> sys;
-        std::vector<int2> work;
-        BatchEngine::append(bs, sys, work);
-        S.eng.set(ctx, sys, work);
-    };
-    set_system(0);
-    const int chunk = o.check_interval > 0 ? o.check_interval : 16;
-    int prec = 0, refreshes = 0, stall = 0;
-    double best = 1e300;
-    BatchScal q{};
-    for (;;) {
-        if (o.use_graph)
-            S.eng.run_graph(ctx, chunk);
-        else
-            S.eng.run(ctx, chunk);
-        ++refreshes;
-        S.eng.pull(ctx);
-        q = S.eng.h_scal[0];
-        if (q.state == kConverged || q.state == kStopped) break;
-        if (prec == 0 && q.state == kActive) {
-            if (q.true_rr < 0.25 * best) {
-                best = q.true_rr;
-                stall = 0;
-            } else if (++stall >= 16) {
-                // FP32 iterations stopped making progress: continue in FP64
-                S.r64.alloc(n), S.p64.alloc(n), S.ap64.alloc(n);
-                FFH_HIP(hipMemsetAsync(S.r64.p, 0, sizeof(double) * n, s));
-                FFH_HIP(hipMemsetAsync(S.p64.p, 0, sizeof(double) * n, s));
-                prec = 1;
-                S.eng.h_scal[0].state = kCandidate;
-                S.eng.h_scal[0].restart = 2;
-                S.eng.push(ctx, 0);
-                set_system(1);
-            }
-        }
-    }
-    FFH_HIP(hipMemcpyAsync(x, S.x.p, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
-    FFH_HIP(hipEventRecord(ctx->ev_stop, s));
-    FFH_HIP(hipStreamSynchronize(s));
-    float ms = 0.0f;
-    FFH_HIP(hipEventElapsedTime(&ms, ctx->ev_start, ctx->ev_stop));
-    if (ctx->profiling) ctx->prof_end();
-    st.converged = q.state == kConverged;
-    st.iterations = q.iters;
-    st.final_residual = std::sqrt(q.true_rr);
-    st.true_residual = std::sqrt(q.true_rr);
-    st.initial_residual = std::sqrt(q.init_rr);
-    st.solve_time = ms * 1e-3;
-    st.refinements = refreshes;
 }
 
 } // namespace ffh

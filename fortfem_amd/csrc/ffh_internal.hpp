@@ -78,17 +78,22 @@ struct DBuf {
     }
 };
 
-// Control block of one Krylov solve, resident in device memory.  Written
-// only by the last block of each reduction kernel.
+// Control block of one Krylov solve, resident in device memory.  Each field
+// has one writer per launch (block 0 of the launch that owns it), and no
+// other block of that launch reads it.
 struct KrylovCtl {
     double rzs[2], pap, alpha, beta, rr, rz;
     double residual_norm, initial_norm, tol;
     int iter, max_iter, converged, done, breakdown_hit;
-    int breakdown_mode, tol_type, pad0;
+    int breakdown_mode, tol_type, mixed;
     double tolerance;
-    unsigned int counter[4];
-    int error;
-    int pad1;
+    double true_rr;  // FP64 ||b - A x||^2 (final report, mixed refreshes)
+    int k2_done;     // x/r updates completed (written by the update launch)
+    int cur;         // r.z slot of the current iteration (direction launch)
+    int restart;     // the next direction is D r (init, refresh, breakdown)
+    int nb_rz;       // partial count of the last r.r / r.z producer
+    int refreshes;   // mixed: FP64 residual refreshes so far
+    int pad0;
 };
 
 // ---------------------------------------------------------------- matrix --
@@ -134,19 +139,17 @@ struct DevMatrix {
     }
 };
 
-struct SoloState; // batch-engine state of the mixed-precision solve (ffh_batch.hip)
-
 // Solver workspace attached to a matrix (kept resident between solves).
 struct Workspace {
     int n = 0;
-    DBuf<double> r, p, ap, x, b, dinv;
-    DBuf<float> r32, p32, ap32, x32, dinv32;
+    DBuf<double> r, p, p2, ap, x, b, dinv;
+    DBuf<float> r32, p32, p32b, ap32, x32, dinv32;
     DBuf<double> partials;
     DBuf<KrylovCtl> ctl;
-    KrylovCtl *host_ctl = nullptr; // pinned mailbox
+    KrylovCtl *host_ctl = nullptr; // pinned mailboxes (two, polled alternately)
+    hipEvent_t ctl_ev[2] = {nullptr, nullptr};
     hipGraphExec_t graph[2] = {nullptr, nullptr};
     int graph_chunk[2] = {0, 0};
-    SoloState *solo = nullptr;
     ~Workspace();
 };
 

@@ -75,25 +75,86 @@ __device__ __forceinline__ bool is_breakdown(double pap, double rz, int mode)
     return fabs(pap) < 1.0e-14;
 }
 
+// ----------------------------------------------------- XCD-aware blocks --
+// Workgroups are dispatched to the 8 XCDs round-robin (XCD = blockIdx % 8)
+// and each XCD has its own L2.  The logical block index gives every XCD one
+// contiguous range of blocks (so of rows), so the vector entries an XCD's
+// rows gather stay in its own L2 instead of every L2 holding all of them.
+constexpr int kXcds = 8;
+__device__ __forceinline__ int xcd_block()
+{
+    const int g = gridDim.x, b = blockIdx.x;
+    const int per = g / kXcds, rem = g % kXcds;
+    const int x = b % kXcds, i = b / kXcds;
+    return x * per + min(x, rem) + i;
+}
+
+// ------------------------------------------------------------- gathers --
+// What a row engine multiplies the matrix entries by, in two steps so that
+// the loads can be issued before the scalars they are combined with are
+// known: raw(c) loads column c's data, val(raw) forms the value.  XPlain is a
+// plain vector.  XDir forms the new CG direction on the fly,
+// p_new = z + beta p_old with z = D r (p_new = z at a restart), so that the
+// direction update needs no launch of its own; z and p_old are interleaved
+// (one load per column).  The owner of row i stores p_new[i] from the same
+// expression, so a gathered value and the stored one are the same number.
+__device__ __forceinline__ double fma_t(double a, double b, double c) { return fma(a, b, c); }
+__device__ __forceinline__ float fma_t(float a, float b, float c) { return fmaf(a, b, c); }
+
+template <typename VT>
+struct XPlain {
+    using Raw = VT;
+    const VT *__restrict__ x;
+    __device__ __forceinline__ Raw raw(int c) const { return x[c]; }
+    __device__ __forceinline__ VT val(Raw v) const { return v; }
+    __device__ __forceinline__ VT operator()(int c) const { return x[c]; }
+};
+
+template <typename VT>
+struct alignas(2 * sizeof(VT)) ZP {
+    VT z, p;
+};
+
+template <typename VT>
+struct XDir {
+    using Raw = ZP<VT>;
+    const ZP<VT> *__restrict__ zp;
+    VT beta;
+    int restart;
+    __device__ __forceinline__ Raw raw(int c) const { return zp[c]; }
+    __device__ __forceinline__ VT val(Raw v) const
+    {
+        return restart ? v.z : fma_t(beta, v.p, v.z);
+    }
+    __device__ __forceinline__ VT operator()(int c) const { return val(zp[c]); }
+};
+
+// The start hook of an engine: called once by every thread of the block,
+// after the block's first loads are issued and before any value is formed;
+// false stops the block.  NoHook for plain products.
+struct NoHook {
+    __device__ __forceinline__ bool operator()() const { return true; }
+};
+
 // ------------------------------------------------------------ CSR stream --
 // A row block is a run of consecutive rows whose nonzeros fit in LDS.  The
 // block streams the run's values and column indices with fully coalesced
-// loads (each lane takes every BS-th nonzero), multiplies by the gathered x
-// into LDS, then one thread per row adds its row's products in column order.
-// A row longer than the LDS stage is summed by the whole block.
-// Epilogue E receives (row, A x row value) and accumulates up to two partial
-// sums.
-template <typename MT, typename VT, typename E>
+// loads (each lane takes every BS-th nonzero), multiplies by the gathered
+// values into LDS, then one thread per row adds its row's products in column
+// order.  A row longer than the LDS stage is summed by the whole block.
+// Epilogue E receives (row, row value) and accumulates up to three partial
+// sums in d.
+template <typename MT, typename VT, typename XF, typename E, typename H>
 __device__ __forceinline__ void csr_stream(const int *__restrict__ ptr,
                                            const int *__restrict__ idx,
-                                           const MT *__restrict__ val,
-                                           const VT *__restrict__ x,
+                                           const MT *__restrict__ val, const XF &xf,
                                            const int *__restrict__ rowblocks, int nrb,
-                                           E &epi, double &d0, double &d1)
+                                           const E &epi, double *d, const H &hook)
 {
     constexpr int BS = kStreamBlock, NZ = kStreamNnz, U = NZ / BS;
     __shared__ VT prod[NZ];
-    for (int blk = blockIdx.x; blk < nrb; blk += gridDim.x) {
+    bool hooked = false;
+    for (int blk = xcd_block(); blk < nrb; blk += gridDim.x) {
         const int r0 = rowblocks[blk], r1 = rowblocks[blk + 1];
         const int k0 = ptr[r0], k1 = ptr[r1];
         const int cnt = k1 - k0;
@@ -108,10 +169,17 @@ __device__ __forceinline__ void csr_stream(const int *__restrict__ ptr,
                     v[u] = __builtin_nontemporal_load(val + k0 + k);
                 }
             }
+            // the first index/value loads are in flight during the hook
+            // (holding the gathered values across it as well costs more
+            // registers than it saves)
+            if (!hooked) {
+                if (!hook()) return;
+                hooked = true;
+            }
 #pragma unroll
             for (int u = 0; u < U; ++u) {
                 const int k = threadIdx.x + u * BS;
-                if (k < cnt) prod[k] = (VT)v[u] * x[c[u]];
+                if (k < cnt) prod[k] = (VT)v[u] * xf(c[u]);
             }
             __syncthreads();
             const int nrows = r1 - r0;
@@ -120,7 +188,7 @@ __device__ __forceinline__ void csr_stream(const int *__restrict__ ptr,
                     const int b = ptr[r] - k0, e = ptr[r + 1] - k0;
                     VT s = VT(0);
                     for (int k = b; k < e; ++k) s += prod[k];
-                    epi(r, s, d0, d1);
+                    epi(r, s, d);
                 }
             } else {
                 // gs lanes per row (a power of two, 2..64, gs * nrows <= BS):
@@ -135,38 +203,43 @@ __device__ __forceinline__ void csr_stream(const int *__restrict__ ptr,
                     for (int k = b + lane; k < e; k += gs) s += prod[k];
                 }
                 for (int off = gs >> 1; off > 0; off >>= 1) s += __shfl_xor(s, off, gs);
-                if (lane == 0 && r < r1) epi(r, s, d0, d1);
+                if (lane == 0 && r < r1) epi(r, s, d);
             }
             __syncthreads();
         } else {
             // one long row (r1 == r0 + 1)
+            if (!hooked) {
+                if (!hook()) return;
+                hooked = true;
+            }
             VT s = VT(0);
-            for (int k = k0 + threadIdx.x; k < k1; k += BS) s += (VT)val[k] * x[idx[k]];
+            for (int k = k0 + threadIdx.x; k < k1; k += BS) s += (VT)val[k] * xf(idx[k]);
             double t = block_sum<BS>((double)s);
             __shared__ double tot;
             if (threadIdx.x == 0) tot = t;
             __syncthreads();
-            if (threadIdx.x == 0) epi(r0, (VT)tot, d0, d1);
+            if (threadIdx.x == 0) epi(r0, (VT)tot, d);
             __syncthreads();
         }
     }
+    if (!hooked) (void)hook();
 }
 
 // Vector CSR: a group of G lanes per row (grid-stride over rows); each lane
 // issues two independent (index, value) loads before its gathers, then the
 // group reduces by shuffles.  Used when it beats the stream kernel for a
 // matrix (chosen by the autotuner below).
-template <int G, typename MT, typename VT, typename E>
+template <int G, typename MT, typename VT, typename XF, typename E, typename H>
 __device__ __forceinline__ void csr_vector(int n, const int *__restrict__ ptr,
                                            const int *__restrict__ idx,
-                                           const MT *__restrict__ val,
-                                           const VT *__restrict__ x, E &epi, double &d0,
-                                           double &d1)
+                                           const MT *__restrict__ val, const XF &xf,
+                                           const E &epi, double *d, const H &hook)
 {
+    if (!hook()) return;
     constexpr int RPB = kStreamBlock / G;
     const int lane = threadIdx.x % G;
     const int ngrp = gridDim.x * RPB;
-    for (int row = blockIdx.x * RPB + threadIdx.x / G; row < n; row += ngrp) {
+    for (int row = xcd_block() * RPB + threadIdx.x / G; row < n; row += ngrp) {
         const int b = ptr[row], e = ptr[row + 1];
         VT s = VT(0);
         int k = b + lane;
@@ -175,136 +248,153 @@ __device__ __forceinline__ void csr_vector(int n, const int *__restrict__ ptr,
             const int c1 = __builtin_nontemporal_load(idx + k + G);
             const MT v0 = __builtin_nontemporal_load(val + k);
             const MT v1 = __builtin_nontemporal_load(val + k + G);
-            s += (VT)v0 * x[c0];
-            s += (VT)v1 * x[c1];
+            s += (VT)v0 * xf(c0);
+            s += (VT)v1 * xf(c1);
         }
-        if (k < e) s += (VT)val[k] * x[idx[k]];
+        if (k < e) s += (VT)val[k] * xf(idx[k]);
 #pragma unroll
         for (int off = G / 2; off > 0; off >>= 1) s += __shfl_xor(s, off, G);
-        if (lane == 0) epi(row, s, d0, d1);
+        if (lane == 0) epi(row, s, d);
     }
 }
 
 // SELL-64: one wave per slice of 64 rows, one lane per row.  Every index
 // and value load of a wave reads 64 consecutive words; each lane adds its
 // row's products in column order (the CSR order).  Entries go in steps of
-// eight: the sixteen index/value loads of a step are issued together, then
-// the eight gathers, so a wave has many loads in flight.  Here ptr is the
+// eight: the sixteen index/value loads of a step and its eight gathers are
+// issued together, so a wave has many loads in flight.  Here ptr is the
 // slice offset array, idx/val the SELL entries and nrb the slice count.
-template <typename MT, typename VT, typename E>
+template <typename MT, typename VT, typename XF, typename E, typename H>
 __device__ __forceinline__ void sell_rows(int n, const int *__restrict__ off,
                                           const int *__restrict__ sidx,
-                                          const MT *__restrict__ sval,
-                                          const VT *__restrict__ x, int nslice, E &epi,
-                                          double &d0, double &d1)
+                                          const MT *__restrict__ sval, const XF &xf,
+                                          int nslice, const E &epi, double *d, const H &hook)
 {
     constexpr int WPB = kStreamBlock / 64, U = 8;
     const int lane = threadIdx.x & 63;
-    for (int sl = blockIdx.x * WPB + (int)(threadIdx.x >> 6); sl < nslice;
-         sl += gridDim.x * WPB) {
-        const int o = off[sl];
-        const int w = (off[sl + 1] - o) >> 6;
+    bool hooked = false;
+    // all waves of the block call the hook at the same point (it may
+    // synchronise the block): the loop runs the same trip count in every
+    // wave of the block, idle waves included
+    const int first = xcd_block() * WPB;
+    for (int base = first; base < nslice || !hooked; base += gridDim.x * WPB) {
+        const int sl = base + (int)(threadIdx.x >> 6);
+        const bool live = sl < nslice;
+        const int o = live ? off[sl] : 0;
+        const int w = live ? (off[sl + 1] - o) >> 6 : 0;
         const int *ip = sidx + o + lane;
         const MT *vp = sval + o + lane;
         VT s = VT(0);
         int j = 0;
-        for (; j + U <= w; j += U) {
+        for (; j < w || !hooked; j += U) {
             int c[U];
             MT v[U];
-#pragma unroll
-            for (int u = 0; u < U; ++u) {
-                c[u] = __builtin_nontemporal_load(ip + 64 * u);
-                v[u] = __builtin_nontemporal_load(vp + 64 * u);
-            }
-            VT xv[U];
-#pragma unroll
-            for (int u = 0; u < U; ++u) xv[u] = x[c[u]];
-#pragma unroll
-            for (int u = 0; u < U; ++u) s += (VT)v[u] * xv[u];
-            ip += 64 * U;
-            vp += 64 * U;
-        }
-        if (j < w) {
-            // the last w - j (< U) entries, loads first
-            int c[U];
-            MT v[U];
+            typename XF::Raw q[U];
 #pragma unroll
             for (int u = 0; u < U; ++u)
                 if (j + u < w) {
                     c[u] = __builtin_nontemporal_load(ip + 64 * u);
                     v[u] = __builtin_nontemporal_load(vp + 64 * u);
                 }
-            VT xv[U];
 #pragma unroll
             for (int u = 0; u < U; ++u)
-                if (j + u < w) xv[u] = x[c[u]];
+                if (j + u < w) q[u] = xf.raw(c[u]);
+            if (!hooked) {
+                if (!hook()) return;
+                hooked = true;
+            }
 #pragma unroll
             for (int u = 0; u < U; ++u)
-                if (j + u < w) s += (VT)v[u] * xv[u];
+                if (j + u < w) s += (VT)v[u] * xf.val(q[u]);
+            ip += 64 * U;
+            vp += 64 * U;
         }
         const int row = sl * 64 + lane;
-        if (row < n) epi(row, s, d0, d1);
+        if (live && row < n) epi(row, s, d);
     }
 }
 
 // Row engine of a launch: V = 0 stream, V = 1 SELL-64, otherwise vector CSR
 // with G = V lanes.
-template <int V, typename MT, typename VT, typename E>
+template <int V, typename MT, typename VT, typename XF, typename E, typename H = NoHook>
 __device__ __forceinline__ void row_engine(int n, const int *__restrict__ ptr,
                                            const int *__restrict__ idx,
-                                           const MT *__restrict__ val,
-                                           const VT *__restrict__ x,
-                                           const int *__restrict__ rowblocks, int nrb, E &epi,
-                                           double &d0, double &d1)
+                                           const MT *__restrict__ val, const XF &xf,
+                                           const int *__restrict__ rowblocks, int nrb,
+                                           const E &epi, double *d, const H &hook = H())
 {
     if constexpr (V == 0)
-        csr_stream<MT, VT>(ptr, idx, val, x, rowblocks, nrb, epi, d0, d1);
+        csr_stream<MT, VT>(ptr, idx, val, xf, rowblocks, nrb, epi, d, hook);
     else if constexpr (V == 1)
-        sell_rows<MT, VT>(n, ptr, idx, val, x, nrb, epi, d0, d1);
+        sell_rows<MT, VT>(n, ptr, idx, val, xf, nrb, epi, d, hook);
     else
-        csr_vector<V, MT, VT>(n, ptr, idx, val, x, epi, d0, d1);
+        csr_vector<V, MT, VT>(n, ptr, idx, val, xf, epi, d, hook);
 }
 
+// ------------------------------------------------------------ epilogues --
 template <typename VT>
-struct EpiStore {
+struct EpiStore { // y = A x
     VT *y;
-    __device__ void operator()(int r, VT s, double &, double &) const { y[r] = s; }
+    __device__ void operator()(int r, VT s, double *) const { y[r] = s; }
 };
 
 template <typename VT>
-struct EpiAp { // ap = A p ; p.Ap
-    const VT *p;
-    VT *ap;
-    __device__ void operator()(int r, VT s, double &d0, double &) const
+struct EpiInit { // r = b - A x ; z = D r ; r.r, r.z
+    const VT *b, *dinv;
+    VT *r;
+    ZP<VT> *zp;
+    __device__ void operator()(int i, VT s, double *d) const
     {
-        ap[r] = s;
-        d0 += (double)p[r] * (double)s;
+        const VT ri = b[i] - s;
+        const VT zi = dinv[i] * ri;
+        r[i] = ri;
+        zp[i].z = zi;
+        d[0] += (double)ri * (double)ri;
+        d[1] += (double)ri * (double)zi;
     }
 };
 
 template <typename VT>
-struct EpiInit { // r = b - A x ; z = D r ; p = z ; r.r, r.z
-    const VT *b, *dinv;
-    VT *r, *p;
-    __device__ void operator()(int i, VT s, double &d0, double &d1) const
+struct EpiDirAp { // p_new(i) stored ; Ap = A p_new ; p_new.Ap
+    const XDir<VT> *g;
+    ZP<VT> *zpnew;
+    VT *ap;
+    __device__ void operator()(int i, VT s, double *d) const
     {
-        VT ri = b[i] - s;
-        VT zi = dinv[i] * ri;
-        r[i] = ri;
-        p[i] = zi;
-        d0 += (double)ri * (double)ri;
-        d1 += (double)ri * (double)zi;
+        const VT pi = (*g)(i);
+        zpnew[i] = ZP<VT>{VT(0), pi}; // z is written by the update launch
+        ap[i] = s;
+        d[0] += (double)pi * (double)s;
     }
 };
 
 struct EpiResidual { // r = b - A x ; r.r
     const double *b;
     double *r;
-    __device__ void operator()(int i, double s, double &d0, double &) const
+    __device__ void operator()(int i, double s, double *d) const
     {
-        double ri = b[i] - s;
+        const double ri = b[i] - s;
         r[i] = ri;
-        d0 += ri * ri;
+        d[0] += ri * ri;
+    }
+};
+
+struct EpiRefresh { // mixed: FP64 residual replaces the FP32 one ; z ; rr, rz, drift
+    const double *b;
+    const float *dinv32;
+    float *r32;
+    ZP<float> *zp;
+    __device__ void operator()(int i, double s, double *d) const
+    {
+        const double ri = b[i] - s;
+        const double dr = (double)r32[i] - ri;
+        const float r = (float)ri;
+        const float z = dinv32[i] * r;
+        r32[i] = r;
+        zp[i].z = z;
+        d[0] += ri * ri;
+        d[1] += (double)r * (double)z;
+        d[2] += dr * dr;
     }
 };
 
@@ -319,73 +409,158 @@ __global__ __launch_bounds__(kStreamBlock) void k_spmv(int n, const int *__restr
                                                        int nrb)
 {
     EpiStore<VT> epi{y};
-    double d0 = 0.0, d1 = 0.0;
-    row_engine<V, MT, VT>(n, ptr, idx, val, x, rowblocks, nrb, epi, d0, d1);
+    double d[3] = {0.0, 0.0, 0.0};
+    row_engine<V, MT, VT>(n, ptr, idx, val, XPlain<VT>{x}, rowblocks, nrb, epi, d);
 }
 
-// r = b - A x ; z = D r ; p = z ; partials of r.r and r.z
+// Partial-sum slots (stride apart): 0 p.Ap, 1 r.r, 2 r.(D r), 3 drift.
+//
+// r = b - A x ; z = D r into zp0 ; partials of r.r and r.z
+// (initialize_pcg_state :859-878)
 template <int V, typename MT, typename VT>
 __global__ __launch_bounds__(kStreamBlock) void k_pcg_init(
     int n, const int *__restrict__ ptr, const int *__restrict__ idx, const MT *__restrict__ val,
     const VT *__restrict__ x, const VT *__restrict__ b, VT *__restrict__ r,
-    VT *__restrict__ p, const VT *__restrict__ dinv, const int *__restrict__ rowblocks,
+    const VT *__restrict__ dinv, ZP<VT> *__restrict__ zp0, const int *__restrict__ rowblocks,
     int nrb, double *__restrict__ partials, int stride)
 {
-    EpiInit<VT> epi{b, dinv, r, p};
-    double d0 = 0.0, d1 = 0.0;
-    row_engine<V, MT, VT>(n, ptr, idx, val, x, rowblocks, nrb, epi, d0, d1);
-    const double s0 = block_sum<kStreamBlock>(d0);
-    const double s1 = block_sum<kStreamBlock>(d1);
+    EpiInit<VT> epi{b, dinv, r, zp0};
+    double d[3] = {0.0, 0.0, 0.0};
+    row_engine<V, MT, VT>(n, ptr, idx, val, XPlain<VT>{x}, rowblocks, nrb, epi, d);
+    const double s0 = block_sum<kStreamBlock>(d[0]);
+    const double s1 = block_sum<kStreamBlock>(d[1]);
     if (threadIdx.x == 0) {
         partials[stride + blockIdx.x] = s0;
         partials[2 * stride + blockIdx.x] = s1;
     }
 }
 
-// initial norm, tolerance (compute_solver_tolerance, :1063-1073), r.z
+// initial norm and tolerance (compute_solver_tolerance, :1063-1073); the
+// first direction is D r
 __global__ __launch_bounds__(kBlock) void k_pcg_init_finish(const double *__restrict__ partials,
                                                             int stride, int nb, KrylovCtl *ctl)
 {
     const double trr = sum_partials<kBlock>(partials + stride, nb);
-    const double trz = sum_partials<kBlock>(partials + 2 * stride, nb);
     if (threadIdx.x == 0) {
         const double init = sqrt(trr);
         ctl->rr = trr;
         ctl->initial_norm = init;
         ctl->residual_norm = init;
         ctl->tol = ctl->tol_type == 1 ? ctl->tolerance : ctl->tolerance * init;
-        ctl->rzs[0] = trz;
         ctl->iter = 0;
+        ctl->k2_done = 0;
+        ctl->restart = 1;
+        ctl->nb_rz = nb;
         ctl->done = ctl->max_iter <= 0 ? 1 : 0;
     }
 }
 
-// Ap = A p ; partials of p.Ap
+// The tail of an iteration, run by every block of the next launch from the
+// r.r / r.(D r) partials of the last update (or initialisation or refresh):
+// the stop test on the recursive residual, then beta = rz_new / rz_old
+// (pcg_iteration_loop :906-922).  Block 0 records it; the records are the
+// same however often it runs for one update.  True when the solve stops.
+template <int BS>
+__device__ __forceinline__ bool pcg_finish(const double *__restrict__ partials, int stride,
+                                           KrylovCtl *ctl, double &beta)
+{
+    const int nb = ctl->nb_rz;
+    const double trr = sum_partials<BS>(partials + stride, nb);
+    const double trz = sum_partials<BS>(partials + 2 * stride, nb);
+    const int done_it = ctl->k2_done;
+    const int par = done_it & 1;
+    const bool restart = ctl->restart != 0;
+    const double rn = sqrt(trr);
+    bool stop = false, conv = false;
+    if (!restart) {
+        conv = !ctl->mixed && rn <= ctl->tol;
+        stop = conv || done_it >= ctl->max_iter;
+    }
+    beta = restart ? 0.0 : trz / ctl->rzs[par];
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        ctl->iter = done_it;
+        if (!restart) {
+            ctl->rr = trr;
+            ctl->residual_norm = rn;
+        }
+        ctl->rzs[par ^ 1] = trz;
+        ctl->beta = beta;
+        ctl->cur = par ^ 1;
+        if (stop) {
+            ctl->done = 1;
+            if (conv) ctl->converged = 1;
+        }
+    }
+    return stop;
+}
+
+// the tail of the previous iteration as a start hook: beta into the gather
+template <typename VT>
+struct DirHook {
+    const double *partials;
+    int stride;
+    KrylovCtl *ctl;
+    XDir<VT> *g;
+    __device__ __forceinline__ bool operator()() const
+    {
+        double beta;
+        if (pcg_finish<kStreamBlock>(partials, stride, ctl, beta)) return false;
+        g->beta = (VT)beta;
+        return true;
+    }
+};
+
+// Launch 1 of an iteration: the tail of the previous one, then
+// p_new = z + beta p_old (own rows stored, neighbours formed on the fly),
+// Ap = A p_new and the partials of p_new.Ap.  (z, p) pairs alternate between
+// zp0 and zp1: the update's z and the previous direction are read from slot
+// k2_done & 1, the new direction is written to the other slot.
 template <int V, typename MT, typename VT>
-__global__ __launch_bounds__(kStreamBlock) void k_pcg_spmv(
+__global__ __launch_bounds__(kStreamBlock) void k_pcg_dirspmv(
     int n, const int *__restrict__ ptr, const int *__restrict__ idx, const MT *__restrict__ val,
-    const VT *__restrict__ p, VT *__restrict__ ap, const int *__restrict__ rowblocks,
-    int nrb, double *__restrict__ partials, const KrylovCtl *ctl)
+    ZP<VT> *__restrict__ zp0, ZP<VT> *__restrict__ zp1, VT *__restrict__ ap,
+    const int *__restrict__ rowblocks, int nrb, double *__restrict__ partials, int stride,
+    KrylovCtl *ctl)
 {
     if (ctl->done) return;
-    EpiAp<VT> epi{p, ap};
-    double d0 = 0.0, d1 = 0.0;
-    row_engine<V, MT, VT>(n, ptr, idx, val, p, rowblocks, nrb, epi, d0, d1);
-    const double s0 = block_sum<kStreamBlock>(d0);
+    const int par = ctl->k2_done & 1;
+    XDir<VT> g{par ? zp1 : zp0, VT(0), ctl->restart};
+    DirHook<VT> hook{partials, stride, ctl, &g};
+    EpiDirAp<VT> epi{&g, par ? zp0 : zp1, ap};
+    double d[3] = {0.0, 0.0, 0.0};
+    // the block's first matrix and (z, p) loads are in flight while the
+    // hook sums the previous partials
+    row_engine<V, MT, VT>(n, ptr, idx, val, g, rowblocks, nrb, epi, d, hook);
+    const double s0 = block_sum<kStreamBlock>(d[0]);
     if (threadIdx.x == 0) partials[blockIdx.x] = s0;
 }
 
-// p.Ap -> alpha (or breakdown); x += alpha p ; r -= alpha Ap ; partials of
-// r.r and r.(D r)
-template <typename VT>
+// Launch 2: p.Ap -> alpha (or breakdown, :897-899); x += alpha p ;
+// r -= alpha Ap ; z = D r next to p ; partials of r.r and r.z.  x may be
+// FP64 while the iteration vectors are FP32 (mixed precision).
+template <typename VT, typename XT>
 __global__ __launch_bounds__(kBlock) void k_pcg_update(
-    int n, VT *__restrict__ x, VT *__restrict__ r, const VT *__restrict__ p,
-    const VT *__restrict__ ap, const VT *__restrict__ dinv,
-    double *__restrict__ partials, int stride, int nb_spmv, int parity, KrylovCtl *ctl)
+    int n, XT *__restrict__ x, VT *__restrict__ r, ZP<VT> *__restrict__ zp0,
+    ZP<VT> *__restrict__ zp1, const VT *__restrict__ ap, const VT *__restrict__ dinv,
+    double *__restrict__ partials, int stride, int nb_spmv, KrylovCtl *ctl)
 {
     if (ctl->done) return;
+    const int cur = ctl->cur;
+    ZP<VT> *zp = cur ? zp1 : zp0;
+    // this thread's first element, loaded while the partials are summed
+    const int i0 = xcd_block() * kBlock + threadIdx.x;
+    XT x0 = XT(0);
+    VT r0 = VT(0), a0 = VT(0), d0 = VT(0);
+    ZP<VT> q0{};
+    if (i0 < n) {
+        x0 = x[i0];
+        r0 = r[i0];
+        a0 = ap[i0];
+        d0 = dinv[i0];
+        q0 = zp[i0];
+    }
     const double pap = sum_partials<kBlock>(partials, nb_spmv);
-    const double rz_old = ctl->rzs[parity];
+    const double rz_old = ctl->rzs[cur];
     if (is_breakdown(pap, rz_old, ctl->breakdown_mode)) {
         if (blockIdx.x == 0 && threadIdx.x == 0) {
             ctl->pap = pap;
@@ -398,14 +573,29 @@ __global__ __launch_bounds__(kBlock) void k_pcg_update(
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         ctl->pap = pap;
         ctl->alpha = alpha_d;
+        ctl->k2_done = ctl->k2_done + 1;
+        ctl->restart = 0;
+        ctl->nb_rz = gridDim.x;
     }
     const VT alpha = (VT)alpha_d;
+    const XT alpha_x = (XT)alpha_d;
     double rr = 0.0, rz = 0.0;
-    for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
-        x[i] = x[i] + alpha * p[i];
-        VT ri = r[i] - alpha * ap[i];
+    if (i0 < n) {
+        x[i0] = x0 + alpha_x * (XT)q0.p;
+        const VT ri = r0 - alpha * a0;
+        r[i0] = ri;
+        const VT zi = d0 * ri;
+        zp[i0] = ZP<VT>{zi, q0.p};
+        rr += (double)ri * (double)ri;
+        rz += (double)ri * (double)zi;
+    }
+    for (int i = i0 + gridDim.x * kBlock; i < n; i += gridDim.x * kBlock) {
+        const ZP<VT> q = zp[i];
+        x[i] = x[i] + alpha_x * (XT)q.p;
+        const VT ri = r[i] - alpha * ap[i];
         r[i] = ri;
-        VT zi = dinv[i] * ri;
+        const VT zi = dinv[i] * ri;
+        zp[i] = ZP<VT>{zi, q.p};
         rr += (double)ri * (double)ri;
         rz += (double)ri * (double)zi;
     }
@@ -417,48 +607,24 @@ __global__ __launch_bounds__(kBlock) void k_pcg_update(
     }
 }
 
-// stop test on ||r|| ; beta = rz_new / rz_old ; p = D r + beta p
-template <typename VT>
-__global__ __launch_bounds__(kBlock) void k_pcg_dir(int n, VT *__restrict__ p,
-                                                    const VT *__restrict__ r,
-                                                    const VT *__restrict__ dinv,
-                                                    const double *__restrict__ partials,
-                                                    int stride, int nb_update, int parity,
-                                                    KrylovCtl *ctl)
+// end of a chunk: the tail of the last iteration (stop test) for the host
+__global__ __launch_bounds__(kBlock) void k_pcg_fin(const double *__restrict__ partials,
+                                                    int stride, KrylovCtl *ctl)
 {
     if (ctl->done) return;
-    const double trr = sum_partials<kBlock>(partials + stride, nb_update);
-    const double trz = sum_partials<kBlock>(partials + 2 * stride, nb_update);
-    const double rn = sqrt(trr);
-    const bool conv = rn <= ctl->tol;
-    const double rz_old = ctl->rzs[parity];
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        const int it = ctl->iter + 1;
-        ctl->iter = it;
-        ctl->rr = trr;
-        ctl->residual_norm = rn;
-        if (conv) {
-            ctl->converged = 1;
-            ctl->done = 1;
-        } else {
-            ctl->rzs[parity ^ 1] = trz;
-            ctl->beta = trz / rz_old;
-            if (it >= ctl->max_iter) ctl->done = 1;
-        }
-    }
-    if (conv) return;
-    const VT beta = (VT)(trz / rz_old);
-    for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock)
-        p[i] = dinv[i] * r[i] + beta * p[i];
+    double beta;
+    (void)pcg_finish<kBlock>(partials, stride, ctl, beta);
 }
 
 // Jacobi diagonal (build_sparse_preconditioner 'jacobi', :1130-1144):
 // first entry of row i with column i; 1/a_ii if |a_ii| > 1e-14 else 1.
+// dinv32 (optional) receives the FP32 copy.
 template <typename VT>
 __global__ __launch_bounds__(kBlock) void k_jacobi(int n, const int *__restrict__ ptr,
                                                    const int *__restrict__ idx,
                                                    const double *__restrict__ val,
-                                                   int use, VT *__restrict__ dinv)
+                                                   int use, VT *__restrict__ dinv,
+                                                   float *__restrict__ dinv32)
 {
     for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
         double a = 0.0;
@@ -467,12 +633,14 @@ __global__ __launch_bounds__(kBlock) void k_jacobi(int n, const int *__restrict_
                 a = val[k];
                 break;
             }
-        dinv[i] = (use && fabs(a) > 1.0e-14) ? (VT)(1.0 / a) : (VT)1;
+        const double di = (use && fabs(a) > 1.0e-14) ? 1.0 / a : 1.0;
+        dinv[i] = (VT)di;
+        if (dinv32) dinv32[i] = (float)di;
     }
 }
 
 // r = b - A x (FP64) and partials of ||r||^2 -- the true residual of the
-// refinement loop and of the final FP64 report.
+// final FP64 report.
 template <int V>
 __global__ __launch_bounds__(kStreamBlock) void k_residual(
     int n, const int *__restrict__ ptr, const int *__restrict__ idx,
@@ -481,9 +649,9 @@ __global__ __launch_bounds__(kStreamBlock) void k_residual(
     const int *__restrict__ rowblocks, int nrb, double *__restrict__ partials)
 {
     EpiResidual epi{b, r};
-    double d0 = 0.0, d1 = 0.0;
-    row_engine<V, double, double>(n, ptr, idx, val, x, rowblocks, nrb, epi, d0, d1);
-    const double s0 = block_sum<kStreamBlock>(d0);
+    double d[3] = {0.0, 0.0, 0.0};
+    row_engine<V, double, double>(n, ptr, idx, val, XPlain<double>{x}, rowblocks, nrb, epi, d);
+    const double s0 = block_sum<kStreamBlock>(d[0]);
     if (threadIdx.x == 0) partials[blockIdx.x] = s0;
 }
 
@@ -491,7 +659,71 @@ __global__ __launch_bounds__(kBlock) void k_residual_finish(const double *__rest
                                                             int nb, KrylovCtl *ctl)
 {
     const double t = sum_partials<kBlock>(partials, nb);
-    if (threadIdx.x == 0) ctl->rr = t;
+    if (threadIdx.x == 0) ctl->true_rr = t;
+}
+
+// Mixed precision, reliable update: the FP64 residual b - A x (FP64 matrix
+// and x) replaces the FP32 recurrence residual; partials of its r.r and
+// r.(D r) go where the next direction launch reads them, with the drift
+// ||r_recurrence - r||^2.
+template <int V>
+__global__ __launch_bounds__(kStreamBlock) void k_mixed_refresh(
+    int n, const int *__restrict__ ptr, const int *__restrict__ idx,
+    const double *__restrict__ val, const double *__restrict__ x,
+    const double *__restrict__ b, float *__restrict__ r32, const float *__restrict__ dinv32,
+    ZP<float> *__restrict__ zp0, ZP<float> *__restrict__ zp1, const int *__restrict__ rowblocks,
+    int nrb, double *__restrict__ partials, int stride, const KrylovCtl *ctl)
+{
+    if (ctl->converged) return;
+    // z goes where the next direction launch reads it (slot k2_done & 1)
+    EpiRefresh epi{b, dinv32, r32, (ctl->k2_done & 1) ? zp1 : zp0};
+    double d[3] = {0.0, 0.0, 0.0};
+    row_engine<V, double, double>(n, ptr, idx, val, XPlain<double>{x}, rowblocks, nrb, epi, d);
+    const double s0 = block_sum<kStreamBlock>(d[0]);
+    const double s1 = block_sum<kStreamBlock>(d[1]);
+    const double s2 = block_sum<kStreamBlock>(d[2]);
+    if (threadIdx.x == 0) {
+        partials[stride + blockIdx.x] = s0;
+        partials[2 * stride + blockIdx.x] = s1;
+        partials[3 * stride + blockIdx.x] = s2;
+    }
+}
+
+// Mixed precision: convergence on the FP64 residual (||r|| <= tol, with tol
+// from the first one, :1063-1073), the iteration cap, and whether the
+// direction is kept (the recurrence tracked the FP64 residual to 1 %) or CG
+// restarts from D r (larger drift, a breakdown, the first refresh).
+__global__ __launch_bounds__(kBlock) void k_mixed_refresh_finish(const double *__restrict__ partials,
+                                                                 int stride, int nb, int first,
+                                                                 KrylovCtl *ctl)
+{
+    if (ctl->converged) return;
+    const double trr = sum_partials<kBlock>(partials + stride, nb);
+    const double tdd = sum_partials<kBlock>(partials + 3 * stride, nb);
+    if (threadIdx.x == 0) {
+        const double rn = sqrt(trr);
+        if (first) {
+            ctl->initial_norm = rn;
+            ctl->tol = ctl->tol_type == 1 ? ctl->tolerance : ctl->tolerance * rn;
+            ctl->iter = 0;
+            ctl->k2_done = 0;
+        }
+        ctl->true_rr = trr;
+        ctl->rr = trr;
+        ctl->residual_norm = rn;
+        ctl->nb_rz = nb;
+        ctl->refreshes = ctl->refreshes + 1;
+        if (rn <= ctl->tol) {
+            ctl->converged = 1;
+            ctl->done = 1;
+        } else if (ctl->k2_done >= ctl->max_iter) {
+            ctl->done = 1;
+        } else {
+            if (first || ctl->breakdown_hit || tdd > 1.0e-4 * trr) ctl->restart = 1;
+            ctl->breakdown_hit = 0;
+            ctl->done = 0;
+        }
+    }
 }
 
 __global__ __launch_bounds__(kBlock) void k_to_f32(int64_t n, const double *__restrict__ a,
@@ -646,9 +878,10 @@ void build_rowblocks(Context *ctx, DevMatrix &A, const std::vector<int> &ptr)
 
 Workspace::~Workspace()
 {
-    free_solo(solo);
     for (auto &g : graph)
         if (g) (void)hipGraphExecDestroy(g);
+    for (auto &e : ctl_ev)
+        if (e) (void)hipEventDestroy(e);
     if (host_ctl) (void)hipHostFree(host_ctl);
 }
 
@@ -660,24 +893,27 @@ void ensure_workspace(Context *ctx, Matrix &M)
     const int n = M.m.n;
     if (w.n != n) invalidate_graphs(w);
     w.n = n;
-    w.r.alloc(n), w.p.alloc(n), w.ap.alloc(n), w.x.alloc(n), w.b.alloc(n),
-        w.dinv.alloc(n);
+    // p and p2 hold the interleaved (z, p) pairs of the PCG iteration
+    w.r.alloc(n), w.p.alloc(2 * (size_t)n), w.p2.alloc(2 * (size_t)n), w.ap.alloc(n),
+        w.x.alloc(n), w.b.alloc(n), w.dinv.alloc(n);
     if (!w.partials.p) {
-        w.partials.alloc(3 * (size_t)partial_stride(ctx));
+        w.partials.alloc(4 * (size_t)partial_stride(ctx));
         FFH_HIP(hipMemsetAsync(w.partials.p, 0, sizeof(double) * w.partials.n, ctx->stream));
     }
     w.ctl.alloc(1);
     if (!w.host_ctl)
-        FFH_HIP(hipHostMalloc((void **)&w.host_ctl, sizeof(KrylovCtl),
+        FFH_HIP(hipHostMalloc((void **)&w.host_ctl, 2 * sizeof(KrylovCtl),
                               hipHostMallocDefault));
+    for (auto &e : w.ctl_ev)
+        if (!e) FFH_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
 }
 
 static void ensure_workspace32(Context *ctx, Matrix &M)
 {
     Workspace &w = M.ws;
     const int n = M.m.n;
-    w.r32.alloc(n), w.p32.alloc(n), w.ap32.alloc(n), w.x32.alloc(n),
-        w.dinv32.alloc(n);
+    w.r32.alloc(n), w.p32.alloc(2 * (size_t)n), w.p32b.alloc(2 * (size_t)n), w.ap32.alloc(n),
+        w.x32.alloc(n), w.dinv32.alloc(n);
     (void)ctx;
 }
 
@@ -912,11 +1148,13 @@ void spmv32(Context *ctx, DevMatrix &A, const float *x, float *y)
     launch_spmv<float, float>(ctx, A, x, y, vi);
 }
 
-// Launch `iters` PCG iterations (3 kernels each) of precision VT.
-template <typename MT, typename VT>
-static void launch_iterations(Context *ctx, Matrix &M, int iters, VT *x, VT *r,
-                              VT *p, VT *ap, const VT *dinv, int vi, int prof_slot)
+// `iters` iterations, two launches each: direction + SpMV (matrix MT,
+// vectors VT) and the x/r update (x in XT).
+template <typename MT, typename VT, typename XT>
+static void launch_iterations(Context *ctx, Matrix &M, int iters, XT *x, VT *r, VT *zpa, VT *zpb,
+                              VT *ap, const VT *dinv, int vi, int prof_slot)
 {
+    ZP<VT> *zp0 = reinterpret_cast<ZP<VT> *>(zpa), *zp1 = reinterpret_cast<ZP<VT> *>(zpb);
     const DevMatrix &A = M.m;
     Workspace &w = M.ws;
     const int n = A.n;
@@ -925,89 +1163,86 @@ static void launch_iterations(Context *ctx, Matrix &M, int iters, VT *x, VT *r,
     const int stride = partial_stride(ctx);
     const GArgs<MT> g = gargs<MT>(A, vi);
     for (int it = 0; it < iters; ++it) {
-        const int parity = it & 1; // chunks are even, so this is the global parity
         if (prof_slot >= 0) ctx->prof_mark(prof_slot, true);
         FFH_ENGINE(kVariants[vi].engine, {
-            hipLaunchKernelGGL((k_pcg_spmv<VV, MT, VT>), dim3(grids), dim3(kStreamBlock), 0,
-                               ctx->stream, n, g.ptr, g.idx, g.val, p, ap, g.rb, g.nrb,
-                               w.partials.p, w.ctl.p);
+            hipLaunchKernelGGL((k_pcg_dirspmv<VV, MT, VT>), dim3(grids), dim3(kStreamBlock), 0,
+                               ctx->stream, n, g.ptr, g.idx, g.val, zp0, zp1, ap, g.rb, g.nrb,
+                               w.partials.p, stride, w.ctl.p);
         });
         if (prof_slot >= 0) ctx->prof_mark(prof_slot, false);
-        hipLaunchKernelGGL((k_pcg_update<VT>), dim3(gridv), dim3(kBlock), 0, ctx->stream,
-                           n, x, r, p, ap, dinv, w.partials.p, stride, grids, parity,
-                           w.ctl.p);
-        hipLaunchKernelGGL((k_pcg_dir<VT>), dim3(gridv), dim3(kBlock), 0, ctx->stream, n,
-                           p, r, dinv, w.partials.p, stride, gridv, parity, w.ctl.p);
+        hipLaunchKernelGGL((k_pcg_update<VT, XT>), dim3(gridv), dim3(kBlock), 0, ctx->stream, n, x,
+                           r, zp0, zp1, ap, dinv, w.partials.p, stride, grids, w.ctl.p);
     }
     FFH_HIP(hipGetLastError());
 }
 
-// One PCG solve on device vectors b, x (x = initial guess, overwritten).
-template <typename MT, typename VT>
-static void pcg_device(Context *ctx, Matrix &M, const VT *b, VT *x,
-                       VT *r, VT *p, VT *ap, VT *dinv, int method, int precond,
-                       double tolerance, int tol_type, int max_iter,
-                       int breakdown_mode, int check_interval, int use_graph,
-                       int slot, KrylovCtl &out)
+// The FP64 residual refresh of the mixed solve (FP64 matrix, variant vi64).
+static void launch_refresh(Context *ctx, Matrix &M, int vi64, int first)
 {
-    DevMatrix &A = M.m;
+    const DevMatrix &A = M.m;
     Workspace &w = M.ws;
-    const int n = A.n;
-    // Jacobi diagonal ('none' and method cg use the identity)
-    hipLaunchKernelGGL((k_jacobi<VT>), dim3(grid_for(ctx, n, kBlock)), dim3(kBlock), 0,
-                       ctx->stream, n, A.gptr(), A.gidx(), A.gval(),
-                       (method == 1 && precond == 1) ? 1 : 0, dinv);
-    KrylovCtl init{};
-    init.max_iter = max_iter;
-    init.breakdown_mode = breakdown_mode;
-    init.tol_type = tol_type;
-    init.tolerance = tolerance;
-    *w.host_ctl = init;
-    FFH_HIP(hipMemcpyAsync(w.ctl.p, w.host_ctl, sizeof(KrylovCtl),
-                           hipMemcpyHostToDevice, ctx->stream));
-    const int vi = spmv_variant(ctx, M, sizeof(VT) == 4);
-    const int gi = variant_grid(ctx, A, vi);
-    const GArgs<MT> g = gargs<MT>(A, vi);
-    FFH_ENGINE(kVariants[vi].engine, {
-        hipLaunchKernelGGL((k_pcg_init<VV, MT, VT>), dim3(gi), dim3(kStreamBlock), 0,
-                           ctx->stream, n, g.ptr, g.idx, g.val, x, b, r, p, dinv, g.rb, g.nrb,
-                           w.partials.p, partial_stride(ctx));
+    const int gr = variant_grid(ctx, A, vi64);
+    const int stride = partial_stride(ctx);
+    const GArgs<double> g = gargs<double>(A, vi64);
+    if (ctx->profiling) ctx->prof_mark(0, true);
+    FFH_ENGINE(kVariants[vi64].engine, {
+        hipLaunchKernelGGL((k_mixed_refresh<VV>), dim3(gr), dim3(kStreamBlock), 0, ctx->stream,
+                           A.n, g.ptr, g.idx, g.val, w.x.p, w.b.p, w.r32.p, w.dinv32.p,
+                           reinterpret_cast<ZP<float> *>(w.p32.p),
+                           reinterpret_cast<ZP<float> *>(w.p32b.p), g.rb, g.nrb, w.partials.p,
+                           stride, w.ctl.p);
     });
-    hipLaunchKernelGGL(k_pcg_init_finish, dim3(1), dim3(kBlock), 0, ctx->stream, w.partials.p,
-                       partial_stride(ctx), gi, w.ctl.p);
+    if (ctx->profiling) ctx->prof_mark(0, false);
+    hipLaunchKernelGGL(k_mixed_refresh_finish, dim3(1), dim3(kBlock), 0, ctx->stream, w.partials.p,
+                       stride, gr, first, w.ctl.p);
     FFH_HIP(hipGetLastError());
-    // iteration chunks are even so that the parity slots line up
-    int chunk = check_interval > 0 ? std::max(2, (check_interval + 1) & ~1) : 16;
-    const bool profiling = ctx->profiling;
-    if (profiling) use_graph = 0;
-    const int gslot = sizeof(VT) == 8 ? 0 : 1;
-    if (use_graph && (w.graph[gslot] == nullptr || w.graph_chunk[gslot] != chunk * 64 + vi)) {
+}
+
+// Runs chunks (enqueue_chunk) until the control block says done.  The next
+// chunk is queued before the previous one's control block is read (it
+// returns at once after the stop), so the host poll overlaps the GPU.
+// `watch` sees each polled control block and may return false to stop early
+// (the stream is then drained).  A chunk may be replayed from a HIP graph.
+template <typename Enq, typename Watch>
+static KrylovCtl run_chunks(Context *ctx, Matrix &M, int gslot, int graph_key, bool use_graph,
+                            Enq &&enqueue_chunk, Watch &&watch)
+{
+    Workspace &w = M.ws;
+    hipStream_t s = ctx->stream;
+    if (ctx->profiling) use_graph = false;
+    if (use_graph && (w.graph[gslot] == nullptr || w.graph_chunk[gslot] != graph_key)) {
         if (w.graph[gslot]) {
             (void)hipGraphExecDestroy(w.graph[gslot]);
             w.graph[gslot] = nullptr;
         }
         hipGraph_t g;
-        FFH_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-        launch_iterations<MT, VT>(ctx, M, chunk, x, r, p, ap, dinv, vi, -1);
-        FFH_HIP(hipStreamEndCapture(ctx->stream, &g));
-        FFH_HIP(hipGraphInstantiate(&w.graph[gslot], g, nullptr, nullptr, 0));
+        FFH_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        enqueue_chunk();
+        FFH_HIP(hipStreamEndCapture(s, &g));
+        const hipError_t e = hipGraphInstantiate(&w.graph[gslot], g, nullptr, nullptr, 0);
         (void)hipGraphDestroy(g);
-        w.graph_chunk[gslot] = chunk * 64 + vi;
+        FFH_HIP(e);
+        w.graph_chunk[gslot] = graph_key;
     }
-    // graphs bake in the vector pointers; they are the workspace's own
-    for (;;) {
+    auto one = [&](int k) {
         if (use_graph)
-            FFH_HIP(hipGraphLaunch(w.graph[gslot], ctx->stream));
+            FFH_HIP(hipGraphLaunch(w.graph[gslot], s));
         else
-            launch_iterations<MT, VT>(ctx, M, chunk, x, r, p, ap, dinv, vi,
-                                      profiling ? slot : -1);
-        FFH_HIP(hipMemcpyAsync(w.host_ctl, w.ctl.p, sizeof(KrylovCtl),
-                               hipMemcpyDeviceToHost, ctx->stream));
-        FFH_HIP(hipStreamSynchronize(ctx->stream));
-        if (w.host_ctl->done) break;
-        if (!use_graph && chunk < 64) chunk *= 2;
+            enqueue_chunk();
+        FFH_HIP(hipMemcpyAsync(w.host_ctl + (k & 1), w.ctl.p, sizeof(KrylovCtl),
+                               hipMemcpyDeviceToHost, s));
+        FFH_HIP(hipEventRecord(w.ctl_ev[k & 1], s));
+    };
+    one(0);
+    for (int k = 0;; ++k) {
+        one(k + 1);
+        FFH_HIP(hipEventSynchronize(w.ctl_ev[k & 1]));
+        const KrylovCtl c = w.host_ctl[k & 1];
+        if (c.done || !watch(c)) break;
     }
-    out = *w.host_ctl;
+    FFH_HIP(hipMemcpyAsync(w.host_ctl, w.ctl.p, sizeof(KrylovCtl), hipMemcpyDeviceToHost, s));
+    FFH_HIP(hipStreamSynchronize(s));
+    return *w.host_ctl;
 }
 
 static void launch_residual(Context *ctx, Matrix &M)
@@ -1027,6 +1262,52 @@ static void launch_residual(Context *ctx, Matrix &M)
     FFH_HIP(hipGetLastError());
 }
 
+// FP64 Jacobi-PCG (or CG) of the reference on the workspace vectors b, x
+// (x = initial guess, overwritten): initialisation, then chunks of
+// iterations each ending with the stop test.
+static KrylovCtl pcg_fp64(Context *ctx, Matrix &M, int method, int precond, double tolerance,
+                          int tol_type, int max_iter, int breakdown_mode, int check_interval,
+                          int use_graph)
+{
+    DevMatrix &A = M.m;
+    Workspace &w = M.ws;
+    const int n = A.n;
+    hipStream_t s = ctx->stream;
+    hipLaunchKernelGGL((k_jacobi<double>), dim3(grid_for(ctx, n, kBlock)), dim3(kBlock), 0, s, n,
+                       A.gptr(), A.gidx(), A.gval(), (method == 1 && precond == 1) ? 1 : 0,
+                       w.dinv.p, (float *)nullptr);
+    KrylovCtl init{};
+    init.max_iter = max_iter;
+    init.breakdown_mode = breakdown_mode;
+    init.tol_type = tol_type;
+    init.tolerance = tolerance;
+    *w.host_ctl = init;
+    FFH_HIP(hipMemcpyAsync(w.ctl.p, w.host_ctl, sizeof(KrylovCtl), hipMemcpyHostToDevice, s));
+    const int vi = spmv_variant(ctx, M, false);
+    const int gi = variant_grid(ctx, A, vi);
+    const int stride = partial_stride(ctx);
+    const GArgs<double> g = gargs<double>(A, vi);
+    FFH_ENGINE(kVariants[vi].engine, {
+        hipLaunchKernelGGL((k_pcg_init<VV, double, double>), dim3(gi), dim3(kStreamBlock), 0, s, n,
+                           g.ptr, g.idx, g.val, w.x.p, w.b.p, w.r.p, w.dinv.p,
+                           reinterpret_cast<ZP<double> *>(w.p.p), g.rb, g.nrb, w.partials.p,
+                           stride);
+    });
+    hipLaunchKernelGGL(k_pcg_init_finish, dim3(1), dim3(kBlock), 0, s, w.partials.p, stride, gi,
+                       w.ctl.p);
+    FFH_HIP(hipGetLastError());
+    const int chunk = check_interval > 0 ? check_interval : 16;
+    const int prof = ctx->profiling ? 0 : -1;
+    auto enqueue = [&]() {
+        launch_iterations<double, double, double>(ctx, M, chunk, w.x.p, w.r.p, w.p.p, w.p2.p,
+                                                  w.ap.p, w.dinv.p, vi, prof);
+        hipLaunchKernelGGL(k_pcg_fin, dim3(1), dim3(kBlock), 0, s, w.partials.p, stride, w.ctl.p);
+        FFH_HIP(hipGetLastError());
+    };
+    return run_chunks(ctx, M, 0, chunk * 64 + vi, use_graph != 0, enqueue,
+                      [](const KrylovCtl &) { return true; });
+}
+
 // Public FP64 solve on device vectors.
 void solve_fp64(Context *ctx, Matrix &M, const double *b, double *x,
                 const ffh_solver_options &o, ffh_solver_stats &st)
@@ -1035,26 +1316,21 @@ void solve_fp64(Context *ctx, Matrix &M, const double *b, double *x,
     ensure_val64(ctx, M.m);
     spmv_variant(ctx, M, false);
     Workspace &w = M.ws;
-    KrylovCtl out{};
+    const int n = M.m.n;
+    hipStream_t s = ctx->stream;
     if (ctx->profiling) ctx->prof_begin();
-    FFH_HIP(hipEventRecord(ctx->ev_start, ctx->stream));
+    FFH_HIP(hipEventRecord(ctx->ev_start, s));
     // graphs capture the workspace vectors; copy b/x in and out
-    FFH_HIP(hipMemcpyAsync(w.b.p, b, sizeof(double) * M.m.n,
-                           hipMemcpyDeviceToDevice, ctx->stream));
-    FFH_HIP(hipMemcpyAsync(w.x.p, x, sizeof(double) * M.m.n,
-                           hipMemcpyDeviceToDevice, ctx->stream));
-    pcg_device<double, double>(ctx, M, w.b.p, w.x.p, w.r.p, w.p.p,
-                               w.ap.p, w.dinv.p, o.method, o.preconditioner,
-                               o.tolerance, o.tolerance_type, o.max_iterations,
-                               o.breakdown_mode, o.check_interval, o.use_graph, 0,
-                               out);
-    FFH_HIP(hipMemcpyAsync(x, w.x.p, sizeof(double) * M.m.n,
-                           hipMemcpyDeviceToDevice, ctx->stream));
+    FFH_HIP(hipMemcpyAsync(w.b.p, b, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+    FFH_HIP(hipMemcpyAsync(w.x.p, x, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+    const KrylovCtl out = pcg_fp64(ctx, M, o.method, o.preconditioner, o.tolerance,
+                                   o.tolerance_type, o.max_iterations, o.breakdown_mode,
+                                   o.check_interval, o.use_graph);
+    FFH_HIP(hipMemcpyAsync(x, w.x.p, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
     launch_residual(ctx, M); // true residual in FP64
-    FFH_HIP(hipEventRecord(ctx->ev_stop, ctx->stream));
-    FFH_HIP(hipMemcpyAsync(w.host_ctl, w.ctl.p, sizeof(KrylovCtl),
-                           hipMemcpyDeviceToHost, ctx->stream));
-    FFH_HIP(hipStreamSynchronize(ctx->stream));
+    FFH_HIP(hipEventRecord(ctx->ev_stop, s));
+    FFH_HIP(hipMemcpyAsync(w.host_ctl, w.ctl.p, sizeof(KrylovCtl), hipMemcpyDeviceToHost, s));
+    FFH_HIP(hipStreamSynchronize(s));
     float ms = 0.0f;
     FFH_HIP(hipEventElapsedTime(&ms, ctx->ev_start, ctx->ev_stop));
     if (ctx->profiling) ctx->prof_end();
@@ -1065,8 +1341,101 @@ void solve_fp64(Context *ctx, Matrix &M, const double *b, double *x,
     st.final_residual = out.residual_norm;
     st.solve_time = ms * 1e-3;
     st.refinements = 0;
-    st.true_residual = std::sqrt(w.host_ctl->rr);
+    st.true_residual = std::sqrt(w.host_ctl->true_rr);
     st.initial_residual = out.initial_norm;
+}
+
+// Mixed precision: FP32 matrix and FP32 r, p, Ap, D^-1 in the two-launch
+// Jacobi-PCG; x and every reduction in FP64.  After every chunk the residual
+// b - A x is recomputed in FP64 with the FP64 matrix and replaces the FP32
+// one (reliable update; the direction is kept while the recurrence tracked
+// it), and it alone decides convergence: ||b - A x|| <= tol ||b - A x0||
+// (relative) or <= tol (absolute).  A solve whose FP64 residual stops
+// improving (16 refreshes without a 4x drop) finishes with the FP64
+// iteration from its current x.
+void solve_mixed(Context *ctx, Matrix &M, const double *b, double *x,
+                 const ffh_solver_options &o, ffh_solver_stats &st)
+{
+    ensure_workspace(ctx, M);
+    ensure_workspace32(ctx, M);
+    DevMatrix &A = M.m;
+    ensure_val64(ctx, A);
+    ensure_val32(ctx, A);
+    const int vi64 = spmv_variant(ctx, M, false);
+    const int vi32 = spmv_variant(ctx, M, true);
+    Workspace &w = M.ws;
+    const int n = A.n;
+    hipStream_t s = ctx->stream;
+    if (ctx->profiling) ctx->prof_begin();
+    FFH_HIP(hipEventRecord(ctx->ev_start, s));
+    FFH_HIP(hipMemcpyAsync(w.b.p, b, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+    FFH_HIP(hipMemcpyAsync(w.x.p, x, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+    FFH_HIP(hipMemsetAsync(w.r32.p, 0, sizeof(float) * n, s));
+    const int use = o.method == 1 && o.preconditioner == 1;
+    hipLaunchKernelGGL((k_jacobi<float>), dim3(grid_for(ctx, n, kBlock)), dim3(kBlock), 0, s, n,
+                       A.gptr(), A.gidx(), A.gval(), use, w.dinv32.p, (float *)nullptr);
+    KrylovCtl init{};
+    init.max_iter = o.max_iterations;
+    init.breakdown_mode = 1;
+    init.tol_type = o.tolerance_type;
+    init.tolerance = o.tolerance;
+    init.mixed = 1;
+    *w.host_ctl = init;
+    FFH_HIP(hipMemcpyAsync(w.ctl.p, w.host_ctl, sizeof(KrylovCtl), hipMemcpyHostToDevice, s));
+    launch_refresh(ctx, M, vi64, 1);
+    FFH_HIP(hipGetLastError());
+    const int chunk = o.check_interval > 0 ? o.check_interval : 16;
+    const int prof = ctx->profiling ? 1 : -1;
+    auto enqueue = [&]() {
+        launch_iterations<float, float, double>(ctx, M, chunk, w.x.p, w.r32.p, w.p32.p, w.p32b.p,
+                                                w.ap32.p, w.dinv32.p, vi32, prof);
+        launch_refresh(ctx, M, vi64, 0);
+    };
+    double best = 1e300;
+    int stall = 0;
+    bool stagnated = false;
+    KrylovCtl out = run_chunks(ctx, M, 1, (chunk * 64 + vi32) * 64 + vi64, o.use_graph != 0, enqueue,
+                               [&](const KrylovCtl &c) {
+                                   if (c.true_rr < 0.25 * best) {
+                                       best = c.true_rr;
+                                       stall = 0;
+                                   } else if (++stall >= 16) {
+                                       stagnated = true;
+                                       return false;
+                                   }
+                                   return true;
+                               });
+    int iters = out.k2_done;
+    const int refreshes = out.refreshes;
+    if (stagnated && !out.converged && !out.done) {
+        // FP32 iterations stopped making progress: FP64 from the current x
+        // to the same absolute tolerance
+        const KrylovCtl f = pcg_fp64(ctx, M, o.method, o.preconditioner, out.tol, 1,
+                                     std::max(0, o.max_iterations - iters), 1, o.check_interval,
+                                     o.use_graph);
+        iters += f.iter;
+        const double init_norm = out.initial_norm;
+        out = f;
+        out.initial_norm = init_norm;
+        launch_residual(ctx, M);
+        FFH_HIP(hipMemcpyAsync(w.host_ctl, w.ctl.p, sizeof(KrylovCtl), hipMemcpyDeviceToHost, s));
+        FFH_HIP(hipStreamSynchronize(s));
+        out.true_rr = w.host_ctl->true_rr;
+        if (!out.converged && std::sqrt(out.true_rr) <= out.tol) out.converged = 1;
+    }
+    FFH_HIP(hipMemcpyAsync(x, w.x.p, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+    FFH_HIP(hipEventRecord(ctx->ev_stop, s));
+    FFH_HIP(hipStreamSynchronize(s));
+    float ms = 0.0f;
+    FFH_HIP(hipEventElapsedTime(&ms, ctx->ev_start, ctx->ev_stop));
+    if (ctx->profiling) ctx->prof_end();
+    st.converged = out.converged;
+    st.iterations = iters;
+    st.final_residual = std::sqrt(out.true_rr);
+    st.true_residual = std::sqrt(out.true_rr);
+    st.initial_residual = out.initial_norm;
+    st.solve_time = ms * 1e-3;
+    st.refinements = refreshes;
 }
 
 double bench_spmv(Context *ctx, Matrix &M, int precision, int reps, double *bytes)

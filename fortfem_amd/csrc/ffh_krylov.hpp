@@ -38,7 +38,6 @@ namespace ffh {
 bool sell_structure(Context *ctx, DevMatrix &A);
 // the matrix's own SELL value copy of one precision, made current
 void ensure_sell(Context *ctx, DevMatrix &A, bool f32);
-void free_solo(SoloState *s);
 void sell_values(Context *ctx, const DevMatrix &A, const double *val, double *out64,
                  float *out32);
 } // namespace ffh

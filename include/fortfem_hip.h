@@ -53,7 +53,7 @@ extern "C" {
 
 /* precision */
 #define FFH_FP64 0
-#define FFH_MIXED 1 /* FP32 matrix/vectors inside FP64 iterative refinement */
+#define FFH_MIXED 1 /* FP32 matrix/vectors, FP64 x, reductions and residual refreshes */
 #define FFH_FP32 1  /* element contractions in FP32 (ffh_assemble)        */
 
 /* which matrix / vector of a problem */
@@ -78,9 +78,9 @@ typedef struct ffh_solver_options {
     int precision;       /* FFH_FP64 or FFH_MIXED                          */
     int breakdown_mode;  /* 0 reference |pAp| < 1e-14 (:897),
                             1 scale-invariant |pAp| <= 1e-14 |rz|         */
-    double inner_tolerance; /* MIXED: relative tolerance per FP32 solve   */
-    int max_refinements;    /* MIXED: outer FP64 refinement steps cap    */
-    int check_interval;     /* iterations per host convergence poll, 0 auto */
+    double inner_tolerance; /* accepted for compatibility, not used      */
+    int max_refinements;    /* accepted for compatibility, not used      */
+    int check_interval;     /* iterations per chunk (host poll; MIXED: per FP64 refresh), 0: 16 */
     int use_graph;          /* replay iteration chunks from a HIP graph   */
 } ffh_solver_options;
 
@@ -91,7 +91,7 @@ typedef struct ffh_solver_stats {
     int iterations;        /* Krylov iterations (MIXED: summed inner)    */
     double final_residual; /* recursive ||r|| (FP64) / true ||b-Ax|| (MIXED) */
     double solve_time;     /* seconds, HIP events on the context stream   */
-    int refinements;       /* MIXED: outer steps                          */
+    int refinements;       /* MIXED: FP64 residual refreshes            */
     double true_residual;  /* ||b - A x||_2 recomputed in FP64 at exit    */
     double initial_residual;
 } ffh_solver_stats;

@@ -164,7 +164,7 @@ def test_poisson_pcg_parity(ff, degree, N):
         assert st.converged
         assert abs(st.iterations - st_ref.iterations) <= 1
         assert np.linalg.norm(x - x_ref) / np.linalg.norm(x_ref) <= 1e-9
-    # mixed precision: FP32 inner PCG inside FP64 refinement
+    # mixed precision: FP32 iterations with FP64 residual refreshes
     opts = ff.solver_options(method="pcg", preconditioner="jacobi", tolerance=1e-10,
                              max_iterations=20000, breakdown="relative", precision="mixed")
     xm, stm = p.solve(opts)

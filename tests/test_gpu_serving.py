@@ -44,7 +44,7 @@ def oracle_system(ff, family, k, N, coeff):
     return v, t, red
 
 
-CASES = [(1, 1, 5, 3), (1, 2, 3, 2), (2, 1, 3, 1), (2, 2, 2, 4)]  # family, order, N, nrhs
+CASES = [(1, 1, 5, 6), (1, 2, 3, 2), (2, 1, 3, 7), (2, 2, 2, 3), (1, 1, 3, 1)]  # family, order, N, nrhs
 
 
 def make_requests(ff, server, seed=0):
