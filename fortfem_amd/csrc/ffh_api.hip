@@ -119,10 +119,13 @@ void check_options(const ffh_solver_options *o)
 void Context::prof_begin()
 {
     prof_used[0] = prof_used[1] = 0;
+    prof_seen[0] = prof_seen[1] = 0;
 }
 
 void Context::prof_mark(int which, bool start)
 {
+    if (start) prof_on[which] = prof_seen[which]++ % prof_stride == 0;
+    if (!prof_on[which]) return;
     auto &v = prof_ev[which];
     int &u = prof_used[which];
     if ((int)v.size() <= u) {
@@ -132,7 +135,6 @@ void Context::prof_mark(int which, bool start)
     }
     FFH_HIP(hipEventRecord(v[u], stream));
     ++u;
-    (void)start;
 }
 
 void Context::prof_end()

@@ -39,7 +39,7 @@ constexpr int kRowsPerItem = 256; // four SELL-64 slices, one block
 constexpr int kMaxRhs = 8;
 
 struct BatchSys {
-    const int *off, *sidx; // SELL-64 structure of the cached reduced pattern
+    const int *off, *sidx, *srow; // SELL-64 structure of the cached reduced pattern
     const float *val32;    // SELL values of the request
     const double *val64;
     const float *dinv32;
@@ -177,8 +177,8 @@ __device__ __forceinline__ void item_spmv(const BatchSys &s, int slice0, const M
         ip += 64;
         vp += 64;
     }
-    const int row = sl * 64 + lane;
-    if (row < s.n) {
+    const int row = s.srow[sl * 64 + lane];
+    if (row >= 0) {
         const Pack<VT, R> own = ld<VT, R>(p, row);
         Pack<VT, R> out;
 #pragma unroll
@@ -482,8 +482,8 @@ __device__ __forceinline__ void resid_item(const BatchSys &s, int slice0, const 
                 vp += 64 * U;
             }
         }
-        const int row = sl * 64 + lane;
-        if (row < s.n) {
+        const int row = s.srow[sl * 64 + lane];
+        if (row >= 0) {
             const Pack<double, R> bi = ld<double, R>(s.b, row);
             const double di = s.dinv64[row];
             Pack<double, R> r64;
@@ -939,6 +939,7 @@ static void rebuild_batch(ffh_server *S)
         BatchSys b{};
         b.off = A.sell_off.p;
         b.sidx = A.sell_idx.p;
+        b.srow = A.sell_row.p;
         b.nslice = A.nslice;
         b.val32 = q->val32.p;
         b.val64 = q->val64.p;

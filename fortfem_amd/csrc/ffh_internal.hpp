@@ -119,8 +119,10 @@ struct DevMatrix {
     DBuf<int> rowblocks; // CSR-stream row blocks of the gather arrays
     int nrb = 0;
     int variant[2] = {-1, -1}; // autotuned SpMV variant (FP64, FP32)
-    // SELL-64 copy of the gather arrays, built on first use: slices of 64
-    // consecutive rows, entry j of the slice's row l at sell_off[s] + 64 j + l,
+    // SELL-64 copy of the gather arrays, built on first use: within each
+    // window of 1024 rows the rows are ordered by decreasing length and cut
+    // into slices of 64 (sell_row: the row at each slice position, -1 for
+    // padding); entry j of a slice's row l sits at sell_off[s] + 64 j + l,
     // each row padded to its slice's longest row with zero values on its own
     // column.  sell_src gives the gather-array position of each entry (-1 for
     // padding); the value copies are refreshed when the values change.
@@ -128,7 +130,7 @@ struct DevMatrix {
     bool sell64_valid = false, sell32_valid = false;
     int nslice = 0;
     int64_t sell_total = 0;
-    DBuf<int> sell_off, sell_idx, sell_src;
+    DBuf<int> sell_off, sell_idx, sell_src, sell_row;
     DBuf<double> sell_val;
     DBuf<float> sell_val32;
     // the values (FP64, FP32 or both) were rewritten
@@ -169,8 +171,14 @@ struct Context {
     std::string err;
     hipEvent_t ev_start = nullptr, ev_stop = nullptr;
     bool profiling = false;
+    // launch timing inside solves: every prof_stride-th bracketed launch of
+    // each kind is timed (events cost host time; sampling keeps the host
+    // ahead of the GPU)
     std::vector<hipEvent_t> prof_ev[2];
     int prof_used[2] = {0, 0};
+    int prof_seen[2] = {0, 0};
+    bool prof_on[2] = {false, false};
+    int prof_stride = 8;
     double prof_avg[2] = {0.0, 0.0};
     int prof_count[2] = {0, 0};
     void prof_begin();

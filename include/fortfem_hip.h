@@ -287,10 +287,13 @@ int ffh_bench_spmv(ffh_matrix *mat, int precision, int reps, double *avg_ms,
  * the variant in use. */
 #define FFH_SPMV_VARIANTS 11
 int ffh_matrix_spmv_variant(ffh_matrix *mat, int precision, int request, int *chosen);
-/* Kernel timing inside solves: when enabled, every SpMV launch of a solve
- * (which 0 = FP64 SpMV+dot, 1 = FP32 SpMV+dot) is bracketed by HIP events on
- * the context stream; ffh_last_solve_profile returns the launch count and
- * mean duration (ms) of the last solve. */
+/* Kernel timing inside solves: when enabled, every 8th direction + SpMV
+ * launch of a solve (which 0 = FP64 direction + SpMV of the FP64 solve, or
+ * the FP64 residual refresh of a mixed solve; 1 = FP32 direction + SpMV of a
+ * mixed solve) is bracketed by HIP events on the context stream (sampling
+ * keeps the event cost off the solve's critical path);
+ * ffh_last_solve_profile returns the timed launch count and their mean
+ * duration (ms) for the last solve. */
 int ffh_set_profiling(ffh_context *ctx, int enable);
 int ffh_last_solve_profile(ffh_context *ctx, int which, int *count,
                            double *avg_ms);
