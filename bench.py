@@ -32,11 +32,11 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 
 
-# HBM bytes per launch of the roofline kernel, from rocprofv3 --pmc FETCH_SIZE
-# and WRITE_SIZE passes over this script, each scaled by the factor the same
-# counter reads for a device copy of known size.  None until measured for
-# the current kernel.
-MEASURED_TRAFFIC = {}
+# HBM bytes per launch of the roofline kernel (k_pcg_dirspmv<float>), from
+# rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE passes over this script, scaled by
+# 2.0 and 1.0 (what the two counters read for a 256 MiB device copy).
+# Re-measure when the kernel changes.
+MEASURED_TRAFFIC = {"c2": 60.80e6}
 
 
 def spmv_bytes(nnz: int, n: int, sv: int) -> float:

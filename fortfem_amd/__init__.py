@@ -641,6 +641,51 @@ def assemble_tetra_lagrange_scalar_load(mesh_vertices, tetrahedra, degree, quadr
                                                        quadrature_degree, samples, ctx)
 
 
+def nedelec_load_points(order):
+    """The reference's Nedelec load quadrature (4-point rule at k = 1, Duffy
+    2k + 2 otherwise): array (nq, 4) of (x, y, z, weight)."""
+    lib = capi.load()
+    nq = C.c_int(0)
+    capi.check(lib.ffh_nedelec_load_points(order, C.byref(nq), None))
+    pts = np.zeros((nq.value, 4))
+    capi.check(lib.ffh_nedelec_load_points(order, C.byref(nq), _ptr(pts)))
+    return pts
+
+
+def assemble_tetra_nedelec_vector_load_samples(mesh_vertices, tetrahedra, source_values, order=1,
+                                               ctx: Context | None = None):
+    """Nedelec vector load from samples of the source at the mapped load
+    points: source_values (T, nq, 3) == f(3, nq, T) column-major.
+    -> (vector, status)."""
+    status = FortsparseStatus(FORTSPARSE_INVALID_MATRIX, "Tetrahedral Nedelec vector load failed")
+    try:
+        p = Problem(mesh_vertices, tetrahedra, ctx)
+        p.pattern_build(capi.FAMILY_NEDELEC, order)
+        p.load(-1, capi.SOURCE_SAMPLES, _f64(source_values).reshape(-1))
+        return p.rhs(capi.FULL), FortsparseStatus(0, "")
+    except FFHError as e:
+        if e.code == capi.FFH_ERR_DEVICE:
+            raise
+        return None, status
+
+
+def assemble_tetra_nedelec_vector_load(mesh_vertices, tetrahedra, source, order=1,
+                                       ctx: Context | None = None):
+    """assemble_tetra_nedelec_vector_load (src/assembly/assembly_tetra_nedelec_3d.f90:
+    2002-2040) / _order (:71-127) with a vectorised source(x, y, z) returning
+    three arrays; the callback is sampled at the mapped load points."""
+    pts = nedelec_load_points(order)
+    v = np.asarray(mesh_vertices, dtype=np.float64)
+    t = np.asarray(tetrahedra) - 1
+    v0 = v[t[:, 0]]
+    J = np.stack([v[t[:, 1]] - v0, v[t[:, 2]] - v0, v[t[:, 3]] - v0], axis=2)
+    X = v0[:, None, :] + np.einsum("tij,jq->tqi", J, pts[:, :3].T)
+    fx, fy, fz = source(X[..., 0], X[..., 1], X[..., 2])
+    samples = np.stack([np.broadcast_to(fx, X.shape[:2]), np.broadcast_to(fy, X.shape[:2]),
+                        np.broadcast_to(fz, X.shape[:2])], axis=-1)
+    return assemble_tetra_nedelec_vector_load_samples(mesh_vertices, tetrahedra, samples, order, ctx)
+
+
 def solve_tetra_lagrange_poisson(vertices, tetrahedra, degree, source, boundary_value=None,
                                  opts: SolverOptionsT | None = None, ctx: Context | None = None,
                                  cell_coefficient=None):

@@ -48,6 +48,7 @@ EXPORTS = (
     "ffh_problem_solve", "ffh_matrix_upload", "ffh_matrix_free", "ffh_matrix_download",
     "ffh_spmv", "ffh_solve_sparse", "ffh_bench_spmv", "ffh_set_profiling",
     "ffh_last_solve_profile", "ffh_matrix_spmv_variant", "ffh_nedelec_dof_map", "ffh_nedelec_basis",
+    "ffh_nedelec_load_points",
     "ffh_server_create", "ffh_server_free", "ffh_server_prepare", "ffh_server_submit", "ffh_server_step",
     "ffh_server_poll", "ffh_server_stats", "ffh_server_progress", "ffh_problem_solution", "ffh_server_set_precision",
 )
@@ -136,6 +137,7 @@ def load() -> C.CDLL:
         "ffh_matrix_spmv_variant": (C.c_int, [vp, C.c_int, C.c_int, ip]),
         "ffh_nedelec_dof_map": (C.c_int, [C.c_int, C.c_int, vp, C.c_int, vp, vp, vp, ip]),
         "ffh_nedelec_basis": (C.c_int, [C.c_int, C.c_int, vp, vp, vp]),
+        "ffh_nedelec_load_points": (C.c_int, [C.c_int, ip, vp]),
         "ffh_server_create": (C.c_int, [vp, C.c_int64, C.c_int, C.c_int, C.POINTER(vp)]),
         "ffh_server_free": (C.c_int, [vp]),
         "ffh_server_prepare": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, C.c_int, vp, C.c_int, vp, ip, i64p]),

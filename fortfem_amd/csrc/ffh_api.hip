@@ -301,6 +301,17 @@ int ffh_nedelec_basis(int k, int npts, const double *points, double *values, dou
     });
 }
 
+int ffh_nedelec_load_points(int k, int *nq, double *points)
+{
+    if (k < 1 || k > 6 || !nq) return FFH_ERR_INVALID;
+    return guarded(nullptr, [&] {
+        std::vector<double> pts;
+        nedelec_load_rule(k, pts);
+        *nq = (int)(pts.size() / 4);
+        if (points) std::copy(pts.begin(), pts.end(), points);
+    });
+}
+
 int ffh_problem_create(ffh_context *ctx, int dim, int nvert, const double *vertices,
                        int ncell, const int *cells, ffh_problem **out)
 {

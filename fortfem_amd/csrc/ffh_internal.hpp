@@ -218,6 +218,10 @@ struct Problem {
     int ned_rule = -1;
     DBuf<double> d_gterm;
     int n_edges = 0, n_faces = 0;
+    // Nedelec vector load: oriented load tables per class slot, the load
+    // points (x, y, z, w), and the per-cell g_q = det w_q J^-1 f(x_q)
+    DBuf<double> d_ned_load_tab, d_ned_load_pts, d_ned_g;
+    int ned_load_nq = 0;
     // full matrix + reduced matrix
     Matrix full, reduced;
     DBuf<double> rhs_full, rhs_red;

@@ -610,11 +610,12 @@ void whitney(const xd p[3], xd *vals, xd *curls)
     }
 }
 
-// 12 reference tensors (full n x n, row-major) at the element rule
-int reference_tensors(int k, bool weighted_rule, std::vector<xd> &R)
+// The element rule: the 4-point degree-2 rule at k = 1, else the Duffy rule
+// of degree 2k (curl-mass element) or 2k + 2 (weighted element and load).
+void element_rule(int k, bool weighted_rule, std::vector<xd> &px, std::vector<xd> &py,
+                  std::vector<xd> &pz, std::vector<xd> &pw)
 {
-    const int n = nedelec_dof_count(k);
-    std::vector<xd> px, py, pz, pw;
+    px.clear(), py.clear(), pz.clear(), pw.clear();
     if (k == 1) {
         // 4-point degree-2 rule of assemble_tetra_nedelec_element (:2364-2366)
         const xd a = (5 + 3 * std::sqrt((xd)5)) / 20, b = (5 - std::sqrt((xd)5)) / 20;
@@ -635,9 +636,15 @@ int reference_tensors(int k, bool weighted_rule, std::vector<xd> &R)
             pw.push_back((xd)w[q]);
         }
     }
-    const int nq = (int)pw.size();
-    // basis values and curls at every point: B[q][j][0..5]
-    std::vector<xd> B((size_t)nq * n * 6);
+}
+
+// basis curls and values at points: B[q][j][0..2] curl, [3..5] value
+int basis_at_points(int k, const std::vector<xd> &px, const std::vector<xd> &py,
+                    const std::vector<xd> &pz, std::vector<xd> &B)
+{
+    const int n = nedelec_dof_count(k);
+    const int nq = (int)px.size();
+    B.assign((size_t)nq * n * 6, 0);
     if (k == 1) {
         for (int q = 0; q < nq; ++q) {
             const xd p[3] = {px[q], py[q], pz[q]};
@@ -675,6 +682,18 @@ int reference_tensors(int k, bool weighted_rule, std::vector<xd> &R)
             }
         }, 1);
     }
+    return 0;
+}
+
+// 12 reference tensors (full n x n, row-major) at the element rule
+int reference_tensors(int k, bool weighted_rule, std::vector<xd> &R)
+{
+    const int n = nedelec_dof_count(k);
+    std::vector<xd> px, py, pz, pw;
+    element_rule(k, weighted_rule, px, py, pz, pw);
+    const int nq = (int)pw.size();
+    std::vector<xd> B;
+    if (basis_at_points(k, px, py, pz, B)) return 1;
     static const int pa[6] = {0, 1, 2, 0, 0, 1}, pb[6] = {0, 1, 2, 1, 2, 2};
     const size_t NN = (size_t)n * n;
     R.assign(12 * NN, 0);
@@ -829,6 +848,59 @@ void nedelec_class_tables(int k, bool weighted_rule, const std::vector<int> &cla
             }
         }, 1);
     }
+}
+
+} // namespace ffh
+
+namespace ffh {
+
+void nedelec_load_rule(int k, std::vector<double> &pts)
+{
+    std::vector<xd> px, py, pz, pw;
+    element_rule(k, true, px, py, pz, pw);
+    pts.resize(4 * px.size());
+    for (size_t q = 0; q < px.size(); ++q) {
+        pts[4 * q + 0] = (double)px[q];
+        pts[4 * q + 1] = (double)py[q];
+        pts[4 * q + 2] = (double)pz[q];
+        pts[4 * q + 3] = (double)pw[q];
+    }
+}
+
+void nedelec_load_tables(int k, const std::vector<int> &classes, std::vector<double> &tab,
+                         std::vector<double> &pts)
+{
+    const int n = nedelec_dof_count(k);
+    std::vector<xd> px, py, pz, pw, B;
+    element_rule(k, true, px, py, pz, pw);
+    const int nq = (int)pw.size();
+    if (basis_at_points(k, px, py, pz, B)) throw std::runtime_error("Nedelec basis construction failed");
+    pts.resize((size_t)4 * nq);
+    for (int q = 0; q < nq; ++q) {
+        pts[4 * q + 0] = (double)px[q];
+        pts[4 * q + 1] = (double)py[q];
+        pts[4 * q + 2] = (double)pz[q];
+        pts[4 * q + 3] = (double)pw[q];
+    }
+    const size_t per = (size_t)nq * 3 * n;
+    tab.assign(classes.size() * per, 0.0);
+    parallel_for((int64_t)classes.size(), [&](int64_t lo, int64_t hi) {
+        for (int64_t i = lo; i < hi; ++i) {
+            std::vector<xd> T;
+            if (class_transform(k, classes[i], T)) throw std::runtime_error("Nedelec face transform failed");
+            double *out = tab.data() + (size_t)i * per;
+            for (int q = 0; q < nq; ++q)
+                for (int c = 0; c < 3; ++c)
+                    for (int r = 0; r < n; ++r) {
+                        xd acc = 0;
+                        for (int j = 0; j < n; ++j) {
+                            const xd t = T[(size_t)j * n + r];
+                            if (t != 0) acc += B[((size_t)q * n + j) * 6 + 3 + c] * t;
+                        }
+                        out[((size_t)q * 3 + c) * n + r] = (double)acc;
+                    }
+        }
+    }, 1);
 }
 
 } // namespace ffh

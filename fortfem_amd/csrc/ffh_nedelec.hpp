@@ -43,6 +43,19 @@ void nedelec_boundary_dofs(int k, int ntet, const int *tets, const std::vector<i
 void nedelec_class_tables(int k, bool weighted_rule, const std::vector<int> &classes,
                           std::vector<double> &out);
 
+// Load tables of assemble_tetra_nedelec_vector_load(_order)
+// (src/assembly/assembly_tetra_nedelec_3d.f90:71-127, 525-582, 2002-2040,
+// 2516-2553): the reference's load rule (4-point degree-2 rule at k = 1,
+// Duffy 2k + 2 otherwise) as pts (x, y, z, w per point) and, for class slot i,
+// the oriented reference values tab[((i * nq + q) * 3 + c) * n + r] =
+// sum_j value_j,c(point q) T_jr, with T the class's basis transform, so that
+// the oriented element load is T^T b = sum_(q,c) g_qc tab[q][c][r] with
+// g_q = det w_q J^-1 f(x_q).
+void nedelec_load_tables(int k, const std::vector<int> &classes, std::vector<double> &tab,
+                         std::vector<double> &pts);
+// the load points alone (x, y, z, w per point)
+void nedelec_load_rule(int k, std::vector<double> &pts);
+
 // Edge orientations (+1 / -1 per local edge) and face permutations (position
 // of each local face vertex in the sorted triple, 1..3) of a class.
 void nedelec_class_orientation(int cls, int edge_orient[6], int face_perm[12]);

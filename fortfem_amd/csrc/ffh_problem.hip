@@ -282,6 +282,55 @@ __global__ __launch_bounds__(256) void k_ned_assemble(
     }
 }
 
+// Nedelec vector load (assemble_tetra_nedelec_vector_load(_order),
+// src/assembly/assembly_tetra_nedelec_3d.f90:71-127, 525-582, 2002-2040,
+// 2516-2553), step 1, one thread per cell: g_q = det w_q J^-1 f(x_q) at the
+// reference's load points, so that f . (J^-T v_r) = (J^-1 f) . v_r.  f is a
+// constant vector, or sampled at the mapped points (samples(3, nq, ncell) in
+// the reference's cell order).
+__global__ __launch_bounds__(kBlockA) void k_ned_load_g(
+    int ncell, int nq, const int *__restrict__ cells_c, const double *__restrict__ verts,
+    const int *__restrict__ perm, const double *__restrict__ pts, int kind, double f0, double f1,
+    double f2, const double *__restrict__ samples, double *__restrict__ g, int *__restrict__ err)
+{
+    const int pos = blockIdx.x * kBlockA + threadIdx.x;
+    if (pos >= ncell) return;
+    const int4 c = reinterpret_cast<const int4 *>(cells_c)[pos];
+    double jac[9], inv[9], det;
+    if (!tet_geometry(verts, c.x, c.y, c.z, c.w, jac, inv, det)) {
+        atomicOr(err, 1);
+        return;
+    }
+    const size_t e = (size_t)perm[pos];
+    for (int q = 0; q < nq; ++q) {
+        double f[3] = {f0, f1, f2};
+        if (kind == FFH_SOURCE_SAMPLES)
+            for (int t = 0; t < 3; ++t) f[t] = samples[(e * nq + q) * 3 + t];
+        const double s = det * pts[4 * q + 3];
+        for (int a = 0; a < 3; ++a)
+            g[((size_t)pos * nq + q) * 3 + a] =
+                s * (inv[a] * f[0] + inv[a + 3] * f[1] + inv[a + 6] * f[2]);
+    }
+}
+
+// step 2, one thread per (cell, oriented local dof r):
+// b_r = sum_(q,c) g_qc tab[class][q][c][r] (the class table folds in T^T)
+__global__ __launch_bounds__(256) void k_ned_load_elem(int ncell, int n, int nq3,
+                                                        const unsigned char *__restrict__ cls_c,
+                                                        const double *__restrict__ g,
+                                                        const double *__restrict__ tab,
+                                                        double *__restrict__ elem)
+{
+    const int64_t t = blockIdx.x * 256ll + threadIdx.x;
+    if (t >= (int64_t)ncell * n) return;
+    const int pos = (int)(t / n), r = (int)(t - (int64_t)pos * n);
+    const double *gp = g + (size_t)pos * nq3;
+    const double *tp = tab + (size_t)cls_c[pos] * nq3 * n + r;
+    double acc = 0.0;
+    for (int j = 0; j < nq3; ++j) acc += gp[j] * tp[(size_t)j * n];
+    elem[(size_t)r * ncell + pos] = acc;
+}
+
 // 2D P1 load (assemble_scalar_p1_load, src/fortfem_forms_simple.f90:1730-1761):
 // f area / 3 to each vertex, one colour per launch.
 __global__ __launch_bounds__(kBlockA) void k_p1_2d_load(int pos0, int pos1,
@@ -615,6 +664,7 @@ void problem_pattern(Problem *P, int family, int order)
         P->d_slots_cm.upload(slots_cm, s);
         P->d_cls_c.upload(cls_c, s);
         P->ned_rule = -1;
+        P->d_ned_load_tab.reset();
     } else {
         P->d_slots.upload(slots, s);
         P->d_slots_cm.reset();
@@ -835,6 +885,56 @@ void problem_assemble(Problem *P, int qdeg, int coeff_kind, const double *coeff,
     P->red_dirty = true;
 }
 
+// Nedelec vector load into rhs_full: tables once per problem, then g per
+// cell, the oriented element vectors and a colour-by-colour scatter.
+static void nedelec_load(Problem *P, int kind, const double *params, int64_t np)
+{
+    Context *ctx = P->ctx;
+    hipStream_t s = ctx->stream;
+    const int T = P->ncell, G = P->n_global, nl = P->n_local;
+    if (!P->d_ned_load_tab.p) {
+        std::vector<double> tab, pts;
+        nedelec_load_tables(P->order, P->class_list, tab, pts);
+        P->d_ned_load_tab.upload(tab, s);
+        P->d_ned_load_pts.upload(pts, s);
+        P->ned_load_nq = (int)(pts.size() / 4);
+    }
+    const int nq = P->ned_load_nq;
+    double f[3] = {0.0, 0.0, 0.0};
+    const double *samples = nullptr;
+    if (kind == FFH_SOURCE_CONSTANT) {
+        FFH_REQUIRE(params && np >= 3, FFH_ERR_INVALID, "constant vector load needs params[0..2]");
+        for (int i = 0; i < 3; ++i) f[i] = params[i];
+    } else if (kind == FFH_SOURCE_SAMPLES) {
+        FFH_REQUIRE(params && np == 3ll * nq * T, FFH_ERR_INVALID,
+                    "samples must be (3, nq, ncell) at the Nedelec load points");
+        P->d_samples.upload(params, (size_t)np, s);
+        samples = P->d_samples.p;
+    } else {
+        throw Error{FFH_ERR_INVALID, "Nedelec loads take a constant vector or samples"};
+    }
+    P->d_ned_g.alloc((size_t)T * nq * 3);
+    P->d_elem.alloc(std::max<size_t>(P->d_elem.n, (size_t)nl * T));
+    FFH_HIP(hipMemsetAsync(P->rhs_full.p, 0, sizeof(double) * G, s));
+    hipLaunchKernelGGL(k_ned_load_g, dim3((T + kBlockA - 1) / kBlockA), dim3(kBlockA), 0, s, T, nq,
+                       P->d_cells_c.p, P->d_vertices.p, P->d_perm.p, P->d_ned_load_pts.p, kind,
+                       f[0], f[1], f[2], samples, P->d_ned_g.p, P->d_err.p);
+    const int64_t work = (int64_t)T * nl;
+    hipLaunchKernelGGL(k_ned_load_elem, dim3((unsigned)((work + 255) / 256)), dim3(256), 0, s, T,
+                       nl, 3 * nq, P->d_cls_c.p, P->d_ned_g.p, P->d_ned_load_tab.p, P->d_elem.p);
+    for (int c = 0; c < P->n_colors; ++c) {
+        const int p0 = P->color_start[c], cnt = P->color_start[c + 1] - p0;
+        const int64_t w = (int64_t)cnt * nl;
+        if (w == 0) continue;
+        hipLaunchKernelGGL(k_scatter_vec_colour, dim3((int)((w + 255) / 256)), dim3(256), 0, s, p0,
+                           cnt, T, nl, P->d_elem.p, P->d_gdofs_c.p, P->rhs_full.p);
+    }
+    FFH_HIP(hipGetLastError());
+    check_err(P, "Nedelec vector load");
+    P->loaded = true;
+    P->red_dirty = true;
+}
+
 void problem_load(Problem *P, int qdeg, int kind, const double *params, int64_t np)
 {
     Context *ctx = P->ctx;
@@ -866,8 +966,12 @@ void problem_load(Problem *P, int qdeg, int kind, const double *params, int64_t 
         P->red_dirty = true;
         return;
     }
+    if (P->family == FFH_FAMILY_NEDELEC) {
+        nedelec_load(P, kind, params, np);
+        return;
+    }
     FFH_REQUIRE(P->family == FFH_FAMILY_H1, FFH_ERR_INVALID,
-                "closed-form loads are implemented for the H1 and 2D P1 families");
+                "closed-form loads are implemented for the H1, Nedelec and 2D P1 families");
     FFH_REQUIRE(qdeg >= 0, FFH_ERR_INVALID, "quadrature degree must be >= 0");
     if (P->load_qdeg != qdeg) {
         std::vector<double> tab;

@@ -60,7 +60,7 @@ module fortfem_hip_capi
 
     public :: ffh_init, ffh_finalize, ffh_last_error, ffh_solver_options_default
     public :: ffh_box_mesh_size, ffh_box_mesh, ffh_unit_square_mesh, ffh_tetra_duffy
-    public :: ffh_nedelec_dof_map, ffh_nedelec_basis
+    public :: ffh_nedelec_dof_map, ffh_nedelec_basis, ffh_nedelec_load_points
     public :: ffh_problem_create, ffh_problem_free, ffh_pattern_build
     public :: ffh_problem_sizes, ffh_global_dofs_download, ffh_assemble, ffh_load
     public :: ffh_dirichlet, ffh_constrained_download, ffh_set_rhs, ffh_rhs_download
@@ -132,6 +132,15 @@ module fortfem_hip_capi
             integer(c_int), intent(out) :: global_dofs(*), edge_orient(*), face_perm(*)
             integer(c_int), intent(out) :: ndofs
         end function ffh_nedelec_dof_map
+
+        ! points: c_null_ptr to query nq, else c_loc of a (4, nq) array
+        integer(c_int) function ffh_nedelec_load_points(k, nq, points) &
+                bind(C, name="ffh_nedelec_load_points")
+            import :: c_int, c_ptr
+            integer(c_int), value :: k
+            integer(c_int), intent(out) :: nq
+            type(c_ptr), value :: points
+        end function ffh_nedelec_load_points
 
         integer(c_int) function ffh_nedelec_basis(k, npts, points, values, curls) &
                 bind(C, name="ffh_nedelec_basis")

@@ -377,7 +377,7 @@ end module fortfem_assembly_tetra_lagrange_arbitrary_order_3d
 ! src/assembly/assembly_tetra_nedelec_3d.f90: curl-mass and tensor-weighted
 ! first-kind Nedelec CSC assembly, orders 1..6, on the GPU.
 module fortfem_assembly_tetra_nedelec_3d
-    use, intrinsic :: iso_c_binding, only: c_int, c_int64_t, c_double, c_ptr
+    use, intrinsic :: iso_c_binding, only: c_int, c_int64_t, c_double, c_ptr, c_null_ptr, c_loc
     use fortfem_hip_types, only: dp, csc_t, fortsparse_status_t, status_set, &
         FORTSPARSE_INVALID_MATRIX
     use fortfem_hip_capi
@@ -389,7 +389,10 @@ module fortfem_assembly_tetra_nedelec_3d
     public :: assemble_tetra_nedelec_curl_mass_csc
     public :: assemble_tetra_nedelec_weighted_csc
     public :: assemble_tetra_nedelec_weighted_csc_cells
+    public :: assemble_tetra_nedelec_vector_load
+    public :: assemble_tetra_nedelec_vector_load_order
     public :: tensor_coefficient_3d
+    public :: vector_source_3d
 
     abstract interface
         pure subroutine tensor_coefficient_3d(x, y, z, value)
@@ -397,9 +400,85 @@ module fortfem_assembly_tetra_nedelec_3d
             real(dp), intent(in) :: x, y, z
             real(dp), intent(out) :: value(3, 3)
         end subroutine tensor_coefficient_3d
+        pure subroutine vector_source_3d(x, y, z, value)
+            import :: dp
+            real(dp), intent(in) :: x, y, z
+            real(dp), intent(out) :: value(3)
+        end subroutine vector_source_3d
     end interface
 
 contains
+
+    ! :2002-2040 (order 1).  The callback runs on the host at the mapped load
+    ! points x_q = v_1 + J xi_q (:2539-2541); the samples go to the device.
+    subroutine assemble_tetra_nedelec_vector_load(mesh_vertices, tetrahedra, &
+            source, vector, status)
+        real(dp), intent(in) :: mesh_vertices(:, :)
+        integer, intent(in) :: tetrahedra(:, :)
+        procedure(vector_source_3d) :: source
+        real(dp), allocatable, intent(out) :: vector(:)
+        type(fortsparse_status_t), intent(out) :: status
+
+        call assemble_tetra_nedelec_vector_load_order(mesh_vertices, tetrahedra, &
+            1, source, vector, status)
+    end subroutine assemble_tetra_nedelec_vector_load
+
+    ! :71-127 (orders 2..6; order 1 as above).
+    subroutine assemble_tetra_nedelec_vector_load_order(mesh_vertices, &
+            tetrahedra, order, source, vector, status)
+        real(dp), intent(in) :: mesh_vertices(:, :)
+        integer, intent(in) :: tetrahedra(:, :), order
+        procedure(vector_source_3d) :: source
+        real(dp), allocatable, intent(out) :: vector(:)
+        type(fortsparse_status_t), intent(out) :: status
+
+        real(c_double), allocatable, target :: pts(:, :)
+        real(c_double), allocatable :: samples(:, :, :)
+        real(dp) :: jac(3, 3), point(3)
+        type(c_ptr) :: prob
+        integer(c_int) :: nq, n_global, code
+        integer(c_int64_t) :: nnz
+        integer :: cell, q
+
+        call status_set(status, FORTSPARSE_INVALID_MATRIX, &
+            "Tetrahedral Nedelec vector load assembly failed")
+        if (size(mesh_vertices, 1) /= 3 .or. size(tetrahedra, 1) /= 4) return
+        if (size(tetrahedra, 2) < 1) return
+        if (any(tetrahedra < 1) .or. &
+            any(tetrahedra > size(mesh_vertices, 2))) return
+        if (order < 1 .or. order > 6) return
+        if (ffh_nedelec_load_points(int(order, c_int), nq, c_null_ptr) /= FFH_OK) return
+        allocate(pts(4, nq))
+        if (ffh_nedelec_load_points(int(order, c_int), nq, c_loc(pts)) /= FFH_OK) return
+        allocate(samples(3, nq, size(tetrahedra, 2)))
+        do cell = 1, size(tetrahedra, 2)
+            jac(:, 1) = mesh_vertices(:, tetrahedra(2, cell)) - &
+                mesh_vertices(:, tetrahedra(1, cell))
+            jac(:, 2) = mesh_vertices(:, tetrahedra(3, cell)) - &
+                mesh_vertices(:, tetrahedra(1, cell))
+            jac(:, 3) = mesh_vertices(:, tetrahedra(4, cell)) - &
+                mesh_vertices(:, tetrahedra(1, cell))
+            do q = 1, nq
+                point = mesh_vertices(:, tetrahedra(1, cell)) + &
+                    matmul(jac, pts(1:3, q))
+                call source(point(1), point(2), point(3), samples(:, q, cell))
+            end do
+        end do
+        code = ffh_problem_create(hip_context(), 3_c_int, &
+            int(size(mesh_vertices, 2), c_int), mesh_vertices, &
+            int(size(tetrahedra, 2), c_int), tetrahedra, prob)
+        if (code /= FFH_OK) return
+        code = ffh_pattern_build(prob, FFH_FAMILY_NEDELEC, int(order, c_int), &
+            n_global, nnz)
+        if (code == FFH_OK) code = ffh_load(prob, -1_c_int, FFH_SOURCE_SAMPLES, &
+            samples, int(size(samples), c_int64_t))
+        if (code == FFH_OK) then
+            allocate(vector(n_global))
+            code = ffh_rhs_download(prob, FFH_FULL, vector)
+        end if
+        if (code == FFH_OK) call status_set(status, 0, "")
+        code = ffh_problem_free(prob)
+    end subroutine assemble_tetra_nedelec_vector_load_order
 
     ! Shared driver: pattern + assembly of one Nedelec problem, downloaded.
     subroutine nedelec_csc(mesh_vertices, tetrahedra, k, kind, coeff, mass, &

@@ -138,6 +138,14 @@ int ffh_nedelec_dof_map(int k, int ntet, const int *tets, int base, int *global_
  * (3 x n x npts).  Host routine. */
 int ffh_nedelec_basis(int k, int npts, const double *points, double *values,
                       double *curls);
+/* The quadrature of the Nedelec vector load (assemble_tetra_nedelec_load_element,
+ * src/assembly/assembly_tetra_nedelec_3d.f90:2516-2553: the 4-point degree-2
+ * rule at k = 1; assemble_tetra_nedelec_load_element_order :525-582: the
+ * Duffy rule of degree 2k + 2 otherwise), points (4 x nq: x, y, z, weight)
+ * in the reference's order.  With points == NULL only nq is returned.  Hosts
+ * sample a vector source callback at the mapped points for
+ * FFH_SOURCE_SAMPLES.  Host routine. */
+int ffh_nedelec_load_points(int k, int *nq, double *points);
 
 /* ---- problems: mesh + space + operator resident in HBM -------------------- */
 /* Takes the (mesh_vertices, tetrahedra) arguments of
@@ -175,7 +183,12 @@ int ffh_assemble(ffh_problem *prob, int quadrature_degree, int coeff_kind,
                  const double *coeff, double mass_coefficient, int precision);
 
 /* assemble_tetra_lagrange_scalar_load (:705-774) / _load_samples (:473-528)
- * with the source given as a closed form, samples or a dof vector. */
+ * with the source given as a closed form, samples or a dof vector.  For the
+ * Nedelec family: assemble_tetra_nedelec_vector_load(_order)
+ * (src/assembly/assembly_tetra_nedelec_3d.f90:71-127, 2002-2040) with a
+ * constant vector (FFH_SOURCE_CONSTANT, params[0..2]) or samples
+ * f(3, nq, ncell) at the ffh_nedelec_load_points mapped into each cell
+ * (FFH_SOURCE_SAMPLES); quadrature_degree is ignored (the reference's rule). */
 int ffh_load(ffh_problem *prob, int quadrature_degree, int source_kind,
              const double *params, int64_t nparams);
 

@@ -502,3 +502,70 @@ def nedelec_csc(k, vertices, tets, alpha=1.0, beta=1.0, cell_tensor=None):
                                        rows.ctypes.data_as(C.c_void_p), cols.ctypes.data_as(C.c_void_p),
                                        vals.ctypes.data_as(C.c_void_p), C.byref(nd)), "nedelec_triplets")
     return csc_from_triplet(nd.value, nd.value, rows, cols, vals)
+
+
+def nedelec_load_rule(k):
+    """The load quadrature: tetra_degree_two_quadrature at k = 1
+    (src/assembly/assembly_tetra_nedelec_3d.f90:2570-2581), the Duffy rule
+    of degree 2k + 2 otherwise (:546-547).  -> (points (nq, 3), weights)."""
+    if k == 1:
+        a = (5.0 + 3.0 * np.sqrt(5.0)) / 20.0
+        b = (5.0 - np.sqrt(5.0)) / 20.0
+        return np.array([[b, b, b], [a, b, b], [b, a, b], [b, b, a]]), np.full(4, 1.0 / 24.0)
+    x, y, z, w = tetra_duffy(2 * k + 2)
+    return np.stack([x, y, z], axis=1), np.asarray(w)
+
+
+def nedelec_vector_load(k, vertices, tets, source):
+    """assemble_tetra_nedelec_vector_load (k = 1: :2002-2040 with the element
+    :2516-2553) and assemble_tetra_nedelec_vector_load_order (k >= 2: :71-127
+    with :525-582) restated: b_r = sum_q det w_q f(x_q) . J^-T v_r(xi_q),
+    x_q = v1 + J xi_q, oriented by the edge signs (k = 1) or T^T (k >= 2) and
+    summed into the global vector.  source(x, y, z) -> 3-vector.  Python
+    loops: small meshes only."""
+    v, t = _f(vertices), _i(tets)
+    n = nedelec_dof_count(k)
+    pts, w = nedelec_load_rule(k)
+    cf = nedelec_coefficients(k) if k >= 2 else None
+    V = [nedelec_eval(k, p, cf)[0] for p in pts]  # (n, 3) per point
+    if k == 1:
+        st, edges, gd, orient = edge_dof_map(t)
+        _check(st, "edge_dof_map")
+        nd = edges.shape[0]
+    else:
+        gd, eo, fp, nd, ne, nf = nedelec_dof_map(k, t)
+    vec = np.zeros(nd)
+    for e in range(t.shape[0]):
+        x = v[t[e] - 1]
+        J = np.stack([x[1] - x[0], x[2] - x[0], x[3] - x[0]], axis=1)
+        det = np.linalg.det(J)
+        Jinv = np.linalg.inv(J)
+        elem = np.zeros(n)
+        for q in range(len(w)):
+            phys = V[q] @ Jinv  # row j: (J^-T v_j)^T
+            f = np.asarray(source(*(x[0] + J @ pts[q])), dtype=np.float64)
+            elem += det * w[q] * (phys @ f)
+        if k == 1:
+            vec[gd[e] - 1] += orient[e] * elem
+        else:
+            T = nedelec_basis_transform(k, eo[e], fp[e])
+            vec[gd[e] - 1] += T.T @ elem
+    return vec
+
+
+def nedelec_field(k, vertices, tets, dofs, cell, xi):
+    """The global field sum_i dofs_i psi_i at reference point xi of a cell:
+    the cell's local coefficients T d_loc on the mapped reference basis."""
+    v, t = _f(vertices), _i(tets)
+    x = v[t[cell] - 1]
+    J = np.stack([x[1] - x[0], x[2] - x[0], x[3] - x[0]], axis=1)
+    Jinv = np.linalg.inv(J)
+    cf = nedelec_coefficients(k) if k >= 2 else None
+    vals = nedelec_eval(k, xi, cf)[0] @ Jinv
+    if k == 1:
+        st, edges, gd, orient = edge_dof_map(t)
+        c = orient[cell] * dofs[gd[cell] - 1]
+    else:
+        gd, eo, fp, nd, ne, nf = nedelec_dof_map(k, t)
+        c = nedelec_basis_transform(k, eo[cell], fp[cell]) @ dofs[gd[cell] - 1]
+    return c @ vals

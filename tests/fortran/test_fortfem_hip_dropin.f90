@@ -3,7 +3,8 @@
 ! Restated checks: test/test_structured_tetra_box_mesh.f90:19-51,
 ! test/test_tetra_lagrange_arbitrary_order_assembly.f90:37-66,
 ! test/test_advanced_solvers.f90:39-84,342-348,
-! test/test_tetra_lagrange_poisson_p_convergence.f90:26-40.
+! test/test_tetra_lagrange_poisson_p_convergence.f90:26-40,
+! test/test_tetra_nedelec_sparse_assembly_slow.f90:31-59.
 module dropin_test_sources
     use fortfem_hip_types, only: dp
     implicit none
@@ -21,6 +22,12 @@ contains
         value = 0.0_dp*(x + y + z)
     end subroutine zero_boundary
 
+    pure subroutine constant_field(x, y, z, value)
+        real(dp), intent(in) :: x, y, z
+        real(dp), intent(out) :: value(3)
+        value = [0.7_dp, -1.3_dp, 0.4_dp] + 0.0_dp*(x + y + z)
+    end subroutine constant_field
+
     pure subroutine sin_source(x, y, z, value)
         real(dp), intent(in) :: x, y, z
         real(dp), intent(out) :: value
@@ -36,12 +43,13 @@ program test_fortfem_hip_dropin
     use fortfem_structured_tetra_box_mesh, only: generate_structured_tetra_box_mesh
     use fortfem_assembly_tetra_lagrange_arbitrary_order_3d, only: &
         assemble_tetra_lagrange_stiffness_csc
-    use fortfem_assembly_tetra_nedelec_3d, only: assemble_tetra_nedelec_curl_mass_csc
+    use fortfem_assembly_tetra_nedelec_3d, only: assemble_tetra_nedelec_curl_mass_csc, &
+        assemble_tetra_nedelec_vector_load
     use fortfem_sparse_matrix, only: spmv
     use fortfem_advanced_solvers, only: solver_options_t, solver_stats_t, &
         solver_options, solve_sparse
     use fortfem_tetra_lagrange_solver_3d, only: solve_tetra_lagrange_poisson
-    use dropin_test_sources, only: bubble_source, zero_boundary, sin_source
+    use dropin_test_sources, only: bubble_source, zero_boundary, sin_source, constant_field
     implicit none
 
     logical :: all_passed
@@ -53,6 +61,7 @@ program test_fortfem_hip_dropin
     call check_quartic_p4()
     call check_p2_box_poisson()
     call check_nedelec_rotational_energy()
+    call check_nedelec_constant_load()
     call fortfem_hip_finalize()
     if (.not. all_passed) error stop 1
     write (*, '(a)') "drop-in Fortran checks passed"
@@ -159,6 +168,54 @@ contains
         call record(abs(dot_product(dofs, product) - 28.0_dp/3.0_dp) < 3.0e-12_dp, &
             "Nedelec rotational-field energy is 28/3")
     end subroutine check_nedelec_rotational_energy
+
+    ! assemble_tetra_nedelec_vector_load (:2002-2040): a constant field f lies
+    ! in the Whitney space with dofs d_e = f . (v_b - v_a) on the edges
+    ! (a, b) = (min, max) by first appearance, so its load is M d with M the
+    ! mass matrix (curl coefficient 0).
+    subroutine check_nedelec_constant_load()
+        real(dp) :: mesh_vertices(3, 5)
+        real(dp), allocatable :: dofs(:), product(:), load(:)
+        integer :: tetrahedra(4, 2), edges(2, 12), cell, l, a, b, e, nedges
+        integer, parameter :: local(2, 6) = reshape([1, 2, 1, 3, 1, 4, 2, 3, 2, 4, 3, 4], [2, 6])
+        logical :: found
+        type(csc_t) :: mass
+        type(fortsparse_status_t) :: status
+
+        mesh_vertices = reshape([0.0_dp, 0.0_dp, 0.0_dp, 1.0_dp, 0.1_dp, 0.0_dp, &
+            0.2_dp, 1.0_dp, 0.0_dp, 0.0_dp, 0.3_dp, 1.0_dp, 0.1_dp, 0.0_dp, -1.0_dp], [3, 5])
+        tetrahedra(:, 1) = [1, 2, 3, 4]
+        tetrahedra(:, 2) = [1, 3, 2, 5]
+        call assemble_tetra_nedelec_curl_mass_csc(mesh_vertices, tetrahedra, mass, &
+            status, curl_coefficient=0.0_dp, mass_coefficient=1.0_dp, order=1)
+        call record(status%code == 0, "Nedelec k=1 mass assembly succeeds")
+        call assemble_tetra_nedelec_vector_load(mesh_vertices, tetrahedra, &
+            constant_field, load, status)
+        call record(status%code == 0 .and. size(load) == 9, "Nedelec k=1 vector load succeeds")
+        nedges = 0
+        do cell = 1, 2
+            do l = 1, 6
+                a = minval(tetrahedra(local(:, l), cell))
+                b = maxval(tetrahedra(local(:, l), cell))
+                found = .false.
+                do e = 1, nedges
+                    if (edges(1, e) == a .and. edges(2, e) == b) found = .true.
+                end do
+                if (.not. found) then
+                    nedges = nedges + 1
+                    edges(:, nedges) = [a, b]
+                end if
+            end do
+        end do
+        allocate(dofs(nedges), product(nedges))
+        do e = 1, nedges
+            dofs(e) = dot_product([0.7_dp, -1.3_dp, 0.4_dp], &
+                mesh_vertices(:, edges(2, e)) - mesh_vertices(:, edges(1, e)))
+        end do
+        call spmv(mass, dofs, product)
+        call record(maxval(abs(load - product)) < 1.0e-13_dp*maxval(abs(product)), &
+            "Nedelec constant-field load equals the mass times the interpolant")
+    end subroutine check_nedelec_constant_load
 
     subroutine check_linear_energy()
         real(dp) :: mesh_vertices(3, 5), coefficients(5), product(5)

@@ -146,7 +146,12 @@ def test_fp32_contraction_error(ff, k):
 def test_zero_tangential_solve_parity(ff):
     """Curl-mass system with zero tangential trace (build_zero_tangential_mask,
     fortfem_tetra_nedelec_solver_3d.f90:431-470) solved by Jacobi-PCG against
-    the oracle's reduction and PCG."""
+    the oracle's reduction and PCG.
+
+    This system takes ~800 iterations; over that many steps CG's iteration
+    count moves with the summation order of the dot products (the GPU sums
+    per-block partials, the oracle left to right), so the count is held to
+    2 % of the oracle's and the solution to 1e-9."""
     v, t = renumbered_box(3, 9)
     k = 2
     p = ff.Problem(v, t)
@@ -165,7 +170,7 @@ def test_zero_tangential_solve_parity(ff):
     x, st = p.solve(opts)
     xo, sto = O.solve_sparse(red, rb, method="pcg", preconditioner="jacobi", tolerance=1e-10,
                              max_iterations=5000, breakdown="relative")
-    assert st.converged and abs(st.iterations - sto.iterations) <= 1
+    assert st.converged and abs(st.iterations - sto.iterations) <= max(1, 0.02 * sto.iterations)
     xr = np.zeros(G)
     xr[free - 1] = xo
     assert np.linalg.norm(x - xr) <= 1e-9 * np.linalg.norm(xr)
@@ -192,3 +197,24 @@ def _tangential_mask(k, v, t):
                 mask[e * k:(e + 1) * k] = True
         mask[ne * k + fi * fdc: ne * k + (fi + 1) * fdc] = True
     return mask
+
+
+@pytest.mark.parametrize("k", [1, 2, 3])
+def test_vector_load_parity(ff, k):
+    """assemble_tetra_nedelec_vector_load(_order) on the GPU (sampled source at
+    the reference's load points, oriented class tables, colour scatter)
+    against the oracle, on a mesh with every orientation class present."""
+    v, t = renumbered_box(2 if k <= 2 else 1, k)
+    f = lambda x, y, z: (np.sin(x) + y, y * z - 0.5, np.cos(z) * x)
+    b, st = ff.assemble_tetra_nedelec_vector_load(v, t, f, order=k)
+    assert st.code == 0
+    bo = O.nedelec_vector_load(k, v, t, lambda x, y, z: np.array(f(x, y, z)))
+    assert b.shape == bo.shape
+    assert np.max(np.abs(b - bo)) <= 1e-12 * np.max(np.abs(bo))
+    # a constant vector through FFH_SOURCE_CONSTANT gives the same as samples
+    p = ff.Problem(v, t)
+    p.pattern_build(ff.capi.FAMILY_NEDELEC, k)
+    p.load(-1, ff.capi.SOURCE_CONSTANT, [0.3, -1.1, 2.0])
+    bc = p.rhs(ff.capi.FULL)
+    bco = O.nedelec_vector_load(k, v, t, lambda x, y, z: np.array([0.3, -1.1, 2.0]))
+    assert np.max(np.abs(bc - bco)) <= 1e-12 * np.max(np.abs(bco))
