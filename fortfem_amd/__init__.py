@@ -456,6 +456,70 @@ class Matrix:
         method = "cg" if opts.method.strip().lower() == "cg" else "pcg"
         return SolverStatsT.from_c(st, method)
 
+    def spmv_jvp(self, x, values_dot, x_dot):
+        """spmv_jvp (fortfem_sparse_matrix.f90:60-91): A x_dot + A_dot x."""
+        n, nnz = self.size()
+        xx, vd, xd = _f64(x), _f64(values_dot), _f64(x_dot)
+        if xx.shape[0] != n:
+            raise ValueError("spmv_jvp: vector has incompatible size")
+        if xd.shape[0] != n:
+            raise ValueError("spmv_jvp: vector_dot has incompatible size")
+        if vd.shape[0] != nnz:
+            raise ValueError("spmv_jvp: matrix value tangent has incompatible size")
+        y = np.zeros(n)
+        self.ctx.check(self._lib.ffh_spmv_jvp(self.handle, _ptr(xx), _ptr(vd), _ptr(xd), _ptr(y)))
+        return y
+
+    def spmv_vjp(self, x, y_bar):
+        """spmv_vjp (fortfem_sparse_matrix.f90:93-126) -> (values_bar, x_bar)."""
+        n, nnz = self.size()
+        xx, yb = _f64(x), _f64(y_bar)
+        if xx.shape[0] != n:
+            raise ValueError("spmv_vjp: vector has incompatible size")
+        if yb.shape[0] != n:
+            raise ValueError("spmv_vjp: product_bar has incompatible size")
+        vb = np.zeros(max(nnz, 1))
+        xb = np.zeros(n)
+        self.ctx.check(self._lib.ffh_spmv_vjp(self.handle, _ptr(xx), _ptr(yb), _ptr(vb), _ptr(xb)))
+        return vb[:nnz], xb
+
+    def solve_jvp(self, x, values_dot, b_dot, opts: SolverOptionsT):
+        """Tangent of a converged solve (pcg_solve_jvp, advanced_solvers.f90:458-489,
+        on the stored pattern): x_dot solves A x_dot = b_dot - A_dot x.
+        Returns (x_dot, status, stats); status 0, or 2 when the tangent solve
+        did not converge."""
+        n, nnz = self.size()
+        xx, vd, bd = _f64(x), _f64(values_dot), _f64(b_dot)
+        if xx.shape[0] != n or bd.shape[0] != n or vd.shape[0] != nnz:
+            raise ValueError("solve_jvp: incompatible sizes")
+        xd = np.zeros(n)
+        st = capi.SolverStats()
+        self.ctx.check(self._lib.ffh_solve_sparse_jvp(self.handle, _ptr(xx), _ptr(vd), _ptr(bd),
+                                                      C.byref(opts.to_c()), _ptr(xd), C.byref(st)))
+        method = "cg" if opts.method.strip().lower() == "cg" else "pcg"
+        stats = SolverStatsT.from_c(st, method)
+        return xd, (0 if stats.converged else 2), stats
+
+    def solve_vjp(self, x, x_bar, opts: SolverOptionsT):
+        """Cotangent of a converged solve (pcg_solve_vjp, advanced_solvers.f90:491-522):
+        lambda solves A^T lambda = x_bar; returns (values_bar, b_bar, status,
+        stats) with b_bar = lambda and values_bar = -lambda x^T on the stored
+        entries (zeros and status 2 when the adjoint solve did not converge)."""
+        n, nnz = self.size()
+        xx, xb = _f64(x), _f64(x_bar)
+        if xx.shape[0] != n or xb.shape[0] != n:
+            raise ValueError("solve_vjp: incompatible sizes")
+        vb = np.zeros(max(nnz, 1))
+        bb = np.zeros(n)
+        st = capi.SolverStats()
+        self.ctx.check(self._lib.ffh_solve_sparse_vjp(self.handle, _ptr(xx), _ptr(xb), C.byref(opts.to_c()),
+                                                      _ptr(vb), _ptr(bb), C.byref(st)))
+        method = "cg" if opts.method.strip().lower() == "cg" else "pcg"
+        stats = SolverStatsT.from_c(st, method)
+        if not stats.converged:
+            return np.zeros(nnz), np.zeros(n), 2, stats
+        return vb[:nnz], bb, 0, stats
+
     def spmv_variant(self, precision="fp64", request=-1):
         """The SpMV variant in use (autotuned on first use); request >= 0 forces one."""
         out = C.c_int(0)
@@ -471,6 +535,40 @@ def spmv(matrix: CSC, vector, ctx: Context | None = None):
     if v.shape[0] != matrix.ncol:
         raise ValueError("spmv: vector has incompatible size")
     return Matrix.upload(matrix, ctx).spmv(v)
+
+
+def spmv_jvp(matrix: CSC, vector, matrix_values_dot, vector_dot, ctx: Context | None = None):
+    """spmv_jvp (fortfem_sparse_matrix.f90:60-91) -> product_dot."""
+    if _f64(vector).shape[0] != matrix.ncol:
+        raise ValueError("spmv_jvp: vector has incompatible size")
+    return Matrix.upload(matrix, ctx).spmv_jvp(vector, matrix_values_dot, vector_dot)
+
+
+def spmv_vjp(matrix: CSC, vector, product_bar, ctx: Context | None = None):
+    """spmv_vjp (fortfem_sparse_matrix.f90:93-126) -> (matrix_values_bar, vector_bar)."""
+    if _f64(vector).shape[0] != matrix.ncol:
+        raise ValueError("spmv_vjp: vector has incompatible size")
+    return Matrix.upload(matrix, ctx).spmv_vjp(vector, product_bar)
+
+
+def solve_sparse_jvp(a_sparse: CSC, b, x, opts: SolverOptionsT, values_dot, b_dot,
+                     ctx: Context | None = None):
+    """Sparse pcg_solve_jvp / cg_solve_jvp (advanced_solvers.f90:387-419, 458-489):
+    (x_dot, status).  b is the primal right-hand side, kept for the
+    converged-state contract as in the reference; it is not read."""
+    if a_sparse.ncol != a_sparse.nrow or _f64(b).shape[0] != a_sparse.nrow:
+        raise ValueError("solve_sparse_jvp: incompatible sizes")
+    xd, status, _ = Matrix.upload(a_sparse, ctx).solve_jvp(x, values_dot, b_dot, opts)
+    return xd, status
+
+
+def solve_sparse_vjp(a_sparse: CSC, b, x, opts: SolverOptionsT, x_bar, ctx: Context | None = None):
+    """Sparse pcg_solve_vjp / cg_solve_vjp (advanced_solvers.f90:421-456, 491-522):
+    (values_bar, b_bar, status)."""
+    if a_sparse.ncol != a_sparse.nrow or _f64(b).shape[0] != a_sparse.nrow:
+        raise ValueError("solve_sparse_vjp: incompatible sizes")
+    vb, bb, status, _ = Matrix.upload(a_sparse, ctx).solve_vjp(x, x_bar, opts)
+    return vb, bb, status
 
 
 def solve_sparse(a_sparse: CSC, b, x, opts: SolverOptionsT, ctx: Context | None = None):

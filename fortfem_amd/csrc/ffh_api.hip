@@ -584,6 +584,122 @@ int ffh_solve_sparse(ffh_matrix *M, const double *b, double *x, const ffh_solver
     });
 }
 
+int ffh_spmv_jvp(ffh_matrix *M, const double *x, const double *values_dot, const double *x_dot,
+                 double *y_dot)
+{
+    if (!M) return FFH_ERR_HANDLE;
+    if (!x || !values_dot || !x_dot || !y_dot) return FFH_ERR_INVALID;
+    return guarded(M->ctx, [&] {
+        Context *ctx = M->ctx;
+        const int n = M->m.n;
+        hipStream_t s = ctx->stream;
+        ensure_val64(ctx, M->m);
+        DBuf<double> dx, dvd, dxd, dy;
+        dx.upload(x, n, s);
+        dvd.upload(values_dot, M->m.nnz, s);
+        dxd.upload(x_dot, n, s);
+        dy.alloc(n);
+        csc_jvp(ctx, M->m, dvd.p, dx.p, dxd.p, dy.p);
+        dy.download(y_dot, n, s);
+        FFH_HIP(hipStreamSynchronize(s));
+    });
+}
+
+int ffh_spmv_vjp(ffh_matrix *M, const double *x, const double *y_bar, double *values_bar,
+                 double *x_bar)
+{
+    if (!M) return FFH_ERR_HANDLE;
+    if (!x || !y_bar || !values_bar || !x_bar) return FFH_ERR_INVALID;
+    return guarded(M->ctx, [&] {
+        Context *ctx = M->ctx;
+        const int n = M->m.n;
+        hipStream_t s = ctx->stream;
+        ensure_val64(ctx, M->m);
+        DBuf<double> dx, dyb, dvb, dxb;
+        dx.upload(x, n, s);
+        dyb.upload(y_bar, n, s);
+        dvb.alloc(std::max<int64_t>(M->m.nnz, 1));
+        dxb.alloc(n);
+        csc_vjp(ctx, M->m, dx.p, dyb.p, 1.0, dvb.p, dxb.p);
+        dvb.download(values_bar, M->m.nnz, s);
+        dxb.download(x_bar, n, s);
+        FFH_HIP(hipStreamSynchronize(s));
+    });
+}
+
+int ffh_solve_sparse_jvp(ffh_matrix *M, const double *x, const double *values_dot,
+                         const double *b_dot, const ffh_solver_options *o, double *x_dot,
+                         ffh_solver_stats *st)
+{
+    if (!M) return FFH_ERR_HANDLE;
+    if (!x || !values_dot || !b_dot || !x_dot || !st) return FFH_ERR_INVALID;
+    return guarded(M->ctx, [&] {
+        check_options(o);
+        FFH_REQUIRE(M->m.symmetric, FFH_ERR_INVALID, "CG tangents need a symmetric matrix");
+        FFH_REQUIRE(o->max_iterations >= 1 && o->tolerance > 0.0, FFH_ERR_INVALID,
+                    "max_iterations >= 1 and tolerance > 0");
+        std::memset(st, 0, sizeof(*st));
+        Context *ctx = M->ctx;
+        const int n = M->m.n;
+        hipStream_t s = ctx->stream;
+        ensure_val64(ctx, M->m);
+        DBuf<double> dx, dvd, rhs, xd;
+        dx.upload(x, n, s);
+        dvd.upload(values_dot, M->m.nnz, s);
+        rhs.alloc(n);
+        // A_dot x, then tangent_rhs = b_dot - A_dot x (:410-411)
+        csc_jvp(ctx, M->m, dvd.p, dx.p, nullptr, rhs.p);
+        std::vector<double> h(n);
+        rhs.download(h.data(), n, s);
+        FFH_HIP(hipStreamSynchronize(s));
+        for (int i = 0; i < n; ++i) h[i] = b_dot[i] - h[i];
+        rhs.upload(h, s);
+        xd.alloc(n);
+        FFH_HIP(hipMemsetAsync(xd.p, 0, sizeof(double) * n, s));
+        if (o->precision == FFH_MIXED)
+            solve_mixed(ctx, *M, rhs.p, xd.p, *o, *st);
+        else
+            solve_fp64(ctx, *M, rhs.p, xd.p, *o, *st);
+        xd.download(x_dot, n, s);
+        FFH_HIP(hipStreamSynchronize(s));
+    });
+}
+
+int ffh_solve_sparse_vjp(ffh_matrix *M, const double *x, const double *x_bar,
+                         const ffh_solver_options *o, double *values_bar, double *b_bar,
+                         ffh_solver_stats *st)
+{
+    if (!M) return FFH_ERR_HANDLE;
+    if (!x || !x_bar || !values_bar || !b_bar || !st) return FFH_ERR_INVALID;
+    return guarded(M->ctx, [&] {
+        check_options(o);
+        FFH_REQUIRE(M->m.symmetric, FFH_ERR_INVALID, "CG adjoints need a symmetric matrix");
+        FFH_REQUIRE(o->max_iterations >= 1 && o->tolerance > 0.0, FFH_ERR_INVALID,
+                    "max_iterations >= 1 and tolerance > 0");
+        std::memset(st, 0, sizeof(*st));
+        Context *ctx = M->ctx;
+        const int n = M->m.n;
+        hipStream_t s = ctx->stream;
+        ensure_val64(ctx, M->m);
+        DBuf<double> dx, dxb, lam, dvb;
+        dx.upload(x, n, s);
+        dxb.upload(x_bar, n, s);
+        lam.alloc(n);
+        FFH_HIP(hipMemsetAsync(lam.p, 0, sizeof(double) * n, s));
+        // A^T lambda = x_bar (A = A^T here), b_bar = lambda,
+        // A_bar = -lambda x^T on the stored entries (:441-448)
+        if (o->precision == FFH_MIXED)
+            solve_mixed(ctx, *M, dxb.p, lam.p, *o, *st);
+        else
+            solve_fp64(ctx, *M, dxb.p, lam.p, *o, *st);
+        dvb.alloc(std::max<int64_t>(M->m.nnz, 1));
+        csc_vjp(ctx, M->m, dx.p, lam.p, -1.0, dvb.p, nullptr);
+        dvb.download(values_bar, M->m.nnz, s);
+        lam.download(b_bar, n, s);
+        FFH_HIP(hipStreamSynchronize(s));
+    });
+}
+
 int ffh_bench_spmv(ffh_matrix *M, int precision, int reps, double *avg_ms, double *bytes)
 {
     if (!M) return FFH_ERR_HANDLE;

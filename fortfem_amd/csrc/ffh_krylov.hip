@@ -1491,3 +1491,73 @@ void invalidate_graphs(Workspace &w)
 }
 
 } // namespace ffh
+
+namespace ffh {
+
+// ------------------------------------------------- differentiable products --
+// Column form of the reference's loops (fortfem_sparse_matrix.f90:60-126):
+// one thread per CSC column j.
+//   jvp: y(row) += val(k) x_dot(j) + val_dot(k) x(j)   (val or x_dot may be
+//        absent: the A_dot x term of the implicit solve tangent)
+//   vjp: val_bar(k) = scale y_bar(row) x(j) ; x_bar(j) = sum_k val(k) y_bar(row)
+// The jvp scatters into rows with FP64 atomics (summation order free, as
+// for any sum); the vjp gathers per column (deterministic).
+__global__ __launch_bounds__(kBlock) void k_csc_jvp(int ncol, const int *__restrict__ ptr,
+                                                    const int *__restrict__ idx,
+                                                    const double *__restrict__ val,
+                                                    const double *__restrict__ vdot,
+                                                    const double *__restrict__ x,
+                                                    const double *__restrict__ xdot,
+                                                    double *__restrict__ y)
+{
+    const int j = blockIdx.x * kBlock + threadIdx.x;
+    if (j >= ncol) return;
+    const double xj = x ? x[j] : 0.0, dj = xdot ? xdot[j] : 0.0;
+    for (int k = ptr[j]; k < ptr[j + 1]; ++k) {
+        double t = 0.0;
+        if (val && xdot) t += val[k] * dj;
+        if (vdot && x) t += vdot[k] * xj;
+        atomicAdd(&y[idx[k]], t);
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_csc_vjp(int ncol, const int *__restrict__ ptr,
+                                                    const int *__restrict__ idx,
+                                                    const double *__restrict__ val,
+                                                    const double *__restrict__ x,
+                                                    const double *__restrict__ ybar,
+                                                    double scale, double *__restrict__ vbar,
+                                                    double *__restrict__ xbar)
+{
+    const int j = blockIdx.x * kBlock + threadIdx.x;
+    if (j >= ncol) return;
+    const double xj = x[j];
+    double s = 0.0;
+    for (int k = ptr[j]; k < ptr[j + 1]; ++k) {
+        const double yb = ybar[idx[k]];
+        if (vbar) vbar[k] = scale * yb * xj;
+        s += val[k] * yb;
+    }
+    if (xbar) xbar[j] = s;
+}
+
+void csc_jvp(Context *ctx, const DevMatrix &A, const double *vdot, const double *x,
+             const double *xdot, double *y)
+{
+    const int n = A.n;
+    FFH_HIP(hipMemsetAsync(y, 0, sizeof(double) * n, ctx->stream));
+    hipLaunchKernelGGL(k_csc_jvp, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, ctx->stream, n,
+                       A.col_ptr.p, A.row_idx.p, xdot ? A.val.p : nullptr, vdot, x, xdot, y);
+    FFH_HIP(hipGetLastError());
+}
+
+void csc_vjp(Context *ctx, const DevMatrix &A, const double *x, const double *ybar, double scale,
+             double *vbar, double *xbar)
+{
+    const int n = A.n;
+    hipLaunchKernelGGL(k_csc_vjp, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, ctx->stream, n,
+                       A.col_ptr.p, A.row_idx.p, A.val.p, x, ybar, scale, vbar, xbar);
+    FFH_HIP(hipGetLastError());
+}
+
+} // namespace ffh

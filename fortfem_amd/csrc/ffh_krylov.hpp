@@ -41,3 +41,14 @@ void ensure_sell(Context *ctx, DevMatrix &A, bool f32);
 void sell_values(Context *ctx, const DevMatrix &A, const double *val, double *out64,
                  float *out32);
 } // namespace ffh
+
+namespace ffh {
+// differentiable products over the CSC arrays (ffh_krylov.hip):
+// y = A x_dot + A_dot x (either term may be absent), and
+// val_bar = scale (y_bar_row x_col) per entry, x_bar = A^T y_bar (either
+// output may be absent)
+void csc_jvp(Context *ctx, const DevMatrix &A, const double *vdot, const double *x,
+             const double *xdot, double *y);
+void csc_vjp(Context *ctx, const DevMatrix &A, const double *x, const double *ybar, double scale,
+             double *vbar, double *xbar);
+} // namespace ffh

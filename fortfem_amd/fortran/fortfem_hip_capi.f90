@@ -67,6 +67,7 @@ module fortfem_hip_capi
     public :: ffh_problem_matrix, ffh_problem_solve, ffh_matrix_upload
     public :: ffh_matrix_free, ffh_matrix_download, ffh_spmv, ffh_solve_sparse
     public :: ffh_bench_spmv, ffh_set_profiling, ffh_last_solve_profile
+    public :: ffh_spmv_jvp, ffh_spmv_vjp, ffh_solve_sparse_jvp, ffh_solve_sparse_vjp
 
     interface
         integer(c_int) function ffh_init(device, ctx) bind(C, name="ffh_init")
@@ -303,6 +304,42 @@ module fortfem_hip_capi
             type(ffh_solver_options), intent(in) :: opts
             type(ffh_solver_stats), intent(out) :: stats
         end function ffh_solve_sparse
+
+        integer(c_int) function ffh_spmv_jvp(mat, x, values_dot, x_dot, y_dot) &
+                bind(C, name="ffh_spmv_jvp")
+            import :: c_int, c_double, c_ptr
+            type(c_ptr), value :: mat
+            real(c_double), intent(in) :: x(*), values_dot(*), x_dot(*)
+            real(c_double), intent(out) :: y_dot(*)
+        end function ffh_spmv_jvp
+
+        integer(c_int) function ffh_spmv_vjp(mat, x, y_bar, values_bar, x_bar) &
+                bind(C, name="ffh_spmv_vjp")
+            import :: c_int, c_double, c_ptr
+            type(c_ptr), value :: mat
+            real(c_double), intent(in) :: x(*), y_bar(*)
+            real(c_double), intent(out) :: values_bar(*), x_bar(*)
+        end function ffh_spmv_vjp
+
+        integer(c_int) function ffh_solve_sparse_jvp(mat, x, values_dot, b_dot, &
+                opts, x_dot, stats) bind(C, name="ffh_solve_sparse_jvp")
+            import :: c_int, c_double, c_ptr, ffh_solver_options, ffh_solver_stats
+            type(c_ptr), value :: mat
+            real(c_double), intent(in) :: x(*), values_dot(*), b_dot(*)
+            type(ffh_solver_options), intent(in) :: opts
+            real(c_double), intent(out) :: x_dot(*)
+            type(ffh_solver_stats), intent(out) :: stats
+        end function ffh_solve_sparse_jvp
+
+        integer(c_int) function ffh_solve_sparse_vjp(mat, x, x_bar, opts, &
+                values_bar, b_bar, stats) bind(C, name="ffh_solve_sparse_vjp")
+            import :: c_int, c_double, c_ptr, ffh_solver_options, ffh_solver_stats
+            type(c_ptr), value :: mat
+            real(c_double), intent(in) :: x(*), x_bar(*)
+            type(ffh_solver_options), intent(in) :: opts
+            real(c_double), intent(out) :: values_bar(*), b_bar(*)
+            type(ffh_solver_stats), intent(out) :: stats
+        end function ffh_solve_sparse_vjp
 
         integer(c_int) function ffh_bench_spmv(mat, precision, reps, avg_ms, &
                 bytes) bind(C, name="ffh_bench_spmv")

@@ -622,12 +622,13 @@ end module fortfem_assembly_tetra_nedelec_3d
 module fortfem_sparse_matrix
     use, intrinsic :: iso_c_binding, only: c_int, c_int64_t, c_ptr
     use fortfem_hip_types, only: dp, sparse_matrix_t => csc_t
-    use fortfem_hip_capi, only: ffh_matrix_upload, ffh_matrix_free, ffh_spmv, FFH_OK
+    use fortfem_hip_capi, only: ffh_matrix_upload, ffh_matrix_free, ffh_spmv, &
+        ffh_spmv_jvp, ffh_spmv_vjp, FFH_OK
     use fortfem_hip_runtime, only: hip_context
     implicit none
     private
 
-    public :: sparse_matrix_t, spmv
+    public :: sparse_matrix_t, spmv, spmv_jvp, spmv_vjp
 
 contains
 
@@ -655,6 +656,69 @@ contains
         code = ffh_matrix_free(mat)
     end subroutine spmv
 
+    ! :60-91.  product_dot = A vector_dot + A_dot vector, A_dot given by its
+    ! CSC values on the matrix's pattern.
+    subroutine spmv_jvp(matrix, vector, matrix_values_dot, vector_dot, product_dot)
+        type(sparse_matrix_t), intent(in) :: matrix
+        real(dp), intent(in) :: vector(:), matrix_values_dot(:)
+        real(dp), intent(in) :: vector_dot(:)
+        real(dp), intent(out) :: product_dot(:)
+
+        type(c_ptr) :: mat
+        integer(c_int) :: code
+
+        if (size(vector) /= matrix%ncol) then
+            error stop "spmv_jvp: vector has incompatible size"
+        end if
+        if (size(vector_dot) /= matrix%ncol) then
+            error stop "spmv_jvp: vector_dot has incompatible size"
+        end if
+        if (size(matrix_values_dot) /= matrix%nnz) then
+            error stop "spmv_jvp: matrix value tangent has incompatible size"
+        end if
+        if (size(product_dot) /= matrix%nrow) then
+            error stop "spmv_jvp: product_dot has incompatible size"
+        end if
+        code = ffh_matrix_upload(hip_context(), int(matrix%ncol, c_int), &
+            int(matrix%nnz, c_int64_t), 1_c_int, matrix%col_ptr, matrix%row_idx, &
+            matrix%val, mat)
+        if (code /= FFH_OK) error stop "spmv_jvp: device upload failed"
+        code = ffh_spmv_jvp(mat, vector, matrix_values_dot, vector_dot, product_dot)
+        if (code /= FFH_OK) error stop "spmv_jvp: device product failed"
+        code = ffh_matrix_free(mat)
+    end subroutine spmv_jvp
+
+    ! :93-126.  matrix_values_bar(k) = product_bar(row_k) vector(col_k),
+    ! vector_bar = A^T product_bar.
+    subroutine spmv_vjp(matrix, vector, product_bar, matrix_values_bar, vector_bar)
+        type(sparse_matrix_t), intent(in) :: matrix
+        real(dp), intent(in) :: vector(:), product_bar(:)
+        real(dp), intent(out) :: matrix_values_bar(:), vector_bar(:)
+
+        type(c_ptr) :: mat
+        integer(c_int) :: code
+
+        if (size(vector) /= matrix%ncol) then
+            error stop "spmv_vjp: vector has incompatible size"
+        end if
+        if (size(product_bar) /= matrix%nrow) then
+            error stop "spmv_vjp: product_bar has incompatible size"
+        end if
+        if (size(matrix_values_bar) /= matrix%nnz) then
+            error stop "spmv_vjp: matrix value cotangent has incompatible size"
+        end if
+        if (size(vector_bar) /= matrix%ncol) then
+            error stop "spmv_vjp: vector_bar has incompatible size"
+        end if
+        code = ffh_matrix_upload(hip_context(), int(matrix%ncol, c_int), &
+            int(matrix%nnz, c_int64_t), 1_c_int, matrix%col_ptr, matrix%row_idx, &
+            matrix%val, mat)
+        if (code /= FFH_OK) error stop "spmv_vjp: device upload failed"
+        code = ffh_spmv_vjp(mat, vector, product_bar, matrix_values_bar, vector_bar)
+        if (code /= FFH_OK) error stop "spmv_vjp: device product failed"
+        code = ffh_matrix_free(mat)
+    end subroutine spmv_vjp
+
 end module fortfem_sparse_matrix
 
 module fortfem_advanced_solvers
@@ -666,6 +730,7 @@ module fortfem_advanced_solvers
     private
 
     public :: solver_options_t, solver_stats_t, solver_options, solve_sparse
+    public :: solve_sparse_jvp, solve_sparse_vjp
     public :: jacobi_preconditioner, to_c_options
 
     ! src/solvers/advanced_solvers.f90:32-44 (+ precision / breakdown fields)
@@ -788,6 +853,90 @@ contains
         stats%method_used = "pcg"
         if (trim(opts%method) == "cg") stats%method_used = "cg"
     end subroutine solve_sparse
+
+    ! Sparse counterpart of pcg_solve_jvp / cg_solve_jvp (:387-419, 458-489)
+    ! for a converged x = A^-1 b: x_dot solves A x_dot = b_dot - A_dot x with
+    ! A_dot given by its CSC values on A's pattern.  status 0, 1 (invalid
+    ! input), 2 (tangent solve not converged).  b is the primal right-hand
+    ! side, kept for the converged-state contract; it is not read.
+    subroutine solve_sparse_jvp(A_sparse, b, x, opts, A_values_dot, b_dot, x_dot, status)
+        type(sparse_matrix_t), intent(in) :: A_sparse
+        real(dp), intent(in) :: b(:), x(:), A_values_dot(:), b_dot(:)
+        type(solver_options_t), intent(in) :: opts
+        real(dp), intent(out) :: x_dot(:)
+        integer, intent(out) :: status
+
+        type(c_ptr) :: mat
+        type(ffh_solver_stats) :: cst
+        integer(c_int) :: code, freed
+        integer :: n
+
+        x_dot = 0.0_dp
+        status = 1
+        n = A_sparse%nrow
+        if (A_sparse%ncol /= n .or. size(b) /= n .or. size(x) /= n .or. &
+            size(b_dot) /= n .or. size(x_dot) /= n .or. &
+            size(A_values_dot) /= A_sparse%nnz) return
+        if (opts%max_iterations < 1 .or. opts%tolerance <= 0.0_dp) return
+        code = ffh_matrix_upload(hip_context(), int(n, c_int), &
+            int(A_sparse%nnz, c_int64_t), 1_c_int, A_sparse%col_ptr, &
+            A_sparse%row_idx, A_sparse%val, mat)
+        if (code /= FFH_OK) return
+        code = ffh_solve_sparse_jvp(mat, x, A_values_dot, b_dot, to_c_options(opts), &
+            x_dot, cst)
+        freed = ffh_matrix_free(mat)
+        if (code /= FFH_OK) then
+            x_dot = 0.0_dp
+            return
+        end if
+        status = 0
+        if (cst%converged == 0) status = 2
+    end subroutine solve_sparse_jvp
+
+    ! Sparse counterpart of pcg_solve_vjp / cg_solve_vjp (:421-456, 491-522):
+    ! lambda solves A^T lambda = x_bar; b_bar = lambda and A_values_bar(k) =
+    ! -lambda(row_k) x(col_k), the reference's A_bar = -lambda x^T on A's
+    ! stored entries.  Zeros and status 2 when the adjoint solve does not
+    ! converge.
+    subroutine solve_sparse_vjp(A_sparse, b, x, opts, x_bar, A_values_bar, b_bar, status)
+        type(sparse_matrix_t), intent(in) :: A_sparse
+        real(dp), intent(in) :: b(:), x(:), x_bar(:)
+        type(solver_options_t), intent(in) :: opts
+        real(dp), intent(out) :: A_values_bar(:), b_bar(:)
+        integer, intent(out) :: status
+
+        type(c_ptr) :: mat
+        type(ffh_solver_stats) :: cst
+        integer(c_int) :: code, freed
+        integer :: n
+
+        A_values_bar = 0.0_dp
+        b_bar = 0.0_dp
+        status = 1
+        n = A_sparse%nrow
+        if (A_sparse%ncol /= n .or. size(b) /= n .or. size(x) /= n .or. &
+            size(x_bar) /= n .or. size(b_bar) /= n .or. &
+            size(A_values_bar) /= A_sparse%nnz) return
+        if (opts%max_iterations < 1 .or. opts%tolerance <= 0.0_dp) return
+        code = ffh_matrix_upload(hip_context(), int(n, c_int), &
+            int(A_sparse%nnz, c_int64_t), 1_c_int, A_sparse%col_ptr, &
+            A_sparse%row_idx, A_sparse%val, mat)
+        if (code /= FFH_OK) return
+        code = ffh_solve_sparse_vjp(mat, x, x_bar, to_c_options(opts), A_values_bar, &
+            b_bar, cst)
+        freed = ffh_matrix_free(mat)
+        if (code /= FFH_OK) then
+            A_values_bar = 0.0_dp
+            b_bar = 0.0_dp
+            return
+        end if
+        status = 0
+        if (cst%converged == 0) then
+            A_values_bar = 0.0_dp
+            b_bar = 0.0_dp
+            status = 2
+        end if
+    end subroutine solve_sparse_vjp
 
 end module fortfem_advanced_solvers
 

@@ -287,6 +287,31 @@ int ffh_server_progress(const ffh_server *server, int64_t ticket, int *state, in
  * FP64 so far). */
 int ffh_server_set_precision(ffh_server *server, int policy, int64_t *switches);
 
+/* ---- differentiable variants --------------------------------------------- */
+/* spmv_jvp (src/solvers/fortfem_sparse_matrix.f90:60-91): y_dot = A x_dot +
+ * A_dot x with A_dot given by its CSC values (nnz, the matrix's pattern).
+ * Host arrays. */
+int ffh_spmv_jvp(ffh_matrix *mat, const double *x, const double *values_dot,
+                 const double *x_dot, double *y_dot);
+/* spmv_vjp (:93-126): values_bar(k) = y_bar(row_k) x(col_k),
+ * x_bar = A^T y_bar.  Host arrays. */
+int ffh_spmv_vjp(ffh_matrix *mat, const double *x, const double *y_bar,
+                 double *values_bar, double *x_bar);
+/* Sparse counterparts of pcg_solve_jvp / cg_solve_jvp
+ * (src/solvers/advanced_solvers.f90:387-419, 458-489): for a converged
+ * x = A^-1 b, x_dot solves A x_dot = b_dot - A_dot x (x_dot from 0) with the
+ * options' method, preconditioner and precision; stats report that solve
+ * (converged = 0 is the reference's status 2).  A must be symmetric. */
+int ffh_solve_sparse_jvp(ffh_matrix *mat, const double *x, const double *values_dot,
+                         const double *b_dot, const ffh_solver_options *opts,
+                         double *x_dot, ffh_solver_stats *stats);
+/* pcg_solve_vjp / cg_solve_vjp (:421-456, 491-522): lambda solves
+ * A^T lambda = x_bar; b_bar = lambda and values_bar(k) = -lambda(row_k)
+ * x(col_k), the cotangent A_bar = -lambda x^T on the stored entries. */
+int ffh_solve_sparse_vjp(ffh_matrix *mat, const double *x, const double *x_bar,
+                         const ffh_solver_options *opts, double *values_bar,
+                         double *b_bar, ffh_solver_stats *stats);
+
 /* ---- measurement ---------------------------------------------------------- */
 /* Times `reps` back-to-back SpMV launches on the context stream with HIP
  * events; precision FFH_FP64 or FFH_FP32.  avg_ms = mean per launch,

@@ -216,6 +216,33 @@ def csc_matvec(a: CSC, x):
     return y
 
 
+def spmv_jvp(a: CSC, x, values_dot, x_dot):
+    """spmv_jvp (src/solvers/fortfem_sparse_matrix.f90:60-91)."""
+    xx, vd, xd = _f(x), _f(values_dot), _f(x_dot)
+    if vd.shape[0] != a.nnz or xx.shape[0] != a.n_cols or xd.shape[0] != a.n_cols:
+        raise ValueError("spmv_jvp: incompatible sizes")
+    y = np.zeros(a.n_rows)
+    _lib().ffo_spmv_jvp(C.c_int(a.n_rows), C.c_int(a.n_cols), a.col_ptr.ctypes.data_as(C.c_void_p),
+                        a.row_idx.ctypes.data_as(C.c_void_p), _f(a.val).ctypes.data_as(C.c_void_p),
+                        xx.ctypes.data_as(C.c_void_p), vd.ctypes.data_as(C.c_void_p),
+                        xd.ctypes.data_as(C.c_void_p), y.ctypes.data_as(C.c_void_p))
+    return y
+
+
+def spmv_vjp(a: CSC, x, y_bar):
+    """spmv_vjp (src/solvers/fortfem_sparse_matrix.f90:93-126) -> (values_bar, x_bar)."""
+    xx, yb = _f(x), _f(y_bar)
+    if xx.shape[0] != a.n_cols or yb.shape[0] != a.n_rows:
+        raise ValueError("spmv_vjp: incompatible sizes")
+    vb = np.zeros(max(a.nnz, 1))
+    xb = np.zeros(a.n_cols)
+    _lib().ffo_spmv_vjp(C.c_int(a.n_rows), C.c_int(a.n_cols), a.col_ptr.ctypes.data_as(C.c_void_p),
+                        a.row_idx.ctypes.data_as(C.c_void_p), _f(a.val).ctypes.data_as(C.c_void_p),
+                        xx.ctypes.data_as(C.c_void_p), yb.ctypes.data_as(C.c_void_p),
+                        vb.ctypes.data_as(C.c_void_p), xb.ctypes.data_as(C.c_void_p))
+    return vb[:a.nnz].copy(), xb
+
+
 def sparse_from_dense(dense):
     d = np.asfortranarray(dense, dtype=np.float64)
     n = d.shape[0]
@@ -354,6 +381,36 @@ def solve_sparse(a: CSC, b, x0=None, method="pcg", preconditioner="jacobi", tole
                             C.c_double(tolerance), C.c_int(TOL_TYPES[tolerance_type]),
                             C.c_int(max_iterations), C.c_int(BREAKDOWN[breakdown]), C.byref(st))
     return x, SolverStats(bool(st.converged), int(st.iterations), float(st.final_residual))
+
+
+def solve_sparse_jvp(a: CSC, x, values_dot, b_dot, **opts):
+    """Sparse form of pcg_solve_jvp / cg_solve_jvp (advanced_solvers.f90:387-419,
+    458-489): x_dot solves A x_dot = b_dot - A_dot x from x_dot = 0 with the
+    same solver; A_dot is the CSC value tangent on A's pattern.
+    Returns (x_dot, stats, status) with status 0 / 2 (not converged)."""
+    a_dot = CSC(a.n_rows, a.n_cols, a.col_ptr, a.row_idx, _f(values_dot))
+    rhs = _f(b_dot) - csc_matvec(a_dot, x)
+    xd, st = solve_sparse(a, rhs, np.zeros(a.n_cols), **opts)
+    return xd, st, 0 if st.converged else 2
+
+
+def solve_sparse_vjp(a: CSC, x, x_bar, **opts):
+    """Sparse form of pcg_solve_vjp / cg_solve_vjp (advanced_solvers.f90:421-456,
+    491-522): lambda solves A^T lambda = x_bar; b_bar = lambda and A_bar =
+    -lambda x^T, here kept on A's stored entries.
+    Returns (values_bar, b_bar, stats, status)."""
+    at = sparse_from_dense(a.to_dense().T) if not _is_symmetric(a) else a
+    lam, st = solve_sparse(at, _f(x_bar), np.zeros(a.n_cols), **opts)
+    if not st.converged:
+        return np.zeros(a.nnz), np.zeros(a.n_cols), st, 2
+    cols = np.repeat(np.arange(a.n_cols), np.diff(a.col_ptr))
+    vb = -lam[a.row_idx - 1] * _f(x)[cols]
+    return vb, lam, st, 0
+
+
+def _is_symmetric(a: CSC):
+    d = a.to_dense()
+    return bool(np.array_equal(d, d.T))
 
 
 # ------------------------------------------------------------------ 2D ---

@@ -87,6 +87,12 @@ int ffo_csc_from_triplet(int nrow, int ncol, int64_t ntrip, const int *rows,
                          int *row_idx, double *val, int64_t *nnz);
 int ffo_csc_matvec(int nrow, int ncol, const int *col_ptr, const int *row_idx,
                    const double *val, const double *x, double *y);
+int ffo_spmv_jvp(int nrow, int ncol, const int *col_ptr, const int *row_idx,
+                 const double *val, const double *x, const double *val_dot,
+                 const double *x_dot, double *y_dot);
+int ffo_spmv_vjp(int nrow, int ncol, const int *col_ptr, const int *row_idx,
+                 const double *val, const double *x, const double *y_bar,
+                 double *val_bar, double *x_bar);
 int ffo_sparse_from_dense(int n, const double *dense, int *col_ptr,
                           int *row_idx, double *val, int64_t *nnz);
 int ffo_constrained_reduction(int n, const int *col_ptr, const int *row_idx,

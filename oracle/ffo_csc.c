@@ -84,6 +84,40 @@ int ffo_csc_matvec(int nrow, int ncol, const int *col_ptr, const int *row_idx,
     return 0;
 }
 
+/* spmv_jvp (src/solvers/fortfem_sparse_matrix.f90:60-91): column loop,
+ * y_dot(row) += a_k x_dot(col) + a_dot_k x(col), rows summed in column order. */
+int ffo_spmv_jvp(int nrow, int ncol, const int *col_ptr, const int *row_idx,
+                 const double *val, const double *x, const double *val_dot,
+                 const double *x_dot, double *y_dot)
+{
+    for (int i = 0; i < nrow; ++i) y_dot[i] = 0.0;
+    for (int c = 0; c < ncol; ++c)
+        for (int k = col_ptr[c] - 1; k < col_ptr[c + 1] - 1; ++k) {
+            int r = row_idx[k] - 1;
+            y_dot[r] = y_dot[r] + val[k] * x_dot[c] + val_dot[k] * x[c];
+        }
+    return 0;
+}
+
+/* spmv_vjp (:93-126): val_bar(k) = y_bar(row_k) x(col_k),
+ * x_bar(col) = sum over the column of a_k y_bar(row_k). */
+int ffo_spmv_vjp(int nrow, int ncol, const int *col_ptr, const int *row_idx,
+                 const double *val, const double *x, const double *y_bar,
+                 double *val_bar, double *x_bar)
+{
+    (void)nrow;
+    for (int c = 0; c < ncol; ++c) {
+        double s = 0.0;
+        for (int k = col_ptr[c] - 1; k < col_ptr[c + 1] - 1; ++k) {
+            int r = row_idx[k] - 1;
+            val_bar[k] = y_bar[r] * x[c];
+            s = s + val[k] * y_bar[r];
+        }
+        x_bar[c] = s;
+    }
+    return 0;
+}
+
 /* sparse_from_dense (src/solvers/fortfem_sparse_matrix.f90:16-44): keep
  * |a| > 1e-14, column-major triplets -> CSC.  dense is n x n column-major. */
 int ffo_sparse_from_dense(int n, const double *dense, int *col_ptr,

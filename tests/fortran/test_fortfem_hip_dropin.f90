@@ -4,7 +4,8 @@
 ! test/test_tetra_lagrange_arbitrary_order_assembly.f90:37-66,
 ! test/test_advanced_solvers.f90:39-84,342-348,
 ! test/test_tetra_lagrange_poisson_p_convergence.f90:26-40,
-! test/test_tetra_nedelec_sparse_assembly_slow.f90:31-59.
+! test/test_tetra_nedelec_sparse_assembly_slow.f90:31-59,
+! test/test_sparse_matrix_ad.f90:19-65, test/test_pcg_solver_ad.f90:18-60.
 module dropin_test_sources
     use fortfem_hip_types, only: dp
     implicit none
@@ -45,9 +46,9 @@ program test_fortfem_hip_dropin
         assemble_tetra_lagrange_stiffness_csc
     use fortfem_assembly_tetra_nedelec_3d, only: assemble_tetra_nedelec_curl_mass_csc, &
         assemble_tetra_nedelec_vector_load
-    use fortfem_sparse_matrix, only: spmv
+    use fortfem_sparse_matrix, only: spmv, spmv_jvp, spmv_vjp
     use fortfem_advanced_solvers, only: solver_options_t, solver_stats_t, &
-        solver_options, solve_sparse
+        solver_options, solve_sparse, solve_sparse_jvp, solve_sparse_vjp
     use fortfem_tetra_lagrange_solver_3d, only: solve_tetra_lagrange_poisson
     use dropin_test_sources, only: bubble_source, zero_boundary, sin_source, constant_field
     implicit none
@@ -62,6 +63,8 @@ program test_fortfem_hip_dropin
     call check_p2_box_poisson()
     call check_nedelec_rotational_energy()
     call check_nedelec_constant_load()
+    call check_spmv_ad()
+    call check_pcg_ad()
     call fortfem_hip_finalize()
     if (.not. all_passed) error stop 1
     write (*, '(a)') "drop-in Fortran checks passed"
@@ -216,6 +219,105 @@ contains
         call record(maxval(abs(load - product)) < 1.0e-13_dp*maxval(abs(product)), &
             "Nedelec constant-field load equals the mass times the interpolant")
     end subroutine check_nedelec_constant_load
+
+    ! a dense 3x3 matrix stored with all nine entries (column-major CSC)
+    subroutine full_csc(dense, a)
+        real(dp), intent(in) :: dense(3, 3)
+        type(csc_t), intent(out) :: a
+        integer :: c, r
+
+        a%nrow = 3
+        a%ncol = 3
+        a%nnz = 9
+        allocate(a%col_ptr(4), a%row_idx(9), a%val(9))
+        do c = 1, 3
+            a%col_ptr(c) = 3*(c - 1) + 1
+            do r = 1, 3
+                a%row_idx(3*(c - 1) + r) = r
+                a%val(3*(c - 1) + r) = dense(r, c)
+            end do
+        end do
+        a%col_ptr(4) = 10
+    end subroutine full_csc
+
+    subroutine check_spmv_ad()
+        real(dp), parameter :: step = 2.0e-7_dp
+        real(dp) :: matrix(3, 3), matrix_dot(3, 3)
+        real(dp) :: vector(3), vector_dot(3), product_dot(3), plus(3), minus(3)
+        real(dp) :: product_bar(3), vector_bar(3), values_bar(9), values_dot(9)
+        real(dp) :: lhs, rhs, relative_error
+        type(csc_t) :: a, ap, am
+
+        matrix = reshape([4.0_dp, -1.0_dp, 0.5_dp, 0.7_dp, 3.0_dp, -0.4_dp, &
+            -0.2_dp, 0.8_dp, 2.5_dp], shape(matrix))
+        matrix_dot = reshape([0.12_dp, -0.03_dp, 0.05_dp, -0.07_dp, 0.09_dp, &
+            0.02_dp, 0.04_dp, -0.06_dp, 0.08_dp], shape(matrix_dot))
+        vector = [0.6_dp, -0.8_dp, 0.35_dp]
+        vector_dot = [-0.11_dp, 0.07_dp, 0.09_dp]
+        product_bar = [0.31_dp, -0.27_dp, 0.44_dp]
+        call full_csc(matrix, a)
+        values_dot = reshape(matrix_dot, [9])
+        call spmv_jvp(a, vector, values_dot, vector_dot, product_dot)
+        call full_csc(matrix + step*matrix_dot, ap)
+        call full_csc(matrix - step*matrix_dot, am)
+        call spmv(ap, vector + step*vector_dot, plus)
+        call spmv(am, vector - step*vector_dot, minus)
+        relative_error = maxval(abs(product_dot - (plus - minus)/(2.0_dp*step)))/ &
+            max(1.0_dp, maxval(abs(product_dot)))
+        call record(relative_error < 2.0e-8_dp, &
+            "Sparse matrix JVP matches an independent re-evaluation")
+        call spmv_vjp(a, vector, product_bar, values_bar, vector_bar)
+        lhs = dot_product(product_bar, product_dot)
+        rhs = dot_product(vector_bar, vector_dot) + dot_product(values_bar, values_dot)
+        call record(abs(lhs - rhs) < 2.0e-10_dp*max(1.0_dp, abs(lhs), abs(rhs)), &
+            "Sparse matrix VJP satisfies the adjoint identity")
+    end subroutine check_spmv_ad
+
+    subroutine check_pcg_ad()
+        real(dp), parameter :: step = 2.0e-7_dp
+        real(dp) :: matrix(3, 3), matrix_dot(3, 3), rhs(3), rhs_dot(3)
+        real(dp) :: solution(3), solution_dot(3), plus(3), minus(3), solution_bar(3)
+        real(dp) :: values_dot(9), values_bar(9), rhs_bar(3), lhs, identity
+        real(dp) :: relative_error
+        type(csc_t) :: a, ap, am
+        type(solver_options_t) :: opts
+        type(solver_stats_t) :: stats
+        integer :: status
+
+        matrix = reshape([5.0_dp, 1.0_dp, 0.0_dp, 1.0_dp, 4.0_dp, 0.3_dp, &
+            0.0_dp, 0.3_dp, 2.5_dp], shape(matrix))
+        matrix_dot = reshape([0.09_dp, -0.02_dp, 0.01_dp, -0.02_dp, 0.05_dp, &
+            0.03_dp, 0.01_dp, 0.03_dp, -0.06_dp], shape(matrix_dot))
+        rhs = [0.8_dp, -0.25_dp, 0.55_dp]
+        rhs_dot = [-0.06_dp, 0.11_dp, 0.04_dp]
+        solution_bar = [0.22_dp, -0.35_dp, 0.41_dp]
+        opts = solver_options(method="pcg", tolerance=1.0e-12_dp, max_iterations=40, &
+            preconditioner="jacobi", tolerance_type="absolute")
+        call full_csc(matrix, a)
+        values_dot = reshape(matrix_dot, [9])
+        solution = 0.0_dp
+        call solve_sparse(a, rhs, solution, opts, stats)
+        call record(stats%converged, "PCG primal solve converges")
+        call solve_sparse_jvp(a, rhs, solution, opts, values_dot, rhs_dot, &
+            solution_dot, status)
+        call full_csc(matrix + step*matrix_dot, ap)
+        call full_csc(matrix - step*matrix_dot, am)
+        plus = 0.0_dp
+        call solve_sparse(ap, rhs + step*rhs_dot, plus, opts, stats)
+        minus = 0.0_dp
+        call solve_sparse(am, rhs - step*rhs_dot, minus, opts, stats)
+        relative_error = maxval(abs(solution_dot - (plus - minus)/(2.0_dp*step)))/ &
+            max(1.0_dp, maxval(abs(solution_dot)))
+        call record(status == 0 .and. relative_error < 2.0e-7_dp, &
+            "PCG JVP matches re-evaluation of the converged state")
+        call solve_sparse_vjp(a, rhs, solution, opts, solution_bar, values_bar, &
+            rhs_bar, status)
+        lhs = dot_product(solution_bar, solution_dot)
+        identity = dot_product(values_bar, values_dot) + dot_product(rhs_bar, rhs_dot)
+        call record(status == 0 .and. abs(lhs - identity) < &
+            3.0e-8_dp*max(1.0_dp, abs(lhs), abs(identity)), &
+            "PCG VJP satisfies the implicit-solve adjoint identity")
+    end subroutine check_pcg_ad
 
     subroutine check_linear_energy()
         real(dp) :: mesh_vertices(3, 5), coefficients(5), product(5)
