@@ -273,6 +273,7 @@ class Problem:
                                                     c.shape[0], _ptr(c), C.byref(h)))
         self.handle = h
         self.ncell = c.shape[0]
+        self.nvert = v.shape[0]
         _OWNED.add(self)
 
     def close(self):
@@ -328,6 +329,49 @@ class Problem:
             k = _f64([kappa])
             self._check(self._lib.ffh_assemble(self.handle, quadrature_degree, capi.COEFF_SCALAR,
                                                _ptr(k), float(mass), prec))
+
+    def _coefficients(self, kappa, cell_kappa):
+        if cell_kappa is not None:
+            ck = _f64(cell_kappa)
+            if ck.shape[0] != self.ncell:
+                raise ValueError("cell_kappa needs one value per cell")
+            return capi.COEFF_CELL, ck
+        return capi.COEFF_SCALAR, _f64([kappa])
+
+    def assemble_jvp(self, quadrature_degree, kappa=1.0, mass=0.0, cell_kappa=None, vertices_dot=None,
+                     kappa_dot=0.0, cell_kappa_dot=None, mass_dot=0.0):
+        """Tangent of the H1 matrix values (ffh_assemble_jvp) on the pattern
+        of matrix(FULL); vertices_dot is (nv, 3)."""
+        kind, k = self._coefficients(kappa, cell_kappa)
+        if kind == capi.COEFF_CELL:
+            kd = np.zeros(self.ncell) if cell_kappa_dot is None else _f64(cell_kappa_dot)
+            if kd.shape[0] != self.ncell:
+                raise ValueError("cell_kappa_dot needs one value per cell")
+        else:
+            if cell_kappa_dot is not None:
+                raise ValueError("cell_kappa_dot needs cell_kappa")
+            kd = _f64([kappa_dot])
+        vd = None if vertices_dot is None else _f64(vertices_dot)
+        out = np.zeros(max(self.sizes()["nnz"], 1))
+        self._check(self._lib.ffh_assemble_jvp(self.handle, quadrature_degree, kind, _ptr(k), float(mass),
+                                               _ptr(vd), _ptr(kd), float(mass_dot), _ptr(out)))
+        return out[:self.sizes()["nnz"]]
+
+    def assemble_vjp(self, quadrature_degree, values_bar, kappa=1.0, mass=0.0, cell_kappa=None):
+        """Cotangent of the H1 matrix values (ffh_assemble_vjp) ->
+        (vertices_bar (nv, 3), kappa_bar (scalar, or per cell with
+        cell_kappa), mass_bar)."""
+        kind, k = self._coefficients(kappa, cell_kappa)
+        vb = _f64(values_bar)
+        if vb.shape[0] != self.sizes()["nnz"]:
+            raise ValueError("values_bar needs one value per stored entry")
+        nv = self.nvert
+        xb = np.zeros((nv, 3))
+        kb = np.zeros(self.ncell if kind == capi.COEFF_CELL else 1)
+        mb = C.c_double(0)
+        self._check(self._lib.ffh_assemble_vjp(self.handle, quadrature_degree, kind, _ptr(k), float(mass),
+                                               _ptr(vb), _ptr(xb), _ptr(kb), C.byref(mb)))
+        return xb, (kb if kind == capi.COEFF_CELL else float(kb[0])), mb.value
 
     def load(self, quadrature_degree, kind, params):
         p = _f64(np.atleast_1d(params))
@@ -605,6 +649,62 @@ def assemble_tetra_lagrange_stiffness_csc(mesh_vertices, tetrahedra, degree, qua
             raise
         return None, status
     return a, FortsparseStatus(0, "")
+
+
+def assemble_tetra_lagrange_stiffness_csc_jvp(mesh_vertices, tetrahedra, degree, quadrature_degree,
+                                              stiffness_coefficient, mass_coefficient, mesh_vertices_dot,
+                                              stiffness_coefficient_dot, mass_coefficient_dot,
+                                              ctx: Context | None = None):
+    """assemble_tetra_lagrange_stiffness_csc_jvp (:336-386) -> (matrix_dot CSC, status).
+    mesh_vertices and mesh_vertices_dot are (nv, 3); the result has the
+    primal matrix's pattern."""
+    status = FortsparseStatus(FORTSPARSE_INVALID_MATRIX, "Tetrahedral H1 JVP assembly failed")
+    v = np.asarray(mesh_vertices)
+    t = np.asarray(tetrahedra)
+    vd = np.asarray(mesh_vertices_dot)
+    if v.ndim != 2 or v.shape[1] != 3 or t.ndim != 2 or t.shape[1] != 4 or t.shape[0] < 1:
+        return None, status
+    if vd.shape != v.shape or t.min() < 1 or t.max() > v.shape[0] or not 1 <= degree <= 4:
+        return None, status
+    try:
+        p = Problem(v, t, ctx)
+        p.pattern_build(capi.FAMILY_H1, degree)
+        vals = p.assemble_jvp(quadrature_degree, kappa=stiffness_coefficient, mass=mass_coefficient,
+                              vertices_dot=vd, kappa_dot=stiffness_coefficient_dot, mass_dot=mass_coefficient_dot)
+        a = p.matrix(capi.FULL).download()
+    except FFHError as e:
+        if e.code == capi.FFH_ERR_DEVICE:
+            raise
+        return None, status
+    a.val = vals
+    return a, FortsparseStatus(0, "")
+
+
+def assemble_tetra_lagrange_stiffness_csc_vjp(mesh_vertices, tetrahedra, degree, quadrature_degree,
+                                              stiffness_coefficient, mass_coefficient, matrix_values_bar,
+                                              ctx: Context | None = None):
+    """assemble_tetra_lagrange_stiffness_csc_vjp (:388-471) ->
+    (mesh_vertices_bar (nv, 3), stiffness_coefficient_bar, mass_coefficient_bar, status)."""
+    status = FortsparseStatus(FORTSPARSE_INVALID_MATRIX, "Tetrahedral H1 VJP shapes differ")
+    v = np.asarray(mesh_vertices)
+    t = np.asarray(tetrahedra)
+    zero = (np.zeros_like(v, dtype=np.float64), 0.0, 0.0)
+    if v.ndim != 2 or v.shape[1] != 3 or t.ndim != 2 or t.shape[1] != 4 or t.shape[0] < 1:
+        return (*zero, status)
+    if t.min() < 1 or t.max() > v.shape[0] or not 1 <= degree <= 4:
+        return (*zero, status)
+    try:
+        p = Problem(v, t, ctx)
+        _, nnz = p.pattern_build(capi.FAMILY_H1, degree)
+        if np.asarray(matrix_values_bar).shape[0] != nnz:
+            return (*zero, status)
+        xb, kb, mb = p.assemble_vjp(quadrature_degree, matrix_values_bar, kappa=stiffness_coefficient,
+                                    mass=mass_coefficient)
+    except FFHError as e:
+        if e.code == capi.FFH_ERR_DEVICE:
+            raise
+        return (*zero, status)
+    return xb, kb, mb, FortsparseStatus(0, "")
 
 
 def nedelec_dof_count(k):

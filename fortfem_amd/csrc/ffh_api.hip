@@ -395,6 +395,30 @@ int ffh_assemble(ffh_problem *P, int qdeg, int coeff_kind, const double *coeff,
     });
 }
 
+int ffh_assemble_jvp(ffh_problem *P, int qdeg, int coeff_kind, const double *coeff,
+                     double mass, const double *vertices_dot, const double *coeff_dot,
+                     double mass_dot, double *values_dot)
+{
+    if (!P) return FFH_ERR_HANDLE;
+    if (!values_dot) return FFH_ERR_INVALID;
+    return guarded(P->ctx, [&] {
+        problem_assemble_jvp(P, qdeg, coeff_kind, coeff, mass, vertices_dot, coeff_dot,
+                             mass_dot, values_dot);
+    });
+}
+
+int ffh_assemble_vjp(ffh_problem *P, int qdeg, int coeff_kind, const double *coeff,
+                     double mass, const double *values_bar, double *vertices_bar,
+                     double *coeff_bar, double *mass_bar)
+{
+    if (!P) return FFH_ERR_HANDLE;
+    if (!values_bar || !vertices_bar || !coeff_bar || !mass_bar) return FFH_ERR_INVALID;
+    return guarded(P->ctx, [&] {
+        problem_assemble_vjp(P, qdeg, coeff_kind, coeff, mass, values_bar, vertices_bar,
+                             coeff_bar, mass_bar);
+    });
+}
+
 int ffh_load(ffh_problem *P, int qdeg, int kind, const double *params, int64_t np)
 {
     if (!P) return FFH_ERR_HANDLE;

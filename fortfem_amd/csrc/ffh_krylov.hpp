@@ -21,6 +21,12 @@ double bench_spmv(Context *ctx, Matrix &M, int precision, int reps,
 namespace ffh {
 // problems (ffh_problem.hip)
 void problem_pattern(Problem *P, int family, int order);
+void problem_assemble_jvp(Problem *P, int qdeg, int coeff_kind, const double *coeff,
+                          double mass, const double *vertices_dot, const double *coeff_dot,
+                          double mass_dot, double *values_dot);
+void problem_assemble_vjp(Problem *P, int qdeg, int coeff_kind, const double *coeff,
+                          double mass, const double *values_bar, double *vertices_bar,
+                          double *coeff_bar, double *mass_bar);
 void problem_assemble(Problem *P, int qdeg, int coeff_kind, const double *coeff,
                       double mass, int precision);
 void problem_load(Problem *P, int qdeg, int kind, const double *params,

@@ -68,6 +68,38 @@ __device__ __forceinline__ bool tet_geometry(const double *__restrict__ verts, i
     return true;
 }
 
+// G = J^-1 J^-T (symmetric) as g[0..5] = G00, G11, G22, G01, G02, G12
+// (inv column-major).
+__device__ __forceinline__ void h1_gterms(const double inv[9], double g[6])
+{
+    const int a_[6] = {0, 1, 2, 0, 0, 1}, b_[6] = {0, 1, 2, 1, 2, 2};
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+        const int a = a_[k], b = b_[k];
+        g[k] = inv[a + 0] * inv[b + 0] + inv[a + 3] * inv[b + 3] + inv[a + 6] * inv[b + 6];
+    }
+}
+
+// Upper triangle of sum_k coef[k] S_k (+ cm M when mass) into E[u][pos].
+template <int NL, typename CT>
+__device__ __forceinline__ void h1_emit(const CT coef[6], CT cm, bool mass,
+                                        const CT *__restrict__ tab, CT *__restrict__ E,
+                                        size_t T, int pos)
+{
+    constexpr int NN = NL * NL;
+    int u = 0;
+    for (int r = 0; r < NL; ++r) {
+        for (int cc = r; cc < NL; ++cc, ++u) {
+            const int rc = r * NL + cc;
+            CT s = coef[0] * tab[rc] + coef[1] * tab[NN + rc] + coef[2] * tab[2 * NN + rc] +
+                   coef[3] * tab[3 * NN + rc] + coef[4] * tab[4 * NN + rc] +
+                   coef[5] * tab[5 * NN + rc];
+            if (mass) s += cm * tab[6 * NN + rc];
+            E[(size_t)u * T + pos] = s;
+        }
+    }
+}
+
 // Phase 1 of the assembly: one thread per cell (all cells, colour order),
 // the upper triangle of the element matrix into E[u][pos] (u = packed upper
 // index, lanes consecutive in pos).  tab = 7 tables of NL x NL:
@@ -86,35 +118,154 @@ __global__ __launch_bounds__(kBlockA) void k_h1_elem(
         atomicOr(err, 1);
         return;
     }
-    // G = J^-1 J^-T (symmetric)
     double g[6];
+    h1_gterms(inv, g);
+    const double ke = kappa_cell ? kappa_cell[perm[pos]] : kappa;
+    const CT kd = (CT)(ke * det);
+    CT coef[6];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) coef[k] = kd * (CT)g[k];
+    h1_emit<NL, CT>(coef, (CT)(mu * det), mu != 0.0, tab, E, (size_t)ncell, pos);
+}
+
+// Tangent of the H1 element matrices (the derivative of the element
+// integral of assembly_tetra_lagrange_arbitrary_order_3d.f90:107-174) in the
+// same reference-tensor form: with c_k = kappa det G_k and c_m = mu det,
+//   dc_k = dkappa det G_k + kappa ddet G_k + kappa det dG_k,
+//   dc_m = dmu det + mu ddet,
+// ddet = det tr(J^-1 dJ), dJ^-1 = -J^-1 dJ J^-1, dG = dJ^-1 J^-T + J^-1 dJ^-T.
+// verts_dot (3 per vertex) and the per-cell kappa tangents may be null.
+template <int NL>
+__global__ __launch_bounds__(kBlockA) void k_h1_elem_jvp(
+    int ncell, const int *__restrict__ cells_c, const double *__restrict__ verts,
+    const double *__restrict__ verts_dot, const int *__restrict__ perm,
+    const double *__restrict__ kappa_cell, const double *__restrict__ kappa_cell_dot,
+    double kappa, double kappa_dot, double mu, double mu_dot, const double *__restrict__ tab,
+    double *__restrict__ E, int *__restrict__ err)
+{
+    const int pos = blockIdx.x * kBlockA + threadIdx.x;
+    if (pos >= ncell) return;
+    const int4 c = reinterpret_cast<const int4 *>(cells_c)[pos];
+    double jac[9], inv[9], det;
+    if (!tet_geometry(verts, c.x, c.y, c.z, c.w, jac, inv, det)) {
+        atomicOr(err, 1);
+        return;
+    }
+    double jd[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    if (verts_dot) {
+        const int id[4] = {c.x, c.y, c.z, c.w};
+        const double *v0 = verts_dot + 3 * (size_t)c.x;
+        for (int cc = 0; cc < 3; ++cc) {
+            const double *v = verts_dot + 3 * (size_t)id[cc + 1];
+            for (int r = 0; r < 3; ++r) jd[r + 3 * cc] = v[r] - v0[r];
+        }
+    }
+    // m = J^-1 dJ; ddet = det tr(m); dinv = -m J^-1
+    double m[9], invd[9];
+    for (int j = 0; j < 3; ++j)
+        for (int i = 0; i < 3; ++i)
+            m[i + 3 * j] = inv[i] * jd[3 * j] + inv[i + 3] * jd[1 + 3 * j] +
+                           inv[i + 6] * jd[2 + 3 * j];
+    const double detd = det * (m[0] + m[4] + m[8]);
+    for (int j = 0; j < 3; ++j)
+        for (int i = 0; i < 3; ++i)
+            invd[i + 3 * j] = -(m[i] * inv[3 * j] + m[i + 3] * inv[1 + 3 * j] +
+                                m[i + 6] * inv[2 + 3 * j]);
+    double g[6], gd[6];
+    h1_gterms(inv, g);
     {
         const int a_[6] = {0, 1, 2, 0, 0, 1}, b_[6] = {0, 1, 2, 1, 2, 2};
 #pragma unroll
         for (int k = 0; k < 6; ++k) {
             const int a = a_[k], b = b_[k];
-            g[k] = inv[a + 0] * inv[b + 0] + inv[a + 3] * inv[b + 3] +
-                   inv[a + 6] * inv[b + 6];
+            double s = 0.0;
+            for (int q = 0; q < 3; ++q)
+                s += invd[a + 3 * q] * inv[b + 3 * q] + inv[a + 3 * q] * invd[b + 3 * q];
+            gd[k] = s;
         }
     }
-    const double ke = kappa_cell ? kappa_cell[perm[pos]] : kappa;
-    const CT kd = (CT)(ke * det), md = (CT)(mu * det);
-    CT gg[6];
+    const int cell = perm[pos];
+    const double ke = kappa_cell ? kappa_cell[cell] : kappa;
+    const double ked = kappa_cell_dot ? kappa_cell_dot[cell] : kappa_dot;
+    double coef[6];
 #pragma unroll
-    for (int k = 0; k < 6; ++k) gg[k] = (CT)g[k];
+    for (int k = 0; k < 6; ++k) coef[k] = ked * det * g[k] + ke * detd * g[k] + ke * det * gd[k];
+    h1_emit<NL, double>(coef, mu_dot * det + mu * detd, true, tab, E, (size_t)ncell, pos);
+}
+
+// Cotangent of the H1 element matrices (assembly_tetra_lagrange_arbitrary_
+// order_3d.f90:176-275, 388-471), one colour per launch.  The element
+// cotangent is gathered from the CSC cotangent through the entry slots
+// (csc_value_bar_at), contracted with the 7 tables, t_k = sum_rc Eb_rc S_k,rc,
+// and taken back through c_k = kappa det G_k, c_m = mu det:
+//   kappa_bar = det sum_k t_k G_k, mu_bar = det t_m,
+//   det_bar = kappa sum_k t_k G_k + mu t_m, G_bar_k = kappa det t_k,
+//   Inv_bar from G = Inv Inv^T, J_bar = -Inv^T Inv_bar Inv^T + det_bar det Inv^T,
+// then to the four vertices (tetra_jacobian_vjp).  Cells of one colour
+// share no vertex (every vertex carries a dof), so the vertex sums need no
+// atomics.  Per-cell kappa / mu bars go to kb / mb [pos].
+template <int NL>
+__global__ __launch_bounds__(kBlockA) void k_h1_elem_vjp(
+    int pos0, int cnt, int ncell, const int *__restrict__ cells_c,
+    const double *__restrict__ verts, const int *__restrict__ perm,
+    const double *__restrict__ kappa_cell, double kappa, double mu,
+    const double *__restrict__ tab, const int *__restrict__ slots,
+    const double *__restrict__ vbar, double *__restrict__ verts_bar, double *__restrict__ kb,
+    double *__restrict__ mb, int *__restrict__ err)
+{
+    const int i = blockIdx.x * kBlockA + threadIdx.x;
+    if (i >= cnt) return;
+    const int pos = pos0 + i;
+    const int4 c = reinterpret_cast<const int4 *>(cells_c)[pos];
+    double jac[9], inv[9], det;
+    if (!tet_geometry(verts, c.x, c.y, c.z, c.w, jac, inv, det)) {
+        atomicOr(err, 1);
+        return;
+    }
     constexpr int NN = NL * NL;
     const size_t T = (size_t)ncell;
-    int u = 0;
-    for (int r = 0; r < NL; ++r) {
-        for (int cc = r; cc < NL; ++cc, ++u) {
-            const int rc = r * NL + cc;
-            CT s = gg[0] * tab[rc] + gg[1] * tab[NN + rc] + gg[2] * tab[2 * NN + rc] +
-                   gg[3] * tab[3 * NN + rc] + gg[4] * tab[4 * NN + rc] +
-                   gg[5] * tab[5 * NN + rc];
-            CT v = kd * s;
-            if (mu != 0.0) v += md * tab[6 * NN + rc];
-            E[(size_t)u * T + pos] = v;
+    double t[7] = {0, 0, 0, 0, 0, 0, 0};
+    for (int rc = 0; rc < NN; ++rc) {
+        const double e = vbar[slots[(size_t)rc * T + pos]];
+#pragma unroll
+        for (int k = 0; k < 7; ++k) t[k] += e * tab[k * NN + rc];
+    }
+    double g[6];
+    h1_gterms(inv, g);
+    const double ke = kappa_cell ? kappa_cell[perm[pos]] : kappa;
+    double tg = 0.0;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) tg += t[k] * g[k];
+    kb[pos] = det * tg;
+    mb[pos] = det * t[6];
+    const double det_bar = ke * tg + mu * t[6];
+    double ib[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    {
+        const int a_[6] = {0, 1, 2, 0, 0, 1}, b_[6] = {0, 1, 2, 1, 2, 2};
+#pragma unroll
+        for (int k = 0; k < 6; ++k) {
+            const int a = a_[k], b = b_[k];
+            const double gb = ke * det * t[k];
+            for (int q = 0; q < 3; ++q) {
+                ib[a + 3 * q] += gb * inv[b + 3 * q];
+                ib[b + 3 * q] += gb * inv[a + 3 * q];
+            }
         }
+    }
+    // J_bar = -Inv^T ib Inv^T + det_bar det Inv^T
+    double w[9], jb[9];
+    for (int j = 0; j < 3; ++j)
+        for (int r = 0; r < 3; ++r)
+            w[r + 3 * j] = inv[3 * r] * ib[3 * j] + inv[1 + 3 * r] * ib[1 + 3 * j] +
+                           inv[2 + 3 * r] * ib[2 + 3 * j];
+    for (int j = 0; j < 3; ++j)
+        for (int r = 0; r < 3; ++r)
+            jb[r + 3 * j] = -(w[r] * inv[j] + w[r + 3] * inv[j + 3] + w[r + 6] * inv[j + 6]) +
+                            det_bar * det * inv[j + 3 * r];
+    const int id[4] = {c.x, c.y, c.z, c.w};
+    for (int r = 0; r < 3; ++r) {
+        verts_bar[3 * (size_t)id[0] + r] -= jb[r] + jb[r + 3] + jb[r + 6];
+        for (int q = 0; q < 3; ++q) verts_bar[3 * (size_t)id[q + 1] + r] += jb[r + 3 * q];
     }
 }
 
@@ -786,6 +937,41 @@ static void assemble_nedelec(Problem *P, int coeff_kind, const double *coeff, do
     P->red_dirty = true;
 }
 
+static void ensure_h1_tables(Problem *P, int qdeg)
+{
+    FFH_REQUIRE(qdeg >= 0, FFH_ERR_INVALID, "quadrature degree must be >= 0");
+    if (P->tab_qdeg == qdeg) return;
+    hipStream_t s = P->ctx->stream;
+    std::vector<double> tab;
+    h1_tables(P->order, qdeg, tab);
+    P->d_tab64.upload(tab, s);
+    std::vector<float> t32(tab.begin(), tab.end());
+    P->d_tab32.upload(t32, s);
+    P->tab_qdeg = qdeg;
+}
+
+// Scalar stiffness coefficient: FFH_COEFF_SCALAR sets kappa (coeff[0], 1 when
+// null) and returns null; FFH_COEFF_CELL uploads coeff (T values, reusing the
+// last upload when null) and returns the device array.
+static const double *scalar_coefficients(Problem *P, int coeff_kind, const double *coeff,
+                                         double &kappa)
+{
+    kappa = 1.0;
+    if (coeff_kind == FFH_COEFF_SCALAR) {
+        if (coeff) kappa = coeff[0];
+        return nullptr;
+    }
+    FFH_REQUIRE(coeff_kind == FFH_COEFF_CELL, FFH_ERR_INVALID,
+                "unsupported coefficient kind for this family");
+    if (coeff) {
+        P->d_coeff.upload(coeff, (size_t)P->ncell, P->ctx->stream);
+        P->coeff_kind = FFH_COEFF_CELL;
+    }
+    FFH_REQUIRE(P->coeff_kind == FFH_COEFF_CELL && P->d_coeff.p, FFH_ERR_INVALID,
+                "per-cell coefficients were never uploaded");
+    return P->d_coeff.p;
+}
+
 void problem_assemble(Problem *P, int qdeg, int coeff_kind, const double *coeff,
                       double mass, int precision)
 {
@@ -798,20 +984,7 @@ void problem_assemble(Problem *P, int qdeg, int coeff_kind, const double *coeff,
     }
     const int T = P->ncell;
     double kappa = 1.0;
-    const double *dk = nullptr;
-    if (coeff_kind == FFH_COEFF_SCALAR) {
-        if (coeff) kappa = coeff[0];
-    } else if (coeff_kind == FFH_COEFF_CELL) {
-        if (coeff) {
-            P->d_coeff.upload(coeff, (size_t)T, s);
-            P->coeff_kind = FFH_COEFF_CELL;
-        }
-        FFH_REQUIRE(P->coeff_kind == FFH_COEFF_CELL && P->d_coeff.p, FFH_ERR_INVALID,
-                    "per-cell coefficients were never uploaded");
-        dk = P->d_coeff.p;
-    } else {
-        throw Error{FFH_ERR_INVALID, "unsupported coefficient kind for this family"};
-    }
+    const double *dk = scalar_coefficients(P, coeff_kind, coeff, kappa);
     DevMatrix &A = P->full.m;
     const int nc = P->n_colors;
     if (P->family == FFH_FAMILY_P1_2D) {
@@ -829,15 +1002,7 @@ void problem_assemble(Problem *P, int qdeg, int coeff_kind, const double *coeff,
         FFH_HIP(hipGetLastError());
         check_err(P, "2D P1 assembly");
     } else {
-        FFH_REQUIRE(qdeg >= 0, FFH_ERR_INVALID, "quadrature degree must be >= 0");
-        if (P->tab_qdeg != qdeg) {
-            std::vector<double> tab;
-            h1_tables(P->order, qdeg, tab);
-            P->d_tab64.upload(tab, s);
-            std::vector<float> t32(tab.begin(), tab.end());
-            P->d_tab32.upload(t32, s);
-            P->tab_qdeg = qdeg;
-        }
+        ensure_h1_tables(P, qdeg);
         const bool f32 = precision == FFH_FP32;
         if (f32) {
             A.val32.alloc(A.nnz);
@@ -883,6 +1048,122 @@ void problem_assemble(Problem *P, int qdeg, int coeff_kind, const double *coeff,
     }
     P->assembled = true;
     P->red_dirty = true;
+}
+
+// Tangent of the tetrahedral H1 matrix on the problem's pattern
+// (assemble_tetra_lagrange_stiffness_csc_jvp,
+// assembly_tetra_lagrange_arbitrary_order_3d.f90:336-386): the element
+// tangents of k_h1_elem_jvp through the primal's colour scatter, so the
+// result has the primal's CSC order.  values_dot: host, nnz of the full
+// matrix.
+void problem_assemble_jvp(Problem *P, int qdeg, int coeff_kind, const double *coeff,
+                          double mass, const double *vertices_dot, const double *coeff_dot,
+                          double mass_dot, double *values_dot)
+{
+    Context *ctx = P->ctx;
+    hipStream_t s = ctx->stream;
+    FFH_REQUIRE(P->n_global > 0, FFH_ERR_INVALID, "pattern_build must come first");
+    FFH_REQUIRE(P->family == FFH_FAMILY_H1, FFH_ERR_INVALID,
+                "assembly tangents are built for the tetrahedral H1 family");
+    const int T = P->ncell;
+    double kappa = 1.0;
+    const double *dk = scalar_coefficients(P, coeff_kind, coeff, kappa);
+    ensure_h1_tables(P, qdeg);
+    double kappa_dot = 0.0;
+    DBuf<double> dkd, dvd;
+    if (coeff_kind == FFH_COEFF_SCALAR) {
+        if (coeff_dot) kappa_dot = coeff_dot[0];
+    } else if (coeff_dot) {
+        dkd.upload(coeff_dot, (size_t)T, s);
+    }
+    if (vertices_dot) dvd.upload(vertices_dot, 3 * (size_t)P->nvert, s);
+    const int nl = P->n_local;
+    const size_t nup = (size_t)nl * (nl + 1) / 2;
+    P->d_elem.alloc(nup * T);
+    const int64_t nnz = P->full.m.nnz;
+    DBuf<double> vals;
+    vals.alloc(std::max<int64_t>(nnz, 1));
+    FFH_HIP(hipMemsetAsync(vals.p, 0, sizeof(double) * nnz, s));
+    const int gridc = (T + kBlockA - 1) / kBlockA;
+    FFH_NL_DISPATCH(nl, {
+        hipLaunchKernelGGL((k_h1_elem_jvp<NL>), dim3(gridc), dim3(kBlockA), 0, s, T,
+                           P->d_cells_c.p, P->d_vertices.p, dvd.p, P->d_perm.p, dk, dkd.p,
+                           kappa, kappa_dot, mass, mass_dot, P->d_tab64.p, P->d_elem.p,
+                           P->d_err.p);
+    });
+    for (int c = 0; c < P->n_colors; ++c) {
+        const int p0 = P->color_start[c], cnt = P->color_start[c + 1] - p0;
+        const int64_t work = (int64_t)cnt * nl * nl;
+        if (work == 0) continue;
+        hipLaunchKernelGGL((k_scatter_colour<double, double>), dim3((int)((work + 255) / 256)),
+                           dim3(256), 0, s, p0, cnt, T, nl, P->d_elem.p, P->d_slots.p, vals.p);
+    }
+    FFH_HIP(hipGetLastError());
+    check_err(P, "tetrahedral H1 assembly tangent");
+    vals.download(values_dot, nnz, s);
+    FFH_HIP(hipStreamSynchronize(s));
+}
+
+// Cotangent of the tetrahedral H1 matrix
+// (assemble_tetra_lagrange_stiffness_csc_vjp, :388-471): one k_h1_elem_vjp
+// launch per colour; the coefficient bars are summed on the host in cell
+// order.  vertices_bar: 3 nv; coeff_bar: 1 value (FFH_COEFF_SCALAR) or one
+// per cell (FFH_COEFF_CELL); mass_bar: 1 value.
+void problem_assemble_vjp(Problem *P, int qdeg, int coeff_kind, const double *coeff,
+                          double mass, const double *values_bar, double *vertices_bar,
+                          double *coeff_bar, double *mass_bar)
+{
+    Context *ctx = P->ctx;
+    hipStream_t s = ctx->stream;
+    FFH_REQUIRE(P->n_global > 0, FFH_ERR_INVALID, "pattern_build must come first");
+    FFH_REQUIRE(P->family == FFH_FAMILY_H1, FFH_ERR_INVALID,
+                "assembly cotangents are built for the tetrahedral H1 family");
+    const int T = P->ncell;
+    double kappa = 1.0;
+    const double *dk = scalar_coefficients(P, coeff_kind, coeff, kappa);
+    ensure_h1_tables(P, qdeg);
+    const int64_t nnz = P->full.m.nnz;
+    DBuf<double> vb, xb, kb, mb;
+    vb.upload(values_bar, (size_t)nnz, s);
+    xb.alloc(3 * (size_t)P->nvert);
+    FFH_HIP(hipMemsetAsync(xb.p, 0, sizeof(double) * 3 * (size_t)P->nvert, s));
+    kb.alloc(std::max(T, 1));
+    mb.alloc(std::max(T, 1));
+    const int nl = P->n_local;
+    for (int c = 0; c < P->n_colors; ++c) {
+        const int p0 = P->color_start[c], cnt = P->color_start[c + 1] - p0;
+        if (cnt == 0) continue;
+        FFH_NL_DISPATCH(nl, {
+            hipLaunchKernelGGL((k_h1_elem_vjp<NL>), dim3((cnt + kBlockA - 1) / kBlockA),
+                               dim3(kBlockA), 0, s, p0, cnt, T, P->d_cells_c.p,
+                               P->d_vertices.p, P->d_perm.p, dk, kappa, mass,
+                               P->d_tab64.p, P->d_slots.p, vb.p, xb.p, kb.p, mb.p,
+                               P->d_err.p);
+        });
+    }
+    FFH_HIP(hipGetLastError());
+    check_err(P, "tetrahedral H1 assembly cotangent");
+    std::vector<double> hk(T), hm(T);
+    kb.download(hk.data(), T, s);
+    mb.download(hm.data(), T, s);
+    xb.download(vertices_bar, 3 * (size_t)P->nvert, s);
+    FFH_HIP(hipStreamSynchronize(s));
+    // colour position -> cell, then sums in cell order
+    std::vector<double> ck(T), cm(T);
+    for (int pos = 0; pos < T; ++pos) {
+        ck[P->perm[pos]] = hk[pos];
+        cm[P->perm[pos]] = hm[pos];
+    }
+    double ksum = 0.0, msum = 0.0;
+    for (int e = 0; e < T; ++e) {
+        ksum += ck[e];
+        msum += cm[e];
+    }
+    if (coeff_kind == FFH_COEFF_CELL)
+        std::copy(ck.begin(), ck.end(), coeff_bar);
+    else
+        coeff_bar[0] = ksum;
+    *mass_bar = msum;
 }
 
 // Nedelec vector load into rhs_full: tables once per problem, then g per

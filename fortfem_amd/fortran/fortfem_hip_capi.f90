@@ -68,6 +68,7 @@ module fortfem_hip_capi
     public :: ffh_matrix_free, ffh_matrix_download, ffh_spmv, ffh_solve_sparse
     public :: ffh_bench_spmv, ffh_set_profiling, ffh_last_solve_profile
     public :: ffh_spmv_jvp, ffh_spmv_vjp, ffh_solve_sparse_jvp, ffh_solve_sparse_vjp
+    public :: ffh_assemble_jvp, ffh_assemble_vjp
 
     interface
         integer(c_int) function ffh_init(device, ctx) bind(C, name="ffh_init")
@@ -202,6 +203,28 @@ module fortfem_hip_capi
             real(c_double), intent(in) :: coeff(*)
             real(c_double), value :: mass_coefficient
         end function ffh_assemble
+
+        integer(c_int) function ffh_assemble_jvp(prob, quadrature_degree, &
+                coeff_kind, coeff, mass_coefficient, vertices_dot, coeff_dot, &
+                mass_dot, values_dot) bind(C, name="ffh_assemble_jvp")
+            import :: c_int, c_double, c_ptr
+            type(c_ptr), value :: prob
+            integer(c_int), value :: quadrature_degree, coeff_kind
+            real(c_double), intent(in) :: coeff(*), vertices_dot(*), coeff_dot(*)
+            real(c_double), value :: mass_coefficient, mass_dot
+            real(c_double), intent(out) :: values_dot(*)
+        end function ffh_assemble_jvp
+
+        integer(c_int) function ffh_assemble_vjp(prob, quadrature_degree, &
+                coeff_kind, coeff, mass_coefficient, values_bar, vertices_bar, &
+                coeff_bar, mass_bar) bind(C, name="ffh_assemble_vjp")
+            import :: c_int, c_double, c_ptr
+            type(c_ptr), value :: prob
+            integer(c_int), value :: quadrature_degree, coeff_kind
+            real(c_double), intent(in) :: coeff(*), values_bar(*)
+            real(c_double), value :: mass_coefficient
+            real(c_double), intent(out) :: vertices_bar(*), coeff_bar(*), mass_bar
+        end function ffh_assemble_vjp
 
         integer(c_int) function ffh_load(prob, quadrature_degree, source_kind, &
                 params, nparams) bind(C, name="ffh_load")

@@ -193,6 +193,8 @@ module fortfem_assembly_tetra_lagrange_arbitrary_order_3d
     private
 
     public :: assemble_tetra_lagrange_stiffness_csc
+    public :: assemble_tetra_lagrange_stiffness_csc_jvp
+    public :: assemble_tetra_lagrange_stiffness_csc_vjp
     public :: assemble_tetra_lagrange_scalar_load
     public :: assemble_tetra_lagrange_scalar_load_samples
     public :: scalar_source_3d
@@ -284,6 +286,107 @@ contains
         code = ffh_problem_free(prob)
         if (ok) call status_set(status, 0, "")
     end subroutine assemble_tetra_lagrange_stiffness_csc
+
+    ! :336-386.  matrix_dot has the primal matrix's pattern.
+    subroutine assemble_tetra_lagrange_stiffness_csc_jvp(mesh_vertices, &
+            tetrahedra, degree, quadrature_degree, stiffness_coefficient, &
+            mass_coefficient, mesh_vertices_dot, stiffness_coefficient_dot, &
+            mass_coefficient_dot, matrix_dot, status)
+        real(dp), intent(in) :: mesh_vertices(:, :), mesh_vertices_dot(:, :)
+        integer, intent(in) :: tetrahedra(:, :), degree, quadrature_degree
+        real(dp), intent(in) :: stiffness_coefficient, mass_coefficient
+        real(dp), intent(in) :: stiffness_coefficient_dot
+        real(dp), intent(in) :: mass_coefficient_dot
+        type(csc_t), intent(out) :: matrix_dot
+        type(fortsparse_status_t), intent(out) :: status
+
+        type(c_ptr) :: prob, mat
+        integer(c_int) :: n_global, code
+        integer(c_int64_t) :: nnz
+        logical :: ok
+
+        call status_set(status, FORTSPARSE_INVALID_MATRIX, &
+            "Tetrahedral H1 JVP assembly failed")
+        if (size(mesh_vertices, 1) /= 3) return
+        if (any(shape(mesh_vertices_dot) /= shape(mesh_vertices))) return
+        if (size(tetrahedra, 1) /= 4 .or. size(tetrahedra, 2) < 1) return
+        if (any(tetrahedra < 1) .or. &
+            any(tetrahedra > size(mesh_vertices, 2))) return
+        if (degree < 1 .or. degree > 4) return
+        code = ffh_problem_create(hip_context(), 3_c_int, &
+            int(size(mesh_vertices, 2), c_int), mesh_vertices, &
+            int(size(tetrahedra, 2), c_int), tetrahedra, prob)
+        if (code /= FFH_OK) return
+        code = ffh_pattern_build(prob, FFH_FAMILY_H1, int(degree, c_int), &
+            n_global, nnz)
+        ok = .false.
+        if (code == FFH_OK) code = ffh_problem_matrix(prob, FFH_FULL, mat)
+        if (code == FFH_OK) call download_csc(mat, matrix_dot, ok)
+        if (ok) then
+            code = ffh_assemble_jvp(prob, int(quadrature_degree, c_int), &
+                FFH_COEFF_SCALAR, [stiffness_coefficient], mass_coefficient, &
+                mesh_vertices_dot, [stiffness_coefficient_dot], &
+                mass_coefficient_dot, matrix_dot%val)
+            ok = code == FFH_OK
+        end if
+        code = ffh_problem_free(prob)
+        if (ok) call status_set(status, 0, "")
+    end subroutine assemble_tetra_lagrange_stiffness_csc_jvp
+
+    ! :388-471.
+    subroutine assemble_tetra_lagrange_stiffness_csc_vjp(mesh_vertices, &
+            tetrahedra, degree, quadrature_degree, stiffness_coefficient, &
+            mass_coefficient, matrix_values_bar, mesh_vertices_bar, &
+            stiffness_coefficient_bar, mass_coefficient_bar, status)
+        real(dp), intent(in) :: mesh_vertices(:, :)
+        integer, intent(in) :: tetrahedra(:, :), degree, quadrature_degree
+        real(dp), intent(in) :: stiffness_coefficient, mass_coefficient
+        real(dp), intent(in) :: matrix_values_bar(:)
+        real(dp), intent(out) :: mesh_vertices_bar(:, :)
+        real(dp), intent(out) :: stiffness_coefficient_bar
+        real(dp), intent(out) :: mass_coefficient_bar
+        type(fortsparse_status_t), intent(out) :: status
+
+        type(c_ptr) :: prob
+        integer(c_int) :: n_global, code
+        integer(c_int64_t) :: nnz
+        real(c_double) :: kbar(1)
+        logical :: ok
+
+        mesh_vertices_bar = 0.0_dp
+        stiffness_coefficient_bar = 0.0_dp
+        mass_coefficient_bar = 0.0_dp
+        call status_set(status, FORTSPARSE_INVALID_MATRIX, &
+            "Tetrahedral H1 VJP shapes differ")
+        if (size(mesh_vertices, 1) /= 3) return
+        if (any(shape(mesh_vertices_bar) /= shape(mesh_vertices))) return
+        if (size(tetrahedra, 1) /= 4 .or. size(tetrahedra, 2) < 1) return
+        if (any(tetrahedra < 1) .or. &
+            any(tetrahedra > size(mesh_vertices, 2))) return
+        if (degree < 1 .or. degree > 4) return
+        code = ffh_problem_create(hip_context(), 3_c_int, &
+            int(size(mesh_vertices, 2), c_int), mesh_vertices, &
+            int(size(tetrahedra, 2), c_int), tetrahedra, prob)
+        if (code /= FFH_OK) return
+        code = ffh_pattern_build(prob, FFH_FAMILY_H1, int(degree, c_int), &
+            n_global, nnz)
+        ok = code == FFH_OK
+        if (ok) ok = size(matrix_values_bar) == nnz
+        if (ok) then
+            code = ffh_assemble_vjp(prob, int(quadrature_degree, c_int), &
+                FFH_COEFF_SCALAR, [stiffness_coefficient], mass_coefficient, &
+                matrix_values_bar, mesh_vertices_bar, kbar, mass_coefficient_bar)
+            ok = code == FFH_OK
+        end if
+        code = ffh_problem_free(prob)
+        if (ok) then
+            stiffness_coefficient_bar = kbar(1)
+            call status_set(status, 0, "")
+        else
+            mesh_vertices_bar = 0.0_dp
+            mass_coefficient_bar = 0.0_dp
+        end if
+    end subroutine assemble_tetra_lagrange_stiffness_csc_vjp
 
     ! :473-528.  source_values(nq, ncell) at the Duffy points of each cell.
     subroutine assemble_tetra_lagrange_scalar_load_samples(mesh_vertices, &

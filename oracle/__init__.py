@@ -278,6 +278,58 @@ def lagrange_stiffness_csc(vertices, tets, d, q, kappa=1.0, mu=0.0, cell_kappa=N
     return csc_from_triplet(G, G, rows, cols, vals), g, G
 
 
+def lagrange_stiffness_csc_jvp(vertices, tets, d, q, kappa=1.0, mu=0.0, vertices_dot=None,
+                               kappa_dot=0.0, mu_dot=0.0, cell_kappa=None, cell_kappa_dot=None):
+    """assemble_tetra_lagrange_stiffness_csc_jvp (:336-386) -> matrix_dot CSC
+    (the primal's pattern: same triplet positions)."""
+    v, t = _f(vertices), _i(tets)
+    vd = np.zeros_like(v) if vertices_dot is None else _f(vertices_dot)
+    if vd.shape != v.shape:
+        raise ValueError("vertices_dot shape differs")
+    st, g, G = lagrange_dof_map(d, t)
+    _check(st, "lagrange_dof_map")
+    n = g.shape[1]
+    nt = t.shape[0] * n * n
+    rows = np.zeros(nt, dtype=np.int32)
+    cols = np.zeros(nt, dtype=np.int32)
+    vals = np.zeros(nt)
+    ck = None if cell_kappa is None else _f(cell_kappa)
+    ckd = None if cell_kappa_dot is None else _f(cell_kappa_dot)
+    _check(_lib().ffo_lagrange_stiffness_jvp_triplets(
+        C.c_int(v.shape[0]), v.ctypes.data_as(C.c_void_p), vd.ctypes.data_as(C.c_void_p),
+        C.c_int(t.shape[0]), t.ctypes.data_as(C.c_void_p), C.c_int(d), C.c_int(q),
+        None if ck is None else ck.ctypes.data_as(C.c_void_p),
+        None if ckd is None else ckd.ctypes.data_as(C.c_void_p), C.c_double(kappa), C.c_double(mu),
+        C.c_double(kappa_dot), C.c_double(mu_dot), g.ctypes.data_as(C.c_void_p),
+        rows.ctypes.data_as(C.c_void_p), cols.ctypes.data_as(C.c_void_p),
+        vals.ctypes.data_as(C.c_void_p)), "stiffness_jvp_triplets")
+    return csc_from_triplet(G, G, rows, cols, vals)
+
+
+def lagrange_stiffness_csc_vjp(vertices, tets, d, q, values_bar, kappa=1.0, mu=0.0, cell_kappa=None):
+    """assemble_tetra_lagrange_stiffness_csc_vjp (:388-471) ->
+    (vertices_bar (nv, 3), kappa_bar, mu_bar, cell_kappa_bar (per tet))."""
+    a, g, G = lagrange_stiffness_csc(vertices, tets, d, q, kappa=kappa, mu=mu, cell_kappa=cell_kappa)
+    vb_in = _f(values_bar)
+    if vb_in.shape[0] != a.nnz:
+        raise ValueError("values_bar has the wrong size")
+    v, t = _f(vertices), _i(tets)
+    vb = np.zeros_like(v)
+    kb = C.c_double(0)
+    mb = C.c_double(0)
+    ckb = np.zeros(t.shape[0])
+    ck = None if cell_kappa is None else _f(cell_kappa)
+    _check(_lib().ffo_lagrange_stiffness_vjp(
+        C.c_int(v.shape[0]), v.ctypes.data_as(C.c_void_p), C.c_int(t.shape[0]),
+        t.ctypes.data_as(C.c_void_p), C.c_int(d), C.c_int(q),
+        None if ck is None else ck.ctypes.data_as(C.c_void_p), C.c_double(kappa), C.c_double(mu),
+        g.ctypes.data_as(C.c_void_p), a.col_ptr.ctypes.data_as(C.c_void_p),
+        a.row_idx.ctypes.data_as(C.c_void_p), vb_in.ctypes.data_as(C.c_void_p),
+        vb.ctypes.data_as(C.c_void_p), C.byref(kb), ckb.ctypes.data_as(C.c_void_p), C.byref(mb)),
+        "stiffness_vjp")
+    return vb, kb.value, mb.value, ckb
+
+
 def tetra_physical_points(vertices, tets, q):
     v, t = _f(vertices), _i(tets)
     nq = _lib().ffo_tetra_duffy_count(C.c_int(q))

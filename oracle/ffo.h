@@ -58,6 +58,21 @@ int ffo_lagrange_stiffness_element(const double vertices[12], int degree,
                                    int quadrature_degree, double kappa,
                                    double mu, double *matrix);
 /* triplets are column-outer / row-inner per tet, 1-based */
+int ffo_lagrange_stiffness_jvp_triplets(int nvert, const double *vertices,
+                                        const double *vertices_dot, int ntet,
+                                        const int *tets, int degree, int qdeg,
+                                        const double *cell_kappa,
+                                        const double *cell_kappa_dot, double kappa,
+                                        double mu, double kappa_dot, double mu_dot,
+                                        const int *global_dofs, int *rows, int *cols,
+                                        double *vals);
+int ffo_lagrange_stiffness_vjp(int nvert, const double *vertices, int ntet,
+                               const int *tets, int degree, int qdeg,
+                               const double *cell_kappa, double kappa, double mu,
+                               const int *global_dofs, const int *col_ptr,
+                               const int *row_idx, const double *values_bar,
+                               double *vertices_bar, double *kappa_bar,
+                               double *cell_kappa_bar, double *mu_bar);
 int ffo_lagrange_stiffness_triplets(int nvert, const double *vertices,
                                     int ntet, const int *tetrahedra,
                                     int degree, int quadrature_degree,

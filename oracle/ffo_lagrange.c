@@ -398,6 +398,251 @@ int ffo_lagrange_stiffness_triplets(int nvert, const double *vertices,
     return st;
 }
 
+/* fortnum's det3_jvp / inv3_jvp / inv3_vjp / det3_vjp (absent dependency,
+ * used at assembly_tetra_lagrange_arbitrary_order_3d.f90:135-139, 265-269),
+ * restated from their closed forms: d det = det tr(J^-1 dJ),
+ * d J^-1 = -J^-1 dJ J^-1, and the adjoints J_bar = -J^-T Inv_bar J^-T,
+ * J_bar = det_bar det J^-T.  Matrices column-major. */
+static void mat3_mul(const double *a, const double *b, double *c)
+{
+    for (int j = 0; j < 3; ++j)
+        for (int i = 0; i < 3; ++i) {
+            double s = 0.0;
+            for (int k = 0; k < 3; ++k) s += a[i + 3 * k] * b[k + 3 * j];
+            c[i + 3 * j] = s;
+        }
+}
+
+static void mat3_transpose(const double *a, double *t)
+{
+    for (int j = 0; j < 3; ++j)
+        for (int i = 0; i < 3; ++i) t[j + 3 * i] = a[i + 3 * j];
+}
+
+/* element JVP (:107-174): per point physical gradients and their tangents
+ * J^-T g, dJ^-T g; m_dot(r,c) += w (det_dot (k G + mu M) + det (k_dot G +
+ * k (dG_r.G_c + G_r.dG_c) + mu_dot M)). */
+static int element_jvp_with_table(const lagrange_table *t, const double v[12],
+                                  const double vd[12], double kappa, double mu,
+                                  double kappa_dot, double mu_dot, double *md,
+                                  double *pg, double *pgd)
+{
+    double jac[9], inv[9], det, jd[9], tmp[9], invd[9];
+    if (tetra_geometry(v, jac, inv, &det)) return 1;
+    for (int c = 0; c < 3; ++c)
+        for (int r = 0; r < 3; ++r) jd[r + 3 * c] = vd[3 * (c + 1) + r] - vd[r];
+    mat3_mul(inv, jd, tmp);
+    double det_dot = det * (tmp[0] + tmp[4] + tmp[8]);
+    mat3_mul(tmp, inv, invd);
+    for (int i = 0; i < 9; ++i) invd[i] = -invd[i];
+    int n = t->n;
+    for (int i = 0; i < n * n; ++i) md[i] = 0.0;
+    for (int q = 0; q < t->nq; ++q) {
+        const double *val = t->values + (size_t)q * n;
+        const double *g = t->gradients + (size_t)q * 3 * n;
+        for (int r = 0; r < n; ++r)
+            for (int i = 0; i < 3; ++i) {
+                pg[3 * r + i] = inv[0 + 3 * i] * g[3 * r + 0] + inv[1 + 3 * i] * g[3 * r + 1] +
+                                inv[2 + 3 * i] * g[3 * r + 2];
+                pgd[3 * r + i] = invd[0 + 3 * i] * g[3 * r + 0] +
+                                 invd[1 + 3 * i] * g[3 * r + 1] +
+                                 invd[2 + 3 * i] * g[3 * r + 2];
+            }
+        for (int c = 0; c < n; ++c)
+            for (int r = 0; r < n; ++r) {
+                double ge = pg[3 * r] * pg[3 * c] + pg[3 * r + 1] * pg[3 * c + 1] +
+                            pg[3 * r + 2] * pg[3 * c + 2];
+                double me = val[r] * val[c];
+                double cross = (pgd[3 * r] * pg[3 * c] + pgd[3 * r + 1] * pg[3 * c + 1] +
+                                pgd[3 * r + 2] * pg[3 * c + 2]) +
+                               (pg[3 * r] * pgd[3 * c] + pg[3 * r + 1] * pgd[3 * c + 1] +
+                                pg[3 * r + 2] * pgd[3 * c + 2]);
+                md[r + n * c] = md[r + n * c] +
+                                t->qw[q] * (det_dot * (kappa * ge + mu * me) +
+                                            det * (kappa_dot * ge + kappa * cross +
+                                                   mu_dot * me));
+            }
+    }
+    return 0;
+}
+
+/* element VJP (:176-275): accumulate det_bar, kappa_bar, mu_bar and
+ * Inv_bar = sum_q g pg_bar^T over the points, then through inv3_vjp,
+ * det3_vjp and tetra_jacobian_vjp (:428-437) to the vertices. */
+static int element_vjp_with_table(const lagrange_table *t, const double v[12],
+                                  double kappa, double mu, const double *mb,
+                                  double vb[12], double *kappa_bar, double *mu_bar,
+                                  double *pg, double *pgb)
+{
+    double jac[9], inv[9], det;
+    if (tetra_geometry(v, jac, inv, &det)) return 1;
+    int n = t->n;
+    double det_bar = 0.0, kb = 0.0, mub = 0.0, invb[9] = {0};
+    for (int q = 0; q < t->nq; ++q) {
+        const double *val = t->values + (size_t)q * n;
+        const double *g = t->gradients + (size_t)q * 3 * n;
+        for (int r = 0; r < n; ++r)
+            for (int i = 0; i < 3; ++i) {
+                pg[3 * r + i] = inv[0 + 3 * i] * g[3 * r + 0] + inv[1 + 3 * i] * g[3 * r + 1] +
+                                inv[2 + 3 * i] * g[3 * r + 2];
+                pgb[3 * r + i] = 0.0;
+            }
+        for (int c = 0; c < n; ++c)
+            for (int r = 0; r < n; ++r) {
+                double seed = t->qw[q] * mb[r + n * c];
+                double ge = pg[3 * r] * pg[3 * c] + pg[3 * r + 1] * pg[3 * c + 1] +
+                            pg[3 * r + 2] * pg[3 * c + 2];
+                double me = val[r] * val[c];
+                det_bar += seed * (kappa * ge + mu * me);
+                kb += seed * det * ge;
+                mub += seed * det * me;
+                for (int i = 0; i < 3; ++i) {
+                    pgb[3 * r + i] += seed * det * kappa * pg[3 * c + i];
+                    pgb[3 * c + i] += seed * det * kappa * pg[3 * r + i];
+                }
+            }
+        /* Inv_bar += gradients pg_bar^T: Inv_bar(j, i) += sum_r g(j,r) pgb(i,r) */
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j) {
+                double s = 0.0;
+                for (int r = 0; r < n; ++r) s += g[3 * r + j] * pgb[3 * r + i];
+                invb[j + 3 * i] += s;
+            }
+    }
+    double it[9], a[9], jb[9];
+    mat3_transpose(inv, it);
+    mat3_mul(it, invb, a);
+    mat3_mul(a, it, jb);
+    for (int i = 0; i < 9; ++i) jb[i] = -jb[i] + det_bar * det * it[i];
+    for (int r = 0; r < 3; ++r) {
+        vb[r] = -(jb[r] + jb[r + 3] + jb[r + 6]);
+        for (int c = 0; c < 3; ++c) vb[3 * (c + 1) + r] = jb[r + 3 * c];
+    }
+    *kappa_bar = kb;
+    *mu_bar = mub;
+    return 0;
+}
+
+static int gather_vertices(int nvert, const double *vertices, const int *tet, double v[12])
+{
+    for (int k = 0; k < 4; ++k) {
+        int id = tet[k];
+        if (id < 1 || id > nvert) return 1;
+        for (int c = 0; c < 3; ++c) v[3 * k + c] = vertices[3 * (id - 1) + c];
+    }
+    return 0;
+}
+
+/* assemble_tetra_lagrange_stiffness_csc_jvp (:336-386), triplet stage in
+ * the primal's order; cell_kappa / cell_kappa_dot (optional) are the
+ * per-cell form of the scalar coefficient and its tangent. */
+int ffo_lagrange_stiffness_jvp_triplets(int nvert, const double *vertices,
+                                        const double *vertices_dot, int ntet,
+                                        const int *tets, int degree, int qdeg,
+                                        const double *cell_kappa,
+                                        const double *cell_kappa_dot, double kappa,
+                                        double mu, double kappa_dot, double mu_dot,
+                                        const int *global_dofs, int *rows, int *cols,
+                                        double *vals)
+{
+    lagrange_table t;
+    if (degree < 1 || degree > 4 || qdeg < 0) return 1;
+    if (table_init(&t, degree, qdeg)) return 1;
+    int n = t.n;
+    double *m = malloc(sizeof(double) * n * n);
+    double *pg = malloc(sizeof(double) * 3 * n);
+    double *pgd = malloc(sizeof(double) * 3 * n);
+    size_t entry = 0;
+    int st = 0;
+    for (int e = 0; e < ntet && !st; ++e) {
+        double v[12], vd[12];
+        if (gather_vertices(nvert, vertices, tets + 4 * e, v) ||
+            gather_vertices(nvert, vertices_dot, tets + 4 * e, vd)) {
+            st = 1;
+            break;
+        }
+        double ke = cell_kappa ? cell_kappa[e] : kappa;
+        double ked = cell_kappa_dot ? cell_kappa_dot[e] : kappa_dot;
+        if (element_jvp_with_table(&t, v, vd, ke, mu, ked, mu_dot, m, pg, pgd)) {
+            st = 1;
+            break;
+        }
+        const int *g = global_dofs + (size_t)n * e;
+        for (int c = 0; c < n; ++c)
+            for (int r = 0; r < n; ++r, ++entry) {
+                rows[entry] = g[r];
+                cols[entry] = g[c];
+                vals[entry] = m[r + n * c];
+            }
+    }
+    free(m);
+    free(pg);
+    free(pgd);
+    table_free(&t);
+    return st;
+}
+
+/* assemble_tetra_lagrange_stiffness_csc_vjp (:388-471): element cotangents
+ * read from the primal CSC (csc_value_bar_at, :836-851: the entry of
+ * (row, col), 0 when absent), element VJPs, vertex bars summed per tet in
+ * tet order.  cell_kappa_bar (optional) receives each cell's kappa bar;
+ * kappa_bar is their sum in tet order. */
+int ffo_lagrange_stiffness_vjp(int nvert, const double *vertices, int ntet,
+                               const int *tets, int degree, int qdeg,
+                               const double *cell_kappa, double kappa, double mu,
+                               const int *global_dofs, const int *col_ptr,
+                               const int *row_idx, const double *values_bar,
+                               double *vertices_bar, double *kappa_bar,
+                               double *cell_kappa_bar, double *mu_bar)
+{
+    lagrange_table t;
+    if (degree < 1 || degree > 4 || qdeg < 0) return 1;
+    if (table_init(&t, degree, qdeg)) return 1;
+    int n = t.n;
+    double *eb = malloc(sizeof(double) * n * n);
+    double *pg = malloc(sizeof(double) * 3 * n);
+    double *pgb = malloc(sizeof(double) * 3 * n);
+    for (int i = 0; i < 3 * nvert; ++i) vertices_bar[i] = 0.0;
+    *kappa_bar = 0.0;
+    *mu_bar = 0.0;
+    int st = 0;
+    for (int e = 0; e < ntet && !st; ++e) {
+        double v[12], vb[12], kb, mb;
+        if (gather_vertices(nvert, vertices, tets + 4 * e, v)) {
+            st = 1;
+            break;
+        }
+        const int *g = global_dofs + (size_t)n * e;
+        for (int c = 0; c < n; ++c)
+            for (int r = 0; r < n; ++r) {
+                double b = 0.0;
+                int col = g[c];
+                for (int k = col_ptr[col - 1] - 1; k < col_ptr[col] - 1; ++k)
+                    if (row_idx[k] == g[r]) {
+                        b = values_bar[k];
+                        break;
+                    }
+                eb[r + n * c] = b;
+            }
+        double ke = cell_kappa ? cell_kappa[e] : kappa;
+        if (element_vjp_with_table(&t, v, ke, mu, eb, vb, &kb, &mb, pg, pgb)) {
+            st = 1;
+            break;
+        }
+        for (int k = 0; k < 4; ++k)
+            for (int c = 0; c < 3; ++c)
+                vertices_bar[3 * (tets[4 * e + k] - 1) + c] += vb[3 * k + c];
+        *kappa_bar += kb;
+        *mu_bar += mb;
+        if (cell_kappa_bar) cell_kappa_bar[e] = kb;
+    }
+    free(eb);
+    free(pg);
+    free(pgb);
+    table_free(&t);
+    return st;
+}
+
 /* assemble_tetra_lagrange_scalar_load_samples (:473-528):
  * b(g) += det w_q f(q,e) phi(q).  source_values is (nq, T) column-major. */
 int ffo_lagrange_load_samples(int nvert, const double *vertices, int ntet,

@@ -5,7 +5,8 @@
 ! test/test_advanced_solvers.f90:39-84,342-348,
 ! test/test_tetra_lagrange_poisson_p_convergence.f90:26-40,
 ! test/test_tetra_nedelec_sparse_assembly_slow.f90:31-59,
-! test/test_sparse_matrix_ad.f90:19-65, test/test_pcg_solver_ad.f90:18-60.
+! test/test_sparse_matrix_ad.f90:19-65, test/test_pcg_solver_ad.f90:18-60,
+! test/test_tetra_lagrange_stiffness_csc_ad.f90:10-93.
 module dropin_test_sources
     use fortfem_hip_types, only: dp
     implicit none
@@ -43,7 +44,9 @@ program test_fortfem_hip_dropin
     use fortfem_hip_runtime, only: fortfem_hip_finalize
     use fortfem_structured_tetra_box_mesh, only: generate_structured_tetra_box_mesh
     use fortfem_assembly_tetra_lagrange_arbitrary_order_3d, only: &
-        assemble_tetra_lagrange_stiffness_csc
+        assemble_tetra_lagrange_stiffness_csc, &
+        assemble_tetra_lagrange_stiffness_csc_jvp, &
+        assemble_tetra_lagrange_stiffness_csc_vjp
     use fortfem_assembly_tetra_nedelec_3d, only: assemble_tetra_nedelec_curl_mass_csc, &
         assemble_tetra_nedelec_vector_load
     use fortfem_sparse_matrix, only: spmv, spmv_jvp, spmv_vjp
@@ -65,6 +68,7 @@ program test_fortfem_hip_dropin
     call check_nedelec_constant_load()
     call check_spmv_ad()
     call check_pcg_ad()
+    call check_h1_csc_ad()
     call fortfem_hip_finalize()
     if (.not. all_passed) error stop 1
     write (*, '(a)') "drop-in Fortran checks passed"
@@ -318,6 +322,64 @@ contains
             3.0e-8_dp*max(1.0_dp, abs(lhs), abs(identity)), &
             "PCG VJP satisfies the implicit-solve adjoint identity")
     end subroutine check_pcg_ad
+
+    subroutine check_h1_csc_ad()
+        integer, parameter :: degree = 2, quadrature_degree = 6
+        integer, parameter :: tets(4, 2) = reshape([1, 2, 3, 4, 2, 3, 4, 5], [4, 2])
+        real(dp), parameter :: step = 2.0e-7_dp
+        real(dp) :: vertices(3, 5), vertices_dot(3, 5), vertices_bar(3, 5)
+        real(dp), allocatable :: values_bar(:)
+        type(csc_t) :: matrix, matrix_dot, minus, plus
+        type(fortsparse_status_t) :: st
+        real(dp) :: kappa, mu, kappa_dot, mu_dot, kappa_bar, mu_bar
+        real(dp) :: lhs, rhs, relative_error
+        integer :: entry
+
+        vertices = reshape([0.0_dp, 0.0_dp, 0.0_dp, 1.0_dp, 0.0_dp, 0.0_dp, &
+            0.0_dp, 1.0_dp, 0.0_dp, 0.0_dp, 0.0_dp, 1.0_dp, 1.0_dp, 1.0_dp, &
+            1.0_dp], [3, 5])
+        vertices_dot = reshape([0.01_dp, -0.02_dp, 0.015_dp, -0.03_dp, 0.01_dp, &
+            0.02_dp, 0.025_dp, -0.015_dp, 0.005_dp, -0.01_dp, 0.03_dp, -0.02_dp, &
+            0.02_dp, -0.01_dp, 0.025_dp], [3, 5])
+        kappa = 1.7_dp
+        mu = 0.8_dp
+        kappa_dot = 0.13_dp
+        mu_dot = -0.09_dp
+        call assemble_tetra_lagrange_stiffness_csc(vertices, tets, degree, &
+            quadrature_degree, matrix, st, kappa, mu)
+        call record(st%code == 0, "global H1 primal assembly succeeds")
+        call assemble_tetra_lagrange_stiffness_csc_jvp(vertices, tets, degree, &
+            quadrature_degree, kappa, mu, vertices_dot, kappa_dot, mu_dot, &
+            matrix_dot, st)
+        call record(st%code == 0, "global H1 JVP succeeds")
+        call assemble_tetra_lagrange_stiffness_csc(vertices + step*vertices_dot, &
+            tets, degree, quadrature_degree, plus, st, kappa + step*kappa_dot, &
+            mu + step*mu_dot)
+        call assemble_tetra_lagrange_stiffness_csc(vertices - step*vertices_dot, &
+            tets, degree, quadrature_degree, minus, st, kappa - step*kappa_dot, &
+            mu - step*mu_dot)
+        call record(matrix_dot%nnz == matrix%nnz .and. &
+            all(matrix_dot%col_ptr == matrix%col_ptr) .and. &
+            all(matrix_dot%row_idx == matrix%row_idx), &
+            "global H1 JVP preserves the merged CSC pattern")
+        relative_error = maxval(abs(matrix_dot%val - &
+            (plus%val - minus%val)/(2.0_dp*step)))/ &
+            max(1.0_dp, maxval(abs(matrix_dot%val)))
+        call record(relative_error < 5.0e-8_dp, &
+            "global H1 JVP matches independent complete mesh reassembly")
+        allocate(values_bar(matrix%nnz))
+        do entry = 1, matrix%nnz
+            values_bar(entry) = 0.003_dp*entry - 0.00001_dp*entry**2
+        end do
+        call assemble_tetra_lagrange_stiffness_csc_vjp(vertices, tets, degree, &
+            quadrature_degree, kappa, mu, values_bar, vertices_bar, kappa_bar, &
+            mu_bar, st)
+        call record(st%code == 0, "global H1 VJP succeeds")
+        lhs = dot_product(values_bar, matrix_dot%val)
+        rhs = sum(vertices_bar*vertices_dot) + kappa_bar*kappa_dot + mu_bar*mu_dot
+        call record(abs(lhs - rhs) < 3.0e-10_dp*max(1.0_dp, abs(lhs), abs(rhs)), &
+            "global H1 products accumulate the shared-vertex adjoint")
+    end subroutine check_h1_csc_ad
 
     subroutine check_linear_energy()
         real(dp) :: mesh_vertices(3, 5), coefficients(5), product(5)
