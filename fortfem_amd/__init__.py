@@ -340,8 +340,9 @@ class Problem:
 
     def assemble_jvp(self, quadrature_degree, kappa=1.0, mass=0.0, cell_kappa=None, vertices_dot=None,
                      kappa_dot=0.0, cell_kappa_dot=None, mass_dot=0.0):
-        """Tangent of the H1 matrix values (ffh_assemble_jvp) on the pattern
-        of matrix(FULL); vertices_dot is (nv, 3)."""
+        """Tangent of the H1 or Nedelec matrix values (ffh_assemble_jvp) on
+        the pattern of matrix(FULL); vertices_dot is (nv, 3).  Nedelec: kappa
+        is the curl coefficient, mass the mass coefficient."""
         kind, k = self._coefficients(kappa, cell_kappa)
         if kind == capi.COEFF_CELL:
             kd = np.zeros(self.ncell) if cell_kappa_dot is None else _f64(cell_kappa_dot)
@@ -358,7 +359,7 @@ class Problem:
         return out[:self.sizes()["nnz"]]
 
     def assemble_vjp(self, quadrature_degree, values_bar, kappa=1.0, mass=0.0, cell_kappa=None):
-        """Cotangent of the H1 matrix values (ffh_assemble_vjp) ->
+        """Cotangent of the H1 or Nedelec matrix values (ffh_assemble_vjp) ->
         (vertices_bar (nv, 3), kappa_bar (scalar, or per cell with
         cell_kappa), mass_bar)."""
         kind, k = self._coefficients(kappa, cell_kappa)
@@ -763,6 +764,58 @@ def assemble_tetra_nedelec_curl_mass_csc(mesh_vertices, tetrahedra, curl_coeffic
             raise
         return None, status
     return a, FortsparseStatus(0, "")
+
+
+def assemble_tetra_nedelec_curl_mass_csc_jvp(mesh_vertices, tetrahedra, order, curl_coefficient,
+                                             mass_coefficient, mesh_vertices_dot, curl_coefficient_dot,
+                                             mass_coefficient_dot, ctx: Context | None = None):
+    """assemble_tetra_nedelec_curl_mass_csc_jvp (assembly_tetra_nedelec_3d.f90:
+    2114-2180) -> (matrix_dot CSC on the primal's pattern, status)."""
+    status = FortsparseStatus(FORTSPARSE_INVALID_MATRIX, "Tetrahedral Nedelec curl-mass JVP assembly failed")
+    v = np.asarray(mesh_vertices)
+    t = np.asarray(tetrahedra)
+    vd = np.asarray(mesh_vertices_dot)
+    if order < 1 or v.ndim != 2 or v.shape[1] != 3 or t.ndim != 2 or t.shape[1] != 4 or t.shape[0] < 1:
+        return None, status
+    if vd.shape != v.shape or t.min() < 1 or t.max() > v.shape[0]:
+        return None, status
+    try:
+        p = Problem(v, t, ctx)
+        p.pattern_build(capi.FAMILY_NEDELEC, order)
+        vals = p.assemble_jvp(-1, kappa=curl_coefficient, mass=mass_coefficient, vertices_dot=vd,
+                              kappa_dot=curl_coefficient_dot, mass_dot=mass_coefficient_dot)
+        a = p.matrix(capi.FULL).download()
+    except FFHError as e:
+        if e.code == capi.FFH_ERR_DEVICE:
+            raise
+        return None, status
+    a.val = vals
+    return a, FortsparseStatus(0, "")
+
+
+def assemble_tetra_nedelec_curl_mass_csc_vjp(mesh_vertices, tetrahedra, order, curl_coefficient,
+                                             mass_coefficient, matrix_values_bar, ctx: Context | None = None):
+    """assemble_tetra_nedelec_curl_mass_csc_vjp (:2182-2287) ->
+    (mesh_vertices_bar (nv, 3), curl_coefficient_bar, mass_coefficient_bar, status)."""
+    status = FortsparseStatus(FORTSPARSE_INVALID_MATRIX, "Tetrahedral Nedelec curl-mass VJP shapes differ")
+    v = np.asarray(mesh_vertices)
+    t = np.asarray(tetrahedra)
+    zero = (np.zeros_like(v, dtype=np.float64), 0.0, 0.0)
+    if order < 1 or v.ndim != 2 or v.shape[1] != 3 or t.ndim != 2 or t.shape[1] != 4 or t.shape[0] < 1:
+        return (*zero, status)
+    if t.min() < 1 or t.max() > v.shape[0]:
+        return (*zero, status)
+    try:
+        p = Problem(v, t, ctx)
+        _, nnz = p.pattern_build(capi.FAMILY_NEDELEC, order)
+        if np.asarray(matrix_values_bar).shape[0] != nnz:
+            return (*zero, status)
+        xb, ab, bb = p.assemble_vjp(-1, matrix_values_bar, kappa=curl_coefficient, mass=mass_coefficient)
+    except FFHError as e:
+        if e.code == capi.FFH_ERR_DEVICE:
+            raise
+        return (*zero, status)
+    return xb, ab, bb, FortsparseStatus(0, "")
 
 
 def assemble_tetra_nedelec_weighted_csc(mesh_vertices, tetrahedra, coefficient, mass_coefficient,

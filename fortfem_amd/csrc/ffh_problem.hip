@@ -433,6 +433,157 @@ __global__ __launch_bounds__(256) void k_ned_assemble(
     }
 }
 
+// Tangent of the 12 Nedelec terms of the curl-mass variant
+// (assemble_tetra_nedelec_curl_mass_element_jvp,
+// assembly_tetra_nedelec_3d.f90:645-716, in the reference-tensor form):
+// c_s = alpha H_s / det with H = J^T J and v_s = beta det G_s with
+// G = J^-1 J^-T, so
+//   dc_s = (dalpha H_s + alpha dH_s) / det - alpha H_s ddet / det^2,
+//   dv_s = dbeta det G_s + beta ddet G_s + beta det dG_s,
+// dH = dJ^T J + J^T dJ, ddet = det tr(J^-1 dJ), dJ^-1 = -J^-1 dJ J^-1.
+__global__ __launch_bounds__(kBlockA) void k_ned_terms_jvp(
+    int ncell, const int *__restrict__ cells_c, const double *__restrict__ verts,
+    const double *__restrict__ verts_dot, const int *__restrict__ perm,
+    const double *__restrict__ alpha_cell, const double *__restrict__ alpha_cell_dot,
+    double alpha, double alpha_dot, double beta, double beta_dot, double *__restrict__ gterm,
+    int *__restrict__ err)
+{
+    const int pos = blockIdx.x * kBlockA + threadIdx.x;
+    if (pos >= ncell) return;
+    const int4 c = reinterpret_cast<const int4 *>(cells_c)[pos];
+    double jac[9], inv[9], det;
+    if (!tet_geometry(verts, c.x, c.y, c.z, c.w, jac, inv, det)) {
+        atomicOr(err, 1);
+        return;
+    }
+    double jd[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    if (verts_dot) {
+        const int id[4] = {c.x, c.y, c.z, c.w};
+        const double *v0 = verts_dot + 3 * (size_t)c.x;
+        for (int cc = 0; cc < 3; ++cc) {
+            const double *v = verts_dot + 3 * (size_t)id[cc + 1];
+            for (int r = 0; r < 3; ++r) jd[r + 3 * cc] = v[r] - v0[r];
+        }
+    }
+    double m[9], invd[9];
+    for (int j = 0; j < 3; ++j)
+        for (int i = 0; i < 3; ++i)
+            m[i + 3 * j] = inv[i] * jd[3 * j] + inv[i + 3] * jd[1 + 3 * j] +
+                           inv[i + 6] * jd[2 + 3 * j];
+    const double detd = det * (m[0] + m[4] + m[8]);
+    for (int j = 0; j < 3; ++j)
+        for (int i = 0; i < 3; ++i)
+            invd[i + 3 * j] = -(m[i] * inv[3 * j] + m[i + 3] * inv[1 + 3 * j] +
+                                m[i + 6] * inv[2 + 3 * j]);
+    const int e = perm[pos];
+    const double a = alpha_cell ? alpha_cell[e] : alpha;
+    const double ad = alpha_cell_dot ? alpha_cell_dot[e] : alpha_dot;
+    const int pa[6] = {0, 1, 2, 0, 0, 1}, pb[6] = {0, 1, 2, 1, 2, 2};
+    const size_t T = (size_t)ncell;
+    const double rdet = 1.0 / det;
+#pragma unroll
+    for (int s = 0; s < 6; ++s) {
+        const int ia = pa[s], ib = pb[s];
+        double H = 0, Hd = 0, G = 0, Gd = 0;
+        for (int t = 0; t < 3; ++t) {
+            H += jac[t + 3 * ia] * jac[t + 3 * ib];
+            Hd += jd[t + 3 * ia] * jac[t + 3 * ib] + jac[t + 3 * ia] * jd[t + 3 * ib];
+            G += inv[ia + 3 * t] * inv[ib + 3 * t];
+            Gd += invd[ia + 3 * t] * inv[ib + 3 * t] + inv[ia + 3 * t] * invd[ib + 3 * t];
+        }
+        gterm[(size_t)s * T + pos] = (ad * H + a * Hd) * rdet - a * H * detd * rdet * rdet;
+        gterm[(size_t)(6 + s) * T + pos] = beta_dot * det * G + beta * detd * G + beta * det * Gd;
+    }
+}
+
+// Cotangent of the Nedelec curl-mass matrix
+// (assemble_tetra_nedelec_curl_mass_csc_vjp, :2182-2287), one colour per
+// launch, one wave per cell: the lanes contract the cell's oriented element
+// cotangent (read through the entry slots) with the class's 12 tables,
+// t_s = sum_rc Eb_rc Tab_s,rc; lane 0 takes the 12 sums back through
+// c_s = alpha H_s / det and v_s = beta det G_s to alpha, beta and the
+// vertices.  Cells of one colour may share a vertex (the dofs sit on edges,
+// faces and interiors), so the vertex sums are atomic.
+__global__ __launch_bounds__(256) void k_ned_vjp(
+    int pos0, int cnt, int ncell, int n, const unsigned char *__restrict__ cls_c,
+    const int *__restrict__ cells_c, const double *__restrict__ verts,
+    const int *__restrict__ perm, const double *__restrict__ alpha_cell, double alpha,
+    double beta, const double *__restrict__ tab, const int *__restrict__ slots_cm,
+    const double *__restrict__ vbar, double *__restrict__ verts_bar, double *__restrict__ ab,
+    double *__restrict__ bb, int *__restrict__ err)
+{
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int i = blockIdx.x * 4 + wave;
+    if (i >= cnt) return; // uniform per wave
+    const int pos = pos0 + i;
+    const int nn = n * n, np = n * (n + 1) / 2;
+    const int cl = cls_c[pos];
+    const double *tb = tab + (size_t)cl * 12 * np;
+    double acc[12];
+#pragma unroll
+    for (int s = 0; s < 12; ++s) acc[s] = 0.0;
+    for (int t = lane; t < nn; t += 64) {
+        int sl = slots_cm[(size_t)pos * nn + t];
+        if (sl < 0) sl = ~sl;
+        const double e = vbar[sl];
+        const int cc = t / n, r = t - cc * n;
+        const int lo = min(r, cc), hi = max(r, cc);
+        const int u = lo * n - (lo * (lo - 1)) / 2 + (hi - lo);
+#pragma unroll
+        for (int s = 0; s < 12; ++s) acc[s] += e * tb[(size_t)s * np + u];
+    }
+#pragma unroll
+    for (int s = 0; s < 12; ++s)
+        for (int off = 32; off > 0; off >>= 1) acc[s] += __shfl_xor(acc[s], off);
+    if (lane != 0) return;
+    const int4 c = reinterpret_cast<const int4 *>(cells_c)[pos];
+    double jac[9], inv[9], det;
+    if (!tet_geometry(verts, c.x, c.y, c.z, c.w, jac, inv, det)) {
+        atomicOr(err, 1);
+        return;
+    }
+    const double a = alpha_cell ? alpha_cell[perm[pos]] : alpha;
+    const int pa[6] = {0, 1, 2, 0, 0, 1}, pb[6] = {0, 1, 2, 1, 2, 2};
+    const double rdet = 1.0 / det;
+    double jb[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, ibar[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    double abar = 0.0, bbar = 0.0, det_bar = 0.0;
+    for (int s = 0; s < 6; ++s) {
+        const int ia = pa[s], ib = pb[s];
+        double H = 0, G = 0;
+        for (int t = 0; t < 3; ++t) {
+            H += jac[t + 3 * ia] * jac[t + 3 * ib];
+            G += inv[ia + 3 * t] * inv[ib + 3 * t];
+        }
+        abar += acc[s] * H * rdet;
+        bbar += acc[6 + s] * det * G;
+        det_bar += -acc[s] * a * H * rdet * rdet + acc[6 + s] * beta * G;
+        const double hb = a * acc[s] * rdet, gb = beta * det * acc[6 + s];
+        for (int t = 0; t < 3; ++t) {
+            jb[t + 3 * ia] += hb * jac[t + 3 * ib];
+            jb[t + 3 * ib] += hb * jac[t + 3 * ia];
+            ibar[ia + 3 * t] += gb * inv[ib + 3 * t];
+            ibar[ib + 3 * t] += gb * inv[ia + 3 * t];
+        }
+    }
+    ab[pos] = abar;
+    bb[pos] = bbar;
+    double w[9];
+    for (int j = 0; j < 3; ++j)
+        for (int r = 0; r < 3; ++r)
+            w[r + 3 * j] = inv[3 * r] * ibar[3 * j] + inv[1 + 3 * r] * ibar[1 + 3 * j] +
+                           inv[2 + 3 * r] * ibar[2 + 3 * j];
+    for (int j = 0; j < 3; ++j)
+        for (int r = 0; r < 3; ++r)
+            jb[r + 3 * j] += -(w[r] * inv[j] + w[r + 3] * inv[j + 3] + w[r + 6] * inv[j + 6]) +
+                             det_bar * det * inv[j + 3 * r];
+    const int id[4] = {c.x, c.y, c.z, c.w};
+    for (int r = 0; r < 3; ++r) {
+        atomicAdd(&verts_bar[3 * (size_t)id[0] + r], -(jb[r] + jb[r + 3] + jb[r + 6]));
+        for (int q = 0; q < 3; ++q)
+            atomicAdd(&verts_bar[3 * (size_t)id[q + 1] + r], jb[r + 3 * q]);
+    }
+}
+
 // Nedelec vector load (assemble_tetra_nedelec_vector_load(_order),
 // src/assembly/assembly_tetra_nedelec_3d.f90:71-127, 525-582, 2002-2040,
 // 2516-2553), step 1, one thread per cell: g_q = det w_q J^-1 f(x_q) at the
@@ -853,6 +1004,22 @@ void problem_pattern(Problem *P, int family, int order)
 // (assemble_tetra_nedelec_weighted_csc, :1870-2000: a symmetric 3x3 tensor per
 // cell on the curl term).  Each cell's matrix is T^T M T of the reference, so
 // the assembled values match its triplet sums.
+// class tables for the element rule of the variant (k = 1: 4 points)
+static void ensure_ned_tables(Problem *P, bool weighted)
+{
+    const int k = P->order;
+    const int rule = (k == 1) ? 0 : (weighted ? 2 : 1);
+    if (P->ned_rule == rule) return;
+    hipStream_t s = P->ctx->stream;
+    std::vector<double> tab;
+    nedelec_class_tables(k, weighted, P->class_list, tab);
+    P->d_ned64.upload(tab, s);
+    std::vector<float> t32(tab.begin(), tab.end());
+    P->d_ned32.upload(t32, s);
+    FFH_HIP(hipStreamSynchronize(s));
+    P->ned_rule = rule;
+}
+
 static void assemble_nedelec(Problem *P, int coeff_kind, const double *coeff, double mass,
                              int precision)
 {
@@ -894,17 +1061,7 @@ static void assemble_nedelec(Problem *P, int coeff_kind, const double *coeff, do
     } else {
         throw Error{FFH_ERR_INVALID, "unknown coefficient kind"};
     }
-    // class tables for the element rule of the variant (k = 1: 4 points)
-    const int rule = (k == 1) ? 0 : (weighted ? 2 : 1);
-    if (P->ned_rule != rule) {
-        std::vector<double> tab;
-        nedelec_class_tables(k, weighted, P->class_list, tab);
-        P->d_ned64.upload(tab, s);
-        std::vector<float> t32(tab.begin(), tab.end());
-        P->d_ned32.upload(t32, s);
-        FFH_HIP(hipStreamSynchronize(s));
-        P->ned_rule = rule;
-    }
+    ensure_ned_tables(P, weighted);
     P->d_gterm.alloc((size_t)12 * T);
     hipLaunchKernelGGL(k_ned_terms, dim3((T + kBlockA - 1) / kBlockA), dim3(kBlockA), 0, s, T,
                        P->d_cells_c.p, P->d_vertices.p, P->d_perm.p, dalpha, dtensor, alpha,
@@ -1050,6 +1207,98 @@ void problem_assemble(Problem *P, int qdeg, int coeff_kind, const double *coeff,
     P->red_dirty = true;
 }
 
+// Nedelec curl-mass tangent (assemble_tetra_nedelec_curl_mass_csc_jvp,
+// assembly_tetra_nedelec_3d.f90:2114-2180): tangent terms, then the
+// primal's class-table assembly, colour by colour (every slot gets a first
+// store, so the output needs no clearing).
+static void nedelec_jvp(Problem *P, int coeff_kind, const double *coeff, double beta,
+                        const double *vertices_dot, const double *coeff_dot, double beta_dot,
+                        double *values_dot)
+{
+    hipStream_t s = P->ctx->stream;
+    const int T = P->ncell, nl = P->n_local;
+    double alpha = 1.0;
+    const double *da = scalar_coefficients(P, coeff_kind, coeff, alpha);
+    ensure_ned_tables(P, false);
+    double alpha_dot = 0.0;
+    DBuf<double> dad, dvd, vals;
+    if (coeff_kind == FFH_COEFF_SCALAR) {
+        if (coeff_dot) alpha_dot = coeff_dot[0];
+    } else if (coeff_dot) {
+        dad.upload(coeff_dot, (size_t)T, s);
+    }
+    if (vertices_dot) dvd.upload(vertices_dot, 3 * (size_t)P->nvert, s);
+    P->d_gterm.alloc((size_t)12 * T);
+    hipLaunchKernelGGL(k_ned_terms_jvp, dim3((T + kBlockA - 1) / kBlockA), dim3(kBlockA), 0, s,
+                       T, P->d_cells_c.p, P->d_vertices.p, dvd.p, P->d_perm.p, da, dad.p, alpha,
+                       alpha_dot, beta, beta_dot, P->d_gterm.p, P->d_err.p);
+    const int64_t nnz = P->full.m.nnz;
+    vals.alloc(std::max<int64_t>(nnz, 1));
+    const int gy = (nl * nl + 255) / 256;
+    for (int c = 0; c < P->n_colors; ++c) {
+        const int p0 = P->color_start[c], cnt = P->color_start[c + 1] - p0;
+        if (cnt == 0) continue;
+        hipLaunchKernelGGL((k_ned_assemble<double, double>),
+                           dim3((cnt + kNedCells - 1) / kNedCells, gy), dim3(256), 0, s, p0, cnt,
+                           T, nl, P->d_cls_c.p, P->d_gterm.p, P->d_ned64.p, P->d_slots_cm.p,
+                           vals.p);
+    }
+    FFH_HIP(hipGetLastError());
+    check_err(P, "Nedelec assembly tangent");
+    vals.download(values_dot, nnz, s);
+    FFH_HIP(hipStreamSynchronize(s));
+}
+
+// Nedelec curl-mass cotangent (:2182-2287): k_ned_vjp per colour, per-cell
+// alpha / beta bars summed on the host in cell order.
+static void nedelec_vjp(Problem *P, int coeff_kind, const double *coeff, double beta,
+                        const double *values_bar, double *vertices_bar, double *coeff_bar,
+                        double *beta_bar)
+{
+    hipStream_t s = P->ctx->stream;
+    const int T = P->ncell, nl = P->n_local;
+    double alpha = 1.0;
+    const double *da = scalar_coefficients(P, coeff_kind, coeff, alpha);
+    ensure_ned_tables(P, false);
+    const int64_t nnz = P->full.m.nnz;
+    DBuf<double> vb, xb, ab, bb;
+    vb.upload(values_bar, (size_t)nnz, s);
+    xb.alloc(3 * (size_t)P->nvert);
+    FFH_HIP(hipMemsetAsync(xb.p, 0, sizeof(double) * 3 * (size_t)P->nvert, s));
+    ab.alloc(std::max(T, 1));
+    bb.alloc(std::max(T, 1));
+    for (int c = 0; c < P->n_colors; ++c) {
+        const int p0 = P->color_start[c], cnt = P->color_start[c + 1] - p0;
+        if (cnt == 0) continue;
+        hipLaunchKernelGGL(k_ned_vjp, dim3((cnt + 3) / 4), dim3(256), 0, s, p0, cnt, T, nl,
+                           P->d_cls_c.p, P->d_cells_c.p, P->d_vertices.p, P->d_perm.p, da,
+                           alpha, beta, P->d_ned64.p, P->d_slots_cm.p, vb.p, xb.p, ab.p, bb.p,
+                           P->d_err.p);
+    }
+    FFH_HIP(hipGetLastError());
+    check_err(P, "Nedelec assembly cotangent");
+    std::vector<double> ha(T), hb(T);
+    ab.download(ha.data(), T, s);
+    bb.download(hb.data(), T, s);
+    xb.download(vertices_bar, 3 * (size_t)P->nvert, s);
+    FFH_HIP(hipStreamSynchronize(s));
+    std::vector<double> ca(T), cb(T);
+    for (int pos = 0; pos < T; ++pos) {
+        ca[P->perm[pos]] = ha[pos];
+        cb[P->perm[pos]] = hb[pos];
+    }
+    double asum = 0.0, bsum = 0.0;
+    for (int e = 0; e < T; ++e) {
+        asum += ca[e];
+        bsum += cb[e];
+    }
+    if (coeff_kind == FFH_COEFF_CELL)
+        std::copy(ca.begin(), ca.end(), coeff_bar);
+    else
+        coeff_bar[0] = asum;
+    *beta_bar = bsum;
+}
+
 // Tangent of the tetrahedral H1 matrix on the problem's pattern
 // (assemble_tetra_lagrange_stiffness_csc_jvp,
 // assembly_tetra_lagrange_arbitrary_order_3d.f90:336-386): the element
@@ -1063,8 +1312,12 @@ void problem_assemble_jvp(Problem *P, int qdeg, int coeff_kind, const double *co
     Context *ctx = P->ctx;
     hipStream_t s = ctx->stream;
     FFH_REQUIRE(P->n_global > 0, FFH_ERR_INVALID, "pattern_build must come first");
+    if (P->family == FFH_FAMILY_NEDELEC) {
+        nedelec_jvp(P, coeff_kind, coeff, mass, vertices_dot, coeff_dot, mass_dot, values_dot);
+        return;
+    }
     FFH_REQUIRE(P->family == FFH_FAMILY_H1, FFH_ERR_INVALID,
-                "assembly tangents are built for the tetrahedral H1 family");
+                "assembly tangents are built for the tetrahedral H1 and Nedelec families");
     const int T = P->ncell;
     double kappa = 1.0;
     const double *dk = scalar_coefficients(P, coeff_kind, coeff, kappa);
@@ -1116,8 +1369,12 @@ void problem_assemble_vjp(Problem *P, int qdeg, int coeff_kind, const double *co
     Context *ctx = P->ctx;
     hipStream_t s = ctx->stream;
     FFH_REQUIRE(P->n_global > 0, FFH_ERR_INVALID, "pattern_build must come first");
+    if (P->family == FFH_FAMILY_NEDELEC) {
+        nedelec_vjp(P, coeff_kind, coeff, mass, values_bar, vertices_bar, coeff_bar, mass_bar);
+        return;
+    }
     FFH_REQUIRE(P->family == FFH_FAMILY_H1, FFH_ERR_INVALID,
-                "assembly cotangents are built for the tetrahedral H1 family");
+                "assembly cotangents are built for the tetrahedral H1 and Nedelec families");
     const int T = P->ncell;
     double kappa = 1.0;
     const double *dk = scalar_coefficients(P, coeff_kind, coeff, kappa);

@@ -490,6 +490,8 @@ module fortfem_assembly_tetra_nedelec_3d
     private
 
     public :: assemble_tetra_nedelec_curl_mass_csc
+    public :: assemble_tetra_nedelec_curl_mass_csc_jvp
+    public :: assemble_tetra_nedelec_curl_mass_csc_vjp
     public :: assemble_tetra_nedelec_weighted_csc
     public :: assemble_tetra_nedelec_weighted_csc_cells
     public :: assemble_tetra_nedelec_vector_load
@@ -652,6 +654,104 @@ contains
         end if
         if (ok) call status_set(status, 0, "")
     end subroutine assemble_tetra_nedelec_curl_mass_csc
+
+    ! :2114-2180.  matrix_dot has the primal matrix's pattern.
+    subroutine assemble_tetra_nedelec_curl_mass_csc_jvp(mesh_vertices, &
+            tetrahedra, order, curl_coefficient, mass_coefficient, &
+            mesh_vertices_dot, curl_coefficient_dot, mass_coefficient_dot, &
+            matrix_dot, status)
+        real(dp), intent(in) :: mesh_vertices(:, :), mesh_vertices_dot(:, :)
+        integer, intent(in) :: tetrahedra(:, :), order
+        real(dp), intent(in) :: curl_coefficient, mass_coefficient
+        real(dp), intent(in) :: curl_coefficient_dot, mass_coefficient_dot
+        type(csc_t), intent(out) :: matrix_dot
+        type(fortsparse_status_t), intent(out) :: status
+
+        type(c_ptr) :: prob, mat
+        integer(c_int) :: n_global, code
+        integer(c_int64_t) :: nnz
+        logical :: ok
+
+        call status_set(status, FORTSPARSE_INVALID_MATRIX, &
+            "Tetrahedral Nedelec curl-mass JVP assembly failed")
+        if (size(mesh_vertices, 1) /= 3) return
+        if (any(shape(mesh_vertices_dot) /= shape(mesh_vertices))) return
+        if (size(tetrahedra, 1) /= 4 .or. size(tetrahedra, 2) < 1) return
+        if (any(tetrahedra < 1) .or. &
+            any(tetrahedra > size(mesh_vertices, 2))) return
+        if (order < 1 .or. order > 6) return
+        code = ffh_problem_create(hip_context(), 3_c_int, &
+            int(size(mesh_vertices, 2), c_int), mesh_vertices, &
+            int(size(tetrahedra, 2), c_int), tetrahedra, prob)
+        if (code /= FFH_OK) return
+        code = ffh_pattern_build(prob, FFH_FAMILY_NEDELEC, int(order, c_int), &
+            n_global, nnz)
+        ok = .false.
+        if (code == FFH_OK) code = ffh_problem_matrix(prob, FFH_FULL, mat)
+        if (code == FFH_OK) call download_csc(mat, matrix_dot, ok)
+        if (ok) then
+            code = ffh_assemble_jvp(prob, -1_c_int, FFH_COEFF_SCALAR, &
+                [curl_coefficient], mass_coefficient, mesh_vertices_dot, &
+                [curl_coefficient_dot], mass_coefficient_dot, matrix_dot%val)
+            ok = code == FFH_OK
+        end if
+        code = ffh_problem_free(prob)
+        if (ok) call status_set(status, 0, "")
+    end subroutine assemble_tetra_nedelec_curl_mass_csc_jvp
+
+    ! :2182-2287.
+    subroutine assemble_tetra_nedelec_curl_mass_csc_vjp(mesh_vertices, &
+            tetrahedra, order, curl_coefficient, mass_coefficient, &
+            matrix_values_bar, mesh_vertices_bar, curl_coefficient_bar, &
+            mass_coefficient_bar, status)
+        real(dp), intent(in) :: mesh_vertices(:, :)
+        integer, intent(in) :: tetrahedra(:, :), order
+        real(dp), intent(in) :: curl_coefficient, mass_coefficient
+        real(dp), intent(in) :: matrix_values_bar(:)
+        real(dp), intent(out) :: mesh_vertices_bar(:, :)
+        real(dp), intent(out) :: curl_coefficient_bar, mass_coefficient_bar
+        type(fortsparse_status_t), intent(out) :: status
+
+        type(c_ptr) :: prob
+        integer(c_int) :: n_global, code
+        integer(c_int64_t) :: nnz
+        real(c_double) :: abar(1)
+        logical :: ok
+
+        mesh_vertices_bar = 0.0_dp
+        curl_coefficient_bar = 0.0_dp
+        mass_coefficient_bar = 0.0_dp
+        call status_set(status, FORTSPARSE_INVALID_MATRIX, &
+            "Tetrahedral Nedelec curl-mass VJP shapes differ")
+        if (size(mesh_vertices, 1) /= 3) return
+        if (any(shape(mesh_vertices_bar) /= shape(mesh_vertices))) return
+        if (size(tetrahedra, 1) /= 4 .or. size(tetrahedra, 2) < 1) return
+        if (any(tetrahedra < 1) .or. &
+            any(tetrahedra > size(mesh_vertices, 2))) return
+        if (order < 1 .or. order > 6) return
+        code = ffh_problem_create(hip_context(), 3_c_int, &
+            int(size(mesh_vertices, 2), c_int), mesh_vertices, &
+            int(size(tetrahedra, 2), c_int), tetrahedra, prob)
+        if (code /= FFH_OK) return
+        code = ffh_pattern_build(prob, FFH_FAMILY_NEDELEC, int(order, c_int), &
+            n_global, nnz)
+        ok = code == FFH_OK
+        if (ok) ok = size(matrix_values_bar) == nnz
+        if (ok) then
+            code = ffh_assemble_vjp(prob, -1_c_int, FFH_COEFF_SCALAR, &
+                [curl_coefficient], mass_coefficient, matrix_values_bar, &
+                mesh_vertices_bar, abar, mass_coefficient_bar)
+            ok = code == FFH_OK
+        end if
+        code = ffh_problem_free(prob)
+        if (ok) then
+            curl_coefficient_bar = abar(1)
+            call status_set(status, 0, "")
+        else
+            mesh_vertices_bar = 0.0_dp
+            mass_coefficient_bar = 0.0_dp
+        end if
+    end subroutine assemble_tetra_nedelec_curl_mass_csc_vjp
 
     ! :1870-2000.  The device element takes one symmetric tensor per cell:
     ! the callback is sampled at each cell's centroid and four vertices and

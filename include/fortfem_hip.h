@@ -182,19 +182,24 @@ int ffh_global_dofs_download(ffh_problem *prob, int base, int *global_dofs);
 int ffh_assemble(ffh_problem *prob, int quadrature_degree, int coeff_kind,
                  const double *coeff, double mass_coefficient, int precision);
 /* assemble_tetra_lagrange_stiffness_csc_jvp (src/assembly/
- * assembly_tetra_lagrange_arbitrary_order_3d.f90:336-386), FFH_FAMILY_H1:
+ * assembly_tetra_lagrange_arbitrary_order_3d.f90:336-386) for
+ * FFH_FAMILY_H1 and assemble_tetra_nedelec_curl_mass_csc_jvp
+ * (assembly_tetra_nedelec_3d.f90:2114-2180) for FFH_FAMILY_NEDELEC
+ * (coeff = curl alpha, mass = beta; the quadrature degree is not used):
  * the tangent of the full matrix for tangents of the vertices
- * (vertices_dot(3, nv), NULL = 0), of the stiffness coefficient (coeff_dot:
- * 1 value for FFH_COEFF_SCALAR, one per cell for FFH_COEFF_CELL; NULL = 0)
- * and of the mass coefficient.  values_dot: nnz host values on the problem's
- * pattern (the primal's CSC order; ffh_problem_matrix + ffh_matrix_download
- * give the pattern).  coeff/mass as in ffh_assemble. */
+ * (vertices_dot(3, nv), NULL = 0), of the stiffness / curl coefficient
+ * (coeff_dot: 1 value for FFH_COEFF_SCALAR, one per cell for FFH_COEFF_CELL;
+ * NULL = 0) and of the mass coefficient.  values_dot: nnz host values on the
+ * problem's pattern (the primal's CSC order; ffh_problem_matrix +
+ * ffh_matrix_download give the pattern).  coeff/mass as in ffh_assemble. */
 int ffh_assemble_jvp(ffh_problem *prob, int quadrature_degree, int coeff_kind,
                      const double *coeff, double mass, const double *vertices_dot,
                      const double *coeff_dot, double mass_dot, double *values_dot);
-/* assemble_tetra_lagrange_stiffness_csc_vjp (:388-471), FFH_FAMILY_H1:
- * from the cotangent of the full matrix's values (nnz), vertices_bar(3, nv),
- * coeff_bar (1 value, or one per cell for FFH_COEFF_CELL) and mass_bar. */
+/* assemble_tetra_lagrange_stiffness_csc_vjp (:388-471) and
+ * assemble_tetra_nedelec_curl_mass_csc_vjp (assembly_tetra_nedelec_3d.f90:
+ * 2182-2287): from the cotangent of the full matrix's values (nnz),
+ * vertices_bar(3, nv), coeff_bar (1 value, or one per cell for
+ * FFH_COEFF_CELL) and mass_bar. */
 int ffh_assemble_vjp(ffh_problem *prob, int quadrature_degree, int coeff_kind,
                      const double *coeff, double mass, const double *values_bar,
                      double *vertices_bar, double *coeff_bar, double *mass_bar);

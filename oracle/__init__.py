@@ -613,6 +613,51 @@ def nedelec_csc(k, vertices, tets, alpha=1.0, beta=1.0, cell_tensor=None):
     return csc_from_triplet(nd.value, nd.value, rows, cols, vals)
 
 
+def nedelec_csc_jvp(k, vertices, tets, alpha=1.0, beta=1.0, vertices_dot=None, alpha_dot=0.0,
+                    beta_dot=0.0, cell_alpha=None, cell_alpha_dot=None):
+    """assemble_tetra_nedelec_curl_mass_csc_jvp (:2114-2180) -> matrix_dot CSC."""
+    v, t = _f(vertices), _i(tets)
+    vd = np.zeros_like(v) if vertices_dot is None else _f(vertices_dot)
+    n = nedelec_dof_count(k)
+    nt = t.shape[0] * n * n
+    rows = np.zeros(nt, dtype=np.int32)
+    cols = np.zeros(nt, dtype=np.int32)
+    vals = np.zeros(nt)
+    nd = C.c_int(0)
+    ca = None if cell_alpha is None else _f(cell_alpha)
+    cad = None if cell_alpha_dot is None else _f(cell_alpha_dot)
+    _check(_lib().ffo_nedelec_jvp_triplets(
+        C.c_int(k), C.c_int(v.shape[0]), v.ctypes.data_as(C.c_void_p), vd.ctypes.data_as(C.c_void_p),
+        C.c_int(t.shape[0]), t.ctypes.data_as(C.c_void_p), C.c_double(alpha), C.c_double(beta),
+        C.c_double(alpha_dot), C.c_double(beta_dot), None if ca is None else ca.ctypes.data_as(C.c_void_p),
+        None if cad is None else cad.ctypes.data_as(C.c_void_p), rows.ctypes.data_as(C.c_void_p),
+        cols.ctypes.data_as(C.c_void_p), vals.ctypes.data_as(C.c_void_p), C.byref(nd)), "nedelec_jvp")
+    return csc_from_triplet(nd.value, nd.value, rows, cols, vals)
+
+
+def nedelec_csc_vjp(k, vertices, tets, values_bar, alpha=1.0, beta=1.0, cell_alpha=None):
+    """assemble_tetra_nedelec_curl_mass_csc_vjp (:2182-2287) ->
+    (vertices_bar (nv, 3), alpha_bar, beta_bar, cell_alpha_bar)."""
+    a = nedelec_csc_jvp(k, vertices, tets)  # the primal pattern
+    vb_in = _f(values_bar)
+    if vb_in.shape[0] != a.nnz:
+        raise ValueError("values_bar has the wrong size")
+    v, t = _f(vertices), _i(tets)
+    vb = np.zeros_like(v)
+    ab = C.c_double(0)
+    bb = C.c_double(0)
+    cab = np.zeros(t.shape[0])
+    ca = None if cell_alpha is None else _f(cell_alpha)
+    _check(_lib().ffo_nedelec_vjp(
+        C.c_int(k), C.c_int(v.shape[0]), v.ctypes.data_as(C.c_void_p), C.c_int(t.shape[0]),
+        t.ctypes.data_as(C.c_void_p), C.c_double(alpha), C.c_double(beta),
+        None if ca is None else ca.ctypes.data_as(C.c_void_p), a.col_ptr.ctypes.data_as(C.c_void_p),
+        a.row_idx.ctypes.data_as(C.c_void_p), vb_in.ctypes.data_as(C.c_void_p),
+        vb.ctypes.data_as(C.c_void_p), C.byref(ab), cab.ctypes.data_as(C.c_void_p), C.byref(bb)),
+        "nedelec_vjp")
+    return vb, ab.value, bb.value, cab
+
+
 def nedelec_load_rule(k):
     """The load quadrature: tetra_degree_two_quadrature at k = 1
     (src/assembly/assembly_tetra_nedelec_3d.f90:2570-2581), the Duffy rule

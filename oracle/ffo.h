@@ -153,6 +153,15 @@ int ffo_nedelec_basis_transform(int k, const int edge_orient[6],
 int ffo_nedelec_element(int k, const double *coeff, const double v[12],
                         int qdeg, double alpha, const double *tensor,
                         double beta, double *m);
+int ffo_nedelec_jvp_triplets(int k, int nvert, const double *vertices,
+                             const double *vertices_dot, int ntet, const int *tets,
+                             double alpha, double beta, double alpha_dot, double beta_dot,
+                             const double *cell_alpha, const double *cell_alpha_dot,
+                             int *rows, int *cols, double *vals, int *ndofs);
+int ffo_nedelec_vjp(int k, int nvert, const double *vertices, int ntet, const int *tets,
+                    double alpha, double beta, const double *cell_alpha, const int *col_ptr,
+                    const int *row_idx, const double *values_bar, double *vertices_bar,
+                    double *alpha_bar, double *cell_alpha_bar, double *beta_bar);
 int ffo_nedelec_triplets(int k, int nvert, const double *vertices, int ntet,
                          const int *tets, double alpha, double beta,
                          const double *cell_tensor, int *rows, int *cols,

@@ -892,3 +892,333 @@ int ffo_nedelec_triplets(int k, int nvert, const double *vertices, int ntet, con
     free(gd), free(eo), free(fp), free(coeff), free(E), free(T), free(ET), free(O);
     return st;
 }
+
+/* ------------------------------------------------------ tangents (AD) */
+/* reference values and curls of the basis at the 2k + 2 Duffy points (the
+ * rule of the curl-mass tangents, :2160, 2266), shared by all cells */
+typedef struct {
+    int nq;
+    double *x, *w, *rv, *rc;
+} ned_rule;
+
+static void ned_rule_init(ned_rule *R, int k, const double *coeff)
+{
+    int n = ffo_nedelec_dof_count(k);
+    R->nq = ffo_tetra_duffy_count(2 * k + 2);
+    R->x = malloc(sizeof(double) * 4 * R->nq);
+    ffo_tetra_duffy(2 * k + 2, R->x, R->x + R->nq, R->x + 2 * R->nq, R->x + 3 * R->nq);
+    R->w = R->x + 3 * R->nq;
+    R->rv = malloc(sizeof(double) * 3 * n * R->nq);
+    R->rc = malloc(sizeof(double) * 3 * n * R->nq);
+    for (int q = 0; q < R->nq; ++q) {
+        double p[3] = {R->x[q], R->x[R->nq + q], R->x[2 * R->nq + q]};
+        double *rv = R->rv + (size_t)3 * n * q, *rc = R->rc + (size_t)3 * n * q;
+        if (k == 1) ffo_whitney(p, rv, rc);
+        else ffo_nedelec_eval(k, coeff, p, rv, rc);
+    }
+}
+
+static void ned_rule_free(ned_rule *R)
+{
+    free(R->x), free(R->rv), free(R->rc);
+}
+
+/* Element tangent of assemble_tetra_nedelec_curl_mass_element_jvp
+ * (assembly_tetra_nedelec_3d.f90:645-716).  map_tetra_nedelec_covariant_jvp
+ * (tetra_piola_maps.f90) is restated from the map's closed form:
+ * d(J^-T v^) = -J^-T dJ^T J^-T v^, d(J c^ / det) = dJ c^ / det - J c^ ddet / det^2,
+ * ddet = det tr(J^-1 dJ).  The element rule is the reference's 2k + 2. */
+static int element_jvp(int k, const ned_rule *R, const double v[12], const double vd[12],
+                       double alpha, double beta, double alpha_dot, double beta_dot,
+                       double *md)
+{
+    int n = ffo_nedelec_dof_count(k);
+    double jac[9], inv[9], det, jd[9];
+    if (geometry(v, jac, inv, &det)) return 1;
+    for (int c = 0; c < 3; ++c)
+        for (int r = 0; r < 3; ++r) jd[r + 3 * c] = vd[3 * (c + 1) + r] - vd[r];
+    double tr = 0.0;
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) tr += inv[i + 3 * j] * jd[j + 3 * i];
+    double det_dot = det * tr;
+    /* A = J^-T dJ^T J^-T, so values_dot = -A v^ */
+    double a1[9], A[9];
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) {
+            double s = 0.0;
+            for (int q = 0; q < 3; ++q) s += inv[q + 3 * i] * jd[j + 3 * q]; /* (J^-T dJ^T)_ij */
+            a1[i + 3 * j] = s;
+        }
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) {
+            double s = 0.0;
+            for (int q = 0; q < 3; ++q) s += a1[i + 3 * q] * inv[j + 3 * q]; /* (.. J^-T)_ij */
+            A[i + 3 * j] = s;
+        }
+    const double *qw = R->w;
+    double *pv = malloc(sizeof(double) * 3 * n), *pc = malloc(sizeof(double) * 3 * n);
+    double *pvd = malloc(sizeof(double) * 3 * n), *pcd = malloc(sizeof(double) * 3 * n);
+    for (int i = 0; i < n * n; ++i) md[i] = 0.0;
+    for (int q = 0; q < R->nq; ++q) {
+        const double *rv = R->rv + (size_t)3 * n * q, *rc = R->rc + (size_t)3 * n * q;
+        for (int j = 0; j < n; ++j)
+            for (int r = 0; r < 3; ++r) {
+                pv[3 * j + r] = inv[0 + 3 * r] * rv[3 * j] + inv[1 + 3 * r] * rv[3 * j + 1] +
+                                inv[2 + 3 * r] * rv[3 * j + 2];
+                pvd[3 * j + r] = -(A[r] * rv[3 * j] + A[r + 3] * rv[3 * j + 1] +
+                                   A[r + 6] * rv[3 * j + 2]);
+                double jc = jac[r] * rc[3 * j] + jac[r + 3] * rc[3 * j + 1] +
+                            jac[r + 6] * rc[3 * j + 2];
+                double jdc = jd[r] * rc[3 * j] + jd[r + 3] * rc[3 * j + 1] +
+                             jd[r + 6] * rc[3 * j + 2];
+                pc[3 * j + r] = jc / det;
+                pcd[3 * j + r] = jdc / det - jc * det_dot / (det * det);
+            }
+        for (int c = 0; c < n; ++c)
+            for (int r = 0; r < n; ++r) {
+                double ce = 0, me = 0, cx = 0, mx = 0;
+                for (int i = 0; i < 3; ++i) {
+                    ce += pc[3 * r + i] * pc[3 * c + i];
+                    me += pv[3 * r + i] * pv[3 * c + i];
+                    cx += pcd[3 * r + i] * pc[3 * c + i] + pc[3 * r + i] * pcd[3 * c + i];
+                    mx += pvd[3 * r + i] * pv[3 * c + i] + pv[3 * r + i] * pvd[3 * c + i];
+                }
+                md[r + (size_t)n * c] += qw[q] * (det_dot * (alpha * ce + beta * me) +
+                                                  det * (alpha_dot * ce + alpha * cx +
+                                                         beta_dot * me + beta * mx));
+            }
+    }
+    free(pv), free(pc), free(pvd), free(pcd);
+    return 0;
+}
+
+/* Element cotangent of assemble_tetra_nedelec_curl_mass_element_vjp
+ * (:718-805); map_tetra_nedelec_covariant_vjp restated from the map:
+ * Inv_bar(j, i) += vbar_i v^_j, J_bar(i, j) += cbar_i c^_j / det, and the
+ * map's 1/det gives J_bar -= (sum cbar . curl) J^-T; then inv3_vjp, det3_vjp
+ * and tetra_jacobian_vjp as in the H1 element. */
+static int element_vjp(int k, const ned_rule *R, const double v[12], double alpha,
+                       double beta, const double *mb, double vb[12], double *alpha_bar,
+                       double *beta_bar)
+{
+    int n = ffo_nedelec_dof_count(k);
+    double jac[9], inv[9], det;
+    if (geometry(v, jac, inv, &det)) return 1;
+    const double *qw = R->w;
+    double *pv = malloc(sizeof(double) * 3 * n), *pc = malloc(sizeof(double) * 3 * n);
+    double *vbar = malloc(sizeof(double) * 3 * n), *cbar = malloc(sizeof(double) * 3 * n);
+    double det_bar = 0, ab = 0, bb = 0, invb[9] = {0}, jb[9] = {0};
+    for (int q = 0; q < R->nq; ++q) {
+        const double *rv = R->rv + (size_t)3 * n * q, *rc = R->rc + (size_t)3 * n * q;
+        for (int j = 0; j < n; ++j)
+            for (int r = 0; r < 3; ++r) {
+                pv[3 * j + r] = inv[0 + 3 * r] * rv[3 * j] + inv[1 + 3 * r] * rv[3 * j + 1] +
+                                inv[2 + 3 * r] * rv[3 * j + 2];
+                pc[3 * j + r] = (jac[r] * rc[3 * j] + jac[r + 3] * rc[3 * j + 1] +
+                                 jac[r + 6] * rc[3 * j + 2]) / det;
+            }
+        for (int i = 0; i < 3 * n; ++i) vbar[i] = cbar[i] = 0.0;
+        for (int c = 0; c < n; ++c)
+            for (int r = 0; r < n; ++r) {
+                double seed = qw[q] * mb[r + (size_t)n * c];
+                double ce = 0, me = 0;
+                for (int i = 0; i < 3; ++i) {
+                    ce += pc[3 * r + i] * pc[3 * c + i];
+                    me += pv[3 * r + i] * pv[3 * c + i];
+                }
+                det_bar += seed * (alpha * ce + beta * me);
+                ab += seed * det * ce;
+                bb += seed * det * me;
+                for (int i = 0; i < 3; ++i) {
+                    cbar[3 * r + i] += seed * det * alpha * pc[3 * c + i];
+                    cbar[3 * c + i] += seed * det * alpha * pc[3 * r + i];
+                    vbar[3 * r + i] += seed * det * beta * pv[3 * c + i];
+                    vbar[3 * c + i] += seed * det * beta * pv[3 * r + i];
+                }
+            }
+        double cdot = 0.0;
+        for (int j = 0; j < n; ++j)
+            for (int i = 0; i < 3; ++i) {
+                cdot += cbar[3 * j + i] * pc[3 * j + i];
+                for (int m = 0; m < 3; ++m) {
+                    invb[m + 3 * i] += vbar[3 * j + i] * rv[3 * j + m];
+                    jb[i + 3 * m] += cbar[3 * j + i] * rc[3 * j + m] / det;
+                }
+            }
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j) jb[i + 3 * j] -= cdot * inv[j + 3 * i];
+    }
+    /* J_bar += -J^-T Inv_bar J^-T + det_bar det J^-T */
+    double w[9];
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) {
+            double s = 0.0;
+            for (int q = 0; q < 3; ++q) s += inv[q + 3 * i] * invb[q + 3 * j];
+            w[i + 3 * j] = s;
+        }
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) {
+            double s = 0.0;
+            for (int q = 0; q < 3; ++q) s += w[i + 3 * q] * inv[j + 3 * q];
+            jb[i + 3 * j] += -s + det_bar * det * inv[j + 3 * i];
+        }
+    for (int r = 0; r < 3; ++r) {
+        vb[r] = -(jb[r] + jb[r + 3] + jb[r + 6]);
+        for (int c = 0; c < 3; ++c) vb[3 * (c + 1) + r] = jb[r + 3 * c];
+    }
+    *alpha_bar = ab;
+    *beta_bar = bb;
+    free(pv), free(pc), free(vbar), free(cbar);
+    return 0;
+}
+
+/* oriented = T^T E T (k >= 2, T row-major) or o_r o_c E (k = 1) */
+static void orient(int k, int n, const int *eo, const int *fp, const double *E, double *O,
+                   double *T, double *ET, int transpose)
+{
+    if (k == 1) {
+        for (int c = 0; c < n; ++c)
+            for (int r = 0; r < n; ++r) O[r + n * c] = (double)(eo[r] * eo[c]) * E[r + n * c];
+        return;
+    }
+    ffo_nedelec_basis_transform(k, eo, fp, T);
+    /* transpose = 0: O = T^T E T; transpose = 1: O = T E T^T */
+    for (int c = 0; c < n; ++c)
+        for (int r = 0; r < n; ++r) {
+            double s = 0;
+            for (int j = 0; j < n; ++j) {
+                double t = transpose ? T[(size_t)c * n + j] : T[(size_t)j * n + c];
+                if (t != 0.0) s += E[r + (size_t)n * j] * t;
+            }
+            ET[r + (size_t)n * c] = s;
+        }
+    for (int c = 0; c < n; ++c)
+        for (int r = 0; r < n; ++r) {
+            double s = 0;
+            for (int j = 0; j < n; ++j) {
+                double t = transpose ? T[(size_t)r * n + j] : T[(size_t)j * n + r];
+                if (t != 0.0) s += t * ET[j + (size_t)n * c];
+            }
+            O[r + (size_t)n * c] = s;
+        }
+}
+
+static int ned_setup(int k, int ntet, const int *tets, int **gd, int **eo, int **fp, int *nd,
+                     double **coeff)
+{
+    int n = ffo_nedelec_dof_count(k);
+    *gd = malloc(sizeof(int) * (size_t)n * ntet);
+    *eo = malloc(sizeof(int) * 6 * (size_t)ntet);
+    *fp = malloc(sizeof(int) * 12 * (size_t)ntet);
+    *coeff = NULL;
+    if (ffo_nedelec_dof_map(k, ntet, tets, *gd, *eo, *fp, nd, NULL, NULL)) return 1;
+    if (k >= 2) {
+        *coeff = malloc(sizeof(double) * n * n);
+        if (ffo_nedelec_coefficients(k, *coeff)) return 1;
+    }
+    return 0;
+}
+
+/* assemble_tetra_nedelec_curl_mass_csc_jvp (:2114-2180): triplets of the
+ * oriented element tangents in the primal's order.  cell_alpha /
+ * cell_alpha_dot (optional): per-cell curl coefficient and its tangent. */
+int ffo_nedelec_jvp_triplets(int k, int nvert, const double *vertices,
+                             const double *vertices_dot, int ntet, const int *tets,
+                             double alpha, double beta, double alpha_dot, double beta_dot,
+                             const double *cell_alpha, const double *cell_alpha_dot,
+                             int *rows, int *cols, double *vals, int *ndofs)
+{
+    int n = ffo_nedelec_dof_count(k), *gd, *eo, *fp, st = 0;
+    double *coeff;
+    if (ned_setup(k, ntet, tets, &gd, &eo, &fp, ndofs, &coeff)) st = 1;
+    double *E = malloc(sizeof(double) * n * n), *T = malloc(sizeof(double) * n * n);
+    double *ET = malloc(sizeof(double) * n * n), *O = malloc(sizeof(double) * n * n);
+    ned_rule R = {0};
+    if (!st) ned_rule_init(&R, k, coeff);
+    size_t entry = 0;
+    for (int e = 0; e < ntet && !st; ++e) {
+        double v[12], vd[12];
+        for (int i = 0; i < 4 && !st; ++i) {
+            int id = tets[4 * e + i];
+            if (id < 1 || id > nvert) st = 1;
+            else
+                for (int c = 0; c < 3; ++c) {
+                    v[3 * i + c] = vertices[3 * (id - 1) + c];
+                    vd[3 * i + c] = vertices_dot[3 * (id - 1) + c];
+                }
+        }
+        if (st) break;
+        double a = cell_alpha ? cell_alpha[e] : alpha;
+        double ad = cell_alpha_dot ? cell_alpha_dot[e] : alpha_dot;
+        if (element_jvp(k, &R, v, vd, a, beta, ad, beta_dot, E)) {
+            st = 1;
+            break;
+        }
+        orient(k, n, eo + 6 * e, fp + 12 * e, E, O, T, ET, 0);
+        for (int c = 0; c < n; ++c)
+            for (int r = 0; r < n; ++r, ++entry) {
+                rows[entry] = gd[(size_t)n * e + r];
+                cols[entry] = gd[(size_t)n * e + c];
+                vals[entry] = O[r + (size_t)n * c];
+            }
+    }
+    if (R.x) ned_rule_free(&R);
+    free(gd), free(eo), free(fp), free(coeff), free(E), free(T), free(ET), free(O);
+    return st;
+}
+
+/* assemble_tetra_nedelec_curl_mass_csc_vjp (:2182-2287): per cell the
+ * oriented cotangent read from the primal CSC, element_bar = T O T^T (k >=
+ * 2) or o_r o_c O (k = 1), the element cotangent, vertex bars summed in tet
+ * order; cell_alpha_bar (optional) gets each cell's alpha bar. */
+int ffo_nedelec_vjp(int k, int nvert, const double *vertices, int ntet, const int *tets,
+                    double alpha, double beta, const double *cell_alpha, const int *col_ptr,
+                    const int *row_idx, const double *values_bar, double *vertices_bar,
+                    double *alpha_bar, double *cell_alpha_bar, double *beta_bar)
+{
+    int n = ffo_nedelec_dof_count(k), *gd, *eo, *fp, nd, st = 0;
+    double *coeff;
+    if (ned_setup(k, ntet, tets, &gd, &eo, &fp, &nd, &coeff)) st = 1;
+    double *Ob = malloc(sizeof(double) * n * n), *Eb = malloc(sizeof(double) * n * n);
+    double *T = malloc(sizeof(double) * n * n), *ET = malloc(sizeof(double) * n * n);
+    ned_rule R = {0};
+    if (!st) ned_rule_init(&R, k, coeff);
+    for (int i = 0; i < 3 * nvert; ++i) vertices_bar[i] = 0.0;
+    *alpha_bar = 0.0;
+    *beta_bar = 0.0;
+    for (int e = 0; e < ntet && !st; ++e) {
+        double v[12], vb[12], ab, bb;
+        for (int i = 0; i < 4 && !st; ++i) {
+            int id = tets[4 * e + i];
+            if (id < 1 || id > nvert) st = 1;
+            else
+                for (int c = 0; c < 3; ++c) v[3 * i + c] = vertices[3 * (id - 1) + c];
+        }
+        if (st) break;
+        const int *g = gd + (size_t)n * e;
+        for (int c = 0; c < n; ++c)
+            for (int r = 0; r < n; ++r) {
+                double b = 0.0;
+                for (int q = col_ptr[g[c] - 1] - 1; q < col_ptr[g[c]] - 1; ++q)
+                    if (row_idx[q] == g[r]) {
+                        b = values_bar[q];
+                        break;
+                    }
+                Ob[r + (size_t)n * c] = b;
+            }
+        orient(k, n, eo + 6 * e, fp + 12 * e, Ob, Eb, T, ET, 1);
+        double a = cell_alpha ? cell_alpha[e] : alpha;
+        if (element_vjp(k, &R, v, a, beta, Eb, vb, &ab, &bb)) {
+            st = 1;
+            break;
+        }
+        for (int i = 0; i < 4; ++i)
+            for (int c = 0; c < 3; ++c) vertices_bar[3 * (tets[4 * e + i] - 1) + c] += vb[3 * i + c];
+        *alpha_bar += ab;
+        *beta_bar += bb;
+        if (cell_alpha_bar) cell_alpha_bar[e] = ab;
+    }
+    if (R.x) ned_rule_free(&R);
+    free(gd), free(eo), free(fp), free(coeff), free(Ob), free(Eb), free(T), free(ET);
+    return st;
+}

@@ -6,7 +6,8 @@
 ! test/test_tetra_lagrange_poisson_p_convergence.f90:26-40,
 ! test/test_tetra_nedelec_sparse_assembly_slow.f90:31-59,
 ! test/test_sparse_matrix_ad.f90:19-65, test/test_pcg_solver_ad.f90:18-60,
-! test/test_tetra_lagrange_stiffness_csc_ad.f90:10-93.
+! test/test_tetra_lagrange_stiffness_csc_ad.f90:10-93,
+! test/test_tetra_nedelec_curl_mass_csc_ad.f90:10-96.
 module dropin_test_sources
     use fortfem_hip_types, only: dp
     implicit none
@@ -48,7 +49,8 @@ program test_fortfem_hip_dropin
         assemble_tetra_lagrange_stiffness_csc_jvp, &
         assemble_tetra_lagrange_stiffness_csc_vjp
     use fortfem_assembly_tetra_nedelec_3d, only: assemble_tetra_nedelec_curl_mass_csc, &
-        assemble_tetra_nedelec_vector_load
+        assemble_tetra_nedelec_vector_load, assemble_tetra_nedelec_curl_mass_csc_jvp, &
+        assemble_tetra_nedelec_curl_mass_csc_vjp
     use fortfem_sparse_matrix, only: spmv, spmv_jvp, spmv_vjp
     use fortfem_advanced_solvers, only: solver_options_t, solver_stats_t, &
         solver_options, solve_sparse, solve_sparse_jvp, solve_sparse_vjp
@@ -69,6 +71,7 @@ program test_fortfem_hip_dropin
     call check_spmv_ad()
     call check_pcg_ad()
     call check_h1_csc_ad()
+    call check_nedelec_csc_ad()
     call fortfem_hip_finalize()
     if (.not. all_passed) error stop 1
     write (*, '(a)') "drop-in Fortran checks passed"
@@ -380,6 +383,59 @@ contains
         call record(abs(lhs - rhs) < 3.0e-10_dp*max(1.0_dp, abs(lhs), abs(rhs)), &
             "global H1 products accumulate the shared-vertex adjoint")
     end subroutine check_h1_csc_ad
+
+    subroutine check_nedelec_csc_ad()
+        integer, parameter :: order = 2
+        integer, parameter :: tets(4, 2) = reshape([1, 2, 3, 4, 2, 3, 4, 5], [4, 2])
+        real(dp), parameter :: step = 2.0e-7_dp
+        real(dp) :: vertices(3, 5), vertices_dot(3, 5), vertices_bar(3, 5)
+        real(dp), allocatable :: values_bar(:)
+        type(csc_t) :: matrix, matrix_dot, minus, plus
+        type(fortsparse_status_t) :: st
+        real(dp) :: alpha, beta, alpha_dot, beta_dot, alpha_bar, beta_bar
+        real(dp) :: lhs, rhs, relative_error
+        integer :: entry
+
+        vertices = reshape([0.0_dp, 0.0_dp, 0.0_dp, 1.0_dp, 0.0_dp, 0.0_dp, &
+            0.0_dp, 1.0_dp, 0.0_dp, 0.0_dp, 0.0_dp, 1.0_dp, 1.0_dp, 1.0_dp, &
+            1.0_dp], [3, 5])
+        vertices_dot = reshape([0.01_dp, -0.02_dp, 0.015_dp, -0.03_dp, 0.01_dp, &
+            0.02_dp, 0.025_dp, -0.015_dp, 0.005_dp, -0.01_dp, 0.03_dp, -0.02_dp, &
+            0.02_dp, -0.01_dp, 0.025_dp], [3, 5])
+        alpha = 1.7_dp
+        beta = -0.8_dp
+        alpha_dot = 0.13_dp
+        beta_dot = -0.09_dp
+        call assemble_tetra_nedelec_curl_mass_csc(vertices, tets, matrix, st, &
+            alpha, beta, order)
+        call assemble_tetra_nedelec_curl_mass_csc_jvp(vertices, tets, order, &
+            alpha, beta, vertices_dot, alpha_dot, beta_dot, matrix_dot, st)
+        call record(st%code == 0, "Nedelec curl-mass JVP succeeds")
+        call assemble_tetra_nedelec_curl_mass_csc(vertices + step*vertices_dot, &
+            tets, plus, st, alpha + step*alpha_dot, beta + step*beta_dot, order)
+        call assemble_tetra_nedelec_curl_mass_csc(vertices - step*vertices_dot, &
+            tets, minus, st, alpha - step*alpha_dot, beta - step*beta_dot, order)
+        call record(matrix_dot%nnz == matrix%nnz .and. &
+            all(matrix_dot%col_ptr == matrix%col_ptr) .and. &
+            all(matrix_dot%row_idx == matrix%row_idx), &
+            "Global curl-mass JVP preserves the merged CSC pattern")
+        relative_error = maxval(abs(matrix_dot%val - &
+            (plus%val - minus%val)/(2.0_dp*step)))/ &
+            max(1.0_dp, maxval(abs(matrix_dot%val)))
+        call record(relative_error < 5.0e-8_dp, &
+            "Global curl-mass JVP matches complete merged-mesh reassembly")
+        allocate(values_bar(matrix%nnz))
+        do entry = 1, matrix%nnz
+            values_bar(entry) = 0.003_dp*entry - 0.00001_dp*entry**2
+        end do
+        call assemble_tetra_nedelec_curl_mass_csc_vjp(vertices, tets, order, &
+            alpha, beta, values_bar, vertices_bar, alpha_bar, beta_bar, st)
+        call record(st%code == 0, "Global curl-mass CSC VJP succeeds")
+        lhs = sum(values_bar*matrix_dot%val)
+        rhs = sum(vertices_bar*vertices_dot) + alpha_bar*alpha_dot + beta_bar*beta_dot
+        call record(abs(lhs - rhs) < 3.0e-10_dp*max(1.0_dp, abs(lhs), abs(rhs)), &
+            "Global curl-mass products accumulate the shared-vertex adjoint")
+    end subroutine check_nedelec_csc_ad
 
     subroutine check_linear_energy()
         real(dp) :: mesh_vertices(3, 5), coefficients(5), product(5)

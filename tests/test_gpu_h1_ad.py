@@ -142,7 +142,6 @@ def test_errors(ff):
         p.assemble_vjp(2, np.zeros(3))
     with pytest.raises(ValueError):
         p.assemble_jvp(2, kappa=1.0, cell_kappa_dot=np.zeros(t.shape[0]))
-    n = ff.Problem(v, t)
-    n.pattern_build(ff.capi.FAMILY_NEDELEC, 1)
+    n = ff.Problem(v, t)  # no pattern yet
     with pytest.raises(ff.FFHError):
         n.assemble_jvp(2, kappa=1.0)
