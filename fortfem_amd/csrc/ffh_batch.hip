@@ -155,8 +155,8 @@ __device__ __forceinline__ void item_spmv(const BatchSys &s, int slice0, const M
         VT v[U];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
-            c[u] = __builtin_nontemporal_load(ip + 64 * u);
-            v[u] = (VT)__builtin_nontemporal_load(vp + 64 * u);
+            c[u] = ldm(ip + 64 * u);
+            v[u] = (VT)ldm(vp + 64 * u);
         }
         Pack<VT, R> pc[U];
 #pragma unroll
@@ -169,8 +169,8 @@ __device__ __forceinline__ void item_spmv(const BatchSys &s, int slice0, const M
         vp += 64 * U;
     }
     for (; j < w; ++j) {
-        const int c = __builtin_nontemporal_load(ip);
-        const VT v = (VT)__builtin_nontemporal_load(vp);
+        const int c = ldm(ip);
+        const VT v = (VT)ldm(vp);
         const Pack<VT, R> pc = ld<VT, R>(p, c);
 #pragma unroll
         for (int r = 0; r < R; ++r) acc[r] += v * pc.v[r];
@@ -466,8 +466,8 @@ __device__ __forceinline__ void resid_item(const BatchSys &s, int slice0, const 
 #pragma unroll
                 for (int u = 0; u < U; ++u) {
                     const bool in = j + u < w;
-                    c[u] = in ? __builtin_nontemporal_load(ip + 64 * u) : 0;
-                    v[u] = in ? __builtin_nontemporal_load(vp + 64 * u) : 0.0;
+                    c[u] = in ? ldm(ip + 64 * u) : 0;
+                    v[u] = in ? ldm(vp + 64 * u) : 0.0;
                 }
                 Pack<double, R> xc[U];
 #pragma unroll

@@ -16,6 +16,16 @@
 
 namespace ffh {
 
+// Matrix load of the batched SpMV kernels (index and value arrays): an
+// ordinary load, so that a batch's matrices stay in the 256 MB memory-side
+// cache between iterations (the single-system kernels stream theirs with
+// non-temporal loads, which measured faster there).
+template <typename T>
+__device__ __forceinline__ T ldm(const T *p)
+{
+    return *p;
+}
+
 // ---------------------------------------------------------------- errors --
 struct Error {
     int code;
