@@ -150,7 +150,9 @@ __device__ __forceinline__ void item_spmv(const BatchSys &s, int slice0, const M
 #pragma unroll
     for (int r = 0; r < R; ++r) acc[r] = VT(0);
     int j = 0;
-    for (; j + U <= w; j += U) {
+    if (w >= U) {
+        // software pipeline: the next step's index/value loads are issued
+        // while this step's gathers are in flight
         int c[U];
         VT v[U];
 #pragma unroll
@@ -158,15 +160,34 @@ __device__ __forceinline__ void item_spmv(const BatchSys &s, int slice0, const M
             c[u] = ldm(ip + 64 * u);
             v[u] = (VT)ldm(vp + 64 * u);
         }
-        Pack<VT, R> pc[U];
+        for (; j + U <= w; j += U) {
+            Pack<VT, R> pc[U];
 #pragma unroll
-        for (int u = 0; u < U; ++u) pc[u] = ld<VT, R>(p, c[u]);
+            for (int u = 0; u < U; ++u) pc[u] = ld<VT, R>(p, c[u]);
+            const bool more = j + 2 * U <= w; // uniform: one slice per wave
+            int cn[U];
+            VT vn[U];
+            if (more) {
 #pragma unroll
-        for (int u = 0; u < U; ++u)
+                for (int u = 0; u < U; ++u) {
+                    cn[u] = ldm(ip + 64 * (U + u));
+                    vn[u] = (VT)ldm(vp + 64 * (U + u));
+                }
+            }
 #pragma unroll
-            for (int r = 0; r < R; ++r) acc[r] += v[u] * pc[u].v[r];
-        ip += 64 * U;
-        vp += 64 * U;
+            for (int u = 0; u < U; ++u)
+#pragma unroll
+                for (int r = 0; r < R; ++r) acc[r] += v[u] * pc[u].v[r];
+            ip += 64 * U;
+            vp += 64 * U;
+            if (more) {
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    c[u] = cn[u];
+                    v[u] = vn[u];
+                }
+            }
+        }
     }
     for (; j < w; ++j) {
         const int c = ldm(ip);
@@ -235,12 +256,25 @@ __device__ __forceinline__ void system_totals(const BatchSys &s, const double *c
     for (int q = 0; q < NQ; ++q)
 #pragma unroll
         for (int r = 0; r < kMaxRhs; ++r) acc[q][r] = 0.0;
-    for (int w = s.w0 + threadIdx.x; w < s.w1; w += 256)
+    // four items per thread and round, their loads issued together
+    for (int w = s.w0 + threadIdx.x; w < s.w1; w += 4 * 256) {
+        double v[4][NQ][kMaxRhs];
 #pragma unroll
-        for (int q = 0; q < NQ; ++q)
+        for (int k = 0; k < 4; ++k)
 #pragma unroll
-            for (int r = 0; r < kMaxRhs; ++r)
-                if (r < s.R) acc[q][r] += part[q][(size_t)w * kMaxRhs + r];
+            for (int q = 0; q < NQ; ++q)
+#pragma unroll
+                for (int r = 0; r < kMaxRhs; ++r) {
+                    const int wk = w + k * 256;
+                    v[k][q][r] = (r < s.R && wk < s.w1) ? part[q][(size_t)wk * kMaxRhs + r] : 0.0;
+                }
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+#pragma unroll
+            for (int q = 0; q < NQ; ++q)
+#pragma unroll
+                for (int r = 0; r < kMaxRhs; ++r) acc[q][r] += v[k][q][r];
+    }
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
 #pragma unroll
     for (int q = 0; q < NQ; ++q)
@@ -642,55 +676,24 @@ __global__ void k_b_interleave(int n, int nrhs, int R, const double *__restrict_
 }
 
 // ----------------------------------------------------------------- engine --
-// Device state of one batch (the server's, or one mixed-precision solve's):
-// the systems, the work table, the per-(slot, right-hand side) scalars with
-// their host mirror, and the per-item partial sums.
-struct BatchEngine {
+// One group of the batch: its systems, work table and per-item partial sums,
+// advanced on a stream of its own.
+struct BatchGroup {
     DBuf<BatchSys> d_sys;
     DBuf<int2> d_work;
-    DBuf<BatchScal> d_scal;
     DBuf<double> d_p0, d_p1, d_p2, d_p3;
-    std::vector<BatchScal> h_scal;
-    int nwork = 0, nsys = 0;
-    // a chunk recorded as a HIP graph (replayed while the batch is unchanged)
-    hipGraphExec_t graph = nullptr;
-    int graph_chunk = -1;
-
-    ~BatchEngine() { drop_graph(); }
-
-    void drop_graph()
-    {
-        if (graph) (void)hipGraphExecDestroy(graph);
-        graph = nullptr;
-        graph_chunk = -1;
-    }
-
-    void init(int slots)
-    {
-        h_scal.assign((size_t)slots * kMaxRhs, BatchScal{});
-        d_scal.alloc(h_scal.size());
-    }
-
-    // append a system's work items (kRowsPerItem rows each) and the system
-    static void append(BatchSys b, std::vector<BatchSys> &sys, std::vector<int2> &work)
-    {
-        b.w0 = (int)work.size();
-        for (int row = 0; row < b.n; row += kRowsPerItem)
-            work.push_back(make_int2((int)sys.size(), row));
-        b.w1 = (int)work.size();
-        sys.push_back(b);
-    }
-
     std::vector<BatchSys> h_sys; // what d_sys / d_work hold
     std::vector<int2> h_work;
+    int nwork = 0, nsys = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t done = nullptr;
 
-    void set(Context *ctx, const std::vector<BatchSys> &sys, const std::vector<int2> &work)
+    void set(hipStream_t s, const std::vector<BatchSys> &sys, const std::vector<int2> &work)
     {
-        hipStream_t s = ctx->stream;
         if (d_sys.p && sys.size() == h_sys.size() && work.size() == h_work.size() &&
             std::memcmp(sys.data(), h_sys.data(), sizeof(BatchSys) * sys.size()) == 0 &&
             std::memcmp(work.data(), h_work.data(), sizeof(int2) * work.size()) == 0)
-            return; // unchanged: the recorded graph stays valid
+            return;
         h_sys = sys;
         h_work = work;
         d_sys.upload(sys, s);
@@ -704,59 +707,109 @@ struct BatchEngine {
         }
         nwork = (int)work.size();
         nsys = (int)sys.size();
-        drop_graph();
-        FFH_HIP(hipStreamSynchronize(s));
     }
 
-    // run() from a HIP graph: recorded on first use after set() (or a new
-    // chunk length), then replayed; profiling runs launch directly
-    void run_graph(Context *ctx, int chunk)
+    // enqueue one PCG iteration of the group's systems
+    void iteration(BatchScal *scal) const
     {
-        if (ctx->profiling) {
-            run(ctx, chunk);
-            return;
+        const int W = nwork, NS = nsys;
+        hipStream_t s = stream;
+        hipLaunchKernelGGL(k_b_spmv, dim3(W), dim3(256), 0, s, d_work.p, d_sys.p, scal, d_p0.p);
+        hipLaunchKernelGGL(k_b_alpha, dim3(NS), dim3(256), 0, s, d_sys.p, scal, d_p0.p);
+        hipLaunchKernelGGL(k_b_update, dim3(W), dim3(256), 0, s, d_work.p, d_sys.p, scal, d_p0.p,
+                           d_p1.p);
+        hipLaunchKernelGGL(k_b_beta, dim3(NS), dim3(256), 0, s, d_sys.p, scal, d_p0.p, d_p1.p);
+        hipLaunchKernelGGL(k_b_dir, dim3(W), dim3(256), 0, s, d_work.p, d_sys.p, scal, 0);
+    }
+
+    // enqueue the FP64 residual refresh (which also starts fresh systems)
+    void refresh(BatchScal *scal) const
+    {
+        const int W = nwork, NS = nsys;
+        hipStream_t s = stream;
+        hipLaunchKernelGGL(k_b_resid, dim3(W), dim3(256), 0, s, d_work.p, d_sys.p, scal, d_p0.p,
+                           d_p1.p, d_p2.p, d_p3.p);
+        hipLaunchKernelGGL(k_b_refresh, dim3(NS), dim3(256), 0, s, d_sys.p, scal, d_p0.p, d_p1.p,
+                           d_p2.p, d_p3.p);
+        hipLaunchKernelGGL(k_b_dir, dim3(W), dim3(256), 0, s, d_work.p, d_sys.p, scal, 1);
+    }
+};
+
+// Device state of the server's batch: the per-(slot, right-hand side)
+// scalars with their host mirror, and kGroups groups of systems.  Each
+// iteration is a chain of five launches, most of them short and latency
+// bound; the groups run their chains on separate streams (default priority)
+// so that one group's small launches overlap another's products.  A
+// system's arithmetic does not depend on its group.
+struct BatchEngine {
+    static constexpr int kGroups = 2;
+    BatchGroup grp[kGroups];
+    DBuf<BatchScal> d_scal;
+    std::vector<BatchScal> h_scal;
+    hipEvent_t start = nullptr;
+    int nwork = 0;
+
+    ~BatchEngine()
+    {
+        for (BatchGroup &g : grp) {
+            if (g.done) (void)hipEventDestroy(g.done);
+            if (g.stream) (void)hipStreamDestroy(g.stream);
         }
-        if (!graph || graph_chunk != chunk) {
-            drop_graph();
-            hipGraph_t g;
-            FFH_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-            run(ctx, chunk);
-            FFH_HIP(hipStreamEndCapture(ctx->stream, &g));
-            const hipError_t e = hipGraphInstantiate(&graph, g, nullptr, nullptr, 0);
-            (void)hipGraphDestroy(g);
-            FFH_HIP(e);
-            graph_chunk = chunk;
+        if (start) (void)hipEventDestroy(start);
+    }
+
+    void init(int slots)
+    {
+        h_scal.assign((size_t)slots * kMaxRhs, BatchScal{});
+        d_scal.alloc(h_scal.size());
+        for (BatchGroup &g : grp) {
+            FFH_HIP(hipStreamCreateWithFlags(&g.stream, hipStreamNonBlocking));
+            FFH_HIP(hipEventCreateWithFlags(&g.done, hipEventDisableTiming));
         }
-        FFH_HIP(hipGraphLaunch(graph, ctx->stream));
+        FFH_HIP(hipEventCreateWithFlags(&start, hipEventDisableTiming));
+    }
+
+    // append a system's work items (kRowsPerItem rows each) and the system
+    static void append(BatchSys b, std::vector<BatchSys> &sys, std::vector<int2> &work)
+    {
+        b.w0 = (int)work.size();
+        for (int row = 0; row < b.n; row += kRowsPerItem)
+            work.push_back(make_int2((int)sys.size(), row));
+        b.w1 = (int)work.size();
+        sys.push_back(b);
+    }
+
+    void set(Context *ctx, const std::vector<BatchSys> (&sys)[kGroups],
+             const std::vector<int2> (&work)[kGroups])
+    {
+        nwork = 0;
+        for (int g = 0; g < kGroups; ++g) {
+            grp[g].set(ctx->stream, sys[g], work[g]);
+            nwork += grp[g].nwork;
+        }
+        FFH_HIP(hipStreamSynchronize(ctx->stream));
     }
 
     // `chunk` PCG iterations of every system with an active right-hand side,
-    // then the FP64 residual refresh (which also starts fresh systems)
+    // then the FP64 residual refresh; ordered after and before the work on
+    // the context's stream
     void run(Context *ctx, int chunk)
     {
-        hipStream_t s = ctx->stream;
-        const int W = nwork, NS = nsys;
-        if (W == 0 || NS == 0) return;
-        for (int it = 0; it < chunk; ++it) {
-            if (ctx->profiling) ctx->prof_mark(1, true);
-            hipLaunchKernelGGL(k_b_spmv, dim3(W), dim3(256), 0, s, d_work.p, d_sys.p, d_scal.p,
-                               d_p0.p);
-            if (ctx->profiling) ctx->prof_mark(1, false);
-            hipLaunchKernelGGL(k_b_alpha, dim3(NS), dim3(256), 0, s, d_sys.p, d_scal.p, d_p0.p);
-            hipLaunchKernelGGL(k_b_update, dim3(W), dim3(256), 0, s, d_work.p, d_sys.p, d_scal.p,
-                               d_p0.p, d_p1.p);
-            hipLaunchKernelGGL(k_b_beta, dim3(NS), dim3(256), 0, s, d_sys.p, d_scal.p, d_p0.p,
-                               d_p1.p);
-            hipLaunchKernelGGL(k_b_dir, dim3(W), dim3(256), 0, s, d_work.p, d_sys.p, d_scal.p, 0);
-        }
-        if (ctx->profiling) ctx->prof_mark(0, true);
-        hipLaunchKernelGGL(k_b_resid, dim3(W), dim3(256), 0, s, d_work.p, d_sys.p, d_scal.p,
-                           d_p0.p, d_p1.p, d_p2.p, d_p3.p);
-        if (ctx->profiling) ctx->prof_mark(0, false);
-        hipLaunchKernelGGL(k_b_refresh, dim3(NS), dim3(256), 0, s, d_sys.p, d_scal.p, d_p0.p,
-                           d_p1.p, d_p2.p, d_p3.p);
-        hipLaunchKernelGGL(k_b_dir, dim3(W), dim3(256), 0, s, d_work.p, d_sys.p, d_scal.p, 1);
+        if (nwork == 0) return;
+        FFH_HIP(hipEventRecord(start, ctx->stream));
+        for (BatchGroup &g : grp)
+            if (g.nwork) FFH_HIP(hipStreamWaitEvent(g.stream, start, 0));
+        for (int it = 0; it < chunk; ++it)
+            for (const BatchGroup &g : grp)
+                if (g.nwork) g.iteration(d_scal.p);
+        for (const BatchGroup &g : grp)
+            if (g.nwork) g.refresh(d_scal.p);
         FFH_HIP(hipGetLastError());
+        for (BatchGroup &g : grp)
+            if (g.nwork) {
+                FFH_HIP(hipEventRecord(g.done, g.stream));
+                FFH_HIP(hipStreamWaitEvent(ctx->stream, g.done, 0));
+            }
     }
 
     void pull(Context *ctx)
@@ -930,11 +983,21 @@ static void admit(ffh_server *S, ServedRequest *q)
     S->changed = true;
 }
 
+// Systems go to the group with the least work so far (largest first), the
+// work of a system being its SELL entries times (1 + R / 4).
 static void rebuild_batch(ffh_server *S)
 {
-    std::vector<BatchSys> sys;
-    std::vector<int2> work;
-    for (ServedRequest *q : S->active) {
+    constexpr int G = BatchEngine::kGroups;
+    std::vector<BatchSys> sys[G];
+    std::vector<int2> work[G];
+    double load[G] = {};
+    std::vector<ServedRequest *> order(S->active);
+    auto cost = [](const ServedRequest *q) {
+        return (double)q->sym->reduced.m.sell_total * (1.0 + q->R / 4.0);
+    };
+    std::stable_sort(order.begin(), order.end(),
+                     [&](const ServedRequest *a, const ServedRequest *b) { return cost(a) > cost(b); });
+    for (ServedRequest *q : order) {
         const DevMatrix &A = q->sym->reduced.m;
         BatchSys b{};
         b.off = A.sell_off.p;
@@ -958,7 +1021,9 @@ static void rebuild_batch(ffh_server *S)
         b.R = q->R;
         b.slot = q->slot;
         b.x_zero = 1;
-        BatchEngine::append(b, sys, work);
+        const int g = (int)(std::min_element(load, load + G) - load);
+        load[g] += cost(q);
+        BatchEngine::append(b, sys[g], work[g]);
     }
     S->eng.set(S->ctx, sys, work);
     S->changed = false;
