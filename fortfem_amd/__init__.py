@@ -832,16 +832,7 @@ def assemble_tetra_nedelec_weighted_csc(mesh_vertices, tetrahedra, coefficient, 
     t = np.asarray(tetrahedra)
     if k < 1 or v.ndim != 2 or v.shape[1] != 3 or t.ndim != 2 or t.shape[1] != 4 or t.shape[0] < 1:
         return None, status
-    if callable(coefficient):
-        pts = v[t - 1]  # (T, 4, 3)
-        cen = pts.mean(axis=1)
-        K = np.array([np.asarray(coefficient(*c), dtype=np.float64) for c in cen])
-        for j in range(4):
-            Kj = np.array([np.asarray(coefficient(*c), dtype=np.float64) for c in pts[:, j]])
-            if np.max(np.abs(Kj - K)) > 1e-14 * max(1.0, np.max(np.abs(K))):
-                raise NotImplementedError("coefficient tensors must be constant on each cell")
-    else:
-        K = np.asarray(coefficient, dtype=np.float64).reshape(t.shape[0], 3, 3)
+    K = _cell_tensors(v, t, coefficient)
     try:
         p = Problem(v, t, ctx)
         p.pattern_build(capi.FAMILY_NEDELEC, k)
@@ -925,6 +916,12 @@ def assemble_tetra_nedelec_vector_load(mesh_vertices, tetrahedra, source, order=
     """assemble_tetra_nedelec_vector_load (src/assembly/assembly_tetra_nedelec_3d.f90:
     2002-2040) / _order (:71-127) with a vectorised source(x, y, z) returning
     three arrays; the callback is sampled at the mapped load points."""
+    samples = _nedelec_source_samples(mesh_vertices, tetrahedra, source, order)
+    return assemble_tetra_nedelec_vector_load_samples(mesh_vertices, tetrahedra, samples, order, ctx)
+
+
+def _nedelec_source_samples(mesh_vertices, tetrahedra, source, order):
+    """source(x, y, z) -> (fx, fy, fz) at the mapped load points, (T, nq, 3)."""
     pts = nedelec_load_points(order)
     v = np.asarray(mesh_vertices, dtype=np.float64)
     t = np.asarray(tetrahedra) - 1
@@ -932,9 +929,85 @@ def assemble_tetra_nedelec_vector_load(mesh_vertices, tetrahedra, source, order=
     J = np.stack([v[t[:, 1]] - v0, v[t[:, 2]] - v0, v[t[:, 3]] - v0], axis=2)
     X = v0[:, None, :] + np.einsum("tij,jq->tqi", J, pts[:, :3].T)
     fx, fy, fz = source(X[..., 0], X[..., 1], X[..., 2])
-    samples = np.stack([np.broadcast_to(fx, X.shape[:2]), np.broadcast_to(fy, X.shape[:2]),
-                        np.broadcast_to(fz, X.shape[:2])], axis=-1)
-    return assemble_tetra_nedelec_vector_load_samples(mesh_vertices, tetrahedra, samples, order, ctx)
+    return np.stack([np.broadcast_to(fx, X.shape[:2]), np.broadcast_to(fy, X.shape[:2]),
+                     np.broadcast_to(fz, X.shape[:2])], axis=-1)
+
+
+def _cell_tensors(mesh_vertices, tetrahedra, coefficient):
+    """Per-cell tensors (T, 3, 3) from an array or from a callable K(x, y, z)
+    that must be constant on every cell (checked at the vertices and the
+    centroid)."""
+    v = np.asarray(mesh_vertices, dtype=np.float64)
+    t = np.asarray(tetrahedra)
+    if not callable(coefficient):
+        return np.asarray(coefficient, dtype=np.float64).reshape(t.shape[0], 3, 3)
+    pts = v[t - 1]
+    K = np.array([np.asarray(coefficient(*c), dtype=np.float64) for c in pts.mean(axis=1)])
+    for j in range(4):
+        Kj = np.array([np.asarray(coefficient(*c), dtype=np.float64) for c in pts[:, j]])
+        if np.max(np.abs(Kj - K)) > 1e-14 * max(1.0, np.max(np.abs(K))):
+            raise NotImplementedError("coefficient tensors must be constant on each cell")
+    return K
+
+
+def _nedelec_solve(vertices, tetrahedra, order, source, assemble, zero_tangential_boundary, opts, ctx,
+                   message):
+    status = FortsparseStatus(FORTSPARSE_INVALID_MATRIX, message)
+    t = np.asarray(tetrahedra)
+    v = np.asarray(vertices)
+    if order < 1 or v.ndim != 2 or v.shape[1] != 3 or t.ndim != 2 or t.shape[1] != 4 or t.shape[0] < 1:
+        return None, status, None
+    p = Problem(vertices, tetrahedra, ctx)
+    try:
+        p.pattern_build(capi.FAMILY_NEDELEC, order)
+        assemble(p)
+        p.load(-1, capi.SOURCE_SAMPLES,
+               _f64(_nedelec_source_samples(vertices, tetrahedra, source, order)).reshape(-1))
+        if zero_tangential_boundary:
+            p.dirichlet(0)
+        o = opts or solver_options(method="pcg", preconditioner="jacobi", tolerance=1e-12,
+                                   max_iterations=100000, breakdown="relative")
+        x, st = p.solve(o)
+    except FFHError as e:
+        if e.code == capi.FFH_ERR_DEVICE:
+            raise
+        return None, status, None
+    return x, FortsparseStatus(0, ""), st
+
+
+def solve_tetra_nedelec_curl_mass(vertices, tetrahedra, order, source, curl_coefficient, mass_coefficient,
+                                  zero_tangential_boundary=False, opts: SolverOptionsT | None = None,
+                                  ctx: Context | None = None, cell_coefficient=None):
+    """solve_tetra_nedelec_curl_mass (fortfem_tetra_nedelec_solver_3d.f90:361-414)
+    with the sparse direct solve replaced by Jacobi-PCG (relative 1e-12 unless
+    opts says otherwise).  source(x, y, z) -> (fx, fy, fz), vectorised;
+    zero_tangential_boundary eliminates the boundary edge and face dofs
+    (build_zero_tangential_mask, :416-470) with zero values.
+    cell_coefficient: optional per-cell curl coefficient (new argument).
+    -> (solution, status, stats)."""
+    if mass_coefficient <= 0.0:
+        return None, FortsparseStatus(FORTSPARSE_INVALID_MATRIX, "Tetrahedral Nedelec curl-mass solve failed"), None
+    return _nedelec_solve(
+        vertices, tetrahedra, order, source,
+        lambda p: p.assemble(-1, kappa=curl_coefficient, mass=mass_coefficient, cell_kappa=cell_coefficient),
+        zero_tangential_boundary, opts, ctx, "Tetrahedral Nedelec curl-mass solve failed")
+
+
+def solve_tetra_nedelec_weighted_curl_mass(vertices, tetrahedra, order, curl_coefficient, source,
+                                           mass_coefficient, zero_tangential_boundary=False,
+                                           opts: SolverOptionsT | None = None, ctx: Context | None = None):
+    """solve_tetra_nedelec_weighted_curl_mass (fortfem_tetra_nedelec_solver_3d.f90:41-95)
+    with Jacobi-PCG; curl_coefficient is a (T, 3, 3) array of symmetric
+    per-cell tensors or a callable K(x, y, z) constant on every cell.
+    -> (solution, status, stats)."""
+    if mass_coefficient <= 0.0:
+        return None, FortsparseStatus(FORTSPARSE_INVALID_MATRIX,
+                                      "Weighted tetrahedral Nedelec curl-mass solve failed"), None
+    K = _cell_tensors(vertices, tetrahedra, curl_coefficient)
+    return _nedelec_solve(
+        vertices, tetrahedra, order, source,
+        lambda p: p.assemble(-1, mass=float(mass_coefficient), cell_tensor=K),
+        zero_tangential_boundary, opts, ctx, "Weighted tetrahedral Nedelec curl-mass solve failed")
 
 
 def solve_tetra_lagrange_poisson(vertices, tetrahedra, degree, source, boundary_value=None,

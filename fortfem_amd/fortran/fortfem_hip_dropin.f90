@@ -498,6 +498,7 @@ module fortfem_assembly_tetra_nedelec_3d
     public :: assemble_tetra_nedelec_vector_load_order
     public :: tensor_coefficient_3d
     public :: vector_source_3d
+    public :: sample_vector_source, sample_cell_tensors
 
     abstract interface
         pure subroutine tensor_coefficient_3d(x, y, z, value)
@@ -537,16 +538,46 @@ contains
         real(dp), allocatable, intent(out) :: vector(:)
         type(fortsparse_status_t), intent(out) :: status
 
-        real(c_double), allocatable, target :: pts(:, :)
         real(c_double), allocatable :: samples(:, :, :)
-        real(dp) :: jac(3, 3), point(3)
         type(c_ptr) :: prob
-        integer(c_int) :: nq, n_global, code
+        integer(c_int) :: n_global, code
         integer(c_int64_t) :: nnz
-        integer :: cell, q
 
         call status_set(status, FORTSPARSE_INVALID_MATRIX, &
             "Tetrahedral Nedelec vector load assembly failed")
+        if (.not. sample_vector_source(mesh_vertices, tetrahedra, order, source, &
+            samples)) return
+        code = ffh_problem_create(hip_context(), 3_c_int, &
+            int(size(mesh_vertices, 2), c_int), mesh_vertices, &
+            int(size(tetrahedra, 2), c_int), tetrahedra, prob)
+        if (code /= FFH_OK) return
+        code = ffh_pattern_build(prob, FFH_FAMILY_NEDELEC, int(order, c_int), &
+            n_global, nnz)
+        if (code == FFH_OK) code = ffh_load(prob, -1_c_int, FFH_SOURCE_SAMPLES, &
+            samples, int(size(samples), c_int64_t))
+        if (code == FFH_OK) then
+            allocate(vector(n_global))
+            code = ffh_rhs_download(prob, FFH_FULL, vector)
+        end if
+        if (code == FFH_OK) call status_set(status, 0, "")
+        code = ffh_problem_free(prob)
+    end subroutine assemble_tetra_nedelec_vector_load_order
+
+    ! The source at the mapped load points of every cell, samples(3, nq, T)
+    ! (the device load's sample layout); false on an invalid mesh or order.
+    logical function sample_vector_source(mesh_vertices, tetrahedra, order, &
+            source, samples) result(ok)
+        real(dp), intent(in) :: mesh_vertices(:, :)
+        integer, intent(in) :: tetrahedra(:, :), order
+        procedure(vector_source_3d) :: source
+        real(c_double), allocatable, intent(out) :: samples(:, :, :)
+
+        real(c_double), allocatable, target :: pts(:, :)
+        real(dp) :: jac(3, 3), point(3)
+        integer(c_int) :: nq
+        integer :: cell, q
+
+        ok = .false.
         if (size(mesh_vertices, 1) /= 3 .or. size(tetrahedra, 1) /= 4) return
         if (size(tetrahedra, 2) < 1) return
         if (any(tetrahedra < 1) .or. &
@@ -569,21 +600,39 @@ contains
                 call source(point(1), point(2), point(3), samples(:, q, cell))
             end do
         end do
-        code = ffh_problem_create(hip_context(), 3_c_int, &
-            int(size(mesh_vertices, 2), c_int), mesh_vertices, &
-            int(size(tetrahedra, 2), c_int), tetrahedra, prob)
-        if (code /= FFH_OK) return
-        code = ffh_pattern_build(prob, FFH_FAMILY_NEDELEC, int(order, c_int), &
-            n_global, nnz)
-        if (code == FFH_OK) code = ffh_load(prob, -1_c_int, FFH_SOURCE_SAMPLES, &
-            samples, int(size(samples), c_int64_t))
-        if (code == FFH_OK) then
-            allocate(vector(n_global))
-            code = ffh_rhs_download(prob, FFH_FULL, vector)
-        end if
-        if (code == FFH_OK) call status_set(status, 0, "")
-        code = ffh_problem_free(prob)
-    end subroutine assemble_tetra_nedelec_vector_load_order
+        ok = .true.
+    end function sample_vector_source
+
+    ! One tensor per cell from the callback, which must return the same
+    ! tensor at each cell's centroid and four vertices (the device element
+    ! takes cell-constant tensors); false otherwise.
+    logical function sample_cell_tensors(mesh_vertices, tetrahedra, coefficient, &
+            tensors) result(ok)
+        real(dp), intent(in) :: mesh_vertices(:, :)
+        integer, intent(in) :: tetrahedra(:, :)
+        procedure(tensor_coefficient_3d) :: coefficient
+        real(dp), allocatable, intent(out) :: tensors(:, :, :)
+        real(dp) :: centre(3), kv(3, 3), scale
+        integer :: cell, corner
+
+        ok = .false.
+        if (size(mesh_vertices, 1) /= 3 .or. size(tetrahedra, 1) /= 4) return
+        if (any(tetrahedra < 1) .or. &
+            any(tetrahedra > size(mesh_vertices, 2))) return
+        allocate(tensors(3, 3, size(tetrahedra, 2)))
+        do cell = 1, size(tetrahedra, 2)
+            centre = 0.25_dp*sum(mesh_vertices(:, tetrahedra(:, cell)), dim=2)
+            call coefficient(centre(1), centre(2), centre(3), tensors(:, :, cell))
+            scale = max(1.0_dp, maxval(abs(tensors(:, :, cell))))
+            do corner = 1, 4
+                associate (x => mesh_vertices(:, tetrahedra(corner, cell)))
+                    call coefficient(x(1), x(2), x(3), kv)
+                end associate
+                if (maxval(abs(kv - tensors(:, :, cell))) > 1.0e-14_dp*scale) return
+            end do
+        end do
+        ok = .true.
+    end function sample_cell_tensors
 
     ! Shared driver: pattern + assembly of one Nedelec problem, downloaded.
     subroutine nedelec_csc(mesh_vertices, tetrahedra, k, kind, coeff, mass, &
@@ -767,30 +816,11 @@ contains
         type(fortsparse_status_t), intent(out) :: status
         integer, intent(in), optional :: order
         real(dp), allocatable :: tensors(:, :, :)
-        real(dp) :: centre(3), kv(3, 3), scale
-        integer :: cell, corner
 
         call status_set(status, FORTSPARSE_INVALID_MATRIX, &
-            "Tetrahedral weighted Nedelec assembly failed")
-        if (size(mesh_vertices, 1) /= 3 .or. size(tetrahedra, 1) /= 4) return
-        if (any(tetrahedra < 1) .or. &
-            any(tetrahedra > size(mesh_vertices, 2))) return
-        allocate(tensors(3, 3, size(tetrahedra, 2)))
-        do cell = 1, size(tetrahedra, 2)
-            centre = 0.25_dp*sum(mesh_vertices(:, tetrahedra(:, cell)), dim=2)
-            call coefficient(centre(1), centre(2), centre(3), tensors(:, :, cell))
-            scale = max(1.0_dp, maxval(abs(tensors(:, :, cell))))
-            do corner = 1, 4
-                associate (x => mesh_vertices(:, tetrahedra(corner, cell)))
-                    call coefficient(x(1), x(2), x(3), kv)
-                end associate
-                if (maxval(abs(kv - tensors(:, :, cell))) > 1.0e-14_dp*scale) then
-                    call status_set(status, FORTSPARSE_INVALID_MATRIX, &
-                        "weighted Nedelec assembly needs a tensor constant on each cell")
-                    return
-                end if
-            end do
-        end do
+            "weighted Nedelec assembly needs a tensor constant on each cell")
+        if (.not. sample_cell_tensors(mesh_vertices, tetrahedra, coefficient, &
+            tensors)) return
         call assemble_tetra_nedelec_weighted_csc_cells(mesh_vertices, &
             tetrahedra, tensors, mass_coefficient, matrix, status, order)
     end subroutine assemble_tetra_nedelec_weighted_csc
@@ -1322,3 +1352,141 @@ contains
     end subroutine node_values
 
 end module fortfem_tetra_lagrange_solver_3d
+
+! src/solvers/fortfem_tetra_nedelec_solver_3d.f90: the curl-mass drivers with
+! the sparse direct solve replaced by Jacobi-PCG (relative 1e-12 unless
+! options says otherwise); zero_tangential_boundary eliminates the boundary
+! edge and face dofs (build_zero_tangential_mask, :416-470) with zero values.
+module fortfem_tetra_nedelec_solver_3d
+    use, intrinsic :: iso_c_binding, only: c_int, c_int64_t, c_double, c_ptr, &
+        c_null_ptr
+    use fortfem_hip_types, only: dp, fortsparse_status_t, status_set, &
+        FORTSPARSE_INVALID_MATRIX, FORTSPARSE_INTERNAL_ERROR
+    use fortfem_hip_capi
+    use fortfem_hip_runtime, only: hip_context
+    use fortfem_assembly_tetra_nedelec_3d, only: tensor_coefficient_3d, &
+        vector_source_3d, sample_vector_source, sample_cell_tensors
+    use fortfem_advanced_solvers, only: solver_options_t, solver_stats_t, &
+        solver_options, to_c_options
+    implicit none
+    private
+
+    public :: solve_tetra_nedelec_curl_mass, solve_tetra_nedelec_weighted_curl_mass
+
+contains
+
+    ! :361-414
+    subroutine solve_tetra_nedelec_curl_mass(vertices, tetrahedra, order, &
+            source, curl_coefficient, mass_coefficient, solution, status, &
+            zero_tangential_boundary, options, stats)
+        real(dp), intent(in) :: vertices(:, :)
+        integer, intent(in) :: tetrahedra(:, :), order
+        procedure(vector_source_3d) :: source
+        real(dp), intent(in) :: curl_coefficient, mass_coefficient
+        real(dp), allocatable, intent(out) :: solution(:)
+        type(fortsparse_status_t), intent(out) :: status
+        logical, intent(in), optional :: zero_tangential_boundary
+        type(solver_options_t), intent(in), optional :: options
+        type(solver_stats_t), intent(out), optional :: stats
+
+        real(c_double) :: alpha(1)
+        type(solver_stats_t) :: st
+
+        call status_set(status, FORTSPARSE_INVALID_MATRIX, &
+            "Tetrahedral Nedelec curl-mass solve failed")
+        if (order < 1 .or. mass_coefficient <= 0.0_dp) return
+        alpha = curl_coefficient
+        call nedelec_solve(vertices, tetrahedra, order, source, FFH_COEFF_SCALAR, &
+            alpha, mass_coefficient, solution, status, zero_tangential_boundary, &
+            options, st)
+        if (present(stats)) stats = st
+    end subroutine solve_tetra_nedelec_curl_mass
+
+    ! :41-95.  The tensor must be constant on each cell (sampled at the
+    ! centroid and the four vertices).
+    subroutine solve_tetra_nedelec_weighted_curl_mass(vertices, tetrahedra, &
+            order, curl_coefficient, source, mass_coefficient, solution, status, &
+            zero_tangential_boundary, options, stats)
+        real(dp), intent(in) :: vertices(:, :)
+        integer, intent(in) :: tetrahedra(:, :), order
+        procedure(tensor_coefficient_3d) :: curl_coefficient
+        procedure(vector_source_3d) :: source
+        real(dp), intent(in) :: mass_coefficient
+        real(dp), allocatable, intent(out) :: solution(:)
+        type(fortsparse_status_t), intent(out) :: status
+        logical, intent(in), optional :: zero_tangential_boundary
+        type(solver_options_t), intent(in), optional :: options
+        type(solver_stats_t), intent(out), optional :: stats
+
+        real(dp), allocatable :: tensors(:, :, :)
+        type(solver_stats_t) :: st
+
+        call status_set(status, FORTSPARSE_INVALID_MATRIX, &
+            "Weighted tetrahedral Nedelec curl-mass solve failed")
+        if (order < 1 .or. mass_coefficient <= 0.0_dp) return
+        if (.not. sample_cell_tensors(vertices, tetrahedra, curl_coefficient, &
+            tensors)) return
+        call nedelec_solve(vertices, tetrahedra, order, source, &
+            FFH_COEFF_CELL_TENSOR, tensors, mass_coefficient, solution, status, &
+            zero_tangential_boundary, options, st)
+        if (present(stats)) stats = st
+    end subroutine solve_tetra_nedelec_weighted_curl_mass
+
+    subroutine nedelec_solve(vertices, tetrahedra, order, source, kind, coeff, &
+            mass, solution, status, zero_tangential_boundary, options, stats)
+        real(dp), intent(in) :: vertices(:, :)
+        integer, intent(in) :: tetrahedra(:, :), order
+        procedure(vector_source_3d) :: source
+        integer(c_int), intent(in) :: kind
+        real(c_double), intent(in) :: coeff(*)
+        real(dp), intent(in) :: mass
+        real(dp), allocatable, intent(out) :: solution(:)
+        type(fortsparse_status_t), intent(inout) :: status
+        logical, intent(in), optional :: zero_tangential_boundary
+        type(solver_options_t), intent(in), optional :: options
+        type(solver_stats_t), intent(out) :: stats
+
+        real(c_double), allocatable :: samples(:, :, :)
+        type(solver_options_t) :: opts
+        type(ffh_solver_stats) :: cst
+        type(c_ptr) :: prob
+        integer(c_int) :: n_global, code
+        integer(c_int64_t) :: nnz
+
+        if (.not. sample_vector_source(vertices, tetrahedra, order, source, &
+            samples)) return
+        code = ffh_problem_create(hip_context(), 3_c_int, &
+            int(size(vertices, 2), c_int), vertices, &
+            int(size(tetrahedra, 2), c_int), tetrahedra, prob)
+        if (code /= FFH_OK) return
+        code = ffh_pattern_build(prob, FFH_FAMILY_NEDELEC, int(order, c_int), &
+            n_global, nnz)
+        if (code == FFH_OK) code = ffh_assemble(prob, -1_c_int, kind, coeff, &
+            real(mass, c_double), FFH_FP64)
+        if (code == FFH_OK) code = ffh_load(prob, -1_c_int, FFH_SOURCE_SAMPLES, &
+            samples, int(size(samples), c_int64_t))
+        if (code == FFH_OK .and. present(zero_tangential_boundary)) then
+            if (zero_tangential_boundary) code = ffh_dirichlet(prob, 0_c_int, c_null_ptr)
+        end if
+        opts = solver_options(method="pcg", preconditioner="jacobi", &
+            tolerance=1.0e-12_dp, max_iterations=100000, breakdown="relative")
+        if (present(options)) opts = options
+        if (code == FFH_OK) then
+            allocate(solution(n_global))
+            code = ffh_problem_solve(prob, to_c_options(opts), solution, cst)
+        end if
+        stats%converged = cst%converged /= 0
+        stats%iterations = cst%iterations
+        stats%final_residual = cst%final_residual
+        stats%solve_time = cst%solve_time
+        stats%method_used = "pcg"
+        if (code == FFH_OK) then
+            call status_set(status, 0, "")
+        else
+            call status_set(status, FORTSPARSE_INTERNAL_ERROR, &
+                "Tetrahedral Nedelec device solve failed")
+        end if
+        code = ffh_problem_free(prob)
+    end subroutine nedelec_solve
+
+end module fortfem_tetra_nedelec_solver_3d

@@ -7,7 +7,8 @@
 ! test/test_tetra_nedelec_sparse_assembly_slow.f90:31-59,
 ! test/test_sparse_matrix_ad.f90:19-65, test/test_pcg_solver_ad.f90:18-60,
 ! test/test_tetra_lagrange_stiffness_csc_ad.f90:10-93,
-! test/test_tetra_nedelec_curl_mass_csc_ad.f90:10-96.
+! test/test_tetra_nedelec_curl_mass_csc_ad.f90:10-96,
+! test/test_public_tetra_nedelec_arbitrary_order_solve_slow.f90:14-46.
 module dropin_test_sources
     use fortfem_hip_types, only: dp
     implicit none
@@ -55,6 +56,8 @@ program test_fortfem_hip_dropin
     use fortfem_advanced_solvers, only: solver_options_t, solver_stats_t, &
         solver_options, solve_sparse, solve_sparse_jvp, solve_sparse_vjp
     use fortfem_tetra_lagrange_solver_3d, only: solve_tetra_lagrange_poisson
+    use fortfem_tetra_nedelec_solver_3d, only: solve_tetra_nedelec_curl_mass
+    use fortfem_hip_capi, only: ffh_nedelec_dof_map
     use dropin_test_sources, only: bubble_source, zero_boundary, sin_source, constant_field
     implicit none
 
@@ -72,6 +75,7 @@ program test_fortfem_hip_dropin
     call check_pcg_ad()
     call check_h1_csc_ad()
     call check_nedelec_csc_ad()
+    call check_nedelec_constant_solve()
     call fortfem_hip_finalize()
     if (.not. all_passed) error stop 1
     write (*, '(a)') "drop-in Fortran checks passed"
@@ -436,6 +440,48 @@ contains
         call record(abs(lhs - rhs) < 3.0e-10_dp*max(1.0_dp, abs(lhs), abs(rhs)), &
             "Global curl-mass products accumulate the shared-vertex adjoint")
     end subroutine check_nedelec_csc_ad
+
+    ! The curl-mass solve with a constant source reproduces the constant
+    ! field (it lies in the space); for k = 1 the global dofs are then the
+    ! Whitney moments o_e f . (x_j - x_i) of every local edge (i, j).
+    subroutine check_nedelec_constant_solve()
+        use, intrinsic :: iso_c_binding, only: c_int
+        integer, parameter :: tets(4, 2) = reshape([1, 2, 3, 4, 2, 3, 4, 5], [4, 2])
+        integer, parameter :: ev(2, 6) = reshape([1, 2, 1, 3, 1, 4, 2, 3, 2, 4, 3, 4], [2, 6])
+        real(dp) :: vertices(3, 5), err
+        real(dp), allocatable :: solution(:)
+        type(fortsparse_status_t) :: st
+        integer(c_int) :: gd(6, 2), eo(6, 2), fp(3, 4, 2), nd
+        integer :: cell, e, order
+        logical :: ok
+
+        vertices = reshape([0.0_dp, 0.0_dp, 0.0_dp, 1.0_dp, 0.0_dp, 0.0_dp, &
+            0.0_dp, 1.0_dp, 0.0_dp, 0.0_dp, 0.0_dp, 1.0_dp, 1.0_dp, 1.0_dp, &
+            1.0_dp], [3, 5])
+        ok = .true.
+        do order = 1, 3
+            call solve_tetra_nedelec_curl_mass(vertices, tets, order, constant_field, &
+                1.0_dp, 1.0_dp, solution, st)
+            ok = ok .and. st%code == 0
+            if (order == 1 .and. st%code == 0) then
+                if (ffh_nedelec_dof_map(1_c_int, 2_c_int, tets, 1_c_int, gd, eo, fp, &
+                    nd) /= 0) ok = .false.
+                err = 0.0_dp
+                do cell = 1, 2
+                    do e = 1, 6
+                        err = max(err, abs(solution(gd(e, cell)) - eo(e, cell)* &
+                            dot_product([0.7_dp, -1.3_dp, 0.4_dp], &
+                            vertices(:, tets(ev(2, e), cell)) - &
+                            vertices(:, tets(ev(1, e), cell)))))
+                    end do
+                end do
+                call record(err < 1.0e-9_dp, &
+                    "Nedelec k=1 solve reproduces the constant field's edge moments")
+            end if
+            if (allocated(solution)) deallocate(solution)
+        end do
+        call record(ok, "Nedelec curl-mass solves succeed for k = 1..3")
+    end subroutine check_nedelec_constant_solve
 
     subroutine check_linear_energy()
         real(dp) :: mesh_vertices(3, 5), coefficients(5), product(5)
