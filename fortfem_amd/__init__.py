@@ -1116,6 +1116,19 @@ class Server:
         self.ctx.check(self._lib.ffh_server_step(self.handle, C.byref(a), C.byref(q), C.byref(f)))
         return a.value, q.value, f.value
 
+    def wait(self, ticket, nrhs, n, max_steps=1000000):
+        """Steps the server until the request has finished (ffh_server_wait);
+        the results as poll(), or None if max_steps were not enough."""
+        x = np.zeros((nrhs, n))
+        it = np.zeros(nrhs, dtype=np.int32)
+        cv = np.zeros(nrhs, dtype=np.int32)
+        rr = np.zeros(nrhs)
+        code = self._lib.ffh_server_wait(self.handle, ticket, max_steps, _ptr(x), _ptr(it), _ptr(cv), _ptr(rr))
+        if code == 1:
+            return None
+        self.ctx.check(code)
+        return x, it, cv.astype(bool), rr
+
     def poll(self, ticket, nrhs, n):
         """None while pending, else (solutions (nrhs, n), iterations, converged, rel_residual)."""
         x = np.zeros((nrhs, n))

@@ -51,7 +51,7 @@ EXPORTS = (
     "ffh_nedelec_load_points", "ffh_spmv_jvp", "ffh_spmv_vjp", "ffh_solve_sparse_jvp",
     "ffh_solve_sparse_vjp", "ffh_assemble_jvp", "ffh_assemble_vjp",
     "ffh_server_create", "ffh_server_free", "ffh_server_prepare", "ffh_server_submit", "ffh_server_step",
-    "ffh_server_poll", "ffh_server_stats", "ffh_server_progress", "ffh_problem_solution", "ffh_server_set_precision",
+    "ffh_server_poll", "ffh_server_wait", "ffh_server_stats", "ffh_server_progress", "ffh_problem_solution", "ffh_server_set_precision",
 )
 
 
@@ -154,6 +154,7 @@ def load() -> C.CDLL:
                                         C.c_int, i64p]),
         "ffh_server_step": (C.c_int, [vp, ip, ip, ip]),
         "ffh_server_poll": (C.c_int, [vp, C.c_int64, vp, vp, vp, vp]),
+        "ffh_server_wait": (C.c_int, [vp, C.c_int64, C.c_int, vp, vp, vp, vp]),
         "ffh_server_stats": (C.c_int, [vp, i64p, i64p]),
         "ffh_server_progress": (C.c_int, [vp, C.c_int64, vp, vp, vp]),
         "ffh_problem_solution": (C.c_int, [vp, vp]),

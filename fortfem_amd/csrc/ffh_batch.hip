@@ -1277,6 +1277,21 @@ int ffh_server_poll(ffh_server *S, int64_t ticket, double *solution, int *iterat
     return FFH_OK;
 }
 
+int ffh_server_wait(ffh_server *S, int64_t ticket, int max_steps, double *solution,
+                    int *iterations, int *converged, double *rel_residual)
+{
+    if (!S) return FFH_ERR_HANDLE;
+    auto it = S->reqs.find(ticket);
+    if (it == S->reqs.end()) return FFH_ERR_HANDLE;
+    if (max_steps < 0) return FFH_ERR_INVALID;
+    const ffh::ServedRequest *q = it->second.get();
+    for (int k = 0; k < max_steps && !q->done; ++k) {
+        const int code = ffh_server_step(S, nullptr, nullptr, nullptr);
+        if (code != FFH_OK) return code;
+    }
+    return ffh_server_poll(S, ticket, solution, iterations, converged, rel_residual);
+}
+
 int ffh_server_progress(const ffh_server *S, int64_t ticket, int *state, int *iterations,
                         double *rel_residual)
 {

@@ -295,6 +295,11 @@ int ffh_server_step(ffh_server *server, int *n_active, int *n_queued, int *n_fin
 /* 0 and the results (then the ticket is released) if finished, 1 if not yet. */
 int ffh_server_poll(ffh_server *server, int64_t ticket, double *solution, int *iterations,
                     int *converged, double *rel_residual);
+/* Steps the server (at most max_steps steps, every active request
+ * advancing) until the ticket's request has finished, then as
+ * ffh_server_poll: 0 and the results, or 1 if it is still running. */
+int ffh_server_wait(ffh_server *server, int64_t ticket, int max_steps, double *solution,
+                    int *iterations, int *converged, double *rel_residual);
 int ffh_server_stats(const ffh_server *server, int64_t *used_bytes, int64_t *iterations_run);
 /* Progress of a request as of the last step: per right-hand side the state
  * (-1 queued, 0 iterating, 1 converged, 2 stopped, 4 waiting for the FP64

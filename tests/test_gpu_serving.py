@@ -136,3 +136,22 @@ def test_admission_by_memory(ff):
     with pytest.raises(ff.FFHError) as e:
         tiny.submit(0, ff.capi.FAMILY_H1, 1, np.ones(t.shape[0]), np.ones(n))
     assert e.value.code == ff.capi.FFH_ERR_ADMISSION
+
+
+def test_wait_returns_the_request_result(ff):
+    """ffh_server_wait steps the whole batch until the ticket finishes and
+    returns what poll would; the result equals the request solved alone."""
+    alone = ff.Server(budget_bytes=1 << 30, max_systems=16, chunk=8)
+    reqs = make_requests(ff, alone, seed=5)
+    target = reqs[2]
+    tk = alone.submit(target["key"], target["fam"], target["k"], target["coeff"], target["b"],
+                      mass=target["mass"])
+    xa = alone.run_until_done([tk], max_steps=20000)[tk[0]][0]
+    srv = ff.Server(budget_bytes=1 << 30, max_systems=16, chunk=8)
+    reqs2 = make_requests(ff, srv, seed=5)
+    tickets = [srv.submit(r["key"], r["fam"], r["k"], r["coeff"], r["b"], mass=r["mass"]) for r in reqs2]
+    assert srv.wait(*tickets[2], max_steps=0) is None  # nothing run yet
+    x, it, conv, rel = srv.wait(*tickets[2], max_steps=20000)
+    assert conv.all() and np.array_equal(x, xa)
+    for tk2 in tickets[:2] + tickets[3:]:
+        assert srv.wait(*tk2, max_steps=20000) is not None
