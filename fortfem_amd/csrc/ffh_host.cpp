@@ -454,11 +454,11 @@ void build_slots(int ncell, int nloc, const std::vector<int> &gd,
 }
 
 // build_dirichlet_data (src/solvers/fortfem_tetra_lagrange_solver_3d.f90:
-// 551-610): a face contained in exactly one cell is a boundary face
-// (:587-597); the local dofs with barycentric index 0 at the opposite vertex
-// are constrained (:598-606).  Face incidence by hashing instead of the
-// O(T^2) scan.
-void lagrange_boundary(int d, int ntet, const int *tets,
+// 551-610): the local dofs with barycentric index 0 at the vertex opposite
+// a boundary face (:598-606) are constrained.  bf[4 e + o] flags the
+// boundary faces (a face contained in exactly one cell, :587-597), found on
+// the device.
+void lagrange_boundary(int d, int ntet, const std::vector<unsigned char> &bf,
                        const std::vector<int> &gd, int n_global,
                        std::vector<int> &constrained)
 {
@@ -466,31 +466,10 @@ void lagrange_boundary(int d, int ntet, const int *tets,
     std::vector<double> nodes;
     lagrange_indices(d, bary, nodes);
     const int n = lagrange_dof_count(d);
-    struct FaceHash {
-        size_t operator()(const std::array<int, 3> &f) const
-        {
-            uint64_t h = (uint64_t)(uint32_t)f[0] * 0x9e3779b97f4a7c15ull;
-            h ^= (uint64_t)(uint32_t)f[1] * 0xbf58476d1ce4e5b9ull + (h >> 17);
-            h ^= (uint64_t)(uint32_t)f[2] * 0x94d049bb133111ebull + (h >> 23);
-            return (size_t)(h ^ (h >> 29));
-        }
-    };
-    auto face = [&](int64_t e, int o) {
-        std::array<int, 3> f;
-        int k = 0;
-        for (int i = 0; i < 4; ++i)
-            if (i != o) f[k++] = tets[4 * e + i];
-        std::sort(f.begin(), f.end());
-        return f;
-    };
-    std::unordered_map<std::array<int, 3>, int, FaceHash> inc;
-    inc.reserve((size_t)ntet * 3);
-    for (int64_t e = 0; e < ntet; ++e)
-        for (int o = 0; o < 4; ++o) inc[face(e, o)]++;
     constrained.assign(n_global, 0);
     for (int64_t e = 0; e < ntet; ++e)
         for (int o = 0; o < 4; ++o) {
-            if (inc[face(e, o)] != 1) continue;
+            if (!bf[4 * e + o]) continue;
             for (int l = 0; l < n; ++l)
                 if (bary[4 * l + o] == 0) constrained[gd[(size_t)n * e + l]] = 1;
         }

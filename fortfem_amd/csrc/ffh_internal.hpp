@@ -287,7 +287,7 @@ void build_entry_slots(int ncell, int nloc, const std::vector<int> &gd,
 void build_slots(int ncell, int nloc, const std::vector<int> &gd,
                  const std::vector<int> &perm, const std::vector<int> &col_ptr,
                  const std::vector<int> &row_idx, std::vector<int> &slots);
-void lagrange_boundary(int d, int ntet, const int *tets,
+void lagrange_boundary(int d, int ntet, const std::vector<unsigned char> &bf,
                        const std::vector<int> &gd, int n_global,
                        std::vector<int> &constrained);
 

@@ -157,32 +157,20 @@ int nedelec_dof_map(int k, int ntet, const int *tets, std::vector<int> &gd,
     return 0;
 }
 
-void nedelec_boundary_dofs(int k, int ntet, const int *tets, const std::vector<int> &gd,
-                           std::vector<unsigned char> &constrained)
+void nedelec_boundary_dofs(int k, int ntet, const std::vector<unsigned char> &bf,
+                           const std::vector<int> &gd, std::vector<unsigned char> &constrained)
 {
     const int n = nedelec_dof_count(k), fdc = k * (k - 1);
     int nd = 0;
     for (int v : gd) nd = std::max(nd, v + 1);
     constrained.assign(nd, 0);
-    // face incidence by sorted triple
-    FirstAppearance faces((size_t)2 * ntet + 16);
-    std::vector<int> fid((size_t)4 * ntet), inc;
-    for (int64_t e = 0; e < ntet; ++e)
-        for (int f = 0; f < 4; ++f) {
-            int v[3] = {tets[4 * e + kFaceV[f][0]], tets[4 * e + kFaceV[f][1]],
-                        tets[4 * e + kFaceV[f][2]]};
-            std::sort(v, v + 3);
-            const int id = faces.get(v[0], v[1], v[2]);
-            if (id >= (int)inc.size()) inc.push_back(0);
-            inc[id]++;
-            fid[4 * e + f] = id;
-        }
-    // local edges of each local face (positions in kEdgeV)
+    // local edges of each local face (positions in kEdgeV); local face f
+    // (kFaceV) is the face opposite local vertex 3 - f
     static const int kFaceEdges[4][3] = {{0, 1, 3}, {0, 2, 4}, {1, 2, 5}, {3, 4, 5}};
     for (int64_t e = 0; e < ntet; ++e) {
         const int *g = gd.data() + (size_t)n * e;
         for (int f = 0; f < 4; ++f) {
-            if (inc[fid[4 * e + f]] != 1) continue;
+            if (!bf[4 * e + (3 - f)]) continue;
             for (int j = 0; j < 3; ++j)
                 for (int m = 0; m < k; ++m) constrained[g[kFaceEdges[f][j] * k + m]] = 1;
             for (int d = 0; d < fdc; ++d) constrained[g[6 * k + f * fdc + d]] = 1;

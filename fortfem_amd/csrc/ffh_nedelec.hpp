@@ -27,10 +27,13 @@ constexpr int kNedelecClasses = 24;
 int nedelec_dof_map(int k, int ntet, const int *tets, std::vector<int> &gd,
                     std::vector<unsigned char> &cls, int &ndofs, int &nedges, int &nfaces);
 
-// Constrained dofs of the tangential trace on the boundary: every dof of an
-// edge or face that lies on a boundary face (a face of one cell).
-void nedelec_boundary_dofs(int k, int ntet, const int *tets, const std::vector<int> &gd,
-                           std::vector<unsigned char> &constrained);
+// Constrained dofs of the tangential trace on the boundary
+// (build_zero_tangential_mask, src/solvers/fortfem_tetra_nedelec_solver_3d.f90:
+// 431-470): every dof of an edge or face that lies on a boundary face.
+// bf[4 e + o] flags the boundary face of cell e opposite its local vertex o
+// (a face of one cell, found on the device).
+void nedelec_boundary_dofs(int k, int ntet, const std::vector<unsigned char> &bf,
+                           const std::vector<int> &gd, std::vector<unsigned char> &constrained);
 
 // Reference tensors of the curl-mass element, transformed into orientation
 // classes.  For class slot i (classes[i]) and term s, the packed upper

@@ -1558,6 +1558,96 @@ void problem_load(Problem *P, int qdeg, int kind, const double *params, int64_t 
 }
 
 // Dirichlet: symbolic reduction (host, once) + numeric (device)
+// Boundary faces of a tetrahedral mesh on the device: a face contained in
+// exactly one cell (build_dirichlet_data,
+// src/solvers/fortfem_tetra_lagrange_solver_3d.f90:587-597, and
+// build_zero_tangential_mask, fortfem_tetra_nedelec_solver_3d.f90:442-462).
+// Vertex -> cell lists by a counting sort, then one thread per (cell, face)
+// looks among the cells around the face's smallest vertex for another one
+// holding the other two vertices.  bf[4 e + o] = 1 for the boundary face of
+// cell e (reference order) opposite its local vertex o.
+__global__ void k_vertex_count(int T, const int4 *__restrict__ cells_c, int *__restrict__ cnt)
+{
+    const int pos = blockIdx.x * 256 + threadIdx.x;
+    if (pos >= T) return;
+    const int4 c = cells_c[pos];
+    atomicAdd(&cnt[c.x], 1);
+    atomicAdd(&cnt[c.y], 1);
+    atomicAdd(&cnt[c.z], 1);
+    atomicAdd(&cnt[c.w], 1);
+}
+
+__global__ void k_vertex_fill(int T, const int4 *__restrict__ cells_c, int *__restrict__ cursor,
+                              int *__restrict__ list)
+{
+    const int pos = blockIdx.x * 256 + threadIdx.x;
+    if (pos >= T) return;
+    const int4 c = cells_c[pos];
+    list[atomicAdd(&cursor[c.x], 1)] = pos;
+    list[atomicAdd(&cursor[c.y], 1)] = pos;
+    list[atomicAdd(&cursor[c.z], 1)] = pos;
+    list[atomicAdd(&cursor[c.w], 1)] = pos;
+}
+
+__global__ void k_boundary_faces(int T, const int4 *__restrict__ cells_c,
+                                 const int *__restrict__ perm, const int *__restrict__ start,
+                                 const int *__restrict__ list, unsigned char *__restrict__ bf)
+{
+    const int64_t t = blockIdx.x * 256ll + threadIdx.x;
+    if (t >= 4ll * T) return;
+    const int pos = (int)(t >> 2), o = (int)(t & 3);
+    const int4 c = cells_c[pos];
+    const int v[4] = {c.x, c.y, c.z, c.w};
+    int f[3], k = 0;
+    for (int i = 0; i < 4; ++i)
+        if (i != o) f[k++] = v[i];
+    int a = min(f[0], min(f[1], f[2]));
+    int b = a == f[0] ? f[1] : f[0];
+    int d = (a == f[0] || b == f[0]) && (a == f[1] || b == f[1]) ? f[2]
+            : ((a == f[0] || b == f[0]) ? f[1] : f[0]);
+    bool shared = false;
+    for (int i = start[a]; i < start[a + 1] && !shared; ++i) {
+        const int q = list[i];
+        if (q == pos) continue;
+        const int4 cq = cells_c[q];
+        const bool hb = cq.x == b || cq.y == b || cq.z == b || cq.w == b;
+        const bool hd = cq.x == d || cq.y == d || cq.z == d || cq.w == d;
+        shared = hb && hd;
+    }
+    bf[4 * (size_t)perm[pos] + o] = shared ? 0 : 1;
+}
+
+static void boundary_faces(Problem *P, std::vector<unsigned char> &bf)
+{
+    Context *ctx = P->ctx;
+    hipStream_t s = ctx->stream;
+    const int T = P->ncell, nv = P->nvert;
+    const int4 *cells = reinterpret_cast<const int4 *>(P->d_cells_c.p);
+    DBuf<int> cnt, start, list;
+    DBuf<unsigned char> dbf;
+    cnt.alloc(nv);
+    FFH_HIP(hipMemsetAsync(cnt.p, 0, sizeof(int) * nv, s));
+    const int g1 = (T + 255) / 256;
+    hipLaunchKernelGGL(k_vertex_count, dim3(g1), dim3(256), 0, s, T, cells, cnt.p);
+    FFH_HIP(hipGetLastError());
+    std::vector<int> h(nv), hs(nv + 1);
+    cnt.download(h.data(), nv, s);
+    FFH_HIP(hipStreamSynchronize(s));
+    hs[0] = 0;
+    for (int i = 0; i < nv; ++i) hs[i + 1] = hs[i] + h[i];
+    start.upload(hs, s);
+    cnt.upload(hs.data(), nv, s); // fill cursors
+    list.alloc(std::max<size_t>(1, (size_t)4 * T));
+    hipLaunchKernelGGL(k_vertex_fill, dim3(g1), dim3(256), 0, s, T, cells, cnt.p, list.p);
+    dbf.alloc((size_t)4 * T);
+    hipLaunchKernelGGL(k_boundary_faces, dim3((unsigned)((4ll * T + 255) / 256)), dim3(256), 0, s,
+                       T, cells, P->d_perm.p, start.p, list.p, dbf.p);
+    FFH_HIP(hipGetLastError());
+    bf.resize((size_t)4 * T);
+    dbf.download(bf.data(), bf.size(), s);
+    FFH_HIP(hipStreamSynchronize(s));
+}
+
 void problem_dirichlet(Problem *P, int mode, const double *values)
 {
     Context *ctx = P->ctx;
@@ -1566,12 +1656,15 @@ void problem_dirichlet(Problem *P, int mode, const double *values)
     const int G = P->n_global;
     std::vector<int> cons;
     if (P->family == FFH_FAMILY_H1) {
-        lagrange_boundary(P->order, P->ncell, P->cells.data(), P->global_dofs, G, cons);
+        std::vector<unsigned char> bf;
+        boundary_faces(P, bf);
+        lagrange_boundary(P->order, P->ncell, bf, P->global_dofs, G, cons);
     } else if (P->family == FFH_FAMILY_NEDELEC) {
         // zero tangential trace (build_zero_tangential_mask,
         // src/solvers/fortfem_tetra_nedelec_solver_3d.f90:431-470)
-        std::vector<unsigned char> mask;
-        nedelec_boundary_dofs(P->order, P->ncell, P->cells.data(), P->global_dofs, mask);
+        std::vector<unsigned char> bf, mask;
+        boundary_faces(P, bf);
+        nedelec_boundary_dofs(P->order, P->ncell, bf, P->global_dofs, mask);
         cons.assign(G, 0);
         for (int i = 0; i < G && i < (int)mask.size(); ++i) cons[i] = mask[i];
     } else if (P->family == FFH_FAMILY_P1_2D) {
