@@ -374,6 +374,24 @@ class Problem:
                                                _ptr(vb), _ptr(xb), _ptr(kb), C.byref(mb)))
         return xb, (kb if kind == capi.COEFF_CELL else float(kb[0])), mb.value
 
+    def nedelec_pml_values(self, stretch, wave_number, curvilinear=False):
+        """Complex PML operator values on the Nedelec pattern
+        (ffh_nedelec_pml_values): stretch (T, 3) complex, or (T, 3, 3) with
+        curvilinear=True (S[t][i, j] the cell's stretch matrix)."""
+        S = np.asarray(stretch, dtype=np.complex128)
+        if curvilinear:
+            if S.shape != (self.ncell, 3, 3):
+                raise ValueError("stretch must be (T, 3, 3)")
+            S = np.transpose(S, (0, 2, 1))
+        elif S.shape != (self.ncell, 3):
+            raise ValueError("stretch must be (T, 3)")
+        st = np.ascontiguousarray(np.stack([S.real, S.imag], axis=-1).reshape(-1))
+        nnz = self.sizes()["nnz"]
+        out = np.zeros(2 * max(nnz, 1))
+        self._check(self._lib.ffh_nedelec_pml_values(self.handle, 1 if curvilinear else 0, _ptr(st),
+                                                      float(wave_number), _ptr(out)))
+        return out[0:2 * nnz:2] + 1j * out[1:2 * nnz:2]
+
     def load(self, quadrature_degree, kind, params):
         p = _f64(np.atleast_1d(params))
         self._check(self._lib.ffh_load(self.handle, quadrature_degree, kind, _ptr(p), p.size))
@@ -816,6 +834,41 @@ def assemble_tetra_nedelec_curl_mass_csc_vjp(mesh_vertices, tetrahedra, order, c
             raise
         return (*zero, status)
     return xb, ab, bb, FortsparseStatus(0, "")
+
+
+def _nedelec_pml(mesh_vertices, tetrahedra, order, stretch, wave_number, curvilinear, message, ctx):
+    status = FortsparseStatus(FORTSPARSE_INVALID_MATRIX, message)
+    v = np.asarray(mesh_vertices)
+    t = np.asarray(tetrahedra)
+    if order < 1 or wave_number <= 0 or v.ndim != 2 or v.shape[1] != 3 or t.ndim != 2 or t.shape[1] != 4:
+        return None, status
+    try:
+        p = Problem(v, t, ctx)
+        p.pattern_build(capi.FAMILY_NEDELEC, order)
+        vals = p.nedelec_pml_values(stretch, wave_number, curvilinear)
+        a = p.matrix(capi.FULL).download()
+    except (FFHError, ValueError) as e:
+        if isinstance(e, FFHError) and e.code == capi.FFH_ERR_DEVICE:
+            raise
+        return None, status
+    a.val = vals
+    return a, FortsparseStatus(0, "")
+
+
+def assemble_tetra_nedelec_pml_csc(mesh_vertices, tetrahedra, order, stretch, wave_number,
+                                   ctx: Context | None = None):
+    """assemble_tetra_nedelec_pml_csc (assembly_tetra_nedelec_3d.f90:1613-1685):
+    stretch (T, 3) complex per cell -> (CSC with complex values (csc_z_t), status)."""
+    return _nedelec_pml(mesh_vertices, tetrahedra, order, stretch, wave_number, False,
+                        "Complex tetrahedral Nedelec PML assembly failed", ctx)
+
+
+def assemble_tetra_nedelec_curvilinear_pml_csc(mesh_vertices, tetrahedra, order, stretch, wave_number,
+                                               ctx: Context | None = None):
+    """assemble_tetra_nedelec_curvilinear_pml_csc (:1372-1444): stretch
+    (T, 3, 3) complex per cell -> (CSC with complex values, status)."""
+    return _nedelec_pml(mesh_vertices, tetrahedra, order, stretch, wave_number, True,
+                        "Curvilinear tetrahedral Nedelec PML assembly failed", ctx)
 
 
 def assemble_tetra_nedelec_weighted_csc(mesh_vertices, tetrahedra, coefficient, mass_coefficient,

@@ -419,6 +419,14 @@ int ffh_assemble_vjp(ffh_problem *P, int qdeg, int coeff_kind, const double *coe
     });
 }
 
+int ffh_nedelec_pml_values(ffh_problem *P, int kind, const double *stretch, double wave_number,
+                           double *values)
+{
+    if (!P) return FFH_ERR_HANDLE;
+    if (!stretch || !values) return FFH_ERR_INVALID;
+    return guarded(P->ctx, [&] { problem_nedelec_pml(P, kind, stretch, wave_number, values); });
+}
+
 int ffh_load(ffh_problem *P, int qdeg, int kind, const double *params, int64_t np)
 {
     if (!P) return FFH_ERR_HANDLE;

@@ -21,6 +21,8 @@ double bench_spmv(Context *ctx, Matrix &M, int precision, int reps,
 namespace ffh {
 // problems (ffh_problem.hip)
 void problem_pattern(Problem *P, int family, int order);
+void problem_nedelec_pml(Problem *P, int kind, const double *stretch, double k0,
+                         double *values);
 void problem_assemble_jvp(Problem *P, int qdeg, int coeff_kind, const double *coeff,
                           double mass, const double *vertices_dot, const double *coeff_dot,
                           double mass_dot, double *values_dot);

@@ -19,6 +19,8 @@
 // (r,c) and (c,r) slots, so the assembled matrix is exactly symmetric and
 // products can read the CSC arrays as CSR.
 #include <algorithm>
+#include <cfloat>
+#include <complex>
 #include <cmath>
 #include <cstring>
 #include <numeric>
@@ -346,13 +348,15 @@ __global__ __launch_bounds__(kBlockA) void k_p1_2d_cells(
 
 // Nedelec terms of one cell (colour order): the 12 coefficients of the
 // reference tensors.  Curl part J^T K J / det with K = alpha I (scalar or per
-// cell) or the cell's tensor; value part beta det J^-1 J^-T.  Terms follow the
-// table order (00, 11, 22, 01, 02, 12), curl then value.
+// cell) or the cell's tensor; value part beta det J^-1 J^-T, or
+// det J^-1 M J^-T with the cell's value tensor M.  Terms follow the table
+// order (00, 11, 22, 01, 02, 12), curl then value.
 __global__ __launch_bounds__(kBlockA) void k_ned_terms(
     int ncell, const int *__restrict__ cells_c, const double *__restrict__ verts,
     const int *__restrict__ perm, const double *__restrict__ alpha_cell,
     const double *__restrict__ tensor, double alpha, double beta,
-    double *__restrict__ gterm, int *__restrict__ err)
+    double *__restrict__ gterm, int *__restrict__ err,
+    const double *__restrict__ mtensor = nullptr)
 {
     const int pos = blockIdx.x * kBlockA + threadIdx.x;
     if (pos >= ncell) return;
@@ -382,9 +386,20 @@ __global__ __launch_bounds__(kBlockA) void k_ned_terms(
             double kj = K[t + 0] * jac[0 + 3 * b] + K[t + 3] * jac[1 + 3 * b] + K[t + 6] * jac[2 + 3 * b];
             cm += jac[t + 3 * a] * kj;
         }
-        const double mm = inv[a + 0] * inv[b + 0] + inv[a + 3] * inv[b + 3] + inv[a + 6] * inv[b + 6];
+        double mm;
+        if (mtensor) {
+            // (J^-1 M J^-T)_ab with the cell's symmetric value tensor M
+            const double *M = mtensor + 9 * (size_t)e;
+            mm = 0.0;
+            for (int t = 0; t < 3; ++t) {
+                const double mi = M[t + 0] * inv[b + 0] + M[t + 3] * inv[b + 3] + M[t + 6] * inv[b + 6];
+                mm += inv[a + 3 * t] * mi;
+            }
+        } else {
+            mm = beta * (inv[a + 0] * inv[b + 0] + inv[a + 3] * inv[b + 3] + inv[a + 6] * inv[b + 6]);
+        }
         gterm[(size_t)s * T + pos] = cm * rdet;
-        gterm[(size_t)(6 + s) * T + pos] = beta * det * mm;
+        gterm[(size_t)(6 + s) * T + pos] = det * mm;
     }
 }
 
@@ -1421,6 +1436,129 @@ void problem_assemble_vjp(Problem *P, int qdeg, int coeff_kind, const double *co
     else
         coeff_bar[0] = ksum;
     *mass_bar = msum;
+}
+
+// Complex curl-curl PML operator on the Nedelec pattern:
+// assemble_tetra_nedelec_pml_csc (src/assembly/assembly_tetra_nedelec_3d.f90:
+// 1613-1685, element :826-887) for kind 0 and
+// assemble_tetra_nedelec_curvilinear_pml_csc (:1372-1444, element :1092-1150)
+// for kind 1.  The element is the weighted curl-mass element with per-cell
+// complex symmetric tensors, curl K and value -k^2 M, from the stretch:
+// cartesian (cartesian_curl_curl_pml_coefficients,
+// src/boundary/cartesian_helmholtz_pml.f90:23-41, 94-108) m_i = s_j s_k / s_i,
+// K = diag(1 / m_i), M = diag(m_i); curvilinear
+// (curvilinear_curl_curl_pml_coefficients, curvilinear_helmholtz_pml.f90:
+// 113-131, 202-216) K = S^T S / det S, M = det S S^-1 S^-T.  The element is
+// linear in (K, M), so the real and imaginary parts are two real tensor
+// assemblies on the class tables.  stretch: complex (re, im) pairs, 3 per
+// cell (kind 0) or 3 x 3 column-major per cell (kind 1), in cell order;
+// values: nnz (re, im) pairs in the pattern's order (host).
+void problem_nedelec_pml(Problem *P, int kind, const double *stretch, double k0,
+                         double *values)
+{
+    using cd = std::complex<double>;
+    Context *ctx = P->ctx;
+    hipStream_t s = ctx->stream;
+    FFH_REQUIRE(P->n_global > 0 && P->family == FFH_FAMILY_NEDELEC, FFH_ERR_INVALID,
+                "the PML operator needs a Nedelec pattern");
+    FFH_REQUIRE(kind == 0 || kind == 1, FFH_ERR_INVALID, "PML kind is 0 or 1");
+    FFH_REQUIRE(k0 > 0.0, FFH_ERR_INVALID, "wave number must be positive");
+    const int T = P->ncell;
+    std::vector<double> kt[2], mt[2];
+    for (int c = 0; c < 2; ++c) {
+        kt[c].assign((size_t)9 * T, 0.0);
+        mt[c].assign((size_t)9 * T, 0.0);
+    }
+    const double k2 = k0 * k0;
+    for (int e = 0; e < T; ++e) {
+        cd K[9], M[9];
+        if (kind == 0) {
+            cd sv[3];
+            for (int i = 0; i < 3; ++i) {
+                sv[i] = cd(stretch[6 * (size_t)e + 2 * i], stretch[6 * (size_t)e + 2 * i + 1]);
+                FFH_REQUIRE(std::abs(sv[i]) > DBL_MIN, FFH_ERR_INVALID, "zero PML stretch");
+            }
+            for (int i = 0; i < 9; ++i) K[i] = M[i] = 0.0;
+            for (int i = 0; i < 3; ++i) {
+                const cd m = sv[(i + 1) % 3] * sv[(i + 2) % 3] / sv[i];
+                M[4 * i] = m;
+                K[4 * i] = 1.0 / m;
+            }
+        } else {
+            cd S[9];
+            double mx = 0.0;
+            for (int i = 0; i < 9; ++i) {
+                S[i] = cd(stretch[18 * (size_t)e + 2 * i], stretch[18 * (size_t)e + 2 * i + 1]);
+                mx = std::max(mx, std::abs(S[i]));
+            }
+#define Sx(r, c) S[(r) + 3 * (c)]
+            cd cof[9]; // cofactor matrix, cof(r, c)
+            for (int r = 0; r < 3; ++r)
+                for (int c = 0; c < 3; ++c) {
+                    const int r1 = (r + 1) % 3, r2 = (r + 2) % 3, c1 = (c + 1) % 3,
+                              c2 = (c + 2) % 3;
+                    cof[r + 3 * c] = Sx(r1, c1) * Sx(r2, c2) - Sx(r1, c2) * Sx(r2, c1);
+                }
+            const cd det = Sx(0, 0) * cof[0] + Sx(0, 1) * cof[3] + Sx(0, 2) * cof[6];
+#undef Sx
+            const double sc = std::max(1.0, mx);
+            FFH_REQUIRE(std::isfinite(det.real()) && std::isfinite(det.imag()) &&
+                            std::abs(det) > 64.0 * DBL_EPSILON * sc * sc * sc,
+                        FFH_ERR_INVALID, "singular PML stretch");
+            cd inv[9]; // S^-1 = cof^T / det
+            for (int r = 0; r < 3; ++r)
+                for (int c = 0; c < 3; ++c) inv[r + 3 * c] = cof[c + 3 * r] / det;
+            for (int r = 0; r < 3; ++r)
+                for (int c = 0; c < 3; ++c) {
+                    cd a = 0.0, b = 0.0;
+                    for (int t = 0; t < 3; ++t) {
+                        a += S[t + 3 * r] * S[t + 3 * c];     // (S^T S)_rc
+                        b += inv[r + 3 * t] * inv[c + 3 * t]; // (S^-1 S^-T)_rc
+                    }
+                    K[r + 3 * c] = a / det;
+                    M[r + 3 * c] = det * b;
+                }
+        }
+        for (int i = 0; i < 9; ++i) {
+            const cd m = -k2 * M[i];
+            kt[0][9 * (size_t)e + i] = K[i].real();
+            kt[1][9 * (size_t)e + i] = K[i].imag();
+            mt[0][9 * (size_t)e + i] = m.real();
+            mt[1][9 * (size_t)e + i] = m.imag();
+        }
+    }
+    ensure_ned_tables(P, true);
+    const int nl = P->n_local;
+    const int64_t nnz = P->full.m.nnz;
+    P->d_gterm.alloc((size_t)12 * T);
+    DBuf<double> dk, dm, vals[2];
+    std::vector<double> part[2];
+    const int gy = (nl * nl + 255) / 256;
+    for (int c = 0; c < 2; ++c) {
+        dk.upload(kt[c], s);
+        dm.upload(mt[c], s);
+        hipLaunchKernelGGL(k_ned_terms, dim3((T + kBlockA - 1) / kBlockA), dim3(kBlockA), 0, s,
+                           T, P->d_cells_c.p, P->d_vertices.p, P->d_perm.p, nullptr, dk.p, 1.0,
+                           0.0, P->d_gterm.p, P->d_err.p, dm.p);
+        vals[c].alloc(std::max<int64_t>(nnz, 1));
+        for (int col = 0; col < P->n_colors; ++col) {
+            const int p0 = P->color_start[col], cnt = P->color_start[col + 1] - p0;
+            if (cnt == 0) continue;
+            hipLaunchKernelGGL((k_ned_assemble<double, double>),
+                               dim3((cnt + kNedCells - 1) / kNedCells, gy), dim3(256), 0, s, p0,
+                               cnt, T, nl, P->d_cls_c.p, P->d_gterm.p, P->d_ned64.p,
+                               P->d_slots_cm.p, vals[c].p);
+        }
+        FFH_HIP(hipGetLastError());
+        check_err(P, "Nedelec PML assembly");
+        part[c].resize(nnz);
+        vals[c].download(part[c].data(), nnz, s);
+        FFH_HIP(hipStreamSynchronize(s)); // dk / dm are reused for the imaginary part
+    }
+    for (int64_t k = 0; k < nnz; ++k) {
+        values[2 * k] = part[0][k];
+        values[2 * k + 1] = part[1][k];
+    }
 }
 
 // Nedelec vector load into rhs_full: tables once per problem, then g per

@@ -6,7 +6,7 @@
 ! vertices(3,nv), tetrahedra(4,T), col_ptr(n+1), ...).
 module fortfem_hip_capi
     use, intrinsic :: iso_c_binding, only: c_int, c_int64_t, c_double, c_ptr, &
-        c_char, c_null_ptr
+        c_char, c_null_ptr, c_double_complex
     implicit none
     private
 
@@ -68,7 +68,7 @@ module fortfem_hip_capi
     public :: ffh_matrix_free, ffh_matrix_download, ffh_spmv, ffh_solve_sparse
     public :: ffh_bench_spmv, ffh_set_profiling, ffh_last_solve_profile
     public :: ffh_spmv_jvp, ffh_spmv_vjp, ffh_solve_sparse_jvp, ffh_solve_sparse_vjp
-    public :: ffh_assemble_jvp, ffh_assemble_vjp
+    public :: ffh_assemble_jvp, ffh_assemble_vjp, ffh_nedelec_pml_values
 
     interface
         integer(c_int) function ffh_init(device, ctx) bind(C, name="ffh_init")
@@ -176,6 +176,18 @@ module fortfem_hip_capi
             integer(c_int), intent(out) :: n_global
             integer(c_int64_t), intent(out) :: nnz
         end function ffh_pattern_build
+
+        ! stretch: complex (3, T) (kind 0) or (3, 3, T) (kind 1); values:
+        ! complex (nnz) on the pattern's FULL CSC order
+        integer(c_int) function ffh_nedelec_pml_values(prob, kind, stretch, &
+                wave_number, values) bind(C, name="ffh_nedelec_pml_values")
+            import :: c_int, c_ptr, c_double, c_double_complex
+            type(c_ptr), value :: prob
+            integer(c_int), value :: kind
+            complex(c_double_complex), intent(in) :: stretch(*)
+            real(c_double), value :: wave_number
+            complex(c_double_complex), intent(out) :: values(*)
+        end function ffh_nedelec_pml_values
 
         integer(c_int) function ffh_problem_sizes(prob, n_global, nnz, &
                 n_free, nnz_free, n_local, n_colors) &

@@ -35,6 +35,16 @@ module fortfem_hip_types
         real(dp), allocatable :: val(:)
     end type csc_t
 
+    ! fortsparse csc_z_t: the same layout with complex values
+    type, public :: csc_z_t
+        integer :: nrow = 0
+        integer :: ncol = 0
+        integer :: nnz = 0
+        integer, allocatable :: col_ptr(:)
+        integer, allocatable :: row_idx(:)
+        complex(dp), allocatable :: val(:)
+    end type csc_z_t
+
     type, public :: fortsparse_status_t
         integer :: code = 0
         character(len=256) :: msg = ""
@@ -481,13 +491,16 @@ end module fortfem_assembly_tetra_lagrange_arbitrary_order_3d
 ! first-kind Nedelec CSC assembly, orders 1..6, on the GPU.
 module fortfem_assembly_tetra_nedelec_3d
     use, intrinsic :: iso_c_binding, only: c_int, c_int64_t, c_double, c_ptr, c_null_ptr, c_loc
-    use fortfem_hip_types, only: dp, csc_t, fortsparse_status_t, status_set, &
+    use fortfem_hip_types, only: dp, csc_t, csc_z_t, fortsparse_status_t, status_set, &
         FORTSPARSE_INVALID_MATRIX
     use fortfem_hip_capi
     use fortfem_hip_runtime, only: hip_context
     use fortfem_assembly_tetra_lagrange_arbitrary_order_3d, only: download_csc
     implicit none
     private
+
+    public :: assemble_tetra_nedelec_pml_csc
+    public :: assemble_tetra_nedelec_curvilinear_pml_csc
 
     public :: assemble_tetra_nedelec_curl_mass_csc
     public :: assemble_tetra_nedelec_curl_mass_csc_jvp
@@ -666,6 +679,111 @@ contains
         if (code == FFH_OK) call download_csc(mat, matrix, ok)
         code = ffh_problem_free(prob)
     end subroutine nedelec_csc
+
+    ! Complex PML operator on the Nedelec pattern: the pattern comes from the
+    ! device matrix, the complex values from ffh_nedelec_pml_values.
+    subroutine nedelec_pml(mesh_vertices, tetrahedra, k, kind, stretch, &
+            wave_number, matrix, ok)
+        real(dp), intent(in) :: mesh_vertices(:, :)
+        integer, intent(in) :: tetrahedra(:, :)
+        integer, intent(in) :: k
+        integer(c_int), intent(in) :: kind
+        complex(dp), intent(in) :: stretch(*)
+        real(dp), intent(in) :: wave_number
+        type(csc_z_t), intent(out) :: matrix
+        logical, intent(out) :: ok
+        type(c_ptr) :: prob, mat
+        type(csc_t) :: pattern
+        integer(c_int) :: n_global, code
+        integer(c_int64_t) :: nnz
+
+        ok = .false.
+        code = ffh_problem_create(hip_context(), 3_c_int, &
+            int(size(mesh_vertices, 2), c_int), mesh_vertices, &
+            int(size(tetrahedra, 2), c_int), tetrahedra, prob)
+        if (code /= FFH_OK) return
+        code = ffh_pattern_build(prob, FFH_FAMILY_NEDELEC, int(k, c_int), &
+            n_global, nnz)
+        if (code == FFH_OK) then
+            allocate(matrix%val(max(nnz, 1_c_int64_t)))
+            code = ffh_nedelec_pml_values(prob, kind, stretch, &
+                real(wave_number, c_double), matrix%val)
+        end if
+        if (code == FFH_OK) code = ffh_problem_matrix(prob, FFH_FULL, mat)
+        if (code == FFH_OK) call download_csc(mat, pattern, ok)
+        code = ffh_problem_free(prob)
+        if (.not. ok) return
+        matrix%nrow = pattern%nrow
+        matrix%ncol = pattern%ncol
+        matrix%nnz = pattern%nnz
+        call move_alloc(pattern%col_ptr, matrix%col_ptr)
+        call move_alloc(pattern%row_idx, matrix%row_idx)
+        if (size(matrix%val) /= matrix%nnz) matrix%val = matrix%val(1:matrix%nnz)
+    end subroutine nedelec_pml
+
+    logical function pml_input_ok(mesh_vertices, tetrahedra, order, &
+            wave_number)
+        real(dp), intent(in) :: mesh_vertices(:, :)
+        integer, intent(in) :: tetrahedra(:, :), order
+        real(dp), intent(in) :: wave_number
+
+        pml_input_ok = .false.
+        if (size(mesh_vertices, 1) /= 3) return
+        if (size(tetrahedra, 1) /= 4 .or. size(tetrahedra, 2) < 1) return
+        if (any(tetrahedra < 1) .or. &
+            any(tetrahedra > size(mesh_vertices, 2))) return
+        if (order < 1 .or. order > 6) return
+        if (.not. (wave_number > 0.0_dp)) return
+        pml_input_ok = .true.
+    end function pml_input_ok
+
+    ! :1613-1685.  stretch(3, n_tetrahedra): the cell's complex stretch
+    ! factors along x, y, z.
+    subroutine assemble_tetra_nedelec_pml_csc(mesh_vertices, tetrahedra, &
+            order, stretch, wave_number, matrix, status)
+        real(dp), intent(in) :: mesh_vertices(:, :)
+        integer, intent(in) :: tetrahedra(:, :), order
+        complex(dp), intent(in) :: stretch(:, :)
+        real(dp), intent(in) :: wave_number
+        type(csc_z_t), intent(out) :: matrix
+        type(fortsparse_status_t), intent(out) :: status
+        complex(dp), allocatable :: packed(:, :)
+        logical :: ok
+
+        call status_set(status, FORTSPARSE_INVALID_MATRIX, &
+            "Complex tetrahedral Nedelec PML assembly failed")
+        if (.not. pml_input_ok(mesh_vertices, tetrahedra, order, wave_number)) return
+        if (size(stretch, 1) /= 3) return
+        if (size(stretch, 2) /= size(tetrahedra, 2)) return
+        packed = stretch
+        call nedelec_pml(mesh_vertices, tetrahedra, order, 0_c_int, packed, &
+            wave_number, matrix, ok)
+        if (ok) call status_set(status, 0, "")
+    end subroutine assemble_tetra_nedelec_pml_csc
+
+    ! :1372-1444.  stretch(3, 3, n_tetrahedra): the cell's complex stretch
+    ! matrix.
+    subroutine assemble_tetra_nedelec_curvilinear_pml_csc(mesh_vertices, &
+            tetrahedra, order, stretch, wave_number, matrix, status)
+        real(dp), intent(in) :: mesh_vertices(:, :)
+        integer, intent(in) :: tetrahedra(:, :), order
+        complex(dp), intent(in) :: stretch(:, :, :)
+        real(dp), intent(in) :: wave_number
+        type(csc_z_t), intent(out) :: matrix
+        type(fortsparse_status_t), intent(out) :: status
+        complex(dp), allocatable :: packed(:, :, :)
+        logical :: ok
+
+        call status_set(status, FORTSPARSE_INVALID_MATRIX, &
+            "Curvilinear tetrahedral Nedelec PML assembly failed")
+        if (.not. pml_input_ok(mesh_vertices, tetrahedra, order, wave_number)) return
+        if (size(stretch, 1) /= 3 .or. size(stretch, 2) /= 3) return
+        if (size(stretch, 3) /= size(tetrahedra, 2)) return
+        packed = stretch
+        call nedelec_pml(mesh_vertices, tetrahedra, order, 1_c_int, packed, &
+            wave_number, matrix, ok)
+        if (ok) call status_set(status, 0, "")
+    end subroutine assemble_tetra_nedelec_curvilinear_pml_csc
 
     ! :2042-2112.  curl_coefficient and mass_coefficient default to 1, order
     ! to 1.  cell_coefficient (optional, new): per-cell curl coefficient.

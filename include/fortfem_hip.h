@@ -204,6 +204,15 @@ int ffh_assemble_vjp(ffh_problem *prob, int quadrature_degree, int coeff_kind,
                      const double *coeff, double mass, const double *values_bar,
                      double *vertices_bar, double *coeff_bar, double *mass_bar);
 
+/* Complex curl-curl PML operator on the problem's Nedelec pattern (csc_z_t
+ * values): assemble_tetra_nedelec_pml_csc (src/assembly/
+ * assembly_tetra_nedelec_3d.f90:1613-1685) for kind 0, stretch = 3 complex
+ * per cell; assemble_tetra_nedelec_curvilinear_pml_csc (:1372-1444) for
+ * kind 1, stretch = complex 3 x 3 column-major per cell.  Complex numbers
+ * are (re, im) double pairs; values receives nnz pairs in the order of the
+ * pattern (ffh_problem_matrix + ffh_matrix_download give it). */
+int ffh_nedelec_pml_values(ffh_problem *prob, int kind, const double *stretch,
+                           double wave_number, double *values);
 /* assemble_tetra_lagrange_scalar_load (:705-774) / _load_samples (:473-528)
  * with the source given as a closed form, samples or a dof vector.  For the
  * Nedelec family: assemble_tetra_nedelec_vector_load(_order)

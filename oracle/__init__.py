@@ -658,6 +658,31 @@ def nedelec_csc_vjp(k, vertices, tets, values_bar, alpha=1.0, beta=1.0, cell_alp
     return vb, ab.value, bb.value, cab
 
 
+def nedelec_pml_csc(k, vertices, tets, stretch, wave_number, kind=0):
+    """assemble_tetra_nedelec_pml_csc (kind 0, stretch (T, 3) complex) and
+    assemble_tetra_nedelec_curvilinear_pml_csc (kind 1, stretch (T, 3, 3)
+    complex, S[t][i, j] = stretch(i, j, t)) restated -> CSC with complex values."""
+    v, t = _f(vertices), _i(tets)
+    S = np.asarray(stretch, dtype=np.complex128)
+    if kind == 1:
+        S = np.transpose(S, (0, 2, 1))  # column-major 3 x 3 per cell
+    st = np.ascontiguousarray(np.stack([S.real, S.imag], axis=-1).reshape(-1))
+    n = nedelec_dof_count(k)
+    nt = t.shape[0] * n * n
+    rows = np.zeros(nt, dtype=np.int32)
+    cols = np.zeros(nt, dtype=np.int32)
+    vals = np.zeros(2 * nt)
+    nd = C.c_int(0)
+    _check(_lib().ffo_nedelec_pml_triplets(
+        C.c_int(k), C.c_int(v.shape[0]), v.ctypes.data_as(C.c_void_p), C.c_int(t.shape[0]),
+        t.ctypes.data_as(C.c_void_p), C.c_int(kind), st.ctypes.data_as(C.c_void_p), C.c_double(wave_number),
+        rows.ctypes.data_as(C.c_void_p), cols.ctypes.data_as(C.c_void_p), vals.ctypes.data_as(C.c_void_p),
+        C.byref(nd)), "nedelec_pml_triplets")
+    re = csc_from_triplet(nd.value, nd.value, rows, cols, vals[0::2])
+    im = csc_from_triplet(nd.value, nd.value, rows, cols, vals[1::2])
+    return CSC(re.n_rows, re.n_cols, re.col_ptr, re.row_idx, re.val + 1j * im.val)
+
+
 def nedelec_load_rule(k):
     """The load quadrature: tetra_degree_two_quadrature at k = 1
     (src/assembly/assembly_tetra_nedelec_3d.f90:2570-2581), the Duffy rule

@@ -153,6 +153,9 @@ int ffo_nedelec_basis_transform(int k, const int edge_orient[6],
 int ffo_nedelec_element(int k, const double *coeff, const double v[12],
                         int qdeg, double alpha, const double *tensor,
                         double beta, double *m);
+int ffo_nedelec_pml_triplets(int k, int nvert, const double *vertices, int ntet,
+                             const int *tets, int kind, const double *stretch, double k0,
+                             int *rows, int *cols, double *vals, int *ndofs);
 int ffo_nedelec_jvp_triplets(int k, int nvert, const double *vertices,
                              const double *vertices_dot, int ntet, const int *tets,
                              double alpha, double beta, double alpha_dot, double beta_dot,

@@ -21,6 +21,7 @@
  */
 #include "ffo.h"
 
+#include <complex.h>
 #include <float.h>
 #include <math.h>
 #include <quadmath.h>
@@ -1220,5 +1221,150 @@ int ffo_nedelec_vjp(int k, int nvert, const double *vertices, int ntet, const in
     }
     if (R.x) ned_rule_free(&R);
     free(gd), free(eo), free(fp), free(coeff), free(Ob), free(Eb), free(T), free(ET);
+    return st;
+}
+
+/* ------------------------------------------------------------ PML (complex) */
+/* Per-cell PML tensors, curl K and value M (complex, column-major):
+ * cartesian_curl_curl_pml_coefficients (src/boundary/cartesian_helmholtz_pml.f90:
+ * 23-41, 94-108): m_i = s_j s_k / s_i, K = diag(1/m), M = diag(m);
+ * curvilinear_curl_curl_pml_coefficients (curvilinear_helmholtz_pml.f90:
+ * 113-131 with inverse_and_determinant :202-216): K = S^T S / det S,
+ * M = det S S^-1 S^-T.  st: (re, im) pairs. */
+static int pml_tensors(int kind, const double *st, double complex K[9], double complex M[9])
+{
+    for (int i = 0; i < 9; ++i) K[i] = M[i] = 0.0;
+    if (kind == 0) {
+        double complex s[3];
+        for (int i = 0; i < 3; ++i) {
+            s[i] = st[2 * i] + I * st[2 * i + 1];
+            if (cabs(s[i]) <= DBL_MIN) return 1;
+        }
+        for (int i = 0; i < 3; ++i) {
+            double complex m = s[(i + 1) % 3] * s[(i + 2) % 3] / s[i];
+            M[4 * i] = m;
+            K[4 * i] = 1.0 / m;
+        }
+        return 0;
+    }
+    double complex S[9], cof[9], inv[9];
+    double mx = 0.0;
+    for (int i = 0; i < 9; ++i) {
+        S[i] = st[2 * i] + I * st[2 * i + 1];
+        if (cabs(S[i]) > mx) mx = cabs(S[i]);
+    }
+    for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c) {
+            int r1 = (r + 1) % 3, r2 = (r + 2) % 3, c1 = (c + 1) % 3, c2 = (c + 2) % 3;
+            cof[r + 3 * c] = S[r1 + 3 * c1] * S[r2 + 3 * c2] - S[r1 + 3 * c2] * S[r2 + 3 * c1];
+        }
+    double complex det = S[0] * cof[0] + S[3] * cof[3] + S[6] * cof[6];
+    double sc = mx > 1.0 ? mx : 1.0;
+    if (!(cabs(det) > 64.0 * DBL_EPSILON * sc * sc * sc)) return 1;
+    for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c) inv[r + 3 * c] = cof[c + 3 * r] / det;
+    for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c) {
+            double complex a = 0, b = 0;
+            for (int t = 0; t < 3; ++t) {
+                a += S[t + 3 * r] * S[t + 3 * c];
+                b += inv[r + 3 * t] * inv[c + 3 * t];
+            }
+            K[r + 3 * c] = a / det;
+            M[r + 3 * c] = det * b;
+        }
+    return 0;
+}
+
+/* PML elements (assembly_tetra_nedelec_3d.f90:826-887 cartesian, :1092-1150
+ * curvilinear): per point the covariant physical values and curls, then
+ * E(r,c) += det w (curl_r . K curl_c - k^2 val_r . M val_c). */
+static int pml_element(int k, const ned_rule *R, const double v[12], const double complex K[9],
+                       const double complex M[9], double k0, double complex *E)
+{
+    int n = ffo_nedelec_dof_count(k);
+    double jac[9], inv[9], det;
+    if (geometry(v, jac, inv, &det)) return 1;
+    double *pv = malloc(sizeof(double) * 3 * n), *pc = malloc(sizeof(double) * 3 * n);
+    for (int i = 0; i < n * n; ++i) E[i] = 0.0;
+    for (int q = 0; q < R->nq; ++q) {
+        const double *rv = R->rv + (size_t)3 * n * q, *rc = R->rc + (size_t)3 * n * q;
+        for (int j = 0; j < n; ++j)
+            for (int r = 0; r < 3; ++r) {
+                pv[3 * j + r] = inv[0 + 3 * r] * rv[3 * j] + inv[1 + 3 * r] * rv[3 * j + 1] +
+                                inv[2 + 3 * r] * rv[3 * j + 2];
+                pc[3 * j + r] = (jac[r] * rc[3 * j] + jac[r + 3] * rc[3 * j + 1] +
+                                 jac[r + 6] * rc[3 * j + 2]) / det;
+            }
+        double pw = det * R->w[q];
+        for (int c = 0; c < n; ++c) {
+            double complex kc[3], mc[3];
+            for (int i = 0; i < 3; ++i) {
+                kc[i] = K[i] * pc[3 * c] + K[i + 3] * pc[3 * c + 1] + K[i + 6] * pc[3 * c + 2];
+                mc[i] = M[i] * pv[3 * c] + M[i + 3] * pv[3 * c + 1] + M[i + 6] * pv[3 * c + 2];
+            }
+            for (int r = 0; r < n; ++r) {
+                double complex cc = pc[3 * r] * kc[0] + pc[3 * r + 1] * kc[1] + pc[3 * r + 2] * kc[2];
+                double complex mm = pv[3 * r] * mc[0] + pv[3 * r + 1] * mc[1] + pv[3 * r + 2] * mc[2];
+                E[r + (size_t)n * c] += pw * (cc - k0 * k0 * mm);
+            }
+        }
+    }
+    free(pv), free(pc);
+    return 0;
+}
+
+/* assemble_tetra_nedelec_pml_csc (:1613-1685, kind 0, stretch 3 complex
+ * per cell) and assemble_tetra_nedelec_curvilinear_pml_csc (:1372-1444,
+ * kind 1, 3 x 3 complex column-major per cell): oriented complex element
+ * matrices as triplets in the primal order; vals holds (re, im) pairs. */
+int ffo_nedelec_pml_triplets(int k, int nvert, const double *vertices, int ntet,
+                             const int *tets, int kind, const double *stretch, double k0,
+                             int *rows, int *cols, double *vals, int *ndofs)
+{
+    if (k < 1 || k0 <= 0.0 || (kind != 0 && kind != 1)) return 1;
+    int n = ffo_nedelec_dof_count(k), *gd, *eo, *fp, st = 0;
+    double *coeff;
+    if (ned_setup(k, ntet, tets, &gd, &eo, &fp, ndofs, &coeff)) st = 1;
+    double complex *E = malloc(sizeof(double complex) * n * n);
+    double *Er = malloc(sizeof(double) * n * n), *Ei = malloc(sizeof(double) * n * n);
+    double *Or = malloc(sizeof(double) * n * n), *Oi = malloc(sizeof(double) * n * n);
+    double *T = malloc(sizeof(double) * n * n), *ET = malloc(sizeof(double) * n * n);
+    ned_rule R = {0};
+    if (!st) ned_rule_init(&R, k, coeff);
+    size_t entry = 0;
+    const int per = kind == 0 ? 6 : 18;
+    for (int e = 0; e < ntet && !st; ++e) {
+        double v[12];
+        for (int i = 0; i < 4 && !st; ++i) {
+            int id = tets[4 * e + i];
+            if (id < 1 || id > nvert) st = 1;
+            else
+                for (int c = 0; c < 3; ++c) v[3 * i + c] = vertices[3 * (id - 1) + c];
+        }
+        if (st) break;
+        double complex K[9], M[9];
+        if (pml_tensors(kind, stretch + (size_t)per * e, K, M) ||
+            pml_element(k, &R, v, K, M, k0, E)) {
+            st = 1;
+            break;
+        }
+        for (int i = 0; i < n * n; ++i) {
+            Er[i] = creal(E[i]);
+            Ei[i] = cimag(E[i]);
+        }
+        orient(k, n, eo + 6 * e, fp + 12 * e, Er, Or, T, ET, 0);
+        orient(k, n, eo + 6 * e, fp + 12 * e, Ei, Oi, T, ET, 0);
+        for (int c = 0; c < n; ++c)
+            for (int r = 0; r < n; ++r, ++entry) {
+                rows[entry] = gd[(size_t)n * e + r];
+                cols[entry] = gd[(size_t)n * e + c];
+                vals[2 * entry] = Or[r + (size_t)n * c];
+                vals[2 * entry + 1] = Oi[r + (size_t)n * c];
+            }
+    }
+    if (R.x) ned_rule_free(&R);
+    free(gd), free(eo), free(fp), free(coeff), free(E), free(Er), free(Ei), free(Or), free(Oi);
+    free(T), free(ET);
     return st;
 }

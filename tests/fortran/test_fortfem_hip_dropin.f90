@@ -42,7 +42,7 @@ contains
 end module dropin_test_sources
 
 program test_fortfem_hip_dropin
-    use fortfem_hip_types, only: dp, csc_t, fortsparse_status_t
+    use fortfem_hip_types, only: dp, csc_t, csc_z_t, fortsparse_status_t
     use fortfem_hip_runtime, only: fortfem_hip_finalize
     use fortfem_structured_tetra_box_mesh, only: generate_structured_tetra_box_mesh
     use fortfem_assembly_tetra_lagrange_arbitrary_order_3d, only: &
@@ -51,7 +51,8 @@ program test_fortfem_hip_dropin
         assemble_tetra_lagrange_stiffness_csc_vjp
     use fortfem_assembly_tetra_nedelec_3d, only: assemble_tetra_nedelec_curl_mass_csc, &
         assemble_tetra_nedelec_vector_load, assemble_tetra_nedelec_curl_mass_csc_jvp, &
-        assemble_tetra_nedelec_curl_mass_csc_vjp
+        assemble_tetra_nedelec_curl_mass_csc_vjp, assemble_tetra_nedelec_pml_csc, &
+        assemble_tetra_nedelec_curvilinear_pml_csc
     use fortfem_sparse_matrix, only: spmv, spmv_jvp, spmv_vjp
     use fortfem_advanced_solvers, only: solver_options_t, solver_stats_t, &
         solver_options, solve_sparse, solve_sparse_jvp, solve_sparse_vjp
@@ -76,6 +77,7 @@ program test_fortfem_hip_dropin
     call check_h1_csc_ad()
     call check_nedelec_csc_ad()
     call check_nedelec_constant_solve()
+    call check_nedelec_pml()
     call fortfem_hip_finalize()
     if (.not. all_passed) error stop 1
     write (*, '(a)') "drop-in Fortran checks passed"
@@ -482,6 +484,79 @@ contains
         end do
         call record(ok, "Nedelec curl-mass solves succeed for k = 1..3")
     end subroutine check_nedelec_constant_solve
+
+    ! test/test_tetra_nedelec_pml_sparse.f90:20-51: on the two-tet mesh the
+    ! constant field (1, 2, -1) has PML energy -k^2/3 sum(m_i f_i^2) with
+    ! m = (s2 s3/s1, s1 s3/s2, s1 s2/s3); a diagonal curvilinear stretch gives
+    ! the cartesian matrix.
+    subroutine check_nedelec_pml()
+        use, intrinsic :: iso_c_binding, only: c_int
+        integer, parameter :: ev(2, 6) = reshape([1, 2, 1, 3, 1, 4, 2, 3, 2, 4, 3, 4], [2, 6])
+        real(dp) :: vertices(3, 5), f(3)
+        integer :: tets(4, 2), cell, e, c, j, order
+        integer(c_int) :: gd(6, 2), eo(6, 2), fp(3, 4, 2), nd
+        complex(dp) :: stretch(3, 2), full(3, 3, 2), m(3), energy, expected
+        complex(dp), allocatable :: d(:), ad(:)
+        type(csc_z_t) :: a, b
+        type(fortsparse_status_t) :: st
+        logical :: ok
+
+        vertices = reshape([0.0_dp, 0.0_dp, 0.0_dp, 1.0_dp, 0.0_dp, 0.0_dp, &
+            0.0_dp, 1.0_dp, 0.0_dp, 0.0_dp, 0.0_dp, 1.0_dp, 0.0_dp, 0.0_dp, -1.0_dp], [3, 5])
+        tets(:, 1) = [1, 2, 3, 4]
+        tets(:, 2) = [1, 3, 2, 5]
+        stretch(:, 1) = [cmplx(1.0_dp, 0.6_dp, dp), cmplx(1.0_dp, 0.0_dp, dp), &
+            cmplx(1.0_dp, 0.0_dp, dp)]
+        stretch(:, 2) = stretch(:, 1)
+        f = [1.0_dp, 2.0_dp, -1.0_dp]
+        call assemble_tetra_nedelec_pml_csc(vertices, tets, 1, stretch, 1.3_dp, a, st)
+        ok = st%code == 0
+        if (ok) ok = ffh_nedelec_dof_map(1_c_int, 2_c_int, tets, 1_c_int, gd, eo, fp, nd) == 0
+        if (ok) then
+            allocate(d(a%ncol), ad(a%nrow))
+            do cell = 1, 2
+                do e = 1, 6
+                    d(gd(e, cell)) = eo(e, cell)*dot_product(f, &
+                        vertices(:, tets(ev(2, e), cell)) - vertices(:, tets(ev(1, e), cell)))
+                end do
+            end do
+            ad = (0.0_dp, 0.0_dp)
+            do c = 1, a%ncol
+                do j = a%col_ptr(c), a%col_ptr(c + 1) - 1
+                    ad(a%row_idx(j)) = ad(a%row_idx(j)) + a%val(j)*d(c)
+                end do
+            end do
+            energy = sum(d*ad)
+            m = [stretch(2, 1)*stretch(3, 1)/stretch(1, 1), &
+                stretch(1, 1)*stretch(3, 1)/stretch(2, 1), &
+                stretch(1, 1)*stretch(2, 1)/stretch(3, 1)]
+            expected = -1.3_dp**2/3.0_dp*sum(m*f**2)
+            ok = abs(energy - expected) < 3.0e-12_dp
+        end if
+        call record(ok, "Nedelec PML constant-field energy")
+        ok = .true.
+        do order = 1, 3
+            full = (0.0_dp, 0.0_dp)
+            do cell = 1, 2
+                do e = 1, 3
+                    full(e, e, cell) = stretch(e, cell)*cmplx(1.0_dp, 0.1_dp*order, dp)
+                end do
+            end do
+            call assemble_tetra_nedelec_pml_csc(vertices, tets, order, &
+                reshape([(full(e, e, 1), e = 1, 3), (full(e, e, 2), e = 1, 3)], [3, 2]), &
+                0.7_dp, a, st)
+            ok = ok .and. st%code == 0
+            call assemble_tetra_nedelec_curvilinear_pml_csc(vertices, tets, order, full, &
+                0.7_dp, b, st)
+            ok = ok .and. st%code == 0
+            if (ok) ok = a%nnz == b%nnz .and. all(a%row_idx == b%row_idx) .and. &
+                maxval(abs(a%val - b%val)) <= 1.0e-12_dp*maxval(abs(a%val))
+        end do
+        call record(ok, "diagonal curvilinear PML equals the cartesian PML")
+        stretch(2, 1) = (0.0_dp, 0.0_dp)
+        call assemble_tetra_nedelec_pml_csc(vertices, tets, 1, stretch, 1.3_dp, a, st)
+        call record(st%code /= 0, "zero PML stretch is rejected")
+    end subroutine check_nedelec_pml
 
     subroutine check_linear_energy()
         real(dp) :: mesh_vertices(3, 5), coefficients(5), product(5)

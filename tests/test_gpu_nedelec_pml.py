@@ -1,0 +1,88 @@
+"""Complex PML Nedelec assembly on the GPU (ffh_nedelec_pml_values) against
+the oracle: pattern bit-exact, real and imaginary values to 1e-12 of the
+largest magnitude, cartesian (assembly_tetra_nedelec_3d.f90:1613-1685) and
+curvilinear (:1372-1444) stretches, orders 1-4; the reference's
+constant-field energy (test/test_tetra_nedelec_pml_sparse.f90:20-51) through
+the product path; invalid stretches rejected with a status.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+from test_oracle_nedelec_pml import REF_T, REF_V, dense, masses, whitney_dofs
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ff():
+    import fortfem_amd
+
+    return fortfem_amd
+
+
+def mesh(n, seed):
+    v, t = O.unit_box_mesh(n)
+    return v + (0.1 / n) * np.random.default_rng(seed).uniform(-1, 1, v.shape), t
+
+
+def same(a, o):
+    assert np.array_equal(a.col_ptr, o.col_ptr) and np.array_equal(a.row_idx, o.row_idx)
+    scale = np.max(np.abs(o.val))
+    assert np.max(np.abs(a.val.real - o.val.real)) <= 1e-12 * scale
+    assert np.max(np.abs(a.val.imag - o.val.imag)) <= 1e-12 * scale
+
+
+@pytest.mark.parametrize("k", [1, 2, 3, 4])
+def test_cartesian_matches_oracle(ff, k):
+    v, t = mesh(2, k)
+    rng = np.random.default_rng(10 + k)
+    s = rng.uniform(0.6, 1.4, (t.shape[0], 3)) + 1j * rng.uniform(-0.6, 0.6, (t.shape[0], 3))
+    a, st = ff.assemble_tetra_nedelec_pml_csc(v, t, k, s, 1.7)
+    assert st.code == 0
+    same(a, O.nedelec_pml_csc(k, v, t, s, 1.7))
+
+
+@pytest.mark.parametrize("k", [1, 2, 3])
+def test_curvilinear_matches_oracle(ff, k):
+    v, t = mesh(2, 20 + k)
+    rng = np.random.default_rng(30 + k)
+    S = (np.eye(3) + rng.uniform(-0.3, 0.3, (t.shape[0], 3, 3))
+         + 1j * rng.uniform(-0.4, 0.4, (t.shape[0], 3, 3)))
+    a, st = ff.assemble_tetra_nedelec_curvilinear_pml_csc(v, t, k, S, 0.8)
+    assert st.code == 0
+    same(a, O.nedelec_pml_csc(k, v, t, S, 0.8, kind=1))
+
+
+def test_larger_mesh_matches_oracle(ff):
+    v, t = mesh(6, 5)
+    rng = np.random.default_rng(6)
+    s = rng.uniform(0.6, 1.4, (t.shape[0], 3)) + 1j * rng.uniform(-0.6, 0.6, (t.shape[0], 3))
+    a, st = ff.assemble_tetra_nedelec_pml_csc(v, t, 2, s, 2.5)
+    assert st.code == 0
+    same(a, O.nedelec_pml_csc(2, v, t, s, 2.5))
+
+
+def test_reference_constant_field_energy(ff):
+    s = np.array([[1 + 0.6j, 1, 1]] * 2)
+    a, st = ff.assemble_tetra_nedelec_pml_csc(REF_V, REF_T, 1, s, 1.3)
+    assert st.code == 0
+    f = np.array([1.0, 2.0, -1.0])
+    d = whitney_dofs(REF_V, REF_T, lambda x: f)
+    energy = d @ (dense(a) @ d)
+    assert abs(energy - (-1.3 ** 2 / 3.0 * np.sum(masses(s[0]) * f ** 2))) < 3.0e-12
+    for k in (2, 3, 4):
+        a, st = ff.assemble_tetra_nedelec_pml_csc(REF_V, REF_T, k, s, 1.3)
+        assert st.code == 0 and a.nrow > 0
+
+
+def test_rejects_invalid_stretch(ff):
+    s = np.array([[1.0, 0.0, 1.0]] * 2, dtype=complex)
+    a, st = ff.assemble_tetra_nedelec_pml_csc(REF_V, REF_T, 1, s, 1.3)
+    assert a is None and st.code != 0
+    S = np.zeros((2, 3, 3), dtype=complex)
+    S[:, 0, 0] = S[:, 1, 1] = 1.0
+    a, st = ff.assemble_tetra_nedelec_curvilinear_pml_csc(REF_V, REF_T, 1, S, 1.3)
+    assert a is None and st.code != 0
+    a, st = ff.assemble_tetra_nedelec_pml_csc(REF_V, REF_T, 1, np.ones((2, 3)), -1.0)
+    assert a is None and st.code != 0
