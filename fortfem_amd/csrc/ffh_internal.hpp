@@ -287,6 +287,15 @@ void build_entry_slots(int ncell, int nloc, const std::vector<int> &gd,
 void build_slots(int ncell, int nloc, const std::vector<int> &gd,
                  const std::vector<int> &perm, const std::vector<int> &col_ptr,
                  const std::vector<int> &row_idx, std::vector<int> &slots);
+// device symbolic phase (ffh_symbolic.hip)
+int64_t device_pattern_bytes(int64_t T, int nl);
+void device_pattern(Context *ctx, int G, int64_t T, int nl, const int *d_gdc,
+                    DBuf<int> &col_ptr, DBuf<int> &row_idx, std::vector<int> &h_col_ptr);
+void device_slots_h1(Context *ctx, int64_t T, int nl, const int *d_gdc, const DBuf<int> &col_ptr,
+                     const DBuf<int> &row_idx, DBuf<int> &slots);
+void device_slots_cm(Context *ctx, int64_t T, int nl, const int *d_gdc, const DBuf<int> &col_ptr,
+                     const DBuf<int> &row_idx, int64_t nnz, const std::vector<int> &cstart,
+                     DBuf<int> &slots);
 void lagrange_boundary(int d, int ntet, const std::vector<unsigned char> &bf,
                        const std::vector<int> &gd, int n_global,
                        std::vector<int> &constrained);

@@ -888,6 +888,23 @@ static void check_err(Problem *P, const char *what)
 }
 
 // -------------------------------------------------------------- symbolic --
+// The device symbolic phase holds 16 bytes per element entry (keys and their
+// sorted copy) besides the 4-byte slots; it runs when that is under a
+// quarter of the free device memory (FFH_SYMBOLIC=host forces the host
+// path, which builds the same pattern and slots).
+static bool device_symbolic_fits(int64_t T, int nl)
+{
+    static const bool host = [] {
+        const char *e = std::getenv("FFH_SYMBOLIC");
+        return e && std::string(e) == "host";
+    }();
+    if (host) return false;
+    size_t free_b = 0, total_b = 0;
+    FFH_HIP(hipMemGetInfo(&free_b, &total_b));
+    const double need = (double)device_pattern_bytes(T, nl) + 4.0 * T * nl * nl;
+    return need < 0.25 * (double)free_b;
+}
+
 void problem_pattern(Problem *P, int family, int order)
 {
     Context *ctx = P->ctx;
@@ -931,12 +948,8 @@ void problem_pattern(Problem *P, int family, int order)
     P->order = order;
     P->n_local = nl;
     P->n_global = G;
-    std::vector<int> col_ptr, row_idx;
-    build_pattern(G, T, nl, gd, col_ptr, row_idx);
     std::vector<int> cstart, perm;
     P->n_colors = color_cells(G, T, nl, gd, cstart, perm);
-    std::vector<int> slots;
-    std::vector<int> slots_cm;
     std::vector<unsigned char> cls_c;
     if (family == FFH_FAMILY_NEDELEC) {
         // inside a colour, cells sorted by orientation class
@@ -955,9 +968,6 @@ void problem_pattern(Problem *P, int family, int order)
             }
             cls_c[pos] = (unsigned char)slot_of[c];
         }
-        build_entry_slots(T, nl, gd, perm, cstart, col_ptr, row_idx, slots_cm);
-    } else {
-        build_slots(T, nl, gd, perm, col_ptr, row_idx, slots);
     }
     P->color_start = cstart;
     P->perm = perm;
@@ -977,20 +987,42 @@ void problem_pattern(Problem *P, int family, int order)
     P->d_cells_c.upload(cells_c, s);
     P->d_gdofs_c.upload(gdofs_c, s);
     if (family == FFH_FAMILY_NEDELEC) {
-        P->d_slots.reset();
-        P->d_slots_cm.upload(slots_cm, s);
         P->d_cls_c.upload(cls_c, s);
         P->ned_rule = -1;
         P->d_ned_load_tab.reset();
-    } else {
-        P->d_slots.upload(slots, s);
-        P->d_slots_cm.reset();
     }
+    // CSC pattern and element-entry slots: on the device (radix sort of the
+    // entry keys) when the keys fit comfortably, else on the host
     DevMatrix &A = P->full.m;
+    std::vector<int> col_ptr;
+    if (device_symbolic_fits(T, nl)) {
+        device_pattern(ctx, G, T, nl, P->d_gdofs_c.p, A.col_ptr, A.row_idx, col_ptr);
+        A.nnz = (int64_t)A.row_idx.n;
+        if (family == FFH_FAMILY_NEDELEC) {
+            P->d_slots.reset();
+            device_slots_cm(ctx, T, nl, P->d_gdofs_c.p, A.col_ptr, A.row_idx, A.nnz, cstart,
+                            P->d_slots_cm);
+        } else {
+            P->d_slots_cm.reset();
+            device_slots_h1(ctx, T, nl, P->d_gdofs_c.p, A.col_ptr, A.row_idx, P->d_slots);
+        }
+    } else {
+        std::vector<int> row_idx, slots;
+        build_pattern(G, T, nl, gd, col_ptr, row_idx);
+        if (family == FFH_FAMILY_NEDELEC) {
+            build_entry_slots(T, nl, gd, perm, cstart, col_ptr, row_idx, slots);
+            P->d_slots.reset();
+            P->d_slots_cm.upload(slots, s);
+        } else {
+            build_slots(T, nl, gd, perm, col_ptr, row_idx, slots);
+            P->d_slots.upload(slots, s);
+            P->d_slots_cm.reset();
+        }
+        A.nnz = (int64_t)row_idx.size();
+        A.col_ptr.upload(col_ptr, s);
+        A.row_idx.upload(row_idx, s);
+    }
     A.n = G;
-    A.nnz = (int64_t)row_idx.size();
-    A.col_ptr.upload(col_ptr, s);
-    A.row_idx.upload(row_idx, s);
     A.val.alloc(A.nnz);
     FFH_HIP(hipMemsetAsync(A.val.p, 0, sizeof(double) * A.nnz, s));
     A.symmetric = true;
