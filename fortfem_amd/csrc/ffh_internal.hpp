@@ -287,7 +287,11 @@ void build_entry_slots(int ncell, int nloc, const std::vector<int> &gd,
 void build_slots(int ncell, int nloc, const std::vector<int> &gd,
                  const std::vector<int> &perm, const std::vector<int> &col_ptr,
                  const std::vector<int> &row_idx, std::vector<int> &slots);
-// device symbolic phase (ffh_symbolic.hip)
+// device symbolic phase (ffh_symbolic.hip); tets 1-based on the device
+void device_nedelec_dofs(Context *ctx, int k, int64_t T, const int *d_tets, std::vector<int> &gd,
+                         int &ndofs, int &nedges, int &nfaces);
+void device_lagrange_dofs(Context *ctx, int d, int64_t T, const int *d_tets,
+                          const std::vector<int> &bary, std::vector<int> &gd, int &count);
 int64_t device_pattern_bytes(int64_t T, int nl);
 void device_pattern(Context *ctx, int G, int64_t T, int nl, const int *d_gdc,
                     DBuf<int> &col_ptr, DBuf<int> &row_idx, std::vector<int> &h_col_ptr);

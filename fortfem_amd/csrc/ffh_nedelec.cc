@@ -102,6 +102,24 @@ const ClassTable &class_table()
 
 } // namespace
 
+void nedelec_cell_classes(int ntet, const int *tets, std::vector<unsigned char> &cls)
+{
+    const ClassTable &ct = class_table();
+    cls.assign(ntet, 0);
+    parallel_for(ntet, [&](int64_t lo, int64_t hi) {
+        for (int64_t e = lo; e < hi; ++e) {
+            const int *t = tets + 4 * e;
+            int rk[4];
+            for (int i = 0; i < 4; ++i) {
+                rk[i] = 0;
+                for (int j = 0; j < 4; ++j) rk[i] += t[j] < t[i];
+            }
+            cls[e] = (unsigned char)ct.code_to_class[rk[0] | (rk[1] << 2) | (rk[2] << 4) |
+                                                     (rk[3] << 6)];
+        }
+    });
+}
+
 int nedelec_dof_map(int k, int ntet, const int *tets, std::vector<int> &gd,
                     std::vector<unsigned char> &cls, int &ndofs, int &nedges, int &nfaces)
 {

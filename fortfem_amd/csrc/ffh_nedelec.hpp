@@ -21,6 +21,8 @@ int nedelec_dof_count(int k);
 // transform depend on nothing else.
 constexpr int kNedelecClasses = 24;
 
+// orientation class of every cell (vertex-id ranks), as nedelec_dof_map
+void nedelec_cell_classes(int ntet, const int *tets, std::vector<unsigned char> &cls);
 // build_tetra_nedelec_dof_map (src/elements/tetra_nedelec_global_dof_map.f90:
 // 15-105) with O(1) hashed lookups: gd (ntet x n, 0-based), cls (ntet).
 // Returns 0, or 1 for invalid cells (ids < 1 or repeated in a cell).

@@ -892,6 +892,15 @@ static void check_err(Problem *P, const char *what)
 // sorted copy) besides the 4-byte slots; it runs when that is under a
 // quarter of the free device memory (FFH_SYMBOLIC=host forces the host
 // path, which builds the same pattern and slots).
+static bool distinct_cell_vertices(int64_t T, const int *tets)
+{
+    for (int64_t e = 0; e < T; ++e)
+        for (int a = 0; a < 4; ++a)
+            for (int b = a + 1; b < 4; ++b)
+                if (tets[4 * e + a] == tets[4 * e + b]) return false;
+    return true;
+}
+
 static bool device_symbolic_fits(int64_t T, int nl)
 {
     static const bool host = [] {
@@ -918,19 +927,39 @@ void problem_pattern(Problem *P, int family, int order)
         for (size_t i = 0; i < P->cells.size(); ++i)
             FFH_REQUIRE(P->cells[i] >= 1 && P->cells[i] <= P->nvert, FFH_ERR_INVALID,
                         "cell vertex id out of range");
-        FFH_REQUIRE(lagrange_dof_map(order, T, P->cells.data(), gd, G) == 0,
-                    FFH_ERR_INVALID, "invalid tetrahedra for the dof map");
         nl = lagrange_dof_count(order);
+        if (device_symbolic_fits(T, nl)) {
+            FFH_REQUIRE(distinct_cell_vertices(T, P->cells.data()), FFH_ERR_INVALID,
+                        "invalid tetrahedra for the dof map");
+            std::vector<int> bary;
+            std::vector<double> nodes;
+            lagrange_indices(order, bary, nodes);
+            DBuf<int> dt;
+            dt.upload(P->cells, ctx->stream);
+            device_lagrange_dofs(ctx, order, T, dt.p, bary, gd, G);
+        } else {
+            FFH_REQUIRE(lagrange_dof_map(order, T, P->cells.data(), gd, G) == 0,
+                        FFH_ERR_INVALID, "invalid tetrahedra for the dof map");
+        }
     } else if (family == FFH_FAMILY_NEDELEC) {
         FFH_REQUIRE(P->dim == 3, FFH_ERR_INVALID, "Nedelec family needs a tetrahedral mesh");
         FFH_REQUIRE(order >= 1 && order <= 6, FFH_ERR_INVALID, "Nedelec order must be 1..6");
         for (size_t i = 0; i < P->cells.size(); ++i)
             FFH_REQUIRE(P->cells[i] >= 1 && P->cells[i] <= P->nvert, FFH_ERR_INVALID,
                         "cell vertex id out of range");
-        FFH_REQUIRE(nedelec_dof_map(order, T, P->cells.data(), gd, P->cell_class, G,
-                                    P->n_edges, P->n_faces) == 0,
-                    FFH_ERR_INVALID, "invalid tetrahedra for the Nedelec dof map");
         nl = nedelec_dof_count(order);
+        if (device_symbolic_fits(T, nl)) {
+            FFH_REQUIRE(distinct_cell_vertices(T, P->cells.data()), FFH_ERR_INVALID,
+                        "invalid tetrahedra for the Nedelec dof map");
+            nedelec_cell_classes(T, P->cells.data(), P->cell_class);
+            DBuf<int> dt;
+            dt.upload(P->cells, ctx->stream);
+            device_nedelec_dofs(ctx, order, T, dt.p, gd, G, P->n_edges, P->n_faces);
+        } else {
+            FFH_REQUIRE(nedelec_dof_map(order, T, P->cells.data(), gd, P->cell_class, G,
+                                        P->n_edges, P->n_faces) == 0,
+                        FFH_ERR_INVALID, "invalid tetrahedra for the Nedelec dof map");
+        }
     } else if (family == FFH_FAMILY_P1_2D) {
         FFH_REQUIRE(P->dim == 2, FFH_ERR_INVALID, "P1_2D family needs a triangle mesh");
         nl = 3;

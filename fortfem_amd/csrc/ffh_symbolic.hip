@@ -157,7 +157,277 @@ int bits_for(int64_t v)
     return b;
 }
 
+// ------------------------------------------------------------- dof maps --
+// First-appearance numbering of keyed occurrences: occurrence i (in the
+// order cells, then local entities) carries a key; entities are the distinct
+// keys, numbered in the order of their first occurrence.  A stable radix
+// sort of (key, i) puts each key's first occurrence at the head of its run;
+// the heads are flagged in occurrence order and an exclusive scan of the
+// flags numbers them.  This is the reference's numbering (its linear search
+// and the host's hash give each new key the next number, in the same order).
+
+__global__ __launch_bounds__(kSymBlock) void k_iota(int64_t n, int *__restrict__ a)
+{
+    for (int64_t t = blockIdx.x * (int64_t)kSymBlock + threadIdx.x; t < n;
+         t += (int64_t)gridDim.x * kSymBlock)
+        a[t] = (int)t;
+}
+
+__global__ __launch_bounds__(kSymBlock) void k_heads(int64_t n, const uint64_t *__restrict__ k,
+                                                     int *__restrict__ h)
+{
+    for (int64_t t = blockIdx.x * (int64_t)kSymBlock + threadIdx.x; t < n;
+         t += (int64_t)gridDim.x * kSymBlock)
+        h[t] = (t == 0 || k[t] != k[t - 1]) ? 1 : 0;
+}
+
+// run g = gid[j] - 1: its first occurrence, flagged in occurrence order
+__global__ __launch_bounds__(kSymBlock) void k_first_flags(int64_t n, const int *__restrict__ h,
+                                                           const int *__restrict__ gid,
+                                                           const int *__restrict__ occ,
+                                                           int *__restrict__ first,
+                                                           int *__restrict__ flag)
+{
+    for (int64_t t = blockIdx.x * (int64_t)kSymBlock + threadIdx.x; t < n;
+         t += (int64_t)gridDim.x * kSymBlock)
+        if (h[t]) {
+            first[gid[t] - 1] = occ[t];
+            flag[occ[t]] = 1;
+        }
+}
+
+__global__ __launch_bounds__(kSymBlock) void k_number(int64_t n, const int *__restrict__ gid,
+                                                      const int *__restrict__ occ,
+                                                      const int *__restrict__ first,
+                                                      const int *__restrict__ num,
+                                                      int *__restrict__ out)
+{
+    for (int64_t t = blockIdx.x * (int64_t)kSymBlock + threadIdx.x; t < n;
+         t += (int64_t)gridDim.x * kSymBlock)
+        out[occ[t]] = num[first[gid[t] - 1]];
+}
+
+// number[i] = first-appearance number of keys[i]; returns the count
+int first_appearance(Context *ctx, int64_t n, const uint64_t *keys, int end_bit, int *number)
+{
+    hipStream_t s = ctx->stream;
+    if (n == 0) return 0;
+    DBuf<uint64_t> ks;
+    DBuf<int> occ_in, occ, h, gid, first, flag, num;
+    ks.alloc(n), occ_in.alloc(n), occ.alloc(n), h.alloc(n), gid.alloc(n), flag.alloc(n),
+        num.alloc(n), first.alloc(n);
+    hipLaunchKernelGGL(k_iota, dim3(sym_grid(n)), dim3(kSymBlock), 0, s, n, occ_in.p);
+    size_t b1 = 0, b2 = 0, b3 = 0;
+    FFH_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, b1, keys, ks.p, occ_in.p, occ.p, n, 0,
+                                               end_bit, s));
+    FFH_HIP(hipcub::DeviceScan::InclusiveSum(nullptr, b2, h.p, gid.p, n, s));
+    FFH_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, b3, flag.p, num.p, n, s));
+    DBuf<unsigned char> tmp;
+    tmp.alloc(std::max<size_t>({b1, b2, b3, (size_t)1}));
+    FFH_HIP(hipcub::DeviceRadixSort::SortPairs(tmp.p, b1, keys, ks.p, occ_in.p, occ.p, n, 0,
+                                               end_bit, s));
+    hipLaunchKernelGGL(k_heads, dim3(sym_grid(n)), dim3(kSymBlock), 0, s, n, ks.p, h.p);
+    FFH_HIP(hipcub::DeviceScan::InclusiveSum(tmp.p, b2, h.p, gid.p, n, s));
+    hipLaunchKernelGGL(k_fill_int, dim3(sym_grid(n)), dim3(kSymBlock), 0, s, n, 0, flag.p);
+    hipLaunchKernelGGL(k_first_flags, dim3(sym_grid(n)), dim3(kSymBlock), 0, s, n, h.p, gid.p,
+                       occ.p, first.p, flag.p);
+    FFH_HIP(hipcub::DeviceScan::ExclusiveSum(tmp.p, b3, flag.p, num.p, n, s));
+    hipLaunchKernelGGL(k_number, dim3(sym_grid(n)), dim3(kSymBlock), 0, s, n, gid.p, occ.p,
+                       first.p, num.p, number);
+    FFH_HIP(hipGetLastError());
+    int count = 0;
+    FFH_HIP(hipMemcpyAsync(&count, gid.p + (n - 1), sizeof(int), hipMemcpyDeviceToHost, s));
+    FFH_HIP(hipStreamSynchronize(s));
+    return count;
+}
+
+// local edges (pairs of local vertices) and faces, in the reference order
+__constant__ int c_edge_v[6][2] = {{0, 1}, {0, 2}, {0, 3}, {1, 2}, {1, 3}, {2, 3}};
+__constant__ int c_face_v[4][3] = {{0, 1, 2}, {0, 1, 3}, {0, 2, 3}, {1, 2, 3}};
+
+__device__ __forceinline__ int local_edge(int a, int b) // a != b local positions
+{
+    const int p = min(a, b), q = max(a, b);
+    return p == 0 ? q - 1 : (p == 1 ? q + 1 : 5);
+}
+
+// edge keys (min id, max id) over (cell, local edge); tets 1-based
+__global__ __launch_bounds__(kSymBlock) void k_edge_keys(int64_t T, const int *__restrict__ tets,
+                                                         uint64_t *__restrict__ keys)
+{
+    for (int64_t t = blockIdx.x * (int64_t)kSymBlock + threadIdx.x; t < 6 * T;
+         t += (int64_t)gridDim.x * kSymBlock) {
+        const int64_t e = t / 6;
+        const int l = (int)(t - 6 * e);
+        const uint32_t a = (uint32_t)tets[4 * e + c_edge_v[l][0]];
+        const uint32_t b = (uint32_t)tets[4 * e + c_edge_v[l][1]];
+        keys[t] = ((uint64_t)min(a, b) << 32) | max(a, b);
+    }
+}
+
+// face keys (smallest id, number of the edge of the other two) over
+// (cell, local face)
+__global__ __launch_bounds__(kSymBlock) void k_face_keys(int64_t T, const int *__restrict__ tets,
+                                                         const int *__restrict__ eid,
+                                                         uint64_t *__restrict__ keys)
+{
+    for (int64_t t = blockIdx.x * (int64_t)kSymBlock + threadIdx.x; t < 4 * T;
+         t += (int64_t)gridDim.x * kSymBlock) {
+        const int64_t e = t / 4;
+        const int f = (int)(t - 4 * e);
+        int p[3] = {c_face_v[f][0], c_face_v[f][1], c_face_v[f][2]};
+        // the local position of the smallest id first
+        int m = 0;
+        for (int i = 1; i < 3; ++i)
+            if (tets[4 * e + p[i]] < tets[4 * e + p[m]]) m = i;
+        const int a = p[m], b = p[(m + 1) % 3], c = p[(m + 2) % 3];
+        keys[t] = ((uint64_t)(uint32_t)tets[4 * e + a] << 32) |
+                  (uint32_t)eid[6 * e + local_edge(b, c)];
+    }
+}
+
+__global__ __launch_bounds__(kSymBlock) void k_nedelec_gd(int64_t T, int k,
+                                                          const int *__restrict__ eid,
+                                                          const int *__restrict__ fid, int ne,
+                                                          int nf, int *__restrict__ gd)
+{
+    const int fdc = k * (k - 1), cdc = k * (k - 1) * (k - 2) / 2;
+    const int n = 6 * k + 4 * fdc + cdc;
+    const int64_t edc = (int64_t)k * ne;
+    for (int64_t t = blockIdx.x * (int64_t)kSymBlock + threadIdx.x; t < n * T;
+         t += (int64_t)gridDim.x * kSymBlock) {
+        const int64_t e = t / n;
+        const int l = (int)(t - e * n);
+        int64_t g;
+        if (l < 6 * k) {
+            g = (int64_t)eid[6 * e + l / k] * k + l % k;
+        } else if (l < 6 * k + 4 * fdc) {
+            const int q = l - 6 * k;
+            g = edc + (int64_t)fid[4 * e + q / fdc] * fdc + q % fdc;
+        } else {
+            g = edc + (int64_t)nf * fdc + e * cdc + (l - 6 * k - 4 * fdc);
+        }
+        gd[t] = (int)g;
+    }
+}
+
+// Lagrange dof keys: support (vertices of nonzero barycentric index) and the
+// indices in ascending vertex-id order.  type 0 vertex (its id), 1 edge (its
+// number, index at the smaller id), 2 face (its number, indices at the two
+// smallest ids), 3 interior (the occurrence itself: never shared).
+__global__ __launch_bounds__(kSymBlock) void k_lagrange_keys(int64_t T, int n,
+                                                             const int *__restrict__ tets,
+                                                             const int *__restrict__ bary,
+                                                             const int *__restrict__ eid,
+                                                             const int *__restrict__ fid,
+                                                             uint64_t *__restrict__ keys)
+{
+    for (int64_t t = blockIdx.x * (int64_t)kSymBlock + threadIdx.x; t < n * T;
+         t += (int64_t)gridDim.x * kSymBlock) {
+        const int64_t e = t / n;
+        const int l = (int)(t - e * n);
+        int pos[4], w[4], m = 0;
+        for (int c = 0; c < 4; ++c)
+            if (bary[4 * l + c]) {
+                pos[m] = c;
+                w[m] = bary[4 * l + c];
+                ++m;
+            }
+        // ascending vertex id
+        for (int a = 0; a < m; ++a)
+            for (int b = a + 1; b < m; ++b)
+                if (tets[4 * e + pos[b]] < tets[4 * e + pos[a]]) {
+                    int x = pos[a];
+                    pos[a] = pos[b];
+                    pos[b] = x;
+                    x = w[a];
+                    w[a] = w[b];
+                    w[b] = x;
+                }
+        uint64_t type, id, wa = 0, wb = 0;
+        if (m == 1) {
+            type = 0;
+            id = (uint64_t)(tets[4 * e + pos[0]] - 1);
+        } else if (m == 2) {
+            type = 1;
+            id = (uint64_t)eid[6 * e + local_edge(pos[0], pos[1])];
+            wa = (uint64_t)w[0];
+        } else if (m == 3) {
+            type = 2;
+            const int missing = 6 - pos[0] - pos[1] - pos[2];
+            // local face opposite `missing` in the reference order
+            const int f = 3 - missing;
+            id = (uint64_t)fid[4 * e + f];
+            wa = (uint64_t)w[0];
+            wb = (uint64_t)w[1];
+        } else {
+            type = 3;
+            id = (uint64_t)t;
+        }
+        keys[t] = (type << 62) | (id << 8) | (wa << 4) | wb;
+    }
+}
+
 } // namespace
+
+// First-appearance edge and face numbers of a tetrahedral mesh (tets
+// 1-based on the device): eid (T x 6), fid (T x 4).
+static void edge_face_numbers(Context *ctx, int64_t T, const int *d_tets, DBuf<int> &eid,
+                              DBuf<int> &fid, int &ne, int &nf)
+{
+    hipStream_t s = ctx->stream;
+    DBuf<uint64_t> keys;
+    keys.alloc(6 * T);
+    eid.alloc(6 * T);
+    fid.alloc(4 * T);
+    hipLaunchKernelGGL(k_edge_keys, dim3(sym_grid(6 * T)), dim3(kSymBlock), 0, s, T, d_tets,
+                       keys.p);
+    ne = first_appearance(ctx, 6 * T, keys.p, 64, eid.p);
+    hipLaunchKernelGGL(k_face_keys, dim3(sym_grid(4 * T)), dim3(kSymBlock), 0, s, T, d_tets,
+                       eid.p, keys.p);
+    nf = first_appearance(ctx, 4 * T, keys.p, 64, fid.p);
+}
+
+void device_nedelec_dofs(Context *ctx, int k, int64_t T, const int *d_tets, std::vector<int> &gd,
+                         int &ndofs, int &nedges, int &nfaces)
+{
+    hipStream_t s = ctx->stream;
+    DBuf<int> eid, fid, dgd;
+    edge_face_numbers(ctx, T, d_tets, eid, fid, nedges, nfaces);
+    const int fdc = k * (k - 1), cdc = k * (k - 1) * (k - 2) / 2;
+    const int n = 6 * k + 4 * fdc + cdc;
+    const int64_t total = (int64_t)k * nedges + (int64_t)nfaces * fdc + T * cdc;
+    FFH_REQUIRE(total < INT_MAX, FFH_ERR_INVALID, "Nedelec dof count exceeds int32");
+    ndofs = (int)total;
+    dgd.alloc(n * T);
+    hipLaunchKernelGGL(k_nedelec_gd, dim3(sym_grid(n * T)), dim3(kSymBlock), 0, s, T, k, eid.p,
+                       fid.p, nedges, nfaces, dgd.p);
+    FFH_HIP(hipGetLastError());
+    gd.resize((size_t)n * T);
+    dgd.download(gd.data(), gd.size(), s);
+    FFH_HIP(hipStreamSynchronize(s));
+}
+
+void device_lagrange_dofs(Context *ctx, int d, int64_t T, const int *d_tets,
+                          const std::vector<int> &bary, std::vector<int> &gd, int &count)
+{
+    hipStream_t s = ctx->stream;
+    const int n = (int)bary.size() / 4;
+    DBuf<int> eid, fid, dbary, dgd;
+    int ne = 0, nf = 0;
+    if (d >= 2) edge_face_numbers(ctx, T, d_tets, eid, fid, ne, nf);
+    dbary.upload(bary, s);
+    DBuf<uint64_t> keys;
+    keys.alloc(n * T);
+    dgd.alloc(n * T);
+    hipLaunchKernelGGL(k_lagrange_keys, dim3(sym_grid(n * T)), dim3(kSymBlock), 0, s, T, n,
+                       d_tets, dbary.p, eid.p, fid.p, keys.p);
+    FFH_HIP(hipGetLastError());
+    count = first_appearance(ctx, n * T, keys.p, 64, dgd.p);
+    gd.resize((size_t)n * T);
+    dgd.download(gd.data(), gd.size(), s);
+    FFH_HIP(hipStreamSynchronize(s));
+}
 
 int64_t device_pattern_bytes(int64_t T, int nl)
 {
