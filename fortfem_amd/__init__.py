@@ -392,6 +392,33 @@ class Problem:
                                                       float(wave_number), _ptr(out)))
         return out[0:2 * nnz:2] + 1j * out[1:2 * nnz:2]
 
+    def nedelec_pml_values_jvp(self, stretch, wave_number, stretch_dot, wave_number_dot,
+                               vertices_dot=None, curvilinear=False):
+        """Tangent of nedelec_pml_values (ffh_nedelec_pml_values_jvp) along
+        stretch_dot (the layout of stretch), wave_number_dot and vertices_dot
+        ((nv, 3) or None)."""
+        packed = []
+        for a in (stretch, stretch_dot):
+            S = np.asarray(a, dtype=np.complex128)
+            if curvilinear:
+                if S.shape != (self.ncell, 3, 3):
+                    raise ValueError("stretch must be (T, 3, 3)")
+                S = np.transpose(S, (0, 2, 1))
+            elif S.shape != (self.ncell, 3):
+                raise ValueError("stretch must be (T, 3)")
+            packed.append(np.ascontiguousarray(np.stack([S.real, S.imag], axis=-1).reshape(-1)))
+        vd = None
+        if vertices_dot is not None:
+            vd = _f64(np.asarray(vertices_dot, dtype=np.float64).reshape(-1))
+            if vd.shape[0] != 3 * self.nvert:
+                raise ValueError("vertices_dot must be (nv, 3)")
+        nnz = self.sizes()["nnz"]
+        out = np.zeros(2 * max(nnz, 1))
+        self._check(self._lib.ffh_nedelec_pml_values_jvp(
+            self.handle, 1 if curvilinear else 0, _ptr(packed[0]), _ptr(packed[1]), float(wave_number),
+            float(wave_number_dot), None if vd is None else _ptr(vd), _ptr(out)))
+        return out[0:2 * nnz:2] + 1j * out[1:2 * nnz:2]
+
     def load(self, quadrature_degree, kind, params):
         p = _f64(np.atleast_1d(params))
         self._check(self._lib.ffh_load(self.handle, quadrature_degree, kind, _ptr(p), p.size))
@@ -853,6 +880,47 @@ def _nedelec_pml(mesh_vertices, tetrahedra, order, stretch, wave_number, curvili
         return None, status
     a.val = vals
     return a, FortsparseStatus(0, "")
+
+
+def _nedelec_pml_jvp(mesh_vertices, tetrahedra, order, stretch, wave_number, mesh_vertices_dot, stretch_dot,
+                     wave_number_dot, curvilinear, message, ctx):
+    status = FortsparseStatus(FORTSPARSE_INVALID_MATRIX, message)
+    v = np.asarray(mesh_vertices)
+    t = np.asarray(tetrahedra)
+    if order < 1 or wave_number <= 0 or v.ndim != 2 or v.shape[1] != 3 or t.ndim != 2 or t.shape[1] != 4:
+        return None, status
+    try:
+        p = Problem(v, t, ctx)
+        p.pattern_build(capi.FAMILY_NEDELEC, order)
+        vals = p.nedelec_pml_values_jvp(stretch, wave_number, stretch_dot, wave_number_dot,
+                                        mesh_vertices_dot, curvilinear)
+        a = p.matrix(capi.FULL).download()
+    except (FFHError, ValueError) as e:
+        if isinstance(e, FFHError) and e.code == capi.FFH_ERR_DEVICE:
+            raise
+        return None, status
+    a.val = vals
+    return a, FortsparseStatus(0, "")
+
+
+def assemble_tetra_nedelec_pml_csc_jvp(mesh_vertices, tetrahedra, order, stretch, wave_number, mesh_vertices_dot,
+                                       stretch_dot, wave_number_dot, ctx: Context | None = None):
+    """assemble_tetra_nedelec_pml_csc_jvp (assembly_tetra_nedelec_3d.f90:
+    1687-1768) -> (matrix_dot: CSC with complex values on the primal pattern,
+    status)."""
+    return _nedelec_pml_jvp(mesh_vertices, tetrahedra, order, stretch, wave_number, mesh_vertices_dot,
+                            stretch_dot, wave_number_dot, False,
+                            "Complex tetrahedral Nedelec PML JVP failed", ctx)
+
+
+def assemble_tetra_nedelec_curvilinear_pml_csc_jvp(mesh_vertices, tetrahedra, order, stretch, wave_number,
+                                                   mesh_vertices_dot, stretch_dot, wave_number_dot,
+                                                   ctx: Context | None = None):
+    """assemble_tetra_nedelec_curvilinear_pml_csc_jvp (:1446-1521); stretch
+    and stretch_dot (T, 3, 3) complex."""
+    return _nedelec_pml_jvp(mesh_vertices, tetrahedra, order, stretch, wave_number, mesh_vertices_dot,
+                            stretch_dot, wave_number_dot, True,
+                            "Curvilinear tetrahedral Nedelec PML JVP failed", ctx)
 
 
 def assemble_tetra_nedelec_pml_csc(mesh_vertices, tetrahedra, order, stretch, wave_number,

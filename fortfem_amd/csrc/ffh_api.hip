@@ -427,6 +427,19 @@ int ffh_nedelec_pml_values(ffh_problem *P, int kind, const double *stretch, doub
     return guarded(P->ctx, [&] { problem_nedelec_pml(P, kind, stretch, wave_number, values); });
 }
 
+int ffh_nedelec_pml_values_jvp(ffh_problem *P, int kind, const double *stretch,
+                               const double *stretch_dot, double wave_number,
+                               double wave_number_dot, const double *vertices_dot,
+                               double *values_dot)
+{
+    if (!P) return FFH_ERR_HANDLE;
+    if (!stretch || !stretch_dot || !values_dot) return FFH_ERR_INVALID;
+    return guarded(P->ctx, [&] {
+        problem_nedelec_pml_jvp(P, kind, stretch, stretch_dot, wave_number, wave_number_dot,
+                                vertices_dot, values_dot);
+    });
+}
+
 int ffh_load(ffh_problem *P, int qdeg, int kind, const double *params, int64_t np)
 {
     if (!P) return FFH_ERR_HANDLE;

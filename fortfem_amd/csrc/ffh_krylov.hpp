@@ -23,6 +23,9 @@ namespace ffh {
 void problem_pattern(Problem *P, int family, int order);
 void problem_nedelec_pml(Problem *P, int kind, const double *stretch, double k0,
                          double *values);
+void problem_nedelec_pml_jvp(Problem *P, int kind, const double *stretch,
+                             const double *stretch_dot, double k0, double k0_dot,
+                             const double *vertices_dot, double *values_dot);
 void problem_assemble_jvp(Problem *P, int qdeg, int coeff_kind, const double *coeff,
                           double mass, const double *vertices_dot, const double *coeff_dot,
                           double mass_dot, double *values_dot);

@@ -1622,6 +1622,202 @@ void problem_nedelec_pml(Problem *P, int kind, const double *stretch, double k0,
     }
 }
 
+// Tangent of the complex PML operator (assemble_tetra_nedelec_pml_csc_jvp,
+// assembly_tetra_nedelec_3d.f90:1687-1768; curvilinear :1446-1521) along
+// (vertices_dot, stretch_dot, k0_dot).  The operator is linear in its twelve
+// per-cell terms c (curl: J^T K J / det; value: det J^-1 M' J^-T with
+// M' = -k0^2 M), so the tangent is the same class-table assembly applied to
+// the terms' tangents dc, formed per cell in complex arithmetic on the host:
+//   dK, dM from the stretch tangent (cartesian: dm_i = m_i (ds_j/s_j +
+//   ds_k/s_k - ds_i/s_i), dK_ii = -dm_i / m_i^2; curvilinear: dK = (dS^T S +
+//   S^T dS)/det S - K ddet/det, dM = ddet S^-1 S^-T + det (dI S^-T + S^-1 dI^T),
+//   dI = -S^-1 dS S^-1, ddet = det tr(S^-1 dS)),
+//   dM' = -k0^2 dM - 2 k0 dk0 M, and the geometry tangent from dJ (the
+//   vertex tangents of the cell's edges), ddetJ = det tr(J^-1 dJ),
+//   d(J^-1) = -J^-1 dJ J^-1.
+void problem_nedelec_pml_jvp(Problem *P, int kind, const double *stretch,
+                             const double *stretch_dot, double k0, double k0_dot,
+                             const double *vertices_dot, double *values_dot)
+{
+    using cd = std::complex<double>;
+    Context *ctx = P->ctx;
+    hipStream_t s = ctx->stream;
+    FFH_REQUIRE(P->n_global > 0 && P->family == FFH_FAMILY_NEDELEC, FFH_ERR_INVALID,
+                "the PML operator needs a Nedelec pattern");
+    FFH_REQUIRE(kind == 0 || kind == 1, FFH_ERR_INVALID, "PML kind is 0 or 1");
+    FFH_REQUIRE(k0 > 0.0 && std::isfinite(k0_dot), FFH_ERR_INVALID,
+                "wave number must be positive");
+    const int T = P->ncell;
+    const int per = kind == 0 ? 3 : 9;
+    std::vector<double> gt[2];
+    gt[0].assign((size_t)12 * T, 0.0);
+    gt[1].assign((size_t)12 * T, 0.0);
+    const int pa[6] = {0, 1, 2, 0, 0, 1}, pb[6] = {0, 1, 2, 1, 2, 2};
+    auto mul = [](const cd *A, const cd *B, cd *C) { // column-major 3x3
+        for (int r = 0; r < 3; ++r)
+            for (int c = 0; c < 3; ++c) {
+                cd a = 0.0;
+                for (int t = 0; t < 3; ++t) a += A[r + 3 * t] * B[t + 3 * c];
+                C[r + 3 * c] = a;
+            }
+    };
+    auto tr = [](const cd *A, cd *B) {
+        for (int r = 0; r < 3; ++r)
+            for (int c = 0; c < 3; ++c) B[r + 3 * c] = A[c + 3 * r];
+    };
+    auto inv3 = [](const cd *A, cd *I, cd &det) {
+        cd cof[9];
+        for (int r = 0; r < 3; ++r)
+            for (int c = 0; c < 3; ++c) {
+                const int r1 = (r + 1) % 3, r2 = (r + 2) % 3, c1 = (c + 1) % 3, c2 = (c + 2) % 3;
+                cof[r + 3 * c] = A[r1 + 3 * c1] * A[r2 + 3 * c2] - A[r1 + 3 * c2] * A[r2 + 3 * c1];
+            }
+        det = A[0] * cof[0] + A[3] * cof[3] + A[6] * cof[6];
+        for (int r = 0; r < 3; ++r)
+            for (int c = 0; c < 3; ++c) I[r + 3 * c] = cof[c + 3 * r] / det;
+    };
+    for (int pos = 0; pos < T; ++pos) {
+        const int e = P->perm[pos];
+        // coefficient tensors and their tangents
+        cd K[9], M[9], dK[9], dM[9];
+        for (int i = 0; i < 9; ++i) K[i] = M[i] = dK[i] = dM[i] = 0.0;
+        const double *st = stretch + 2 * (size_t)per * e;
+        const double *sd = stretch_dot + 2 * (size_t)per * e;
+        if (kind == 0) {
+            cd sv[3], sdv[3];
+            for (int i = 0; i < 3; ++i) {
+                sv[i] = cd(st[2 * i], st[2 * i + 1]);
+                sdv[i] = cd(sd[2 * i], sd[2 * i + 1]);
+                FFH_REQUIRE(std::abs(sv[i]) > DBL_MIN, FFH_ERR_INVALID, "zero PML stretch");
+            }
+            for (int i = 0; i < 3; ++i) {
+                const int j = (i + 1) % 3, k = (i + 2) % 3;
+                const cd m = sv[j] * sv[k] / sv[i];
+                const cd dm = m * (sdv[j] / sv[j] + sdv[k] / sv[k] - sdv[i] / sv[i]);
+                M[4 * i] = m;
+                dM[4 * i] = dm;
+                K[4 * i] = 1.0 / m;
+                dK[4 * i] = -dm / (m * m);
+            }
+        } else {
+            cd S[9], dS[9], Si[9], det;
+            double mx = 0.0;
+            for (int i = 0; i < 9; ++i) {
+                S[i] = cd(st[2 * i], st[2 * i + 1]);
+                dS[i] = cd(sd[2 * i], sd[2 * i + 1]);
+                mx = std::max(mx, std::abs(S[i]));
+            }
+            inv3(S, Si, det);
+            const double sc = std::max(1.0, mx);
+            FFH_REQUIRE(std::isfinite(det.real()) && std::isfinite(det.imag()) &&
+                            std::abs(det) > 64.0 * DBL_EPSILON * sc * sc * sc,
+                        FFH_ERR_INVALID, "singular PML stretch");
+            cd St[9], dSt[9], StS[9], a1[9], a2[9], SidS[9], dI[9], dIt[9], Sit[9], W[9];
+            tr(S, St);
+            tr(dS, dSt);
+            mul(St, S, StS);
+            mul(Si, dS, SidS);
+            const cd ddet = det * (SidS[0] + SidS[4] + SidS[8]);
+            mul(SidS, Si, dI); // S^-1 dS S^-1
+            for (int i = 0; i < 9; ++i) dI[i] = -dI[i];
+            tr(dI, dIt);
+            tr(Si, Sit);
+            mul(dSt, S, a1);
+            mul(St, dS, a2);
+            for (int i = 0; i < 9; ++i) {
+                K[i] = StS[i] / det;
+                dK[i] = (a1[i] + a2[i]) / det - K[i] * ddet / det;
+            }
+            mul(Si, Sit, W);
+            mul(dI, Sit, a1);
+            mul(Si, dIt, a2);
+            for (int i = 0; i < 9; ++i) {
+                M[i] = det * W[i];
+                dM[i] = ddet * W[i] + det * (a1[i] + a2[i]);
+            }
+        }
+        cd Mp[9], dMp[9];
+        for (int i = 0; i < 9; ++i) {
+            Mp[i] = -k0 * k0 * M[i];
+            dMp[i] = -k0 * k0 * dM[i] - 2.0 * k0 * k0_dot * M[i];
+        }
+        // geometry and its tangent
+        const int *cv = P->cells.data() + 4 * (size_t)e;
+        cd J[9], dJ[9];
+        for (int c = 0; c < 3; ++c)
+            for (int r = 0; r < 3; ++r) {
+                const size_t a = 3 * (size_t)(cv[c + 1] - 1) + r, o = 3 * (size_t)(cv[0] - 1) + r;
+                J[r + 3 * c] = P->vertices[a] - P->vertices[o];
+                dJ[r + 3 * c] = vertices_dot ? vertices_dot[a] - vertices_dot[o] : 0.0;
+            }
+        cd Ji[9], det;
+        inv3(J, Ji, det);
+        FFH_REQUIRE(det.real() > 0.0, FFH_ERR_INVALID, "degenerate or inverted cell");
+        cd JidJ[9], dJi[9], Jt[9], dJt[9], Jit[9], dJit[9];
+        mul(Ji, dJ, JidJ);
+        const cd ddet = det * (JidJ[0] + JidJ[4] + JidJ[8]);
+        mul(JidJ, Ji, dJi);
+        for (int i = 0; i < 9; ++i) dJi[i] = -dJi[i];
+        tr(J, Jt);
+        tr(dJ, dJt);
+        tr(Ji, Jit);
+        tr(dJi, dJit);
+        // curl terms: J^T K J / det
+        cd KJ[9], C0[9], t1[9], t2[9], t3[9], tmp[9];
+        mul(K, J, KJ);
+        mul(Jt, KJ, C0);
+        mul(dJt, KJ, t1);
+        mul(K, dJ, tmp);
+        mul(Jt, tmp, t2);
+        mul(dK, J, tmp);
+        mul(Jt, tmp, t3);
+        // value terms: det J^-1 M' J^-T
+        cd MJ[9], V0[9], u1[9], u2[9], u3[9];
+        mul(Mp, Jit, MJ);
+        mul(Ji, MJ, V0);
+        mul(dJi, MJ, u1);
+        mul(dMp, Jit, tmp);
+        mul(Ji, tmp, u2);
+        mul(Mp, dJit, tmp);
+        mul(Ji, tmp, u3);
+        for (int q = 0; q < 6; ++q) {
+            const int i = pa[q] + 3 * pb[q];
+            const cd dc = (t1[i] + t2[i] + t3[i]) / det - C0[i] * ddet / (det * det);
+            const cd dv = ddet * V0[i] + det * (u1[i] + u2[i] + u3[i]);
+            gt[0][(size_t)q * T + pos] = dc.real();
+            gt[1][(size_t)q * T + pos] = dc.imag();
+            gt[0][(size_t)(6 + q) * T + pos] = dv.real();
+            gt[1][(size_t)(6 + q) * T + pos] = dv.imag();
+        }
+    }
+    ensure_ned_tables(P, true);
+    const int nl = P->n_local;
+    const int64_t nnz = P->full.m.nnz;
+    const int gy = (nl * nl + 255) / 256;
+    DBuf<double> vals[2];
+    std::vector<double> part[2];
+    for (int c = 0; c < 2; ++c) {
+        P->d_gterm.upload(gt[c], s);
+        vals[c].alloc(std::max<int64_t>(nnz, 1));
+        for (int col = 0; col < P->n_colors; ++col) {
+            const int p0 = P->color_start[col], cnt = P->color_start[col + 1] - p0;
+            if (cnt == 0) continue;
+            hipLaunchKernelGGL((k_ned_assemble<double, double>),
+                               dim3((cnt + kNedCells - 1) / kNedCells, gy), dim3(256), 0, s, p0,
+                               cnt, T, nl, P->d_cls_c.p, P->d_gterm.p, P->d_ned64.p,
+                               P->d_slots_cm.p, vals[c].p);
+        }
+        FFH_HIP(hipGetLastError());
+        part[c].resize(nnz);
+        vals[c].download(part[c].data(), nnz, s);
+        FFH_HIP(hipStreamSynchronize(s)); // d_gterm is rewritten for the imaginary part
+    }
+    for (int64_t k = 0; k < nnz; ++k) {
+        values_dot[2 * k] = part[0][k];
+        values_dot[2 * k + 1] = part[1][k];
+    }
+}
+
 // Nedelec vector load into rhs_full: tables once per problem, then g per
 // cell, the oriented element vectors and a colour-by-colour scatter.
 static void nedelec_load(Problem *P, int kind, const double *params, int64_t np)

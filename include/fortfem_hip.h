@@ -213,6 +213,15 @@ int ffh_assemble_vjp(ffh_problem *prob, int quadrature_degree, int coeff_kind,
  * pattern (ffh_problem_matrix + ffh_matrix_download give it). */
 int ffh_nedelec_pml_values(ffh_problem *prob, int kind, const double *stretch,
                            double wave_number, double *values);
+/* Tangent of that operator (assemble_tetra_nedelec_pml_csc_jvp, :1687-1768;
+ * assemble_tetra_nedelec_curvilinear_pml_csc_jvp, :1446-1521) along the
+ * stretch tangent (same layout as stretch), the wave-number tangent and the
+ * vertex tangents (3 per vertex, or NULL for none); values_dot receives nnz
+ * complex pairs on the pattern. */
+int ffh_nedelec_pml_values_jvp(ffh_problem *prob, int kind, const double *stretch,
+                               const double *stretch_dot, double wave_number,
+                               double wave_number_dot, const double *vertices_dot,
+                               double *values_dot);
 /* assemble_tetra_lagrange_scalar_load (:705-774) / _load_samples (:473-528)
  * with the source given as a closed form, samples or a dof vector.  For the
  * Nedelec family: assemble_tetra_nedelec_vector_load(_order)

@@ -86,3 +86,46 @@ def test_rejects_invalid_stretch(ff):
     assert a is None and st.code != 0
     a, st = ff.assemble_tetra_nedelec_pml_csc(REF_V, REF_T, 1, np.ones((2, 3)), -1.0)
     assert a is None and st.code != 0
+
+
+def _fd(k, v, t, S, k0, vd, sd, kd, kind, h=1e-6):
+    ap = O.nedelec_pml_csc(k, v + h * vd, t, S + h * sd, k0 + h * kd, kind=kind)
+    am = O.nedelec_pml_csc(k, v - h * vd, t, S - h * sd, k0 - h * kd, kind=kind)
+    return ap, (ap.val - am.val) / (2 * h)
+
+
+@pytest.mark.parametrize("k", [1, 2])
+def test_cartesian_jvp_matches_central_differences(ff, k):
+    """assemble_tetra_nedelec_pml_csc_jvp (:1687-1768) against central
+    differences of the oracle's primal along (vertices, stretch, wave number)."""
+    v, t = mesh(2, 40 + k)
+    rng = np.random.default_rng(50 + k)
+    s = rng.uniform(0.6, 1.4, (t.shape[0], 3)) + 1j * rng.uniform(-0.6, 0.6, (t.shape[0], 3))
+    vd = rng.uniform(-1, 1, v.shape)
+    sd = rng.uniform(-1, 1, s.shape) + 1j * rng.uniform(-1, 1, s.shape)
+    a, st = ff.assemble_tetra_nedelec_pml_csc_jvp(v, t, k, s, 1.7, vd, sd, 0.3)
+    assert st.code == 0
+    o, fd = _fd(k, v, t, s, 1.7, vd, sd, 0.3, 0)
+    assert np.array_equal(a.col_ptr, o.col_ptr) and np.array_equal(a.row_idx, o.row_idx)
+    assert np.max(np.abs(a.val - fd)) <= 2e-7 * np.max(np.abs(fd))
+    # each direction alone, and linearity in the tangent
+    parts = [ff.assemble_tetra_nedelec_pml_csc_jvp(v, t, k, s, 1.7, vd if i == 0 else np.zeros_like(vd),
+                                                   sd if i == 1 else np.zeros_like(sd), 0.3 if i == 2 else 0.0)[0].val
+             for i in range(3)]
+    assert np.max(np.abs(sum(parts) - a.val)) <= 1e-12 * np.max(np.abs(a.val))
+
+
+@pytest.mark.parametrize("k", [1, 2])
+def test_curvilinear_jvp_matches_central_differences(ff, k):
+    """assemble_tetra_nedelec_curvilinear_pml_csc_jvp (:1446-1521)."""
+    v, t = mesh(2, 60 + k)
+    rng = np.random.default_rng(70 + k)
+    S = (np.eye(3) + rng.uniform(-0.3, 0.3, (t.shape[0], 3, 3))
+         + 1j * rng.uniform(-0.4, 0.4, (t.shape[0], 3, 3)))
+    vd = rng.uniform(-1, 1, v.shape)
+    sd = rng.uniform(-1, 1, S.shape) + 1j * rng.uniform(-1, 1, S.shape)
+    a, st = ff.assemble_tetra_nedelec_curvilinear_pml_csc_jvp(v, t, k, S, 0.8, vd, sd, -0.2)
+    assert st.code == 0
+    o, fd = _fd(k, v, t, S, 0.8, vd, sd, -0.2, 1)
+    assert np.array_equal(a.row_idx, o.row_idx)
+    assert np.max(np.abs(a.val - fd)) <= 2e-7 * np.max(np.abs(fd))
