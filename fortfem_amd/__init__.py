@@ -419,6 +419,33 @@ class Problem:
             float(wave_number_dot), None if vd is None else _ptr(vd), _ptr(out)))
         return out[0:2 * nnz:2] + 1j * out[1:2 * nnz:2]
 
+    def nedelec_pml_values_vjp(self, stretch, wave_number, values_bar, curvilinear=False):
+        """Cotangent of nedelec_pml_values (ffh_nedelec_pml_values_vjp) for
+        complex seeds values_bar (nnz) -> (vertices_bar (nv, 3), stretch_bar
+        (the stretch layout, complex), wave_number_bar)."""
+        S = np.asarray(stretch, dtype=np.complex128)
+        shape = (self.ncell, 3, 3) if curvilinear else (self.ncell, 3)
+        if S.shape != shape:
+            raise ValueError("stretch must be %s" % (shape,))
+        if curvilinear:
+            S = np.transpose(S, (0, 2, 1))
+        st = np.ascontiguousarray(np.stack([S.real, S.imag], axis=-1).reshape(-1))
+        nnz = self.sizes()["nnz"]
+        w = np.asarray(values_bar, dtype=np.complex128).reshape(-1)
+        if w.shape[0] != nnz:
+            raise ValueError("values_bar must have nnz entries")
+        wb = np.ascontiguousarray(np.stack([w.real, w.imag], axis=-1).reshape(-1))
+        vb = np.zeros(3 * self.nvert)
+        sb = np.zeros(st.shape[0])
+        kb = C.c_double(0.0)
+        self._check(self._lib.ffh_nedelec_pml_values_vjp(self.handle, 1 if curvilinear else 0, _ptr(st),
+                                                         float(wave_number), _ptr(wb), _ptr(vb), _ptr(sb),
+                                                         C.byref(kb)))
+        sbar = (sb[0::2] + 1j * sb[1::2]).reshape(S.shape)
+        if curvilinear:
+            sbar = np.transpose(sbar, (0, 2, 1))
+        return vb.reshape(self.nvert, 3), sbar, kb.value
+
     def load(self, quadrature_degree, kind, params):
         p = _f64(np.atleast_1d(params))
         self._check(self._lib.ffh_load(self.handle, quadrature_degree, kind, _ptr(p), p.size))
@@ -921,6 +948,40 @@ def assemble_tetra_nedelec_curvilinear_pml_csc_jvp(mesh_vertices, tetrahedra, or
     return _nedelec_pml_jvp(mesh_vertices, tetrahedra, order, stretch, wave_number, mesh_vertices_dot,
                             stretch_dot, wave_number_dot, True,
                             "Curvilinear tetrahedral Nedelec PML JVP failed", ctx)
+
+
+def _nedelec_pml_vjp(mesh_vertices, tetrahedra, order, stretch, wave_number, matrix_values_bar, curvilinear,
+                     message, ctx):
+    status = FortsparseStatus(FORTSPARSE_INVALID_MATRIX, message)
+    v = np.asarray(mesh_vertices)
+    t = np.asarray(tetrahedra)
+    if order < 1 or wave_number <= 0 or v.ndim != 2 or v.shape[1] != 3 or t.ndim != 2 or t.shape[1] != 4:
+        return None, None, None, status
+    try:
+        p = Problem(v, t, ctx)
+        p.pattern_build(capi.FAMILY_NEDELEC, order)
+        vb, sb, kb = p.nedelec_pml_values_vjp(stretch, wave_number, matrix_values_bar, curvilinear)
+    except (FFHError, ValueError) as e:
+        if isinstance(e, FFHError) and e.code == capi.FFH_ERR_DEVICE:
+            raise
+        return None, None, None, status
+    return vb, sb, kb, FortsparseStatus(0, "")
+
+
+def assemble_tetra_nedelec_pml_csc_vjp(mesh_vertices, tetrahedra, order, stretch, wave_number, matrix_values_bar,
+                                       ctx: Context | None = None):
+    """assemble_tetra_nedelec_pml_csc_vjp (assembly_tetra_nedelec_3d.f90:
+    1770-1868): complex seeds on the primal pattern -> (mesh_vertices_bar
+    (nv, 3), stretch_bar (T, 3) complex, wave_number_bar, status)."""
+    return _nedelec_pml_vjp(mesh_vertices, tetrahedra, order, stretch, wave_number, matrix_values_bar, False,
+                            "Complex tetrahedral Nedelec PML VJP failed", ctx)
+
+
+def assemble_tetra_nedelec_curvilinear_pml_csc_vjp(mesh_vertices, tetrahedra, order, stretch, wave_number,
+                                                   matrix_values_bar, ctx: Context | None = None):
+    """assemble_tetra_nedelec_curvilinear_pml_csc_vjp (:1523-1611)."""
+    return _nedelec_pml_vjp(mesh_vertices, tetrahedra, order, stretch, wave_number, matrix_values_bar, True,
+                            "Curvilinear tetrahedral Nedelec PML VJP failed", ctx)
 
 
 def assemble_tetra_nedelec_pml_csc(mesh_vertices, tetrahedra, order, stretch, wave_number,

@@ -50,7 +50,7 @@ EXPORTS = (
     "ffh_last_solve_profile", "ffh_matrix_spmv_variant", "ffh_nedelec_dof_map", "ffh_nedelec_basis",
     "ffh_nedelec_load_points", "ffh_spmv_jvp", "ffh_spmv_vjp", "ffh_solve_sparse_jvp",
     "ffh_solve_sparse_vjp", "ffh_assemble_jvp", "ffh_assemble_vjp", "ffh_nedelec_pml_values",
-    "ffh_nedelec_pml_values_jvp",
+    "ffh_nedelec_pml_values_jvp", "ffh_nedelec_pml_values_vjp",
     "ffh_server_create", "ffh_server_free", "ffh_server_prepare", "ffh_server_submit", "ffh_server_step",
     "ffh_server_poll", "ffh_server_wait", "ffh_server_stats", "ffh_server_progress", "ffh_problem_solution", "ffh_server_set_precision",
 )
@@ -136,6 +136,7 @@ def load() -> C.CDLL:
         "ffh_bench_spmv": (C.c_int, [vp, C.c_int, C.c_int, dp, dp]),
         "ffh_nedelec_pml_values": (C.c_int, [vp, C.c_int, vp, C.c_double, vp]),
         "ffh_nedelec_pml_values_jvp": (C.c_int, [vp, C.c_int, vp, vp, C.c_double, C.c_double, vp, vp]),
+        "ffh_nedelec_pml_values_vjp": (C.c_int, [vp, C.c_int, vp, C.c_double, vp, vp, vp, vp]),
         "ffh_assemble_jvp": (C.c_int, [vp, C.c_int, C.c_int, vp, C.c_double, vp, vp, C.c_double, vp]),
         "ffh_assemble_vjp": (C.c_int, [vp, C.c_int, C.c_int, vp, C.c_double, vp, vp, vp, dp]),
         "ffh_spmv_jvp": (C.c_int, [vp, vp, vp, vp, vp]),

@@ -427,6 +427,20 @@ int ffh_nedelec_pml_values(ffh_problem *P, int kind, const double *stretch, doub
     return guarded(P->ctx, [&] { problem_nedelec_pml(P, kind, stretch, wave_number, values); });
 }
 
+int ffh_nedelec_pml_values_vjp(ffh_problem *P, int kind, const double *stretch,
+                               double wave_number, const double *values_bar,
+                               double *vertices_bar, double *stretch_bar,
+                               double *wave_number_bar)
+{
+    if (!P) return FFH_ERR_HANDLE;
+    if (!stretch || !values_bar || !vertices_bar || !stretch_bar || !wave_number_bar)
+        return FFH_ERR_INVALID;
+    return guarded(P->ctx, [&] {
+        problem_nedelec_pml_vjp(P, kind, stretch, wave_number, values_bar, vertices_bar,
+                                stretch_bar, wave_number_bar);
+    });
+}
+
 int ffh_nedelec_pml_values_jvp(ffh_problem *P, int kind, const double *stretch,
                                const double *stretch_dot, double wave_number,
                                double wave_number_dot, const double *vertices_dot,

@@ -26,6 +26,9 @@ void problem_nedelec_pml(Problem *P, int kind, const double *stretch, double k0,
 void problem_nedelec_pml_jvp(Problem *P, int kind, const double *stretch,
                              const double *stretch_dot, double k0, double k0_dot,
                              const double *vertices_dot, double *values_dot);
+void problem_nedelec_pml_vjp(Problem *P, int kind, const double *stretch, double k0,
+                             const double *values_bar, double *vertices_bar,
+                             double *stretch_bar, double *k0_bar);
 void problem_assemble_jvp(Problem *P, int qdeg, int coeff_kind, const double *coeff,
                           double mass, const double *vertices_dot, const double *coeff_dot,
                           double mass_dot, double *values_dot);

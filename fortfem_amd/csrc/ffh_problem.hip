@@ -519,6 +519,42 @@ __global__ __launch_bounds__(kBlockA) void k_ned_terms_jvp(
 // c_s = alpha H_s / det and v_s = beta det G_s to alpha, beta and the
 // vertices.  Cells of one colour may share a vertex (the dofs sit on edges,
 // faces and interiors), so the vertex sums are atomic.
+// Cotangents of the twelve per-cell terms: gbar[s * T + pos] = the sum over
+// the cell's element entries of vbar(slot) times the class table of term s
+// (one wave per cell, as k_ned_vjp).
+__global__ __launch_bounds__(256) void k_ned_term_bar(int cnt, int ncell, int n,
+                                                      const unsigned char *__restrict__ cls_c,
+                                                      const double *__restrict__ tab,
+                                                      const int *__restrict__ slots_cm,
+                                                      const double *__restrict__ vbar,
+                                                      double *__restrict__ gbar)
+{
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int pos = blockIdx.x * 4 + wave;
+    if (pos >= cnt) return; // uniform per wave
+    const int nn = n * n, np = n * (n + 1) / 2;
+    const double *tb = tab + (size_t)cls_c[pos] * 12 * np;
+    double acc[12];
+#pragma unroll
+    for (int s = 0; s < 12; ++s) acc[s] = 0.0;
+    for (int t = lane; t < nn; t += 64) {
+        int sl = slots_cm[(size_t)pos * nn + t];
+        if (sl < 0) sl = ~sl;
+        const double e = vbar[sl];
+        const int cc = t / n, r = t - cc * n;
+        const int lo = min(r, cc), hi = max(r, cc);
+        const int u = lo * n - (lo * (lo - 1)) / 2 + (hi - lo);
+#pragma unroll
+        for (int s = 0; s < 12; ++s) acc[s] += e * tb[(size_t)s * np + u];
+    }
+#pragma unroll
+    for (int s = 0; s < 12; ++s)
+        for (int off = 32; off > 0; off >>= 1) acc[s] += __shfl_xor(acc[s], off);
+    if (lane == 0)
+#pragma unroll
+        for (int s = 0; s < 12; ++s) gbar[(size_t)s * ncell + pos] = acc[s];
+}
+
 __global__ __launch_bounds__(256) void k_ned_vjp(
     int pos0, int cnt, int ncell, int n, const unsigned char *__restrict__ cls_c,
     const int *__restrict__ cells_c, const double *__restrict__ verts,
@@ -1816,6 +1852,231 @@ void problem_nedelec_pml_jvp(Problem *P, int kind, const double *stretch,
         values_dot[2 * k] = part[0][k];
         values_dot[2 * k + 1] = part[1][k];
     }
+}
+
+// Cotangent of the complex PML operator (assemble_tetra_nedelec_pml_csc_vjp,
+// assembly_tetra_nedelec_3d.f90:1770-1868; curvilinear :1523-1611) in the
+// reference's convention: for matrix cotangent seeds w, the cotangent of an
+// input x is sum w conj(dA/dx) (its real part for the real vertices and wave
+// number; the stretch cotangent is complex).  The device contracts the seeds
+// (real and imaginary parts) with the class tables into the twelve per-cell
+// term cotangents; the host chains them per cell to J, K, M' = -k0^2 M and
+// on to the vertices, the wave number and the stretch (the stretch part
+// through the coefficient tangents of problem_nedelec_pml_jvp, one unit
+// direction per stretch entry).
+void problem_nedelec_pml_vjp(Problem *P, int kind, const double *stretch, double k0,
+                             const double *values_bar, double *vertices_bar,
+                             double *stretch_bar, double *k0_bar)
+{
+    using cd = std::complex<double>;
+    Context *ctx = P->ctx;
+    hipStream_t s = ctx->stream;
+    FFH_REQUIRE(P->n_global > 0 && P->family == FFH_FAMILY_NEDELEC, FFH_ERR_INVALID,
+                "the PML operator needs a Nedelec pattern");
+    FFH_REQUIRE(kind == 0 || kind == 1, FFH_ERR_INVALID, "PML kind is 0 or 1");
+    FFH_REQUIRE(k0 > 0.0, FFH_ERR_INVALID, "wave number must be positive");
+    const int T = P->ncell, nl = P->n_local;
+    const int64_t nnz = P->full.m.nnz;
+    const int per = kind == 0 ? 3 : 9;
+    ensure_ned_tables(P, true);
+    std::vector<double> gb[2];
+    {
+        std::vector<double> h(std::max<int64_t>(nnz, 1));
+        DBuf<double> vb, gbar;
+        gbar.alloc((size_t)12 * T);
+        for (int c = 0; c < 2; ++c) {
+            for (int64_t k = 0; k < nnz; ++k) h[k] = values_bar[2 * k + c];
+            vb.upload(h.data(), h.size(), s);
+            hipLaunchKernelGGL(k_ned_term_bar, dim3((T + 3) / 4), dim3(256), 0, s, T, T, nl,
+                               P->d_cls_c.p, P->d_ned64.p, P->d_slots_cm.p, vb.p, gbar.p);
+            FFH_HIP(hipGetLastError());
+            gb[c].resize((size_t)12 * T);
+            gbar.download(gb[c].data(), gb[c].size(), s);
+            FFH_HIP(hipStreamSynchronize(s)); // vb is rewritten for the imaginary part
+        }
+    }
+    auto mul = [](const cd *A, const cd *B, cd *C) {
+        for (int r = 0; r < 3; ++r)
+            for (int c = 0; c < 3; ++c) {
+                cd a = 0.0;
+                for (int t = 0; t < 3; ++t) a += A[r + 3 * t] * B[t + 3 * c];
+                C[r + 3 * c] = a;
+            }
+    };
+    auto tr = [](const cd *A, cd *B) {
+        for (int r = 0; r < 3; ++r)
+            for (int c = 0; c < 3; ++c) B[r + 3 * c] = A[c + 3 * r];
+    };
+    auto cj = [](const cd *A, cd *B) {
+        for (int i = 0; i < 9; ++i) B[i] = std::conj(A[i]);
+    };
+    auto inv3 = [](const cd *A, cd *I, cd &det) {
+        cd cof[9];
+        for (int r = 0; r < 3; ++r)
+            for (int c = 0; c < 3; ++c) {
+                const int r1 = (r + 1) % 3, r2 = (r + 2) % 3, c1 = (c + 1) % 3, c2 = (c + 2) % 3;
+                cof[r + 3 * c] = A[r1 + 3 * c1] * A[r2 + 3 * c2] - A[r1 + 3 * c2] * A[r2 + 3 * c1];
+            }
+        det = A[0] * cof[0] + A[3] * cof[3] + A[6] * cof[6];
+        for (int r = 0; r < 3; ++r)
+            for (int c = 0; c < 3; ++c) I[r + 3 * c] = cof[c + 3 * r] / det;
+    };
+    // K, M of a stretch, and their tangents along dS (same formulas as the JVP)
+    auto coeffs = [&](const cd *sv, const cd *dsv, cd *K, cd *M, cd *dK, cd *dM) {
+        for (int i = 0; i < 9; ++i) K[i] = M[i] = dK[i] = dM[i] = 0.0;
+        if (kind == 0) {
+            for (int i = 0; i < 3; ++i) {
+                const int j = (i + 1) % 3, k = (i + 2) % 3;
+                const cd m = sv[j] * sv[k] / sv[i];
+                const cd dm = m * (dsv[j] / sv[j] + dsv[k] / sv[k] - dsv[i] / sv[i]);
+                M[4 * i] = m;
+                dM[4 * i] = dm;
+                K[4 * i] = 1.0 / m;
+                dK[4 * i] = -dm / (m * m);
+            }
+            return;
+        }
+        cd Si[9], det, St[9], dSt[9], StS[9], a1[9], a2[9], SidS[9], dI[9], dIt[9], Sit[9], W[9];
+        inv3(sv, Si, det);
+        tr(sv, St);
+        tr(dsv, dSt);
+        mul(St, sv, StS);
+        mul(Si, dsv, SidS);
+        const cd ddet = det * (SidS[0] + SidS[4] + SidS[8]);
+        mul(SidS, Si, dI);
+        for (int i = 0; i < 9; ++i) dI[i] = -dI[i];
+        tr(dI, dIt);
+        tr(Si, Sit);
+        mul(dSt, sv, a1);
+        mul(St, dsv, a2);
+        for (int i = 0; i < 9; ++i) {
+            K[i] = StS[i] / det;
+            dK[i] = (a1[i] + a2[i]) / det - K[i] * ddet / det;
+        }
+        mul(Si, Sit, W);
+        mul(dI, Sit, a1);
+        mul(Si, dIt, a2);
+        for (int i = 0; i < 9; ++i) {
+            M[i] = det * W[i];
+            dM[i] = ddet * W[i] + det * (a1[i] + a2[i]);
+        }
+    };
+    const int pa[6] = {0, 1, 2, 0, 0, 1}, pb[6] = {0, 1, 2, 1, 2, 2};
+    std::fill(vertices_bar, vertices_bar + 3 * (size_t)P->nvert, 0.0);
+    std::fill(stretch_bar, stretch_bar + 2 * (size_t)per * T, 0.0);
+    double kbar = 0.0;
+    for (int pos = 0; pos < T; ++pos) {
+        const int e = P->perm[pos];
+        const double *st = stretch + 2 * (size_t)per * e;
+        cd sv[9], zero[9];
+        for (int i = 0; i < per; ++i) {
+            sv[i] = cd(st[2 * i], st[2 * i + 1]);
+            zero[i] = 0.0;
+        }
+        if (kind == 0) {
+            for (int i = 0; i < 3; ++i)
+                FFH_REQUIRE(std::abs(sv[i]) > DBL_MIN, FFH_ERR_INVALID, "zero PML stretch");
+        } else {
+            cd Si[9], d;
+            inv3(sv, Si, d);
+            double mx = 0.0;
+            for (int i = 0; i < 9; ++i) mx = std::max(mx, std::abs(sv[i]));
+            const double sc = std::max(1.0, mx);
+            FFH_REQUIRE(std::isfinite(d.real()) && std::isfinite(d.imag()) &&
+                            std::abs(d) > 64.0 * DBL_EPSILON * sc * sc * sc,
+                        FFH_ERR_INVALID, "singular PML stretch");
+        }
+        cd K[9], M[9], dK[9], dM[9];
+        coeffs(sv, zero, K, M, dK, dM);
+        cd Mp[9];
+        for (int i = 0; i < 9; ++i) Mp[i] = -k0 * k0 * M[i];
+        // term cotangents as (upper-triangle) 3x3 matrices
+        cd Cb[9], Vb[9];
+        for (int i = 0; i < 9; ++i) Cb[i] = Vb[i] = 0.0;
+        for (int q = 0; q < 6; ++q) {
+            const int i = pa[q] + 3 * pb[q];
+            Cb[i] = cd(gb[0][(size_t)q * T + pos], gb[1][(size_t)q * T + pos]);
+            Vb[i] = cd(gb[0][(size_t)(6 + q) * T + pos], gb[1][(size_t)(6 + q) * T + pos]);
+        }
+        const int *cv = P->cells.data() + 4 * (size_t)e;
+        cd J[9], Ji[9], det;
+        for (int c = 0; c < 3; ++c)
+            for (int r = 0; r < 3; ++r)
+                J[r + 3 * c] = P->vertices[3 * (size_t)(cv[c + 1] - 1) + r] -
+                               P->vertices[3 * (size_t)(cv[0] - 1) + r];
+        inv3(J, Ji, det);
+        FFH_REQUIRE(det.real() > 0.0, FFH_ERR_INVALID, "degenerate or inverted cell");
+        cd Jt[9], Jit[9], tmp[9], tmp2[9];
+        tr(J, Jt);
+        tr(Ji, Jit);
+        // K_bar = J Cb J^T / det ; M'_bar = det J^-T Vb J^-1
+        cd Kb[9], Mpb[9];
+        mul(J, Cb, tmp);
+        mul(tmp, Jt, Kb);
+        for (int i = 0; i < 9; ++i) Kb[i] /= det;
+        mul(Jit, Vb, tmp);
+        mul(tmp, Ji, Mpb);
+        for (int i = 0; i < 9; ++i) Mpb[i] *= det;
+        // J_bar from the curl terms: Re[(K* J Cb^T + K*^T J Cb)/det - tr(Cb^T C*) J^-T]
+        cd Kc[9], Kct[9], Cbt[9], C0[9], Cc[9], Jb[9];
+        cj(K, Kc);
+        tr(Kc, Kct);
+        tr(Cb, Cbt);
+        mul(K, J, tmp);
+        mul(Jt, tmp, C0);
+        for (int i = 0; i < 9; ++i) C0[i] /= det;
+        cj(C0, Cc);
+        cd trc = 0.0;
+        for (int i = 0; i < 9; ++i) trc += Cb[i] * Cc[i];
+        mul(Kc, J, tmp);
+        mul(tmp, Cbt, Jb);
+        mul(Kct, J, tmp);
+        mul(tmp, Cb, tmp2);
+        for (int i = 0; i < 9; ++i) Jb[i] = (Jb[i] + tmp2[i]) / det - trc * Jit[i];
+        // J_bar from the value terms: Re[det tr(Vb^T W) J^-T - J^-T D J^-T],
+        // W = J^-1 M'* J^-T, D = det (Vb J^-1 M'*^T + Vb^T J^-1 M'*)
+        cd Mc[9], Mct[9], W[9], Vbt[9], D[9];
+        cj(Mp, Mc);
+        tr(Mc, Mct);
+        mul(Mc, Jit, tmp);
+        mul(Ji, tmp, W);
+        cd trv = 0.0;
+        for (int i = 0; i < 9; ++i) trv += Vb[i] * W[i];
+        tr(Vb, Vbt);
+        mul(Vb, Ji, tmp);
+        mul(tmp, Mct, D);
+        mul(Vbt, Ji, tmp);
+        mul(tmp, Mc, tmp2);
+        for (int i = 0; i < 9; ++i) D[i] = det * (D[i] + tmp2[i]);
+        mul(Jit, D, tmp);
+        mul(tmp, Jit, tmp2);
+        for (int i = 0; i < 9; ++i) Jb[i] += det * trv * Jit[i] - tmp2[i];
+        for (int c = 0; c < 3; ++c)
+            for (int r = 0; r < 3; ++r) {
+                const double g = Jb[r + 3 * c].real();
+                vertices_bar[3 * (size_t)(cv[c + 1] - 1) + r] += g;
+                vertices_bar[3 * (size_t)(cv[0] - 1) + r] -= g;
+            }
+        // wave number and the M cotangent
+        cd Mb[9];
+        for (int i = 0; i < 9; ++i) {
+            kbar += (Mpb[i] * std::conj(-2.0 * k0 * M[i])).real();
+            Mb[i] = -k0 * k0 * Mpb[i];
+        }
+        // stretch: sum over entries of K_bar conj(dK/dS_pq) + M_bar conj(dM/dS_pq)
+        double *sb = stretch_bar + 2 * (size_t)per * e;
+        for (int q = 0; q < per; ++q) {
+            cd dir[9];
+            for (int i = 0; i < per; ++i) dir[i] = i == q ? 1.0 : 0.0;
+            cd K1[9], M1[9], dK1[9], dM1[9];
+            coeffs(sv, dir, K1, M1, dK1, dM1);
+            cd acc = 0.0;
+            for (int i = 0; i < 9; ++i) acc += Kb[i] * std::conj(dK1[i]) + Mb[i] * std::conj(dM1[i]);
+            sb[2 * q] += acc.real();
+            sb[2 * q + 1] += acc.imag();
+        }
+    }
+    *k0_bar = kbar;
 }
 
 // Nedelec vector load into rhs_full: tables once per problem, then g per

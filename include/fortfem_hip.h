@@ -222,6 +222,15 @@ int ffh_nedelec_pml_values_jvp(ffh_problem *prob, int kind, const double *stretc
                                const double *stretch_dot, double wave_number,
                                double wave_number_dot, const double *vertices_dot,
                                double *values_dot);
+/* Cotangent of that operator (assemble_tetra_nedelec_pml_csc_vjp, :1770-1868;
+ * assemble_tetra_nedelec_curvilinear_pml_csc_vjp, :1523-1611) for nnz
+ * complex seeds on the pattern, in the reference's convention (sum of
+ * seed * conj(d value / d input); real part for the vertices, 3 per vertex,
+ * and the wave number; stretch_bar complex in the stretch layout). */
+int ffh_nedelec_pml_values_vjp(ffh_problem *prob, int kind, const double *stretch,
+                               double wave_number, const double *values_bar,
+                               double *vertices_bar, double *stretch_bar,
+                               double *wave_number_bar);
 /* assemble_tetra_lagrange_scalar_load (:705-774) / _load_samples (:473-528)
  * with the source given as a closed form, samples or a dof vector.  For the
  * Nedelec family: assemble_tetra_nedelec_vector_load(_order)

@@ -129,3 +129,37 @@ def test_curvilinear_jvp_matches_central_differences(ff, k):
     o, fd = _fd(k, v, t, S, 0.8, vd, sd, -0.2, 1)
     assert np.array_equal(a.row_idx, o.row_idx)
     assert np.max(np.abs(a.val - fd)) <= 2e-7 * np.max(np.abs(fd))
+
+
+@pytest.mark.parametrize("kind,k", [(0, 1), (0, 2), (1, 1), (1, 2)])
+def test_vjp_is_the_adjoint_of_the_jvp(ff, kind, k):
+    """assemble_tetra_nedelec_(curvilinear_)pml_csc_vjp (:1770-1868,
+    :1523-1611) in the reference's convention (cotangent = sum seed *
+    conj(d value / d input), real part for real inputs):
+    Re sum conj(w) A_dot = <v_bar, v_dot> + Re sum conj(s_bar) s_dot + k_bar k_dot
+    for the JVP checked above against central differences."""
+    v, t = mesh(2, 80 + k)
+    rng = np.random.default_rng(90 + 10 * kind + k)
+    shape = (t.shape[0], 3) if kind == 0 else (t.shape[0], 3, 3)
+    S = rng.uniform(0.6, 1.4, shape) + 1j * rng.uniform(-0.5, 0.5, shape)
+    if kind == 1:
+        S = S * 0.3 + np.eye(3)
+    vd = rng.uniform(-1, 1, v.shape)
+    sd = rng.uniform(-1, 1, shape) + 1j * rng.uniform(-1, 1, shape)
+    kd = 0.37
+    jvp = ff.assemble_tetra_nedelec_pml_csc_jvp if kind == 0 else ff.assemble_tetra_nedelec_curvilinear_pml_csc_jvp
+    vjp = ff.assemble_tetra_nedelec_pml_csc_vjp if kind == 0 else ff.assemble_tetra_nedelec_curvilinear_pml_csc_vjp
+    a, st = jvp(v, t, k, S, 1.3, vd, sd, kd)
+    assert st.code == 0
+    w = rng.uniform(-1, 1, a.val.shape) + 1j * rng.uniform(-1, 1, a.val.shape)
+    vb, sb, kb, st = vjp(v, t, k, S, 1.3, w)
+    assert st.code == 0
+    lhs = np.real(np.sum(np.conj(w) * a.val))
+    rhs = np.sum(vb * vd) + np.real(np.sum(np.conj(sb) * sd)) + kb * kd
+    assert abs(lhs - rhs) <= 1e-10 * max(abs(lhs), 1.0)
+    # each input separately
+    for i in range(3):
+        ai, _ = jvp(v, t, k, S, 1.3, vd if i == 0 else 0 * vd, sd if i == 1 else 0 * sd, kd if i == 2 else 0.0)
+        part = [np.sum(vb * vd), np.real(np.sum(np.conj(sb) * sd)), kb * kd][i]
+        li = np.real(np.sum(np.conj(w) * ai.val))
+        assert abs(li - part) <= 1e-10 * max(abs(li), 1.0), i
