@@ -50,7 +50,8 @@ EXPORTS = (
     "ffh_last_solve_profile", "ffh_matrix_spmv_variant", "ffh_nedelec_dof_map", "ffh_nedelec_basis",
     "ffh_nedelec_load_points", "ffh_spmv_jvp", "ffh_spmv_vjp", "ffh_solve_sparse_jvp",
     "ffh_solve_sparse_vjp", "ffh_assemble_jvp", "ffh_assemble_vjp", "ffh_nedelec_pml_values",
-    "ffh_nedelec_pml_values_jvp", "ffh_nedelec_pml_values_vjp",
+    "ffh_nedelec_pml_values_jvp", "ffh_nedelec_pml_values_vjp", "ffh_zmatrix_upload",
+    "ffh_zmatrix_solve", "ffh_zmatrix_free",
     "ffh_server_create", "ffh_server_free", "ffh_server_prepare", "ffh_server_submit", "ffh_server_step",
     "ffh_server_poll", "ffh_server_wait", "ffh_server_stats", "ffh_server_progress", "ffh_problem_solution", "ffh_server_set_precision",
 )
@@ -137,6 +138,9 @@ def load() -> C.CDLL:
         "ffh_nedelec_pml_values": (C.c_int, [vp, C.c_int, vp, C.c_double, vp]),
         "ffh_nedelec_pml_values_jvp": (C.c_int, [vp, C.c_int, vp, vp, C.c_double, C.c_double, vp, vp]),
         "ffh_nedelec_pml_values_vjp": (C.c_int, [vp, C.c_int, vp, C.c_double, vp, vp, vp, vp]),
+        "ffh_zmatrix_upload": (C.c_int, [vp, C.c_int, C.c_int64, C.c_int, vp, vp, vp, C.POINTER(vp)]),
+        "ffh_zmatrix_solve": (C.c_int, [vp, vp, vp, C.c_double, C.c_int, ip, C.POINTER(C.c_double), ip]),
+        "ffh_zmatrix_free": (C.c_int, [vp]),
         "ffh_assemble_jvp": (C.c_int, [vp, C.c_int, C.c_int, vp, C.c_double, vp, vp, C.c_double, vp]),
         "ffh_assemble_vjp": (C.c_int, [vp, C.c_int, C.c_int, vp, C.c_double, vp, vp, vp, dp]),
         "ffh_spmv_jvp": (C.c_int, [vp, vp, vp, vp, vp]),

@@ -12,6 +12,10 @@ using namespace ffh;
 
 struct ffh_problem : Problem {};
 struct ffh_matrix : Matrix {};
+struct ffh_zmatrix {
+    Context *ctx = nullptr;
+    ZMatrix *m = nullptr;
+};
 
 namespace {
 thread_local std::string g_err;
@@ -425,6 +429,39 @@ int ffh_nedelec_pml_values(ffh_problem *P, int kind, const double *stretch, doub
     if (!P) return FFH_ERR_HANDLE;
     if (!stretch || !values) return FFH_ERR_INVALID;
     return guarded(P->ctx, [&] { problem_nedelec_pml(P, kind, stretch, wave_number, values); });
+}
+
+int ffh_zmatrix_upload(ffh_context *ctx, int n, int64_t nnz, int base, const int *col_ptr,
+                       const int *row_idx, const double *values, ffh_zmatrix **out)
+{
+    if (!ctx || !out) return FFH_ERR_HANDLE;
+    *out = nullptr;
+    ffh_zmatrix *z = nullptr;
+    const int code = guarded(ctx, [&] {
+        ZMatrix *m = zmatrix_create(ctx, n, nnz, base, col_ptr, row_idx, values);
+        z = new ffh_zmatrix{ctx, m};
+    });
+    *out = z;
+    return code;
+}
+
+int ffh_zmatrix_free(ffh_zmatrix *z)
+{
+    if (!z) return FFH_ERR_HANDLE;
+    zmatrix_destroy(z->m);
+    delete z;
+    return FFH_OK;
+}
+
+int ffh_zmatrix_solve(ffh_zmatrix *z, const double *b, double *x, double tolerance,
+                      int max_iterations, int *iterations, double *relative_residual,
+                      int *converged)
+{
+    if (!z) return FFH_ERR_HANDLE;
+    return guarded(z->ctx, [&] {
+        zmatrix_solve(z->m, b, x, tolerance, max_iterations, iterations, relative_residual,
+                      converged);
+    });
 }
 
 int ffh_nedelec_pml_values_vjp(ffh_problem *P, int kind, const double *stretch,

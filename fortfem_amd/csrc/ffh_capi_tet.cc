@@ -23,7 +23,8 @@ struct MeshSlot {
 struct OperatorSlot {
     bool used = false;
     ffh_problem *prob = nullptr; // assembled from a mesh, or
-    ffh_matrix *mat = nullptr;   // uploaded from CSC
+    ffh_matrix *mat = nullptr;   // uploaded from CSC, or
+    ffh_zmatrix *zmat = nullptr; // a complex CSC
     int n = 0;
 };
 
@@ -156,7 +157,7 @@ void retain_problem(int mesh, int family, int order, int qdeg, double coeff, dou
         *status = c;
         return;
     }
-    g_op[slot] = OperatorSlot{true, p, nullptr, ng};
+    g_op[slot] = OperatorSlot{true, p, nullptr, nullptr, ng};
     *handle = slot + 1;
     *n_dofs = ng;
     *status = 0;
@@ -290,7 +291,7 @@ void fortfem_real_operator_csc(int n, int nnz, const int *col_ptr, const int *ro
         *status = c;
         return;
     }
-    g_op[slot] = OperatorSlot{true, nullptr, a, n};
+    g_op[slot] = OperatorSlot{true, nullptr, a, nullptr, n};
     *handle = slot + 1;
     *status = 0;
 }
@@ -307,7 +308,7 @@ void fortfem_operator_solve(int handle, int n, double tolerance, int max_iterati
         return;
     }
     *status = -1;
-    if (n != op->n || !rhs || !solution || !(tolerance > 0.0) || max_iterations < 1 ||
+    if (op->zmat || n != op->n || !rhs || !solution || !(tolerance > 0.0) || max_iterations < 1 ||
         (precision != 0 && precision != 1))
         return;
     ffh_solver_options o;
@@ -336,6 +337,69 @@ void fortfem_operator_solve(int handle, int n, double tolerance, int max_iterati
     *status = st.converged ? 0 : 2;
 }
 
+void fortfem_complex_operator_csc(int n, int nnz, const int *col_ptr, const int *row_ind,
+                                  const double *values, int *handle, int *status)
+{
+    if (!status) return;
+    std::lock_guard<std::mutex> lk(g_lock);
+    *status = -1;
+    if (!handle) return;
+    *handle = 0;
+    if (n < 1 || nnz < 1 || !col_ptr || !row_ind || !values) return;
+    if (col_ptr[0] != 0 || col_ptr[n] != nnz) return;
+    for (int j = 0; j < n; ++j)
+        if (col_ptr[j + 1] < col_ptr[j]) return;
+    for (int k = 0; k < nnz; ++k)
+        if (row_ind[k] < 0 || row_ind[k] >= n) return;
+    const int slot = free_slot(g_op);
+    if (slot < 0) {
+        *status = -3;
+        return;
+    }
+    ffh_context *ctx = context(status);
+    if (!ctx) return;
+    ffh_zmatrix *z = nullptr;
+    const int c = ffh_zmatrix_upload(ctx, n, nnz, 0, col_ptr, row_ind, values, &z);
+    if (c != FFH_OK) {
+        *status = c;
+        return;
+    }
+    OperatorSlot o;
+    o.used = true;
+    o.zmat = z;
+    o.n = n;
+    g_op[slot] = o;
+    *handle = slot + 1;
+    *status = 0;
+}
+
+void fortfem_complex_operator_solve(int handle, int n, double tolerance, int max_iterations,
+                                    const double *rhs, double *solution, int *iterations,
+                                    int *status)
+{
+    if (!status) return;
+    std::lock_guard<std::mutex> lk(g_lock);
+    OperatorSlot *op = op_of(handle);
+    if (!op) {
+        *status = -3;
+        return;
+    }
+    *status = -1;
+    if (!op->zmat || n != op->n || !rhs || !solution || !(tolerance > 0.0) ||
+        max_iterations < 0)
+        return;
+    int it = 0, conv = 0;
+    double rel = 0.0;
+    const int c = ffh_zmatrix_solve(op->zmat, rhs, solution, tolerance, max_iterations, &it,
+                                    &rel, &conv);
+    if (iterations) *iterations = it;
+    if (c != FFH_OK) {
+        *status = c;
+        return;
+    }
+    *status = conv ? 0 : 2;
+}
+
 void fortfem_operator_free(int handle, int *status)
 {
     if (!status) return;
@@ -347,6 +411,7 @@ void fortfem_operator_free(int handle, int *status)
     }
     if (op->prob) ffh_problem_free(op->prob);
     if (op->mat) ffh_matrix_free(op->mat);
+    if (op->zmat) ffh_zmatrix_free(op->zmat);
     *op = OperatorSlot{};
     *status = 0;
 }

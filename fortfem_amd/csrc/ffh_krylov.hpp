@@ -26,6 +26,14 @@ void problem_nedelec_pml(Problem *P, int kind, const double *stretch, double k0,
 void problem_nedelec_pml_jvp(Problem *P, int kind, const double *stretch,
                              const double *stretch_dot, double k0, double k0_dot,
                              const double *vertices_dot, double *values_dot);
+// complex-symmetric operators and their Jacobi-COCG solve (ffh_complex.hip)
+struct ZMatrix;
+ZMatrix *zmatrix_create(Context *ctx, int n, int64_t nnz, int base, const int *col_ptr,
+                        const int *row_idx, const double *values);
+void zmatrix_destroy(ZMatrix *m);
+int zmatrix_size(const ZMatrix *m);
+void zmatrix_solve(ZMatrix *m, const double *b, double *x, double tolerance, int max_iterations,
+                   int *iterations, double *relative_residual, int *converged);
 void problem_nedelec_pml_vjp(Problem *P, int kind, const double *stretch, double k0,
                              const double *values_bar, double *vertices_bar,
                              double *stretch_bar, double *k0_bar);

@@ -222,6 +222,20 @@ int ffh_nedelec_pml_values_jvp(ffh_problem *prob, int kind, const double *stretc
                                const double *stretch_dot, double wave_number,
                                double wave_number_dot, const double *vertices_dot,
                                double *values_dot);
+/* A complex CSC matrix resident in HBM (values as (re, im) pairs, base 0 or
+ * 1), in place of the reference's retained complex factorisation
+ * (fortfem_complex_factor_csc, src/bindings/fortfem_capi_solver.f90:10-70),
+ * and its solve A x = b from x = 0 by Jacobi-COCG to
+ * ||b - A x|| <= tolerance ||b|| (fortfem_complex_solve, :72-124, which
+ * solves directly).  COCG needs a complex-symmetric A (A = A^T, as the PML
+ * curl-curl operators are). */
+typedef struct ffh_zmatrix ffh_zmatrix;
+int ffh_zmatrix_upload(ffh_context *ctx, int n, int64_t nnz, int base, const int *col_ptr,
+                       const int *row_idx, const double *values, ffh_zmatrix **out);
+int ffh_zmatrix_solve(ffh_zmatrix *mat, const double *b, double *x, double tolerance,
+                      int max_iterations, int *iterations, double *relative_residual,
+                      int *converged);
+int ffh_zmatrix_free(ffh_zmatrix *mat);
 /* Cotangent of that operator (assemble_tetra_nedelec_pml_csc_vjp, :1770-1868;
  * assemble_tetra_nedelec_curvilinear_pml_csc_vjp, :1523-1611) for nnz
  * complex seeds on the pattern, in the reference's convention (sum of

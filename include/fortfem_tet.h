@@ -73,6 +73,18 @@ void fortfem_real_operator_csc(int n, int nnz, const int *col_ptr, const int *ro
 void fortfem_operator_solve(int handle, int n, double tolerance, int max_iterations,
                             int precision, const double *rhs, double *solution,
                             int *iterations, int *status);
+/* A zero-based complex CSC matrix (values as (re, im) double pairs, the
+ * layout of fortfem_complex) kept resident, in place of
+ * fortfem_complex_factor_csc (include/fortfem.h); solved from x = 0 by
+ * Jacobi-COCG, which needs A = A^T (complex symmetric, as the PML curl-curl
+ * operators are), instead of fortfem_complex_solve's direct solve.  Status
+ * as fortfem_operator_solve; -1 also for a real operator's handle.
+ * fortfem_operator_free releases either kind. */
+void fortfem_complex_operator_csc(int n, int nnz, const int *col_ptr, const int *row_ind,
+                                  const double *values, int *handle, int *status);
+void fortfem_complex_operator_solve(int handle, int n, double tolerance, int max_iterations,
+                                    const double *rhs, double *solution, int *iterations,
+                                    int *status);
 void fortfem_operator_free(int handle, int *status);
 
 #ifdef __cplusplus

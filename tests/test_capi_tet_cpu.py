@@ -19,7 +19,7 @@ def lib():
 def test_every_declared_function_is_exported():
     text = open(os.path.join(ROOT, "include", "fortfem_tet.h")).read()
     names = re.findall(r"^void\s+(fortfem_\w+)\s*\(", text, re.M)
-    assert len(names) == 9
+    assert len(names) == 11
     L = lib()
     for n in names:
         assert hasattr(L, n), n
@@ -82,3 +82,22 @@ def test_invalid_input_and_handles():
     L.fortfem_real_operator_csc(2, 2, cp.ctypes.data_as(C.c_void_p), ri.ctypes.data_as(C.c_void_p),
                                 va.ctypes.data_as(C.c_void_p), C.byref(h), C.byref(st))
     assert st.value == -1 and h.value == 0
+
+
+def test_complex_operator_argument_checks():
+    """fortfem_complex_operator_csc validates the zero-based CSC like
+    valid_csc_input (fortfem_capi_solver.f90:126-142) before any device
+    call; a complex handle is refused by the real solve and vice versa."""
+    L = lib()
+    cp = np.array([0, 1, 2], dtype=np.int32)
+    ri = np.array([0, 2], dtype=np.int32)  # row out of range
+    va = np.ones(4)
+    h, st = C.c_int(7), C.c_int(9)
+    L.fortfem_complex_operator_csc(2, 2, cp.ctypes.data_as(C.c_void_p), ri.ctypes.data_as(C.c_void_p),
+                                   va.ctypes.data_as(C.c_void_p), C.byref(h), C.byref(st))
+    assert st.value == -1 and h.value == 0
+    it = C.c_int(0)
+    x = np.zeros(4)
+    L.fortfem_complex_operator_solve(5, 2, C.c_double(1e-10), 10, va.ctypes.data_as(C.c_void_p),
+                                     x.ctypes.data_as(C.c_void_p), C.byref(it), C.byref(st))
+    assert st.value == -3  # unknown handle
