@@ -491,6 +491,50 @@ class Problem:
         return x, SolverStatsT.from_c(st, method)
 
 
+class ComplexMatrix:
+    """A complex CSC matrix resident in HBM (ffh_zmatrix), in place of the
+    reference's retained complex factorisation (fortfem_complex_factor_csc);
+    solve() runs Jacobi-COCG, which needs a complex-symmetric matrix (the PML
+    curl-curl operators are)."""
+
+    def __init__(self, a: CSC, ctx: Context | None = None, base: int = 1):
+        self.ctx = ctx or default_context()
+        self._lib = capi.load()
+        self.n = int(a.ncol)
+        val = np.asarray(a.val, dtype=np.complex128)
+        va = np.ascontiguousarray(np.stack([val.real, val.imag], axis=-1).reshape(-1))
+        cp, ri = _i32(a.col_ptr), _i32(a.row_idx)
+        h = C.c_void_p()
+        self.ctx.check(self._lib.ffh_zmatrix_upload(self.ctx.handle, self.n, a.nnz, base, _ptr(cp), _ptr(ri),
+                                                    _ptr(va), C.byref(h)))
+        self.handle = h
+
+    def solve(self, b, tolerance=1e-10, max_iterations=10000):
+        """x with ||b - A x|| <= tolerance ||b|| from x = 0 -> (x, iterations,
+        relative residual, converged)."""
+        bb = np.asarray(b, dtype=np.complex128).reshape(-1)
+        if bb.shape[0] != self.n:
+            raise ValueError("b must have n entries")
+        bi = np.ascontiguousarray(np.stack([bb.real, bb.imag], axis=-1).reshape(-1))
+        x = np.zeros(2 * self.n)
+        it, conv = C.c_int(0), C.c_int(0)
+        rel = C.c_double(0.0)
+        self.ctx.check(self._lib.ffh_zmatrix_solve(self.handle, _ptr(bi), _ptr(x), float(tolerance),
+                                                   int(max_iterations), C.byref(it), C.byref(rel), C.byref(conv)))
+        return x[0::2] + 1j * x[1::2], it.value, rel.value, bool(conv.value)
+
+    def close(self):
+        if getattr(self, "handle", None):
+            self._lib.ffh_zmatrix_free(self.handle)
+        self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 class Matrix:
     """A CSC matrix resident in HBM (ffh_matrix)."""
 

@@ -163,3 +163,20 @@ def test_vjp_is_the_adjoint_of_the_jvp(ff, kind, k):
         part = [np.sum(vb * vd), np.real(np.sum(np.conj(sb) * sd)), kb * kd][i]
         li = np.real(np.sum(np.conj(w) * ai.val))
         assert abs(li - part) <= 1e-10 * max(abs(li), 1.0), i
+
+
+def test_pml_system_solved_by_cocg(ff):
+    """The device PML operator solved by ComplexMatrix (Jacobi-COCG): the
+    residual of the oracle's matrix at the computed x is below 1e-9 ||b||."""
+    v, t = mesh(2, 99)
+    rng = np.random.default_rng(99)
+    s = rng.uniform(0.8, 1.2, (t.shape[0], 3)) + 1j * rng.uniform(0.2, 0.6, (t.shape[0], 3))
+    a, st = ff.assemble_tetra_nedelec_pml_csc(v, t, 2, s, 0.9)
+    assert st.code == 0
+    b = rng.uniform(-1, 1, a.nrow) + 1j * rng.uniform(-1, 1, a.nrow)
+    m = ff.ComplexMatrix(a)
+    x, it, rel, conv = m.solve(b, tolerance=1e-11, max_iterations=50000)
+    assert conv and rel <= 1e-11
+    A = dense(O.nedelec_pml_csc(2, v, t, s, 0.9))
+    assert np.linalg.norm(A @ x - b) <= 1e-9 * np.linalg.norm(b)
+    m.close()
