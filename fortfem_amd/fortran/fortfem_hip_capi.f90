@@ -69,6 +69,7 @@ module fortfem_hip_capi
     public :: ffh_bench_spmv, ffh_set_profiling, ffh_last_solve_profile
     public :: ffh_spmv_jvp, ffh_spmv_vjp, ffh_solve_sparse_jvp, ffh_solve_sparse_vjp
     public :: ffh_assemble_jvp, ffh_assemble_vjp, ffh_nedelec_pml_values
+    public :: ffh_nedelec_pml_values_jvp, ffh_nedelec_pml_values_vjp
 
     interface
         integer(c_int) function ffh_init(device, ctx) bind(C, name="ffh_init")
@@ -188,6 +189,31 @@ module fortfem_hip_capi
             real(c_double), value :: wave_number
             complex(c_double_complex), intent(out) :: values(*)
         end function ffh_nedelec_pml_values
+
+        integer(c_int) function ffh_nedelec_pml_values_jvp(prob, kind, stretch, &
+                stretch_dot, wave_number, wave_number_dot, vertices_dot, values_dot) &
+                bind(C, name="ffh_nedelec_pml_values_jvp")
+            import :: c_int, c_ptr, c_double, c_double_complex
+            type(c_ptr), value :: prob
+            integer(c_int), value :: kind
+            complex(c_double_complex), intent(in) :: stretch(*), stretch_dot(*)
+            real(c_double), value :: wave_number, wave_number_dot
+            real(c_double), intent(in) :: vertices_dot(*)
+            complex(c_double_complex), intent(out) :: values_dot(*)
+        end function ffh_nedelec_pml_values_jvp
+
+        integer(c_int) function ffh_nedelec_pml_values_vjp(prob, kind, stretch, &
+                wave_number, values_bar, vertices_bar, stretch_bar, wave_number_bar) &
+                bind(C, name="ffh_nedelec_pml_values_vjp")
+            import :: c_int, c_ptr, c_double, c_double_complex
+            type(c_ptr), value :: prob
+            integer(c_int), value :: kind
+            complex(c_double_complex), intent(in) :: stretch(*), values_bar(*)
+            real(c_double), value :: wave_number
+            real(c_double), intent(out) :: vertices_bar(*)
+            complex(c_double_complex), intent(out) :: stretch_bar(*)
+            real(c_double), intent(out) :: wave_number_bar
+        end function ffh_nedelec_pml_values_vjp
 
         integer(c_int) function ffh_problem_sizes(prob, n_global, nnz, &
                 n_free, nnz_free, n_local, n_colors) &

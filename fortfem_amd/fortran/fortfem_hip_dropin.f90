@@ -501,6 +501,9 @@ module fortfem_assembly_tetra_nedelec_3d
 
     public :: assemble_tetra_nedelec_pml_csc
     public :: assemble_tetra_nedelec_curvilinear_pml_csc
+    public :: assemble_tetra_nedelec_pml_csc_jvp, assemble_tetra_nedelec_pml_csc_vjp
+    public :: assemble_tetra_nedelec_curvilinear_pml_csc_jvp
+    public :: assemble_tetra_nedelec_curvilinear_pml_csc_vjp
 
     public :: assemble_tetra_nedelec_curl_mass_csc
     public :: assemble_tetra_nedelec_curl_mass_csc_jvp
@@ -736,6 +739,174 @@ contains
         if (.not. (wave_number > 0.0_dp)) return
         pml_input_ok = .true.
     end function pml_input_ok
+
+    ! Tangent or cotangent of the PML operator on a fresh pattern (mode 0:
+    ! JVP into matrix_dot; mode 1: VJP of values_bar).
+    subroutine nedelec_pml_ad(mesh_vertices, tetrahedra, k, kind, mode, stretch, &
+            wave_number, vertices_dot, stretch_dot, wave_number_dot, matrix_dot, &
+            values_bar, vertices_bar, stretch_bar, wave_number_bar, ok)
+        real(dp), intent(in) :: mesh_vertices(:, :), vertices_dot(*)
+        integer, intent(in) :: tetrahedra(:, :), k, mode
+        integer(c_int), intent(in) :: kind
+        complex(dp), intent(in) :: stretch(*), stretch_dot(*), values_bar(*)
+        real(dp), intent(in) :: wave_number, wave_number_dot
+        type(csc_z_t), intent(out) :: matrix_dot
+        real(dp), intent(out) :: vertices_bar(*), wave_number_bar
+        complex(dp), intent(out) :: stretch_bar(*)
+        logical, intent(out) :: ok
+        type(c_ptr) :: prob, mat
+        type(csc_t) :: pattern
+        integer(c_int) :: n_global, code
+        integer(c_int64_t) :: nnz
+        real(c_double) :: kb
+
+        ok = .false.
+        code = ffh_problem_create(hip_context(), 3_c_int, &
+            int(size(mesh_vertices, 2), c_int), mesh_vertices, &
+            int(size(tetrahedra, 2), c_int), tetrahedra, prob)
+        if (code /= FFH_OK) return
+        code = ffh_pattern_build(prob, FFH_FAMILY_NEDELEC, int(k, c_int), n_global, nnz)
+        if (code == FFH_OK .and. mode == 0) then
+            allocate(matrix_dot%val(max(nnz, 1_c_int64_t)))
+            code = ffh_nedelec_pml_values_jvp(prob, kind, stretch, stretch_dot, &
+                real(wave_number, c_double), real(wave_number_dot, c_double), &
+                vertices_dot, matrix_dot%val)
+            if (code == FFH_OK) code = ffh_problem_matrix(prob, FFH_FULL, mat)
+            if (code == FFH_OK) call download_csc(mat, pattern, ok)
+            if (ok) then
+                matrix_dot%nrow = pattern%nrow
+                matrix_dot%ncol = pattern%ncol
+                matrix_dot%nnz = pattern%nnz
+                call move_alloc(pattern%col_ptr, matrix_dot%col_ptr)
+                call move_alloc(pattern%row_idx, matrix_dot%row_idx)
+                if (size(matrix_dot%val) /= matrix_dot%nnz) &
+                    matrix_dot%val = matrix_dot%val(1:matrix_dot%nnz)
+            end if
+        else if (code == FFH_OK) then
+            code = ffh_nedelec_pml_values_vjp(prob, kind, stretch, &
+                real(wave_number, c_double), values_bar, vertices_bar, stretch_bar, kb)
+            wave_number_bar = kb
+            ok = code == FFH_OK
+        end if
+        code = ffh_problem_free(prob)
+    end subroutine nedelec_pml_ad
+
+    ! :1687-1768.  matrix_dot has the primal pattern.
+    subroutine assemble_tetra_nedelec_pml_csc_jvp(mesh_vertices, tetrahedra, order, &
+            stretch, wave_number, mesh_vertices_dot, stretch_dot, wave_number_dot, &
+            matrix_dot, status)
+        real(dp), intent(in) :: mesh_vertices(:, :), mesh_vertices_dot(:, :)
+        integer, intent(in) :: tetrahedra(:, :), order
+        complex(dp), intent(in) :: stretch(:, :), stretch_dot(:, :)
+        real(dp), intent(in) :: wave_number, wave_number_dot
+        type(csc_z_t), intent(out) :: matrix_dot
+        type(fortsparse_status_t), intent(out) :: status
+        complex(dp) :: cz(1)
+        real(dp) :: rz(1), wb
+        logical :: ok
+
+        call status_set(status, FORTSPARSE_INVALID_MATRIX, &
+            "Complex tetrahedral Nedelec PML JVP failed")
+        if (.not. pml_input_ok(mesh_vertices, tetrahedra, order, wave_number)) return
+        if (size(stretch, 1) /= 3 .or. size(stretch, 2) /= size(tetrahedra, 2)) return
+        if (any(shape(stretch_dot) /= shape(stretch))) return
+        if (any(shape(mesh_vertices_dot) /= shape(mesh_vertices))) return
+        call nedelec_pml_ad(mesh_vertices, tetrahedra, order, 0_c_int, 0, stretch, &
+            wave_number, mesh_vertices_dot, stretch_dot, wave_number_dot, matrix_dot, &
+            cz, rz, cz, wb, ok)
+        if (ok) call status_set(status, 0, "")
+    end subroutine assemble_tetra_nedelec_pml_csc_jvp
+
+    ! :1770-1868: the reference's convention (cotangent = sum of seed times
+    ! the conjugate derivative; real part for the vertices and wave number).
+    subroutine assemble_tetra_nedelec_pml_csc_vjp(mesh_vertices, tetrahedra, order, &
+            stretch, wave_number, matrix_values_bar, mesh_vertices_bar, stretch_bar, &
+            wave_number_bar, status)
+        real(dp), intent(in) :: mesh_vertices(:, :)
+        integer, intent(in) :: tetrahedra(:, :), order
+        complex(dp), intent(in) :: stretch(:, :), matrix_values_bar(:)
+        real(dp), intent(in) :: wave_number
+        real(dp), intent(out) :: mesh_vertices_bar(:, :), wave_number_bar
+        complex(dp), intent(out) :: stretch_bar(:, :)
+        type(fortsparse_status_t), intent(out) :: status
+        type(csc_z_t) :: unused
+        complex(dp) :: cz(1)
+        real(dp) :: rz(1)
+        logical :: ok
+
+        mesh_vertices_bar = 0.0_dp
+        stretch_bar = (0.0_dp, 0.0_dp)
+        wave_number_bar = 0.0_dp
+        call status_set(status, FORTSPARSE_INVALID_MATRIX, &
+            "Complex tetrahedral Nedelec PML VJP failed")
+        if (.not. pml_input_ok(mesh_vertices, tetrahedra, order, wave_number)) return
+        if (size(stretch, 1) /= 3 .or. size(stretch, 2) /= size(tetrahedra, 2)) return
+        if (any(shape(stretch_bar) /= shape(stretch))) return
+        if (any(shape(mesh_vertices_bar) /= shape(mesh_vertices))) return
+        call nedelec_pml_ad(mesh_vertices, tetrahedra, order, 0_c_int, 1, stretch, &
+            wave_number, rz, cz, 0.0_dp, unused, matrix_values_bar, mesh_vertices_bar, &
+            stretch_bar, wave_number_bar, ok)
+        if (ok) call status_set(status, 0, "")
+    end subroutine assemble_tetra_nedelec_pml_csc_vjp
+
+    ! :1446-1521
+    subroutine assemble_tetra_nedelec_curvilinear_pml_csc_jvp(mesh_vertices, &
+            tetrahedra, order, stretch, wave_number, mesh_vertices_dot, stretch_dot, &
+            wave_number_dot, matrix_dot, status)
+        real(dp), intent(in) :: mesh_vertices(:, :), mesh_vertices_dot(:, :)
+        integer, intent(in) :: tetrahedra(:, :), order
+        complex(dp), intent(in) :: stretch(:, :, :), stretch_dot(:, :, :)
+        real(dp), intent(in) :: wave_number, wave_number_dot
+        type(csc_z_t), intent(out) :: matrix_dot
+        type(fortsparse_status_t), intent(out) :: status
+        complex(dp) :: cz(1)
+        real(dp) :: rz(1), wb
+        logical :: ok
+
+        call status_set(status, FORTSPARSE_INVALID_MATRIX, &
+            "Curvilinear tetrahedral Nedelec PML JVP failed")
+        if (.not. pml_input_ok(mesh_vertices, tetrahedra, order, wave_number)) return
+        if (size(stretch, 1) /= 3 .or. size(stretch, 2) /= 3) return
+        if (size(stretch, 3) /= size(tetrahedra, 2)) return
+        if (any(shape(stretch_dot) /= shape(stretch))) return
+        if (any(shape(mesh_vertices_dot) /= shape(mesh_vertices))) return
+        call nedelec_pml_ad(mesh_vertices, tetrahedra, order, 1_c_int, 0, stretch, &
+            wave_number, mesh_vertices_dot, stretch_dot, wave_number_dot, matrix_dot, &
+            cz, rz, cz, wb, ok)
+        if (ok) call status_set(status, 0, "")
+    end subroutine assemble_tetra_nedelec_curvilinear_pml_csc_jvp
+
+    ! :1523-1611
+    subroutine assemble_tetra_nedelec_curvilinear_pml_csc_vjp(mesh_vertices, &
+            tetrahedra, order, stretch, wave_number, matrix_values_bar, &
+            mesh_vertices_bar, stretch_bar, wave_number_bar, status)
+        real(dp), intent(in) :: mesh_vertices(:, :)
+        integer, intent(in) :: tetrahedra(:, :), order
+        complex(dp), intent(in) :: stretch(:, :, :), matrix_values_bar(:)
+        real(dp), intent(in) :: wave_number
+        real(dp), intent(out) :: mesh_vertices_bar(:, :), wave_number_bar
+        complex(dp), intent(out) :: stretch_bar(:, :, :)
+        type(fortsparse_status_t), intent(out) :: status
+        type(csc_z_t) :: unused
+        complex(dp) :: cz(1)
+        real(dp) :: rz(1)
+        logical :: ok
+
+        mesh_vertices_bar = 0.0_dp
+        stretch_bar = (0.0_dp, 0.0_dp)
+        wave_number_bar = 0.0_dp
+        call status_set(status, FORTSPARSE_INVALID_MATRIX, &
+            "Curvilinear tetrahedral Nedelec PML VJP failed")
+        if (.not. pml_input_ok(mesh_vertices, tetrahedra, order, wave_number)) return
+        if (size(stretch, 1) /= 3 .or. size(stretch, 2) /= 3) return
+        if (size(stretch, 3) /= size(tetrahedra, 2)) return
+        if (any(shape(stretch_bar) /= shape(stretch))) return
+        if (any(shape(mesh_vertices_bar) /= shape(mesh_vertices))) return
+        call nedelec_pml_ad(mesh_vertices, tetrahedra, order, 1_c_int, 1, stretch, &
+            wave_number, rz, cz, 0.0_dp, unused, matrix_values_bar, mesh_vertices_bar, &
+            stretch_bar, wave_number_bar, ok)
+        if (ok) call status_set(status, 0, "")
+    end subroutine assemble_tetra_nedelec_curvilinear_pml_csc_vjp
 
     ! :1613-1685.  stretch(3, n_tetrahedra): the cell's complex stretch
     ! factors along x, y, z.

@@ -52,7 +52,8 @@ program test_fortfem_hip_dropin
     use fortfem_assembly_tetra_nedelec_3d, only: assemble_tetra_nedelec_curl_mass_csc, &
         assemble_tetra_nedelec_vector_load, assemble_tetra_nedelec_curl_mass_csc_jvp, &
         assemble_tetra_nedelec_curl_mass_csc_vjp, assemble_tetra_nedelec_pml_csc, &
-        assemble_tetra_nedelec_curvilinear_pml_csc
+        assemble_tetra_nedelec_curvilinear_pml_csc, assemble_tetra_nedelec_pml_csc_jvp, &
+        assemble_tetra_nedelec_pml_csc_vjp
     use fortfem_sparse_matrix, only: spmv, spmv_jvp, spmv_vjp
     use fortfem_advanced_solvers, only: solver_options_t, solver_stats_t, &
         solver_options, solve_sparse, solve_sparse_jvp, solve_sparse_vjp
@@ -78,6 +79,7 @@ program test_fortfem_hip_dropin
     call check_nedelec_csc_ad()
     call check_nedelec_constant_solve()
     call check_nedelec_pml()
+    call check_nedelec_pml_ad()
     call fortfem_hip_finalize()
     if (.not. all_passed) error stop 1
     write (*, '(a)') "drop-in Fortran checks passed"
@@ -557,6 +559,51 @@ contains
         call assemble_tetra_nedelec_pml_csc(vertices, tets, 1, stretch, 1.3_dp, a, st)
         call record(st%code /= 0, "zero PML stretch is rejected")
     end subroutine check_nedelec_pml
+
+    ! The PML cotangent is the adjoint of the tangent in the reference's
+    ! convention: Re sum conj(w) A_dot = <v_bar, v_dot> + Re sum conj(s_bar)
+    ! s_dot + k_bar k_dot.
+    subroutine check_nedelec_pml_ad()
+        real(dp) :: vertices(3, 5), vd(3, 5), vb(3, 5), kb, lhs, rhs
+        integer :: tets(4, 2), i
+        complex(dp) :: stretch(3, 2), sd(3, 2), sb(3, 2)
+        complex(dp), allocatable :: w(:)
+        type(csc_z_t) :: a
+        type(fortsparse_status_t) :: st
+        logical :: ok
+
+        vertices = reshape([0.0_dp, 0.0_dp, 0.0_dp, 1.0_dp, 0.0_dp, 0.0_dp, &
+            0.0_dp, 1.0_dp, 0.0_dp, 0.0_dp, 0.0_dp, 1.0_dp, 0.0_dp, 0.0_dp, -1.0_dp], [3, 5])
+        tets(:, 1) = [1, 2, 3, 4]
+        tets(:, 2) = [1, 3, 2, 5]
+        stretch(:, 1) = [cmplx(1.0_dp, 0.6_dp, dp), cmplx(0.9_dp, 0.1_dp, dp), &
+            cmplx(1.2_dp, -0.2_dp, dp)]
+        stretch(:, 2) = [cmplx(1.1_dp, 0.3_dp, dp), cmplx(1.0_dp, 0.0_dp, dp), &
+            cmplx(0.8_dp, 0.4_dp, dp)]
+        do i = 1, 15
+            vd(mod(i - 1, 3) + 1, (i - 1) / 3 + 1) = sin(0.7_dp * i)
+        end do
+        sd = cmplx(cos(1.3_dp * reshape([(real(i, dp), i = 1, 6)], [3, 2])), &
+            sin(0.4_dp * reshape([(real(i, dp), i = 1, 6)], [3, 2])), dp)
+        call assemble_tetra_nedelec_pml_csc_jvp(vertices, tets, 2, stretch, 1.3_dp, vd, &
+            sd, 0.25_dp, a, st)
+        ok = st%code == 0
+        if (ok) then
+            allocate(w(a%nnz))
+            do i = 1, a%nnz
+                w(i) = cmplx(cos(0.37_dp * i), sin(0.91_dp * i), dp)
+            end do
+            call assemble_tetra_nedelec_pml_csc_vjp(vertices, tets, 2, stretch, 1.3_dp, w, &
+                vb, sb, kb, st)
+            ok = st%code == 0
+        end if
+        if (ok) then
+            lhs = real(sum(conjg(w) * a%val), dp)
+            rhs = sum(vb * vd) + real(sum(conjg(sb) * sd), dp) + kb * 0.25_dp
+            ok = abs(lhs - rhs) <= 1.0e-10_dp * max(abs(lhs), 1.0_dp)
+        end if
+        call record(ok, "Nedelec PML VJP is the adjoint of the JVP")
+    end subroutine check_nedelec_pml_ad
 
     subroutine check_linear_energy()
         real(dp) :: mesh_vertices(3, 5), coefficients(5), product(5)
