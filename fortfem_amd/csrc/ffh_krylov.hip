@@ -1168,7 +1168,12 @@ static void launch_iterations(Context *ctx, Matrix &M, int iters, XT *x, VT *r, 
     const DevMatrix &A = M.m;
     Workspace &w = M.ws;
     const int n = A.n;
-    const int gridv = grid_for(ctx, n, kBlock);
+    // about four rows per thread, at least one block per CU: every block of
+    // the next direction launch re-sums this launch's partials, so fewer
+    // blocks shorten that launch's start (c2: 977 -> 256 blocks)
+    const int gridv = (int)std::min<int64_t>(
+        grid_for(ctx, n, kBlock),
+        std::max<int64_t>(ctx->cu_count, ((int64_t)n + 4 * kBlock - 1) / (4 * kBlock)));
     const int grids = variant_grid(ctx, A, vi);
     const int stride = partial_stride(ctx);
     const GArgs<MT> g = gargs<MT>(A, vi);
