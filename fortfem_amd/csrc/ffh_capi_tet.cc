@@ -3,7 +3,9 @@
 // zero-based indices, positive handles from 32-slot registries,
 // src/bindings/fortfem_capi_solver.f90:10-124) over this library's C-ABI.
 // Host code only: every device step goes through include/fortfem_hip.h.
+#include <algorithm>
 #include <mutex>
+#include <tuple>
 #include <vector>
 
 #include "../../include/fortfem_hip.h"
@@ -163,12 +165,68 @@ void retain_problem(int mesh, int family, int order, int qdeg, double coeff, dou
     *status = 0;
 }
 
+// true when the complex CSC (zero-based, values as (re, im) pairs) equals its
+// transpose entry for entry (duplicates summed first)
+bool complex_symmetric(int n, int nnz, const int *col_ptr, const int *row_ind,
+                       const double *values)
+{
+    using E = std::tuple<int, int, double, double>;
+    std::vector<E> a, t;
+    a.reserve(nnz);
+    t.reserve(nnz);
+    for (int j = 0; j < n; ++j)
+        for (int k = col_ptr[j]; k < col_ptr[j + 1]; ++k) {
+            a.emplace_back(j, row_ind[k], values[2 * k], values[2 * k + 1]);
+            t.emplace_back(row_ind[k], j, values[2 * k], values[2 * k + 1]);
+        }
+    auto merged = [](std::vector<E> &v) {
+        std::sort(v.begin(), v.end(), [](const E &x, const E &y) {
+            return std::tie(std::get<0>(x), std::get<1>(x)) <
+                   std::tie(std::get<0>(y), std::get<1>(y));
+        });
+        std::vector<E> m;
+        for (const E &e : v) {
+            if (!m.empty() && std::get<0>(m.back()) == std::get<0>(e) &&
+                std::get<1>(m.back()) == std::get<1>(e)) {
+                std::get<2>(m.back()) += std::get<2>(e);
+                std::get<3>(m.back()) += std::get<3>(e);
+            } else {
+                m.push_back(e);
+            }
+        }
+        return m;
+    };
+    const std::vector<E> ma = merged(a), mt = merged(t);
+    if (ma.size() != mt.size()) return false;
+    double scale = 0.0;
+    for (const E &e : ma) scale = std::max(scale, std::abs(std::get<2>(e)) + std::abs(std::get<3>(e)));
+    const double tol = 1e-12 * scale;
+    for (size_t i = 0; i < ma.size(); ++i) {
+        if (std::get<0>(ma[i]) != std::get<0>(mt[i]) || std::get<1>(ma[i]) != std::get<1>(mt[i]))
+            return false;
+        if (std::abs(std::get<2>(ma[i]) - std::get<2>(mt[i])) > tol ||
+            std::abs(std::get<3>(ma[i]) - std::get<3>(mt[i])) > tol)
+            return false;
+    }
+    return true;
+}
+
 } // namespace
+
+// Every export runs inside a function-try-block: an exception (a failed host
+// allocation, say) must not unwind through the C boundary, where it would end
+// the host program; it becomes status -4 and the registries stay as they were.
+#define FFH_CAPI_CATCH(status)                                                 \
+    catch (...)                                                              \
+    {                                                                        \
+        if (status) *status = -4;                                            \
+    }
 
 extern "C" {
 
 void fortfem_tetra_mesh_create(int n_vertices, const double *vertices, int n_tetrahedra,
                                const int *tetrahedra, int *handle, int *status)
+try
 {
     if (!status) return;
     std::lock_guard<std::mutex> lk(g_lock);
@@ -183,18 +241,21 @@ void fortfem_tetra_mesh_create(int n_vertices, const double *vertices, int n_tet
         *status = -3;
         return;
     }
-    MeshSlot &m = g_mesh[slot];
-    m.used = true;
+    MeshSlot m;
     m.nv = n_vertices;
     m.nt = n_tetrahedra;
     m.vertices.assign(vertices, vertices + 3ll * n_vertices);
     m.tets.resize(4ll * n_tetrahedra);
     for (long long i = 0; i < 4ll * n_tetrahedra; ++i) m.tets[i] = tetrahedra[i] + 1;
+    m.used = true;
+    g_mesh[slot] = std::move(m);
     *handle = slot + 1;
     *status = 0;
 }
+FFH_CAPI_CATCH(status)
 
 void fortfem_tetra_mesh_free(int handle, int *status)
+try
 {
     if (!status) return;
     std::lock_guard<std::mutex> lk(g_lock);
@@ -206,12 +267,14 @@ void fortfem_tetra_mesh_free(int handle, int *status)
     *m = MeshSlot{};
     *status = 0;
 }
+FFH_CAPI_CATCH(status)
 
 void fortfem_tetra_lagrange_stiffness_csc(int mesh, int degree, int quadrature_degree,
                                           double stiffness_coefficient,
                                           double mass_coefficient, int nnz_capacity,
                                           int *n_dofs, int *nnz, int *col_ptr, int *row_ind,
                                           double *values, int *status)
+try
 {
     if (!status) return;
     if (degree < 1 || degree > 4 || quadrature_degree < 0) {
@@ -221,11 +284,13 @@ void fortfem_tetra_lagrange_stiffness_csc(int mesh, int degree, int quadrature_d
     export_csc(mesh, FFH_FAMILY_H1, degree, quadrature_degree, stiffness_coefficient,
                mass_coefficient, nnz_capacity, n_dofs, nnz, col_ptr, row_ind, values, status);
 }
+FFH_CAPI_CATCH(status)
 
 void fortfem_tetra_nedelec_curl_mass_csc(int mesh, int order, double curl_coefficient,
                                          double mass_coefficient, int nnz_capacity,
                                          int *n_dofs, int *nnz, int *col_ptr, int *row_ind,
                                          double *values, int *status)
+try
 {
     if (!status) return;
     if (order < 1 || order > 6) {
@@ -235,10 +300,12 @@ void fortfem_tetra_nedelec_curl_mass_csc(int mesh, int order, double curl_coeffi
     export_csc(mesh, FFH_FAMILY_NEDELEC, order, -1, curl_coefficient, mass_coefficient,
                nnz_capacity, n_dofs, nnz, col_ptr, row_ind, values, status);
 }
+FFH_CAPI_CATCH(status)
 
 void fortfem_tetra_lagrange_operator(int mesh, int degree, int quadrature_degree,
                                      double stiffness_coefficient, double mass_coefficient,
                                      int boundary, int *handle, int *n_dofs, int *status)
+try
 {
     if (!status) return;
     if (degree < 1 || degree > 4 || quadrature_degree < 0) {
@@ -248,10 +315,12 @@ void fortfem_tetra_lagrange_operator(int mesh, int degree, int quadrature_degree
     retain_problem(mesh, FFH_FAMILY_H1, degree, quadrature_degree, stiffness_coefficient,
                    mass_coefficient, boundary, handle, n_dofs, status);
 }
+FFH_CAPI_CATCH(status)
 
 void fortfem_tetra_nedelec_operator(int mesh, int order, double curl_coefficient,
                                     double mass_coefficient, int boundary, int *handle,
                                     int *n_dofs, int *status)
+try
 {
     if (!status) return;
     if (order < 1 || order > 6) {
@@ -261,9 +330,11 @@ void fortfem_tetra_nedelec_operator(int mesh, int order, double curl_coefficient
     retain_problem(mesh, FFH_FAMILY_NEDELEC, order, -1, curl_coefficient, mass_coefficient,
                    boundary, handle, n_dofs, status);
 }
+FFH_CAPI_CATCH(status)
 
 void fortfem_real_operator_csc(int n, int nnz, const int *col_ptr, const int *row_ind,
                                const double *values, int *handle, int *status)
+try
 {
     if (!status) return;
     std::lock_guard<std::mutex> lk(g_lock);
@@ -295,10 +366,12 @@ void fortfem_real_operator_csc(int n, int nnz, const int *col_ptr, const int *ro
     *handle = slot + 1;
     *status = 0;
 }
+FFH_CAPI_CATCH(status)
 
 void fortfem_operator_solve(int handle, int n, double tolerance, int max_iterations,
                             int precision, const double *rhs, double *solution,
                             int *iterations, int *status)
+try
 {
     if (!status) return;
     std::lock_guard<std::mutex> lk(g_lock);
@@ -336,9 +409,11 @@ void fortfem_operator_solve(int handle, int n, double tolerance, int max_iterati
     if (iterations) *iterations = st.iterations;
     *status = st.converged ? 0 : 2;
 }
+FFH_CAPI_CATCH(status)
 
 void fortfem_complex_operator_csc(int n, int nnz, const int *col_ptr, const int *row_ind,
                                   const double *values, int *handle, int *status)
+try
 {
     if (!status) return;
     std::lock_guard<std::mutex> lk(g_lock);
@@ -351,6 +426,8 @@ void fortfem_complex_operator_csc(int n, int nnz, const int *col_ptr, const int 
         if (col_ptr[j + 1] < col_ptr[j]) return;
     for (int k = 0; k < nnz; ++k)
         if (row_ind[k] < 0 || row_ind[k] >= n) return;
+    // COCG needs A = A^T (the PML curl-curl operators are complex-symmetric)
+    if (!complex_symmetric(n, nnz, col_ptr, row_ind, values)) return;
     const int slot = free_slot(g_op);
     if (slot < 0) {
         *status = -3;
@@ -372,10 +449,12 @@ void fortfem_complex_operator_csc(int n, int nnz, const int *col_ptr, const int 
     *handle = slot + 1;
     *status = 0;
 }
+FFH_CAPI_CATCH(status)
 
 void fortfem_complex_operator_solve(int handle, int n, double tolerance, int max_iterations,
                                     const double *rhs, double *solution, int *iterations,
                                     int *status)
+try
 {
     if (!status) return;
     std::lock_guard<std::mutex> lk(g_lock);
@@ -399,8 +478,10 @@ void fortfem_complex_operator_solve(int handle, int n, double tolerance, int max
     }
     *status = conv ? 0 : 2;
 }
+FFH_CAPI_CATCH(status)
 
 void fortfem_operator_free(int handle, int *status)
+try
 {
     if (!status) return;
     std::lock_guard<std::mutex> lk(g_lock);
@@ -415,5 +496,6 @@ void fortfem_operator_free(int handle, int *status)
     *op = OperatorSlot{};
     *status = 0;
 }
+FFH_CAPI_CATCH(status)
 
 } // extern "C"

@@ -8,7 +8,8 @@
  * 545-571: status -2 when the capacity is too small, sizes still returned).
  *
  * status: 0 OK, -1 invalid input, -2 capacity too small, -3 bad handle or
- * registry full (as in the reference), -4 device (HIP) error, 2 a solve that
+ * registry full (as in the reference), -4 device (HIP) error or a failed
+ * host allocation (no exception leaves these functions), 2 a solve that
  * reached its iteration limit (the solution holds the last iterate).
  *
  * Meshes and operators live on the MI355X: a mesh handle keeps the mesh, an
@@ -77,8 +78,10 @@ void fortfem_operator_solve(int handle, int n, double tolerance, int max_iterati
  * layout of fortfem_complex) kept resident, in place of
  * fortfem_complex_factor_csc (include/fortfem.h); solved from x = 0 by
  * Jacobi-COCG, which needs A = A^T (complex symmetric, as the PML curl-curl
- * operators are), instead of fortfem_complex_solve's direct solve.  Status
- * as fortfem_operator_solve; -1 also for a real operator's handle.
+ * operators are), instead of fortfem_complex_solve's direct solve: a matrix
+ * that is not complex symmetric (to 1e-12 of its largest entry) is refused
+ * with status -1 at upload.  Status as fortfem_operator_solve; -1 also for a
+ * real operator's handle.
  * fortfem_operator_free releases either kind. */
 void fortfem_complex_operator_csc(int n, int nnz, const int *col_ptr, const int *row_ind,
                                   const double *values, int *handle, int *status);

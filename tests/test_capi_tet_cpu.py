@@ -101,3 +101,25 @@ def test_complex_operator_argument_checks():
     L.fortfem_complex_operator_solve(5, 2, C.c_double(1e-10), 10, va.ctypes.data_as(C.c_void_p),
                                      x.ctypes.data_as(C.c_void_p), C.byref(it), C.byref(st))
     assert st.value == -3  # unknown handle
+
+
+def test_complex_operator_refuses_nonsymmetric_matrix():
+    """COCG needs A = A^T: a complex CSC whose (0, 1) and (1, 0) entries
+    differ is refused with -1 at upload, before any device call; the same
+    pattern with equal off-diagonal entries passes the check (and then
+    reaches the device, which is absent here: -4, or 0 on a GPU host)."""
+    L = lib()
+    cp = np.array([0, 2, 4], dtype=np.int32)
+    ri = np.array([0, 1, 0, 1], dtype=np.int32)
+    # column-major (re, im): A00 = 2, A10 = 1+1j, A01 = 1-1j, A11 = 3
+    va = np.array([2, 0, 1, 1, 1, -1, 3, 0], dtype=np.float64)
+    h, st = C.c_int(7), C.c_int(9)
+    L.fortfem_complex_operator_csc(2, 4, cp.ctypes.data_as(C.c_void_p), ri.ctypes.data_as(C.c_void_p),
+                                   va.ctypes.data_as(C.c_void_p), C.byref(h), C.byref(st))
+    assert st.value == -1 and h.value == 0
+    va[4:6] = [1, 1]  # A01 = A10: complex symmetric (not Hermitian)
+    L.fortfem_complex_operator_csc(2, 4, cp.ctypes.data_as(C.c_void_p), ri.ctypes.data_as(C.c_void_p),
+                                   va.ctypes.data_as(C.c_void_p), C.byref(h), C.byref(st))
+    assert st.value in (0, -4)
+    if st.value == 0:
+        L.fortfem_operator_free(h.value, C.byref(st))
