@@ -54,7 +54,10 @@ EXPORTS = (
     "ffh_zmatrix_solve", "ffh_zmatrix_free",
     "ffh_server_create", "ffh_server_free", "ffh_server_prepare", "ffh_server_submit", "ffh_server_step",
     "ffh_server_poll", "ffh_server_wait", "ffh_server_stats", "ffh_server_progress", "ffh_problem_solution", "ffh_server_set_precision",
+    "ffh_solve_sparse_device_noalloc", "ffh_noalloc_spmv_variant",
 )
+
+NOALLOC_MAX_ROWS = 1 << 21
 
 
 class SolverOptions(C.Structure):
@@ -167,6 +170,9 @@ def load() -> C.CDLL:
         "ffh_server_progress": (C.c_int, [vp, C.c_int64, vp, vp, vp]),
         "ffh_problem_solution": (C.c_int, [vp, vp]),
         "ffh_server_set_precision": (C.c_int, [vp, C.c_int, i64p]),
+        "ffh_solve_sparse_device_noalloc": (C.c_int, [vp, C.c_int, C.c_int64, vp, vp, vp, vp, vp,
+                                                      C.POINTER(SolverOptions), C.POINTER(SolverStats)]),
+        "ffh_noalloc_spmv_variant": (C.c_int, [C.c_int, C.c_int64]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)

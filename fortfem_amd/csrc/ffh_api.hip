@@ -176,6 +176,7 @@ int ffh_init(int device, ffh_context **out)
         FFH_HIP(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
         FFH_HIP(hipEventCreate(&ctx->ev_start));
         FFH_HIP(hipEventCreate(&ctx->ev_stop));
+        scratch_setup(device); // device address of the allocation-free solve's array
     });
     if (rc != FFH_OK) {
         delete ctx;
@@ -679,6 +680,23 @@ int ffh_solve_sparse(ffh_matrix *M, const double *b, double *x, const ffh_solver
         FFH_HIP(hipStreamSynchronize(ctx->stream));
     });
 }
+
+int ffh_solve_sparse_device_noalloc(ffh_context *ctx, int n, int64_t nnz, const int *col_ptr,
+                                    const int *row_idx, const double *val, const double *b,
+                                    double *x, const ffh_solver_options *o, ffh_solver_stats *st)
+{
+    if (!ctx) return FFH_ERR_HANDLE;
+    if (!col_ptr || !row_idx || !val || !b || !x || !st) return FFH_ERR_INVALID;
+    return guarded(ctx, [&] {
+        check_options(o);
+        FFH_REQUIRE(o->precision == FFH_FP64, FFH_ERR_INVALID,
+                    "allocation-free solve: precision must be FP64");
+        std::memset(st, 0, sizeof(*st));
+        solve_fp64_noalloc(ctx, n, nnz, col_ptr, row_idx, val, b, x, *o, *st);
+    });
+}
+
+int ffh_noalloc_spmv_variant(int n, int64_t nnz) { return noalloc_variant(n, nnz); }
 
 int ffh_spmv_jvp(ffh_matrix *M, const double *x, const double *values_dot, const double *x_dot,
                  double *y_dot)

@@ -21,6 +21,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdlib>
+#include <mutex>
 
 #include "ffh_internal.hpp"
 #include "ffh_krylov.hpp"
@@ -1161,12 +1162,11 @@ void spmv32(Context *ctx, DevMatrix &A, const float *x, float *y)
 // `iters` iterations, two launches each: direction + SpMV (matrix MT,
 // vectors VT) and the x/r update (x in XT).
 template <typename MT, typename VT, typename XT>
-static void launch_iterations(Context *ctx, Matrix &M, int iters, XT *x, VT *r, VT *zpa, VT *zpb,
-                              VT *ap, const VT *dinv, int vi, int prof_slot)
+static void launch_iterations(Context *ctx, const DevMatrix &A, double *partials, KrylovCtl *ctl,
+                              int iters, XT *x, VT *r, VT *zpa, VT *zpb, VT *ap, const VT *dinv,
+                              int vi, int prof_slot)
 {
     ZP<VT> *zp0 = reinterpret_cast<ZP<VT> *>(zpa), *zp1 = reinterpret_cast<ZP<VT> *>(zpb);
-    const DevMatrix &A = M.m;
-    Workspace &w = M.ws;
     const int n = A.n;
     // about four rows per thread, at least one block per CU: every block of
     // the next direction launch re-sums this launch's partials, so fewer
@@ -1182,11 +1182,11 @@ static void launch_iterations(Context *ctx, Matrix &M, int iters, XT *x, VT *r, 
         FFH_ENGINE(kVariants[vi].engine, {
             hipLaunchKernelGGL((k_pcg_dirspmv<VV, MT, VT>), dim3(grids), dim3(kStreamBlock), 0,
                                ctx->stream, n, g.ptr, g.idx, g.val, zp0, zp1, ap, g.rb, g.nrb,
-                               w.partials.p, stride, w.ctl.p);
+                               partials, stride, ctl);
         });
         if (prof_slot >= 0) ctx->prof_mark(prof_slot, false);
         hipLaunchKernelGGL((k_pcg_update<VT, XT>), dim3(gridv), dim3(kBlock), 0, ctx->stream, n, x,
-                           r, zp0, zp1, ap, dinv, w.partials.p, stride, grids, w.ctl.p);
+                           r, zp0, zp1, ap, dinv, partials, stride, grids, ctl);
     }
     FFH_HIP(hipGetLastError());
 }
@@ -1260,67 +1260,95 @@ static KrylovCtl run_chunks(Context *ctx, Matrix &M, int gslot, int graph_key, b
     return *w.host_ctl;
 }
 
-static void launch_residual(Context *ctx, Matrix &M)
+// r = b - A x in FP64 with variant vi and ||r||^2 into ctl->true_rr
+static void launch_residual_on(Context *ctx, const DevMatrix &A, int vi, const double *x,
+                               const double *b, double *r, double *partials, KrylovCtl *ctl)
 {
-    DevMatrix &A = M.m;
-    Workspace &w = M.ws;
-    const int vi = spmv_variant(ctx, M, false);
     const int gr = variant_grid(ctx, A, vi);
     const GArgs<double> g = gargs<double>(A, vi);
     FFH_ENGINE(kVariants[vi].engine, {
         hipLaunchKernelGGL((k_residual<VV>), dim3(gr), dim3(kStreamBlock), 0, ctx->stream, A.n,
-                           g.ptr, g.idx, g.val, w.x.p, w.b.p, w.r.p, g.rb, g.nrb,
-                           w.partials.p);
+                           g.ptr, g.idx, g.val, x, b, r, g.rb, g.nrb, partials);
     });
-    hipLaunchKernelGGL(k_residual_finish, dim3(1), dim3(kBlock), 0, ctx->stream, w.partials.p,
-                       gr, w.ctl.p);
+    hipLaunchKernelGGL(k_residual_finish, dim3(1), dim3(kBlock), 0, ctx->stream, partials, gr,
+                       ctl);
     FFH_HIP(hipGetLastError());
 }
 
-// FP64 Jacobi-PCG (or CG) of the reference on the workspace vectors b, x
-// (x = initial guess, overwritten): initialisation, then chunks of
-// iterations each ending with the stop test.
-static KrylovCtl pcg_fp64(Context *ctx, Matrix &M, int method, int precond, double tolerance,
-                          int tol_type, int max_iter, int breakdown_mode, int check_interval,
-                          int use_graph)
+static void launch_residual(Context *ctx, Matrix &M)
 {
-    DevMatrix &A = M.m;
     Workspace &w = M.ws;
+    const int vi = spmv_variant(ctx, M, false);
+    launch_residual_on(ctx, M.m, vi, w.x.p, w.b.p, w.r.p, w.partials.p, w.ctl.p);
+}
+
+// The device vectors of one FP64 Jacobi-PCG solve: x (initial guess,
+// overwritten), b, r, the two interleaved (z, p) arrays (2 n each), A p,
+// D^-1, the partial sums (4 banks of partial_stride) and the control block.
+struct PcgBufs {
+    double *x, *b, *r, *zpa, *zpb, *ap, *dinv, *partials;
+    KrylovCtl *ctl;
+};
+
+__global__ void k_ctl_set(KrylovCtl *ctl, KrylovCtl v) { *ctl = v; }
+
+// FP64 Jacobi-PCG (or CG) of the reference on the vectors of w with SpMV
+// variant vi: initialisation, then chunks of iterations each ending with the
+// stop test; run(enqueue_chunk, chunk) drives the chunks until the control
+// block says done and returns its last state.
+template <typename Run>
+static KrylovCtl pcg_fp64_on(Context *ctx, const DevMatrix &A, int vi, const PcgBufs &w,
+                             int method, int precond, double tolerance, int tol_type,
+                             int max_iter, int breakdown_mode, int check_interval, Run &&run)
+{
     const int n = A.n;
     hipStream_t s = ctx->stream;
     hipLaunchKernelGGL((k_jacobi<double>), dim3(grid_for(ctx, n, kBlock)), dim3(kBlock), 0, s, n,
                        A.gptr(), A.gidx(), A.gval(), (method == 1 && precond == 1) ? 1 : 0,
-                       w.dinv.p, (float *)nullptr);
+                       w.dinv, (float *)nullptr);
     KrylovCtl init{};
     init.max_iter = max_iter;
     init.breakdown_mode = breakdown_mode;
     init.tol_type = tol_type;
     init.tolerance = tolerance;
-    *w.host_ctl = init;
-    FFH_HIP(hipMemcpyAsync(w.ctl.p, w.host_ctl, sizeof(KrylovCtl), hipMemcpyHostToDevice, s));
-    const int vi = spmv_variant(ctx, M, false);
+    hipLaunchKernelGGL(k_ctl_set, dim3(1), dim3(1), 0, s, w.ctl, init);
     const int gi = variant_grid(ctx, A, vi);
     const int stride = partial_stride(ctx);
     const GArgs<double> g = gargs<double>(A, vi);
     FFH_ENGINE(kVariants[vi].engine, {
         hipLaunchKernelGGL((k_pcg_init<VV, double, double>), dim3(gi), dim3(kStreamBlock), 0, s, n,
-                           g.ptr, g.idx, g.val, w.x.p, w.b.p, w.r.p, w.dinv.p,
-                           reinterpret_cast<ZP<double> *>(w.p.p), g.rb, g.nrb, w.partials.p,
+                           g.ptr, g.idx, g.val, w.x, w.b, w.r, w.dinv,
+                           reinterpret_cast<ZP<double> *>(w.zpa), g.rb, g.nrb, w.partials,
                            stride);
     });
-    hipLaunchKernelGGL(k_pcg_init_finish, dim3(1), dim3(kBlock), 0, s, w.partials.p, stride, gi,
-                       w.ctl.p);
+    hipLaunchKernelGGL(k_pcg_init_finish, dim3(1), dim3(kBlock), 0, s, w.partials, stride, gi,
+                       w.ctl);
     FFH_HIP(hipGetLastError());
     const int chunk = check_interval > 0 ? check_interval : 16;
     const int prof = ctx->profiling ? 0 : -1;
     auto enqueue = [&]() {
-        launch_iterations<double, double, double>(ctx, M, chunk, w.x.p, w.r.p, w.p.p, w.p2.p,
-                                                  w.ap.p, w.dinv.p, vi, prof);
-        hipLaunchKernelGGL(k_pcg_fin, dim3(1), dim3(kBlock), 0, s, w.partials.p, stride, w.ctl.p);
+        launch_iterations<double, double, double>(ctx, A, w.partials, w.ctl, chunk, w.x, w.r,
+                                                  w.zpa, w.zpb, w.ap, w.dinv, vi, prof);
+        hipLaunchKernelGGL(k_pcg_fin, dim3(1), dim3(kBlock), 0, s, w.partials, stride, w.ctl);
         FFH_HIP(hipGetLastError());
     };
-    return run_chunks(ctx, M, 0, chunk * 64 + vi, use_graph != 0, enqueue,
-                      [](const KrylovCtl &) { return true; });
+    return run(enqueue, chunk);
+}
+
+// FP64 Jacobi-PCG on a matrix's workspace vectors b, x (x = initial guess,
+// overwritten), chunks polled one behind the GPU (run_chunks).
+static KrylovCtl pcg_fp64(Context *ctx, Matrix &M, int method, int precond, double tolerance,
+                          int tol_type, int max_iter, int breakdown_mode, int check_interval,
+                          int use_graph)
+{
+    Workspace &w = M.ws;
+    const int vi = spmv_variant(ctx, M, false);
+    const PcgBufs bufs{w.x.p, w.b.p, w.r.p, w.p.p, w.p2.p, w.ap.p, w.dinv.p, w.partials.p, w.ctl.p};
+    return pcg_fp64_on(ctx, M.m, vi, bufs, method, precond, tolerance, tol_type, max_iter,
+                       breakdown_mode, check_interval, [&](auto &enqueue, int chunk) {
+                           return run_chunks(ctx, M, 0, chunk * 64 + vi, use_graph != 0, enqueue,
+                                             [](const KrylovCtl &) { return true; });
+                       });
 }
 
 // Public FP64 solve on device vectors.
@@ -1402,8 +1430,9 @@ void solve_mixed(Context *ctx, Matrix &M, const double *b, double *x,
     const int chunk = o.check_interval > 0 ? o.check_interval : 16;
     const int prof = ctx->profiling ? 1 : -1;
     auto enqueue = [&]() {
-        launch_iterations<float, float, double>(ctx, M, chunk, w.x.p, w.r32.p, w.p32.p, w.p32b.p,
-                                                w.ap32.p, w.dinv32.p, vi32, prof);
+        launch_iterations<float, float, double>(ctx, A, w.partials.p, w.ctl.p, chunk, w.x.p,
+                                                w.r32.p, w.p32.p, w.p32b.p, w.ap32.p, w.dinv32.p,
+                                                vi32, prof);
         launch_refresh(ctx, M, vi64, 0);
     };
     double best = 1e300;
@@ -1484,6 +1513,197 @@ double bench_spmv(Context *ctx, Matrix &M, int precision, int reps, double *byte
     float ms = 0.0f;
     FFH_HIP(hipEventElapsedTime(&ms, ctx->ev_start, ctx->ev_stop));
     return (double)ms / std::max(1, reps);
+}
+
+// ------------------------------------------------ allocation-free solve --
+// FP64 Jacobi-PCG on a caller's device CSC and device b / x with every work
+// vector in `scratch`, a fixed array in the library's code object: each GPU
+// the library runs on holds its own copy from the moment the code object is
+// loaded there until the process ends.  Layout (in doubles): the control
+// block and a status word (256 B), the partial sums (4 banks of
+// kNoAllocStride), then r, (z, p) a, (z, p) b, A p and D^-1 for up to
+// FFH_NOALLOC_MAX_ROWS rows.  Calls on one GPU share it, so calls that
+// overlap on one GPU race.
+constexpr int64_t kNoAllocHead = 32;     // doubles: KrylovCtl + status word
+constexpr int64_t kNoAllocStride = 8192; // partial-sum bank (>= partial_stride)
+constexpr int64_t kNoAllocRows = FFH_NOALLOC_MAX_ROWS;
+static_assert(kNoAllocRows % 32 == 0, "rows are padded to multiples of 32");
+constexpr int64_t kNoAllocDoubles = kNoAllocHead + 4 * kNoAllocStride + 7 * kNoAllocRows;
+static_assert(sizeof(KrylovCtl) + sizeof(int) <= kNoAllocHead * sizeof(double),
+              "control block does not fit the scratch head");
+
+static __device__ __align__(256) double scratch[kNoAllocDoubles];
+
+// device address of `scratch` per device, looked up once per device with
+// that device current (the array's name is only a host-side stand-in)
+static std::mutex g_scratch_lock;
+static std::vector<double *> g_scratch_ptr;
+
+void scratch_setup(int device)
+{
+    std::lock_guard<std::mutex> lk(g_scratch_lock);
+    if ((int)g_scratch_ptr.size() <= device) g_scratch_ptr.resize(device + 1, nullptr);
+    if (g_scratch_ptr[device]) return;
+    void *ptr = nullptr;
+    FFH_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    FFH_REQUIRE(ptr != nullptr, FFH_ERR_DEVICE, "scratch array has no device address");
+    g_scratch_ptr[device] = static_cast<double *>(ptr);
+}
+
+static double *scratch_of(int device)
+{
+    std::lock_guard<std::mutex> lk(g_scratch_lock);
+    FFH_REQUIRE(device >= 0 && device < (int)g_scratch_ptr.size() && g_scratch_ptr[device],
+                FFH_ERR_DEVICE, "scratch array not set up on this device (ffh_init)");
+    return g_scratch_ptr[device];
+}
+
+__global__ __launch_bounds__(kBlock) void k_fill_zero(int64_t n, double *__restrict__ a)
+{
+    for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * kBlock)
+        a[i] = 0.0;
+}
+
+// Checks a 0-based CSC for the products of the allocation-free solve:
+// col_ptr(0) = 0, col_ptr(n) = nnz, columns monotone, rows in range and
+// strictly ascending in each column, and every entry (i, j) mirrored by an
+// entry (j, i) of the same value (A = A^T, so the CSC reads as CSR).  Any
+// failure writes 1 to *bad; every read is bounds-checked first.
+__global__ __launch_bounds__(kBlock) void k_check_symmetric(int n, int64_t nnz,
+                                                             const int *__restrict__ ptr,
+                                                             const int *__restrict__ idx,
+                                                             const double *__restrict__ val,
+                                                             int *__restrict__ bad)
+{
+    const int j = blockIdx.x * kBlock + threadIdx.x;
+    if (j >= n) return;
+    if (j == 0 && (ptr[0] != 0 || (int64_t)ptr[n] != nnz)) *bad = 1;
+    const int b = ptr[j], e = ptr[j + 1];
+    if (b < 0 || e < b || (int64_t)e > nnz) {
+        *bad = 1;
+        return;
+    }
+    int prev = -1;
+    for (int k = b; k < e; ++k) {
+        const int i = idx[k];
+        if (i <= prev || i >= n) {
+            *bad = 1;
+            return;
+        }
+        prev = i;
+        const int cb = ptr[i], ce = ptr[i + 1];
+        if (cb < 0 || ce < cb || (int64_t)ce > nnz) {
+            *bad = 1;
+            return;
+        }
+        int lo = cb, hi = ce;
+        while (lo < hi) { // first entry of column i with row >= j
+            const int mid = lo + ((hi - lo) >> 1);
+            if (idx[mid] < j) lo = mid + 1;
+            else hi = mid;
+        }
+        if (lo >= ce || idx[lo] != j || val[lo] != val[k]) {
+            *bad = 1;
+            return;
+        }
+    }
+}
+
+// SpMV engine of the allocation-free solve: vector CSR (no row blocks or
+// SELL copy to build), lanes per row from the mean row length.
+int noalloc_variant(int n, int64_t nnz)
+{
+    const double avg = n > 0 ? (double)nnz / n : 0.0;
+    if (avg <= 6.0) return 3;  // 4 lanes, grid x2
+    if (avg <= 16.0) return 5; // 8 lanes, grid x2
+    if (avg <= 40.0) return 6; // 16 lanes, grid x2
+    return 7;                  // 32 lanes, grid x2
+}
+
+void solve_fp64_noalloc(Context *ctx, int n, int64_t nnz, const int *col_ptr,
+                        const int *row_idx, const double *val, const double *b, double *x,
+                        const ffh_solver_options &o, ffh_solver_stats &st)
+{
+    FFH_REQUIRE(n >= 1 && n <= kNoAllocRows, FFH_ERR_INVALID,
+                "allocation-free solve: n must be 1 .. FFH_NOALLOC_MAX_ROWS");
+    FFH_REQUIRE(nnz >= 1 && nnz <= INT32_MAX, FFH_ERR_INVALID,
+                "allocation-free solve: nnz must be 1 .. 2^31-1");
+    FFH_REQUIRE(partial_stride(ctx) <= kNoAllocStride, FFH_ERR_DEVICE,
+                "allocation-free solve: more compute units than the scratch partials cover");
+    FFH_HIP(hipSetDevice(ctx->device));
+    double *base = scratch_of(ctx->device);
+    hipStream_t s = ctx->stream;
+    KrylovCtl *ctl = reinterpret_cast<KrylovCtl *>(base);
+    int *bad = reinterpret_cast<int *>(reinterpret_cast<char *>(base) + sizeof(KrylovCtl));
+    double *partials = base + kNoAllocHead;
+    double *v = partials + 4 * kNoAllocStride;
+    // each vector starts on a 256-byte boundary (the (z, p) pairs are read
+    // as 16-byte words)
+    const int64_t nr = ((int64_t)n + 31) & ~(int64_t)31;
+    const PcgBufs w{x, const_cast<double *>(b), v, v + nr, v + 3 * nr, v + 5 * nr, v + 6 * nr,
+                    partials, ctl};
+    // the caller's arrays, borrowed (never freed here)
+    struct Borrowed {
+        DevMatrix m;
+        ~Borrowed()
+        {
+            m.col_ptr.p = m.row_idx.p = nullptr;
+            m.val.p = nullptr;
+            m.col_ptr.n = m.row_idx.n = m.val.n = 0;
+        }
+    } A;
+    A.m.n = n;
+    A.m.nnz = nnz;
+    A.m.symmetric = true;
+    A.m.col_ptr.p = const_cast<int *>(col_ptr);
+    A.m.row_idx.p = const_cast<int *>(row_idx);
+    A.m.val.p = const_cast<double *>(val);
+    const int vi = noalloc_variant(n, nnz);
+
+    // what must start at zero: the head (control block, status word) and the
+    // partial banks, cleared by a kernel (the array keeps the last call's data)
+    hipLaunchKernelGGL(k_fill_zero, dim3(grid_for(ctx, kNoAllocHead + 4 * kNoAllocStride, kBlock)),
+                       dim3(kBlock), 0, s, kNoAllocHead + 4 * kNoAllocStride, base);
+    hipLaunchKernelGGL(k_check_symmetric, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, n,
+                       nnz, col_ptr, row_idx, val, bad);
+    FFH_HIP(hipGetLastError());
+    int h_bad = 0;
+    FFH_HIP(hipMemcpyAsync(&h_bad, bad, sizeof(int), hipMemcpyDeviceToHost, s));
+    FFH_HIP(hipStreamSynchronize(s));
+    FFH_REQUIRE(h_bad == 0, FFH_ERR_INVALID,
+                "allocation-free solve: the CSC must be 0-based, rows ascending in each column, "
+                "and exactly symmetric");
+
+    FFH_HIP(hipEventRecord(ctx->ev_start, s));
+    KrylovCtl out = pcg_fp64_on(
+        ctx, A.m, vi, w, o.method, o.preconditioner, o.tolerance, o.tolerance_type,
+        o.max_iterations, o.breakdown_mode, o.check_interval, [&](auto &enqueue, int) {
+            // one chunk at a time, the control block read back into host
+            // stack memory after each
+            KrylovCtl c{};
+            do {
+                enqueue();
+                FFH_HIP(hipMemcpyAsync(&c, ctl, sizeof(KrylovCtl), hipMemcpyDeviceToHost, s));
+                FFH_HIP(hipStreamSynchronize(s));
+            } while (!c.done);
+            return c;
+        });
+    launch_residual_on(ctx, A.m, vi, x, b, w.r, partials, ctl);
+    FFH_HIP(hipEventRecord(ctx->ev_stop, s));
+    KrylovCtl fin{};
+    FFH_HIP(hipMemcpyAsync(&fin, ctl, sizeof(KrylovCtl), hipMemcpyDeviceToHost, s));
+    FFH_HIP(hipStreamSynchronize(s));
+    float ms = 0.0f;
+    FFH_HIP(hipEventElapsedTime(&ms, ctx->ev_start, ctx->ev_stop));
+    st.converged = out.converged;
+    if (!st.converged && out.residual_norm <= out.tol) st.converged = 1;
+    st.iterations = out.iter;
+    st.final_residual = out.residual_norm;
+    st.solve_time = ms * 1e-3;
+    st.refinements = 0;
+    st.true_residual = std::sqrt(fin.true_rr);
+    st.initial_residual = out.initial_norm;
 }
 
 void invalidate_graphs(Workspace &w)

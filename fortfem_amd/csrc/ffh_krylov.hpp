@@ -16,6 +16,12 @@ void solve_mixed(Context *ctx, Matrix &M, const double *b, double *x,
 int select_spmv_variant(Context *ctx, Matrix &M, bool f32, int request);
 double bench_spmv(Context *ctx, Matrix &M, int precision, int reps,
                   double *bytes);
+// allocation-free FP64 solve on device arrays (the scratch array)
+void scratch_setup(int device);
+int noalloc_variant(int n, int64_t nnz);
+void solve_fp64_noalloc(Context *ctx, int n, int64_t nnz, const int *col_ptr,
+                        const int *row_idx, const double *val, const double *b, double *x,
+                        const ffh_solver_options &o, ffh_solver_stats &st);
 } // namespace ffh
 
 namespace ffh {

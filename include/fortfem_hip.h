@@ -303,6 +303,36 @@ int ffh_spmv(ffh_matrix *mat, const double *x, double *y);
 int ffh_solve_sparse(ffh_matrix *mat, const double *b, double *x,
                      const ffh_solver_options *opts, ffh_solver_stats *stats);
 
+/* ---- allocation-free solve -------------------------------------------------
+ * No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): the FP64 Jacobi-PCG / CG of ffh_solve_sparse on a CSC
+ * matrix and vectors the caller already holds in device memory, with every
+ * work vector in a fixed array of the library (about 56 bytes per row for up
+ * to FFH_NOALLOC_MAX_ROWS rows plus 256 KiB, about 113 MiB in all, held by
+ * each GPU the library runs on from ffh_init until the process ends).  No
+ * hipMalloc, hipFree or allocating library routine is called.
+ *   col_ptr(n+1), row_idx(nnz), val(nnz): 0-based device arrays, rows
+ *     strictly ascending in each column, the matrix exactly symmetric (it is
+ *     read as CSR; checked on the device, -1 otherwise);
+ *   b(n) device, x(n) device: initial guess on entry, solution on exit.
+ * opts->precision must be FFH_FP64 and use_graph is ignored; n above
+ * FFH_NOALLOC_MAX_ROWS is refused with -1.  Work runs on the context's
+ * stream.  All calls on one GPU share the array: calls that overlap on one
+ * GPU (other threads, other contexts of the device) race and must be
+ * serialised by the caller.  For a symmetric matrix the result is that of
+ * ffh_solve_sparse with the SpMV variant ffh_noalloc_spmv_variant(n, nnz)
+ * (bit for bit). */
+#define FFH_NOALLOC_MAX_ROWS (1 << 21)
+int ffh_solve_sparse_device_noalloc(ffh_context *ctx, int n, int64_t nnz, const int *col_ptr,
+                                    const int *row_idx, const double *val, const double *b,
+                                    double *x, const ffh_solver_options *opts,
+                                    ffh_solver_stats *stats);
+/* The SpMV variant (ffh_matrix_spmv_variant numbering) that
+ * ffh_solve_sparse_device_noalloc uses for a matrix of n rows and nnz
+ * entries: vector CSR, lanes per row from the mean row length. */
+int ffh_noalloc_spmv_variant(int n, int64_t nnz);
+
 /* ---- served solves (continuous batching) ----------------------------------
  * No single reference routine: the request loop serves what the reference
  * does one call at a time (assemble_* + solve_sparse per right-hand side,
