@@ -308,6 +308,15 @@ class Problem:
         self._check(self._lib.ffh_global_dofs_download(self.handle, base, _ptr(g)))
         return g
 
+    def colouring(self, base=0):
+        """(color_start, cells): the element colouring of the pattern build,
+        cells listed colour by colour (ffh_problem_colouring)."""
+        s = self.sizes()
+        cs = np.zeros(s["n_colors"] + 1, dtype=np.int32)
+        cells = np.zeros(self.ncell, dtype=np.int32)
+        self._check(self._lib.ffh_problem_colouring(self.handle, base, _ptr(cs), _ptr(cells)))
+        return cs, cells
+
     def assemble(self, quadrature_degree, kappa=1.0, mass=0.0, cell_kappa=None, precision="fp64",
                  reuse_coefficients=False, cell_tensor=None, reuse_tensor=False):
         """Assemble the family's operator.  H1: kappa (stiffness) and mass;

@@ -54,7 +54,7 @@ EXPORTS = (
     "ffh_zmatrix_solve", "ffh_zmatrix_free",
     "ffh_server_create", "ffh_server_free", "ffh_server_prepare", "ffh_server_submit", "ffh_server_step",
     "ffh_server_poll", "ffh_server_wait", "ffh_server_stats", "ffh_server_progress", "ffh_problem_solution", "ffh_server_set_precision",
-    "ffh_solve_sparse_device_noalloc", "ffh_noalloc_spmv_variant",
+    "ffh_solve_sparse_device_noalloc", "ffh_noalloc_spmv_variant", "ffh_problem_colouring",
 )
 
 NOALLOC_MAX_ROWS = 1 << 21
@@ -173,6 +173,7 @@ def load() -> C.CDLL:
         "ffh_solve_sparse_device_noalloc": (C.c_int, [vp, C.c_int, C.c_int64, vp, vp, vp, vp, vp,
                                                       C.POINTER(SolverOptions), C.POINTER(SolverStats)]),
         "ffh_noalloc_spmv_variant": (C.c_int, [C.c_int, C.c_int64]),
+        "ffh_problem_colouring": (C.c_int, [vp, C.c_int, vp, vp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)

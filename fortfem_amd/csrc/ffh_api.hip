@@ -389,6 +389,16 @@ int ffh_global_dofs_download(ffh_problem *P, int base, int *gd)
     return FFH_OK;
 }
 
+int ffh_problem_colouring(ffh_problem *P, int base, int *color_start, int *cells)
+{
+    if (!P) return FFH_ERR_HANDLE;
+    if (!color_start || !cells || (base != 0 && base != 1) || P->n_global == 0)
+        return FFH_ERR_INVALID;
+    for (size_t c = 0; c < P->color_start.size(); ++c) color_start[c] = P->color_start[c];
+    for (size_t i = 0; i < P->perm.size(); ++i) cells[i] = P->perm[i] + base;
+    return FFH_OK;
+}
+
 int ffh_assemble(ffh_problem *P, int qdeg, int coeff_kind, const double *coeff,
                  double mass, int precision)
 {

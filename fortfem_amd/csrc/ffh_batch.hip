@@ -62,6 +62,10 @@ struct BatchScal {
     int pad;
 };
 constexpr int kAbsTol = 1;
+// BatchScal::restart codes: 1 restart the direction, 2 breakdown (restart
+// after the refresh), 3 converged by the FP64 recurrence, kCapped stopped
+// at the iteration cap (the refresh records the true residual of x)
+constexpr int kCapped = 4;
 
 // R values of one row, moved with vector accesses (R = 6: three pairs)
 template <typename T, int R>
@@ -414,7 +418,11 @@ __global__ __launch_bounds__(256) void k_b_beta(const BatchSys *__restrict__ sys
                 q.state = kCandidate;
                 q.restart = 3;
             } else if (q.iters >= q.max_iter) {
-                q.state = kStopped;
+                // iteration cap: x stays as it is and the chunk's FP64
+                // refresh measures its true residual, which decides
+                // converged or stopped (finalize_solver_status, :1075-1086)
+                q.state = kCandidate;
+                q.restart = kCapped;
             }
         }
     }
@@ -606,6 +614,16 @@ __global__ __launch_bounds__(256) void k_b_refresh(const BatchSys *__restrict__ 
         const bool broke = q.restart == 2;
         const bool done_by_recurrence = q.state == kCandidate && q.restart == 3;
         q.restart = 0;
+        if ((q.state == kActive || q.state == kCandidate || q.state == kFresh) &&
+            !(trr <= 1.0e300 && tbb <= 1.0e300)) {
+            // a non-finite residual (overflow, or non-finite data): this
+            // right-hand side stops, unconverged, and the others go on
+            q.true_rr = trr;
+            q.rr = trr;
+            if (q.state == kFresh) q.bb = tbb;
+            q.state = kStopped;
+            return;
+        }
         if (q.state == kFresh) {
             // compute_solver_tolerance (:1063-1073): tol2 holds tol^2 until
             // here; relative means tol ||b - A x0||
@@ -1148,6 +1166,17 @@ int64_t server_submit(ffh_server *S, int mesh_key, int family, int order, const 
     FFH_REQUIRE(nrhs >= 1 && nrhs <= kMaxRhs, FFH_ERR_INVALID, "1..8 right-hand sides");
     FFH_REQUIRE(coeff && rhs, FFH_ERR_INVALID, "coefficients and right-hand sides required");
     FFH_REQUIRE(tol > 0 && max_iter > 0, FFH_ERR_INVALID, "tolerance and iteration cap");
+    FFH_REQUIRE(std::isfinite(mass) && std::isfinite(tol), FFH_ERR_INVALID,
+                "non-finite mass coefficient or tolerance");
+    for (int c = 0; c < P->ncell; ++c)
+        FFH_REQUIRE(std::isfinite(coeff[c]), FFH_ERR_INVALID,
+                    "non-finite cell coefficient (cell " + std::to_string(c) + ")");
+    const int nfree = P->reduced.m.n;
+    for (int r = 0; r < nrhs; ++r)
+        for (int i = 0; i < nfree; ++i)
+            FFH_REQUIRE(std::isfinite(rhs[(size_t)r * nfree + i]), FFH_ERR_INVALID,
+                        "non-finite right-hand side " + std::to_string(r) + " (row " +
+                            std::to_string(i) + ")");
     auto q = std::make_unique<ServedRequest>();
     q->ticket = S->next_ticket++;
     q->sym = P;

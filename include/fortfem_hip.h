@@ -172,6 +172,12 @@ int ffh_problem_sizes(const ffh_problem *prob, int *n_global, int64_t *nnz,
                       int *n_colors);
 /* global_dofs(n_local, ncell) as the reference dof map returns it */
 int ffh_global_dofs_download(ffh_problem *prob, int base, int *global_dofs);
+/* The element colouring of ffh_pattern_build (no reference counterpart: the
+ * reference adds its triplets one element at a time): color_start
+ * (n_colors + 1 offsets into cells) and cells (ncell cell ids of the given
+ * base, colour by colour, in the order the scatter visits them).  No two
+ * cells of one colour share a dof; the scatter relies on it. */
+int ffh_problem_colouring(ffh_problem *prob, int base, int *color_start, int *cells);
 
 /* Numeric assembly into the fixed pattern: the element loop + triplets +
  * csc_from_triplet of assemble_tetra_lagrange_stiffness_csc (:276-334, per

@@ -14,6 +14,7 @@ import numpy as np
 import pytest
 
 import oracle as O
+from colouring import assert_valid_colouring
 
 pytestmark = pytest.mark.gpu
 
@@ -50,6 +51,7 @@ def test_lagrange_dofs_on_scrambled_mesh(ff, degree):
     st, g, G = O.lagrange_dof_map(degree, t)
     assert st == 0 and n == G
     assert np.array_equal(p.global_dofs(base=1), g)
+    assert_valid_colouring(p)
 
 
 @pytest.mark.parametrize("k", [1, 2, 3, 4])
@@ -59,6 +61,7 @@ def test_nedelec_dofs_on_scrambled_mesh(ff, k):
     n, _ = p.pattern_build(ff.capi.FAMILY_NEDELEC, k)
     gdo, eo, fp, nd, ne, nf = O.nedelec_dof_map(k, t)
     assert n == nd and np.array_equal(p.global_dofs(), gdo)
+    assert_valid_colouring(p)
 
 
 @pytest.mark.parametrize("degree", [1, 2])

@@ -46,6 +46,9 @@ def test_dof_map_matches_oracle(ff, degree):
     st, g, G = O.lagrange_dof_map(degree, t)
     assert n == G
     assert np.array_equal(p.global_dofs(base=1), g)
+    from colouring import assert_valid_colouring
+
+    assert_valid_colouring(p)
 
 
 @pytest.mark.parametrize("degree,N", [(1, 5), (2, 4), (3, 2), (4, 2)])

@@ -27,6 +27,9 @@ def test_c1_system_and_solution(ff, n, f, g, mass):
     p = ff.Problem(v, t)
     G, _ = p.pattern_build(ff.capi.FAMILY_P1_2D, 1)
     assert G == n * n
+    from colouring import assert_valid_colouring
+
+    assert_valid_colouring(p)
     p.assemble(0, kappa=1.0, mass=mass)
     p.load(0, ff.capi.SOURCE_CONSTANT, [f])
     p.dirichlet(1, values=np.full(G, g))

@@ -155,3 +155,59 @@ def test_wait_returns_the_request_result(ff):
     assert conv.all() and np.array_equal(x, xa)
     for tk2 in tickets[:2] + tickets[3:]:
         assert srv.wait(*tk2, max_steps=20000) is not None
+
+
+def test_non_finite_request_fails_alone(ff):
+    """A request whose right-hand side (or coefficient) holds a NaN or an
+    infinity is refused at submission with -1 (FFH_ERR_INVALID, the
+    reference C API's invalid-input code, include/fortfem.h:22-23) naming the
+    offending entry; the server keeps serving, and the other requests'
+    results are bit for bit what they are alone."""
+    alone = ff.Server(budget_bytes=1 << 30, max_systems=16, chunk=8)
+    reqs = make_requests(ff, alone, seed=7)
+    ref = {}
+    for i in (0, 2):
+        r = reqs[i]
+        tk = alone.submit(r["key"], r["fam"], r["k"], r["coeff"], r["b"], mass=r["mass"])
+        ref[i] = alone.run_until_done([tk], max_steps=20000)[tk[0]][0]
+    mix = ff.Server(budget_bytes=1 << 30, max_systems=16, chunk=8)
+    reqs2 = make_requests(ff, mix, seed=7)
+    t0 = mix.submit(reqs2[0]["key"], reqs2[0]["fam"], reqs2[0]["k"], reqs2[0]["coeff"], reqs2[0]["b"],
+                    mass=reqs2[0]["mass"])
+    mix.step()
+    bad = reqs2[1]
+    b = bad["b"].copy()
+    b[0, 3] = np.nan
+    with pytest.raises(ff.FFHError) as e:
+        mix.submit(bad["key"], bad["fam"], bad["k"], bad["coeff"], b, mass=bad["mass"])
+    assert e.value.code == ff.capi.FFH_ERR_INVALID and "non-finite" in e.value.message
+    c = bad["coeff"].copy()
+    c[5] = np.inf
+    with pytest.raises(ff.FFHError) as e:
+        mix.submit(bad["key"], bad["fam"], bad["k"], c, bad["b"], mass=bad["mass"])
+    assert e.value.code == ff.capi.FFH_ERR_INVALID
+    t2 = mix.submit(reqs2[2]["key"], reqs2[2]["fam"], reqs2[2]["k"], reqs2[2]["coeff"], reqs2[2]["b"],
+                    mass=reqs2[2]["mass"])
+    out = mix.run_until_done([t0, t2], max_steps=20000)
+    assert np.array_equal(out[t0[0]][0], ref[0])
+    assert np.array_equal(out[t2[0]][0], ref[2])
+
+
+@pytest.mark.parametrize("policy", [0, 1])
+def test_iteration_cap_reports_true_residual(ff, policy):
+    """A right-hand side stopped by its iteration cap in mid-chunk (cap 13,
+    chunks of 8) reports iterations = 13, converged = false, and the relative
+    residual ||b - A x|| / ||b|| of the x it returns (the FP64 refresh after
+    the cap measures it), checked against the oracle's matrix."""
+    srv = ff.Server(budget_bytes=1 << 30, max_systems=16, chunk=8)
+    srv.set_precision(policy)
+    reqs = make_requests(ff, srv, seed=9)
+    r = reqs[0]
+    tk = srv.submit(r["key"], r["fam"], r["k"], r["coeff"], r["b"], mass=r["mass"], tol=1e-14,
+                    max_iterations=13)
+    x, it, conv, rel = srv.run_until_done([tk], max_steps=100)[tk[0]]
+    assert (it == 13).all() and not conv.any()
+    _, _, red = oracle_system(ff, r["fam"], r["k"], r["N"], r["coeff"])
+    for j in range(r["b"].shape[0]):
+        res = np.linalg.norm(r["b"][j] - red.matvec(x[j])) / np.linalg.norm(r["b"][j])
+        assert abs(rel[j] - res) <= 1e-9 * res, (j, rel[j], res)

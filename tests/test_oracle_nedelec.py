@@ -231,3 +231,15 @@ def test_weighted_identity_tensor_equals_curl_mass(k):
     W = O.nedelec_csc(k, v, t, beta=0.7, cell_tensor=K)
     assert np.array_equal(A.row_idx, W.row_idx)
     assert np.abs(A.val - W.val).max() < 1e-12 * np.abs(A.val).max()
+
+
+def test_high_order_fixture_is_the_oracle():
+    """tests/golden/nedelec_high_order.npz holds the oracle's output: the
+    cheapest of its cases (tensor-weighted k = 4 on the relabelled box) is
+    recomputed here and must match bit for bit."""
+    import os
+
+    g = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "nedelec_high_order.npz"))
+    a = O.nedelec_csc(4, g["box_v"], g["box_t"], alpha=1.0, beta=0.6, cell_tensor=g["box_K"])
+    assert np.array_equal(a.col_ptr, g["box_k4w_col_ptr"]) and np.array_equal(a.row_idx, g["box_k4w_row_idx"])
+    assert np.array_equal(a.val, g["box_k4w_val"])
