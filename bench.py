@@ -1,20 +1,25 @@
 """Benchmark of the fortfem sparse-assembly + Jacobi-PCG path on one MI355X.
 
-Workload (default ``c2``): P2 tetrahedral Poisson on the unit cube split into
-32^3 hexes x 6 tets (196,608 tets, 274,625 dofs, 250,047 free), per-cell
-kappa = exp(N(0,1)) from a seed, load sin(pi x) sin(pi y) sin(pi z), u = 0 on
-the boundary, Jacobi-PCG to a relative residual of 1e-10.
+Workload (default ``c4``): P1 tetrahedral Poisson on the unit cube split into
+128^3 hexes x 6 tets (12,582,912 tets, 2,146,689 dofs, 2,048,383 free), per-cell
+kappa = exp(N(0,1)) from a seed, right-hand side U[-1, 1] per free dof, u = 0
+on the boundary, Jacobi-PCG to a relative residual of 1e-10 (``--workload c2``:
+P2 on 32^3 x 6 with the sin load).
 
 One step = numeric assembly of the stiffness into the resident CSC pattern +
-load + Dirichlet reduction + the solve, with every input already in HBM.
+right-hand side + Dirichlet reduction + the solve, with every input already in
+HBM.
 The headline solve uses the mixed path (FP32 Jacobi-PCG iterations with FP64
 x, reductions and residual refreshes); the FP64 path (the reference's
 recurrence) is timed beside it and the mixed solution's relative error
 against it is reported.
 
 Prints one JSON line.  The dominant kernel's roofline figure comes from HIP
-events recorded around every FP32 direction + SpMV launch inside the timed
-solves.
+events recorded around every 8th FP32 direction + SpMV launch inside the timed
+solves.  The same line carries the served-stream keys (``served``: the first
+requests of the seeded request stream, batched mixed against one-at-a-time
+FP64, saturated and under Poisson arrivals) and the CPU baseline (the oracle
+on all host cores on the full configuration).
 """
 from __future__ import annotations
 
@@ -32,11 +37,13 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 
 
-# HBM bytes per launch of the roofline kernel (k_pcg_dirspmv<float>), from
-# rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE passes over this script, scaled by
-# 2.0 and 1.0 (what the two counters read for a 256 MiB device copy).
-# Re-measure when the kernel changes.
-MEASURED_TRAFFIC = {"c2": 60.80e6}
+# HBM bytes per launch of the roofline kernel, from rocprofv3 --pmc FETCH_SIZE
+# and WRITE_SIZE passes over this script at the named commit, each scaled by
+# what the counter reads for a device copy of known size (the summary and the
+# commands are in profiles/).  None where no such measurement exists for the
+# kernel as it is.
+MEASURED_TRAFFIC = {}
+TRAFFIC_SOURCE = {}
 
 
 def spmv_bytes(nnz: int, n: int, sv: int) -> float:
@@ -153,17 +160,22 @@ def run_c3(args):
     print(json.dumps(out))
 
 
-def run_c5(args):
-    """Served stream: requests of the seeded stream (P1 35 %, P2 35 %, Nedelec
-    k=1 15 %, k=2 15 %; cube N^3 x 6; lognormal per-cell coefficient; 1..8
-    right-hand sides U[-1, 1]) solved to a relative residual of 1e-10 by the
-    batched server (FP32 iterations, FP64 refresh) and, for the baseline, one
-    right-hand side at a time by the FP64 Jacobi-PCG of the drop-in path.
-    Patterns are built once per (N, family, order) before timing."""
+def served_stream(args):
+    """Served stream: the first `requests` requests of the seeded stream of
+    4096 (P1 35 %, P2 35 %, Nedelec k=1 15 %, k=2 15 %; cube N^3 x 6 with N in
+    8..48; lognormal per-cell coefficient; 1..8 right-hand sides U[-1, 1]),
+    every one of them, solved to a relative residual of 1e-10 by the batched
+    server (FP32 iterations, FP64 refresh).  Three timed passes over the same
+    requests: saturated (all queued at t = 0: served solves/s), Poisson
+    arrivals at 0.8 x the saturated request rate (latency p50 / p99 from
+    arrival to result), and the baseline, one right-hand side at a time by the
+    FP64 Jacobi-PCG of the drop-in path on the resident problem (assembly per
+    request included).  Patterns are built once per (N, family, order)
+    before timing."""
     import fortfem_amd as ff
     from fortfem_amd import workloads
 
-    stream = [r for r in workloads.served_stream(args.stream) if r["N"] <= args.max_n][:args.requests]
+    stream = workloads.served_stream(args.stream)[:args.requests]
     meshes = {}
     for r in stream:
         if r["N"] not in meshes:
@@ -190,25 +202,38 @@ def run_c5(args):
     solves = sum(r["nrhs"] for r in stream)
     dofs = sum(r["nrhs"] * r["n_free"] for r in stream)
 
-    # served, saturated: every request queued at t = 0
-    t0 = time.perf_counter()
-    tickets = []
-    for r in stream:
-        tk = srv.submit(r["N"], r["family"], r["order"], r["coeff"], r["rhs"], mass=mass(r), tol=1e-10,
-                        max_iterations=args.max_iter)
-        tickets.append((tk, r))
-    pending = dict((tk[0], (tk, r)) for tk, r in tickets)
-    served = {}
-    while pending:
-        srv.step()
-        for tid in list(pending):
-            tk, r = pending[tid]
-            res = srv.poll(*tk)
-            if res is not None:
-                served[r["index"]] = (res, time.perf_counter() - t0)
-                del pending[tid]
-    t_served = time.perf_counter() - t0
-    lat = np.array([served[r["index"]][1] for r in stream]) * 1e3
+    def serve(arrivals):
+        """Submits request i at arrivals[i] seconds (steps the server while
+        waiting); returns {index: (result, latency_s)} and the wall time."""
+        t0 = time.perf_counter()
+        pending, served, nxt = {}, {}, 0
+        while nxt < len(stream) or pending:
+            now = time.perf_counter() - t0
+            while nxt < len(stream) and arrivals[nxt] <= now:
+                r = stream[nxt]
+                tk = srv.submit(r["N"], r["family"], r["order"], r["coeff"], r["rhs"], mass=mass(r),
+                                tol=1e-10, max_iterations=args.max_iter)
+                pending[tk[0]] = (tk, r, arrivals[nxt])
+                nxt += 1
+            if not pending:
+                time.sleep(max(0.0, min(arrivals[nxt] - now, 0.05)))
+                continue
+            srv.step()
+            for tid in list(pending):
+                tk, r, ta = pending[tid]
+                res = srv.poll(*tk)
+                if res is not None:
+                    served[r["index"]] = (res, time.perf_counter() - t0 - ta)
+                    del pending[tid]
+        return served, time.perf_counter() - t0
+
+    served, t_sat = serve([0.0] * len(stream))
+    lat_sat = np.array([served[r["index"]][1] for r in stream]) * 1e3
+    rate = args.arrival_factor * len(stream) / t_sat  # requests per second
+    gaps = -np.log(1.0 - workloads.uniform(workloads.BASE_SEED + 5 + 77, len(stream))) / rate
+    arrivals = np.concatenate([[0.0], np.cumsum(gaps)[:-1]])
+    served_p, t_poi = serve(list(arrivals))
+    lat = np.array([served_p[r["index"]][1] for r in stream]) * 1e3
 
     # FP64 baseline: one right-hand side at a time on the resident problem
     o64 = ff.solver_options(method="pcg", preconditioner="jacobi", tolerance=1e-10, max_iterations=args.max_iter,
@@ -230,28 +255,41 @@ def run_c5(args):
             x64 = p.solution()[free[(r["N"], r["family"], r["order"])]]
             errs.append(float(np.linalg.norm(x_srv[j] - x64) / max(np.linalg.norm(x64), 1e-300)))
     iters_srv = int(sum(int(served[r["index"]][0][1].sum()) for r in stream))
-    conv = all(bool(served[r["index"]][0][2].all()) for r in stream)
+    conv = all(bool(served[r["index"]][0][2].all()) and bool(served_p[r["index"]][0][2].all()) for r in stream)
+    same = all(np.array_equal(served[r["index"]][0][0], served_p[r["index"]][0][0]) for r in stream)
     kinds = {}
     for r in stream:
         kk = ("P%d" % r["order"]) if r["family"] == 1 else ("N%d" % r["order"])
         kinds[kk] = kinds.get(kk, 0) + 1
-    out = {"metric": "served solves/s at 1e-8 rel err vs FP64 (batched mixed vs one-at-a-time FP64)",
-           "value": round(solves / t_served, 2), "unit": "solves/s", "n_gpus": 1, "higher_is_better": True,
-           "dtype": "fp32-iter+fp64-refresh", "data": "synthetic (seeded request stream)",
-           "config": {"workload": "c5 served stream, requests with N <= %d among the first %d of the seeded "
-                                  "stream (%d requests, %d solves): %s" % (args.max_n, args.stream, len(stream),
-                                                                            solves, kinds),
-                      "chunk": args.chunk, "slots": args.slots, "budget_gb": args.budget_gb},
-           "fp64_one_at_a_time_solves_per_s": round(solves / t_base, 2),
-           "speedup_vs_fp64": round((solves / t_served) / (solves / t_base), 2),
-           "dof_per_s": round(dofs / t_served, 1), "fp64_dof_per_s": round(dofs / t_base, 1),
-           "p50_ms": round(float(np.percentile(lat, 50)), 1), "p99_ms": round(float(np.percentile(lat, 99)), 1),
-           "arrival": "all requests queued at t = 0 (saturated)",
-           "max_rel_err_vs_fp64": max(errs), "all_converged": conv,
-           "iterations_served": iters_srv, "iterations_fp64": iters64, "prepare_s": round(t_prep, 2),
-           "precision_policy": args.precision, "switched_to_fp64": srv.set_precision(-1),
-           "served_s": round(t_served, 3), "fp64_s": round(t_base, 3)}
-    print(json.dumps(out))
+    return {
+        "workload": "c5 served stream: requests 0..%d of the seeded 4096-request stream, all of them "
+                    "(%d solves: %s), N in %d..%d" % (len(stream) - 1, solves, kinds,
+                                                      min(r["N"] for r in stream), max(r["N"] for r in stream)),
+        "solves_per_s": round(solves / t_sat, 2),
+        "fp64_one_at_a_time_solves_per_s": round(solves / t_base, 2),
+        "speedup_vs_fp64": round(t_base / t_sat, 2),
+        "dof_per_s": round(dofs / t_sat, 1), "fp64_dof_per_s": round(dofs / t_base, 1),
+        "p50_ms": round(float(np.percentile(lat, 50)), 1), "p99_ms": round(float(np.percentile(lat, 99)), 1),
+        "arrival": "Poisson, rate %.3f requests/s (%.2f x the saturated rate), seeded" % (rate, args.arrival_factor),
+        "poisson_wall_s": round(t_poi, 3),
+        "saturated_p50_ms": round(float(np.percentile(lat_sat, 50)), 1),
+        "saturated_p99_ms": round(float(np.percentile(lat_sat, 99)), 1),
+        "max_rel_err_vs_fp64": max(errs), "all_converged": conv, "poisson_equals_saturated": same,
+        "iterations_served": iters_srv, "iterations_fp64": iters64, "prepare_s": round(t_prep, 2),
+        "chunk": args.chunk, "slots": args.slots, "precision_policy": args.precision,
+        "switched_to_fp64": srv.set_precision(-1),
+        "served_s": round(t_sat, 3), "fp64_s": round(t_base, 3),
+    }
+
+
+def run_c5(args):
+    out = served_stream(args)
+    line = {"metric": "served solves/s at 1e-8 rel err vs FP64 (batched mixed vs one-at-a-time FP64)",
+            "value": out["solves_per_s"], "unit": "solves/s", "n_gpus": 1, "higher_is_better": True,
+            "dtype": "fp32-iter+fp64-refresh", "data": "synthetic (seeded request stream)",
+            "config": {"workload": out["workload"]}}
+    line.update({k: v for k, v in out.items() if k != "workload"})
+    print(json.dumps(line))
 
 
 def one_step(ff, p, meta, rhs, opts):
@@ -263,40 +301,83 @@ def one_step(ff, p, meta, rhs, opts):
     return p.solve(opts, download=False)
 
 
-def cpu_baseline(workload: str, meta: dict, iterations: int, budget_s: float = 20.0):
-    """Oracle (C, 1 core) on a bounded sample of the same workload: assembly +
-    load + reduction on an N/2 sub-box (1/8 of the cells) scaled by 8, plus
-    the per-iteration cost of the oracle PCG on that reduced system scaled by
-    the nnz ratio, times the GPU FP64 iteration count."""
+def host_description():
+    """nproc and the CPU model of the host the baseline runs on."""
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except AttributeError:
+        usable = os.cpu_count() or 1
+    return {"nproc": usable, "cpu_count": os.cpu_count(), "model": model}
+
+
+def cpu_baseline(ff, p, meta, rhs, x_gpu, threads: int, reps: int):
+    """The oracle's restatement of the reference algorithm (C, OpenMP) timed
+    on the full configuration on `threads` host threads: one step = the
+    triplet assembly (the reference's quadrature-point element loop) +
+    csc_from_triplet + the load (P2 workload) + the constrained reduction +
+    Jacobi-PCG to a relative residual of 1e-10, the same work as one GPU
+    step.  The dof map and the boundary flags are set-up on both sides (taken
+    from the GPU problem, whose dof map and boundary equal the oracle's bit
+    for bit).  One warm-up, then `reps` timed steps: median and p99.  The
+    FP64 solution of the last step gives the GPU mixed solution's error
+    against the oracle."""
     import oracle as O
     from fortfem_amd import workloads
 
     N, d, q = meta["N"], meta["degree"], meta["q"]
-    Ns = max(2, N // 2)
-    v, t = O.unit_box_mesh(Ns)
+    v, t = ff.unit_box_mesh(N)
     kappa = workloads.lognormal_kappa(t.shape[0], meta["config"])
-    t0 = time.perf_counter()
-    a, g, G = O.lagrange_stiffness_csc(v, t, d, q, cell_kappa=kappa)
-    b = O.lagrange_load(v, t, d, q, lambda x, y, z: np.sin(np.pi * x) * np.sin(np.pi * y) * np.sin(np.pi * z),
-                        g, G)
-    cons, _ = O.lagrange_dirichlet(v, t, d, g, G)
-    red, free, rb = O.constrained_reduction(a, cons, np.zeros(G), b)
-    t_asm = time.perf_counter() - t0
-    k = 20
-    t0 = time.perf_counter()
-    O.solve_sparse(red, rb, tolerance=0.0, max_iterations=k, preconditioner="jacobi", breakdown="relative")
-    t_it = (time.perf_counter() - t0) / k
-    scale = (N / Ns) ** 3
-    t_solve = scale * t_asm + iterations * scale * t_it
+    g = p.global_dofs(base=1)
+    cons = p.constrained().astype(np.int32)
+    G = int(g.max())
+    sin_f = lambda x, y, z: np.sin(np.pi * x) * np.sin(np.pi * y) * np.sin(np.pi * z)
+
+    def step():
+        a = O.par_lagrange_stiffness_csc(v, t, d, q, g, G, threads, cell_kappa=kappa)
+        b = O.lagrange_load(v, t, d, q, sin_f, g, G) if rhs is None else np.zeros(G)
+        red, free, rb = O.constrained_reduction(a, cons, np.zeros(G), b)
+        del a
+        if rhs is not None:
+            rb = rhs
+        x, st = O.par_solve_sparse(red, rb, threads, tolerance=1e-10, max_iterations=100000,
+                                   breakdown="relative")
+        return x, st, free
+
+    times = []
+    for k in range(reps + 1):
+        t0 = time.perf_counter()
+        x, st, free = step()
+        if k > 0:
+            times.append(time.perf_counter() - t0)
+        print(json.dumps({"progress": "cpu baseline step %d of %d: %.2f s" % (k, reps, time.perf_counter() - t0)}),
+              file=sys.stderr, flush=True)
+    med = float(np.median(times))
+    err = float(np.linalg.norm(x_gpu[free - 1] - x) / np.linalg.norm(x))
+    host = host_description()
     return {
-        "value": 1.0 / t_solve,
+        "value": round(1.0 / med, 4),
         "unit": "solves/s",
-        "cores": 1,
+        "cores": threads,
         "kind": "port",
-        "sample": (f"oracle FP64 on the {Ns}^3x6 sub-box (1/{int(scale)} of the cells): assembly+load+"
-                   f"reduction {t_asm:.2f}s and {k} Jacobi-PCG iterations ({t_it * 1e3:.2f} ms/it), each "
-                   f"scaled by {int(scale)}; solve = {iterations} iterations (GPU FP64 count)"),
-        "seconds_per_solve": t_solve,
+        "sample": (f"the full configuration, no scaling: oracle (C restatement of the reference's triplet "
+                   f"assembly, csc_from_triplet, constrained reduction and Jacobi-PCG) on {threads} OpenMP "
+                   f"threads, 1 warm-up + {reps} timed steps; dof map and boundary flags are set-up"),
+        "median_s": round(med, 3),
+        "p99_s": round(float(np.percentile(times, 99)), 3),
+        "times_s": [round(x_, 3) for x_ in times],
+        "iterations": st.iterations,
+        "host": host,
+        "compiler": "gcc -O2 -ffp-contract=off -fopenmp",
+        "gpu_mixed_rel_err_vs_oracle_fp64": err,
     }
 
 
@@ -305,20 +386,24 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="c2")
-    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--workload", default="c4", help="c4 (default), c2, c3 or c5")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-served", action="store_true", help="skip the served-stream keys")
+    ap.add_argument("--cpu-threads", type=int, default=0, help="CPU baseline threads (0: min(16, nproc))")
+    ap.add_argument("--cpu-reps", type=int, default=5, help="CPU baseline timed steps (after 1 warm-up)")
     ap.add_argument("--graph", action="store_true", help="replay iteration chunks from HIP graphs")
     ap.add_argument("--orders", type=int, nargs="+", default=[1, 2, 3, 4, 5, 6],
                     help="Nedelec orders of the c3 workload")
-    ap.add_argument("--requests", type=int, default=48, help="c5: requests served")
-    ap.add_argument("--stream", type=int, default=4096, help="c5: length of the seeded stream")
-    ap.add_argument("--max-n", type=int, default=24, help="c5: largest cube size taken from the stream")
-    ap.add_argument("--slots", type=int, default=64, help="c5: batch slots")
-    ap.add_argument("--chunk", type=int, default=16, help="c5: iterations per server step")
-    ap.add_argument("--budget-gb", type=float, default=64.0, help="c5: device memory for admitted requests")
-    ap.add_argument("--max-iter", type=int, default=200000, help="c5: iteration cap per right-hand side")
+    ap.add_argument("--requests", type=int, default=16, help="served: the first REQUESTS of the stream")
+    ap.add_argument("--stream", type=int, default=4096, help="served: length of the seeded stream")
+    ap.add_argument("--arrival-factor", type=float, default=0.8,
+                    help="served: Poisson arrival rate as a fraction of the saturated request rate")
+    ap.add_argument("--slots", type=int, default=64, help="served: batch slots")
+    ap.add_argument("--chunk", type=int, default=16, help="served: iterations per server step")
+    ap.add_argument("--budget-gb", type=float, default=64.0, help="served: device memory for admitted requests")
+    ap.add_argument("--max-iter", type=int, default=200000, help="served: iteration cap per right-hand side")
     ap.add_argument("--precision", type=int, default=0,
-                    help="c5: 0 FP32 iterations (FP64 on stagnation), 1 FP64, 2 FP64 for Nedelec")
+                    help="served: 0 FP32 iterations (FP64 on stagnation), 1 FP64, 2 FP64 for Nedelec")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
@@ -416,13 +501,18 @@ def main():
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": MEASURED_TRAFFIC.get(args.workload),
+            "traffic_source": TRAFFIC_SOURCE.get(args.workload),
             "avg_launch_ms": avg_ms,
             "launches": cnt,
             "bytes_per_launch": b32,
         },
     }
+    if not args.no_served:
+        print(json.dumps({"progress": "served stream"}), file=sys.stderr, flush=True)
+        out["served"] = served_stream(args)
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(args.workload, meta, st64.iterations)
+        threads = args.cpu_threads or min(16, host_description()["nproc"])
+        out["cpu_baseline"] = cpu_baseline(ff, p, meta, rhs, xm, threads, args.cpu_reps)
     print(json.dumps(out))
 
 

@@ -748,3 +748,53 @@ def nedelec_field(k, vertices, tets, dofs, cell, xi):
         gd, eo, fp, nd, ne, nf = nedelec_dof_map(k, t)
         c = nedelec_basis_transform(k, eo[cell], fp[cell]) @ dofs[gd[cell] - 1]
     return c @ vals
+
+
+# ------------------------------------------- threaded CPU-baseline stages ---
+def par_lagrange_stiffness_csc(vertices, tets, d, q, global_dofs, G, nthreads, kappa=1.0, mu=0.0,
+                               cell_kappa=None):
+    """lagrange_stiffness_csc's triplet stage and csc_from_triplet on
+    nthreads OpenMP threads, given the dof map: the same CSC, for timing the
+    CPU baseline on all host cores."""
+    v, t, g = _f(vertices), _i(tets), _i(global_dofs)
+    n = g.shape[1]
+    nt = t.shape[0] * n * n
+    rows = np.empty(nt, dtype=np.int32)
+    cols = np.empty(nt, dtype=np.int32)
+    vals = np.empty(nt)
+    ck = None if cell_kappa is None else _f(cell_kappa)
+    _check(_lib().ffo_par_lagrange_stiffness_triplets(
+        C.c_int(nthreads), C.c_int(v.shape[0]), v.ctypes.data_as(C.c_void_p), C.c_int(t.shape[0]),
+        t.ctypes.data_as(C.c_void_p), C.c_int(d), C.c_int(q),
+        None if ck is None else ck.ctypes.data_as(C.c_void_p), C.c_double(kappa), C.c_double(mu),
+        g.ctypes.data_as(C.c_void_p), rows.ctypes.data_as(C.c_void_p), cols.ctypes.data_as(C.c_void_p),
+        vals.ctypes.data_as(C.c_void_p)), "par_stiffness_triplets")
+    cp = np.zeros(G + 1, dtype=np.int32)
+    ri = np.empty(max(nt, 1), dtype=np.int32)
+    va = np.empty(max(nt, 1))
+    nnz = C.c_int64(0)
+    _check(_lib().ffo_par_csc_from_triplet(C.c_int(nthreads), C.c_int(G), C.c_int(G), C.c_int64(nt),
+                                           rows.ctypes.data_as(C.c_void_p), cols.ctypes.data_as(C.c_void_p),
+                                           vals.ctypes.data_as(C.c_void_p), cp.ctypes.data_as(C.c_void_p),
+                                           ri.ctypes.data_as(C.c_void_p), va.ctypes.data_as(C.c_void_p),
+                                           C.byref(nnz)), "par_csc_from_triplet")
+    k = nnz.value
+    del rows, cols, vals
+    return CSC(G, G, cp, ri[:k].copy(), va[:k].copy())
+
+
+def par_solve_sparse(a: CSC, b, nthreads, x0=None, method="pcg", preconditioner="jacobi", tolerance=1e-6,
+                     tolerance_type="relative", max_iterations=1000, breakdown="absolute"):
+    """solve_sparse's loop on nthreads threads (symmetric CSC read as CSR)."""
+    n = a.n_cols
+    x = np.zeros(n) if x0 is None else _f(x0).copy()
+    bb = _f(b)
+    st = _Stats()
+    _check(_lib().ffo_par_solve_sparse(C.c_int(nthreads), C.c_int(n), a.col_ptr.ctypes.data_as(C.c_void_p),
+                                       a.row_idx.ctypes.data_as(C.c_void_p), _f(a.val).ctypes.data_as(C.c_void_p),
+                                       bb.ctypes.data_as(C.c_void_p), x.ctypes.data_as(C.c_void_p),
+                                       C.c_int(METHODS[method]), C.c_int(PRECONDS[preconditioner]),
+                                       C.c_double(tolerance), C.c_int(TOL_TYPES[tolerance_type]),
+                                       C.c_int(max_iterations), C.c_int(BREAKDOWN[breakdown]), C.byref(st)),
+           "par_solve_sparse")
+    return x, SolverStats(bool(st.converged), int(st.iterations), float(st.final_residual))

@@ -134,6 +134,24 @@ int ffo_solve_sparse(int n, const int *col_ptr, const int *row_idx,
                      int max_iterations, int breakdown_mode,
                      ffo_solver_stats *stats);
 
+/* Threaded forms of the assembly and solve stages, for timing the CPU
+ * baseline on all host cores (OpenMP, nthreads threads).  The triplets and
+ * the CSC are the sequential routines' numbers exactly; the PCG reads a
+ * symmetric CSC as CSR and sums its dot products per thread, so it agrees
+ * with ffo_solve_sparse to rounding. */
+int ffo_par_lagrange_stiffness_triplets(int nthreads, int nvert, const double *vertices,
+                                        int ntet, const int *tets, int degree, int qdeg,
+                                        const double *cell_kappa, double kappa, double mu,
+                                        const int *global_dofs, int *rows, int *cols,
+                                        double *vals);
+int ffo_par_csc_from_triplet(int nthreads, int nrow, int ncol, int64_t ntrip, const int *rows,
+                             const int *cols, const double *vals, int *col_ptr, int *row_idx,
+                             double *val, int64_t *nnz);
+int ffo_par_solve_sparse(int nthreads, int n, const int *col_ptr, const int *row_idx,
+                         const double *val, const double *b, double *x, int method, int precond,
+                         double tolerance, int tol_type, int max_iterations, int breakdown_mode,
+                         ffo_solver_stats *stats);
+
 /* ---- tetrahedral first-kind Nedelec (A11-A20) ---------------------------- */
 int ffo_nedelec_dof_count(int k);
 int ffo_nedelec_coefficients(int k, double *coeff);

@@ -13,6 +13,7 @@
 #include "ffo.h"
 
 #include <math.h>
+#include <omp.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -186,5 +187,107 @@ int ffo_constrained_reduction(int n, const int *col_ptr, const int *row_idx,
         }
     }
     free(free_index);
+    return 0;
+}
+
+/* ffo_csc_from_triplet on nthreads threads, with the same result: triplets
+ * are split into nthreads contiguous ranges whose per-column counts place
+ * each range's entries after the earlier ranges' (a stable counting sort by
+ * column); each column is then sorted by row with an insertion sort that
+ * keeps equal rows in triplet order, and equal (row, col) are summed in
+ * triplet order.  For timing the CPU baseline on all host cores. */
+int ffo_par_csc_from_triplet(int nthreads, int nrow, int ncol, int64_t ntrip, const int *rows,
+                             const int *cols, const double *vals, int *col_ptr, int *row_idx,
+                             double *val, int64_t *nnz)
+{
+    if (nrow < 0 || ncol < 0 || ntrip < 0 || nthreads < 1) return 1;
+    int bad = 0;
+#pragma omp parallel for num_threads(nthreads) reduction(| : bad) schedule(static)
+    for (int64_t t = 0; t < ntrip; ++t)
+        if (rows[t] < 1 || rows[t] > nrow || cols[t] < 1 || cols[t] > ncol) bad |= 1;
+    if (bad) return 1;
+    const int T = nthreads;
+    int64_t *hist = calloc((size_t)T * ((size_t)ncol + 1), sizeof(int64_t));
+    int64_t *start = calloc((size_t)ncol + 1, sizeof(int64_t));
+    int64_t *order = malloc(sizeof(int64_t) * (size_t)(ntrip ? ntrip : 1));
+    int64_t *ucount = calloc((size_t)ncol + 1, sizeof(int64_t));
+    if (!hist || !start || !order || !ucount) {
+        free(hist), free(start), free(order), free(ucount);
+        return 1;
+    }
+#pragma omp parallel num_threads(T)
+    {
+        const int th = omp_get_thread_num();
+        const int64_t lo = ntrip * th / T, hi = ntrip * (th + 1) / T;
+        int64_t *h = hist + (size_t)th * ((size_t)ncol + 1);
+        for (int64_t t = lo; t < hi; ++t) h[cols[t] - 1]++;
+#pragma omp barrier
+#pragma omp single
+        {
+            int64_t s = 0;
+            for (int c = 0; c < ncol; ++c) {
+                start[c] = s;
+                for (int k = 0; k < T; ++k) {
+                    int64_t *hk = hist + (size_t)k * ((size_t)ncol + 1);
+                    const int64_t cnt = hk[c];
+                    hk[c] = s;
+                    s += cnt;
+                }
+            }
+            start[ncol] = s;
+        }
+        for (int64_t t = lo; t < hi; ++t) order[h[cols[t] - 1]++] = t;
+#pragma omp barrier
+#pragma omp for schedule(dynamic, 1024)
+        for (int c = 0; c < ncol; ++c) {
+            const int64_t a = start[c], b = start[c + 1];
+            for (int64_t i = a + 1; i < b; ++i) { /* stable insertion sort by row */
+                const int64_t key = order[i];
+                const int r = rows[key];
+                int64_t j = i - 1;
+                while (j >= a && rows[order[j]] > r) {
+                    order[j + 1] = order[j];
+                    --j;
+                }
+                order[j + 1] = key;
+            }
+            int64_t u = 0;
+            for (int64_t i = a; i < b; ++i)
+                if (i == a || rows[order[i]] != rows[order[i - 1]]) ++u;
+            ucount[c] = u;
+        }
+#pragma omp single
+        {
+            int64_t s = 0;
+            for (int c = 0; c < ncol; ++c) {
+                const int64_t u = ucount[c];
+                ucount[c] = s;
+                s += u;
+            }
+            ucount[ncol] = s;
+        }
+#pragma omp for schedule(dynamic, 1024)
+        for (int c = 0; c < ncol; ++c) {
+            int64_t out = ucount[c];
+            col_ptr[c] = (int)(out + 1);
+            const int64_t a = start[c], b = start[c + 1];
+            for (int64_t i = a; i < b;) {
+                const int r = rows[order[i]];
+                double s = vals[order[i]];
+                int64_t k = i + 1;
+                while (k < b && rows[order[k]] == r) {
+                    s = s + vals[order[k]];
+                    ++k;
+                }
+                row_idx[out] = r;
+                val[out] = s;
+                ++out;
+                i = k;
+            }
+        }
+    }
+    col_ptr[ncol] = (int)(ucount[ncol] + 1);
+    *nnz = ucount[ncol];
+    free(hist), free(start), free(order), free(ucount);
     return 0;
 }

@@ -15,6 +15,7 @@
 #include "ffo.h"
 
 #include <math.h>
+#include <omp.h>
 #include <stdlib.h>
 
 static double dot(int n, const double *a, const double *b)
@@ -124,5 +125,93 @@ int ffo_solve_sparse(int n, const int *col_ptr, const int *row_idx,
     free(z);
     free(p);
     free(ap);
+    return 0;
+}
+
+/* The Jacobi-PCG / CG loop of ffo_solve_sparse on nthreads threads, for
+ * timing the CPU baseline on all host cores.  The matrix must be symmetric:
+ * the product reads the CSC as CSR (one row per thread, no write conflicts)
+ * and the dot products are OpenMP reductions, so the sums differ from the
+ * sequential loop's in the last bits (iterations within one or two). */
+static double pdot(int T, int n, const double *a, const double *b)
+{
+    double s = 0.0;
+#pragma omp parallel for num_threads(T) reduction(+ : s) schedule(static)
+    for (int i = 0; i < n; ++i) s += a[i] * b[i];
+    return s;
+}
+
+static void pmatvec(int T, int n, const int *col_ptr, const int *row_idx, const double *val,
+                    const double *x, double *y)
+{
+#pragma omp parallel for num_threads(T) schedule(static)
+    for (int i = 0; i < n; ++i) {
+        double s = 0.0;
+        for (int k = col_ptr[i] - 1; k < col_ptr[i + 1] - 1; ++k) s += val[k] * x[row_idx[k] - 1];
+        y[i] = s;
+    }
+}
+
+int ffo_par_solve_sparse(int nthreads, int n, const int *col_ptr, const int *row_idx,
+                         const double *val, const double *b, double *x, int method, int precond,
+                         double tolerance, int tol_type, int max_iterations, int breakdown_mode,
+                         ffo_solver_stats *stats)
+{
+    const int T = nthreads < 1 ? 1 : nthreads;
+    stats->converged = 0;
+    stats->iterations = 0;
+    stats->final_residual = 0.0;
+    double *d = malloc(sizeof(double) * n), *r = malloc(sizeof(double) * n);
+    double *z = malloc(sizeof(double) * n), *p = malloc(sizeof(double) * n);
+    double *ap = malloc(sizeof(double) * n);
+#pragma omp parallel for num_threads(T) schedule(static)
+    for (int i = 0; i < n; ++i) {
+        double aii = 0.0;
+        for (int k = col_ptr[i] - 1; k < col_ptr[i + 1] - 1; ++k)
+            if (row_idx[k] == i + 1) {
+                aii = val[k];
+                break;
+            }
+        d[i] = (method == 1 && precond == 1 && fabs(aii) > 1.0e-14) ? 1.0 / aii : 1.0;
+    }
+    pmatvec(T, n, col_ptr, row_idx, val, x, ap);
+#pragma omp parallel for num_threads(T) schedule(static)
+    for (int i = 0; i < n; ++i) {
+        r[i] = b[i] - ap[i];
+        z[i] = d[i] * r[i];
+        p[i] = z[i];
+    }
+    double rr = pdot(T, n, r, r);
+    double residual_norm = sqrt(rr);
+    const double tol = tol_type == 1 ? tolerance : tolerance * residual_norm;
+    double rz_old = pdot(T, n, r, z);
+    for (int iter = 1; iter <= max_iterations; ++iter) {
+        pmatvec(T, n, col_ptr, row_idx, val, p, ap);
+        const double pap = pdot(T, n, p, ap);
+        if (breakdown(pap, rz_old, breakdown_mode)) break;
+        const double alpha = rz_old / pap;
+        double rrn = 0.0, rzn = 0.0;
+#pragma omp parallel for num_threads(T) reduction(+ : rrn, rzn) schedule(static)
+        for (int i = 0; i < n; ++i) {
+            x[i] = x[i] + alpha * p[i];
+            r[i] = r[i] - alpha * ap[i];
+            z[i] = d[i] * r[i];
+            rrn += r[i] * r[i];
+            rzn += r[i] * z[i];
+        }
+        residual_norm = sqrt(rrn);
+        stats->iterations = iter;
+        if (residual_norm <= tol) {
+            stats->converged = 1;
+            break;
+        }
+        const double beta = rzn / rz_old;
+#pragma omp parallel for num_threads(T) schedule(static)
+        for (int i = 0; i < n; ++i) p[i] = z[i] + beta * p[i];
+        rz_old = rzn;
+    }
+    if (!stats->converged && residual_norm <= tol) stats->converged = 1;
+    stats->final_residual = residual_norm;
+    free(d), free(r), free(z), free(p), free(ap);
     return 0;
 }

@@ -856,3 +856,51 @@ int ffo_lagrange_dirichlet_search(int ntet, const int *tets, int degree,
     free(nodes);
     return 0;
 }
+
+/* The triplet stage of ffo_lagrange_stiffness_triplets on nthreads threads
+ * (OpenMP over cells, each thread with its own element scratch): the same
+ * numbers in the same positions.  For timing the CPU baseline on all host
+ * cores; the sequential routine above is the one the tests pin. */
+int ffo_par_lagrange_stiffness_triplets(int nthreads, int nvert, const double *vertices,
+                                        int ntet, const int *tets, int degree, int qdeg,
+                                        const double *cell_kappa, double kappa, double mu,
+                                        const int *global_dofs, int *rows, int *cols,
+                                        double *vals)
+{
+    lagrange_table t;
+    if (table_init(&t, degree, qdeg)) return 1;
+    const int n = t.n;
+    int st = 0;
+#pragma omp parallel num_threads(nthreads) reduction(| : st)
+    {
+        double *m = malloc(sizeof(double) * n * n);
+        double *pg = malloc(sizeof(double) * 3 * n);
+#pragma omp for schedule(static)
+        for (int e = 0; e < ntet; ++e) {
+            double v[12];
+            int bad = 0;
+            for (int k = 0; k < 4; ++k) {
+                int id = tets[4 * e + k];
+                if (id < 1 || id > nvert) { bad = 1; break; }
+                for (int c = 0; c < 3; ++c) v[3 * k + c] = vertices[3 * (id - 1) + c];
+            }
+            double ke = cell_kappa ? cell_kappa[e] : kappa;
+            if (bad || element_with_table(&t, v, ke, mu, m, pg)) {
+                st |= 1;
+                continue;
+            }
+            const int *g = global_dofs + (size_t)n * e;
+            size_t entry = (size_t)n * n * e;
+            for (int c = 0; c < n; ++c)
+                for (int r = 0; r < n; ++r, ++entry) {
+                    rows[entry] = g[r];
+                    cols[entry] = g[c];
+                    vals[entry] = m[r + n * c];
+                }
+        }
+        free(m);
+        free(pg);
+    }
+    table_free(&t);
+    return st;
+}
