@@ -26,6 +26,19 @@ __device__ __forceinline__ T ldm(const T *p)
     return *p;
 }
 
+// Workgroups are dispatched to the 8 XCDs round-robin (XCD = blockIdx % 8)
+// and each XCD has its own L2.  The logical block index gives every XCD one
+// contiguous range of blocks (so of rows), so the vector entries an XCD's
+// rows gather stay in its own L2 instead of every L2 holding all of them.
+constexpr int kXcds = 8;
+__device__ __forceinline__ int xcd_block()
+{
+    const int g = gridDim.x, b = blockIdx.x;
+    const int per = g / kXcds, rem = g % kXcds;
+    const int x = b % kXcds, i = b / kXcds;
+    return x * per + min(x, rem) + i;
+}
+
 // ---------------------------------------------------------------- errors --
 struct Error {
     int code;
@@ -128,7 +141,8 @@ struct DevMatrix {
     bool val64_stale = false; // FP32 assembly: val holds no FP64 view yet
     DBuf<int> rowblocks; // CSR-stream row blocks of the gather arrays
     int nrb = 0;
-    int variant[2] = {-1, -1}; // autotuned SpMV variant (FP64, FP32)
+    int variant[2] = {-1, -1};     // autotuned SpMV variant (FP64, FP32)
+    int variant_dir[2] = {-1, -1}; // that of the CG direction launch
     // SELL-64 copy of the gather arrays, built on first use: within each
     // window of 1024 rows the rows are ordered by decreasing length and cut
     // into slices of 64 (sell_row: the row at each slice position, -1 for
@@ -143,6 +157,18 @@ struct DevMatrix {
     DBuf<int> sell_off, sell_idx, sell_src, sell_row;
     DBuf<double> sell_val;
     DBuf<float> sell_val32;
+    // Column tiles of the SELL copy for the batched products (built on first
+    // use by the server): per group of four slices (256 rows, one work item)
+    // the sorted distinct columns its entries read (tile_cols from
+    // tile_off[g]), and per SELL entry its column's position in that list
+    // (16 bits), so a block stages each column once in LDS.
+    bool tiles_built = false, tiles_ok = false;
+    int tile_max = 0; // most distinct columns of one group
+    DBuf<int> tile_off, tile_cols;
+    DBuf<uint16_t> tile_lidx;
+    // the row of each slice position, the tile offsets and the tile columns
+    // in one array (the tiled row engine's argument)
+    DBuf<int> tile_pack;
     // the values (FP64, FP32 or both) were rewritten
     void values_changed()
     {

@@ -76,20 +76,6 @@ __device__ __forceinline__ bool is_breakdown(double pap, double rz, int mode)
     return fabs(pap) < 1.0e-14;
 }
 
-// ----------------------------------------------------- XCD-aware blocks --
-// Workgroups are dispatched to the 8 XCDs round-robin (XCD = blockIdx % 8)
-// and each XCD has its own L2.  The logical block index gives every XCD one
-// contiguous range of blocks (so of rows), so the vector entries an XCD's
-// rows gather stay in its own L2 instead of every L2 holding all of them.
-constexpr int kXcds = 8;
-__device__ __forceinline__ int xcd_block()
-{
-    const int g = gridDim.x, b = blockIdx.x;
-    const int per = g / kXcds, rem = g % kXcds;
-    const int x = b % kXcds, i = b / kXcds;
-    return x * per + min(x, rem) + i;
-}
-
 // ------------------------------------------------------------- gathers --
 // What a row engine multiplies the matrix entries by, in two steps so that
 // the loads can be issued before the scalars they are combined with are
@@ -319,8 +305,83 @@ __device__ __forceinline__ void sell_rows(int n, const int *__restrict__ off,
     }
 }
 
+// SELL-64 with column tiles: a block takes a group of four slices (256
+// rows); the group's distinct columns (its tile, sorted) are loaded once,
+// their values formed (for the direction, with the beta the start hook
+// finds) and staged in LDS, then each wave adds its slice's rows from LDS
+// through 16-bit tile positions.  The matrix stream is the value plus a
+// 2-byte position per entry, and the vector leaves L2 once per group.  Here
+// ptr is the slice offset array, idx the 16-bit positions (as int words),
+// val the SELL values, and pack holds the row of each slice position
+// (nslice * 64), then the tile offsets (ngroup + 1), then the tile columns.
+template <typename MT, typename VT, typename XF, typename E, typename H>
+__device__ __forceinline__ void sell_tiled(const int *__restrict__ off,
+                                           const int *__restrict__ idx,
+                                           const MT *__restrict__ sval,
+                                           const int *__restrict__ pack, const XF &xf,
+                                           int nslice, const E &epi, double *d, const H &hook)
+{
+    extern __shared__ __align__(16) unsigned char dyn_lds[];
+    VT *lds = reinterpret_cast<VT *>(dyn_lds);
+    const uint16_t *lidx = reinterpret_cast<const uint16_t *>(idx);
+    const int ngroup = (nslice + 3) >> 2;
+    const int *srow = pack;
+    const int *toff = pack + (size_t)nslice * 64;
+    const int *tcols = toff + ngroup + 1;
+    const int lane = threadIdx.x & 63;
+    bool hooked = false;
+    for (int g = xcd_block(); g < ngroup || !hooked; g += gridDim.x) {
+        const bool live = g < ngroup;
+        const int c0 = live ? toff[g] : 0;
+        const int nc = live ? toff[g + 1] - c0 : 0;
+        const bool first = (int)threadIdx.x < nc;
+        typename XF::Raw q0{};
+        if (first) q0 = xf.raw(tcols[c0 + threadIdx.x]);
+        // the first column loads are in flight during the hook
+        if (!hooked) {
+            if (!hook()) return;
+            hooked = true;
+        }
+        if (!live) break;
+        if (first) lds[threadIdx.x] = xf.val(q0);
+        for (int k = threadIdx.x + kStreamBlock; k < nc; k += kStreamBlock) lds[k] = xf(tcols[c0 + k]);
+        __syncthreads();
+        const int sl = 4 * g + (int)(threadIdx.x >> 6);
+        if (sl < nslice) {
+            const int o = off[sl], w = (off[sl + 1] - o) >> 6;
+            const uint16_t *ip = lidx + o + lane;
+            const MT *vp = sval + o + lane;
+            VT s = VT(0);
+            constexpr int U = 8;
+            int j = 0;
+            for (; j + U <= w; j += U) {
+                int c[U];
+                MT v[U];
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    c[u] = __builtin_nontemporal_load(ip + 64 * u);
+                    v[u] = __builtin_nontemporal_load(vp + 64 * u);
+                }
+#pragma unroll
+                for (int u = 0; u < U; ++u) s += (VT)v[u] * lds[c[u]];
+                ip += 64 * U;
+                vp += 64 * U;
+            }
+            for (; j < w; ++j) {
+                s += (VT)vp[0] * lds[ip[0]];
+                ip += 64;
+                vp += 64;
+            }
+            const int row = srow[sl * 64 + lane];
+            if (row >= 0) epi(row, s, d);
+        }
+        __syncthreads();
+    }
+}
+
 // Row engine of a launch: V = 0 stream, V = 1 SELL-64 (rowblocks = the row
-// of each slice position), otherwise vector CSR with G = V lanes.
+// of each slice position), V = 2 SELL-64 with column tiles (rowblocks = the
+// tile pack), otherwise vector CSR with G = V lanes.
 template <int V, typename MT, typename VT, typename XF, typename E, typename H = NoHook>
 __device__ __forceinline__ void row_engine(int n, const int *__restrict__ ptr,
                                            const int *__restrict__ idx,
@@ -332,6 +393,8 @@ __device__ __forceinline__ void row_engine(int n, const int *__restrict__ ptr,
         csr_stream<MT, VT>(ptr, idx, val, xf, rowblocks, nrb, epi, d, hook);
     else if constexpr (V == 1)
         sell_rows<MT, VT>(n, ptr, idx, val, rowblocks, xf, nrb, epi, d, hook);
+    else if constexpr (V == 2)
+        sell_tiled<MT, VT>(ptr, idx, val, rowblocks, xf, nrb, epi, d, hook);
     else
         csr_vector<V, MT, VT>(n, ptr, idx, val, xf, epi, d, hook);
 }
@@ -796,7 +859,7 @@ struct SpmvVariant {
 };
 static const SpmvVariant kVariants[] = {{0, 1}, {0, 2}, {0, 4}, {4, 2}, {8, 1},
                                         {8, 2}, {16, 2}, {32, 2}, {32, 4}, {1, 1},
-                                        {1, 2}};
+                                        {1, 2}, {2, 1}, {2, 2}};
 constexpr int kNumVariants = sizeof(kVariants) / sizeof(kVariants[0]);
 
 static int variant_grid(const Context *ctx, const DevMatrix &A, int vi)
@@ -806,11 +869,18 @@ static int variant_grid(const Context *ctx, const DevMatrix &A, int vi)
     int64_t need;
     if (v.engine == 0)
         need = A.nrb;
-    else if (v.engine == 1)
+    else if (v.engine == 1 || v.engine == 2)
         need = ((int64_t)A.nslice + kStreamBlock / 64 - 1) / (kStreamBlock / 64);
     else
         need = ((int64_t)A.n * v.engine + kStreamBlock - 1) / kStreamBlock;
     return (int)std::max<int64_t>(1, std::min(need, cap));
+}
+
+// dynamic LDS of a launch: the tiled engine stages one value (bytes vbytes)
+// per distinct column of a group
+static size_t variant_lds(const DevMatrix &A, int vi, size_t vbytes)
+{
+    return kVariants[vi].engine == 2 ? (size_t)A.tile_max * vbytes : 0;
 }
 
 // Gather arrays of a launch: the CSR (CSC read as CSR) arrays with the row
@@ -826,11 +896,28 @@ struct GArgs {
 template <typename MT>
 static GArgs<MT> gargs(const DevMatrix &A, int vi)
 {
+    // a SELL variant launches only on the arrays ready_variant made current
+    FFH_REQUIRE(kVariants[vi].engine != 1 || A.sell_ok, FFH_ERR_DEVICE,
+                "SELL-64 variant launched without its SELL copy");
+    FFH_REQUIRE(kVariants[vi].engine != 2 || (A.sell_ok && A.tiles_ok && A.tile_pack.p &&
+                                              A.tile_lidx.p),
+                FFH_ERR_DEVICE, "tiled SELL-64 variant launched without its column tiles");
     if (kVariants[vi].engine == 1) {
         if constexpr (sizeof(MT) == 8)
             return {A.sell_off.p, A.sell_idx.p, A.sell_val.p, A.sell_row.p, A.nslice};
         else
             return {A.sell_off.p, A.sell_idx.p, A.sell_val32.p, A.sell_row.p, A.nslice};
+    }
+    if (kVariants[vi].engine == 1 || kVariants[vi].engine == 2) {
+        const void *sv = sizeof(MT) == 8 ? (const void *)A.sell_val.p : (const void *)A.sell_val32.p;
+        FFH_REQUIRE(sv != nullptr, FFH_ERR_DEVICE, "SELL-64 variant launched without its values");
+    }
+    if (kVariants[vi].engine == 2) {
+        const int *lidx = reinterpret_cast<const int *>(A.tile_lidx.p);
+        if constexpr (sizeof(MT) == 8)
+            return {A.sell_off.p, lidx, A.sell_val.p, A.tile_pack.p, A.nslice};
+        else
+            return {A.sell_off.p, lidx, A.sell_val32.p, A.tile_pack.p, A.nslice};
     }
     if constexpr (sizeof(MT) == 8)
         return {A.gptr(), A.gidx(), A.gval(), A.rowblocks.p, A.nrb};
@@ -842,6 +929,7 @@ static GArgs<MT> gargs(const DevMatrix &A, int vi)
     switch (eng) {                                                           \
     case 0: { constexpr int VV = 0; __VA_ARGS__; } break;                     \
     case 1: { constexpr int VV = 1; __VA_ARGS__; } break;                     \
+    case 2: { constexpr int VV = 2; __VA_ARGS__; } break;                     \
     case 4: { constexpr int VV = 4; __VA_ARGS__; } break;                     \
     case 8: { constexpr int VV = 8; __VA_ARGS__; } break;                     \
     case 16: { constexpr int VV = 16; __VA_ARGS__; } break;                   \
@@ -867,7 +955,9 @@ void build_rowblocks(Context *ctx, DevMatrix &A, const std::vector<int> &ptr)
     A.nrb = (int)rb.size() - 1;
     A.rowblocks.upload(rb, ctx->stream);
     A.variant[0] = A.variant[1] = -1;
+    A.variant_dir[0] = A.variant_dir[1] = -1;
     A.sell_built = A.sell_ok = false;
+    A.tiles_built = A.tiles_ok = false;
     A.sell64_valid = A.sell32_valid = false;
     FFH_HIP(hipStreamSynchronize(ctx->stream));
 }
@@ -1012,6 +1102,60 @@ static void ensure_sell_values(Context *ctx, DevMatrix &A, bool f32)
 
 void ensure_sell(Context *ctx, DevMatrix &A, bool f32) { ensure_sell_values(ctx, A, f32); }
 
+// Column tiles of the SELL-64 copy (see DevMatrix::tile_*): for each group of
+// four slices the sorted distinct columns of its entries and the 16-bit
+// position of every entry's column in that list.  Host work, once per
+// structure.
+void ensure_sell_tiles(Context *ctx, DevMatrix &A)
+{
+    if (A.tiles_built) return;
+    ensure_sell_structure(ctx, A);
+    A.tiles_built = true;
+    A.tiles_ok = false;
+    if (!A.sell_ok) return;
+    const int nslice = A.nslice;
+    const int ngroup = (nslice + 3) / 4;
+    hipStream_t s = ctx->stream;
+    std::vector<int> off(nslice + 1), idx((size_t)A.sell_total), srow((size_t)nslice * 64);
+    FFH_HIP(hipMemcpyAsync(off.data(), A.sell_off.p, sizeof(int) * (nslice + 1), hipMemcpyDeviceToHost, s));
+    FFH_HIP(hipMemcpyAsync(idx.data(), A.sell_idx.p, sizeof(int) * idx.size(), hipMemcpyDeviceToHost, s));
+    FFH_HIP(hipMemcpyAsync(srow.data(), A.sell_row.p, sizeof(int) * srow.size(), hipMemcpyDeviceToHost, s));
+    FFH_HIP(hipStreamSynchronize(s));
+    std::vector<std::vector<int>> cols(ngroup);
+    std::vector<uint16_t> lidx(idx.size());
+    parallel_for(ngroup, [&](int64_t lo, int64_t hi) {
+        for (int64_t g = lo; g < hi; ++g) {
+            const int e0 = off[4 * g], e1 = off[std::min<int64_t>(4 * g + 4, nslice)];
+            std::vector<int> &c = cols[g];
+            c.assign(idx.begin() + e0, idx.begin() + e1);
+            std::sort(c.begin(), c.end());
+            c.erase(std::unique(c.begin(), c.end()), c.end());
+            if (c.size() > 65536) continue;
+            for (int e = e0; e < e1; ++e)
+                lidx[e] = (uint16_t)(std::lower_bound(c.begin(), c.end(), idx[e]) - c.begin());
+        }
+    }, 64);
+    std::vector<int> toff(ngroup + 1, 0);
+    int mx = 0;
+    for (int g = 0; g < ngroup; ++g) {
+        if (cols[g].size() > 65536) return;
+        toff[g + 1] = toff[g] + (int)cols[g].size();
+        mx = std::max(mx, (int)cols[g].size());
+    }
+    std::vector<int> pack(srow.size() + (size_t)ngroup + 1 + (size_t)toff[ngroup]);
+    std::copy(srow.begin(), srow.end(), pack.begin());
+    std::copy(toff.begin(), toff.end(), pack.begin() + srow.size());
+    int *tc = pack.data() + srow.size() + ngroup + 1;
+    for (int g = 0; g < ngroup; ++g) std::copy(cols[g].begin(), cols[g].end(), tc + toff[g]);
+    A.tile_off.upload(toff, s);
+    A.tile_cols.upload(tc, (size_t)toff[ngroup], s);
+    A.tile_lidx.upload(lidx, s);
+    A.tile_pack.upload(pack, s);
+    FFH_HIP(hipStreamSynchronize(s));
+    A.tile_max = mx;
+    A.tiles_ok = true;
+}
+
 bool sell_structure(Context *ctx, DevMatrix &A)
 {
     ensure_sell_structure(ctx, A);
@@ -1044,9 +1188,18 @@ void sell_values(Context *ctx, const DevMatrix &A, const double *val, double *ou
 // engine when vi is a SELL variant and the matrix has no SELL copy.
 static int ready_variant(Context *ctx, DevMatrix &A, int vi, bool f32)
 {
-    if (kVariants[vi].engine != 1) return vi;
+    const int eng = kVariants[vi].engine;
+    if (eng != 1 && eng != 2) return vi;
     ensure_sell_values(ctx, A, f32);
-    return A.sell_ok ? vi : 0;
+    if (!A.sell_ok) return 0;
+    if (eng == 2) {
+        ensure_sell_tiles(ctx, A);
+        // the FP64 launches stage 8 bytes per column in at most 64 KiB of
+        // dynamic LDS (their static LDS comes on top, within the 160 KiB a
+        // workgroup may hold)
+        if (!A.tiles_ok || (size_t)A.tile_max * 8 > 65536) return 0;
+    }
+    return vi;
 }
 
 // y = A x with variant vi (its arrays current: see ready_variant)
@@ -1056,7 +1209,8 @@ static void launch_spmv(Context *ctx, const DevMatrix &A, const VT *x, VT *y, in
     const int grid = variant_grid(ctx, A, vi);
     const GArgs<MT> g = gargs<MT>(A, vi);
     FFH_ENGINE(kVariants[vi].engine, {
-        hipLaunchKernelGGL((k_spmv<VV, MT, VT>), dim3(grid), dim3(kStreamBlock), 0,
+        hipLaunchKernelGGL((k_spmv<VV, MT, VT>), dim3(grid), dim3(kStreamBlock),
+                           variant_lds(A, vi, sizeof(VT)),
                            ctx->stream, A.n, g.ptr, g.idx, g.val, x, y, g.rb, g.nrb);
     });
     FFH_HIP(hipGetLastError());
@@ -1066,13 +1220,18 @@ static void launch_spmv(Context *ctx, const DevMatrix &A, const VT *x, VT *y, in
 // with events of its own (10 launches each after a warm-up) on the context
 // stream, into scratch vectors of its own, so that no solver vector or timing
 // event is touched.  FFH_SPMV_VARIANT=<index> forces a variant.
+// best_dir receives the fastest variant that gathers from L2 (not the tiled
+// engine): the CG direction launch forms each gathered column's direction on
+// the fly, which the tiled engine's staging serialises behind its start
+// hook, so that launch takes the best gathering variant.
 template <typename MT, typename VT>
-static int tune_spmv(Context *ctx, DevMatrix &A)
+static int tune_spmv(Context *ctx, DevMatrix &A, int &best_dir)
 {
     constexpr bool f32 = sizeof(MT) == 4;
+    best_dir = 0;
     if (const char *env = std::getenv("FFH_SPMV_VARIANT")) {
         int v = std::atoi(env);
-        if (v >= 0 && v < kNumVariants) return ready_variant(ctx, A, v, f32);
+        if (v >= 0 && v < kNumVariants) return best_dir = ready_variant(ctx, A, v, f32);
     }
     if (A.nnz == 0 || A.n == 0) return 0;
     DBuf<VT> tx, ty;
@@ -1084,13 +1243,14 @@ static int tune_spmv(Context *ctx, DevMatrix &A)
     int best = 0;
     try {
         FFH_HIP(hipEventCreate(&e1));
-        float best_ms = 1e30f;
+        float best_ms = 1e30f, best_dir_ms = 1e30f;
         for (int vi = 0; vi < kNumVariants; ++vi) {
             if (kVariants[vi].engine == 0 && kVariants[vi].gmul > 1 &&
                 A.nrb <= grid_cap(ctx) * (kVariants[vi].gmul / 2))
                 continue; // same grid as the variant before it
-            if (kVariants[vi].engine == 1 && ready_variant(ctx, A, vi, f32) != vi)
-                continue; // no SELL copy of this matrix
+            if ((kVariants[vi].engine == 1 || kVariants[vi].engine == 2) &&
+                ready_variant(ctx, A, vi, f32) != vi)
+                continue; // no SELL copy (or no column tiles) of this matrix
             launch_spmv<MT, VT>(ctx, A, tx.p, ty.p, vi);
             FFH_HIP(hipEventRecord(e0, ctx->stream));
             for (int r = 0; r < 10; ++r) launch_spmv<MT, VT>(ctx, A, tx.p, ty.p, vi);
@@ -1101,6 +1261,10 @@ static int tune_spmv(Context *ctx, DevMatrix &A)
             if (ms < best_ms) {
                 best_ms = ms;
                 best = vi;
+            }
+            if (kVariants[vi].engine != 2 && ms < best_dir_ms) {
+                best_dir_ms = ms;
+                best_dir = vi;
             }
         }
     } catch (...) {
@@ -1119,16 +1283,28 @@ static int spmv_variant(Context *ctx, Matrix &M, bool f32)
 {
     DevMatrix &A = M.m;
     int &v = A.variant[f32 ? 1 : 0];
+    int &vd = A.variant_dir[f32 ? 1 : 0];
     if (v < 0) {
         if (f32) {
             ensure_val32(ctx, A);
-            v = tune_spmv<float, float>(ctx, A);
+            v = tune_spmv<float, float>(ctx, A, vd);
         } else {
-            v = tune_spmv<double, double>(ctx, A);
+            v = tune_spmv<double, double>(ctx, A, vd);
         }
     }
     v = ready_variant(ctx, A, v, f32);
     return v;
+}
+
+// the variant of the CG direction launch (tuned with spmv_variant)
+static int dir_variant(Context *ctx, Matrix &M, bool f32)
+{
+    (void)spmv_variant(ctx, M, f32);
+    DevMatrix &A = M.m;
+    int &vd = A.variant_dir[f32 ? 1 : 0];
+    if (vd < 0) vd = A.variant[f32 ? 1 : 0];
+    vd = ready_variant(ctx, A, vd, f32);
+    return vd;
 }
 
 int select_spmv_variant(Context *ctx, Matrix &M, bool f32, int request)
@@ -1139,6 +1315,7 @@ int select_spmv_variant(Context *ctx, Matrix &M, bool f32, int request)
     if (request >= 0) {
         request = ready_variant(ctx, M.m, request, f32);
         M.m.variant[f32 ? 1 : 0] = request;
+        M.m.variant_dir[f32 ? 1 : 0] = request;
         invalidate_graphs(M.ws);
         return request;
     }
@@ -1180,7 +1357,8 @@ static void launch_iterations(Context *ctx, const DevMatrix &A, double *partials
     for (int it = 0; it < iters; ++it) {
         if (prof_slot >= 0) ctx->prof_mark(prof_slot, true);
         FFH_ENGINE(kVariants[vi].engine, {
-            hipLaunchKernelGGL((k_pcg_dirspmv<VV, MT, VT>), dim3(grids), dim3(kStreamBlock), 0,
+            hipLaunchKernelGGL((k_pcg_dirspmv<VV, MT, VT>), dim3(grids), dim3(kStreamBlock),
+                               variant_lds(A, vi, sizeof(VT)),
                                ctx->stream, n, g.ptr, g.idx, g.val, zp0, zp1, ap, g.rb, g.nrb,
                                partials, stride, ctl);
         });
@@ -1201,7 +1379,8 @@ static void launch_refresh(Context *ctx, Matrix &M, int vi64, int first)
     const GArgs<double> g = gargs<double>(A, vi64);
     if (ctx->profiling) ctx->prof_mark(0, true);
     FFH_ENGINE(kVariants[vi64].engine, {
-        hipLaunchKernelGGL((k_mixed_refresh<VV>), dim3(gr), dim3(kStreamBlock), 0, ctx->stream,
+        hipLaunchKernelGGL((k_mixed_refresh<VV>), dim3(gr), dim3(kStreamBlock),
+                           variant_lds(A, vi64, sizeof(double)), ctx->stream,
                            A.n, g.ptr, g.idx, g.val, w.x.p, w.b.p, w.r32.p, w.dinv32.p,
                            reinterpret_cast<ZP<float> *>(w.p32.p),
                            reinterpret_cast<ZP<float> *>(w.p32b.p), g.rb, g.nrb, w.partials.p,
@@ -1267,7 +1446,8 @@ static void launch_residual_on(Context *ctx, const DevMatrix &A, int vi, const d
     const int gr = variant_grid(ctx, A, vi);
     const GArgs<double> g = gargs<double>(A, vi);
     FFH_ENGINE(kVariants[vi].engine, {
-        hipLaunchKernelGGL((k_residual<VV>), dim3(gr), dim3(kStreamBlock), 0, ctx->stream, A.n,
+        hipLaunchKernelGGL((k_residual<VV>), dim3(gr), dim3(kStreamBlock),
+                           variant_lds(A, vi, sizeof(double)), ctx->stream, A.n,
                            g.ptr, g.idx, g.val, x, b, r, g.rb, g.nrb, partials);
     });
     hipLaunchKernelGGL(k_residual_finish, dim3(1), dim3(kBlock), 0, ctx->stream, partials, gr,
@@ -1297,7 +1477,7 @@ __global__ void k_ctl_set(KrylovCtl *ctl, KrylovCtl v) { *ctl = v; }
 // stop test; run(enqueue_chunk, chunk) drives the chunks until the control
 // block says done and returns its last state.
 template <typename Run>
-static KrylovCtl pcg_fp64_on(Context *ctx, const DevMatrix &A, int vi, const PcgBufs &w,
+static KrylovCtl pcg_fp64_on(Context *ctx, const DevMatrix &A, int vi, int vi_it, const PcgBufs &w,
                              int method, int precond, double tolerance, int tol_type,
                              int max_iter, int breakdown_mode, int check_interval, Run &&run)
 {
@@ -1316,7 +1496,8 @@ static KrylovCtl pcg_fp64_on(Context *ctx, const DevMatrix &A, int vi, const Pcg
     const int stride = partial_stride(ctx);
     const GArgs<double> g = gargs<double>(A, vi);
     FFH_ENGINE(kVariants[vi].engine, {
-        hipLaunchKernelGGL((k_pcg_init<VV, double, double>), dim3(gi), dim3(kStreamBlock), 0, s, n,
+        hipLaunchKernelGGL((k_pcg_init<VV, double, double>), dim3(gi), dim3(kStreamBlock),
+                           variant_lds(A, vi, sizeof(double)), s, n,
                            g.ptr, g.idx, g.val, w.x, w.b, w.r, w.dinv,
                            reinterpret_cast<ZP<double> *>(w.zpa), g.rb, g.nrb, w.partials,
                            stride);
@@ -1328,7 +1509,7 @@ static KrylovCtl pcg_fp64_on(Context *ctx, const DevMatrix &A, int vi, const Pcg
     const int prof = ctx->profiling ? 0 : -1;
     auto enqueue = [&]() {
         launch_iterations<double, double, double>(ctx, A, w.partials, w.ctl, chunk, w.x, w.r,
-                                                  w.zpa, w.zpb, w.ap, w.dinv, vi, prof);
+                                                  w.zpa, w.zpb, w.ap, w.dinv, vi_it, prof);
         hipLaunchKernelGGL(k_pcg_fin, dim3(1), dim3(kBlock), 0, s, w.partials, stride, w.ctl);
         FFH_HIP(hipGetLastError());
     };
@@ -1343,11 +1524,12 @@ static KrylovCtl pcg_fp64(Context *ctx, Matrix &M, int method, int precond, doub
 {
     Workspace &w = M.ws;
     const int vi = spmv_variant(ctx, M, false);
+    const int vd = dir_variant(ctx, M, false);
     const PcgBufs bufs{w.x.p, w.b.p, w.r.p, w.p.p, w.p2.p, w.ap.p, w.dinv.p, w.partials.p, w.ctl.p};
-    return pcg_fp64_on(ctx, M.m, vi, bufs, method, precond, tolerance, tol_type, max_iter,
+    return pcg_fp64_on(ctx, M.m, vi, vd, bufs, method, precond, tolerance, tol_type, max_iter,
                        breakdown_mode, check_interval, [&](auto &enqueue, int chunk) {
-                           return run_chunks(ctx, M, 0, chunk * 64 + vi, use_graph != 0, enqueue,
-                                             [](const KrylovCtl &) { return true; });
+                           return run_chunks(ctx, M, 0, (chunk * 64 + vi) * 64 + vd, use_graph != 0,
+                                             enqueue, [](const KrylovCtl &) { return true; });
                        });
 }
 
@@ -1405,7 +1587,7 @@ void solve_mixed(Context *ctx, Matrix &M, const double *b, double *x,
     ensure_val64(ctx, A);
     ensure_val32(ctx, A);
     const int vi64 = spmv_variant(ctx, M, false);
-    const int vi32 = spmv_variant(ctx, M, true);
+    const int vi32 = dir_variant(ctx, M, true);
     Workspace &w = M.ws;
     const int n = A.n;
     hipStream_t s = ctx->stream;
@@ -1677,7 +1859,7 @@ void solve_fp64_noalloc(Context *ctx, int n, int64_t nnz, const int *col_ptr,
 
     FFH_HIP(hipEventRecord(ctx->ev_start, s));
     KrylovCtl out = pcg_fp64_on(
-        ctx, A.m, vi, w, o.method, o.preconditioner, o.tolerance, o.tolerance_type,
+        ctx, A.m, vi, vi, w, o.method, o.preconditioner, o.tolerance, o.tolerance_type,
         o.max_iterations, o.breakdown_mode, o.check_interval, [&](auto &enqueue, int) {
             // one chunk at a time, the control block read back into host
             // stack memory after each

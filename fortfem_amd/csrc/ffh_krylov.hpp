@@ -68,6 +68,9 @@ bool sell_structure(Context *ctx, DevMatrix &A);
 void ensure_sell(Context *ctx, DevMatrix &A, bool f32);
 void sell_values(Context *ctx, const DevMatrix &A, const double *val, double *out64,
                  float *out32);
+// column tiles of the SELL copy (DevMatrix::tile_*), built once; tiles_ok
+// false when a group reads more than 2^16 columns
+void ensure_sell_tiles(Context *ctx, DevMatrix &A);
 } // namespace ffh
 
 namespace ffh {

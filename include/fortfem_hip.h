@@ -427,7 +427,7 @@ int ffh_bench_spmv(ffh_matrix *mat, int precision, int reps, double *avg_ms,
  * precision; request < 0 keeps the current choice, running the autotuner
  * (HIP-event timing of every variant) if none is made yet.  *chosen receives
  * the variant in use. */
-#define FFH_SPMV_VARIANTS 11
+#define FFH_SPMV_VARIANTS 13
 int ffh_matrix_spmv_variant(ffh_matrix *mat, int precision, int request, int *chosen);
 /* Kernel timing inside solves: when enabled, every 8th direction + SpMV
  * launch of a solve (which 0 = FP64 direction + SpMV of the FP64 solve, or
