@@ -17,52 +17,16 @@
 !   fortfem_tetra_lagrange_solver_3d     src/solvers/fortfem_tetra_lagrange_solver_3d.f90
 
 module fortfem_hip_types
+    ! csc_t, csc_z_t, fortsparse_status_t, status_set and the status codes are
+    ! fortsparse's (the CSC layout of the path is unchanged); dp is real64
     use, intrinsic :: iso_fortran_env, only: dp => real64
+    use fortsparse, only: csc_t, csc_z_t, fortsparse_status_t, status_set, &
+        FORTSPARSE_OK, FORTSPARSE_INVALID_MATRIX, FORTSPARSE_INTERNAL_ERROR
     implicit none
     private
 
-    integer, parameter, public :: FORTSPARSE_OK = 0
-    integer, parameter, public :: FORTSPARSE_INVALID_MATRIX = 2
-    integer, parameter, public :: FORTSPARSE_INTERNAL_ERROR = 3
-
-    ! fortsparse csc_t: nrow, ncol, nnz, col_ptr(ncol+1), row_idx(nnz), val(nnz)
-    type, public :: csc_t
-        integer :: nrow = 0
-        integer :: ncol = 0
-        integer :: nnz = 0
-        integer, allocatable :: col_ptr(:)
-        integer, allocatable :: row_idx(:)
-        real(dp), allocatable :: val(:)
-    end type csc_t
-
-    ! fortsparse csc_z_t: the same layout with complex values
-    type, public :: csc_z_t
-        integer :: nrow = 0
-        integer :: ncol = 0
-        integer :: nnz = 0
-        integer, allocatable :: col_ptr(:)
-        integer, allocatable :: row_idx(:)
-        complex(dp), allocatable :: val(:)
-    end type csc_z_t
-
-    type, public :: fortsparse_status_t
-        integer :: code = 0
-        character(len=256) :: msg = ""
-    end type fortsparse_status_t
-
-    public :: status_set, dp
-
-contains
-
-    subroutine status_set(status, code, msg)
-        type(fortsparse_status_t), intent(inout) :: status
-        integer, intent(in) :: code
-        character(len=*), intent(in) :: msg
-
-        status%code = code
-        status%msg = msg
-    end subroutine status_set
-
+    public :: csc_t, csc_z_t, fortsparse_status_t, status_set, dp
+    public :: FORTSPARSE_OK, FORTSPARSE_INVALID_MATRIX, FORTSPARSE_INTERNAL_ERROR
 end module fortfem_hip_types
 
 module fortfem_hip_runtime
@@ -199,9 +163,21 @@ module fortfem_assembly_tetra_lagrange_arbitrary_order_3d
     use fortfem_hip_capi
     use fortfem_hip_runtime, only: hip_context
     use fortfem_tetra_duffy_quadrature, only: tetra_duffy_quadrature
+    ! element-level routines and the sampled-load tangents stay on the CPU
+    use fortfem_assembly_tetra_lagrange_arbitrary_order_3d_cpu, only: &
+        assemble_tetra_lagrange_stiffness_element, &
+        assemble_tetra_lagrange_stiffness_element_jvp, &
+        assemble_tetra_lagrange_stiffness_element_vjp, &
+        assemble_tetra_lagrange_scalar_load_samples_jvp, &
+        assemble_tetra_lagrange_scalar_load_samples_vjp
     implicit none
     private
 
+    public :: assemble_tetra_lagrange_stiffness_element
+    public :: assemble_tetra_lagrange_stiffness_element_jvp
+    public :: assemble_tetra_lagrange_stiffness_element_vjp
+    public :: assemble_tetra_lagrange_scalar_load_samples_jvp
+    public :: assemble_tetra_lagrange_scalar_load_samples_vjp
     public :: assemble_tetra_lagrange_stiffness_csc
     public :: assemble_tetra_lagrange_stiffness_csc_jvp
     public :: assemble_tetra_lagrange_stiffness_csc_vjp
@@ -496,9 +472,34 @@ module fortfem_assembly_tetra_nedelec_3d
     use fortfem_hip_capi
     use fortfem_hip_runtime, only: hip_context
     use fortfem_assembly_tetra_lagrange_arbitrary_order_3d, only: download_csc
+    ! element-level routines and the sampled vector loads stay on the CPU
+    use fortfem_assembly_tetra_nedelec_3d_cpu, only: &
+        assemble_tetra_nedelec_curl_mass_element, &
+        assemble_tetra_nedelec_curl_mass_element_jvp, &
+        assemble_tetra_nedelec_curl_mass_element_vjp, &
+        assemble_tetra_nedelec_pml_element, assemble_tetra_nedelec_pml_element_jvp, &
+        assemble_tetra_nedelec_pml_element_vjp, &
+        assemble_tetra_nedelec_curvilinear_pml_element, &
+        assemble_tetra_nedelec_curvilinear_pml_element_jvp, &
+        assemble_tetra_nedelec_curvilinear_pml_element_vjp, &
+        assemble_tetra_nedelec_vector_load_samples, &
+        assemble_tetra_nedelec_vector_load_samples_jvp, &
+        assemble_tetra_nedelec_vector_load_samples_vjp
     implicit none
     private
 
+    public :: assemble_tetra_nedelec_curl_mass_element
+    public :: assemble_tetra_nedelec_curl_mass_element_jvp
+    public :: assemble_tetra_nedelec_curl_mass_element_vjp
+    public :: assemble_tetra_nedelec_pml_element
+    public :: assemble_tetra_nedelec_pml_element_jvp
+    public :: assemble_tetra_nedelec_pml_element_vjp
+    public :: assemble_tetra_nedelec_curvilinear_pml_element
+    public :: assemble_tetra_nedelec_curvilinear_pml_element_jvp
+    public :: assemble_tetra_nedelec_curvilinear_pml_element_vjp
+    public :: assemble_tetra_nedelec_vector_load_samples
+    public :: assemble_tetra_nedelec_vector_load_samples_jvp
+    public :: assemble_tetra_nedelec_vector_load_samples_vjp
     public :: assemble_tetra_nedelec_pml_csc
     public :: assemble_tetra_nedelec_curvilinear_pml_csc
     public :: assemble_tetra_nedelec_pml_csc_jvp, assemble_tetra_nedelec_pml_csc_vjp
@@ -1147,10 +1148,11 @@ module fortfem_sparse_matrix
     use fortfem_hip_capi, only: ffh_matrix_upload, ffh_matrix_free, ffh_spmv, &
         ffh_spmv_jvp, ffh_spmv_vjp, FFH_OK
     use fortfem_hip_runtime, only: hip_context
+    use fortfem_sparse_matrix_cpu, only: sparse_from_dense
     implicit none
     private
 
-    public :: sparse_matrix_t, spmv, spmv_jvp, spmv_vjp
+    public :: sparse_matrix_t, sparse_from_dense, spmv, spmv_jvp, spmv_vjp
 
 contains
 
@@ -1248,77 +1250,72 @@ module fortfem_advanced_solvers
     use fortfem_hip_types, only: dp, sparse_matrix_t => csc_t
     use fortfem_hip_capi
     use fortfem_hip_runtime, only: hip_context
+    ! the reference's option and statistics types, the dense solvers and
+    ! their tangents, and the preconditioner names stay those of the CPU
+    ! module (src/solvers/advanced_solvers.f90:21-29); solve_sparse moves to
+    ! the GPU for the methods and preconditioners it has there
+    use fortfem_advanced_solvers_cpu, only: solver_options_t, solver_stats_t, solve, &
+        solver_options, cg_solve, pcg_solve, bicgstab_solve, gmres_solve, cg_solve_jvp, &
+        cg_solve_vjp, pcg_solve_jvp, pcg_solve_vjp, bicgstab_solve_jvp, bicgstab_solve_vjp, &
+        gmres_solve_jvp, gmres_solve_vjp, jacobi_preconditioner, ilu_preconditioner, &
+        ichol_preconditioner, ichol_controlled_preconditioner, &
+        cpu_solve_sparse => solve_sparse
     implicit none
     private
 
-    public :: solver_options_t, solver_stats_t, solver_options, solve_sparse
+    public :: solver_options_t, solver_stats_t
+    public :: solve, solve_sparse, solver_options
+    public :: cg_solve, pcg_solve, bicgstab_solve, gmres_solve
+    public :: cg_solve_jvp, cg_solve_vjp
+    public :: pcg_solve_jvp, pcg_solve_vjp
+    public :: bicgstab_solve_jvp, bicgstab_solve_vjp
+    public :: gmres_solve_jvp, gmres_solve_vjp
+    public :: jacobi_preconditioner, ilu_preconditioner, ichol_preconditioner
+    public :: ichol_controlled_preconditioner
+    ! additions of the GPU path
     public :: solve_sparse_jvp, solve_sparse_vjp
-    public :: jacobi_preconditioner, to_c_options
+    public :: set_hip_solver_mode, to_c_options, solve_sparse_on_gpu
 
-    ! src/solvers/advanced_solvers.f90:32-44 (+ precision / breakdown fields)
-    type :: solver_options_t
-        character(len=32) :: method = "auto"
-        real(dp) :: tolerance = 1.0e-6_dp
-        integer :: max_iterations = 1000
-        character(len=32) :: preconditioner = "none"
-        character(len=32) :: tolerance_type = "relative"
-        integer :: restart = 50
-        logical :: parallel = .false.
-        integer :: num_threads = 1
-        integer :: verbosity = 0
-        real(dp) :: drop_tolerance = 1.0e-4_dp
-        integer :: fill_level = 0
-        character(len=16) :: precision = "fp64"
-        character(len=16) :: breakdown = "absolute"
-        real(dp) :: inner_tolerance = 1.0e-4_dp
-    end type solver_options_t
-
-    ! :47-57
-    type :: solver_stats_t
-        logical :: converged = .false.
-        integer :: iterations = 0
-        real(dp) :: final_residual = 0.0_dp
-        real(dp) :: solve_time = 0.0_dp
-        integer :: memory_usage = 0
-        character(len=32) :: method_used = ""
-        integer :: restarts = 0
-        real(dp) :: parallel_efficiency = 0.0_dp
-        real(dp) :: condition_estimate = 0.0_dp
-    end type solver_stats_t
+    ! GPU solve settings outside solver_options_t (which stays the
+    ! reference's): precision "fp64" (the reference's recurrence) or "mixed"
+    ! (FP32 iterations, FP64 x, reductions and residual refreshes); breakdown
+    ! "absolute" (the reference's |p.Ap| < 1e-14, :897) or "relative"
+    ! (|p.Ap| <= 1e-14 |r.z|)
+    character(len=16), save :: hip_precision = "fp64"
+    character(len=16), save :: hip_breakdown = "absolute"
 
 contains
 
-    ! :1430-1465
-    function solver_options(method, tolerance, max_iterations, preconditioner, &
-            tolerance_type, restart, parallel, num_threads, verbosity, &
-            drop_tolerance, fill_level, precision, breakdown) result(opts)
-        character(len=*), intent(in), optional :: method, preconditioner, &
-            tolerance_type, precision, breakdown
-        real(dp), intent(in), optional :: tolerance, drop_tolerance
-        integer, intent(in), optional :: max_iterations, restart, num_threads, &
-            verbosity, fill_level
-        logical, intent(in), optional :: parallel
-        type(solver_options_t) :: opts
+    subroutine set_hip_solver_mode(precision, breakdown)
+        character(len=*), intent(in), optional :: precision, breakdown
 
-        if (present(method)) opts%method = method
-        if (present(tolerance)) opts%tolerance = tolerance
-        if (present(max_iterations)) opts%max_iterations = max_iterations
-        if (present(preconditioner)) opts%preconditioner = preconditioner
-        if (present(tolerance_type)) opts%tolerance_type = tolerance_type
-        if (present(restart)) opts%restart = restart
-        if (present(parallel)) opts%parallel = parallel
-        if (present(num_threads)) opts%num_threads = num_threads
-        if (present(verbosity)) opts%verbosity = verbosity
-        if (present(drop_tolerance)) opts%drop_tolerance = drop_tolerance
-        if (present(fill_level)) opts%fill_level = fill_level
-        if (present(precision)) opts%precision = precision
-        if (present(breakdown)) opts%breakdown = breakdown
-    end function solver_options
+        if (present(precision)) then
+            if (trim(precision) /= "fp64" .and. trim(precision) /= "mixed") &
+                error stop "set_hip_solver_mode: precision must be fp64 or mixed"
+            hip_precision = precision
+        end if
+        if (present(breakdown)) then
+            if (trim(breakdown) /= "absolute" .and. trim(breakdown) /= "relative") &
+                error stop "set_hip_solver_mode: breakdown must be absolute or relative"
+            hip_breakdown = breakdown
+        end if
+    end subroutine set_hip_solver_mode
 
-    function jacobi_preconditioner() result(precond_name)
-        character(len=32) :: precond_name
-        precond_name = "jacobi"
-    end function jacobi_preconditioner
+    ! The methods and preconditioners the GPU path runs: solve_sparse's
+    ! 'auto' -> 'pcg' and its default branch (any other method name runs
+    ! pcg, :159-161, :183-186), with no or the Jacobi preconditioner.
+    ! gmres, bicgstab and the ILU / IC preconditioners stay on the CPU.
+    logical function solve_sparse_on_gpu(opts)
+        type(solver_options_t), intent(in) :: opts
+
+        select case (trim(opts%method))
+        case ("gmres", "bicgstab")
+            solve_sparse_on_gpu = .false.
+        case default
+            solve_sparse_on_gpu = trim(opts%preconditioner) == "none" .or. &
+                trim(opts%preconditioner) == "jacobi"
+        end select
+    end function solve_sparse_on_gpu
 
     function to_c_options(opts) result(c)
         type(solver_options_t), intent(in) :: opts
@@ -1335,10 +1332,9 @@ contains
         c%max_iterations = int(opts%max_iterations, c_int)
         c%tolerance = opts%tolerance
         c%precision = FFH_FP64
-        if (trim(opts%precision) == "mixed") c%precision = FFH_MIXED
+        if (trim(hip_precision) == "mixed") c%precision = FFH_MIXED
         c%breakdown_mode = 0
-        if (trim(opts%breakdown) == "relative") c%breakdown_mode = 1
-        c%inner_tolerance = opts%inner_tolerance
+        if (trim(hip_breakdown) == "relative") c%breakdown_mode = 1
     end function to_c_options
 
     ! :135-195.  Size errors stop the program as in the reference.
@@ -1358,6 +1354,10 @@ contains
         end if
         if (size(b) /= A_sparse%nrow .or. size(x) /= A_sparse%nrow) then
             error stop "solve_sparse: incompatible vector size"
+        end if
+        if (.not. solve_sparse_on_gpu(opts)) then
+            call cpu_solve_sparse(A_sparse, b, x, opts, stats)
+            return
         end if
         code = ffh_matrix_upload(hip_context(), int(A_sparse%ncol, c_int), &
             int(A_sparse%nnz, c_int64_t), 1_c_int, A_sparse%col_ptr, &
@@ -1400,6 +1400,7 @@ contains
             size(b_dot) /= n .or. size(x_dot) /= n .or. &
             size(A_values_dot) /= A_sparse%nnz) return
         if (opts%max_iterations < 1 .or. opts%tolerance <= 0.0_dp) return
+        if (.not. solve_sparse_on_gpu(opts)) return ! CG / PCG with none or Jacobi only
         code = ffh_matrix_upload(hip_context(), int(n, c_int), &
             int(A_sparse%nnz, c_int64_t), 1_c_int, A_sparse%col_ptr, &
             A_sparse%row_idx, A_sparse%val, mat)
@@ -1440,6 +1441,7 @@ contains
             size(x_bar) /= n .or. size(b_bar) /= n .or. &
             size(A_values_bar) /= A_sparse%nnz) return
         if (opts%max_iterations < 1 .or. opts%tolerance <= 0.0_dp) return
+        if (.not. solve_sparse_on_gpu(opts)) return ! CG / PCG with none or Jacobi only
         code = ffh_matrix_upload(hip_context(), int(n, c_int), &
             int(A_sparse%nnz, c_int64_t), 1_c_int, A_sparse%col_ptr, &
             A_sparse%row_idx, A_sparse%val, mat)
@@ -1471,12 +1473,27 @@ module fortfem_tetra_lagrange_solver_3d
     use fortfem_hip_runtime, only: hip_context
     use fortfem_assembly_tetra_lagrange_arbitrary_order_3d, only: scalar_source_3d
     use fortfem_tetra_duffy_quadrature, only: tetra_duffy_quadrature
-    use fortfem_advanced_solvers, only: solver_options_t, solver_stats_t, &
-        solver_options, to_c_options
+    use fortfem_advanced_solvers, only: solver_options_t, solver_stats_t, to_c_options
+    ! the solution evaluators stay on the CPU
+    use fortfem_tetra_lagrange_solver_3d_cpu, only: assignment(=), &
+        evaluate_tetra_lagrange_solution, evaluate_tetra_lagrange_solution_jvp, &
+        evaluate_tetra_lagrange_solution_vjp, evaluate_tetra_lagrange_solution_at_point, &
+        evaluate_tetra_lagrange_solution_at_point_jvp, &
+        evaluate_tetra_lagrange_solution_at_point_vjp, &
+        evaluate_tetra_lagrange_solution_prepared, &
+        initialize_tetra_lagrange_solution_evaluator, tetra_lagrange_solution_evaluator_t
     implicit none
     private
 
     public :: solve_tetra_lagrange_poisson, solve_tetra_lagrange_diffusion_reaction
+    public :: assignment(=)
+    public :: evaluate_tetra_lagrange_solution, evaluate_tetra_lagrange_solution_jvp
+    public :: evaluate_tetra_lagrange_solution_vjp, evaluate_tetra_lagrange_solution_at_point
+    public :: evaluate_tetra_lagrange_solution_at_point_jvp
+    public :: evaluate_tetra_lagrange_solution_at_point_vjp
+    public :: evaluate_tetra_lagrange_solution_prepared
+    public :: initialize_tetra_lagrange_solution_evaluator
+    public :: tetra_lagrange_solution_evaluator_t
 
 contains
 
@@ -1516,6 +1533,7 @@ contains
 
         type(c_ptr) :: prob
         type(solver_options_t) :: opts
+        type(ffh_solver_options) :: copts
         type(ffh_solver_stats) :: cst
         integer(c_int) :: n_global, code, q
         integer(c_int64_t) :: nnz
@@ -1553,12 +1571,18 @@ contains
             call node_values(vertices, tetrahedra, degree, prob, boundary_value, values)
             code = ffh_dirichlet(prob, 0_c_int, c_loc(values))
         end if
-        opts = solver_options(method="pcg", preconditioner="jacobi", &
-            tolerance=1.0e-10_dp, max_iterations=100000, breakdown="relative")
+        opts%method = "pcg"
+        opts%preconditioner = "jacobi"
+        opts%tolerance = 1.0e-10_dp
+        opts%max_iterations = 100000
         if (present(options)) opts = options
         if (code == FFH_OK) then
             allocate(solution(n_global))
-            code = ffh_problem_solve(prob, to_c_options(opts), solution, cst)
+            ! the scale-invariant breakdown test: these drivers replace a
+            ! direct solve, and the absolute one fires early on fine meshes
+            copts = to_c_options(opts)
+            copts%breakdown_mode = 1
+            code = ffh_problem_solve(prob, copts, solution, cst)
         end if
         if (present(stats)) then
             stats%converged = cst%converged /= 0
@@ -1655,12 +1679,15 @@ module fortfem_tetra_nedelec_solver_3d
     use fortfem_hip_runtime, only: hip_context
     use fortfem_assembly_tetra_nedelec_3d, only: tensor_coefficient_3d, &
         vector_source_3d, sample_vector_source, sample_cell_tensors
-    use fortfem_advanced_solvers, only: solver_options_t, solver_stats_t, &
-        solver_options, to_c_options
+    use fortfem_advanced_solvers, only: solver_options_t, solver_stats_t, to_c_options
+    ! the PML solves stay on the CPU
+    use fortfem_tetra_nedelec_solver_3d_cpu, only: solve_tetra_nedelec_pml, &
+        solve_tetra_nedelec_curvilinear_pml
     implicit none
     private
 
     public :: solve_tetra_nedelec_curl_mass, solve_tetra_nedelec_weighted_curl_mass
+    public :: solve_tetra_nedelec_pml, solve_tetra_nedelec_curvilinear_pml
 
 contains
 
@@ -1737,6 +1764,7 @@ contains
 
         real(c_double), allocatable :: samples(:, :, :)
         type(solver_options_t) :: opts
+        type(ffh_solver_options) :: copts
         type(ffh_solver_stats) :: cst
         type(c_ptr) :: prob
         integer(c_int) :: n_global, code
@@ -1757,12 +1785,17 @@ contains
         if (code == FFH_OK .and. present(zero_tangential_boundary)) then
             if (zero_tangential_boundary) code = ffh_dirichlet(prob, 0_c_int, c_null_ptr)
         end if
-        opts = solver_options(method="pcg", preconditioner="jacobi", &
-            tolerance=1.0e-12_dp, max_iterations=100000, breakdown="relative")
+        opts%method = "pcg"
+        opts%preconditioner = "jacobi"
+        opts%tolerance = 1.0e-12_dp
+        opts%max_iterations = 100000
         if (present(options)) opts = options
         if (code == FFH_OK) then
             allocate(solution(n_global))
-            code = ffh_problem_solve(prob, to_c_options(opts), solution, cst)
+            ! the scale-invariant breakdown test (a direct solve is replaced)
+            copts = to_c_options(opts)
+            copts%breakdown_mode = 1
+            code = ffh_problem_solve(prob, copts, solution, cst)
         end if
         stats%converged = cst%converged /= 0
         stats%iterations = cst%iterations

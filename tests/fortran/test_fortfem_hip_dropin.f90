@@ -686,7 +686,7 @@ contains
         type(solver_stats_t) :: stats
 
         opts = solver_options(method="pcg", preconditioner="jacobi", &
-            tolerance=1.0e-14_dp, max_iterations=10000, breakdown="relative")
+            tolerance=1.0e-14_dp, max_iterations=10000)
         call solve_tetra_lagrange_poisson(vertices, tetrahedra, 4, bubble_source, &
             zero_boundary, solution, status, opts, stats)
         call record(status%code == 0 .and. stats%converged, "P4 bubble solve succeeds")

@@ -110,3 +110,40 @@ def test_fortran_host_dropin_program():
     out = subprocess.run([exe], capture_output=True, text=True, timeout=60)
     assert out.returncode == 0, out.stdout + out.stderr
     assert "host Fortran checks passed" in out.stdout
+
+
+def test_fortran_reference_import_lists_build_and_run():
+    """A Fortran program whose modules import exactly what the reference's
+    callers import from the replaced modules (fortfem_api.f90:1780-1785,
+    fortfem_api_solvers.f90:9-17, fortfem_solvers_laplacian.f90:6-7,
+    bem/laplace_fem_bem_coupling_3d.f90:8-9, solvers/tetra_lagrange_state_3d.f90:
+    3-6, and every other public name of those modules) builds against the
+    drop-in, and the forwarded host helpers (sparse_from_dense, solver_options,
+    the preconditioner names) behave as the reference's."""
+    import subprocess
+
+    exe = os.path.join(ROOT, "tests", "fortran", "test_reference_imports")
+    if not os.path.exists(exe):
+        pytest.skip("Fortran drop-in not built (no flang)")
+    out = subprocess.run([exe], capture_output=True, text=True, timeout=60)
+    assert out.returncode == 0, out.stdout + out.stderr
+    assert "failures 0" in out.stdout
+
+
+@pytest.mark.parametrize("case", ["gmres", "bicgstab", "ilu"])
+def test_fortran_solve_sparse_refuses_cpu_only_methods(case):
+    """solve_sparse with gmres, bicgstab or an ILU preconditioner goes to
+    fortfem's CPU solve_sparse (advanced_solvers.f90:172-186); without
+    fortfem's CPU modules the call stops naming the method, instead of
+    running PCG with status OK."""
+    import subprocess
+
+    exe = os.path.join(ROOT, "tests", "fortran", "test_gmres_refusal")
+    if not os.path.exists(exe):
+        pytest.skip("Fortran drop-in not built (no flang)")
+    out = subprocess.run([exe, case], capture_output=True, text=True, timeout=60)
+    assert out.returncode != 0
+    assert "solve_sparse returned" not in out.stdout
+    text = out.stdout + out.stderr
+    assert "runs on fortfem's CPU solvers" in text
+    assert (case if case != "ilu" else "'ilu'") in text
