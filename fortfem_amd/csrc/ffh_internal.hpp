@@ -275,6 +275,16 @@ struct Problem {
     int tab_qdeg = -1, tab_nq = 0;
     DBuf<double> d_load_tab; // per-point xi (3) + w*phi (n_local)
     DBuf<double> d_elem;     // element matrices / vectors, [entry][pos]
+    // H1 gather assembly: per CSC column (dof) its occurrences (cell e,
+    // local index l) as e * n_local + l in ascending e (gat_ptr, gat_occ),
+    // per occurrence the n_local value positions inside the column
+    // (gat_slot, relative to col_ptr), the column blocks (a run of columns
+    // whose values fit one block's LDS) and per-cell coefficient terms
+    bool gat_ok = false;
+    int n_gblk = 0, gat_nz_max = 0;
+    DBuf<int> d_gat_ptr, d_gat_occ, d_gat_blk;
+    DBuf<uint16_t> d_gat_slot;
+    DBuf<double> d_h1_terms; // 8 per cell (7 used), cell order
     int load_qdeg = -1, load_nq = 0;
     DBuf<int> d_err;
     DBuf<double> d_coeff;

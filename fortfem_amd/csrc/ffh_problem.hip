@@ -271,6 +271,122 @@ __global__ __launch_bounds__(kBlockA) void k_h1_elem_vjp(
     }
 }
 
+// ------------------------------------------------- gather assembly (H1) --
+// The per-cell reference-tensor coefficients of the element matrix,
+// c_k = kappa det G_k (k = 0..5, the G pairs of h1_gterms) and c_6 = mu det,
+// in cell order, 8 values per cell (c_7 unused).
+template <typename CT>
+__global__ __launch_bounds__(kBlockA) void k_h1_terms(int ncell, const int *__restrict__ cells_c,
+                                                      const double *__restrict__ verts,
+                                                      const int *__restrict__ perm,
+                                                      const double *__restrict__ kappa_cell,
+                                                      double kappa, double mu,
+                                                      CT *__restrict__ terms,
+                                                      int *__restrict__ err)
+{
+    const int pos = blockIdx.x * kBlockA + threadIdx.x;
+    if (pos >= ncell) return;
+    const int4 c = reinterpret_cast<const int4 *>(cells_c)[pos];
+    double jac[9], inv[9], det;
+    if (!tet_geometry(verts, c.x, c.y, c.z, c.w, jac, inv, det)) {
+        atomicOr(err, 1);
+        return;
+    }
+    double g[6];
+    h1_gterms(inv, g);
+    const int e = perm[pos];
+    const double ke = kappa_cell ? kappa_cell[e] : kappa;
+    const double kd = ke * det;
+    CT *t = terms + (size_t)8 * e;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) t[k] = (CT)(kd * g[k]);
+    t[6] = (CT)(mu * det);
+    t[7] = CT(0);
+}
+
+// Largest value run of one gather block (values of CT in LDS) and most
+// columns per block (one thread per column).
+constexpr int kGatherNz = 8192;
+constexpr int kGatherCols = 256;
+
+// Assembly by columns: a block takes a run of CSC columns whose values fit
+// LDS; one thread per column walks the column's occurrences (the cells that
+// hold the column's dof, ascending) and adds, for each local row index l_r,
+// the element entry sum_k c_k S_k(l_r, l_c) (+ c_6 M) at its value position;
+// the block then writes the run's values once.  No colours, no atomics, no
+// element matrices in HBM.  Entries (i, j) and (j, i) add the same numbers
+// (the upper-triangle table entry) over the same cells in the same order, so
+// the matrix is exactly symmetric and the result does not depend on the
+// launch.
+template <int NL, typename CT>
+__global__ __launch_bounds__(kGatherCols) void k_h1_gather(
+    const int *__restrict__ blk, const int *__restrict__ col_ptr,
+    const int *__restrict__ gptr, const int *__restrict__ occ,
+    const uint16_t *__restrict__ gslot, const CT *__restrict__ terms,
+    const CT *__restrict__ tab, int mass, CT *__restrict__ val)
+{
+    constexpr int NN = NL * NL;
+    extern __shared__ __align__(16) unsigned char gather_lds[];
+    CT *acc = reinterpret_cast<CT *>(gather_lds);
+    __shared__ CT st[7 * NN];
+    const int c0 = blk[blockIdx.x], c1 = blk[blockIdx.x + 1];
+    const int base = col_ptr[c0], nz = col_ptr[c1] - base;
+    for (int i = threadIdx.x; i < nz; i += kGatherCols) acc[i] = CT(0);
+    for (int i = threadIdx.x; i < 7 * NN; i += kGatherCols) st[i] = tab[i];
+    __syncthreads();
+    const int j = c0 + (int)threadIdx.x;
+    if (j < c1) {
+        const int cb = col_ptr[j] - base;
+        const int k1 = gptr[j + 1];
+        for (int k = gptr[j]; k < k1; ++k) {
+            const int o = occ[k];
+            const int e = o / NL, lc = o - e * NL;
+            const CT *t = terms + (size_t)8 * e;
+            const CT t0 = t[0], t1 = t[1], t2 = t[2], t3 = t[3], t4 = t[4], t5 = t[5],
+                     t6 = t[6];
+            const uint16_t *sl = gslot + (size_t)k * NL;
+#pragma unroll
+            for (int lr = 0; lr < NL; ++lr) {
+                const int rc = lr < lc ? lr * NL + lc : lc * NL + lr;
+                CT v = t0 * st[rc] + t1 * st[NN + rc] + t2 * st[2 * NN + rc] +
+                       t3 * st[3 * NN + rc] + t4 * st[4 * NN + rc] + t5 * st[5 * NN + rc];
+                if (mass) v += t6 * st[6 * NN + rc];
+                acc[cb + sl[lr]] += v;
+            }
+        }
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < nz; i += kGatherCols) val[base + i] = acc[i];
+}
+
+// value positions of every occurrence, from the element slots ([r n + c][pos])
+__global__ __launch_bounds__(256) void k_h1_gather_slots(int ncols, int ncell, int nl,
+                                                         const int *__restrict__ col_ptr,
+                                                         const int *__restrict__ gptr,
+                                                         const int *__restrict__ occ,
+                                                         const int *__restrict__ invperm,
+                                                         const int *__restrict__ slots,
+                                                         uint16_t *__restrict__ gslot)
+{
+    const int j = blockIdx.x * 256 + threadIdx.x;
+    if (j >= ncols) return;
+    const int cb = col_ptr[j];
+    for (int k = gptr[j]; k < gptr[j + 1]; ++k) {
+        const int o = occ[k];
+        const int e = o / nl, lc = o - e * nl;
+        const int pos = invperm[e];
+        for (int lr = 0; lr < nl; ++lr)
+            gslot[(size_t)k * nl + lr] =
+                (uint16_t)(slots[(size_t)(lr * nl + lc) * ncell + pos] - cb);
+    }
+}
+
+__global__ void k_invert_perm(int n, const int *__restrict__ perm, int *__restrict__ inv)
+{
+    const int pos = blockIdx.x * 256 + threadIdx.x;
+    if (pos < n) inv[perm[pos]] = pos;
+}
+
 // Phase 2: scatter-add of one colour, one thread per (cell, entry).  Cells of
 // a colour share no dof, so every value slot has a single writer.  The upper
 // entry u(r,c) goes to both (r,c) and (c,r), so the matrix is exactly
@@ -950,6 +1066,60 @@ static bool device_symbolic_fits(int64_t T, int nl)
     return need < 0.25 * (double)free_b;
 }
 
+// Structures of the H1 gather assembly: per dof its occurrences (cell e,
+// local index l, as e * n_local + l) in ascending cell order, each
+// occurrence's n_local value positions inside the column, and the column
+// blocks.  Left off (gat_ok false, the colour scatter assembles) when a
+// column holds more values than a block's LDS takes.
+static void build_h1_gather(Problem *P, const std::vector<int> &gd,
+                            const std::vector<int> &col_ptr)
+{
+    hipStream_t s = P->ctx->stream;
+    const int T = P->ncell, nl = P->n_local, G = P->n_global;
+    P->gat_ok = false;
+    P->n_gblk = P->gat_nz_max = 0;
+    P->d_gat_ptr.reset();
+    P->d_gat_occ.reset();
+    P->d_gat_blk.reset();
+    P->d_gat_slot.reset();
+    P->d_h1_terms.reset();
+    if (G == 0 || T == 0 || (int64_t)T * nl >= INT_MAX) return;
+    std::vector<int> blk(1, 0);
+    int nz_max = 0;
+    for (int j = 0; j < G; ++j) {
+        if (col_ptr[j + 1] - col_ptr[j] > kGatherNz) return;
+        const int b0 = blk.back();
+        if (j > b0 && (col_ptr[j + 1] - col_ptr[b0] > kGatherNz || j - b0 + 1 > kGatherCols)) {
+            nz_max = std::max(nz_max, col_ptr[j] - col_ptr[b0]);
+            blk.push_back(j);
+        }
+    }
+    nz_max = std::max(nz_max, col_ptr[G] - col_ptr[blk.back()]);
+    blk.push_back(G);
+    std::vector<int> ptr((size_t)G + 1, 0);
+    for (size_t i = 0; i < gd.size(); ++i) ++ptr[(size_t)gd[i] + 1];
+    for (int j = 0; j < G; ++j) ptr[j + 1] += ptr[j];
+    std::vector<int> occ(gd.size()), next(ptr.begin(), ptr.end() - 1);
+    for (int e = 0; e < T; ++e)
+        for (int l = 0; l < nl; ++l) occ[next[gd[(size_t)nl * e + l]]++] = e * nl + l;
+    P->d_gat_ptr.upload(ptr, s);
+    P->d_gat_occ.upload(occ, s);
+    P->d_gat_blk.upload(blk, s);
+    DBuf<int> inv;
+    inv.alloc(T);
+    hipLaunchKernelGGL(k_invert_perm, dim3((T + 255) / 256), dim3(256), 0, s, T, P->d_perm.p,
+                       inv.p);
+    P->d_gat_slot.alloc(occ.size() * (size_t)nl);
+    hipLaunchKernelGGL(k_h1_gather_slots, dim3((G + 255) / 256), dim3(256), 0, s, G, T, nl,
+                       P->full.m.col_ptr.p, P->d_gat_ptr.p, P->d_gat_occ.p, inv.p,
+                       P->d_slots.p, P->d_gat_slot.p);
+    FFH_HIP(hipGetLastError());
+    FFH_HIP(hipStreamSynchronize(s)); // occ, blk and inv go out of scope
+    P->n_gblk = (int)blk.size() - 1;
+    P->gat_nz_max = nz_max;
+    P->gat_ok = true;
+}
+
 void problem_pattern(Problem *P, int family, int order)
 {
     Context *ctx = P->ctx;
@@ -1088,6 +1258,14 @@ void problem_pattern(Problem *P, int family, int order)
         A.row_idx.upload(row_idx, s);
     }
     A.n = G;
+    if (family == FFH_FAMILY_H1) {
+        build_h1_gather(P, gd, col_ptr);
+    } else {
+        P->gat_ok = false;
+        P->d_gat_slot.reset();
+        P->d_gat_occ.reset();
+        P->d_h1_terms.reset();
+    }
     A.val.alloc(A.nnz);
     FFH_HIP(hipMemsetAsync(A.val.p, 0, sizeof(double) * A.nnz, s));
     A.symmetric = true;
@@ -1273,13 +1451,48 @@ void problem_assemble(Problem *P, int qdeg, int coeff_kind, const double *coeff,
     } else {
         ensure_h1_tables(P, qdeg);
         const bool f32 = precision == FFH_FP32;
-        if (f32) {
-            A.val32.alloc(A.nnz);
-            FFH_HIP(hipMemsetAsync(A.val32.p, 0, sizeof(float) * A.nnz, s));
-        } else {
-            FFH_HIP(hipMemsetAsync(A.val.p, 0, sizeof(double) * A.nnz, s));
-        }
         const int nl = P->n_local;
+        if (f32) A.val32.alloc(A.nnz);
+        if (P->gat_ok) {
+            // per-cell coefficients, then the column gather (every value
+            // written, no clearing)
+            P->d_h1_terms.alloc((size_t)8 * T);
+            const int gridc = (T + kBlockA - 1) / kBlockA;
+            const size_t lds = (size_t)P->gat_nz_max * (f32 ? sizeof(float) : sizeof(double));
+            FFH_NL_DISPATCH(nl, {
+                if (f32) {
+                    float *t32 = reinterpret_cast<float *>(P->d_h1_terms.p);
+                    hipLaunchKernelGGL((k_h1_terms<float>), dim3(gridc), dim3(kBlockA), 0, s, T,
+                                       P->d_cells_c.p, P->d_vertices.p, P->d_perm.p, dk, kappa,
+                                       mass, t32, P->d_err.p);
+                    hipLaunchKernelGGL((k_h1_gather<NL, float>), dim3(P->n_gblk),
+                                       dim3(kGatherCols), lds, s, P->d_gat_blk.p, A.col_ptr.p,
+                                       P->d_gat_ptr.p, P->d_gat_occ.p, P->d_gat_slot.p, t32,
+                                       P->d_tab32.p, mass != 0.0 ? 1 : 0, A.val32.p);
+                } else {
+                    hipLaunchKernelGGL((k_h1_terms<double>), dim3(gridc), dim3(kBlockA), 0, s, T,
+                                       P->d_cells_c.p, P->d_vertices.p, P->d_perm.p, dk, kappa,
+                                       mass, P->d_h1_terms.p, P->d_err.p);
+                    hipLaunchKernelGGL((k_h1_gather<NL, double>), dim3(P->n_gblk),
+                                       dim3(kGatherCols), lds, s, P->d_gat_blk.p, A.col_ptr.p,
+                                       P->d_gat_ptr.p, P->d_gat_occ.p, P->d_gat_slot.p,
+                                       P->d_h1_terms.p, P->d_tab64.p, mass != 0.0 ? 1 : 0,
+                                       A.val.p);
+                }
+            });
+            FFH_HIP(hipGetLastError());
+            check_err(P, "tetrahedral H1 assembly");
+            A.values_changed();
+            A.val32_valid = f32;
+            A.val64_stale = f32;
+            P->assembled = true;
+            P->red_dirty = true;
+            return;
+        }
+        if (f32)
+            FFH_HIP(hipMemsetAsync(A.val32.p, 0, sizeof(float) * A.nnz, s));
+        else
+            FFH_HIP(hipMemsetAsync(A.val.p, 0, sizeof(double) * A.nnz, s));
         const size_t nup = (size_t)nl * (nl + 1) / 2;
         P->d_elem.alloc(nup * T);
         double *E64 = P->d_elem.p;
