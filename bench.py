@@ -381,12 +381,36 @@ def cpu_baseline(ff, p, meta, rhs, x_gpu, threads: int, reps: int):
     }
 
 
+def run_copy(args):
+    """Calibration for the HBM-traffic counters: a device copy of known size
+    (1 GiB of FP32 read, 1 GiB written, through torch's elementwise kernel),
+    `steps` times.  FETCH_SIZE and WRITE_SIZE of these dispatches against
+    2^30 bytes give the scale factors applied to the counters of the
+    roofline kernel."""
+    import torch
+
+    n = 1 << 28
+    x = torch.rand(n, device="cuda:0")
+    y = torch.empty_like(x)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        torch.mul(x, 1.0, out=y)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / args.steps
+    assert torch.equal(x, y)
+    print(json.dumps({"workload": "copy", "bytes_read": 4 * n, "bytes_written": 4 * n,
+                      "launches": args.steps, "ms_per_copy": round(dt * 1e3, 3),
+                      "gbs": round(8.0 * n / dt / 1e9, 1)}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="c4", help="c4 (default), c2, c3 or c5")
+    ap.add_argument("--workload", default="c4",
+                    help="c4 (default), c2, c3, c5, or copy (traffic-counter calibration)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-served", action="store_true", help="skip the served-stream keys")
     ap.add_argument("--cpu-threads", type=int, default=0, help="CPU baseline threads (0: min(16, nproc))")
@@ -415,6 +439,9 @@ def main():
         return
     if args.workload == "c5":
         run_c5(args)
+        return
+    if args.workload == "copy":
+        run_copy(args)
         return
 
     import fortfem_amd as ff

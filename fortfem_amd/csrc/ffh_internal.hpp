@@ -275,16 +275,20 @@ struct Problem {
     int tab_qdeg = -1, tab_nq = 0;
     DBuf<double> d_load_tab; // per-point xi (3) + w*phi (n_local)
     DBuf<double> d_elem;     // element matrices / vectors, [entry][pos]
-    // H1 gather assembly: per CSC column (dof) its occurrences (cell e,
-    // local index l) as e * n_local + l in ascending e (gat_ptr, gat_occ),
-    // per occurrence the n_local value positions inside the column
-    // (gat_slot, relative to col_ptr), the column blocks (a run of columns
-    // whose values fit one block's LDS) and per-cell coefficient terms
+    // H1 gather assembly.  Column blocks (runs of at most 256 CSC columns
+    // whose values fit one block's LDS, gat_blk) are cut into slices of 64
+    // columns (gat_bslice: first slice of a block); slice s holds its
+    // columns' occurrences (cells e holding the column's dof, local index l,
+    // as e * n_local + l, ascending e; -1 pads) step-major in gat_occ
+    // [(gat_soff[s] + k) * 64 + lane] and their n_local value positions
+    // relative to the block's first value as groups of four in gat_slot
+    // [((gat_soff[s] + k) * Q + q) * 64 + lane], Q = ceil(n_local / 4).
     bool gat_ok = false;
     int n_gblk = 0, gat_nz_max = 0;
-    DBuf<int> d_gat_ptr, d_gat_occ, d_gat_blk;
+    DBuf<int> d_gat_occ, d_gat_blk, d_gat_bslice, d_gat_soff;
     DBuf<uint16_t> d_gat_slot;
     DBuf<double> d_h1_terms; // 8 per cell (7 used), cell order
+    DBuf<int> d_cells_e;     // 4 vertex ids per cell (0-based), cell order
     int load_qdeg = -1, load_nq = 0;
     DBuf<int> d_err;
     DBuf<double> d_coeff;

@@ -275,18 +275,19 @@ __global__ __launch_bounds__(kBlockA) void k_h1_elem_vjp(
 // The per-cell reference-tensor coefficients of the element matrix,
 // c_k = kappa det G_k (k = 0..5, the G pairs of h1_gterms) and c_6 = mu det,
 // in cell order, 8 values per cell (c_7 unused).
+// One thread per cell in the mesh's own cell order (cells_e: 0-based
+// vertex ids), so the coefficient reads and the term writes are coalesced.
 template <typename CT>
-__global__ __launch_bounds__(kBlockA) void k_h1_terms(int ncell, const int *__restrict__ cells_c,
+__global__ __launch_bounds__(kBlockA) void k_h1_terms(int ncell, const int *__restrict__ cells_e,
                                                       const double *__restrict__ verts,
-                                                      const int *__restrict__ perm,
                                                       const double *__restrict__ kappa_cell,
                                                       double kappa, double mu,
                                                       CT *__restrict__ terms,
                                                       int *__restrict__ err)
 {
-    const int pos = blockIdx.x * kBlockA + threadIdx.x;
-    if (pos >= ncell) return;
-    const int4 c = reinterpret_cast<const int4 *>(cells_c)[pos];
+    const int e = blockIdx.x * kBlockA + threadIdx.x;
+    if (e >= ncell) return;
+    const int4 c = reinterpret_cast<const int4 *>(cells_e)[e];
     double jac[9], inv[9], det;
     if (!tet_geometry(verts, c.x, c.y, c.z, c.w, jac, inv, det)) {
         atomicOr(err, 1);
@@ -294,7 +295,6 @@ __global__ __launch_bounds__(kBlockA) void k_h1_terms(int ncell, const int *__re
     }
     double g[6];
     h1_gterms(inv, g);
-    const int e = perm[pos];
     const double ke = kappa_cell ? kappa_cell[e] : kappa;
     const double kd = ke * det;
     CT *t = terms + (size_t)8 * e;
@@ -305,53 +305,63 @@ __global__ __launch_bounds__(kBlockA) void k_h1_terms(int ncell, const int *__re
 }
 
 // Largest value run of one gather block (values of CT in LDS) and most
-// columns per block (one thread per column).
+// columns per block (one thread per column, 64-column slices per wave).
 constexpr int kGatherNz = 8192;
 constexpr int kGatherCols = 256;
 
 // Assembly by columns: a block takes a run of CSC columns whose values fit
-// LDS; one thread per column walks the column's occurrences (the cells that
+// LDS; one lane per column walks the column's occurrences (the cells that
 // hold the column's dof, ascending) and adds, for each local row index l_r,
 // the element entry sum_k c_k S_k(l_r, l_c) (+ c_6 M) at its value position;
 // the block then writes the run's values once.  No colours, no atomics, no
-// element matrices in HBM.  Entries (i, j) and (j, i) add the same numbers
-// (the upper-triangle table entry) over the same cells in the same order, so
-// the matrix is exactly symmetric and the result does not depend on the
-// launch.
+// element matrices in HBM.  The occurrences and value positions are stored
+// step-major per 64-column slice, so each step of a wave reads 256 + 512 Q
+// contiguous bytes.  Entries (i, j) and (j, i) add the same numbers (the
+// upper-triangle table entry) over the same cells in the same order, so the
+// matrix is exactly symmetric and the result does not depend on the launch.
 template <int NL, typename CT>
 __global__ __launch_bounds__(kGatherCols) void k_h1_gather(
-    const int *__restrict__ blk, const int *__restrict__ col_ptr,
-    const int *__restrict__ gptr, const int *__restrict__ occ,
-    const uint16_t *__restrict__ gslot, const CT *__restrict__ terms,
-    const CT *__restrict__ tab, int mass, CT *__restrict__ val)
+    const int *__restrict__ blk, const int *__restrict__ bslice,
+    const int *__restrict__ soff, const int *__restrict__ col_ptr,
+    const int *__restrict__ occ, const ushort4 *__restrict__ gslot,
+    const CT *__restrict__ terms, const CT *__restrict__ tab, int mass,
+    CT *__restrict__ val)
 {
     constexpr int NN = NL * NL;
+    constexpr int Q = (NL + 3) / 4;
     extern __shared__ __align__(16) unsigned char gather_lds[];
     CT *acc = reinterpret_cast<CT *>(gather_lds);
     __shared__ CT st[7 * NN];
-    const int c0 = blk[blockIdx.x], c1 = blk[blockIdx.x + 1];
-    const int base = col_ptr[c0], nz = col_ptr[c1] - base;
+    const int b = xcd_block();
+    const int base = col_ptr[blk[b]], nz = col_ptr[blk[b + 1]] - base;
     for (int i = threadIdx.x; i < nz; i += kGatherCols) acc[i] = CT(0);
     for (int i = threadIdx.x; i < 7 * NN; i += kGatherCols) st[i] = tab[i];
     __syncthreads();
-    const int j = c0 + (int)threadIdx.x;
-    if (j < c1) {
-        const int cb = col_ptr[j] - base;
-        const int k1 = gptr[j + 1];
-        for (int k = gptr[j]; k < k1; ++k) {
-            const int o = occ[k];
+    const int s = bslice[b] + (int)(threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (s < bslice[b + 1]) {
+        const int k1 = soff[s + 1];
+        for (int k = soff[s]; k < k1; ++k) {
+            const int o = occ[(size_t)k * 64 + lane];
+            if (o < 0) continue;
             const int e = o / NL, lc = o - e * NL;
             const CT *t = terms + (size_t)8 * e;
             const CT t0 = t[0], t1 = t[1], t2 = t[2], t3 = t[3], t4 = t[4], t5 = t[5],
                      t6 = t[6];
-            const uint16_t *sl = gslot + (size_t)k * NL;
 #pragma unroll
-            for (int lr = 0; lr < NL; ++lr) {
-                const int rc = lr < lc ? lr * NL + lc : lc * NL + lr;
-                CT v = t0 * st[rc] + t1 * st[NN + rc] + t2 * st[2 * NN + rc] +
-                       t3 * st[3 * NN + rc] + t4 * st[4 * NN + rc] + t5 * st[5 * NN + rc];
-                if (mass) v += t6 * st[6 * NN + rc];
-                acc[cb + sl[lr]] += v;
+            for (int q = 0; q < Q; ++q) {
+                const ushort4 g = gslot[((size_t)k * Q + q) * 64 + lane];
+                const unsigned short gl[4] = {g.x, g.y, g.z, g.w};
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    const int lr = 4 * q + i;
+                    if (lr >= NL) break;
+                    const int rc = lr < lc ? lr * NL + lc : lc * NL + lr;
+                    CT v = t0 * st[rc] + t1 * st[NN + rc] + t2 * st[2 * NN + rc] +
+                           t3 * st[3 * NN + rc] + t4 * st[4 * NN + rc] + t5 * st[5 * NN + rc];
+                    if (mass) v += t6 * st[6 * NN + rc];
+                    acc[gl[i]] += v;
+                }
             }
         }
     }
@@ -359,25 +369,32 @@ __global__ __launch_bounds__(kGatherCols) void k_h1_gather(
     for (int i = threadIdx.x; i < nz; i += kGatherCols) val[base + i] = acc[i];
 }
 
-// value positions of every occurrence, from the element slots ([r n + c][pos])
-__global__ __launch_bounds__(256) void k_h1_gather_slots(int ncols, int ncell, int nl,
-                                                         const int *__restrict__ col_ptr,
-                                                         const int *__restrict__ gptr,
+// Value positions of the gather layout, one thread per (slice, lane), from
+// the element slots ([r n + c][pos], absolute) less the block's first value.
+__global__ __launch_bounds__(256) void k_h1_gather_slots(int nslice, int ncell, int nl,
+                                                         const int *__restrict__ soff,
+                                                         const int *__restrict__ sbase,
                                                          const int *__restrict__ occ,
                                                          const int *__restrict__ invperm,
                                                          const int *__restrict__ slots,
                                                          uint16_t *__restrict__ gslot)
 {
-    const int j = blockIdx.x * 256 + threadIdx.x;
-    if (j >= ncols) return;
-    const int cb = col_ptr[j];
-    for (int k = gptr[j]; k < gptr[j + 1]; ++k) {
-        const int o = occ[k];
+    const int64_t t = blockIdx.x * 256ll + threadIdx.x;
+    if (t >= (int64_t)nslice * 64) return;
+    const int s = (int)(t >> 6), lane = (int)(t & 63);
+    const int q4 = 4 * ((nl + 3) / 4);
+    for (int k = soff[s]; k < soff[s + 1]; ++k) {
+        const int o = occ[(size_t)k * 64 + lane];
+        uint16_t *g = gslot + (size_t)k * q4 * 64;
+        if (o < 0) {
+            for (int lr = 0; lr < q4; ++lr) g[((lr >> 2) * 64 + lane) * 4 + (lr & 3)] = 0;
+            continue;
+        }
         const int e = o / nl, lc = o - e * nl;
         const int pos = invperm[e];
-        for (int lr = 0; lr < nl; ++lr)
-            gslot[(size_t)k * nl + lr] =
-                (uint16_t)(slots[(size_t)(lr * nl + lc) * ncell + pos] - cb);
+        for (int lr = 0; lr < q4; ++lr)
+            g[((lr >> 2) * 64 + lane) * 4 + (lr & 3)] =
+                lr < nl ? (uint16_t)(slots[(size_t)(lr * nl + lc) * ncell + pos] - sbase[s]) : 0;
     }
 }
 
@@ -1066,11 +1083,11 @@ static bool device_symbolic_fits(int64_t T, int nl)
     return need < 0.25 * (double)free_b;
 }
 
-// Structures of the H1 gather assembly: per dof its occurrences (cell e,
-// local index l, as e * n_local + l) in ascending cell order, each
-// occurrence's n_local value positions inside the column, and the column
-// blocks.  Left off (gat_ok false, the colour scatter assembles) when a
-// column holds more values than a block's LDS takes.
+// Structures of the H1 gather assembly (k_h1_gather): column blocks, their
+// 64-column slices, per slice the step-major occurrences (ascending cell
+// order within a column) and value positions.  Left off (gat_ok false, the
+// colour scatter assembles) when a column holds more values than a block's
+// LDS takes.
 static void build_h1_gather(Problem *P, const std::vector<int> &gd,
                             const std::vector<int> &col_ptr)
 {
@@ -1078,11 +1095,13 @@ static void build_h1_gather(Problem *P, const std::vector<int> &gd,
     const int T = P->ncell, nl = P->n_local, G = P->n_global;
     P->gat_ok = false;
     P->n_gblk = P->gat_nz_max = 0;
-    P->d_gat_ptr.reset();
     P->d_gat_occ.reset();
     P->d_gat_blk.reset();
+    P->d_gat_bslice.reset();
+    P->d_gat_soff.reset();
     P->d_gat_slot.reset();
     P->d_h1_terms.reset();
+    P->d_cells_e.reset();
     if (G == 0 || T == 0 || (int64_t)T * nl >= INT_MAX) return;
     std::vector<int> blk(1, 0);
     int nz_max = 0;
@@ -1096,26 +1115,63 @@ static void build_h1_gather(Problem *P, const std::vector<int> &gd,
     }
     nz_max = std::max(nz_max, col_ptr[G] - col_ptr[blk.back()]);
     blk.push_back(G);
+    const int nblk = (int)blk.size() - 1;
+    // occurrences per column, ascending cell (counting sort)
     std::vector<int> ptr((size_t)G + 1, 0);
     for (size_t i = 0; i < gd.size(); ++i) ++ptr[(size_t)gd[i] + 1];
     for (int j = 0; j < G; ++j) ptr[j + 1] += ptr[j];
     std::vector<int> occ(gd.size()), next(ptr.begin(), ptr.end() - 1);
     for (int e = 0; e < T; ++e)
         for (int l = 0; l < nl; ++l) occ[next[gd[(size_t)nl * e + l]]++] = e * nl + l;
-    P->d_gat_ptr.upload(ptr, s);
-    P->d_gat_occ.upload(occ, s);
+    std::vector<int>().swap(next);
+    // slices of 64 columns inside each block, step-major occurrences
+    std::vector<int> bslice(1, 0), soff(1, 0), sbase;
+    std::vector<int> scol, send;
+    for (int b = 0; b < nblk; ++b) {
+        for (int c = blk[b]; c < blk[b + 1]; c += 64) {
+            const int ce = std::min(c + 64, blk[b + 1]);
+            int kmax = 0;
+            for (int j = c; j < ce; ++j) kmax = std::max(kmax, ptr[j + 1] - ptr[j]);
+            soff.push_back(soff.back() + kmax);
+            sbase.push_back(col_ptr[blk[b]]);
+            scol.push_back(c);
+            send.push_back(ce);
+        }
+        bslice.push_back((int)scol.size());
+    }
+    const int nslice = (int)scol.size();
+    const size_t nent = (size_t)soff.back() * 64;
+    std::vector<int> occ_s(nent, -1);
+    parallel_for(nslice, [&](int64_t lo, int64_t hi) {
+        for (int64_t si = lo; si < hi; ++si) {
+            const int c = scol[si];
+            for (int j = c; j < send[si]; ++j)
+                for (int k = ptr[j]; k < ptr[j + 1]; ++k)
+                    occ_s[(size_t)(soff[si] + k - ptr[j]) * 64 + (j - c)] = occ[k];
+        }
+    });
+    std::vector<int>().swap(occ);
+    P->d_gat_occ.upload(occ_s, s);
+    std::vector<int>().swap(occ_s);
+    std::vector<int> cells_e(P->cells.size());
+    for (size_t i = 0; i < cells_e.size(); ++i) cells_e[i] = P->cells[i] - 1;
+    P->d_cells_e.upload(cells_e, s);
     P->d_gat_blk.upload(blk, s);
-    DBuf<int> inv;
+    P->d_gat_bslice.upload(bslice, s);
+    P->d_gat_soff.upload(soff, s);
+    DBuf<int> inv, dsbase;
+    dsbase.upload(sbase, s);
     inv.alloc(T);
     hipLaunchKernelGGL(k_invert_perm, dim3((T + 255) / 256), dim3(256), 0, s, T, P->d_perm.p,
                        inv.p);
-    P->d_gat_slot.alloc(occ.size() * (size_t)nl);
-    hipLaunchKernelGGL(k_h1_gather_slots, dim3((G + 255) / 256), dim3(256), 0, s, G, T, nl,
-                       P->full.m.col_ptr.p, P->d_gat_ptr.p, P->d_gat_occ.p, inv.p,
-                       P->d_slots.p, P->d_gat_slot.p);
+    const int q4 = 4 * ((nl + 3) / 4);
+    P->d_gat_slot.alloc(nent * q4);
+    hipLaunchKernelGGL(k_h1_gather_slots, dim3((int)(((int64_t)nslice * 64 + 255) / 256)),
+                       dim3(256), 0, s, nslice, T, nl, P->d_gat_soff.p, dsbase.p,
+                       P->d_gat_occ.p, inv.p, P->d_slots.p, P->d_gat_slot.p);
     FFH_HIP(hipGetLastError());
-    FFH_HIP(hipStreamSynchronize(s)); // occ, blk and inv go out of scope
-    P->n_gblk = (int)blk.size() - 1;
+    FFH_HIP(hipStreamSynchronize(s)); // host vectors and inv go out of scope
+    P->n_gblk = nblk;
     P->gat_nz_max = nz_max;
     P->gat_ok = true;
 }
@@ -1265,6 +1321,7 @@ void problem_pattern(Problem *P, int family, int order)
         P->d_gat_slot.reset();
         P->d_gat_occ.reset();
         P->d_h1_terms.reset();
+        P->d_cells_e.reset();
     }
     A.val.alloc(A.nnz);
     FFH_HIP(hipMemsetAsync(A.val.p, 0, sizeof(double) * A.nnz, s));
@@ -1459,23 +1516,24 @@ void problem_assemble(Problem *P, int qdeg, int coeff_kind, const double *coeff,
             P->d_h1_terms.alloc((size_t)8 * T);
             const int gridc = (T + kBlockA - 1) / kBlockA;
             const size_t lds = (size_t)P->gat_nz_max * (f32 ? sizeof(float) : sizeof(double));
+            const ushort4 *gsl = reinterpret_cast<const ushort4 *>(P->d_gat_slot.p);
             FFH_NL_DISPATCH(nl, {
                 if (f32) {
                     float *t32 = reinterpret_cast<float *>(P->d_h1_terms.p);
                     hipLaunchKernelGGL((k_h1_terms<float>), dim3(gridc), dim3(kBlockA), 0, s, T,
-                                       P->d_cells_c.p, P->d_vertices.p, P->d_perm.p, dk, kappa,
-                                       mass, t32, P->d_err.p);
+                                       P->d_cells_e.p, P->d_vertices.p, dk, kappa, mass, t32,
+                                       P->d_err.p);
                     hipLaunchKernelGGL((k_h1_gather<NL, float>), dim3(P->n_gblk),
-                                       dim3(kGatherCols), lds, s, P->d_gat_blk.p, A.col_ptr.p,
-                                       P->d_gat_ptr.p, P->d_gat_occ.p, P->d_gat_slot.p, t32,
+                                       dim3(kGatherCols), lds, s, P->d_gat_blk.p, P->d_gat_bslice.p,
+                                       P->d_gat_soff.p, A.col_ptr.p, P->d_gat_occ.p, gsl, t32,
                                        P->d_tab32.p, mass != 0.0 ? 1 : 0, A.val32.p);
                 } else {
                     hipLaunchKernelGGL((k_h1_terms<double>), dim3(gridc), dim3(kBlockA), 0, s, T,
-                                       P->d_cells_c.p, P->d_vertices.p, P->d_perm.p, dk, kappa,
-                                       mass, P->d_h1_terms.p, P->d_err.p);
+                                       P->d_cells_e.p, P->d_vertices.p, dk, kappa, mass,
+                                       P->d_h1_terms.p, P->d_err.p);
                     hipLaunchKernelGGL((k_h1_gather<NL, double>), dim3(P->n_gblk),
-                                       dim3(kGatherCols), lds, s, P->d_gat_blk.p, A.col_ptr.p,
-                                       P->d_gat_ptr.p, P->d_gat_occ.p, P->d_gat_slot.p,
+                                       dim3(kGatherCols), lds, s, P->d_gat_blk.p, P->d_gat_bslice.p,
+                                       P->d_gat_soff.p, A.col_ptr.p, P->d_gat_occ.p, gsl,
                                        P->d_h1_terms.p, P->d_tab64.p, mass != 0.0 ? 1 : 0,
                                        A.val.p);
                 }
