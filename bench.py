@@ -42,8 +42,13 @@ HBM_PEAK_GBS = 8000.0
 # what the counter reads for a device copy of known size (the summary and the
 # commands are in profiles/).  None where no such measurement exists for the
 # kernel as it is.
-MEASURED_TRAFFIC = {}
-TRAFFIC_SOURCE = {}
+MEASURED_TRAFFIC = {"c4": 254.34e6}
+TRAFFIC_SOURCE = {
+    "c4": "k_pcg_apdot<2, float, float>: 2.0 x FETCH_SIZE 120,147.6 KiB + 1.0 x WRITE_SIZE 8,081.7 KiB "
+          "per live launch, rocprofv3 --pmc over bench.py --no-cpu --no-served --steps 1 --warmup 0, "
+          "factors from the 1 GiB copy of bench.py --workload copy; "
+          "profiles/k_pcg_apdot_p1_poisson_128cube.md",
+}
 
 
 def spmv_bytes(nnz: int, n: int, sv: int) -> float:
@@ -485,11 +490,19 @@ def main():
     n_free, nnz_free = sizes["n_free"], sizes["nnz_free"]
     cnt = sum(c for c, _ in prof)
     avg_ms = sum(c * m for c, m in prof) / max(cnt, 1)
-    b32 = dirspmv_bytes(nnz_free, n_free, 4)
-    achieved = b32 / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
     red = p.matrix(ff.capi.REDUCED)
+    if ff.pcg_launches_per_iteration(n_free, "fp32") == 3:
+        # the SpMV launch of the three-launch iteration: the plain SpMV's
+        # bytes in the format of its variant (p gathered, A p written)
+        kname = "k_pcg_apdot<float> (FP32 SpMV A p + p.Ap partials)"
+        b32 = red.spmv_bytes("fp32")
+    else:
+        kname = "k_pcg_dirspmv<float> (FP32 direction update + SpMV + p.Ap)"
+        b32 = dirspmv_bytes(nnz_free, n_free, 4)
+    achieved = b32 / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
     ms64, by64 = red.bench_spmv("fp64", 50)
     ms32, by32 = red.bench_spmv("fp32", 50)
+    csr64, csr32 = spmv_bytes(nnz_free, n_free, 8), spmv_bytes(nnz_free, n_free, 4)
 
     value = args.steps / t_all
     lat_ms = np.array(lat) * 1e3
@@ -519,9 +532,17 @@ def main():
         "fp64_p99_ms": round(float(np.percentile(np.array(lat64) * 1e3, 99)), 3),
         "spmv_fp64_gbs": round(by64 / (ms64 * 1e-3) / 1e9, 1),
         "spmv_fp32_gbs": round(by32 / (ms32 * 1e-3) / 1e9, 1),
+        "spmv_bytes_basis": "bytes of the tuned variant's own storage format (SELL padding, 16-bit "
+                            "tile positions and tile column lists included); the csr_equivalent keys "
+                            "divide the int32 CSR count nnz (s + 4) + 4 (n + 1) + 2 n s instead",
+        "spmv_fp64_csr_equivalent_gbs": round(csr64 / (ms64 * 1e-3) / 1e9, 1),
+        "spmv_fp32_csr_equivalent_gbs": round(csr32 / (ms32 * 1e-3) / 1e9, 1),
+        "spmv_variants": {"fp64": red.spmv_variant("fp64"), "fp32": red.spmv_variant("fp32")},
+        "pcg_launches_per_iteration": {"fp64": ff.pcg_launches_per_iteration(n_free, "fp64"),
+                                       "fp32": ff.pcg_launches_per_iteration(n_free, "fp32")},
         "setup_s": round(t_setup, 2),
         "roofline": {
-            "kernel": "k_pcg_dirspmv<float> (FP32 direction update + SpMV + p.Ap)",
+            "kernel": kname,
             "bound": "hbm",
             "achieved": round(achieved, 1),
             "peak": HBM_PEAK_GBS,

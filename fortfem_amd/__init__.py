@@ -690,6 +690,13 @@ class Matrix:
             return np.zeros(nnz), np.zeros(n), 2, stats
         return vb[:nnz], bb, 0, stats
 
+    def spmv_bytes(self, precision="fp64"):
+        """Bytes one SpMV launch moves in the format of the variant in use."""
+        by = C.c_double(0)
+        prec = capi.FP32 if precision == "fp32" else capi.FP64
+        self.ctx.check(self._lib.ffh_matrix_spmv_bytes(self.handle, prec, C.byref(by)))
+        return by.value
+
     def spmv_variant(self, precision="fp64", request=-1):
         """The SpMV variant in use (autotuned on first use); request >= 0 forces one."""
         out = C.c_int(0)
@@ -699,6 +706,18 @@ class Matrix:
 
 
 # ----------------------------------------------------- reference-named calls
+def pcg_launches_per_iteration(n: int, precision: str = "fp64") -> int:
+    """Launches per Jacobi-PCG iteration on n rows: 3 (direction update,
+    SpMV with A p and p.A p, x/r/z update) or 2 (direction formed inside the
+    SpMV's gathers, then the update); "fp32" for the mixed solve's
+    iterations."""
+    lib = capi.load()
+    k = lib.ffh_pcg_launches_per_iteration(int(n), capi.FP32 if precision == "fp32" else capi.FP64)
+    if k not in (2, 3):
+        raise ValueError("pcg_launches_per_iteration: invalid arguments")
+    return k
+
+
 def spmv(matrix: CSC, vector, ctx: Context | None = None):
     """spmv (fortfem_sparse_matrix.f90:46-58): size mismatches raise (error stop)."""
     v = _f64(vector)

@@ -47,7 +47,7 @@ EXPORTS = (
     "ffh_constrained_download", "ffh_set_rhs", "ffh_rhs_download", "ffh_problem_matrix",
     "ffh_problem_solve", "ffh_matrix_upload", "ffh_matrix_free", "ffh_matrix_download",
     "ffh_spmv", "ffh_solve_sparse", "ffh_bench_spmv", "ffh_set_profiling",
-    "ffh_last_solve_profile", "ffh_matrix_spmv_variant", "ffh_nedelec_dof_map", "ffh_nedelec_basis",
+    "ffh_last_solve_profile", "ffh_matrix_spmv_variant", "ffh_matrix_spmv_bytes", "ffh_pcg_launches_per_iteration", "ffh_nedelec_dof_map", "ffh_nedelec_basis",
     "ffh_nedelec_load_points", "ffh_spmv_jvp", "ffh_spmv_vjp", "ffh_solve_sparse_jvp",
     "ffh_solve_sparse_vjp", "ffh_assemble_jvp", "ffh_assemble_vjp", "ffh_nedelec_pml_values",
     "ffh_nedelec_pml_values_jvp", "ffh_nedelec_pml_values_vjp", "ffh_zmatrix_upload",
@@ -155,6 +155,8 @@ def load() -> C.CDLL:
         "ffh_set_profiling": (C.c_int, [vp, C.c_int]),
         "ffh_last_solve_profile": (C.c_int, [vp, C.c_int, ip, dp]),
         "ffh_matrix_spmv_variant": (C.c_int, [vp, C.c_int, C.c_int, ip]),
+        "ffh_matrix_spmv_bytes": (C.c_int, [vp, C.c_int, dp]),
+        "ffh_pcg_launches_per_iteration": (C.c_int, [C.c_int64, C.c_int]),
         "ffh_nedelec_dof_map": (C.c_int, [C.c_int, C.c_int, vp, C.c_int, vp, vp, vp, ip]),
         "ffh_nedelec_basis": (C.c_int, [C.c_int, C.c_int, vp, vp, vp]),
         "ffh_nedelec_load_points": (C.c_int, [C.c_int, ip, vp]),

@@ -831,6 +831,19 @@ int ffh_bench_spmv(ffh_matrix *M, int precision, int reps, double *avg_ms, doubl
     return guarded(M->ctx, [&] { *avg_ms = bench_spmv(M->ctx, *M, precision, reps, bytes); });
 }
 
+int ffh_matrix_spmv_bytes(ffh_matrix *M, int precision, double *bytes)
+{
+    if (!M) return FFH_ERR_HANDLE;
+    if (!bytes || (precision != FFH_FP64 && precision != FFH_FP32)) return FFH_ERR_INVALID;
+    return guarded(M->ctx, [&] { *bytes = spmv_launch_bytes(M->ctx, *M, precision == FFH_FP32); });
+}
+
+int ffh_pcg_launches_per_iteration(int64_t n, int precision)
+{
+    if (n < 0 || (precision != FFH_FP64 && precision != FFH_FP32)) return FFH_ERR_INVALID;
+    return three_launch(n, precision == FFH_FP32) ? 3 : 2;
+}
+
 int ffh_matrix_spmv_variant(ffh_matrix *M, int precision, int request, int *chosen)
 {
     if (!M) return FFH_ERR_HANDLE;

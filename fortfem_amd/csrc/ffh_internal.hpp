@@ -164,6 +164,7 @@ struct DevMatrix {
     // (16 bits), so a block stages each column once in LDS.
     bool tiles_built = false, tiles_ok = false;
     int tile_max = 0; // most distinct columns of one group
+    int64_t tile_total = 0; // entries of tile_cols
     DBuf<int> tile_off, tile_cols;
     DBuf<uint16_t> tile_lidx;
     // the row of each slice position, the tile offsets and the tile columns
@@ -187,7 +188,7 @@ struct Workspace {
     KrylovCtl *host_ctl = nullptr; // pinned mailboxes (two, polled alternately)
     hipEvent_t ctl_ev[2] = {nullptr, nullptr};
     hipGraphExec_t graph[2] = {nullptr, nullptr};
-    int graph_chunk[2] = {0, 0};
+    int64_t graph_chunk[2] = {0, 0};
     ~Workspace();
 };
 

@@ -436,6 +436,17 @@ struct EpiDirAp { // p_new(i) stored ; Ap = A p_new ; p_new.Ap
     }
 };
 
+template <typename VT>
+struct EpiApDot { // Ap = A p ; p.Ap
+    const VT *p;
+    VT *ap;
+    __device__ void operator()(int i, VT s, double *d) const
+    {
+        ap[i] = s;
+        d[0] += (double)p[i] * (double)s;
+    }
+};
+
 struct EpiResidual { // r = b - A x ; r.r
     const double *b;
     double *r;
@@ -599,6 +610,53 @@ __global__ __launch_bounds__(kStreamBlock) void k_pcg_dirspmv(
     // the block's first matrix and (z, p) loads are in flight while the
     // hook sums the previous partials
     row_engine<V, MT, VT>(n, ptr, idx, val, g, rowblocks, nrb, epi, d, hook);
+    const double s0 = block_sum<kStreamBlock>(d[0]);
+    if (threadIdx.x == 0) partials[blockIdx.x] = s0;
+}
+
+// Three-launch form of an iteration (large systems, where forming the
+// direction inside the SpMV's gathers costs more than a launch of its own).
+// Launch 1: the tail of the previous iteration (every block), then
+// p = z + beta p in place (the expression of XDir), also stored densely in
+// pl for the SpMV's gathers.
+template <typename VT>
+__global__ __launch_bounds__(kBlock) void k_pcg_dir(int n, ZP<VT> *__restrict__ zp,
+                                                    VT *__restrict__ pl,
+                                                    const double *__restrict__ partials,
+                                                    int stride, KrylovCtl *ctl)
+{
+    if (ctl->done) return;
+    const int restart = ctl->restart;
+    const int i0 = xcd_block() * kBlock + threadIdx.x;
+    ZP<VT> q0{};
+    if (i0 < n) q0 = zp[i0];
+    double beta_d;
+    if (pcg_finish<kBlock>(partials, stride, ctl, beta_d)) return;
+    const VT beta = (VT)beta_d;
+    if (i0 < n) {
+        const VT p = restart ? q0.z : fma_t(beta, q0.p, q0.z);
+        zp[i0].p = p;
+        pl[i0] = p;
+    }
+    for (int i = i0 + gridDim.x * kBlock; i < n; i += gridDim.x * kBlock) {
+        const ZP<VT> q = zp[i];
+        const VT p = restart ? q.z : fma_t(beta, q.p, q.z);
+        zp[i].p = p;
+        pl[i] = p;
+    }
+}
+
+// Launch 2 of the three-launch form: Ap = A p and the partials of p.Ap.
+template <int V, typename MT, typename VT>
+__global__ __launch_bounds__(kStreamBlock) void k_pcg_apdot(
+    int n, const int *__restrict__ ptr, const int *__restrict__ idx, const MT *__restrict__ val,
+    const VT *__restrict__ p, VT *__restrict__ ap, const int *__restrict__ rowblocks,
+    int nrb, double *__restrict__ partials, const KrylovCtl *ctl)
+{
+    if (ctl->done) return;
+    EpiApDot<VT> epi{p, ap};
+    double d[3] = {0.0, 0.0, 0.0};
+    row_engine<V, MT, VT>(n, ptr, idx, val, XPlain<VT>{p}, rowblocks, nrb, epi, d);
     const double s0 = block_sum<kStreamBlock>(d[0]);
     if (threadIdx.x == 0) partials[blockIdx.x] = s0;
 }
@@ -861,6 +919,26 @@ static const SpmvVariant kVariants[] = {{0, 1}, {0, 2}, {0, 4}, {4, 2}, {8, 1},
                                         {8, 2}, {16, 2}, {32, 2}, {32, 4}, {1, 1},
                                         {1, 2}, {2, 1}, {2, 2}};
 constexpr int kNumVariants = sizeof(kVariants) / sizeof(kVariants[0]);
+
+// Bytes one SpMV launch of variant vi must move in the format that variant
+// reads (values s_v bytes each; x read and y written once): CSR engines
+// nnz (s_v + 4) + 4 (n + 1) + 2 n s_v; SELL-64 its padded entries
+// (s_v + 4 each), slice offsets and row map; tiled SELL-64 its padded
+// entries (s_v + 2 each: 16-bit tile positions), slice offsets, row map,
+// tile offsets and tile column lists.
+double variant_bytes(const DevMatrix &A, int vi, int sv)
+{
+    const int engine = kVariants[vi].engine;
+    const double vec = 2.0 * sv * (double)A.n;
+    if (engine == 1 && A.sell_ok)
+        return (double)A.sell_total * (sv + 4) + 4.0 * (A.nslice + 1) + 256.0 * A.nslice + vec;
+    if (engine == 2 && A.tiles_ok) {
+        const int ngroup = (A.nslice + 3) / 4;
+        return (double)A.sell_total * (sv + 2) + 4.0 * (A.nslice + 1) + 256.0 * A.nslice +
+               4.0 * (ngroup + 1) + 4.0 * (double)A.tile_total + vec;
+    }
+    return (double)A.nnz * (sv + 4) + 4.0 * (A.n + 1) + vec;
+}
 
 static int variant_grid(const Context *ctx, const DevMatrix &A, int vi)
 {
@@ -1153,6 +1231,7 @@ void ensure_sell_tiles(Context *ctx, DevMatrix &A)
     A.tile_pack.upload(pack, s);
     FFH_HIP(hipStreamSynchronize(s));
     A.tile_max = mx;
+    A.tile_total = toff[ngroup];
     A.tiles_ok = true;
 }
 
@@ -1336,15 +1415,60 @@ void spmv32(Context *ctx, DevMatrix &A, const float *x, float *y)
     launch_spmv<float, float>(ctx, A, x, y, vi);
 }
 
-// `iters` iterations, two launches each: direction + SpMV (matrix MT,
-// vectors VT) and the x/r update (x in XT).
+// The form of a Jacobi-PCG iteration: three launches (k_pcg_dir,
+// k_pcg_apdot with the plain SpMV variant, k_pcg_update) for FP32 vectors
+// and for FP64 systems below kThreeLaunchRows rows, else two (k_pcg_dirspmv,
+// k_pcg_update), the faster form of each on the P2 32^3 x 6 and P1 128^3 x 6
+// Poisson systems; FFH_PCG_LAUNCHES=2 or 3 forces a form.
+constexpr int kThreeLaunchRows = 1 << 20;
+
+bool three_launch(int64_t n, bool f32)
+{
+    static const int forced = [] {
+        const char *e = std::getenv("FFH_PCG_LAUNCHES");
+        return e ? std::atoi(e) : 0;
+    }();
+    if (forced == 2 || forced == 3) return forced == 3;
+    return f32 || n < kThreeLaunchRows;
+}
+
+// `iters` iterations of Jacobi-PCG (matrix MT, vectors VT, x in XT).  Two
+// launches each (direction + SpMV with variant vi, then the x/r update), or
+// for large systems three (direction in place, SpMV with variant vi_sp,
+// update), both with the reference's recurrence.
 template <typename MT, typename VT, typename XT>
 static void launch_iterations(Context *ctx, const DevMatrix &A, double *partials, KrylovCtl *ctl,
                               int iters, XT *x, VT *r, VT *zpa, VT *zpb, VT *ap, const VT *dinv,
-                              int vi, int prof_slot)
+                              int vi, int vi_sp, int prof_slot)
 {
     ZP<VT> *zp0 = reinterpret_cast<ZP<VT> *>(zpa), *zp1 = reinterpret_cast<ZP<VT> *>(zpb);
     const int n = A.n;
+    if (three_launch(n, sizeof(VT) == 4)) {
+        // one (z, p) array updated in place; the second array holds p
+        // densely (its first n values)
+        VT *pl = zpb;
+        const int gridv = grid_for(ctx, n, kBlock);
+        const int gridd = (int)std::min<int64_t>(
+            gridv, std::max<int64_t>(ctx->cu_count, ((int64_t)n + 4 * kBlock - 1) / (4 * kBlock)));
+        const int grids = variant_grid(ctx, A, vi_sp);
+        const int stride = partial_stride(ctx);
+        const GArgs<MT> g = gargs<MT>(A, vi_sp);
+        for (int it = 0; it < iters; ++it) {
+            hipLaunchKernelGGL((k_pcg_dir<VT>), dim3(gridd), dim3(kBlock), 0, ctx->stream, n, zp0,
+                               pl, partials, stride, ctl);
+            if (prof_slot >= 0) ctx->prof_mark(prof_slot, true);
+            FFH_ENGINE(kVariants[vi_sp].engine, {
+                hipLaunchKernelGGL((k_pcg_apdot<VV, MT, VT>), dim3(grids), dim3(kStreamBlock),
+                                   variant_lds(A, vi_sp, sizeof(VT)), ctx->stream, n, g.ptr,
+                                   g.idx, g.val, pl, ap, g.rb, g.nrb, partials, ctl);
+            });
+            if (prof_slot >= 0) ctx->prof_mark(prof_slot, false);
+            hipLaunchKernelGGL((k_pcg_update<VT, XT>), dim3(gridd), dim3(kBlock), 0, ctx->stream,
+                               n, x, r, zp0, zp0, ap, dinv, partials, stride, grids, ctl);
+        }
+        FFH_HIP(hipGetLastError());
+        return;
+    }
     // about four rows per thread, at least one block per CU: every block of
     // the next direction launch re-sums this launch's partials, so fewer
     // blocks shorten that launch's start (c2: 977 -> 256 blocks)
@@ -1383,7 +1507,8 @@ static void launch_refresh(Context *ctx, Matrix &M, int vi64, int first)
                            variant_lds(A, vi64, sizeof(double)), ctx->stream,
                            A.n, g.ptr, g.idx, g.val, w.x.p, w.b.p, w.r32.p, w.dinv32.p,
                            reinterpret_cast<ZP<float> *>(w.p32.p),
-                           reinterpret_cast<ZP<float> *>(w.p32b.p), g.rb, g.nrb, w.partials.p,
+                           reinterpret_cast<ZP<float> *>(three_launch(A.n, true) ? w.p32.p : w.p32b.p),
+                           g.rb, g.nrb, w.partials.p,
                            stride, w.ctl.p);
     });
     if (ctx->profiling) ctx->prof_mark(0, false);
@@ -1398,7 +1523,7 @@ static void launch_refresh(Context *ctx, Matrix &M, int vi64, int first)
 // `watch` sees each polled control block and may return false to stop early
 // (the stream is then drained).  A chunk may be replayed from a HIP graph.
 template <typename Enq, typename Watch>
-static KrylovCtl run_chunks(Context *ctx, Matrix &M, int gslot, int graph_key, bool use_graph,
+static KrylovCtl run_chunks(Context *ctx, Matrix &M, int gslot, int64_t graph_key, bool use_graph,
                             Enq &&enqueue_chunk, Watch &&watch)
 {
     Workspace &w = M.ws;
@@ -1509,7 +1634,7 @@ static KrylovCtl pcg_fp64_on(Context *ctx, const DevMatrix &A, int vi, int vi_it
     const int prof = ctx->profiling ? 0 : -1;
     auto enqueue = [&]() {
         launch_iterations<double, double, double>(ctx, A, w.partials, w.ctl, chunk, w.x, w.r,
-                                                  w.zpa, w.zpb, w.ap, w.dinv, vi_it, prof);
+                                                  w.zpa, w.zpb, w.ap, w.dinv, vi_it, vi, prof);
         hipLaunchKernelGGL(k_pcg_fin, dim3(1), dim3(kBlock), 0, s, w.partials, stride, w.ctl);
         FFH_HIP(hipGetLastError());
     };
@@ -1528,8 +1653,9 @@ static KrylovCtl pcg_fp64(Context *ctx, Matrix &M, int method, int precond, doub
     const PcgBufs bufs{w.x.p, w.b.p, w.r.p, w.p.p, w.p2.p, w.ap.p, w.dinv.p, w.partials.p, w.ctl.p};
     return pcg_fp64_on(ctx, M.m, vi, vd, bufs, method, precond, tolerance, tol_type, max_iter,
                        breakdown_mode, check_interval, [&](auto &enqueue, int chunk) {
-                           return run_chunks(ctx, M, 0, (chunk * 64 + vi) * 64 + vd, use_graph != 0,
-                                             enqueue, [](const KrylovCtl &) { return true; });
+                           return run_chunks(ctx, M, 0, ((int64_t)chunk * 64 + vi) * 64 + vd,
+                                             use_graph != 0, enqueue,
+                                             [](const KrylovCtl &) { return true; });
                        });
 }
 
@@ -1588,6 +1714,7 @@ void solve_mixed(Context *ctx, Matrix &M, const double *b, double *x,
     ensure_val32(ctx, A);
     const int vi64 = spmv_variant(ctx, M, false);
     const int vi32 = dir_variant(ctx, M, true);
+    const int vi32sp = spmv_variant(ctx, M, true);
     Workspace &w = M.ws;
     const int n = A.n;
     hipStream_t s = ctx->stream;
@@ -1614,13 +1741,14 @@ void solve_mixed(Context *ctx, Matrix &M, const double *b, double *x,
     auto enqueue = [&]() {
         launch_iterations<float, float, double>(ctx, A, w.partials.p, w.ctl.p, chunk, w.x.p,
                                                 w.r32.p, w.p32.p, w.p32b.p, w.ap32.p, w.dinv32.p,
-                                                vi32, prof);
+                                                vi32, vi32sp, prof);
         launch_refresh(ctx, M, vi64, 0);
     };
     double best = 1e300;
     int stall = 0;
     bool stagnated = false;
-    KrylovCtl out = run_chunks(ctx, M, 1, (chunk * 64 + vi32) * 64 + vi64, o.use_graph != 0, enqueue,
+    KrylovCtl out = run_chunks(ctx, M, 1, (((int64_t)chunk * 64 + vi32) * 64 + vi64) * 64 + vi32sp,
+                               o.use_graph != 0, enqueue,
                                [&](const KrylovCtl &c) {
                                    if (c.true_rr < 0.25 * best) {
                                        best = c.true_rr;
@@ -1680,9 +1808,8 @@ double bench_spmv(Context *ctx, Matrix &M, int precision, int reps, double *byte
         FFH_HIP(hipMemsetAsync(w.p.p, 0, sizeof(double) * n, ctx->stream));
     }
     const int sv = f32 ? 4 : 8;
-    // algorithmic bytes: values + column indices + row pointers + x + y
-    *bytes = (double)A.nnz * (sv + 4) + 4.0 * (n + 1) + (double)n * 2 * sv;
-    spmv_variant(ctx, M, f32);
+    const int vi = ready_variant(ctx, A, spmv_variant(ctx, M, f32), f32);
+    *bytes = variant_bytes(A, vi, sv);
     auto once = [&]() {
         if (f32) spmv32(ctx, A, w.p32.p, w.ap32.p);
         else spmv64(ctx, A, w.p.p, w.ap.p);
@@ -1695,6 +1822,17 @@ double bench_spmv(Context *ctx, Matrix &M, int precision, int reps, double *byte
     float ms = 0.0f;
     FFH_HIP(hipEventElapsedTime(&ms, ctx->ev_start, ctx->ev_stop));
     return (double)ms / std::max(1, reps);
+}
+
+// Bytes one plain SpMV launch moves in the format of the variant in use for
+// the precision (tuned first if needed), as variant_bytes.
+double spmv_launch_bytes(Context *ctx, Matrix &M, bool f32)
+{
+    ensure_workspace(ctx, M);
+    ensure_val64(ctx, M.m);
+    if (f32) ensure_val32(ctx, M.m);
+    const int vi = ready_variant(ctx, M.m, spmv_variant(ctx, M, f32), f32);
+    return variant_bytes(M.m, vi, f32 ? 4 : 8);
 }
 
 // ------------------------------------------------ allocation-free solve --

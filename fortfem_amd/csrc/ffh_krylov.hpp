@@ -71,6 +71,11 @@ void sell_values(Context *ctx, const DevMatrix &A, const double *val, double *ou
 // column tiles of the SELL copy (DevMatrix::tile_*), built once; tiles_ok
 // false when a group reads more than 2^16 columns
 void ensure_sell_tiles(Context *ctx, DevMatrix &A);
+// bytes of one SpMV launch of variant vi in that variant's format
+double variant_bytes(const DevMatrix &A, int vi, int sv);
+double spmv_launch_bytes(Context *ctx, Matrix &M, bool f32);
+// whether a Jacobi-PCG iteration on n rows takes three launches
+bool three_launch(int64_t n, bool f32);
 } // namespace ffh
 
 namespace ffh {

@@ -419,9 +419,22 @@ int ffh_solve_sparse_vjp(ffh_matrix *mat, const double *x, const double *x_bar,
 /* ---- measurement ---------------------------------------------------------- */
 /* Times `reps` back-to-back SpMV launches on the context stream with HIP
  * events; precision FFH_FP64 or FFH_FP32.  avg_ms = mean per launch,
- * bytes = algorithmic bytes of one launch. */
+ * bytes = ffh_matrix_spmv_bytes of the variant timed. */
 int ffh_bench_spmv(ffh_matrix *mat, int precision, int reps, double *avg_ms,
                    double *bytes);
+/* Bytes one SpMV launch moves in the storage format of the matrix's SpMV
+ * variant for the precision (no reference counterpart: a measurement aid).
+ * CSR variants: nnz (s + 4) + 4 (n + 1) + 2 n s for value size s; SELL-64:
+ * its padded entries at s + 4 bytes, slice offsets and row map, plus 2 n s;
+ * tiled SELL-64: padded entries at s + 2 bytes (16-bit tile positions),
+ * slice offsets, row map, tile offsets and tile column lists, plus 2 n s. */
+int ffh_matrix_spmv_bytes(ffh_matrix *mat, int precision, double *bytes);
+/* Launches per Jacobi-PCG iteration of a solve with n rows in precision
+ * FFH_FP32 (the mixed solve's iterations) or FFH_FP64: 3 (direction update,
+ * SpMV with A p and p.A p partials, x/r/z update) or 2 (direction formed in
+ * the SpMV's gathers, then the update).  No reference counterpart: it says
+ * which kernels a solve runs.  Returns 2 or 3, or FFH_ERR_INVALID. */
+int ffh_pcg_launches_per_iteration(int64_t n, int precision);
 /* SpMV variant of a matrix (no reference counterpart: a tuning control).
  * request >= 0 forces variant `request` (0 .. FFH_SPMV_VARIANTS-1) for the
  * precision; request < 0 keeps the current choice, running the autotuner
@@ -429,10 +442,12 @@ int ffh_bench_spmv(ffh_matrix *mat, int precision, int reps, double *avg_ms,
  * the variant in use. */
 #define FFH_SPMV_VARIANTS 13
 int ffh_matrix_spmv_variant(ffh_matrix *mat, int precision, int request, int *chosen);
-/* Kernel timing inside solves: when enabled, every 8th direction + SpMV
- * launch of a solve (which 0 = FP64 direction + SpMV of the FP64 solve, or
- * the FP64 residual refresh of a mixed solve; 1 = FP32 direction + SpMV of a
- * mixed solve) is bracketed by HIP events on the context stream (sampling
+/* Kernel timing inside solves: when enabled, every 8th SpMV launch of the
+ * iterations (k_pcg_apdot in the three-launch form, k_pcg_dirspmv in the
+ * two-launch form, see ffh_pcg_launches_per_iteration) of a solve (which
+ * 0 = FP64 iterations of the FP64 solve, or the FP64 residual refresh of a
+ * mixed solve; 1 = FP32 iterations of a mixed solve) is bracketed by HIP
+ * events on the context stream (sampling
  * keeps the event cost off the solve's critical path);
  * ffh_last_solve_profile returns the timed launch count and their mean
  * duration (ms) for the last solve. */
