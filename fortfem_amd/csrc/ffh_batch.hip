@@ -1057,19 +1057,20 @@ static void run_chunk(ffh_server *S)
     S->iterations_run += S->chunk;
 }
 
-// An FP32-iterating request whose FP64 residual has not halved over 16
-// refreshes (the FP32 operator cannot resolve further progress) continues
-// in FP64 from its current x: its directions restart after the next refresh.
+// An FP32-iterating request whose FP64 residual stalls (the stagnation rule
+// of kStallImprove / kStallRefreshes: the FP32 operator cannot resolve
+// further progress) continues in FP64 from its current x: its directions
+// restart after the next refresh.
 static void watch_stagnation(ffh_server *S, ServedRequest *q)
 {
     bool stuck = false;
     for (int r = 0; r < q->nrhs; ++r) {
         const BatchScal &c = S->eng.h_scal[(size_t)q->slot * kMaxRhs + r];
         if (c.state != kActive) continue;
-        if (c.true_rr < 0.25 * q->best_rr[r]) {
+        if (c.true_rr < kStallImprove * q->best_rr[r]) {
             q->best_rr[r] = c.true_rr;
             q->stall[r] = 0;
-        } else if (++q->stall[r] >= 16) {
+        } else if (++q->stall[r] >= kStallRefreshes) {
             stuck = true;
         }
     }

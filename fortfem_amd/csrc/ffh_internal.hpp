@@ -39,6 +39,15 @@ __device__ __forceinline__ int xcd_block()
     return x * per + min(x, rem) + i;
 }
 
+// Mixed-precision stagnation rule (single solves and the served batch): an
+// FP32-iterating solve whose FP64 true residual has not fallen below 0.9 of
+// its best (0.81 in squared norm) for kStallRefreshes consecutive refreshes
+// continues in FP64.  Slowly converging systems (a 1e-10 reduction over tens
+// of thousands of iterations gains ~1 % per refresh) keep their FP32
+// iterations; a residual held at the FP32 operator's accuracy still switches.
+constexpr double kStallImprove = 0.81;
+constexpr int kStallRefreshes = 32;
+
 // ---------------------------------------------------------------- errors --
 struct Error {
     int code;

@@ -1701,9 +1701,9 @@ void solve_fp64(Context *ctx, Matrix &M, const double *b, double *x,
 // b - A x is recomputed in FP64 with the FP64 matrix and replaces the FP32
 // one (reliable update; the direction is kept while the recurrence tracked
 // it), and it alone decides convergence: ||b - A x|| <= tol ||b - A x0||
-// (relative) or <= tol (absolute).  A solve whose FP64 residual stops
-// improving (16 refreshes without a 4x drop) finishes with the FP64
-// iteration from its current x.
+// (relative) or <= tol (absolute).  A solve whose FP64 residual stalls
+// (kStallRefreshes refreshes without falling below kStallImprove of its
+// best squared norm) finishes with the FP64 iteration from its current x.
 void solve_mixed(Context *ctx, Matrix &M, const double *b, double *x,
                  const ffh_solver_options &o, ffh_solver_stats &st)
 {
@@ -1750,10 +1750,10 @@ void solve_mixed(Context *ctx, Matrix &M, const double *b, double *x,
     KrylovCtl out = run_chunks(ctx, M, 1, (((int64_t)chunk * 64 + vi32) * 64 + vi64) * 64 + vi32sp,
                                o.use_graph != 0, enqueue,
                                [&](const KrylovCtl &c) {
-                                   if (c.true_rr < 0.25 * best) {
+                                   if (c.true_rr < kStallImprove * best) {
                                        best = c.true_rr;
                                        stall = 0;
-                                   } else if (++stall >= 16) {
+                                   } else if (++stall >= kStallRefreshes) {
                                        stagnated = true;
                                        return false;
                                    }
