@@ -153,6 +153,8 @@ def run_c3(args):
         d = vals["fp32"] - vals["fp64"]
         row["fp32_rel_fro"] = float(np.linalg.norm(d) / np.linalg.norm(vals["fp64"]))
         row["fp32_rel_max"] = float(np.max(np.abs(d)) / np.max(np.abs(vals["fp64"])))
+        if args.point_tensors:
+            row.update(c3_point_tensors(ff, p, v, t, K, k, nl, args.steps))
         rows.append(row)
         print(json.dumps({"progress": row}), file=sys.stderr, flush=True)
         del p, vals
@@ -163,6 +165,52 @@ def run_c3(args):
                                   "beta = 1" % ",".join(str(k) for k in args.orders)},
            "orders": rows}
     print(json.dumps(out))
+
+
+MFMA_PEAK_TFLOPS = {"fp64": 78.6, "fp32": 157.3}
+
+
+def c3_point_tensors(ff, p, v, t, K, k, nl, steps):
+    """The same assembly with the coefficient evaluated at every point of the
+    rule, K(x) = K_e (1 + 0.25 sin 2 pi x sin 2 pi y sin 2 pi z): the curl
+    part is the dense per-cell C^T blockdiag(M_q) C product on MFMA
+    (k_ned_point_curl), the mass part the class-table kernel.  Times the
+    assembly with the point tensors resident; MFMA rate on the executed
+    count (the kernel's 32 x 32 regions of the upper triangle, 16 x 16 x 4
+    MFMAs over the rule padded to whole chunks) and the useful upper-triangle
+    count n (n + 1) / 2 * 2 * 3 nq per cell."""
+    T = t.shape[0]
+    nq = ff.nedelec_load_points(k).shape[0]
+    X = ff.nedelec_tensor_points(v, t, k)
+    s = 1.0 + 0.25 * np.sin(2 * np.pi * X[..., 0]) * np.sin(2 * np.pi * X[..., 1]) * np.sin(2 * np.pi * X[..., 2])
+    Kp = K[:, None] * s[..., None, None]
+    p.assemble(-1, mass=1.0, point_tensor=Kp)
+    del Kp, X, s
+    ldn = 32 * ((nl + 31) // 32)
+    ns = ldn // 32
+    regions = ns * (ns + 1) // 2
+    out = {"point_nq": nq}
+    vals = {}
+    for prec in ("fp64", "fp32"):
+        pc = 4 if prec == "fp64" else 8
+        steps_k = -(-nq // pc) * 3 * pc // 4
+        executed = float(T) * regions * 4 * steps_k * 2.0 * 16 * 16 * 4
+        useful = float(T) * nl * (nl + 1) / 2 * 2.0 * 3 * nq
+        ts = []
+        for _ in range(max(1, steps)):
+            t1 = time.perf_counter()
+            p.assemble(-1, mass=1.0, reuse_point_tensor=True, precision=prec)
+            ts.append(time.perf_counter() - t1)
+        ms = float(np.median(ts)) * 1e3
+        out["point_" + prec + "_ms"] = round(ms, 3)
+        out["point_" + prec + "_mfma_tflops_executed"] = round(executed / (ms * 1e-3) / 1e12, 2)
+        out["point_" + prec + "_mfma_frac"] = round(executed / (ms * 1e-3) / 1e12 / MFMA_PEAK_TFLOPS[prec], 4)
+        out["point_" + prec + "_tflops_useful"] = round(useful / (ms * 1e-3) / 1e12, 2)
+        vals[prec] = p.matrix(ff.capi.FULL).download().val
+    d = vals["fp32"] - vals["fp64"]
+    out["point_fp32_rel_fro"] = float(np.linalg.norm(d) / np.linalg.norm(vals["fp64"]))
+    out["point_fp32_rel_max"] = float(np.max(np.abs(d)) / np.max(np.abs(vals["fp64"])))
+    return out
 
 
 def served_stream(args):
@@ -421,6 +469,8 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=0, help="CPU baseline threads (0: min(16, nproc))")
     ap.add_argument("--cpu-reps", type=int, default=5, help="CPU baseline timed steps (after 1 warm-up)")
     ap.add_argument("--graph", action="store_true", help="replay iteration chunks from HIP graphs")
+    ap.add_argument("--point-tensors", action="store_true",
+                    help="c3: also time the assembly with the tensor evaluated at every rule point (MFMA)")
     ap.add_argument("--orders", type=int, nargs="+", default=[1, 2, 3, 4, 5, 6],
                     help="Nedelec orders of the c3 workload")
     ap.add_argument("--requests", type=int, default=16, help="served: the first REQUESTS of the stream")

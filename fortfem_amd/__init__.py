@@ -294,6 +294,7 @@ class Problem:
         n = C.c_int(0)
         nnz = C.c_int64(0)
         self._check(self._lib.ffh_pattern_build(self.handle, family, order, C.byref(n), C.byref(nnz)))
+        self._family, self._order = family, order
         return n.value, nnz.value
 
     def sizes(self):
@@ -318,12 +319,29 @@ class Problem:
         return cs, cells
 
     def assemble(self, quadrature_degree, kappa=1.0, mass=0.0, cell_kappa=None, precision="fp64",
-                 reuse_coefficients=False, cell_tensor=None, reuse_tensor=False):
+                 reuse_coefficients=False, cell_tensor=None, reuse_tensor=False, point_tensor=None,
+                 reuse_point_tensor=False):
         """Assemble the family's operator.  H1: kappa (stiffness) and mass;
         Nedelec: kappa = curl coefficient alpha, mass = beta, or cell_tensor
-        (T, 3, 3) symmetric tensors on the curl term (weighted variant)."""
+        (T, 3, 3) symmetric tensors on the curl term (weighted variant), or
+        point_tensor (T, nq, 3, 3): the tensor at every point of
+        nedelec_load_points(order) mapped into each cell (the reference's
+        coefficient(x_q)); reuse_point_tensor keeps the last uploaded ones."""
         prec = capi.FP32 if precision == "fp32" else capi.FP64
-        if reuse_tensor:
+        if reuse_point_tensor:
+            self._check(self._lib.ffh_assemble(self.handle, quadrature_degree, capi.COEFF_POINT_TENSOR,
+                                               None, float(mass), prec))
+        elif point_tensor is not None:
+            K = np.asarray(point_tensor, dtype=np.float64)
+            if getattr(self, "_family", None) != capi.FAMILY_NEDELEC:
+                raise ValueError("point tensors need a Nedelec pattern")
+            nq = nedelec_load_points(self._order).shape[0]
+            if K.shape != (self.ncell, nq, 3, 3):
+                raise ValueError(f"point_tensor must have shape ({self.ncell}, {nq}, 3, 3)")
+            K = np.ascontiguousarray(K.transpose(0, 1, 3, 2))
+            self._check(self._lib.ffh_assemble(self.handle, quadrature_degree, capi.COEFF_POINT_TENSOR,
+                                               _ptr(K.reshape(-1)), float(mass), prec))
+        elif reuse_tensor:
             self._check(self._lib.ffh_assemble(self.handle, quadrature_degree, capi.COEFF_CELL_TENSOR,
                                                None, float(mass), prec))
         elif cell_tensor is not None:
@@ -1075,22 +1093,25 @@ def assemble_tetra_nedelec_curvilinear_pml_csc(mesh_vertices, tetrahedra, order,
 def assemble_tetra_nedelec_weighted_csc(mesh_vertices, tetrahedra, coefficient, mass_coefficient,
                                         order=None, precision="fp64", ctx: Context | None = None):
     """assemble_tetra_nedelec_weighted_csc (:1870-2000).  coefficient: array
-    (T, 3, 3) of symmetric per-cell tensors, or a callable K(x, y, z) -> (3, 3)
-    that must be constant on every cell (it is sampled at the four vertices
-    and the centroid; a field varying inside a cell is rejected, because the
-    device element uses the reference-tensor form for cell-constant K).
-    -> (CSC, status)."""
+    (T, 3, 3) of symmetric per-cell tensors, (T, nq, 3, 3) tensors at the
+    points of nedelec_tensor_points, or a callable K(x, y, z) -> (3, 3),
+    evaluated at those points as the reference does (:2497-2501).  Tensors
+    constant on every cell use the class-table element; others the per-point
+    element (dense curl contraction on MFMA).  -> (CSC, status)."""
     status = FortsparseStatus(FORTSPARSE_INVALID_MATRIX, "Tetrahedral weighted Nedelec assembly failed")
     k = 1 if order is None else int(order)
     v = np.asarray(mesh_vertices, dtype=np.float64)
     t = np.asarray(tetrahedra)
     if k < 1 or v.ndim != 2 or v.shape[1] != 3 or t.ndim != 2 or t.shape[1] != 4 or t.shape[0] < 1:
         return None, status
-    K = _cell_tensors(v, t, coefficient)
+    form, K = _tensor_coefficient(v, t, coefficient, k)
     try:
         p = Problem(v, t, ctx)
         p.pattern_build(capi.FAMILY_NEDELEC, k)
-        p.assemble(-1, mass=float(mass_coefficient), cell_tensor=K, precision=precision)
+        if form == "cell":
+            p.assemble(-1, mass=float(mass_coefficient), cell_tensor=K, precision=precision)
+        else:
+            p.assemble(-1, mass=float(mass_coefficient), point_tensor=K, precision=precision)
         a = p.matrix(capi.FULL).download()
     except FFHError as e:
         if e.code == capi.FFH_ERR_DEVICE:
@@ -1187,21 +1208,35 @@ def _nedelec_source_samples(mesh_vertices, tetrahedra, source, order):
                      np.broadcast_to(fz, X.shape[:2])], axis=-1)
 
 
-def _cell_tensors(mesh_vertices, tetrahedra, coefficient):
-    """Per-cell tensors (T, 3, 3) from an array or from a callable K(x, y, z)
-    that must be constant on every cell (checked at the vertices and the
-    centroid)."""
+def nedelec_tensor_points(mesh_vertices, tetrahedra, order):
+    """The points x_q = v1 + J xi_q of the weighted element's rule
+    (nedelec_load_points) in every cell: (T, nq, 3)."""
+    pts = nedelec_load_points(order)
     v = np.asarray(mesh_vertices, dtype=np.float64)
+    t = np.asarray(tetrahedra) - 1
+    v0 = v[t[:, 0]]
+    J = np.stack([v[t[:, 1]] - v0, v[t[:, 2]] - v0, v[t[:, 3]] - v0], axis=2)
+    return v0[:, None, :] + np.einsum("tij,jq->tqi", J, pts[:, :3].T)
+
+
+def _tensor_coefficient(mesh_vertices, tetrahedra, coefficient, order):
+    """The curl tensor of the weighted element as ("cell", (T, 3, 3)) or
+    ("point", (T, nq, 3, 3)).  An array is taken as given; a callable
+    K(x, y, z) -> (3, 3) is evaluated where the reference evaluates it, at
+    every point of the rule mapped into each cell, and the cell form is used
+    when it is the same at all points of every cell."""
     t = np.asarray(tetrahedra)
     if not callable(coefficient):
-        return np.asarray(coefficient, dtype=np.float64).reshape(t.shape[0], 3, 3)
-    pts = v[t - 1]
-    K = np.array([np.asarray(coefficient(*c), dtype=np.float64) for c in pts.mean(axis=1)])
-    for j in range(4):
-        Kj = np.array([np.asarray(coefficient(*c), dtype=np.float64) for c in pts[:, j]])
-        if np.max(np.abs(Kj - K)) > 1e-14 * max(1.0, np.max(np.abs(K))):
-            raise NotImplementedError("coefficient tensors must be constant on each cell")
-    return K
+        K = np.asarray(coefficient, dtype=np.float64)
+        if K.ndim == 4:
+            return "point", K
+        return "cell", K.reshape(t.shape[0], 3, 3)
+    X = nedelec_tensor_points(mesh_vertices, tetrahedra, order)
+    Kp = np.array([[np.asarray(coefficient(*x), dtype=np.float64) for x in cell] for cell in X])
+    Kp = Kp.reshape(X.shape[0], X.shape[1], 3, 3)
+    if np.array_equal(Kp, np.broadcast_to(Kp[:, :1], Kp.shape)):
+        return "cell", np.ascontiguousarray(Kp[:, 0])
+    return "point", Kp
 
 
 def _nedelec_solve(vertices, tetrahedra, order, source, assemble, zero_tangential_boundary, opts, ctx,
@@ -1251,16 +1286,17 @@ def solve_tetra_nedelec_weighted_curl_mass(vertices, tetrahedra, order, curl_coe
                                            mass_coefficient, zero_tangential_boundary=False,
                                            opts: SolverOptionsT | None = None, ctx: Context | None = None):
     """solve_tetra_nedelec_weighted_curl_mass (fortfem_tetra_nedelec_solver_3d.f90:41-95)
-    with Jacobi-PCG; curl_coefficient is a (T, 3, 3) array of symmetric
-    per-cell tensors or a callable K(x, y, z) constant on every cell.
-    -> (solution, status, stats)."""
+    with Jacobi-PCG; curl_coefficient as in assemble_tetra_nedelec_weighted_csc
+    (per-cell or per-point tensors, or a callable K(x, y, z) evaluated at the
+    rule's points).  -> (solution, status, stats)."""
     if mass_coefficient <= 0.0:
         return None, FortsparseStatus(FORTSPARSE_INVALID_MATRIX,
                                       "Weighted tetrahedral Nedelec curl-mass solve failed"), None
-    K = _cell_tensors(vertices, tetrahedra, curl_coefficient)
+    form, K = _tensor_coefficient(vertices, tetrahedra, curl_coefficient, order)
+    kw = {"cell_tensor": K} if form == "cell" else {"point_tensor": K}
     return _nedelec_solve(
         vertices, tetrahedra, order, source,
-        lambda p: p.assemble(-1, mass=float(mass_coefficient), cell_tensor=K),
+        lambda p: p.assemble(-1, mass=float(mass_coefficient), **kw),
         zero_tangential_boundary, opts, ctx, "Weighted tetrahedral Nedelec curl-mass solve failed")
 
 

@@ -25,6 +25,7 @@ FAMILY_P1_2D = 3
 COEFF_SCALAR = 0
 COEFF_CELL = 1
 COEFF_CELL_TENSOR = 2
+COEFF_POINT_TENSOR = 3
 
 SOURCE_CONSTANT = 0
 SOURCE_SIN_PRODUCT = 1

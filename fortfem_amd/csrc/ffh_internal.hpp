@@ -268,6 +268,14 @@ struct Problem {
     // points (x, y, z, w), and the per-cell g_q = det w_q J^-1 f(x_q)
     DBuf<double> d_ned_load_tab, d_ned_load_pts, d_ned_g;
     int ned_load_nq = 0;
+    // Nedelec with a tensor at every point of the weighted rule: oriented
+    // reference curls per class slot (slot x nq x 3 x ldn, zero padded to
+    // ldn = 32 ceil(n / 32)), FP64 and FP32, the rule (x, y, z, w per
+    // point) and the uploaded point tensors (9 per point, cell-major)
+    DBuf<double> d_ned_ct64, d_ned_pt_pts, d_ptens;
+    DBuf<float> d_ned_ct32;
+    int ned_pt_nq = 0, ned_pt_ldn = 0;
+    bool ned_pt_ready = false;
     // full matrix + reduced matrix
     Matrix full, reduced;
     DBuf<double> rhs_full, rhs_red;

@@ -873,8 +873,11 @@ void nedelec_load_rule(int k, std::vector<double> &pts)
     }
 }
 
-void nedelec_load_tables(int k, const std::vector<int> &classes, std::vector<double> &tab,
-                         std::vector<double> &pts)
+// Oriented reference values (part 3) or curls (part 0) of every class slot
+// at the weighted element's rule: tab[((i * nq + q) * 3 + c) * ldn + r] =
+// sum_j B_j,part+c(point q) T_jr, zero for r >= n.
+static void point_tables(int k, const std::vector<int> &classes, int part, int ldn,
+                         std::vector<double> &tab, std::vector<double> &pts)
 {
     const int n = nedelec_dof_count(k);
     std::vector<xd> px, py, pz, pw, B;
@@ -888,7 +891,7 @@ void nedelec_load_tables(int k, const std::vector<int> &classes, std::vector<dou
         pts[4 * q + 2] = (double)pz[q];
         pts[4 * q + 3] = (double)pw[q];
     }
-    const size_t per = (size_t)nq * 3 * n;
+    const size_t per = (size_t)nq * 3 * ldn;
     tab.assign(classes.size() * per, 0.0);
     parallel_for((int64_t)classes.size(), [&](int64_t lo, int64_t hi) {
         for (int64_t i = lo; i < hi; ++i) {
@@ -901,12 +904,24 @@ void nedelec_load_tables(int k, const std::vector<int> &classes, std::vector<dou
                         xd acc = 0;
                         for (int j = 0; j < n; ++j) {
                             const xd t = T[(size_t)j * n + r];
-                            if (t != 0) acc += B[((size_t)q * n + j) * 6 + 3 + c] * t;
+                            if (t != 0) acc += B[((size_t)q * n + j) * 6 + part + c] * t;
                         }
-                        out[((size_t)q * 3 + c) * n + r] = (double)acc;
+                        out[((size_t)q * 3 + c) * ldn + r] = (double)acc;
                     }
         }
     }, 1);
+}
+
+void nedelec_load_tables(int k, const std::vector<int> &classes, std::vector<double> &tab,
+                         std::vector<double> &pts)
+{
+    point_tables(k, classes, 3, nedelec_dof_count(k), tab, pts);
+}
+
+void nedelec_curl_point_tables(int k, const std::vector<int> &classes, int ldn,
+                               std::vector<double> &tab, std::vector<double> &pts)
+{
+    point_tables(k, classes, 0, ldn, tab, pts);
 }
 
 } // namespace ffh

@@ -58,6 +58,12 @@ void nedelec_class_tables(int k, bool weighted_rule, const std::vector<int> &cla
 // g_q = det w_q J^-1 f(x_q).
 void nedelec_load_tables(int k, const std::vector<int> &classes, std::vector<double> &tab,
                          std::vector<double> &pts);
+// Oriented reference curls at the same rule (the weighted element's rule of
+// assemble_tetra_nedelec_weighted_element(_order), assembly_tetra_nedelec_3d.f90:
+// 2406-2514): tab[((i * nq + q) * 3 + a) * ldn + r] = sum_j curl_j,a(point q)
+// T_jr for class slot i, zero for n <= r < ldn.
+void nedelec_curl_point_tables(int k, const std::vector<int> &classes, int ldn,
+                               std::vector<double> &tab, std::vector<double> &pts);
 // the load points alone (x, y, z, w per point)
 void nedelec_load_rule(int k, std::vector<double> &pts);
 

@@ -581,6 +581,180 @@ __global__ __launch_bounds__(256) void k_ned_assemble(
     }
 }
 
+// ------------------------------------------- per-point weighted Nedelec --
+// The curl part of assemble_tetra_nedelec_weighted_element(_order)
+// (assembly_tetra_nedelec_3d.f90:2406-2514) with its tensor evaluated at
+// every point x_q of the rule: with physical curls J c^ / det, in the class's
+// oriented basis the element curl matrix is the dense product
+//   E = C^T blockdiag(M_q) C,  M_q = w_q J^T K(x_q) J / det,
+// C[(q, a), j] the oriented reference curls (3 rows per point).  One
+// workgroup per cell; each wave owns up to SPW regions of 32 x 32 entries
+// (2 x 2 MFMA tiles of 16 x 16) of E's upper triangle.  The rule is walked in
+// chunks of PC points: their rows of C and of D = blockdiag(M) C are staged
+// in LDS and every 4 rows are one MFMA per tile (FP32 or FP64 accumulation).
+// E's upper entries are added through the cell's slots to both (r, c) and
+// (c, r) from the same number (the mass part and the first-touch stores come
+// before, from the class-table kernel), so the matrix stays exactly
+// symmetric.  A colour's cells share no dof, so the adds do not race.
+template <typename CT>
+struct MfmaTile;
+template <>
+struct MfmaTile<float> {
+    typedef float v4 __attribute__((ext_vector_type(4)));
+    static __device__ __forceinline__ v4 op(float a, float b, v4 c)
+    {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+    }
+    // row of result register r in lane l (column l % 16)
+    static __device__ __forceinline__ int row(int l, int r) { return 4 * (l >> 4) + r; }
+};
+template <>
+struct MfmaTile<double> {
+    typedef double v4 __attribute__((ext_vector_type(4)));
+    static __device__ __forceinline__ v4 op(double a, double b, v4 c)
+    {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+    }
+    static __device__ __forceinline__ int row(int l, int r) { return (l >> 4) + 4 * r; }
+};
+
+template <typename CT, int W, int SPW, int PC>
+__global__ __launch_bounds__(64 * W) void k_ned_point_curl(
+    int pos0, int ncell, int n, int ldn, int nq, const int *__restrict__ cells_c,
+    const double *__restrict__ verts, const int *__restrict__ perm,
+    const unsigned char *__restrict__ cls_c, const double *__restrict__ ptens,
+    const double *__restrict__ pts, const CT *__restrict__ ctab,
+    const int *__restrict__ slots_cm, CT *__restrict__ val, int *__restrict__ err)
+{
+    using MT = MfmaTile<CT>;
+    using v4 = typename MT::v4;
+    constexpr int NT = 64 * W, ROWS = 3 * PC;
+    static_assert(ROWS % 4 == 0, "chunk rows must fill MFMA steps");
+    extern __shared__ __align__(16) unsigned char pc_lds[];
+    CT *sC = reinterpret_cast<CT *>(pc_lds); // [ROWS][ldn]
+    CT *sD = sC + (size_t)ROWS * ldn;        // [ROWS][ldn]
+    __shared__ double sJ[9], sdet;
+    __shared__ int sok;
+    __shared__ CT sM[PC][9];
+    const int pos = pos0 + blockIdx.x;
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    if (tid == 0) {
+        const int4 c = reinterpret_cast<const int4 *>(cells_c)[pos];
+        double jac[9], inv[9], det;
+        sok = tet_geometry(verts, c.x, c.y, c.z, c.w, jac, inv, det) ? 1 : 0;
+        if (!sok) atomicOr(err, 1);
+        for (int i = 0; i < 9; ++i) sJ[i] = jac[i];
+        sdet = det;
+    }
+    __syncthreads();
+    if (!sok) return;
+    const int e = perm[pos];
+    const CT *C = ctab + (size_t)cls_c[pos] * nq * 3 * ldn;
+    // this wave's 32 x 32 regions (bi <= bj), in row-major upper order
+    const int ns = ldn >> 5;
+    int rb[SPW], cb[SPW];
+#pragma unroll
+    for (int t = 0; t < SPW; ++t) {
+        int sidx = wv + W * t, bi = 0;
+        rb[t] = cb[t] = -1;
+        while (bi < ns && sidx >= ns - bi) {
+            sidx -= ns - bi;
+            ++bi;
+        }
+        if (bi < ns) {
+            rb[t] = 32 * bi;
+            cb[t] = 32 * (bi + sidx);
+        }
+    }
+    v4 acc[SPW][2][2];
+#pragma unroll
+    for (int t = 0; t < SPW; ++t)
+#pragma unroll
+        for (int a = 0; a < 2; ++a)
+#pragma unroll
+            for (int b = 0; b < 2; ++b) acc[t][a][b] = v4{CT(0), CT(0), CT(0), CT(0)};
+    for (int q0 = 0; q0 < nq; q0 += PC) {
+        const int nqc = min(PC, nq - q0);
+        // M_q = w_q J^T K_q J / det (K column-major), in FP64
+        if (tid < nqc) {
+            const double *K = ptens + ((size_t)e * nq + q0 + tid) * 9;
+            const double w = pts[4 * (q0 + tid) + 3] / sdet;
+            double KJ[9]; // K J, column-major
+#pragma unroll
+            for (int j = 0; j < 3; ++j)
+#pragma unroll
+                for (int i = 0; i < 3; ++i)
+                    KJ[i + 3 * j] = K[i] * sJ[3 * j] + K[i + 3] * sJ[1 + 3 * j] +
+                                    K[i + 6] * sJ[2 + 3 * j];
+#pragma unroll
+            for (int j = 0; j < 3; ++j)
+#pragma unroll
+                for (int i = 0; i < 3; ++i)
+                    sM[tid][i + 3 * j] = (CT)(w * (sJ[3 * i] * KJ[3 * j] + sJ[1 + 3 * i] * KJ[1 + 3 * j] +
+                                                   sJ[2 + 3 * i] * KJ[2 + 3 * j]));
+        }
+        // rows of C for the chunk (zero past the rule)
+        for (int i = tid; i < ROWS * ldn; i += NT) {
+            const int r = i / ldn;
+            sC[i] = (r < 3 * nqc) ? C[(size_t)(3 * q0 + r) * ldn + (i - r * ldn)] : CT(0);
+        }
+        __syncthreads();
+        // D = blockdiag(M) C
+        for (int i = tid; i < ROWS * ldn; i += NT) {
+            const int r = i / ldn, j = i - r * ldn;
+            const int qq = r / 3, a = r - 3 * qq;
+            CT d = CT(0);
+            if (qq < nqc) {
+                const CT *c3 = sC + (size_t)(3 * qq) * ldn + j;
+                d = sM[qq][a] * c3[0] + sM[qq][a + 3] * c3[ldn] + sM[qq][a + 6] * c3[2 * ldn];
+            }
+            sD[i] = d;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k0 = 0; k0 < ROWS; k0 += 4) {
+            const int kr = k0 + (lane >> 4), li = lane & 15;
+            const CT *ca = sC + (size_t)kr * ldn + li;
+            const CT *db = sD + (size_t)kr * ldn + li;
+#pragma unroll
+            for (int t = 0; t < SPW; ++t) {
+                if (rb[t] < 0) continue;
+                const CT a0 = ca[rb[t]], a1 = ca[rb[t] + 16];
+                const CT b0 = db[cb[t]], b1 = db[cb[t] + 16];
+                acc[t][0][0] = MT::op(a0, b0, acc[t][0][0]);
+                acc[t][0][1] = MT::op(a0, b1, acc[t][0][1]);
+                acc[t][1][0] = MT::op(a1, b0, acc[t][1][0]);
+                acc[t][1][1] = MT::op(a1, b1, acc[t][1][1]);
+            }
+        }
+        __syncthreads();
+    }
+    // upper entries into the CSC values, both (r, c) and (c, r)
+    const int nn = n * n;
+    const int *sl = slots_cm + (size_t)pos * nn;
+#pragma unroll
+    for (int t = 0; t < SPW; ++t) {
+        if (rb[t] < 0) continue;
+#pragma unroll
+        for (int a = 0; a < 2; ++a)
+#pragma unroll
+            for (int b = 0; b < 2; ++b)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int row = rb[t] + 16 * a + MT::row(lane, r);
+                    const int col = cb[t] + 16 * b + (lane & 15);
+                    if (row >= n || col >= n || row > col) continue;
+                    const CT v = acc[t][a][b][r];
+                    const int s1 = sl[col * n + row];
+                    val[s1 < 0 ? ~s1 : s1] += v;
+                    if (row != col) {
+                        const int s2 = sl[row * n + col];
+                        val[s2 < 0 ? ~s2 : s2] += v;
+                    }
+                }
+    }
+}
+
 // Tangent of the 12 Nedelec terms of the curl-mass variant
 // (assemble_tetra_nedelec_curl_mass_element_jvp,
 // assembly_tetra_nedelec_3d.f90:645-716, in the reference-tensor form):
@@ -1281,6 +1455,10 @@ void problem_pattern(Problem *P, int family, int order)
         P->d_cls_c.upload(cls_c, s);
         P->ned_rule = -1;
         P->d_ned_load_tab.reset();
+        P->ned_pt_ready = false;
+        P->d_ned_ct64.reset();
+        P->d_ned_ct32.reset();
+        P->d_ptens.reset();
     }
     // CSC pattern and element-entry slots: on the device (radix sort of the
     // entry keys) when the keys fit comfortably, else on the host
@@ -1367,6 +1545,55 @@ static void ensure_ned_tables(Problem *P, bool weighted)
     P->ned_rule = rule;
 }
 
+// Oriented reference curls at the weighted rule for the classes present
+// (k_ned_point_curl), once per pattern.
+static void ensure_ned_point_tables(Problem *P)
+{
+    if (P->ned_pt_ready) return;
+    hipStream_t s = P->ctx->stream;
+    const int k = P->order, n = P->n_local;
+    const int ldn = 32 * ((n + 31) / 32);
+    std::vector<double> tab, pts;
+    nedelec_curl_point_tables(k, P->class_list, ldn, tab, pts);
+    P->d_ned_ct64.upload(tab, s);
+    std::vector<float> t32(tab.begin(), tab.end());
+    P->d_ned_ct32.upload(t32, s);
+    P->d_ned_pt_pts.upload(pts, s);
+    FFH_HIP(hipStreamSynchronize(s));
+    P->ned_pt_nq = (int)(pts.size() / 4);
+    P->ned_pt_ldn = ldn;
+    P->ned_pt_ready = true;
+}
+
+// The curl part of the per-point weighted element for the cells of one
+// colour (positions p0 .. p0 + cnt - 1): waves and regions per wave by order.
+template <typename CT>
+static void launch_point_curl(Problem *P, int p0, int cnt, CT *val, const CT *ctab)
+{
+    hipStream_t s = P->ctx->stream;
+    const int n = P->n_local, ldn = P->ned_pt_ldn, nq = P->ned_pt_nq;
+    constexpr int PC = sizeof(CT) == 8 ? 4 : 8;
+    const size_t lds = (size_t)2 * 3 * PC * ldn * sizeof(CT);
+    const int ns = ldn / 32, regions = ns * (ns + 1) / 2;
+#define FFH_PC_LAUNCH(W, SPW)                                                                   \
+    FFH_REQUIRE((W) * (SPW) >= regions && lds <= 65536, FFH_ERR_INVALID,                        \
+                "per-point Nedelec kernel shape does not cover the element");                  \
+    hipLaunchKernelGGL((k_ned_point_curl<CT, W, SPW, PC>), dim3(cnt), dim3(64 * W), lds, s, p0, \
+                       P->ncell, n, ldn, nq, P->d_cells_c.p, P->d_vertices.p, P->d_perm.p,     \
+                       P->d_cls_c.p, P->d_ptens.p, P->d_ned_pt_pts.p, ctab, P->d_slots_cm.p,   \
+                       val, P->d_err.p)
+    switch (P->order) { // regions of 32 x 32 in the upper triangle: 1, 1, 3, 6, 15, 28
+    case 1:
+    case 2: FFH_PC_LAUNCH(1, 1); break;
+    case 3: FFH_PC_LAUNCH(3, 1); break;
+    case 4: FFH_PC_LAUNCH(6, 1); break;
+    case 5: FFH_PC_LAUNCH(5, 3); break;
+    case 6: FFH_PC_LAUNCH(7, 4); break;
+    default: throw Error{FFH_ERR_INVALID, "Nedelec order must be 1..6"};
+    }
+#undef FFH_PC_LAUNCH
+}
+
 static void assemble_nedelec(Problem *P, int coeff_kind, const double *coeff, double mass,
                              int precision)
 {
@@ -1405,6 +1632,29 @@ static void assemble_nedelec(Problem *P, int coeff_kind, const double *coeff, do
                     "cell tensors were never uploaded");
         dtensor = P->d_coeff.p;
         weighted = true;
+    } else if (coeff_kind == FFH_COEFF_POINT_TENSOR) {
+        ensure_ned_point_tables(P);
+        const int nq = P->ned_pt_nq;
+        if (coeff) {
+            const int64_t cnt = (int64_t)T * nq;
+            for (int64_t i = 0; i < cnt; ++i) {
+                const double *K = coeff + 9 * i;
+                double mx = 0.0;
+                for (int j = 0; j < 9; ++j) {
+                    FFH_REQUIRE(std::isfinite(K[j]), FFH_ERR_INVALID, "non-finite point tensor");
+                    mx = std::max(mx, std::fabs(K[j]));
+                }
+                FFH_REQUIRE(std::fabs(K[1] - K[3]) <= 1e-12 * mx &&
+                                std::fabs(K[2] - K[6]) <= 1e-12 * mx &&
+                                std::fabs(K[5] - K[7]) <= 1e-12 * mx,
+                            FFH_ERR_INVALID, "point tensors must be symmetric");
+            }
+            P->d_ptens.upload(coeff, (size_t)9 * cnt, s);
+        }
+        FFH_REQUIRE(P->d_ptens.p && P->d_ptens.n == (size_t)9 * T * nq, FFH_ERR_INVALID,
+                    "point tensors were never uploaded");
+        alpha = 0.0; // the class-table pass adds the mass part only
+        weighted = true;
     } else {
         throw Error{FFH_ERR_INVALID, "unknown coefficient kind"};
     }
@@ -1432,7 +1682,17 @@ static void assemble_nedelec(Problem *P, int coeff_kind, const double *coeff, do
                                T, nl, P->d_cls_c.p, P->d_gterm.p, P->d_ned64.p, P->d_slots_cm.p,
                                A.val.p);
     }
+    if (coeff_kind == FFH_COEFF_POINT_TENSOR)
+        for (int c = 0; c < P->n_colors; ++c) {
+            const int p0 = P->color_start[c], cnt = P->color_start[c + 1] - p0;
+            if (cnt == 0) continue;
+            if (f32)
+                launch_point_curl<float>(P, p0, cnt, A.val32.p, P->d_ned_ct32.p);
+            else
+                launch_point_curl<double>(P, p0, cnt, A.val.p, P->d_ned_ct64.p);
+        }
     FFH_HIP(hipGetLastError());
+    check_err(P, "Nedelec assembly");
     A.values_changed();
     A.val32_valid = f32;
     A.val64_stale = f32;

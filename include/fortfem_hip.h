@@ -44,6 +44,9 @@ extern "C" {
 #define FFH_COEFF_SCALAR 0      /* coeff[0]                              */
 #define FFH_COEFF_CELL 1        /* coeff[T], one scalar per cell          */
 #define FFH_COEFF_CELL_TENSOR 2 /* coeff[9*T], column-major 3x3 per cell  */
+#define FFH_COEFF_POINT_TENSOR 3 /* coeff[9*nq*T]: column-major 3x3 per     */
+                                 /* point of ffh_nedelec_load_points (the   */
+                                 /* weighted element's rule), cell-major    */
 
 /* load kinds for ffh_load */
 #define FFH_SOURCE_CONSTANT 0    /* params[0] = f                          */
@@ -184,7 +187,11 @@ int ffh_problem_colouring(ffh_problem *prob, int base, int *color_start, int *ce
  * cell kappa as the stiffness_coefficient of :47-48),
  * assemble_tetra_nedelec_curl_mass_csc (:2042-2112; coeff = curl alpha,
  * mass = beta) and assemble_tetra_nedelec_weighted_csc (:1870-2000; coeff =
- * tensor K).  precision FFH_FP32 runs the element contraction in FP32. */
+ * tensor K: constant per cell with FFH_COEFF_CELL_TENSOR, or the reference's
+ * coefficient(x_q) sampled at every point x_q = v1 + J xi_q of the rule that
+ * ffh_nedelec_load_points returns, FFH_COEFF_POINT_TENSOR: the curl part is
+ * then a dense per-cell C^T blockdiag(M_q) C product on MFMA).  Tensors must
+ * be symmetric.  precision FFH_FP32 runs the element contraction in FP32. */
 int ffh_assemble(ffh_problem *prob, int quadrature_degree, int coeff_kind,
                  const double *coeff, double mass_coefficient, int precision);
 /* assemble_tetra_lagrange_stiffness_csc_jvp (src/assembly/

@@ -592,6 +592,64 @@ def nedelec_element(k, vertices4, qdeg=None, alpha=1.0, beta=1.0, tensor=None):
     return st, m.T.copy()
 
 
+def nedelec_rule_points(k, qdeg=None):
+    """Points of the Nedelec element rule (the 4-point rule at k = 1, Duffy of
+    degree qdeg, default 2k + 2, otherwise): (nq, 4) rows x, y, z, w."""
+    if qdeg is None:
+        qdeg = 2 * k + 2
+    cap = 4 if k == 1 else int(_lib().ffo_tetra_duffy_count(C.c_int(qdeg)))
+    out = np.zeros(4 * cap)
+    n = C.c_int(0)
+    _check(_lib().ffo_nedelec_rule_points(C.c_int(k), C.c_int(qdeg), C.c_int(cap),
+                                          out.ctypes.data_as(C.c_void_p), C.byref(n)), "rule_points")
+    return out[:4 * n.value].reshape(n.value, 4)
+
+
+def nedelec_element_points(k, vertices4, point_tensors, beta=1.0, qdeg=None):
+    """The weighted element (:2406-2514) with a tensor at every point of the
+    rule: point_tensors (nq, 3, 3) in the order of nedelec_rule_points."""
+    n = nedelec_dof_count(k)
+    cf = nedelec_coefficients(k) if k >= 2 else np.zeros(1)
+    if qdeg is None:
+        qdeg = 2 * k + 2
+    K = np.ascontiguousarray(np.transpose(np.asarray(point_tensors, dtype=np.float64), (0, 2, 1)).reshape(-1))
+    m = np.zeros((n, n))
+    st = _lib().ffo_nedelec_element_points(C.c_int(k), cf.ctypes.data_as(C.c_void_p),
+                                           _f(vertices4).ctypes.data_as(C.c_void_p), C.c_int(qdeg),
+                                           K.ctypes.data_as(C.c_void_p), C.c_double(beta),
+                                           m.ctypes.data_as(C.c_void_p))
+    return st, m.T.copy()
+
+
+def nedelec_csc_points(k, vertices, tets, point_tensors, beta=1.0):
+    """assemble_tetra_nedelec_weighted_csc restated with the coefficient
+    evaluated at every point of the weighted rule: point_tensors (T, nq, 3, 3),
+    points nedelec_rule_points(k) mapped into each cell as v1 + J xi."""
+    v, t = _f(vertices), _i(tets)
+    n = nedelec_dof_count(k)
+    nt = t.shape[0] * n * n
+    rows = np.zeros(nt, dtype=np.int32)
+    cols = np.zeros(nt, dtype=np.int32)
+    vals = np.zeros(nt)
+    nd = C.c_int(0)
+    K = np.ascontiguousarray(np.transpose(np.asarray(point_tensors, dtype=np.float64), (0, 1, 3, 2)).reshape(-1))
+    _check(_lib().ffo_nedelec_triplets_points(C.c_int(k), C.c_int(v.shape[0]), v.ctypes.data_as(C.c_void_p),
+                                              C.c_int(t.shape[0]), t.ctypes.data_as(C.c_void_p), C.c_double(beta),
+                                              K.ctypes.data_as(C.c_void_p), rows.ctypes.data_as(C.c_void_p),
+                                              cols.ctypes.data_as(C.c_void_p), vals.ctypes.data_as(C.c_void_p),
+                                              C.byref(nd)), "nedelec_triplets_points")
+    return csc_from_triplet(nd.value, nd.value, rows, cols, vals)
+
+
+def mapped_points(vertices, tets, ref_points):
+    """x = v1 + J xi for every cell and reference point: (T, nq, 3)."""
+    v, t = np.asarray(vertices, dtype=np.float64), np.asarray(tets) - 1
+    v0 = v[t[:, 0]]
+    J = np.stack([v[t[:, 1]] - v0, v[t[:, 2]] - v0, v[t[:, 3]] - v0], axis=2)  # (T, 3, 3) columns
+    xi = np.asarray(ref_points)[:, :3]
+    return v0[:, None, :] + np.einsum("tij,qj->tqi", J, xi)
+
+
 def nedelec_csc(k, vertices, tets, alpha=1.0, beta=1.0, cell_tensor=None):
     """assemble_tetra_nedelec_curl_mass_csc / _weighted_csc restated.
     cell_tensor: (T, 3, 3) per-cell tensors (weighted variant)."""

@@ -183,6 +183,12 @@ int ffo_nedelec_vjp(int k, int nvert, const double *vertices, int ntet, const in
                     double alpha, double beta, const double *cell_alpha, const int *col_ptr,
                     const int *row_idx, const double *values_bar, double *vertices_bar,
                     double *alpha_bar, double *cell_alpha_bar, double *beta_bar);
+int ffo_nedelec_element_points(int k, const double *coeff, const double v[12], int qdeg,
+                               const double *point_tensors, double beta, double *m);
+int ffo_nedelec_rule_points(int k, int qdeg, int cap, double *out, int *npts);
+int ffo_nedelec_triplets_points(int k, int nvert, const double *vertices, int ntet,
+                                const int *tets, double beta, const double *point_tensors,
+                                int *rows, int *cols, double *vals, int *ndofs);
 int ffo_nedelec_triplets(int k, int nvert, const double *vertices, int ntet,
                          const int *tets, double alpha, double beta,
                          const double *cell_tensor, int *rows, int *cols,

@@ -756,8 +756,46 @@ static int element_rule(int k, int qdeg, double **px, double **py, double **pz, 
  * tensor = K(3x3 column-major, constant on the cell), alpha ignored).
  * Covariant map (src/elements/tetra_piola_maps.f90:18-57): v = J^-T v^,
  * curl = J c^ / det. */
+static int element_impl(int k, const double *coeff, const double v[12], int qdeg, double alpha,
+                        const double *tensor, int tensor_stride, double beta, double *m);
+
 int ffo_nedelec_element(int k, const double *coeff, const double v[12], int qdeg, double alpha,
                         const double *tensor, double beta, double *m)
+{
+    return element_impl(k, coeff, v, qdeg, alpha, tensor, 0, beta, m);
+}
+
+/* The weighted element with the reference's coefficient(x_q) evaluated at
+ * every point of the rule (:2406-2514): point_tensors holds 9 values
+ * (column-major) per point of element_rule(k, qdeg), in point order. */
+int ffo_nedelec_element_points(int k, const double *coeff, const double v[12], int qdeg,
+                               const double *point_tensors, double beta, double *m)
+{
+    return element_impl(k, coeff, v, qdeg, 0.0, point_tensors, 9, beta, m);
+}
+
+/* the points of element_rule(k, qdeg): x, y, z, w per point (npts <= cap) */
+int ffo_nedelec_rule_points(int k, int qdeg, int cap, double *out, int *npts)
+{
+    double *qx, *qy, *qz, *qw;
+    int nq = element_rule(k, qdeg, &qx, &qy, &qz, &qw);
+    *npts = nq;
+    if (nq > cap) {
+        free(qx);
+        return 1;
+    }
+    for (int q = 0; q < nq; ++q) {
+        out[4 * q] = qx[q];
+        out[4 * q + 1] = qy[q];
+        out[4 * q + 2] = qz[q];
+        out[4 * q + 3] = qw[q];
+    }
+    free(qx);
+    return 0;
+}
+
+static int element_impl(int k, const double *coeff, const double v[12], int qdeg, double alpha,
+                        const double *tensor, int tensor_stride, double beta, double *m)
 {
     int n = ffo_nedelec_dof_count(k);
     double jac[9], inv[9], det;
@@ -779,11 +817,13 @@ int ffo_nedelec_element(int k, const double *coeff, const double v[12], int qdeg
                 pc[3 * j + r] = (jac[r] * rc[3 * j] + jac[r + 3] * rc[3 * j + 1] +
                                  jac[r + 6] * rc[3 * j + 2]) / det;
             }
-        if (tensor)
+        if (tensor) {
+            const double *K = tensor + (size_t)tensor_stride * q;
             for (int j = 0; j < n; ++j)
                 for (int r = 0; r < 3; ++r)
-                    kc[3 * j + r] = tensor[r] * pc[3 * j] + tensor[r + 3] * pc[3 * j + 1] +
-                                    tensor[r + 6] * pc[3 * j + 2];
+                    kc[3 * j + r] = K[r] * pc[3 * j] + K[r + 3] * pc[3 * j + 1] +
+                                    K[r + 6] * pc[3 * j + 2];
+        }
         double pw = det * qw[q];
         for (int c = 0; c < n; ++c)
             for (int r = 0; r < n; ++r) {
@@ -814,9 +854,36 @@ int ffo_nedelec_element(int k, const double *coeff, const double v[12], int qdeg
  * optional) selects the weighted variant with quadrature 2k+2; otherwise the
  * curl-mass variant with quadrature 2k.  Returns the triplet count and the
  * global dimension. */
+static int triplets_impl(int k, int nvert, const double *vertices, int ntet, const int *tets,
+                         double alpha, double beta, const double *cell_tensor, int point_nq,
+                         int *rows, int *cols, double *vals, int *ndofs);
+
 int ffo_nedelec_triplets(int k, int nvert, const double *vertices, int ntet, const int *tets,
                          double alpha, double beta, const double *cell_tensor, int *rows,
                          int *cols, double *vals, int *ndofs)
+{
+    return triplets_impl(k, nvert, vertices, ntet, tets, alpha, beta, cell_tensor, 0, rows, cols,
+                         vals, ndofs);
+}
+
+/* assemble_tetra_nedelec_weighted_csc (:1870-2000) with the coefficient
+ * evaluated at every point of the weighted rule (2k + 2; 4 points at k = 1):
+ * point_tensors holds 9 values (column-major) per point, cell-major
+ * (cell e, point q at 9 (e nq + q)). */
+int ffo_nedelec_triplets_points(int k, int nvert, const double *vertices, int ntet,
+                                const int *tets, double beta, const double *point_tensors,
+                                int *rows, int *cols, double *vals, int *ndofs)
+{
+    double *qx, *qy, *qz, *qw;
+    int nq = element_rule(k, 2 * k + 2, &qx, &qy, &qz, &qw);
+    free(qx);
+    return triplets_impl(k, nvert, vertices, ntet, tets, 0.0, beta, point_tensors, nq, rows,
+                         cols, vals, ndofs);
+}
+
+static int triplets_impl(int k, int nvert, const double *vertices, int ntet, const int *tets,
+                         double alpha, double beta, const double *cell_tensor, int point_nq,
+                         int *rows, int *cols, double *vals, int *ndofs)
 {
     int n = ffo_nedelec_dof_count(k);
     int *gd = malloc(sizeof(int) * (size_t)n * ntet);
@@ -852,8 +919,9 @@ int ffo_nedelec_triplets(int k, int nvert, const double *vertices, int ntet, con
             for (int c = 0; c < 3; ++c) v[3 * i + c] = vertices[3 * (id - 1) + c];
         }
         if (st) break;
-        const double *K = cell_tensor ? cell_tensor + 9 * (size_t)e : NULL;
-        if (ffo_nedelec_element(k, coeff, v, qdeg, alpha, K, beta, E)) {
+        const double *K = cell_tensor ? cell_tensor + 9 * (size_t)e * (point_nq ? point_nq : 1)
+                                      : NULL;
+        if (element_impl(k, coeff, v, qdeg, alpha, K, point_nq ? 9 : 0, beta, E)) {
             st = 1;
             break;
         }
