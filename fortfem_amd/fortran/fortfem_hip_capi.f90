@@ -22,6 +22,7 @@ module fortfem_hip_capi
     integer(c_int), parameter, public :: FFH_COEFF_SCALAR = 0
     integer(c_int), parameter, public :: FFH_COEFF_CELL = 1
     integer(c_int), parameter, public :: FFH_COEFF_CELL_TENSOR = 2
+    integer(c_int), parameter, public :: FFH_COEFF_POINT_TENSOR = 3
     integer(c_int), parameter, public :: FFH_SOURCE_CONSTANT = 0
     integer(c_int), parameter, public :: FFH_SOURCE_SIN_PRODUCT = 1
     integer(c_int), parameter, public :: FFH_SOURCE_SAMPLES = 2

@@ -515,7 +515,7 @@ module fortfem_assembly_tetra_nedelec_3d
     public :: assemble_tetra_nedelec_vector_load_order
     public :: tensor_coefficient_3d
     public :: vector_source_3d
-    public :: sample_vector_source, sample_cell_tensors
+    public :: sample_vector_source, sample_rule_tensors
 
     abstract interface
         pure subroutine tensor_coefficient_3d(x, y, z, value)
@@ -620,36 +620,62 @@ contains
         ok = .true.
     end function sample_vector_source
 
-    ! One tensor per cell from the callback, which must return the same
-    ! tensor at each cell's centroid and four vertices (the device element
-    ! takes cell-constant tensors); false otherwise.
-    logical function sample_cell_tensors(mesh_vertices, tetrahedra, coefficient, &
-            tensors) result(ok)
+    ! The curl tensor of the weighted element where the reference evaluates
+    ! it (:2435, :2499): at every point of the rule (ffh_nedelec_load_points)
+    ! mapped into each cell.  values holds 9 per point (column-major),
+    ! cell-major, and kind is FFH_COEFF_POINT_TENSOR; when the tensor is the
+    ! same at all points of every cell, one tensor per cell and
+    ! FFH_COEFF_CELL_TENSOR (the class-table element).  False on bad input.
+    logical function sample_rule_tensors(mesh_vertices, tetrahedra, order, &
+            coefficient, values, kind) result(ok)
         real(dp), intent(in) :: mesh_vertices(:, :)
-        integer, intent(in) :: tetrahedra(:, :)
+        integer, intent(in) :: tetrahedra(:, :), order
         procedure(tensor_coefficient_3d) :: coefficient
-        real(dp), allocatable, intent(out) :: tensors(:, :, :)
-        real(dp) :: centre(3), kv(3, 3), scale
-        integer :: cell, corner
+        real(c_double), allocatable, intent(out) :: values(:)
+        integer(c_int), intent(out) :: kind
+
+        real(c_double), allocatable, target :: pts(:, :)
+        real(dp), allocatable :: kp(:, :, :, :)
+        real(dp) :: jac(3, 3), point(3)
+        integer(c_int) :: nq
+        integer :: cell, q, ncell
+        logical :: constant
 
         ok = .false.
+        kind = FFH_COEFF_CELL_TENSOR
         if (size(mesh_vertices, 1) /= 3 .or. size(tetrahedra, 1) /= 4) return
+        if (size(tetrahedra, 2) < 1) return
         if (any(tetrahedra < 1) .or. &
             any(tetrahedra > size(mesh_vertices, 2))) return
-        allocate(tensors(3, 3, size(tetrahedra, 2)))
-        do cell = 1, size(tetrahedra, 2)
-            centre = 0.25_dp*sum(mesh_vertices(:, tetrahedra(:, cell)), dim=2)
-            call coefficient(centre(1), centre(2), centre(3), tensors(:, :, cell))
-            scale = max(1.0_dp, maxval(abs(tensors(:, :, cell))))
-            do corner = 1, 4
-                associate (x => mesh_vertices(:, tetrahedra(corner, cell)))
-                    call coefficient(x(1), x(2), x(3), kv)
-                end associate
-                if (maxval(abs(kv - tensors(:, :, cell))) > 1.0e-14_dp*scale) return
+        if (order < 1 .or. order > 6) return
+        if (ffh_nedelec_load_points(int(order, c_int), nq, c_null_ptr) /= FFH_OK) return
+        allocate(pts(4, nq))
+        if (ffh_nedelec_load_points(int(order, c_int), nq, c_loc(pts)) /= FFH_OK) return
+        ncell = size(tetrahedra, 2)
+        allocate(kp(3, 3, nq, ncell))
+        constant = .true.
+        do cell = 1, ncell
+            jac(:, 1) = mesh_vertices(:, tetrahedra(2, cell)) - &
+                mesh_vertices(:, tetrahedra(1, cell))
+            jac(:, 2) = mesh_vertices(:, tetrahedra(3, cell)) - &
+                mesh_vertices(:, tetrahedra(1, cell))
+            jac(:, 3) = mesh_vertices(:, tetrahedra(4, cell)) - &
+                mesh_vertices(:, tetrahedra(1, cell))
+            do q = 1, nq
+                point = mesh_vertices(:, tetrahedra(1, cell)) + &
+                    matmul(jac, pts(1:3, q))
+                call coefficient(point(1), point(2), point(3), kp(:, :, q, cell))
+                if (any(kp(:, :, q, cell) /= kp(:, :, 1, cell))) constant = .false.
             end do
         end do
+        if (constant) then
+            values = reshape(kp(:, :, 1, :), [9*ncell])
+        else
+            values = reshape(kp, [9*int(nq)*ncell])
+            kind = FFH_COEFF_POINT_TENSOR
+        end if
         ok = .true.
-    end function sample_cell_tensors
+    end function sample_rule_tensors
 
     ! Shared driver: pattern + assembly of one Nedelec problem, downloaded.
     subroutine nedelec_csc(mesh_vertices, tetrahedra, k, kind, coeff, mass, &
@@ -1092,10 +1118,10 @@ contains
         end if
     end subroutine assemble_tetra_nedelec_curl_mass_csc_vjp
 
-    ! :1870-2000.  The device element takes one symmetric tensor per cell:
-    ! the callback is sampled at each cell's centroid and four vertices and
-    ! must return the same tensor at all five points (a tensor varying inside
-    ! a cell fails with a status message instead of being approximated).
+    ! :1870-2000.  The callback is evaluated at every point of the rule in
+    ! every cell, as the reference does; tensors constant on each cell take
+    ! the class-table element, others the per-point element (dense curl
+    ! contraction on MFMA).
     subroutine assemble_tetra_nedelec_weighted_csc(mesh_vertices, tetrahedra, &
             coefficient, mass_coefficient, matrix, status, order)
         real(dp), intent(in) :: mesh_vertices(:, :)
@@ -1105,14 +1131,20 @@ contains
         type(csc_t), intent(out) :: matrix
         type(fortsparse_status_t), intent(out) :: status
         integer, intent(in), optional :: order
-        real(dp), allocatable :: tensors(:, :, :)
+        real(c_double), allocatable :: values(:)
+        integer(c_int) :: kind
+        integer :: k
+        logical :: ok
 
         call status_set(status, FORTSPARSE_INVALID_MATRIX, &
-            "weighted Nedelec assembly needs a tensor constant on each cell")
-        if (.not. sample_cell_tensors(mesh_vertices, tetrahedra, coefficient, &
-            tensors)) return
-        call assemble_tetra_nedelec_weighted_csc_cells(mesh_vertices, &
-            tetrahedra, tensors, mass_coefficient, matrix, status, order)
+            "Tetrahedral weighted Nedelec assembly failed")
+        k = 1
+        if (present(order)) k = order
+        if (.not. sample_rule_tensors(mesh_vertices, tetrahedra, k, coefficient, &
+            values, kind)) return
+        call nedelec_csc(mesh_vertices, tetrahedra, k, kind, values, &
+            mass_coefficient, matrix, ok)
+        if (ok) call status_set(status, 0, "")
     end subroutine assemble_tetra_nedelec_weighted_csc
 
     ! New: one symmetric tensor K(3,3,cell) per cell.
@@ -1678,7 +1710,7 @@ module fortfem_tetra_nedelec_solver_3d
     use fortfem_hip_capi
     use fortfem_hip_runtime, only: hip_context
     use fortfem_assembly_tetra_nedelec_3d, only: tensor_coefficient_3d, &
-        vector_source_3d, sample_vector_source, sample_cell_tensors
+        vector_source_3d, sample_vector_source, sample_rule_tensors
     use fortfem_advanced_solvers, only: solver_options_t, solver_stats_t, to_c_options
     ! the PML solves stay on the CPU
     use fortfem_tetra_nedelec_solver_3d_cpu, only: solve_tetra_nedelec_pml, &
@@ -1718,8 +1750,8 @@ contains
         if (present(stats)) stats = st
     end subroutine solve_tetra_nedelec_curl_mass
 
-    ! :41-95.  The tensor must be constant on each cell (sampled at the
-    ! centroid and the four vertices).
+    ! :41-95.  The tensor is evaluated at the rule's points as in
+    ! assemble_tetra_nedelec_weighted_csc.
     subroutine solve_tetra_nedelec_weighted_curl_mass(vertices, tetrahedra, &
             order, curl_coefficient, source, mass_coefficient, solution, status, &
             zero_tangential_boundary, options, stats)
@@ -1734,16 +1766,17 @@ contains
         type(solver_options_t), intent(in), optional :: options
         type(solver_stats_t), intent(out), optional :: stats
 
-        real(dp), allocatable :: tensors(:, :, :)
+        real(c_double), allocatable :: values(:)
+        integer(c_int) :: kind
         type(solver_stats_t) :: st
 
         call status_set(status, FORTSPARSE_INVALID_MATRIX, &
             "Weighted tetrahedral Nedelec curl-mass solve failed")
         if (order < 1 .or. mass_coefficient <= 0.0_dp) return
-        if (.not. sample_cell_tensors(vertices, tetrahedra, curl_coefficient, &
-            tensors)) return
+        if (.not. sample_rule_tensors(vertices, tetrahedra, order, &
+            curl_coefficient, values, kind)) return
         call nedelec_solve(vertices, tetrahedra, order, source, &
-            FFH_COEFF_CELL_TENSOR, tensors, mass_coefficient, solution, status, &
+            kind, values, mass_coefficient, solution, status, &
             zero_tangential_boundary, options, st)
         if (present(stats)) stats = st
     end subroutine solve_tetra_nedelec_weighted_curl_mass

@@ -39,6 +39,15 @@ contains
         value = 3.0_dp*pi*pi*sin(pi*x)*sin(pi*y)*sin(pi*z)
     end subroutine sin_source
 
+    ! a symmetric tensor field linear in x: K0 + x K1
+    pure subroutine linear_tensor(x, y, z, value)
+        real(dp), intent(in) :: x, y, z
+        real(dp), intent(out) :: value(3, 3)
+        value = reshape([2.0_dp, 0.3_dp, 0.1_dp, 0.3_dp, 1.5_dp, -0.2_dp, &
+            0.1_dp, -0.2_dp, 1.0_dp], [3, 3]) + x*reshape([0.8_dp, 0.2_dp, 0.0_dp, &
+            0.2_dp, -0.4_dp, 0.1_dp, 0.0_dp, 0.1_dp, 0.6_dp], [3, 3]) + 0.0_dp*(y + z)
+    end subroutine linear_tensor
+
 end module dropin_test_sources
 
 program test_fortfem_hip_dropin
@@ -53,14 +62,15 @@ program test_fortfem_hip_dropin
         assemble_tetra_nedelec_vector_load, assemble_tetra_nedelec_curl_mass_csc_jvp, &
         assemble_tetra_nedelec_curl_mass_csc_vjp, assemble_tetra_nedelec_pml_csc, &
         assemble_tetra_nedelec_curvilinear_pml_csc, assemble_tetra_nedelec_pml_csc_jvp, &
-        assemble_tetra_nedelec_pml_csc_vjp
+        assemble_tetra_nedelec_pml_csc_vjp, assemble_tetra_nedelec_weighted_csc
     use fortfem_sparse_matrix, only: spmv, spmv_jvp, spmv_vjp
     use fortfem_advanced_solvers, only: solver_options_t, solver_stats_t, &
         solver_options, solve_sparse, solve_sparse_jvp, solve_sparse_vjp
     use fortfem_tetra_lagrange_solver_3d, only: solve_tetra_lagrange_poisson
     use fortfem_tetra_nedelec_solver_3d, only: solve_tetra_nedelec_curl_mass
     use fortfem_hip_capi, only: ffh_nedelec_dof_map
-    use dropin_test_sources, only: bubble_source, zero_boundary, sin_source, constant_field
+    use dropin_test_sources, only: bubble_source, zero_boundary, sin_source, constant_field, &
+        linear_tensor
     implicit none
 
     logical :: all_passed
@@ -72,6 +82,7 @@ program test_fortfem_hip_dropin
     call check_quartic_p4()
     call check_p2_box_poisson()
     call check_nedelec_rotational_energy()
+    call check_nedelec_point_tensor_energy()
     call check_nedelec_constant_load()
     call check_spmv_ad()
     call check_pcg_ad()
@@ -186,6 +197,63 @@ contains
         call record(abs(dot_product(dofs, product) - 28.0_dp/3.0_dp) < 3.0e-12_dp, &
             "Nedelec rotational-field energy is 28/3")
     end subroutine check_nedelec_rotational_energy
+
+    ! assemble_tetra_nedelec_weighted_csc (:1870-2000) with a tensor varying
+    ! inside the cells, K(x) = K0 + x K1: the callback is evaluated at the
+    ! rule's points (per-point element on MFMA).  The Whitney interpolant of
+    ! 0.5 c x x has curl c on both cells of the two-tet mesh, and the 4-point
+    ! rule integrates the linear field exactly, so the curl energy is
+    ! c^T (|T1| K(centroid 1) + |T2| K(centroid 2)) c = c^T (K0 + K1 / 4) c / 3.
+    subroutine check_nedelec_point_tensor_energy()
+        real(dp) :: mesh_vertices(3, 5), c(3), mid(3), tangent(3), field(3)
+        real(dp) :: k0(3, 3), k1(3, 3), exact
+        real(dp), allocatable :: dofs(:), product(:)
+        integer :: tetrahedra(4, 2), edges(2, 12), cell, l, a, b, e, nedges
+        integer, parameter :: local(2, 6) = reshape([1, 2, 1, 3, 1, 4, 2, 3, 2, 4, 3, 4], [2, 6])
+        logical :: found
+        type(csc_t) :: matrix
+        type(fortsparse_status_t) :: status
+
+        mesh_vertices = reshape([0.0_dp, 0.0_dp, 0.0_dp, 1.0_dp, 0.0_dp, 0.0_dp, &
+            0.0_dp, 1.0_dp, 0.0_dp, 0.0_dp, 0.0_dp, 1.0_dp, 0.0_dp, 0.0_dp, -1.0_dp], [3, 5])
+        tetrahedra(:, 1) = [1, 2, 3, 4]
+        tetrahedra(:, 2) = [1, 3, 2, 5]
+        call assemble_tetra_nedelec_weighted_csc(mesh_vertices, tetrahedra, &
+            linear_tensor, 0.0_dp, matrix, status, order=1)
+        call record(status%code == 0 .and. matrix%nrow == 9, &
+            "weighted Nedelec CSC with a per-point tensor succeeds")
+        nedges = 0
+        do cell = 1, 2
+            do l = 1, 6
+                a = minval(tetrahedra(local(:, l), cell))
+                b = maxval(tetrahedra(local(:, l), cell))
+                found = .false.
+                do e = 1, nedges
+                    if (edges(1, e) == a .and. edges(2, e) == b) found = .true.
+                end do
+                if (.not. found) then
+                    nedges = nedges + 1
+                    edges(:, nedges) = [a, b]
+                end if
+            end do
+        end do
+        allocate(dofs(nedges), product(nedges))
+        c = [1.0_dp, -2.0_dp, 3.0_dp]
+        do e = 1, nedges
+            mid = 0.5_dp*(mesh_vertices(:, edges(1, e)) + mesh_vertices(:, edges(2, e)))
+            tangent = mesh_vertices(:, edges(2, e)) - mesh_vertices(:, edges(1, e))
+            field = 0.5_dp*[c(2)*mid(3) - c(3)*mid(2), c(3)*mid(1) - c(1)*mid(3), &
+                c(1)*mid(2) - c(2)*mid(1)]
+            dofs(e) = dot_product(field, tangent)
+        end do
+        call spmv(matrix, dofs, product)
+        call linear_tensor(0.0_dp, 0.0_dp, 0.0_dp, k0)
+        call linear_tensor(1.0_dp, 0.0_dp, 0.0_dp, k1)
+        k1 = k1 - k0
+        exact = dot_product(c, matmul(k0 + 0.25_dp*k1, c))/3.0_dp
+        call record(abs(dot_product(dofs, product) - exact) < 1.0e-12_dp*abs(exact), &
+            "per-point tensor curl energy is c^T (K0 + K1/4) c / 3")
+    end subroutine check_nedelec_point_tensor_energy
 
     ! assemble_tetra_nedelec_vector_load (:2002-2040): a constant field f lies
     ! in the Whitney space with dofs d_e = f . (v_b - v_a) on the edges
