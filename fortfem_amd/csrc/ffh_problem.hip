@@ -618,6 +618,9 @@ struct MfmaTile<double> {
     static __device__ __forceinline__ int row(int l, int r) { return (l >> 4) + 4 * r; }
 };
 
+template <typename CT>
+constexpr int kPointCurlPad = sizeof(CT) == 4 ? 16 : 0;
+
 template <typename CT, int W, int SPW, int PC>
 __global__ __launch_bounds__(64 * W) void k_ned_point_curl(
     int pos0, int ncell, int n, int ldn, int nq, const int *__restrict__ cells_c,
@@ -629,10 +632,13 @@ __global__ __launch_bounds__(64 * W) void k_ned_point_curl(
     using MT = MfmaTile<CT>;
     using v4 = typename MT::v4;
     constexpr int NT = 64 * W, ROWS = 3 * PC;
-    static_assert(ROWS % 4 == 0, "chunk rows must fill MFMA steps");
+    static_assert(ROWS % 4 == 0 && ROWS < 128, "chunk rows must fill MFMA steps");
+    // LDS row stride: FP32 rows are padded by 16 words so the four rows an
+    // MFMA operand read touches fall on two halves of the banks
+    const int ls = ldn + kPointCurlPad<CT>;
     extern __shared__ __align__(16) unsigned char pc_lds[];
-    CT *sC = reinterpret_cast<CT *>(pc_lds); // [ROWS][ldn]
-    CT *sD = sC + (size_t)ROWS * ldn;        // [ROWS][ldn]
+    CT *sC = reinterpret_cast<CT *>(pc_lds); // [ROWS][ls]
+    CT *sD = sC + (size_t)ROWS * ls;         // [ROWS][ls]
     __shared__ double sJ[9], sdet;
     __shared__ int sok;
     __shared__ CT sM[PC][9];
@@ -693,29 +699,50 @@ __global__ __launch_bounds__(64 * W) void k_ned_point_curl(
                     sM[tid][i + 3 * j] = (CT)(w * (sJ[3 * i] * KJ[3 * j] + sJ[1 + 3 * i] * KJ[1 + 3 * j] +
                                                    sJ[2 + 3 * i] * KJ[2 + 3 * j]));
         }
-        // rows of C for the chunk (zero past the rule)
-        for (int i = tid; i < ROWS * ldn; i += NT) {
-            const int r = i / ldn;
-            sC[i] = (r < 3 * nqc) ? C[(size_t)(3 * q0 + r) * ldn + (i - r * ldn)] : CT(0);
+        // (row, column) of element i = r ldn + j advance without divisions:
+        // i += NT moves by NT / ldn rows and NT % ldn columns
+        const int dr = NT / ldn, dj = NT - dr * ldn;
+        // rows of C for the chunk, contiguous in the table (zero past the rule)
+        {
+            const CT *src = C + (size_t)3 * q0 * ldn;
+            const int live = 3 * nqc * ldn;
+            int r = tid / ldn, j = tid - r * ldn;
+            for (int i = tid; i < ROWS * ldn; i += NT) {
+                sC[r * ls + j] = i < live ? src[i] : CT(0);
+                r += dr;
+                j += dj;
+                if (j >= ldn) {
+                    j -= ldn;
+                    ++r;
+                }
+            }
         }
         __syncthreads();
         // D = blockdiag(M) C
-        for (int i = tid; i < ROWS * ldn; i += NT) {
-            const int r = i / ldn, j = i - r * ldn;
-            const int qq = r / 3, a = r - 3 * qq;
-            CT d = CT(0);
-            if (qq < nqc) {
-                const CT *c3 = sC + (size_t)(3 * qq) * ldn + j;
-                d = sM[qq][a] * c3[0] + sM[qq][a + 3] * c3[ldn] + sM[qq][a + 6] * c3[2 * ldn];
+        {
+            int r = tid / ldn, j = tid - r * ldn;
+            for (int i = tid; i < ROWS * ldn; i += NT) {
+                const int qq = (r * 43) >> 7, a = r - 3 * qq; // r / 3 for r < 128
+                CT d = CT(0);
+                if (qq < nqc) {
+                    const CT *c3 = sC + (size_t)(3 * qq) * ls + j;
+                    d = sM[qq][a] * c3[0] + sM[qq][a + 3] * c3[ls] + sM[qq][a + 6] * c3[2 * ls];
+                }
+                sD[r * ls + j] = d;
+                r += dr;
+                j += dj;
+                if (j >= ldn) {
+                    j -= ldn;
+                    ++r;
+                }
             }
-            sD[i] = d;
         }
         __syncthreads();
-#pragma unroll
+#pragma unroll 1
         for (int k0 = 0; k0 < ROWS; k0 += 4) {
             const int kr = k0 + (lane >> 4), li = lane & 15;
-            const CT *ca = sC + (size_t)kr * ldn + li;
-            const CT *db = sD + (size_t)kr * ldn + li;
+            const CT *ca = sC + (size_t)kr * ls + li;
+            const CT *db = sD + (size_t)kr * ls + li;
 #pragma unroll
             for (int t = 0; t < SPW; ++t) {
                 if (rb[t] < 0) continue;
@@ -1572,23 +1599,25 @@ static void launch_point_curl(Problem *P, int p0, int cnt, CT *val, const CT *ct
 {
     hipStream_t s = P->ctx->stream;
     const int n = P->n_local, ldn = P->ned_pt_ldn, nq = P->ned_pt_nq;
-    constexpr int PC = sizeof(CT) == 8 ? 4 : 8;
-    const size_t lds = (size_t)2 * 3 * PC * ldn * sizeof(CT);
+    constexpr int PC = sizeof(CT) == 8 ? 8 : 16;
+    const size_t lds = (size_t)2 * 3 * PC * (ldn + kPointCurlPad<CT>) * sizeof(CT);
     const int ns = ldn / 32, regions = ns * (ns + 1) / 2;
 #define FFH_PC_LAUNCH(W, SPW)                                                                   \
-    FFH_REQUIRE((W) * (SPW) >= regions && lds <= 65536, FFH_ERR_INVALID,                        \
+    FFH_REQUIRE((W) * (SPW) >= regions && lds <= 98304, FFH_ERR_INVALID,                        \
                 "per-point Nedelec kernel shape does not cover the element");                  \
     hipLaunchKernelGGL((k_ned_point_curl<CT, W, SPW, PC>), dim3(cnt), dim3(64 * W), lds, s, p0, \
                        P->ncell, n, ldn, nq, P->d_cells_c.p, P->d_vertices.p, P->d_perm.p,     \
                        P->d_cls_c.p, P->d_ptens.p, P->d_ned_pt_pts.p, ctab, P->d_slots_cm.p,   \
                        val, P->d_err.p)
-    switch (P->order) { // regions of 32 x 32 in the upper triangle: 1, 1, 3, 6, 15, 28
+    // regions of 32 x 32 in the upper triangle: 1, 1, 3, 6, 15, 28; waves in
+    // multiples of the CU's 4 SIMDs where the regions allow it
+    switch (P->order) {
     case 1:
     case 2: FFH_PC_LAUNCH(1, 1); break;
     case 3: FFH_PC_LAUNCH(3, 1); break;
-    case 4: FFH_PC_LAUNCH(6, 1); break;
-    case 5: FFH_PC_LAUNCH(5, 3); break;
-    case 6: FFH_PC_LAUNCH(7, 4); break;
+    case 4: FFH_PC_LAUNCH(8, 1); break;
+    case 5: FFH_PC_LAUNCH(8, 2); break;
+    case 6: FFH_PC_LAUNCH(8, 4); break;
     default: throw Error{FFH_ERR_INVALID, "Nedelec order must be 1..6"};
     }
 #undef FFH_PC_LAUNCH
