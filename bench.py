@@ -434,6 +434,31 @@ def cpu_baseline(ff, p, meta, rhs, x_gpu, threads: int, reps: int):
     }
 
 
+def run_served_one(args):
+    """One request of the seeded served stream (--request-index) alone on the
+    batched server, with the precision policy --precision, for --steps server
+    steps of --chunk iterations: a fixed kernel sequence for counter runs."""
+    import fortfem_amd as ff
+    from fortfem_amd import workloads
+
+    r = workloads.served_stream(args.request_index + 1)[args.request_index]
+    v, t = ff.unit_box_mesh(r["N"])
+    srv = ff.Server(budget_bytes=int(args.budget_gb * (1 << 30)), max_systems=args.slots, chunk=args.chunk)
+    srv.set_precision(args.precision)
+    n, nnz = srv.prepare(0, r["family"], r["order"], v, t)
+    coeff, rhs = workloads.request_data(r, t.shape[0], n)
+    srv.submit(0, r["family"], r["order"], coeff, rhs, mass=0.0 if r["family"] == 1 else 1.0,
+               tol=1e-10, max_iterations=args.max_iter)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        srv.step()
+    dt = time.perf_counter() - t0
+    print(json.dumps({"workload": "served1", "request": args.request_index, "family": r["family"],
+                      "order": r["order"], "N": r["N"], "nrhs": r["nrhs"], "n": n, "nnz": nnz,
+                      "precision_policy": args.precision, "steps": args.steps, "chunk": args.chunk,
+                      "s": round(dt, 4)}))
+
+
 def run_copy(args):
     """Calibration for the HBM-traffic counters: a device copy of known size
     (1 GiB of FP32 read, 1 GiB written, through torch's elementwise kernel),
@@ -463,7 +488,8 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="c4",
-                    help="c4 (default), c2, c3, c5, or copy (traffic-counter calibration)")
+                    help="c4 (default), c2, c3, c5, copy (traffic-counter calibration) or "
+                         "served1 (one served request alone, for counter runs)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-served", action="store_true", help="skip the served-stream keys")
     ap.add_argument("--cpu-threads", type=int, default=0, help="CPU baseline threads (0: min(16, nproc))")
@@ -474,6 +500,8 @@ def main():
     ap.add_argument("--orders", type=int, nargs="+", default=[1, 2, 3, 4, 5, 6],
                     help="Nedelec orders of the c3 workload")
     ap.add_argument("--requests", type=int, default=16, help="served: the first REQUESTS of the stream")
+    ap.add_argument("--request-index", type=int, default=1,
+                    help="served1: the request of the stream served alone")
     ap.add_argument("--stream", type=int, default=4096, help="served: length of the seeded stream")
     ap.add_argument("--arrival-factor", type=float, default=0.8,
                     help="served: Poisson arrival rate as a fraction of the saturated request rate")
@@ -497,6 +525,9 @@ def main():
         return
     if args.workload == "copy":
         run_copy(args)
+        return
+    if args.workload == "served1":
+        run_served_one(args)
         return
 
     import fortfem_amd as ff
