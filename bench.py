@@ -235,6 +235,7 @@ def served_stream(args):
             meshes[r["N"]] = ff.unit_box_mesh(r["N"])
     srv = ff.Server(budget_bytes=int(args.budget_gb * (1 << 30)), max_systems=args.slots, chunk=args.chunk)
     srv.set_precision(args.precision)
+    srv.use_graph(args.served_graph)
     probs = {}
     t0 = time.perf_counter()
     for r in stream:
@@ -330,6 +331,7 @@ def served_stream(args):
         "max_rel_err_vs_fp64": max(errs), "all_converged": conv, "poisson_equals_saturated": same,
         "iterations_served": iters_srv, "iterations_fp64": iters64, "prepare_s": round(t_prep, 2),
         "chunk": args.chunk, "slots": args.slots, "precision_policy": args.precision,
+        "hip_graph": bool(args.served_graph),
         "switched_to_fp64": srv.set_precision(-1),
         "served_s": round(t_sat, 3), "fp64_s": round(t_base, 3),
     }
@@ -500,6 +502,8 @@ def main():
     ap.add_argument("--orders", type=int, nargs="+", default=[1, 2, 3, 4, 5, 6],
                     help="Nedelec orders of the c3 workload")
     ap.add_argument("--requests", type=int, default=16, help="served: the first REQUESTS of the stream")
+    ap.add_argument("--served-graph", type=int, default=1,
+                    help="served: 1 (default) replays each step's chunk from a HIP graph, 0 launches one by one")
     ap.add_argument("--request-index", type=int, default=1,
                     help="served1: the request of the stream served alone")
     ap.add_argument("--stream", type=int, default=4096, help="served: length of the seeded stream")

@@ -1431,6 +1431,10 @@ class Server:
         self.ctx.check(code)
         return x, it, cv.astype(bool), rr
 
+    def use_graph(self, enable=True):
+        """Replay each step's chunk from a HIP graph (ffh_server_use_graph)."""
+        self.ctx.check(self._lib.ffh_server_use_graph(self.handle, 1 if enable else 0))
+
     def set_precision(self, policy=-1):
         """0 FP32 iterations (switch to FP64 on stagnation), 1 FP64, 2 FP64 for
         Nedelec; returns the number of requests switched so far."""

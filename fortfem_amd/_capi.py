@@ -48,13 +48,15 @@ EXPORTS = (
     "ffh_constrained_download", "ffh_set_rhs", "ffh_rhs_download", "ffh_problem_matrix",
     "ffh_problem_solve", "ffh_matrix_upload", "ffh_matrix_free", "ffh_matrix_download",
     "ffh_spmv", "ffh_solve_sparse", "ffh_bench_spmv", "ffh_set_profiling",
-    "ffh_last_solve_profile", "ffh_matrix_spmv_variant", "ffh_matrix_spmv_bytes", "ffh_pcg_launches_per_iteration", "ffh_nedelec_dof_map", "ffh_nedelec_basis",
+    "ffh_last_solve_profile", "ffh_matrix_spmv_variant", "ffh_matrix_spmv_bytes",
+    "ffh_pcg_launches_per_iteration", "ffh_nedelec_dof_map", "ffh_nedelec_basis",
     "ffh_nedelec_load_points", "ffh_spmv_jvp", "ffh_spmv_vjp", "ffh_solve_sparse_jvp",
     "ffh_solve_sparse_vjp", "ffh_assemble_jvp", "ffh_assemble_vjp", "ffh_nedelec_pml_values",
     "ffh_nedelec_pml_values_jvp", "ffh_nedelec_pml_values_vjp", "ffh_zmatrix_upload",
     "ffh_zmatrix_solve", "ffh_zmatrix_free",
     "ffh_server_create", "ffh_server_free", "ffh_server_prepare", "ffh_server_submit", "ffh_server_step",
-    "ffh_server_poll", "ffh_server_wait", "ffh_server_stats", "ffh_server_progress", "ffh_problem_solution", "ffh_server_set_precision",
+    "ffh_server_poll", "ffh_server_wait", "ffh_server_stats", "ffh_server_progress",
+    "ffh_problem_solution", "ffh_server_set_precision", "ffh_server_use_graph",
     "ffh_solve_sparse_device_noalloc", "ffh_noalloc_spmv_variant", "ffh_problem_colouring",
 )
 
@@ -173,6 +175,7 @@ def load() -> C.CDLL:
         "ffh_server_progress": (C.c_int, [vp, C.c_int64, vp, vp, vp]),
         "ffh_problem_solution": (C.c_int, [vp, vp]),
         "ffh_server_set_precision": (C.c_int, [vp, C.c_int, i64p]),
+        "ffh_server_use_graph": (C.c_int, [vp, C.c_int]),
         "ffh_solve_sparse_device_noalloc": (C.c_int, [vp, C.c_int, C.c_int64, vp, vp, vp, vp, vp,
                                                       C.POINTER(SolverOptions), C.POINTER(SolverStats)]),
         "ffh_noalloc_spmv_variant": (C.c_int, [C.c_int, C.c_int64]),

@@ -703,6 +703,7 @@ struct BatchGroup {
     std::vector<BatchSys> h_sys; // what d_sys / d_work hold
     std::vector<int2> h_work;
     int nwork = 0, nsys = 0;
+    int64_t generation = 0; // bumped whenever the tables change
     hipStream_t stream = nullptr;
     hipEvent_t done = nullptr;
 
@@ -714,6 +715,7 @@ struct BatchGroup {
             return;
         h_sys = sys;
         h_work = work;
+        ++generation;
         d_sys.upload(sys, s);
         d_work.upload(work, s);
         const size_t np = std::max<size_t>(1, work.size() * kMaxRhs);
@@ -766,9 +768,12 @@ struct BatchEngine {
     std::vector<BatchScal> h_scal;
     hipEvent_t start = nullptr;
     int nwork = 0;
+    hipGraphExec_t graph = nullptr;
+    int64_t graph_key = -1;
 
     ~BatchEngine()
     {
+        if (graph) (void)hipGraphExecDestroy(graph);
         for (BatchGroup &g : grp) {
             if (g.done) (void)hipEventDestroy(g.done);
             if (g.stream) (void)hipStreamDestroy(g.stream);
@@ -811,9 +816,10 @@ struct BatchEngine {
     // `chunk` PCG iterations of every system with an active right-hand side,
     // then the FP64 residual refresh; ordered after and before the work on
     // the context's stream
-    void run(Context *ctx, int chunk)
+    // one chunk (iterations + refresh) of every group, forked from and
+    // joined back to the context stream
+    void enqueue(Context *ctx, int chunk)
     {
-        if (nwork == 0) return;
         FFH_HIP(hipEventRecord(start, ctx->stream));
         for (BatchGroup &g : grp)
             if (g.nwork) FFH_HIP(hipStreamWaitEvent(g.stream, start, 0));
@@ -828,6 +834,35 @@ struct BatchEngine {
                 FFH_HIP(hipEventRecord(g.done, g.stream));
                 FFH_HIP(hipStreamWaitEvent(ctx->stream, g.done, 0));
             }
+    }
+
+    // The chunk, launched directly or replayed from a HIP graph captured on
+    // the context stream (re-captured when the batch's tables or the chunk
+    // length change: the graph holds their grid sizes and addresses).
+    void run(Context *ctx, int chunk, bool use_graph)
+    {
+        if (nwork == 0) return;
+        if (!use_graph) {
+            enqueue(ctx, chunk);
+            return;
+        }
+        int64_t key = chunk;
+        for (const BatchGroup &g : grp) key = key * 1000003 + g.generation * 31 + g.nwork;
+        if (!graph || graph_key != key) {
+            if (graph) {
+                (void)hipGraphExecDestroy(graph);
+                graph = nullptr;
+            }
+            hipGraph_t gr;
+            FFH_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+            enqueue(ctx, chunk);
+            FFH_HIP(hipStreamEndCapture(ctx->stream, &gr));
+            const hipError_t e = hipGraphInstantiate(&graph, gr, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(gr);
+            FFH_HIP(e);
+            graph_key = key;
+        }
+        FFH_HIP(hipGraphLaunch(graph, ctx->stream));
     }
 
     void pull(Context *ctx)
@@ -882,6 +917,7 @@ struct ffh_server {
     size_t budget = 0, used = 0;
     int max_sys = 64, chunk = 16;
     int policy = 0; // 0 FP32 iteration + switch on stagnation, 1 FP64, 2 FP64 for Nedelec
+    bool use_graph = false; // replay each chunk from a HIP graph
     int64_t switches = 0;
     std::map<ffh::ServerKey, std::unique_ptr<ffh::Problem>> cache;
     std::map<int64_t, std::unique_ptr<ffh::ServedRequest>> reqs;
@@ -1053,7 +1089,7 @@ static void rebuild_batch(ffh_server *S)
 static void run_chunk(ffh_server *S)
 {
     if (S->eng.nwork == 0 || S->active.empty()) return;
-    S->eng.run(S->ctx, S->chunk);
+    S->eng.run(S->ctx, S->chunk, S->use_graph);
     S->iterations_run += S->chunk;
 }
 
@@ -1351,6 +1387,13 @@ int ffh_server_stats(const ffh_server *S, int64_t *used_bytes, int64_t *iteratio
     if (!S) return FFH_ERR_HANDLE;
     if (used_bytes) *used_bytes = (int64_t)S->used;
     if (iterations_run) *iterations_run = S->iterations_run;
+    return FFH_OK;
+}
+
+int ffh_server_use_graph(ffh_server *S, int enable)
+{
+    if (!S) return FFH_ERR_HANDLE;
+    S->use_graph = enable != 0;
     return FFH_OK;
 }
 

@@ -398,6 +398,12 @@ int ffh_server_progress(const ffh_server *server, int64_t ticket, int *state, in
  * requests and 0 for H1.  policy = -1 only reads *switches (requests moved to
  * FP64 so far). */
 int ffh_server_set_precision(ffh_server *server, int policy, int64_t *switches);
+/* Replay each server step's chunk (the iterations and the FP64 refresh of
+ * every admitted request) from a HIP graph captured on the context stream,
+ * re-captured when the batch's composition or the chunk length changes
+ * (enable != 0), or launch the kernels one by one (0, the default).  The
+ * results are the same either way.  No reference counterpart. */
+int ffh_server_use_graph(ffh_server *server, int enable);
 
 /* ---- differentiable variants --------------------------------------------- */
 /* spmv_jvp (src/solvers/fortfem_sparse_matrix.f90:60-91): y_dot = A x_dot +
