@@ -172,6 +172,10 @@ struct DevMatrix {
     // tile_off[g]), and per SELL entry its column's position in that list
     // (16 bits), so a block stages each column once in LDS.
     bool tiles_built = false, tiles_ok = false;
+    // row-major index of the pattern for the deterministic CSC tangent
+    // product (built on first use)
+    bool tr_built = false;
+    DBuf<int> tr_ptr, tr_col, tr_pos;
     int tile_max = 0; // most distinct columns of one group
     int64_t tile_total = 0; // entries of tile_cols
     DBuf<int> tile_off, tile_cols;

@@ -1037,6 +1037,7 @@ void build_rowblocks(Context *ctx, DevMatrix &A, const std::vector<int> &ptr)
     A.sell_built = A.sell_ok = false;
     A.tiles_built = A.tiles_ok = false;
     A.sell64_valid = A.sell32_valid = false;
+    A.tr_built = false;
     FFH_HIP(hipStreamSynchronize(ctx->stream));
 }
 
@@ -2047,23 +2048,27 @@ namespace ffh {
 //   vjp: val_bar(k) = scale y_bar(row) x(j) ; x_bar(j) = sum_k val(k) y_bar(row)
 // The jvp scatters into rows with FP64 atomics (summation order free, as
 // for any sum); the vjp gathers per column (deterministic).
-__global__ __launch_bounds__(kBlock) void k_csc_jvp(int ncol, const int *__restrict__ ptr,
-                                                    const int *__restrict__ idx,
+// y = A xdot + Adot x row by row through the transposed index (row i's
+// entries in ascending column order: column tcol[k], CSC position tpos[k]),
+// so every y_i is summed in one fixed order.
+__global__ __launch_bounds__(kBlock) void k_csc_jvp(int nrow, const int *__restrict__ tptr,
+                                                    const int *__restrict__ tcol,
+                                                    const int *__restrict__ tpos,
                                                     const double *__restrict__ val,
                                                     const double *__restrict__ vdot,
                                                     const double *__restrict__ x,
                                                     const double *__restrict__ xdot,
                                                     double *__restrict__ y)
 {
-    const int j = blockIdx.x * kBlock + threadIdx.x;
-    if (j >= ncol) return;
-    const double xj = x ? x[j] : 0.0, dj = xdot ? xdot[j] : 0.0;
-    for (int k = ptr[j]; k < ptr[j + 1]; ++k) {
-        double t = 0.0;
-        if (val && xdot) t += val[k] * dj;
-        if (vdot && x) t += vdot[k] * xj;
-        atomicAdd(&y[idx[k]], t);
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= nrow) return;
+    double s = 0.0;
+    for (int k = tptr[i]; k < tptr[i + 1]; ++k) {
+        const int j = tcol[k], p = tpos[k];
+        if (val && xdot) s += val[p] * xdot[j];
+        if (vdot && x) s += vdot[p] * x[j];
     }
+    y[i] = s;
 }
 
 __global__ __launch_bounds__(kBlock) void k_csc_vjp(int ncol, const int *__restrict__ ptr,
@@ -2086,13 +2091,46 @@ __global__ __launch_bounds__(kBlock) void k_csc_vjp(int ncol, const int *__restr
     if (xbar) xbar[j] = s;
 }
 
-void csc_jvp(Context *ctx, const DevMatrix &A, const double *vdot, const double *x,
+// The row-major (transposed) index of a CSC pattern, built on the host once
+// per pattern: row pointers, and per entry its column and CSC position.
+static void ensure_transpose(Context *ctx, DevMatrix &A)
+{
+    if (A.tr_built) return;
+    hipStream_t s = ctx->stream;
+    const int n = A.n;
+    const int64_t nnz = A.nnz;
+    std::vector<int> cp((size_t)n + 1), ri((size_t)nnz);
+    FFH_HIP(hipMemcpyAsync(cp.data(), A.col_ptr.p, sizeof(int) * cp.size(), hipMemcpyDeviceToHost, s));
+    FFH_HIP(hipMemcpyAsync(ri.data(), A.row_idx.p, sizeof(int) * ri.size(), hipMemcpyDeviceToHost, s));
+    FFH_HIP(hipStreamSynchronize(s));
+    std::vector<int> tp((size_t)n + 1, 0), tc((size_t)nnz), tq((size_t)nnz);
+    for (int64_t k = 0; k < nnz; ++k) {
+        FFH_REQUIRE(ri[k] >= 0 && ri[k] < n, FFH_ERR_INVALID, "row index out of range");
+        ++tp[(size_t)ri[k] + 1];
+    }
+    for (int i = 0; i < n; ++i) tp[i + 1] += tp[i];
+    std::vector<int> next(tp.begin(), tp.end() - 1);
+    for (int j = 0; j < n; ++j)
+        for (int k = cp[j]; k < cp[j + 1]; ++k) {
+            const int at = next[ri[k]]++;
+            tc[at] = j;
+            tq[at] = k;
+        }
+    A.tr_ptr.upload(tp, s);
+    A.tr_col.upload(tc, s);
+    A.tr_pos.upload(tq, s);
+    FFH_HIP(hipStreamSynchronize(s));
+    A.tr_built = true;
+}
+
+void csc_jvp(Context *ctx, DevMatrix &A, const double *vdot, const double *x,
              const double *xdot, double *y)
 {
     const int n = A.n;
-    FFH_HIP(hipMemsetAsync(y, 0, sizeof(double) * n, ctx->stream));
+    ensure_transpose(ctx, A);
     hipLaunchKernelGGL(k_csc_jvp, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, ctx->stream, n,
-                       A.col_ptr.p, A.row_idx.p, xdot ? A.val.p : nullptr, vdot, x, xdot, y);
+                       A.tr_ptr.p, A.tr_col.p, A.tr_pos.p, xdot ? A.val.p : nullptr, vdot, x, xdot,
+                       y);
     FFH_HIP(hipGetLastError());
 }
 

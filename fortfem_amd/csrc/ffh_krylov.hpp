@@ -83,7 +83,7 @@ namespace ffh {
 // y = A x_dot + A_dot x (either term may be absent), and
 // val_bar = scale (y_bar_row x_col) per entry, x_bar = A^T y_bar (either
 // output may be absent)
-void csc_jvp(Context *ctx, const DevMatrix &A, const double *vdot, const double *x,
+void csc_jvp(Context *ctx, DevMatrix &A, const double *vdot, const double *x,
              const double *xdot, double *y);
 void csc_vjp(Context *ctx, const DevMatrix &A, const double *x, const double *ybar, double scale,
              double *vbar, double *xbar);

@@ -894,7 +894,7 @@ __global__ __launch_bounds__(256) void k_ned_vjp(
     const int *__restrict__ cells_c, const double *__restrict__ verts,
     const int *__restrict__ perm, const double *__restrict__ alpha_cell, double alpha,
     double beta, const double *__restrict__ tab, const int *__restrict__ slots_cm,
-    const double *__restrict__ vbar, double *__restrict__ verts_bar, double *__restrict__ ab,
+    const double *__restrict__ vbar, double *__restrict__ cell_vbar, double *__restrict__ ab,
     double *__restrict__ bb, int *__restrict__ err)
 {
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -961,11 +961,12 @@ __global__ __launch_bounds__(256) void k_ned_vjp(
         for (int r = 0; r < 3; ++r)
             jb[r + 3 * j] += -(w[r] * inv[j] + w[r + 3] * inv[j + 3] + w[r + 6] * inv[j + 6]) +
                              det_bar * det * inv[j + 3 * r];
-    const int id[4] = {c.x, c.y, c.z, c.w};
+    // the cell's four vertex cotangents (summed per vertex in cell order by
+    // the host, so the result does not depend on the colouring or timing)
+    double *o = cell_vbar + (size_t)pos * 12;
     for (int r = 0; r < 3; ++r) {
-        atomicAdd(&verts_bar[3 * (size_t)id[0] + r], -(jb[r] + jb[r + 3] + jb[r + 6]));
-        for (int q = 0; q < 3; ++q)
-            atomicAdd(&verts_bar[3 * (size_t)id[q + 1] + r], jb[r + 3 * q]);
+        o[r] = -(jb[r] + jb[r + 3] + jb[r + 6]);
+        for (int q = 0; q < 3; ++q) o[3 * (q + 1) + r] = jb[r + 3 * q];
     }
 }
 
@@ -1935,8 +1936,7 @@ static void nedelec_vjp(Problem *P, int coeff_kind, const double *coeff, double 
     const int64_t nnz = P->full.m.nnz;
     DBuf<double> vb, xb, ab, bb;
     vb.upload(values_bar, (size_t)nnz, s);
-    xb.alloc(3 * (size_t)P->nvert);
-    FFH_HIP(hipMemsetAsync(xb.p, 0, sizeof(double) * 3 * (size_t)P->nvert, s));
+    xb.alloc(12 * (size_t)std::max(T, 1)); // per cell: 4 vertices x 3
     ab.alloc(std::max(T, 1));
     bb.alloc(std::max(T, 1));
     for (int c = 0; c < P->n_colors; ++c) {
@@ -1949,15 +1949,26 @@ static void nedelec_vjp(Problem *P, int coeff_kind, const double *coeff, double 
     }
     FFH_HIP(hipGetLastError());
     check_err(P, "Nedelec assembly cotangent");
-    std::vector<double> ha(T), hb(T);
+    std::vector<double> ha(T), hb(T), hx(12 * (size_t)T);
     ab.download(ha.data(), T, s);
     bb.download(hb.data(), T, s);
-    xb.download(vertices_bar, 3 * (size_t)P->nvert, s);
+    xb.download(hx.data(), 12 * (size_t)T, s);
     FFH_HIP(hipStreamSynchronize(s));
     std::vector<double> ca(T), cb(T);
+    std::vector<int> pos_of(T);
     for (int pos = 0; pos < T; ++pos) {
         ca[P->perm[pos]] = ha[pos];
         cb[P->perm[pos]] = hb[pos];
+        pos_of[P->perm[pos]] = pos;
+    }
+    // vertex cotangents summed over the cells in ascending cell order
+    std::fill(vertices_bar, vertices_bar + 3 * (size_t)P->nvert, 0.0);
+    for (int e = 0; e < T; ++e) {
+        const double *c = hx.data() + (size_t)pos_of[e] * 12;
+        for (int q = 0; q < 4; ++q) {
+            double *vbq = vertices_bar + 3 * (size_t)(P->cells[(size_t)4 * e + q] - 1);
+            for (int r = 0; r < 3; ++r) vbq[r] += c[3 * q + r];
+        }
     }
     double asum = 0.0, bsum = 0.0;
     for (int e = 0; e < T; ++e) {

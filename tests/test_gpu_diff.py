@@ -5,8 +5,9 @@ sparse pcg/cg solve tangents and adjoints
 (src/solvers/advanced_solvers.f90:387-522) through the C-ABI
 (ffh_spmv_jvp, ffh_spmv_vjp, ffh_solve_sparse_jvp, ffh_solve_sparse_vjp).
 
-Bars: the products sum in a different order from the oracle's column loop
-(atomic row sums for the JVP), so they agree to 1e-13 relative; the
+Bars: the products round differently from the oracle's column loop (the
+JVP sums each row in ascending column order with fused multiply-adds), so
+they agree to 1e-13 relative and repeat bit for bit from call to call; the
 cotangent values are one product each and agree exactly.  The solve
 tangents are converged Krylov solves, compared with the oracle's at
 tolerance 1e-12 to within 1e-8 relative (FP64 and mixed), and checked with
@@ -113,7 +114,9 @@ def test_spmv_ad_matches_oracle(ff, kind):
     x, xd, yb = rng.uniform(-1, 1, (3, a.n_cols))
     vd = rng.uniform(-1, 1, a.nnz)
     m = upload(ff, a)
-    assert rel(m.spmv_jvp(x, vd, xd), O.spmv_jvp(a, x, vd, xd)) <= 1e-13
+    yd = m.spmv_jvp(x, vd, xd)
+    assert rel(yd, O.spmv_jvp(a, x, vd, xd)) <= 1e-13
+    assert np.array_equal(yd, m.spmv_jvp(x, vd, xd))  # deterministic: no atomics
     vb, xb = m.spmv_vjp(x, yb)
     vbo, xbo = O.spmv_vjp(a, x, yb)
     assert np.array_equal(vb, vbo)

@@ -81,6 +81,9 @@ def test_jvp_vjp_match_oracle(ff, k):
     ovb, oab, obb, _ = O.nedelec_csc_vjp(k, v, t, vb_in, a, b)
     gvb, gab, gbb = p.assemble_vjp(-1, vb_in, kappa=a, mass=b)
     assert relmax(gvb, ovb) <= 1e-12
+    # vertex cotangents are summed per vertex in cell order: bit-for-bit repeatable
+    gvb2, _, _ = p.assemble_vjp(-1, vb_in, kappa=a, mass=b)
+    assert np.array_equal(gvb, gvb2)
     assert abs(gab - oab) <= 1e-12 * max(1.0, abs(oab))
     assert abs(gbb - obb) <= 1e-12 * max(1.0, abs(obb))
 
