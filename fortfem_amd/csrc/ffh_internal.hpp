@@ -102,6 +102,11 @@ struct DBuf {
     {
         upload(h.data(), h.size(), s);
     }
+    void swap(DBuf &o)
+    {
+        std::swap(p, o.p);
+        std::swap(n, o.n);
+    }
     void download(T *h, size_t count, hipStream_t s) const
     {
         if (count)
@@ -350,8 +355,9 @@ void build_slots(int ncell, int nloc, const std::vector<int> &gd,
                  const std::vector<int> &perm, const std::vector<int> &col_ptr,
                  const std::vector<int> &row_idx, std::vector<int> &slots);
 // device symbolic phase (ffh_symbolic.hip); tets 1-based on the device
+// (eid_out, when given, receives the first-appearance edge numbers, T x 6)
 void device_nedelec_dofs(Context *ctx, int k, int64_t T, const int *d_tets, std::vector<int> &gd,
-                         int &ndofs, int &nedges, int &nfaces);
+                         int &ndofs, int &nedges, int &nfaces, DBuf<int> *eid_out = nullptr);
 void device_lagrange_dofs(Context *ctx, int d, int64_t T, const int *d_tets,
                           const std::vector<int> &bary, std::vector<int> &gd, int &count);
 int64_t device_pattern_bytes(int64_t T, int nl);
@@ -362,6 +368,12 @@ void device_slots_h1(Context *ctx, int64_t T, int nl, const int *d_gdc, const DB
 void device_slots_cm(Context *ctx, int64_t T, int nl, const int *d_gdc, const DBuf<int> &col_ptr,
                      const DBuf<int> &row_idx, int64_t nnz, const std::vector<int> &cstart,
                      DBuf<int> &slots);
+// colouring of T cells on the device from their incidence lists (T x m
+// entity ids, starting at base; entities 0 .. n_ent - 1): cells colour by
+// colour in perm (ascending index inside a colour), offsets in cstart; false
+// when a cell would need more than 256 colours
+bool device_colour(Context *ctx, int64_t T, int m, const int *d_inc, int base, int n_ent,
+                   std::vector<int> &cstart, std::vector<int> &perm);
 void lagrange_boundary(int d, int ntet, const std::vector<unsigned char> &bf,
                        const std::vector<int> &gd, int n_global,
                        std::vector<int> &constrained);

@@ -1385,6 +1385,10 @@ void problem_pattern(Problem *P, int family, int order)
     std::vector<int> gd;
     int G = 0;
     int nl = 0;
+    // incidence lists of the device colouring (vertices, or Nedelec edges)
+    DBuf<int> d_inc;
+    int inc_m = 0, inc_base = 0, n_ent = 0;
+    bool dev = false;
     if (family == FFH_FAMILY_H1) {
         FFH_REQUIRE(P->dim == 3, FFH_ERR_INVALID, "H1 family needs a tetrahedral mesh");
         FFH_REQUIRE(order >= 1 && order <= 4, FFH_ERR_INVALID, "H1 degree must be 1..4");
@@ -1392,15 +1396,18 @@ void problem_pattern(Problem *P, int family, int order)
             FFH_REQUIRE(P->cells[i] >= 1 && P->cells[i] <= P->nvert, FFH_ERR_INVALID,
                         "cell vertex id out of range");
         nl = lagrange_dof_count(order);
-        if (device_symbolic_fits(T, nl)) {
+        dev = device_symbolic_fits(T, nl);
+        if (dev) {
             FFH_REQUIRE(distinct_cell_vertices(T, P->cells.data()), FFH_ERR_INVALID,
                         "invalid tetrahedra for the dof map");
             std::vector<int> bary;
             std::vector<double> nodes;
             lagrange_indices(order, bary, nodes);
-            DBuf<int> dt;
-            dt.upload(P->cells, ctx->stream);
-            device_lagrange_dofs(ctx, order, T, dt.p, bary, gd, G);
+            d_inc.upload(P->cells, ctx->stream);
+            device_lagrange_dofs(ctx, order, T, d_inc.p, bary, gd, G);
+            inc_m = 4;
+            inc_base = 1;
+            n_ent = P->nvert;
         } else {
             FFH_REQUIRE(lagrange_dof_map(order, T, P->cells.data(), gd, G) == 0,
                         FFH_ERR_INVALID, "invalid tetrahedra for the dof map");
@@ -1412,13 +1419,17 @@ void problem_pattern(Problem *P, int family, int order)
             FFH_REQUIRE(P->cells[i] >= 1 && P->cells[i] <= P->nvert, FFH_ERR_INVALID,
                         "cell vertex id out of range");
         nl = nedelec_dof_count(order);
-        if (device_symbolic_fits(T, nl)) {
+        dev = device_symbolic_fits(T, nl);
+        if (dev) {
             FFH_REQUIRE(distinct_cell_vertices(T, P->cells.data()), FFH_ERR_INVALID,
                         "invalid tetrahedra for the Nedelec dof map");
             nedelec_cell_classes(T, P->cells.data(), P->cell_class);
             DBuf<int> dt;
             dt.upload(P->cells, ctx->stream);
-            device_nedelec_dofs(ctx, order, T, dt.p, gd, G, P->n_edges, P->n_faces);
+            device_nedelec_dofs(ctx, order, T, dt.p, gd, G, P->n_edges, P->n_faces, &d_inc);
+            inc_m = 6;
+            inc_base = 0;
+            n_ent = P->n_edges;
         } else {
             FFH_REQUIRE(nedelec_dof_map(order, T, P->cells.data(), gd, P->cell_class, G,
                                         P->n_edges, P->n_faces) == 0,
@@ -1434,6 +1445,13 @@ void problem_pattern(Problem *P, int family, int order)
             gd[i] = P->cells[i] - 1;
         }
         G = P->nvert; // dof = vertex index (mesh_2d)
+        dev = device_symbolic_fits(T, nl);
+        if (dev) {
+            d_inc.upload(P->cells, ctx->stream);
+            inc_m = 3;
+            inc_base = 1;
+            n_ent = P->nvert;
+        }
     } else {
         throw Error{FFH_ERR_INVALID, "unsupported element family"};
     }
@@ -1442,7 +1460,13 @@ void problem_pattern(Problem *P, int family, int order)
     P->n_local = nl;
     P->n_global = G;
     std::vector<int> cstart, perm;
-    P->n_colors = color_cells(G, T, nl, gd, cstart, perm);
+    // colouring on the device (Jones-Plassmann, first fit), or the host's
+    // greedy first fit in cell order for meshes past the device budget
+    if (dev && device_colour(ctx, T, inc_m, d_inc.p, inc_base, n_ent, cstart, perm))
+        P->n_colors = (int)cstart.size() - 1;
+    else
+        P->n_colors = color_cells(G, T, nl, gd, cstart, perm);
+    d_inc.reset();
     std::vector<unsigned char> cls_c;
     if (family == FFH_FAMILY_NEDELEC) {
         // inside a colour, cells sorted by orientation class

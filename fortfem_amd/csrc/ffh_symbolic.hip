@@ -389,7 +389,7 @@ static void edge_face_numbers(Context *ctx, int64_t T, const int *d_tets, DBuf<i
 }
 
 void device_nedelec_dofs(Context *ctx, int k, int64_t T, const int *d_tets, std::vector<int> &gd,
-                         int &ndofs, int &nedges, int &nfaces)
+                         int &ndofs, int &nedges, int &nfaces, DBuf<int> *eid_out)
 {
     hipStream_t s = ctx->stream;
     DBuf<int> eid, fid, dgd;
@@ -406,6 +406,7 @@ void device_nedelec_dofs(Context *ctx, int k, int64_t T, const int *d_tets, std:
     gd.resize((size_t)n * T);
     dgd.download(gd.data(), gd.size(), s);
     FFH_HIP(hipStreamSynchronize(s));
+    if (eid_out) eid_out->swap(eid);
 }
 
 void device_lagrange_dofs(Context *ctx, int d, int64_t T, const int *d_tets,
@@ -501,6 +502,275 @@ void device_slots_cm(Context *ctx, int64_t T, int nl, const int *d_gdc, const DB
                        first.p, slots.p);
     FFH_HIP(hipGetLastError());
     FFH_HIP(hipStreamSynchronize(s)); // dcs and first are freed on return
+}
+
+// ------------------------------------------------------------ colouring --
+// Cell colouring on the device.  Two cells conflict when they share an
+// entity of the incidence lists: vertices for the Lagrange and 2D families
+// (every dof they can share lies on a shared vertex, edge or face, whose
+// vertices are then shared too) and edges for Nedelec (its shared dofs lie on
+// shared edges or faces).  No two cells of one colour conflict, so each
+// colour scatters into the CSC values without atomics.
+//
+// Jones-Plassmann rounds with first-fit colours (M. T. Jones and P. E.
+// Plassmann, "A parallel graph coloring heuristic", SIAM J. Sci. Comput. 14,
+// 1993): an uncoloured cell takes the smallest colour its coloured
+// neighbours do not hold once every uncoloured neighbour has a lower
+// priority.  Priorities are a 32-bit integer hash of the cell index with the
+// index as tie-break, so the colouring is the same on every run.  A cell
+// coloured in round k stores (k << 16) | colour, and the cells of round k
+// read such an entry as unset: each round sees the colours of the rounds
+// before it only, whatever the order the threads run in.
+namespace {
+
+constexpr int kMaxColours = 256;
+constexpr int kColourRoundsCap = 16384;
+
+// lowbias32 integer hash (C. Wellons, "Prospecting for hash functions",
+// 2018) of the cell index, the index in the low word
+__device__ __forceinline__ uint64_t jp_priority(uint32_t e)
+{
+    uint32_t x = e;
+    x ^= x >> 16;
+    x *= 0x7feb352du;
+    x ^= x >> 15;
+    x *= 0x846ca68bu;
+    x ^= x >> 16;
+    return ((uint64_t)x << 32) | e;
+}
+
+__global__ __launch_bounds__(kSymBlock) void k_inc_count(int64_t n, const int *__restrict__ inc,
+                                                         int base, int *__restrict__ cnt)
+{
+    for (int64_t t = blockIdx.x * (int64_t)kSymBlock + threadIdx.x; t < n;
+         t += (int64_t)gridDim.x * kSymBlock)
+        atomicAdd(cnt + (inc[t] - base), 1);
+}
+
+// entity -> cells lists (order inside a list is immaterial to the rounds)
+__global__ __launch_bounds__(kSymBlock) void k_inc_fill(int64_t n, int m,
+                                                        const int *__restrict__ inc, int base,
+                                                        const int *__restrict__ ptr,
+                                                        int *__restrict__ cur,
+                                                        int *__restrict__ list)
+{
+    for (int64_t t = blockIdx.x * (int64_t)kSymBlock + threadIdx.x; t < n;
+         t += (int64_t)gridDim.x * kSymBlock) {
+        const int ent = inc[t] - base;
+        const int q = atomicAdd(cur + ent, 1);
+        list[ptr[ent] + q] = (int)(t / m);
+    }
+}
+
+// One round, one thread per cell.  stat[0] += cells coloured, stat[1] =
+// max(stat[1], colour), stat[2] = 1 if a cell found all kMaxColours colours
+// taken.  A cell reads its neighbours eight at a time (list entries, then
+// their colours), so each thread keeps several loads in flight.
+constexpr int kMaxInc = 6; // entities per cell (tetrahedron edges)
+
+__global__ __launch_bounds__(kSymBlock) void k_jp_round(int64_t T, int m,
+                                                        const int *__restrict__ inc, int base,
+                                                        const int *__restrict__ ptr,
+                                                        const int *__restrict__ list,
+                                                        int *__restrict__ col, int round,
+                                                        int *__restrict__ stat)
+{
+    const int64_t e = blockIdx.x * (int64_t)kSymBlock + threadIdx.x;
+    int done = 0, cmax = -1, over = 0;
+    if (e < T && col[e] < 0) {
+        const uint64_t ke = jp_priority((uint32_t)e);
+        int q0[kMaxInc], q1[kMaxInc];
+#pragma unroll
+        for (int k = 0; k < kMaxInc; ++k)
+            if (k < m) {
+                const int ent = inc[e * m + k] - base;
+                q0[k] = ptr[ent];
+                q1[k] = ptr[ent + 1];
+            }
+        uint64_t m0 = 0, m1 = 0, m2 = 0, m3 = 0;
+        bool ready = true;
+#pragma unroll
+        for (int k = 0; k < kMaxInc; ++k) {
+            if (k >= m || !ready) continue;
+            for (int q = q0[k]; q < q1[k] && ready; q += 8) {
+                int f[8], cf[8];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) f[u] = q + u < q1[k] ? list[q + u] : -1;
+#pragma unroll
+                for (int u = 0; u < 8; ++u) cf[u] = f[u] >= 0 ? col[f[u]] : 0;
+#pragma unroll
+                for (int u = 0; u < 8; ++u) {
+                    if (f[u] < 0 || f[u] == (int)e) continue;
+                    if (cf[u] < 0 || (cf[u] >> 16) == round) {
+                        if (jp_priority((uint32_t)f[u]) > ke) ready = false;
+                    } else {
+                        const int c = cf[u] & 0xffff; // < kMaxColours
+                        const uint64_t bit = 1ull << (c & 63);
+                        const int w = c >> 6;
+                        m0 |= w == 0 ? bit : 0ull;
+                        m1 |= w == 1 ? bit : 0ull;
+                        m2 |= w == 2 ? bit : 0ull;
+                        m3 |= w == 3 ? bit : 0ull;
+                    }
+                }
+            }
+        }
+        if (ready) {
+            int c = -1;
+            if (~m0) c = __builtin_ctzll(~m0);
+            else if (~m1) c = 64 + __builtin_ctzll(~m1);
+            else if (~m2) c = 128 + __builtin_ctzll(~m2);
+            else if (~m3) c = 192 + __builtin_ctzll(~m3);
+            if (c < 0) {
+                over = 1;
+            } else {
+                col[e] = (round << 16) | c;
+                done = 1;
+                cmax = c;
+            }
+        }
+    }
+    __shared__ int s_done[kSymBlock / 64], s_max[kSymBlock / 64], s_over[kSymBlock / 64];
+    for (int o = 32; o > 0; o >>= 1) {
+        done += __shfl_down(done, o, 64);
+        cmax = max(cmax, __shfl_down(cmax, o, 64));
+        over |= __shfl_down(over, o, 64);
+    }
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    if (lane == 0) {
+        s_done[wv] = done;
+        s_max[wv] = cmax;
+        s_over[wv] = over;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int d = 0, mx = -1, ov = 0;
+        for (int w = 0; w < kSymBlock / 64; ++w) {
+            d += s_done[w];
+            mx = max(mx, s_max[w]);
+            ov |= s_over[w];
+        }
+        if (d) atomicAdd(stat, d);
+        if (mx >= 0) atomicMax(stat + 1, mx);
+        if (ov) atomicOr(stat + 2, 1);
+    }
+}
+
+__global__ __launch_bounds__(kSymBlock) void k_colour_keys(int64_t T, const int *__restrict__ col,
+                                                           unsigned *__restrict__ key)
+{
+    for (int64_t t = blockIdx.x * (int64_t)kSymBlock + threadIdx.x; t < T;
+         t += (int64_t)gridDim.x * kSymBlock)
+        key[t] = (unsigned)(col[t] & 0xffff);
+}
+
+// cstart[c] = first position of colour c in the sorted keys (colours are
+// 0 .. C-1 without gaps: first fit takes colour c only next to a c - 1)
+__global__ __launch_bounds__(kSymBlock) void k_colour_starts(int64_t T,
+                                                             const unsigned *__restrict__ sk,
+                                                             int *__restrict__ cstart)
+{
+    for (int64_t t = blockIdx.x * (int64_t)kSymBlock + threadIdx.x; t < T;
+         t += (int64_t)gridDim.x * kSymBlock)
+        if (t == 0 || sk[t] != sk[t - 1]) cstart[sk[t]] = (int)t;
+}
+
+} // namespace
+
+// Colours of T cells from their incidence lists d_inc (T x m entity ids,
+// base-based, entities 0 .. n_ent - 1): perm lists the cells colour by
+// colour (ascending cell index inside a colour), cstart the colour offsets.
+// False (nothing returned) when a cell would need more than kMaxColours
+// colours.
+bool device_colour(Context *ctx, int64_t T, int m, const int *d_inc, int base, int n_ent,
+                   std::vector<int> &cstart, std::vector<int> &perm)
+{
+    hipStream_t s = ctx->stream;
+    if (T == 0) {
+        cstart.assign(1, 0);
+        perm.clear();
+        return true;
+    }
+    const int64_t n = T * m;
+    DBuf<int> cnt, ptr, cur, list, col, stat;
+    cnt.alloc(n_ent + 1);
+    ptr.alloc(n_ent + 1);
+    cur.alloc(n_ent);
+    list.alloc(n);
+    col.alloc(T);
+    stat.alloc(3 * (size_t)kColourRoundsCap);
+    hipLaunchKernelGGL(k_fill_int, dim3(sym_grid(n_ent + 1)), dim3(kSymBlock), 0, s,
+                       (int64_t)n_ent + 1, 0, cnt.p);
+    hipLaunchKernelGGL(k_fill_int, dim3(sym_grid(n_ent)), dim3(kSymBlock), 0, s, (int64_t)n_ent,
+                       0, cur.p);
+    hipLaunchKernelGGL(k_fill_int, dim3(sym_grid(T)), dim3(kSymBlock), 0, s, T, -1, col.p);
+    hipLaunchKernelGGL(k_fill_int, dim3(sym_grid(3 * kColourRoundsCap)), dim3(kSymBlock), 0, s,
+                       (int64_t)3 * kColourRoundsCap, 0, stat.p);
+    hipLaunchKernelGGL(k_inc_count, dim3(sym_grid(n)), dim3(kSymBlock), 0, s, n, d_inc, base,
+                       cnt.p);
+    size_t bs = 0;
+    FFH_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, bs, cnt.p, ptr.p, n_ent + 1, s));
+    {
+        DBuf<unsigned char> tmp;
+        tmp.alloc(std::max<size_t>(bs, 1));
+        FFH_HIP(hipcub::DeviceScan::ExclusiveSum(tmp.p, bs, cnt.p, ptr.p, n_ent + 1, s));
+    }
+    hipLaunchKernelGGL(k_inc_fill, dim3(sym_grid(n)), dim3(kSymBlock), 0, s, n, m, d_inc, base,
+                       ptr.p, cur.p, list.p);
+    FFH_HIP(hipGetLastError());
+    // rounds in batches of 8 between checks of the coloured count
+    FFH_REQUIRE(m >= 1 && m <= kMaxInc, FFH_ERR_INVALID, "colouring: 1 to 6 entities per cell");
+    FFH_REQUIRE((T + kSymBlock - 1) / kSymBlock <= INT_MAX, FFH_ERR_INVALID, "colouring: too many cells");
+    const int grid = (int)((T + kSymBlock - 1) / kSymBlock);
+    int64_t coloured = 0;
+    int ncol = 0, round = 0;
+    std::vector<int> h(3 * 8);
+    while (coloured < T) {
+        FFH_REQUIRE(round + 8 <= kColourRoundsCap, FFH_ERR_DEVICE, "colouring did not finish");
+        for (int r = 0; r < 8; ++r)
+            hipLaunchKernelGGL(k_jp_round, dim3(grid), dim3(kSymBlock), 0, s, T, m, d_inc, base,
+                               ptr.p, list.p, col.p, round + r, stat.p + 3 * (round + r));
+        FFH_HIP(hipGetLastError());
+        FFH_HIP(hipMemcpyAsync(h.data(), stat.p + 3 * round, sizeof(int) * h.size(),
+                               hipMemcpyDeviceToHost, s));
+        FFH_HIP(hipStreamSynchronize(s));
+        for (int r = 0; r < 8; ++r) {
+            if (h[3 * r + 2]) return false;
+            coloured += h[3 * r];
+            ncol = std::max(ncol, h[3 * r + 1] + 1);
+        }
+        round += 8;
+    }
+    // cells sorted by colour (stable: ascending index inside a colour)
+    DBuf<unsigned> key, skey;
+    DBuf<int> idx, sidx, dstart;
+    key.alloc(T);
+    skey.alloc(T);
+    idx.alloc(T);
+    sidx.alloc(T);
+    dstart.alloc(ncol);
+    hipLaunchKernelGGL(k_colour_keys, dim3(sym_grid(T)), dim3(kSymBlock), 0, s, T, col.p, key.p);
+    hipLaunchKernelGGL(k_iota, dim3(sym_grid(T)), dim3(kSymBlock), 0, s, T, idx.p);
+    const int end_bit = bits_for(ncol);
+    bs = 0;
+    FFH_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, bs, key.p, skey.p, idx.p, sidx.p, T, 0,
+                                               end_bit, s));
+    {
+        DBuf<unsigned char> tmp;
+        tmp.alloc(std::max<size_t>(bs, 1));
+        FFH_HIP(hipcub::DeviceRadixSort::SortPairs(tmp.p, bs, key.p, skey.p, idx.p, sidx.p, T, 0,
+                                                   end_bit, s));
+        hipLaunchKernelGGL(k_colour_starts, dim3(sym_grid(T)), dim3(kSymBlock), 0, s, T, skey.p,
+                           dstart.p);
+        FFH_HIP(hipGetLastError());
+        perm.resize(T);
+        cstart.resize(ncol + 1);
+        sidx.download(perm.data(), T, s);
+        dstart.download(cstart.data(), ncol, s);
+        FFH_HIP(hipStreamSynchronize(s));
+    }
+    cstart[ncol] = (int)T;
+    return true;
 }
 
 } // namespace ffh

@@ -83,3 +83,51 @@ def test_nedelec_csc_on_scrambled_mesh(ff, k):
     o = O.nedelec_csc(k, v, t, alpha=0.9, beta=1.4)
     assert np.array_equal(a.col_ptr, o.col_ptr) and np.array_equal(a.row_idx, o.row_idx)
     assert np.max(np.abs(a.val - o.val)) <= 1e-12 * np.max(np.abs(o.val))
+
+
+def _neighbour_degree(inc):
+    """Largest number of distinct other cells sharing an entity with one cell
+    (incidence rows of entity ids)."""
+    T = inc.shape[0]
+    ent = inc.reshape(-1)
+    cell = np.repeat(np.arange(T), inc.shape[1])
+    order = np.argsort(ent, kind="stable")
+    ent_s, cell_s = ent[order], cell[order]
+    starts = np.flatnonzero(np.r_[True, ent_s[1:] != ent_s[:-1]])
+    ends = np.r_[starts[1:], ent_s.shape[0]]
+    nb = [set() for _ in range(T)]
+    for a, b in zip(starts, ends):
+        cs = cell_s[a:b]
+        for c in cs:
+            nb[c].update(cs)
+    return max(len(s) - 1 for s in nb)
+
+
+@pytest.mark.parametrize("family,order", [(1, 1), (1, 3), (2, 1), (2, 3)])
+def test_device_colouring_properties(ff, family, order):
+    """The device colouring (Jones-Plassmann rounds, first-fit colours) on a
+    scrambled mesh: a proper colouring of the dof-sharing graph (the
+    validator), cells ascending inside each colour, the same colouring from a
+    second pattern build, and no more colours than the first-fit bound of
+    one plus the largest number of cells a cell conflicts with (vertex
+    sharing for H1, edge sharing for Nedelec).  Nedelec cells are further
+    grouped by orientation class inside a colour (ascending inside each
+    class), so a colour's list falls at most 23 times there."""
+    v, t = scrambled(5, 40 + 10 * family + order)
+    p = ff.Problem(v, t)
+    p.pattern_build(family, order)
+    nc = assert_valid_colouring(p)
+    cs, cells = p.colouring(base=0)
+    for c in range(nc):
+        falls = int(np.sum(np.diff(cells[cs[c]:cs[c + 1]]) < 0))
+        assert falls == 0 if family == 1 else falls <= 23
+    q = ff.Problem(v, t)
+    q.pattern_build(family, order)
+    cs2, cells2 = q.colouring(base=0)
+    assert np.array_equal(cs, cs2) and np.array_equal(cells, cells2)
+    if family == 1:
+        inc = t - 1
+    else:
+        gd = p.global_dofs(base=0)
+        inc = gd[:, 0:6 * order:order] // order  # edge numbers (edge dofs lead)
+    assert nc <= _neighbour_degree(inc) + 1
