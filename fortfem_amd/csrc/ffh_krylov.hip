@@ -410,13 +410,14 @@ template <typename VT>
 struct EpiInit { // r = b - A x ; z = D r ; r.r, r.z
     const VT *b, *dinv;
     VT *r;
-    ZP<VT> *zp;
+    VT *z; // z of row i at z[i * zs]: zs = 2 in (z, p) pairs, 1 dense
+    int zs;
     __device__ void operator()(int i, VT s, double *d) const
     {
         const VT ri = b[i] - s;
         const VT zi = dinv[i] * ri;
         r[i] = ri;
-        zp[i].z = zi;
+        z[(size_t)i * zs] = zi;
         d[0] += (double)ri * (double)ri;
         d[1] += (double)ri * (double)zi;
     }
@@ -462,7 +463,8 @@ struct EpiRefresh { // mixed: FP64 residual replaces the FP32 one ; z ; rr, rz, 
     const double *b;
     const float *dinv32;
     float *r32;
-    ZP<float> *zp;
+    float *zo; // z of row i at zo[i * zs] (as EpiInit)
+    int zs;
     __device__ void operator()(int i, double s, double *d) const
     {
         const double ri = b[i] - s;
@@ -470,7 +472,7 @@ struct EpiRefresh { // mixed: FP64 residual replaces the FP32 one ; z ; rr, rz, 
         const float r = (float)ri;
         const float z = dinv32[i] * r;
         r32[i] = r;
-        zp[i].z = z;
+        zo[(size_t)i * zs] = z;
         d[0] += ri * ri;
         d[1] += (double)r * (double)z;
         d[2] += dr * dr;
@@ -494,16 +496,17 @@ __global__ __launch_bounds__(kStreamBlock) void k_spmv(int n, const int *__restr
 
 // Partial-sum slots (stride apart): 0 p.Ap, 1 r.r, 2 r.(D r), 3 drift.
 //
-// r = b - A x ; z = D r into zp0 ; partials of r.r and r.z
+// r = b - A x ; z = D r into z (stride zs: the first (z, p) array, or the
+// dense z of the three-launch form) ; partials of r.r and r.z
 // (initialize_pcg_state :859-878)
 template <int V, typename MT, typename VT>
 __global__ __launch_bounds__(kStreamBlock) void k_pcg_init(
     int n, const int *__restrict__ ptr, const int *__restrict__ idx, const MT *__restrict__ val,
     const VT *__restrict__ x, const VT *__restrict__ b, VT *__restrict__ r,
-    const VT *__restrict__ dinv, ZP<VT> *__restrict__ zp0, const int *__restrict__ rowblocks,
+    const VT *__restrict__ dinv, VT *__restrict__ z, int zs, const int *__restrict__ rowblocks,
     int nrb, double *__restrict__ partials, int stride)
 {
-    EpiInit<VT> epi{b, dinv, r, zp0};
+    EpiInit<VT> epi{b, dinv, r, z, zs};
     double d[3] = {0.0, 0.0, 0.0};
     row_engine<V, MT, VT>(n, ptr, idx, val, XPlain<VT>{x}, rowblocks, nrb, epi, d);
     const double s0 = block_sum<kStreamBlock>(d[0]);
@@ -614,36 +617,30 @@ __global__ __launch_bounds__(kStreamBlock) void k_pcg_dirspmv(
     if (threadIdx.x == 0) partials[blockIdx.x] = s0;
 }
 
-// Three-launch form of an iteration (large systems, where forming the
-// direction inside the SpMV's gathers costs more than a launch of its own).
+// Three-launch form of an iteration (where forming the direction inside the
+// SpMV's gathers costs more than a launch of its own), on dense z and p.
 // Launch 1: the tail of the previous iteration (every block), then
-// p = z + beta p in place (the expression of XDir), also stored densely in
-// pl for the SpMV's gathers.
+// p = z + beta p in place (the expression of XDir).
 template <typename VT>
-__global__ __launch_bounds__(kBlock) void k_pcg_dir(int n, ZP<VT> *__restrict__ zp,
-                                                    VT *__restrict__ pl,
+__global__ __launch_bounds__(kBlock) void k_pcg_dir(int n, const VT *__restrict__ z,
+                                                    VT *__restrict__ p,
                                                     const double *__restrict__ partials,
                                                     int stride, KrylovCtl *ctl)
 {
     if (ctl->done) return;
     const int restart = ctl->restart;
     const int i0 = xcd_block() * kBlock + threadIdx.x;
-    ZP<VT> q0{};
-    if (i0 < n) q0 = zp[i0];
+    VT z0 = VT(0), p0 = VT(0);
+    if (i0 < n) {
+        z0 = z[i0];
+        p0 = p[i0];
+    }
     double beta_d;
     if (pcg_finish<kBlock>(partials, stride, ctl, beta_d)) return;
     const VT beta = (VT)beta_d;
-    if (i0 < n) {
-        const VT p = restart ? q0.z : fma_t(beta, q0.p, q0.z);
-        zp[i0].p = p;
-        pl[i0] = p;
-    }
-    for (int i = i0 + gridDim.x * kBlock; i < n; i += gridDim.x * kBlock) {
-        const ZP<VT> q = zp[i];
-        const VT p = restart ? q.z : fma_t(beta, q.p, q.z);
-        zp[i].p = p;
-        pl[i] = p;
-    }
+    if (i0 < n) p[i0] = restart ? z0 : fma_t(beta, p0, z0);
+    for (int i = i0 + gridDim.x * kBlock; i < n; i += gridDim.x * kBlock)
+        p[i] = restart ? z[i] : fma_t(beta, p[i], z[i]);
 }
 
 // Launch 2 of the three-launch form: Ap = A p and the partials of p.Ap.
@@ -661,9 +658,88 @@ __global__ __launch_bounds__(kStreamBlock) void k_pcg_apdot(
     if (threadIdx.x == 0) partials[blockIdx.x] = s0;
 }
 
-// Launch 2: p.Ap -> alpha (or breakdown, :897-899); x += alpha p ;
-// r -= alpha Ap ; z = D r next to p ; partials of r.r and r.z.  x may be
-// FP64 while the iteration vectors are FP32 (mixed precision).
+// p.Ap from the SpMV launch's partials -> alpha (or the breakdown stop,
+// :897-899), recorded by block 0; false when the solve stops here.
+__device__ __forceinline__ bool update_alpha(const double *__restrict__ partials, int nb_spmv,
+                                             KrylovCtl *ctl, int cur, double &alpha_d)
+{
+    const double pap = sum_partials<kBlock>(partials, nb_spmv);
+    const double rz_old = ctl->rzs[cur];
+    if (is_breakdown(pap, rz_old, ctl->breakdown_mode)) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) {
+            ctl->pap = pap;
+            ctl->done = 1;
+            ctl->breakdown_hit = 1;
+        }
+        return false;
+    }
+    alpha_d = rz_old / pap;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        ctl->pap = pap;
+        ctl->alpha = alpha_d;
+        ctl->k2_done = ctl->k2_done + 1;
+        ctl->restart = 0;
+        ctl->nb_rz = gridDim.x;
+    }
+    return true;
+}
+
+// Launch 3 of the three-launch form, on dense z and p: alpha ; x += alpha p ;
+// r -= alpha Ap ; z = D r ; partials of r.r and r.z.  x may be FP64 while
+// the iteration vectors are FP32 (mixed precision).
+template <typename VT, typename XT>
+__global__ __launch_bounds__(kBlock) void k_pcg_update3(
+    int n, XT *__restrict__ x, VT *__restrict__ r, VT *__restrict__ z, const VT *__restrict__ p,
+    const VT *__restrict__ ap, const VT *__restrict__ dinv, double *__restrict__ partials,
+    int stride, int nb_spmv, KrylovCtl *ctl)
+{
+    if (ctl->done) return;
+    const int cur = ctl->cur;
+    // this thread's first element, loaded while the partials are summed
+    const int i0 = xcd_block() * kBlock + threadIdx.x;
+    XT x0 = XT(0);
+    VT r0 = VT(0), a0 = VT(0), d0 = VT(0), p0 = VT(0);
+    if (i0 < n) {
+        x0 = x[i0];
+        r0 = r[i0];
+        a0 = ap[i0];
+        d0 = dinv[i0];
+        p0 = p[i0];
+    }
+    double alpha_d;
+    if (!update_alpha(partials, nb_spmv, ctl, cur, alpha_d)) return;
+    const VT alpha = (VT)alpha_d;
+    const XT alpha_x = (XT)alpha_d;
+    double rr = 0.0, rz = 0.0;
+    if (i0 < n) {
+        x[i0] = x0 + alpha_x * (XT)p0;
+        const VT ri = r0 - alpha * a0;
+        r[i0] = ri;
+        const VT zi = d0 * ri;
+        z[i0] = zi;
+        rr += (double)ri * (double)ri;
+        rz += (double)ri * (double)zi;
+    }
+    for (int i = i0 + gridDim.x * kBlock; i < n; i += gridDim.x * kBlock) {
+        x[i] = x[i] + alpha_x * (XT)p[i];
+        const VT ri = r[i] - alpha * ap[i];
+        r[i] = ri;
+        const VT zi = dinv[i] * ri;
+        z[i] = zi;
+        rr += (double)ri * (double)ri;
+        rz += (double)ri * (double)zi;
+    }
+    const double s0 = block_sum<kBlock>(rr);
+    const double s1 = block_sum<kBlock>(rz);
+    if (threadIdx.x == 0) {
+        partials[stride + blockIdx.x] = s0;
+        partials[2 * stride + blockIdx.x] = s1;
+    }
+}
+
+// Launch 2 of the two-launch form: p.Ap -> alpha (or breakdown, :897-899);
+// x += alpha p ; r -= alpha Ap ; z = D r next to p ; partials of r.r and
+// r.z.  x may be FP64 while the iteration vectors are FP32 (mixed precision).
 template <typename VT, typename XT>
 __global__ __launch_bounds__(kBlock) void k_pcg_update(
     int n, XT *__restrict__ x, VT *__restrict__ r, ZP<VT> *__restrict__ zp0,
@@ -685,24 +761,8 @@ __global__ __launch_bounds__(kBlock) void k_pcg_update(
         d0 = dinv[i0];
         q0 = zp[i0];
     }
-    const double pap = sum_partials<kBlock>(partials, nb_spmv);
-    const double rz_old = ctl->rzs[cur];
-    if (is_breakdown(pap, rz_old, ctl->breakdown_mode)) {
-        if (blockIdx.x == 0 && threadIdx.x == 0) {
-            ctl->pap = pap;
-            ctl->done = 1;
-            ctl->breakdown_hit = 1;
-        }
-        return;
-    }
-    const double alpha_d = rz_old / pap;
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        ctl->pap = pap;
-        ctl->alpha = alpha_d;
-        ctl->k2_done = ctl->k2_done + 1;
-        ctl->restart = 0;
-        ctl->nb_rz = gridDim.x;
-    }
+    double alpha_d;
+    if (!update_alpha(partials, nb_spmv, ctl, cur, alpha_d)) return;
     const VT alpha = (VT)alpha_d;
     const XT alpha_x = (XT)alpha_d;
     double rr = 0.0, rz = 0.0;
@@ -797,12 +857,14 @@ __global__ __launch_bounds__(kStreamBlock) void k_mixed_refresh(
     int n, const int *__restrict__ ptr, const int *__restrict__ idx,
     const double *__restrict__ val, const double *__restrict__ x,
     const double *__restrict__ b, float *__restrict__ r32, const float *__restrict__ dinv32,
-    ZP<float> *__restrict__ zp0, ZP<float> *__restrict__ zp1, const int *__restrict__ rowblocks,
+    float *__restrict__ z0, float *__restrict__ z1, int zs, const int *__restrict__ rowblocks,
     int nrb, double *__restrict__ partials, int stride, const KrylovCtl *ctl)
 {
     if (ctl->converged) return;
-    // z goes where the next direction launch reads it (slot k2_done & 1)
-    EpiRefresh epi{b, dinv32, r32, (ctl->k2_done & 1) ? zp1 : zp0};
+    // z goes where the next direction launch reads it: the dense z of the
+    // three-launch form (z0 = z1, zs = 1), or the (z, p) array of slot
+    // k2_done & 1 (zs = 2)
+    EpiRefresh epi{b, dinv32, r32, (ctl->k2_done & 1) ? z1 : z0, zs};
     double d[3] = {0.0, 0.0, 0.0};
     row_engine<V, double, double>(n, ptr, idx, val, XPlain<double>{x}, rowblocks, nrb, epi, d);
     const double s0 = block_sum<kStreamBlock>(d[0]);
@@ -1445,9 +1507,9 @@ static void launch_iterations(Context *ctx, const DevMatrix &A, double *partials
     ZP<VT> *zp0 = reinterpret_cast<ZP<VT> *>(zpa), *zp1 = reinterpret_cast<ZP<VT> *>(zpb);
     const int n = A.n;
     if (three_launch(n, sizeof(VT) == 4)) {
-        // one (z, p) array updated in place; the second array holds p
-        // densely (its first n values)
-        VT *pl = zpb;
+        // dense z (the first n values of the first array) and p (those of
+        // the second)
+        VT *zl = zpa, *pl = zpb;
         const int gridv = grid_for(ctx, n, kBlock);
         const int gridd = (int)std::min<int64_t>(
             gridv, std::max<int64_t>(ctx->cu_count, ((int64_t)n + 4 * kBlock - 1) / (4 * kBlock)));
@@ -1455,7 +1517,7 @@ static void launch_iterations(Context *ctx, const DevMatrix &A, double *partials
         const int stride = partial_stride(ctx);
         const GArgs<MT> g = gargs<MT>(A, vi_sp);
         for (int it = 0; it < iters; ++it) {
-            hipLaunchKernelGGL((k_pcg_dir<VT>), dim3(gridd), dim3(kBlock), 0, ctx->stream, n, zp0,
+            hipLaunchKernelGGL((k_pcg_dir<VT>), dim3(gridd), dim3(kBlock), 0, ctx->stream, n, zl,
                                pl, partials, stride, ctl);
             if (prof_slot >= 0) ctx->prof_mark(prof_slot, true);
             FFH_ENGINE(kVariants[vi_sp].engine, {
@@ -1464,8 +1526,8 @@ static void launch_iterations(Context *ctx, const DevMatrix &A, double *partials
                                    g.idx, g.val, pl, ap, g.rb, g.nrb, partials, ctl);
             });
             if (prof_slot >= 0) ctx->prof_mark(prof_slot, false);
-            hipLaunchKernelGGL((k_pcg_update<VT, XT>), dim3(gridd), dim3(kBlock), 0, ctx->stream,
-                               n, x, r, zp0, zp0, ap, dinv, partials, stride, grids, ctl);
+            hipLaunchKernelGGL((k_pcg_update3<VT, XT>), dim3(gridd), dim3(kBlock), 0, ctx->stream,
+                               n, x, r, zl, pl, ap, dinv, partials, stride, grids, ctl);
         }
         FFH_HIP(hipGetLastError());
         return;
@@ -1507,9 +1569,8 @@ static void launch_refresh(Context *ctx, Matrix &M, int vi64, int first)
         hipLaunchKernelGGL((k_mixed_refresh<VV>), dim3(gr), dim3(kStreamBlock),
                            variant_lds(A, vi64, sizeof(double)), ctx->stream,
                            A.n, g.ptr, g.idx, g.val, w.x.p, w.b.p, w.r32.p, w.dinv32.p,
-                           reinterpret_cast<ZP<float> *>(w.p32.p),
-                           reinterpret_cast<ZP<float> *>(three_launch(A.n, true) ? w.p32.p : w.p32b.p),
-                           g.rb, g.nrb, w.partials.p,
+                           w.p32.p, three_launch(A.n, true) ? w.p32.p : w.p32b.p,
+                           three_launch(A.n, true) ? 1 : 2, g.rb, g.nrb, w.partials.p,
                            stride, w.ctl.p);
     });
     if (ctx->profiling) ctx->prof_mark(0, false);
@@ -1624,9 +1685,8 @@ static KrylovCtl pcg_fp64_on(Context *ctx, const DevMatrix &A, int vi, int vi_it
     FFH_ENGINE(kVariants[vi].engine, {
         hipLaunchKernelGGL((k_pcg_init<VV, double, double>), dim3(gi), dim3(kStreamBlock),
                            variant_lds(A, vi, sizeof(double)), s, n,
-                           g.ptr, g.idx, g.val, w.x, w.b, w.r, w.dinv,
-                           reinterpret_cast<ZP<double> *>(w.zpa), g.rb, g.nrb, w.partials,
-                           stride);
+                           g.ptr, g.idx, g.val, w.x, w.b, w.r, w.dinv, w.zpa,
+                           three_launch(n, false) ? 1 : 2, g.rb, g.nrb, w.partials, stride);
     });
     hipLaunchKernelGGL(k_pcg_init_finish, dim3(1), dim3(kBlock), 0, s, w.partials, stride, gi,
                        w.ctl);
