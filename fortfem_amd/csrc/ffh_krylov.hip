@@ -621,6 +621,10 @@ __global__ __launch_bounds__(kStreamBlock) void k_pcg_dirspmv(
 // SpMV's gathers costs more than a launch of its own), on dense z and p.
 // Launch 1: the tail of the previous iteration (every block), then
 // p = z + beta p in place (the expression of XDir).
+// A thread's first kDirRows rows are loaded before the tail's partial sums,
+// so that their fetch overlaps the sums' latency.
+constexpr int kDirRows = 8;
+
 template <typename VT>
 __global__ __launch_bounds__(kBlock) void k_pcg_dir(int n, const VT *__restrict__ z,
                                                     VT *__restrict__ p,
@@ -630,16 +634,23 @@ __global__ __launch_bounds__(kBlock) void k_pcg_dir(int n, const VT *__restrict_
     if (ctl->done) return;
     const int restart = ctl->restart;
     const int i0 = xcd_block() * kBlock + threadIdx.x;
-    VT z0 = VT(0), p0 = VT(0);
-    if (i0 < n) {
-        z0 = z[i0];
-        p0 = p[i0];
+    const int step = gridDim.x * kBlock;
+    VT zk[kDirRows], pk[kDirRows];
+#pragma unroll
+    for (int k = 0; k < kDirRows; ++k) {
+        const int i = i0 + k * step;
+        zk[k] = i < n ? z[i] : VT(0);
+        pk[k] = i < n ? p[i] : VT(0);
     }
     double beta_d;
     if (pcg_finish<kBlock>(partials, stride, ctl, beta_d)) return;
     const VT beta = (VT)beta_d;
-    if (i0 < n) p[i0] = restart ? z0 : fma_t(beta, p0, z0);
-    for (int i = i0 + gridDim.x * kBlock; i < n; i += gridDim.x * kBlock)
+#pragma unroll
+    for (int k = 0; k < kDirRows; ++k) {
+        const int i = i0 + k * step;
+        if (i < n) p[i] = restart ? zk[k] : fma_t(beta, pk[k], zk[k]);
+    }
+    for (int i = i0 + kDirRows * step; i < n; i += step)
         p[i] = restart ? z[i] : fma_t(beta, p[i], z[i]);
 }
 
@@ -695,32 +706,39 @@ __global__ __launch_bounds__(kBlock) void k_pcg_update3(
 {
     if (ctl->done) return;
     const int cur = ctl->cur;
-    // this thread's first element, loaded while the partials are summed
+    // this thread's first kDirRows rows, loaded while the partials are summed
     const int i0 = xcd_block() * kBlock + threadIdx.x;
-    XT x0 = XT(0);
-    VT r0 = VT(0), a0 = VT(0), d0 = VT(0), p0 = VT(0);
-    if (i0 < n) {
-        x0 = x[i0];
-        r0 = r[i0];
-        a0 = ap[i0];
-        d0 = dinv[i0];
-        p0 = p[i0];
+    const int step = gridDim.x * kBlock;
+    XT xk[kDirRows];
+    VT rk[kDirRows], ak[kDirRows], dk[kDirRows], pk[kDirRows];
+#pragma unroll
+    for (int k = 0; k < kDirRows; ++k) {
+        const int i = i0 + k * step;
+        const bool in = i < n;
+        xk[k] = in ? x[i] : XT(0);
+        rk[k] = in ? r[i] : VT(0);
+        ak[k] = in ? ap[i] : VT(0);
+        dk[k] = in ? dinv[i] : VT(0);
+        pk[k] = in ? p[i] : VT(0);
     }
     double alpha_d;
     if (!update_alpha(partials, nb_spmv, ctl, cur, alpha_d)) return;
     const VT alpha = (VT)alpha_d;
     const XT alpha_x = (XT)alpha_d;
     double rr = 0.0, rz = 0.0;
-    if (i0 < n) {
-        x[i0] = x0 + alpha_x * (XT)p0;
-        const VT ri = r0 - alpha * a0;
-        r[i0] = ri;
-        const VT zi = d0 * ri;
-        z[i0] = zi;
+#pragma unroll
+    for (int k = 0; k < kDirRows; ++k) {
+        const int i = i0 + k * step;
+        if (i >= n) continue;
+        x[i] = xk[k] + alpha_x * (XT)pk[k];
+        const VT ri = rk[k] - alpha * ak[k];
+        r[i] = ri;
+        const VT zi = dk[k] * ri;
+        z[i] = zi;
         rr += (double)ri * (double)ri;
         rz += (double)ri * (double)zi;
     }
-    for (int i = i0 + gridDim.x * kBlock; i < n; i += gridDim.x * kBlock) {
+    for (int i = i0 + kDirRows * step; i < n; i += step) {
         x[i] = x[i] + alpha_x * (XT)p[i];
         const VT ri = r[i] - alpha * ap[i];
         r[i] = ri;
