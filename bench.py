@@ -45,9 +45,9 @@ HBM_PEAK_GBS = 8000.0
 # what the counter reads for a device copy of known size (the summary and the
 # commands are in profiles/).  None where no such measurement exists for the
 # kernel as it is.
-MEASURED_TRAFFIC = {"c4": 254.34e6}
+MEASURED_TRAFFIC = {"c4": 254.33e6}
 TRAFFIC_SOURCE = {
-    "c4": "k_pcg_apdot<2, float, float>: 2.0 x FETCH_SIZE 120,147.6 KiB + 1.0 x WRITE_SIZE 8,081.7 KiB "
+    "c4": "k_pcg_apdot<2, float, float>: 2.0 x FETCH_SIZE 120,149.3 KiB + 1.0 x WRITE_SIZE 8,076.0 KiB "
           "per live launch, rocprofv3 --pmc over bench.py --no-cpu --no-served --steps 1 --warmup 0, "
           "factors from the 1 GiB copy of bench.py --workload copy; "
           "profiles/k_pcg_apdot_p1_poisson_128cube.md",
