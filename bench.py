@@ -40,18 +40,86 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 
 
-# HBM bytes per launch of the roofline kernel, from rocprofv3 --pmc FETCH_SIZE
-# and WRITE_SIZE passes over this script at the named commit, each scaled by
-# what the counter reads for a device copy of known size (the summary and the
-# commands are in profiles/).  None where no such measurement exists for the
-# kernel as it is.
-MEASURED_TRAFFIC = {"c4": 254.33e6}
-TRAFFIC_SOURCE = {
-    "c4": "k_pcg_apdot<2, float, float>: 2.0 x FETCH_SIZE 120,149.3 KiB + 1.0 x WRITE_SIZE 8,076.0 KiB "
-          "per live launch, rocprofv3 --pmc over bench.py --no-cpu --no-served --steps 1 --warmup 0, "
-          "factors from the 1 GiB copy of bench.py --workload copy; "
-          "profiles/k_pcg_apdot_p1_poisson_128cube.md",
-}
+ROOFLINE_KERNEL = {"c4": "k_pcg_apdot", "c2": "k_pcg_apdot"}
+
+
+def _pmc_pass(counter, cmd, tmp, kernel_regex=None, timeout=240):
+    """One rocprofv3 --pmc pass (one counter) over a child run of this
+    script: {kernel name: [(counter value, duration us), ...]} from its
+    counter CSV, or raises."""
+    import csv
+    import glob
+    import shutil
+    import subprocess
+
+    out = os.path.join(tmp, counter)
+    shutil.rmtree(out, ignore_errors=True)
+    prof = [shutil.which("rocprofv3") or "rocprofv3", "--pmc", counter, "--output-format", "csv",
+            "-d", out, "-o", "p"]
+    if kernel_regex:
+        prof += ["--kernel-include-regex", kernel_regex]
+    env = dict(os.environ, TMPDIR=tmp)
+    r = subprocess.run(prof + ["--", sys.executable, os.path.abspath(__file__)] + cmd, cwd=tmp,
+                       env=env, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL, timeout=timeout)
+    files = glob.glob(os.path.join(out, "**", "*counter_collection.csv"), recursive=True)
+    if r.returncode != 0 or not files:
+        raise RuntimeError(f"rocprofv3 --pmc {counter} exited {r.returncode}")
+    res = {}
+    for row in csv.DictReader(open(files[0])):
+        if row["Counter_Name"] != counter:
+            continue
+        d = (int(row["End_Timestamp"]) - int(row["Start_Timestamp"])) / 1e3
+        res.setdefault(row["Kernel_Name"], []).append((float(row["Counter_Value"]), d))
+    return res
+
+
+def measure_traffic(workload: str):
+    """HBM bytes per live launch of the roofline kernel, measured now: two
+    rocprofv3 --pmc passes (FETCH_SIZE, then WRITE_SIZE, in KiB per dispatch)
+    over a one-step child run of this workload, each scaled by what the same
+    counter reads for a device copy of 1 GiB (--workload copy).  Returns
+    (bytes or None, description)."""
+    import shutil
+    import tempfile
+
+    if shutil.which("rocprofv3") is None:
+        return None, "not measured: rocprofv3 is not on PATH"
+    if any("rocprof" in os.environ.get(k, "") for k in ("LD_PRELOAD", "HSA_TOOLS_LIB")):
+        return None, "not measured: this run is itself under a profiler"
+    kname = ROOFLINE_KERNEL.get(workload)
+    tmp = tempfile.mkdtemp(prefix="ffh_pmc_")
+    try:
+        child = ["--workload", workload, "--steps", "1", "--warmup", "0", "--no-cpu", "--no-served",
+                 "--no-traffic"]
+        kf = _pmc_pass("FETCH_SIZE", child, tmp, kname)
+        kw = _pmc_pass("WRITE_SIZE", child, tmp, kname)
+        cf = _pmc_pass("FETCH_SIZE", ["--workload", "copy", "--steps", "3"], tmp)
+        cw = _pmc_pass("WRITE_SIZE", ["--workload", "copy", "--steps", "3"], tmp)
+
+        def live(res):  # the roofline kernel's launches that did work (> 5 us)
+            vals = [v for k, vs in res.items() if kname in k for v, d in vs if d > 5.0]
+            if not vals:
+                raise RuntimeError(f"no live {kname} launch in the counter output")
+            return sum(vals) / len(vals)
+
+        def copy_value(res):  # the copy kernel's dispatches (torch's multiply by 1.0)
+            vals = [v for k, vs in res.items() if "MulFunctor" in k for v, _ in vs]
+            if not vals:
+                raise RuntimeError("no copy dispatch in the calibration counters")
+            return float(np.median(vals))
+
+        gib_kib = float(1 << 20)
+        f_scale, w_scale = gib_kib / copy_value(cf), gib_kib / copy_value(cw)
+        fk, wk = live(kf), live(kw)
+        traffic = (f_scale * fk + w_scale * wk) * 1024.0
+        return traffic, (f"measured in this run: {kname} {f_scale:.2f} x FETCH_SIZE {fk:,.1f} KiB + "
+                         f"{w_scale:.2f} x WRITE_SIZE {wk:,.1f} KiB per live launch (rocprofv3 --pmc "
+                         f"over bench.py --workload {workload} --steps 1 --warmup 0; scale factors from "
+                         f"a 1 GiB device copy, bench.py --workload copy)")
+    except Exception as e:  # noqa: BLE001 -- the line reports why it has no figure
+        return None, f"not measured: {e}"
+    finally:
+        shutil.rmtree(tmp, ignore_errors=True)
 
 
 def spmv_bytes(nnz: int, n: int, sv: int) -> float:
@@ -497,6 +565,8 @@ def main():
                          "served1 (one served request alone, for counter runs)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-served", action="store_true", help="skip the served-stream keys")
+    ap.add_argument("--no-traffic", action="store_true",
+                    help="skip the rocprofv3 --pmc passes that measure roofline.traffic")
     ap.add_argument("--cpu-threads", type=int, default=0, help="CPU baseline threads (0: min(16, nproc))")
     ap.add_argument("--cpu-reps", type=int, default=5, help="CPU baseline timed steps (after 1 warm-up)")
     ap.add_argument("--graph", action="store_true", help="replay iteration chunks from HIP graphs")
@@ -639,8 +709,8 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": MEASURED_TRAFFIC.get(args.workload),
-            "traffic_source": TRAFFIC_SOURCE.get(args.workload),
+            "traffic": None,
+            "traffic_source": "not measured (--no-traffic)",
             "avg_launch_ms": avg_ms,
             "launches": cnt,
             "bytes_per_launch": alg32,
@@ -654,6 +724,11 @@ def main():
                             "offsets, row map, tile offsets and column lists, p, A p)",
         },
     }
+    if not args.no_traffic:
+        print(json.dumps({"progress": "traffic counters"}), file=sys.stderr, flush=True)
+        tr, src = measure_traffic(args.workload)
+        out["roofline"]["traffic"] = tr
+        out["roofline"]["traffic_source"] = src
     if not args.no_served:
         print(json.dumps({"progress": "served stream"}), file=sys.stderr, flush=True)
         out["served"] = served_stream(args)
