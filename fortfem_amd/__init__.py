@@ -1435,6 +1435,17 @@ class Server:
         """Replay each step's chunk from a HIP graph (ffh_server_use_graph)."""
         self.ctx.check(self._lib.ffh_server_use_graph(self.handle, 1 if enable else 0))
 
+    def set_tiling(self, mode=-1):
+        """Batched SpMV per system type: -1 autotune (default), 0 plain SELL-64,
+        1 LDS column tiles wherever they fit (ffh_server_set_tiling)."""
+        self.ctx.check(self._lib.ffh_server_set_tiling(self.handle, mode))
+
+    def tiling_stats(self):
+        """(system types chosen so far, how many run the tiled product)."""
+        n, t = C.c_int(0), C.c_int(0)
+        self.ctx.check(self._lib.ffh_server_tiling_stats(self.handle, C.byref(n), C.byref(t)))
+        return n.value, t.value
+
     def set_precision(self, policy=-1):
         """0 FP32 iterations (switch to FP64 on stagnation), 1 FP64, 2 FP64 for
         Nedelec; returns the number of requests switched so far."""

@@ -40,6 +40,13 @@ constexpr int kMaxRhs = 8;
 
 struct BatchSys {
     const int *off, *sidx, *srow; // SELL-64 structure of the cached reduced pattern
+    // column tiles of that SELL copy (the row map, tile offsets and tile
+    // columns in one array; 16-bit tile positions per entry), used when the
+    // autotuner picked the tiled product for this system: tiled is then the
+    // pattern's largest tile (columns), else 0
+    const int *tpack;
+    const uint16_t *tlidx;
+    int tiled;
     const float *val32;    // SELL values of the request
     const double *val64;
     const float *dinv32;
@@ -215,14 +222,92 @@ __device__ __forceinline__ void item_spmv(const BatchSys &s, int slice0, const M
     }
 }
 
+// The same product with the item's column tile staged in LDS: the item (a
+// group of four slices) gathers the R-wide rows of p at its distinct
+// columns once, then every lane adds its row's entries from LDS through the
+// 16-bit tile positions.  Each lane sums its products in the same order as
+// item_spmv, so both give the same A p to the bit.
+template <int R, typename MT, typename VT>
+__device__ __forceinline__ void item_spmv_tiled(const BatchSys &s, int slice0,
+                                                const MT *__restrict__ val,
+                                                const VT *__restrict__ p, VT *__restrict__ ap,
+                                                double (&d)[R])
+{
+    extern __shared__ __align__(16) unsigned char b_dyn_lds[];
+    VT *tile = reinterpret_cast<VT *>(b_dyn_lds);
+    constexpr int U = R >= 4 ? 4 : 8;
+    const int ngroup = (s.nslice + 3) >> 2;
+    const int g = slice0 >> 2;
+    const int *toff = s.tpack + (size_t)s.nslice * 64;
+    const int *tcols = toff + ngroup + 1;
+    const int c0 = toff[g], nc = toff[g + 1] - c0;
+    for (int k = threadIdx.x; k < nc; k += 256) st<VT, R>(tile, k, ld<VT, R>(p, tcols[c0 + k]));
+    __syncthreads();
+    const int lane = threadIdx.x & 63;
+    const int sl = slice0 + (threadIdx.x >> 6);
+#pragma unroll
+    for (int r = 0; r < R; ++r) d[r] = 0.0;
+    if (sl >= s.nslice) return;
+    const int o = s.off[sl], w = (s.off[sl + 1] - o) >> 6;
+    const uint16_t *ip = s.tlidx + o + lane;
+    const MT *vp = val + o + lane;
+    VT acc[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) acc[r] = VT(0);
+    int j = 0;
+    for (; j + U <= w; j += U) {
+        int c[U];
+        VT v[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            c[u] = ldm(ip + 64 * u);
+            v[u] = (VT)ldm(vp + 64 * u);
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const Pack<VT, R> pc = ld<VT, R>(tile, c[u]);
+#pragma unroll
+            for (int r = 0; r < R; ++r) acc[r] += v[u] * pc.v[r];
+        }
+        ip += 64 * U;
+        vp += 64 * U;
+    }
+    for (; j < w; ++j) {
+        const Pack<VT, R> pc = ld<VT, R>(tile, (int)ldm(ip));
+        const VT v = (VT)ldm(vp);
+#pragma unroll
+        for (int r = 0; r < R; ++r) acc[r] += v * pc.v[r];
+        ip += 64;
+        vp += 64;
+    }
+    const int row = s.srow[sl * 64 + lane];
+    if (row >= 0) {
+        const Pack<VT, R> own = ld<VT, R>(p, row);
+        Pack<VT, R> out;
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            out.v[r] = acc[r];
+            d[r] = (double)own.v[r] * (double)acc[r];
+        }
+        st<VT, R>(ap, row, out);
+    }
+}
+
 template <int R>
 __device__ __forceinline__ void spmv_item(const BatchSys &s, int slice0, double *sh, double *out)
 {
     double d[R];
-    if (s.prec)
-        item_spmv<R, double, double>(s, slice0, s.val64, s.p64, s.ap64, d);
-    else
-        item_spmv<R, float, float>(s, slice0, s.val32, s.p, s.ap, d);
+    if (s.tiled) {
+        if (s.prec)
+            item_spmv_tiled<R, double, double>(s, slice0, s.val64, s.p64, s.ap64, d);
+        else
+            item_spmv_tiled<R, float, float>(s, slice0, s.val32, s.p, s.ap, d);
+    } else {
+        if (s.prec)
+            item_spmv<R, double, double>(s, slice0, s.val64, s.p64, s.ap64, d);
+        else
+            item_spmv<R, float, float>(s, slice0, s.val32, s.p, s.ap, d);
+    }
     block_sums<R>(d, sh, out);
 }
 
@@ -694,6 +779,14 @@ __global__ void k_b_interleave(int n, int nrhs, int R, const double *__restrict_
 }
 
 // ----------------------------------------------------------------- engine --
+// Dynamic LDS of a tiled system's product: its largest column tile, R
+// values of its iteration precision per column.
+constexpr size_t kTileLdsCap = 65536;
+size_t tile_lds_bytes(const BatchSys &b)
+{
+    return (size_t)b.tiled * (size_t)b.R * (b.prec ? sizeof(double) : sizeof(float));
+}
+
 // One group of the batch: its systems, work table and per-item partial sums,
 // advanced on a stream of its own.
 struct BatchGroup {
@@ -703,6 +796,7 @@ struct BatchGroup {
     std::vector<BatchSys> h_sys; // what d_sys / d_work hold
     std::vector<int2> h_work;
     int nwork = 0, nsys = 0;
+    size_t lds = 0;         // dynamic LDS of k_b_spmv: the largest tile of a tiled system
     int64_t generation = 0; // bumped whenever the tables change
     hipStream_t stream = nullptr;
     hipEvent_t done = nullptr;
@@ -727,6 +821,9 @@ struct BatchGroup {
         }
         nwork = (int)work.size();
         nsys = (int)sys.size();
+        lds = 0;
+        for (const BatchSys &b : sys)
+            if (b.tiled) lds = std::max(lds, tile_lds_bytes(b));
     }
 
     // enqueue one PCG iteration of the group's systems
@@ -734,7 +831,7 @@ struct BatchGroup {
     {
         const int W = nwork, NS = nsys;
         hipStream_t s = stream;
-        hipLaunchKernelGGL(k_b_spmv, dim3(W), dim3(256), 0, s, d_work.p, d_sys.p, scal, d_p0.p);
+        hipLaunchKernelGGL(k_b_spmv, dim3(W), dim3(256), lds, s, d_work.p, d_sys.p, scal, d_p0.p);
         hipLaunchKernelGGL(k_b_alpha, dim3(NS), dim3(256), 0, s, d_sys.p, scal, d_p0.p);
         hipLaunchKernelGGL(k_b_update, dim3(W), dim3(256), 0, s, d_work.p, d_sys.p, scal, d_p0.p,
                            d_p1.p);
@@ -928,6 +1025,10 @@ struct ffh_server {
     int64_t next_ticket = 1;
     ffh::BatchEngine eng; // device batch state
     int64_t iterations_run = 0;
+    // autotuned product per system type (pattern, R, iteration precision):
+    // the largest tile when the tiled product won, else 0
+    std::map<std::tuple<const ffh::Problem *, int, int>, int> tiling;
+    int tiling_mode = -1; // -1 autotune, 0 plain, 1 tiled where the tiles fit
 };
 
 namespace ffh {
@@ -976,6 +1077,8 @@ Problem *server_prepare(ffh_server *S, int mesh_key, int family, int order, int 
     problem_dirichlet(P.get(), 0, nullptr);
     FFH_REQUIRE(sell_structure(S->ctx, P->reduced.m), FFH_ERR_INVALID,
                 "the reduced matrix has no SELL-64 form (rows too uneven)");
+    // column tiles for the batched product's autotuner (host work, once)
+    ensure_sell_tiles(S->ctx, P->reduced.m);
     Problem *raw = P.get();
     S->cache[key] = std::move(P);
     return raw;
@@ -1037,6 +1140,105 @@ static void admit(ffh_server *S, ServedRequest *q)
     S->changed = true;
 }
 
+// A request's system with its device arrays (slot and tiling set by the
+// caller).
+static BatchSys batch_sys(const ServedRequest *q)
+{
+    const DevMatrix &A = q->sym->reduced.m;
+    BatchSys b{};
+    b.off = A.sell_off.p;
+    b.sidx = A.sell_idx.p;
+    b.srow = A.sell_row.p;
+    b.tpack = A.tile_pack.p;
+    b.tlidx = A.tile_lidx.p;
+    b.nslice = A.nslice;
+    b.val32 = q->val32.p;
+    b.val64 = q->val64.p;
+    b.dinv32 = q->dinv32.p;
+    b.dinv64 = q->dinv.p;
+    b.b = q->b.p;
+    b.x = q->x.p;
+    b.r = q->r.p;
+    b.p = q->p.p;
+    b.ap = q->ap.p;
+    b.r64 = q->r64.p;
+    b.p64 = q->p64.p;
+    b.ap64 = q->ap64.p;
+    b.prec = q->prec;
+    b.n = q->n;
+    b.R = q->R;
+    b.slot = q->slot;
+    b.x_zero = 1;
+    return b;
+}
+
+// Autotuner of the batched product: for each system type (pattern, R,
+// iteration precision), the first time it is batched, the plain SELL-64
+// product and the one with LDS column tiles (when the pattern's tiles fit
+// kTileLdsCap at R values per column) are timed on that system alone (HIP
+// events, 2 + 5 launches each, its p as it stands) and the faster is kept.
+// Both give the same bits, so the choice changes speed only.
+static int tuned_tiling(ffh_server *S, const ServedRequest *q)
+{
+    Context *ctx = S->ctx;
+    DevMatrix &A = q->sym->reduced.m;
+    const auto key = std::make_tuple((const Problem *)q->sym, q->R, q->prec);
+    auto it = S->tiling.find(key);
+    if (it != S->tiling.end()) return it->second;
+    int pick = 0;
+    ensure_sell_tiles(ctx, A);
+    const size_t sv = q->prec ? sizeof(double) : sizeof(float);
+    const bool fits = A.tiles_ok && A.tile_max > 0 && (size_t)A.tile_max * q->R * sv <= kTileLdsCap;
+    if (fits && S->tiling_mode >= 0) {
+        pick = S->tiling_mode == 1 ? A.tile_max : 0;
+    } else if (fits) {
+        hipStream_t s = ctx->stream;
+        BatchSys b = batch_sys(q);
+        b.slot = 0;
+        std::vector<BatchSys> sys;
+        std::vector<int2> work;
+        BatchEngine::append(b, sys, work);
+        std::vector<BatchScal> sc(kMaxRhs, BatchScal{});
+        for (int r = 0; r < kMaxRhs; ++r) sc[r].state = r < q->R ? kActive : kUnused;
+        DBuf<BatchSys> d_sys;
+        DBuf<int2> d_work;
+        DBuf<BatchScal> d_sc;
+        DBuf<double> d_part;
+        d_work.upload(work, s);
+        d_sc.upload(sc, s);
+        d_part.alloc(work.size() * kMaxRhs);
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        FFH_HIP(hipEventCreate(&e0));
+        float ms[2] = {0.0f, 0.0f};
+        try {
+            FFH_HIP(hipEventCreate(&e1));
+            for (int v = 0; v < 2; ++v) {
+                sys[0].tiled = v ? A.tile_max : 0;
+                d_sys.upload(sys, s);
+                const size_t lds = tile_lds_bytes(sys[0]);
+                for (int k = 0; k < 7; ++k) {
+                    if (k == 2) FFH_HIP(hipEventRecord(e0, s));
+                    hipLaunchKernelGGL(k_b_spmv, dim3((unsigned)work.size()), dim3(256), lds, s,
+                                       d_work.p, d_sys.p, d_sc.p, d_part.p);
+                }
+                FFH_HIP(hipGetLastError());
+                FFH_HIP(hipEventRecord(e1, s));
+                FFH_HIP(hipEventSynchronize(e1));
+                FFH_HIP(hipEventElapsedTime(&ms[v], e0, e1));
+            }
+        } catch (...) {
+            (void)hipEventDestroy(e0);
+            if (e1) (void)hipEventDestroy(e1);
+            throw;
+        }
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        if (ms[1] < 0.98f * ms[0]) pick = A.tile_max;
+    }
+    S->tiling[key] = pick;
+    return pick;
+}
+
 // Systems go to the group with the least work so far (largest first), the
 // work of a system being its SELL entries times (1 + R / 4).
 static void rebuild_batch(ffh_server *S)
@@ -1052,29 +1254,8 @@ static void rebuild_batch(ffh_server *S)
     std::stable_sort(order.begin(), order.end(),
                      [&](const ServedRequest *a, const ServedRequest *b) { return cost(a) > cost(b); });
     for (ServedRequest *q : order) {
-        const DevMatrix &A = q->sym->reduced.m;
-        BatchSys b{};
-        b.off = A.sell_off.p;
-        b.sidx = A.sell_idx.p;
-        b.srow = A.sell_row.p;
-        b.nslice = A.nslice;
-        b.val32 = q->val32.p;
-        b.val64 = q->val64.p;
-        b.dinv32 = q->dinv32.p;
-        b.dinv64 = q->dinv.p;
-        b.b = q->b.p;
-        b.x = q->x.p;
-        b.r = q->r.p;
-        b.p = q->p.p;
-        b.ap = q->ap.p;
-        b.r64 = q->r64.p;
-        b.p64 = q->p64.p;
-        b.ap64 = q->ap64.p;
-        b.prec = q->prec;
-        b.n = q->n;
-        b.R = q->R;
-        b.slot = q->slot;
-        b.x_zero = 1;
+        BatchSys b = batch_sys(q);
+        b.tiled = tuned_tiling(S, q);
         const int g = (int)(std::min_element(load, load + G) - load);
         load[g] += cost(q);
         BatchEngine::append(b, sys[g], work[g]);
@@ -1394,6 +1575,29 @@ int ffh_server_use_graph(ffh_server *S, int enable)
 {
     if (!S) return FFH_ERR_HANDLE;
     S->use_graph = enable != 0;
+    return FFH_OK;
+}
+
+int ffh_server_set_tiling(ffh_server *S, int mode)
+{
+    if (!S) return FFH_ERR_HANDLE;
+    if (mode < -1 || mode > 1) return FFH_ERR_INVALID;
+    S->tiling_mode = mode;
+    S->tiling.clear();
+    S->changed = true;
+    return FFH_OK;
+}
+
+int ffh_server_tiling_stats(const ffh_server *S, int *types, int *tiled)
+{
+    if (!S) return FFH_ERR_HANDLE;
+    int n = 0, t = 0;
+    for (const auto &kv : S->tiling) {
+        ++n;
+        t += kv.second > 0;
+    }
+    if (types) *types = n;
+    if (tiled) *tiled = t;
     return FFH_OK;
 }
 

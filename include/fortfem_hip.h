@@ -404,6 +404,18 @@ int ffh_server_set_precision(ffh_server *server, int policy, int64_t *switches);
  * (enable != 0), or launch the kernels one by one (0, the default).  The
  * results are the same either way.  No reference counterpart. */
 int ffh_server_use_graph(ffh_server *server, int enable);
+/* Batched SpMV of each system type (pattern, right-hand sides, iteration
+ * precision): mode -1 (the default) times, the first time a type is batched,
+ * the plain SELL-64 product against the one that stages each item's column
+ * tile of p in LDS (offered when the pattern's tiles fit 64 KiB at R values
+ * per column) on that system alone, and keeps the faster; 0 always plain; 1
+ * tiled wherever the tiles fit.  Both products give the same bits.  Applies
+ * to types chosen from now on.  No reference counterpart (a tuning
+ * control). */
+int ffh_server_set_tiling(ffh_server *server, int mode);
+/* System types chosen so far (*types) and how many of them run the tiled
+ * product (*tiled). */
+int ffh_server_tiling_stats(const ffh_server *server, int *types, int *tiled);
 
 /* ---- differentiable variants --------------------------------------------- */
 /* spmv_jvp (src/solvers/fortfem_sparse_matrix.f90:60-91): y_dot = A x_dot +

@@ -131,6 +131,31 @@ def test_graph_replay_matches_direct_launches(ff):
         assert ca.all()
 
 
+def test_tiled_product_matches_plain(ff):
+    """The batched SpMV with each item's column tile of p staged in LDS adds
+    every row's products in the order of the plain SELL-64 product, so a
+    server forced to the tiled product (wherever the tiles fit), one forced
+    to the plain product and one left to the autotuner return bit-for-bit
+    the same solutions, iteration counts and residuals.  The autotuner
+    records one choice per system type (pattern, right-hand sides, iteration
+    precision)."""
+    outs, stats = [], []
+    for mode in (0, 1, -1):
+        srv = ff.Server(budget_bytes=1 << 30, max_systems=16, chunk=8)
+        srv.set_tiling(mode)
+        reqs = make_requests(ff, srv, seed=7)
+        tickets = [srv.submit(r["key"], r["fam"], r["k"], r["coeff"], r["b"], mass=r["mass"]) for r in reqs]
+        outs.append(srv.run_until_done(tickets, max_steps=20000))
+        stats.append(srv.tiling_stats())
+    for other in outs[1:]:
+        assert outs[0].keys() == other.keys()
+        for t in outs[0]:
+            for u, w in zip(outs[0][t], other[t]):
+                assert np.array_equal(u, w)
+    assert stats[0][1] == 0 and stats[1][1] >= 1  # plain: none tiled; forced: tiles fit somewhere
+    assert stats[2][0] == len(CASES) and 0 <= stats[2][1] <= stats[2][0]
+
+
 def test_admission_by_memory(ff):
     v, t = O.unit_box_mesh(4)
     srv = ff.Server(budget_bytes=1 << 30, max_systems=8, chunk=8)
