@@ -123,6 +123,22 @@ struct NoHook {
     __device__ __forceinline__ bool operator()() const { return true; }
 };
 
+// start hook that ends the launch once the solve has stopped: the control
+// block is read after the block's first matrix loads are issued, so its
+// latency overlaps them (*stop records that the launch ended here)
+struct DoneHook {
+    const KrylovCtl *ctl;
+    bool *stop;
+    __device__ __forceinline__ bool operator()() const
+    {
+        if (ctl->done) {
+            *stop = true;
+            return false;
+        }
+        return true;
+    }
+};
+
 // ------------------------------------------------------------ CSR stream --
 // A row block is a run of consecutive rows whose nonzeros fit in LDS.  The
 // block streams the run's values and column indices with fully coalesced
@@ -631,8 +647,6 @@ __global__ __launch_bounds__(kBlock) void k_pcg_dir(int n, const VT *__restrict_
                                                     const double *__restrict__ partials,
                                                     int stride, KrylovCtl *ctl)
 {
-    if (ctl->done) return;
-    const int restart = ctl->restart;
     const int i0 = xcd_block() * kBlock + threadIdx.x;
     const int step = gridDim.x * kBlock;
     VT zk[kDirRows], pk[kDirRows];
@@ -642,6 +656,9 @@ __global__ __launch_bounds__(kBlock) void k_pcg_dir(int n, const VT *__restrict_
         zk[k] = i < n ? z[i] : VT(0);
         pk[k] = i < n ? p[i] : VT(0);
     }
+    // the control block is read with the rows in flight
+    if (ctl->done) return;
+    const int restart = ctl->restart;
     double beta_d;
     if (pcg_finish<kBlock>(partials, stride, ctl, beta_d)) return;
     const VT beta = (VT)beta_d;
@@ -661,10 +678,12 @@ __global__ __launch_bounds__(kStreamBlock) void k_pcg_apdot(
     const VT *__restrict__ p, VT *__restrict__ ap, const int *__restrict__ rowblocks,
     int nrb, double *__restrict__ partials, const KrylovCtl *ctl)
 {
-    if (ctl->done) return;
     EpiApDot<VT> epi{p, ap};
     double d[3] = {0.0, 0.0, 0.0};
-    row_engine<V, MT, VT>(n, ptr, idx, val, XPlain<VT>{p}, rowblocks, nrb, epi, d);
+    bool stop = false;
+    row_engine<V, MT, VT>(n, ptr, idx, val, XPlain<VT>{p}, rowblocks, nrb, epi, d,
+                          DoneHook{ctl, &stop});
+    if (stop) return;
     const double s0 = block_sum<kStreamBlock>(d[0]);
     if (threadIdx.x == 0) partials[blockIdx.x] = s0;
 }
@@ -704,9 +723,8 @@ __global__ __launch_bounds__(kBlock) void k_pcg_update3(
     const VT *__restrict__ ap, const VT *__restrict__ dinv, double *__restrict__ partials,
     int stride, int nb_spmv, KrylovCtl *ctl)
 {
-    if (ctl->done) return;
-    const int cur = ctl->cur;
-    // this thread's first kDirRows rows, loaded while the partials are summed
+    // this thread's first kDirRows rows, loaded while the control block is
+    // read and the partials are summed
     const int i0 = xcd_block() * kBlock + threadIdx.x;
     const int step = gridDim.x * kBlock;
     XT xk[kDirRows];
@@ -721,6 +739,8 @@ __global__ __launch_bounds__(kBlock) void k_pcg_update3(
         dk[k] = in ? dinv[i] : VT(0);
         pk[k] = in ? p[i] : VT(0);
     }
+    if (ctl->done) return;
+    const int cur = ctl->cur;
     double alpha_d;
     if (!update_alpha(partials, nb_spmv, ctl, cur, alpha_d)) return;
     const VT alpha = (VT)alpha_d;
