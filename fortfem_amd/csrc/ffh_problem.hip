@@ -1378,6 +1378,9 @@ static void build_h1_gather(Problem *P, const std::vector<int> &gd,
     P->gat_ok = true;
 }
 
+// meshes from this many cells are coloured on the device (see below)
+constexpr int64_t kDeviceColourCells = 1 << 20;
+
 void problem_pattern(Problem *P, int family, int order)
 {
     Context *ctx = P->ctx;
@@ -1460,9 +1463,18 @@ void problem_pattern(Problem *P, int family, int order)
     P->n_local = nl;
     P->n_global = G;
     std::vector<int> cstart, perm;
-    // colouring on the device (Jones-Plassmann, first fit), or the host's
-    // greedy first fit in cell order for meshes past the device budget
-    if (dev && device_colour(ctx, T, inc_m, d_inc.p, inc_base, n_ent, cstart, perm))
+    // Colouring: the host's greedy first fit in cell order, or on the device
+    // (Jones-Plassmann rounds, first fit) for meshes of kDeviceColourCells
+    // cells and more.  Greedy in cell order finds fewer colours on ordered
+    // meshes (box meshes: 6 against 12-13 for Nedelec edges, 32 against 40
+    // for P1 vertices) and costs under ~20 ms below that size; the device
+    // rounds take about half the host time at 12.6 M cells (145 ms against
+    // 300 ms).  FFH_COLOUR=device or host forces either.
+    const char *cenv = std::getenv("FFH_COLOUR");
+    const bool force_dev = cenv && std::string(cenv) == "device";
+    const bool force_host = cenv && std::string(cenv) == "host";
+    const bool use_dev = dev && !force_host && (force_dev || (int64_t)T >= kDeviceColourCells);
+    if (use_dev && device_colour(ctx, T, inc_m, d_inc.p, inc_base, n_ent, cstart, perm))
         P->n_colors = (int)cstart.size() - 1;
     else
         P->n_colors = color_cells(G, T, nl, gd, cstart, perm);

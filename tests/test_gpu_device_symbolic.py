@@ -105,24 +105,31 @@ def _neighbour_degree(inc):
 
 @pytest.mark.parametrize("family,order", [(1, 1), (1, 3), (2, 1), (2, 3)])
 def test_device_colouring_properties(ff, family, order):
-    """The device colouring (Jones-Plassmann rounds, first-fit colours) on a
-    scrambled mesh: a proper colouring of the dof-sharing graph (the
-    validator), cells ascending inside each colour, the same colouring from a
-    second pattern build, and no more colours than the first-fit bound of
+    """The device colouring (Jones-Plassmann rounds, first-fit colours, forced
+    by FFH_COLOUR=device on this small mesh) on a scrambled mesh: a proper
+    colouring of the dof-sharing graph (the validator), cells ascending
+    inside each colour, the same colouring from a second pattern build, and
+    no more colours than the first-fit bound of
     one plus the largest number of cells a cell conflicts with (vertex
     sharing for H1, edge sharing for Nedelec).  Nedelec cells are further
     grouped by orientation class inside a colour (ascending inside each
     class), so a colour's list falls at most 23 times there."""
+    import os
+
     v, t = scrambled(5, 40 + 10 * family + order)
-    p = ff.Problem(v, t)
-    p.pattern_build(family, order)
+    os.environ["FFH_COLOUR"] = "device"  # this mesh is below the device-colouring size
+    try:
+        p = ff.Problem(v, t)
+        p.pattern_build(family, order)
+        q = ff.Problem(v, t)
+        q.pattern_build(family, order)
+    finally:
+        del os.environ["FFH_COLOUR"]
     nc = assert_valid_colouring(p)
     cs, cells = p.colouring(base=0)
     for c in range(nc):
         falls = int(np.sum(np.diff(cells[cs[c]:cs[c + 1]]) < 0))
         assert falls == 0 if family == 1 else falls <= 23
-    q = ff.Problem(v, t)
-    q.pattern_build(family, order)
     cs2, cells2 = q.colouring(base=0)
     assert np.array_equal(cs, cs2) and np.array_equal(cells, cells2)
     if family == 1:
