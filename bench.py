@@ -40,7 +40,9 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 
 
+# the roofline kernel: the FP32 SpMV launch of the mixed solve's iterations
 ROOFLINE_KERNEL = {"c4": "k_pcg_apdot", "c2": "k_pcg_apdot"}
+ROOFLINE_TYPES = "float, float>"
 
 
 def _pmc_pass(counter, cmd, tmp, kernel_regex=None, timeout=240):
@@ -97,7 +99,8 @@ def measure_traffic(workload: str):
         cw = _pmc_pass("WRITE_SIZE", ["--workload", "copy", "--steps", "3"], tmp)
 
         def live(res):  # the roofline kernel's launches that did work (> 5 us)
-            vals = [v for k, vs in res.items() if kname in k for v, d in vs if d > 5.0]
+            vals = [v for k, vs in res.items() if kname in k and ROOFLINE_TYPES in k
+                    for v, d in vs if d > 5.0]
             if not vals:
                 raise RuntimeError(f"no live {kname} launch in the counter output")
             return sum(vals) / len(vals)
@@ -112,7 +115,8 @@ def measure_traffic(workload: str):
         f_scale, w_scale = gib_kib / copy_value(cf), gib_kib / copy_value(cw)
         fk, wk = live(kf), live(kw)
         traffic = (f_scale * fk + w_scale * wk) * 1024.0
-        return traffic, (f"measured in this run: {kname} {f_scale:.2f} x FETCH_SIZE {fk:,.1f} KiB + "
+        return traffic, (f"measured in this run: {kname}<.., {ROOFLINE_TYPES} {f_scale:.2f} x FETCH_SIZE "
+                         f"{fk:,.1f} KiB + "
                          f"{w_scale:.2f} x WRITE_SIZE {wk:,.1f} KiB per live launch (rocprofv3 --pmc "
                          f"over bench.py --workload {workload} --steps 1 --warmup 0; scale factors from "
                          f"a 1 GiB device copy, bench.py --workload copy)")

@@ -1516,21 +1516,20 @@ void spmv32(Context *ctx, DevMatrix &A, const float *x, float *y)
     launch_spmv<float, float>(ctx, A, x, y, vi);
 }
 
-// The form of a Jacobi-PCG iteration: three launches (k_pcg_dir,
-// k_pcg_apdot with the plain SpMV variant, k_pcg_update) for FP32 vectors
-// and for FP64 systems below kThreeLaunchRows rows, else two (k_pcg_dirspmv,
-// k_pcg_update), the faster form of each on the P2 32^3 x 6 and P1 128^3 x 6
-// Poisson systems; FFH_PCG_LAUNCHES=2 or 3 forces a form.
-constexpr int kThreeLaunchRows = 1 << 20;
-
+// The form of a Jacobi-PCG iteration: three launches on dense z and p
+// (k_pcg_dir, k_pcg_apdot with the plain SpMV variant, k_pcg_update3), the
+// faster form in FP32 and FP64 on the P2 32^3 x 6 and P1 128^3 x 6 Poisson
+// systems; FFH_PCG_LAUNCHES=2 (read once per process) selects the two-launch
+// form on (z, p) pairs (k_pcg_dirspmv, k_pcg_update) instead.
 bool three_launch(int64_t n, bool f32)
 {
-    static const int forced = [] {
+    (void)n;
+    (void)f32;
+    static const bool two = [] {
         const char *e = std::getenv("FFH_PCG_LAUNCHES");
-        return e ? std::atoi(e) : 0;
+        return e && std::atoi(e) == 2;
     }();
-    if (forced == 2 || forced == 3) return forced == 3;
-    return f32 || n < kThreeLaunchRows;
+    return !two;
 }
 
 // `iters` iterations of Jacobi-PCG (matrix MT, vectors VT, x in XT).  Two

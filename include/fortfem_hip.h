@@ -457,9 +457,10 @@ int ffh_bench_spmv(ffh_matrix *mat, int precision, int reps, double *avg_ms,
 int ffh_matrix_spmv_bytes(ffh_matrix *mat, int precision, double *bytes);
 /* Launches per Jacobi-PCG iteration of a solve with n rows in precision
  * FFH_FP32 (the mixed solve's iterations) or FFH_FP64: 3 (direction update,
- * SpMV with A p and p.A p partials, x/r/z update) or 2 (direction formed in
- * the SpMV's gathers, then the update).  No reference counterpart: it says
- * which kernels a solve runs.  Returns 2 or 3, or FFH_ERR_INVALID. */
+ * SpMV with A p and p.A p partials, x/r/z update), or 2 (direction formed in
+ * the SpMV's gathers, then the update) in a process started with
+ * FFH_PCG_LAUNCHES=2.  No reference counterpart: it says which kernels a
+ * solve runs.  Returns 2 or 3, or FFH_ERR_INVALID. */
 int ffh_pcg_launches_per_iteration(int64_t n, int precision);
 /* SpMV variant of a matrix (no reference counterpart: a tuning control).
  * request >= 0 forces variant `request` (0 .. FFH_SPMV_VARIANTS-1) for the

@@ -70,17 +70,15 @@ def test_host_mesh_entry_points_match_oracle():
 
 
 def test_pcg_launch_form_rule():
-    """Three launches per iteration for FP32 iterations and for FP64 systems
-    below 2^20 rows, two for larger FP64 systems; a host-side rule (no GPU)."""
+    """Three launches per iteration in both precisions at every size unless
+    the process runs with FFH_PCG_LAUNCHES=2; a host-side rule (no GPU)."""
     import fortfem_amd as ff
 
     if os.environ.get("FFH_PCG_LAUNCHES"):
         pytest.skip("launch form forced by FFH_PCG_LAUNCHES")
-    assert ff.pcg_launches_per_iteration(274625, "fp64") == 3
-    assert ff.pcg_launches_per_iteration((1 << 20) - 1, "fp64") == 3
-    assert ff.pcg_launches_per_iteration(1 << 20, "fp64") == 2
-    assert ff.pcg_launches_per_iteration(2048383, "fp64") == 2
-    assert ff.pcg_launches_per_iteration(2048383, "fp32") == 3
+    for n in (274625, (1 << 20) - 1, 1 << 20, 2048383):
+        assert ff.pcg_launches_per_iteration(n, "fp64") == 3
+        assert ff.pcg_launches_per_iteration(n, "fp32") == 3
     lib = ff.capi.load()
     assert lib.ffh_pcg_launches_per_iteration(-1, ff.capi.FP64) == ff.capi.FFH_ERR_INVALID
     assert lib.ffh_pcg_launches_per_iteration(10, 7) == ff.capi.FFH_ERR_INVALID
