@@ -450,7 +450,7 @@ def host_description():
     return {"nproc": usable, "cpu_count": os.cpu_count(), "model": model}
 
 
-def cpu_baseline(ff, p, meta, rhs, x_gpu, threads: int, reps: int):
+def cpu_baseline(ff, p, meta, rhs, x_gpu, threads: int, reps: int, one_core: bool = True):
     """The oracle's restatement of the reference algorithm (C, OpenMP) timed
     on the full configuration on `threads` host threads: one step = the
     triplet assembly (the reference's quadrature-point element loop) +
@@ -460,7 +460,8 @@ def cpu_baseline(ff, p, meta, rhs, x_gpu, threads: int, reps: int):
     from the GPU problem, whose dof map and boundary equal the oracle's bit
     for bit).  One warm-up, then `reps` timed steps: median and p99.  The
     FP64 solution of the last step gives the GPU mixed solution's error
-    against the oracle."""
+    against the oracle.  With one_core, one more step on a single thread
+    (the reference's own loops are serial), timed once after the others."""
     import oracle as O
     from fortfem_amd import workloads
 
@@ -472,7 +473,7 @@ def cpu_baseline(ff, p, meta, rhs, x_gpu, threads: int, reps: int):
     G = int(g.max())
     sin_f = lambda x, y, z: np.sin(np.pi * x) * np.sin(np.pi * y) * np.sin(np.pi * z)
 
-    def step():
+    def step(threads=threads):
         a = O.par_lagrange_stiffness_csc(v, t, d, q, g, G, threads, cell_kappa=kappa)
         b = O.lagrange_load(v, t, d, q, sin_f, g, G) if rhs is None else np.zeros(G)
         red, free, rb = O.constrained_reduction(a, cons, np.zeros(G), b)
@@ -494,6 +495,15 @@ def cpu_baseline(ff, p, meta, rhs, x_gpu, threads: int, reps: int):
     med = float(np.median(times))
     err = float(np.linalg.norm(x_gpu[free - 1] - x) / np.linalg.norm(x))
     host = host_description()
+    single = None
+    if one_core and threads > 1:
+        t0 = time.perf_counter()
+        step(1)
+        t1 = time.perf_counter() - t0
+        print(json.dumps({"progress": "cpu baseline, one thread: %.2f s" % t1}), file=sys.stderr, flush=True)
+        single = {"value": round(1.0 / t1, 5), "unit": "solves/s", "cores": 1, "s_per_step": round(t1, 3),
+                  "sample": "one timed step of the same full configuration on one thread, after the "
+                            f"{threads}-thread steps"}
     return {
         "value": round(1.0 / med, 4),
         "unit": "solves/s",
@@ -509,6 +519,7 @@ def cpu_baseline(ff, p, meta, rhs, x_gpu, threads: int, reps: int):
         "host": host,
         "compiler": "gcc -O2 -ffp-contract=off -fopenmp",
         "gpu_mixed_rel_err_vs_oracle_fp64": err,
+        "one_core": single,
     }
 
 
@@ -574,6 +585,8 @@ def main():
                     help="skip the rocprofv3 --pmc passes that measure roofline.traffic")
     ap.add_argument("--cpu-threads", type=int, default=0, help="CPU baseline threads (0: min(16, nproc))")
     ap.add_argument("--cpu-reps", type=int, default=5, help="CPU baseline timed steps (after 1 warm-up)")
+    ap.add_argument("--cpu-one-core", type=int, default=1,
+                    help="1 (default): also time one CPU baseline step on a single thread")
     ap.add_argument("--graph", action="store_true", help="replay iteration chunks from HIP graphs")
     ap.add_argument("--point-tensors", action="store_true",
                     help="c3: also time the assembly with the tensor evaluated at every rule point (MFMA)")
@@ -739,7 +752,8 @@ def main():
         out["served"] = served_stream(args)
     if not args.no_cpu:
         threads = args.cpu_threads or min(16, host_description()["nproc"])
-        out["cpu_baseline"] = cpu_baseline(ff, p, meta, rhs, xm, threads, args.cpu_reps)
+        out["cpu_baseline"] = cpu_baseline(ff, p, meta, rhs, xm, threads, args.cpu_reps,
+                                           one_core=bool(args.cpu_one_core))
     print(json.dumps(out))
 
 
