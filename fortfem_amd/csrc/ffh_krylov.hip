@@ -21,6 +21,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdlib>
+#include <map>
 #include <mutex>
 
 #include "ffh_internal.hpp"
@@ -346,21 +347,50 @@ __device__ __forceinline__ void sell_tiled(const int *__restrict__ off,
     const int *tcols = toff + ngroup + 1;
     const int lane = threadIdx.x & 63;
     bool hooked = false;
+    // the tile is staged SB columns per thread at a time: the batch's
+    // column-list loads, then its vector gathers, then its LDS stores, so a
+    // thread has SB gathers in flight instead of one
+    constexpr int SB = 2;
     for (int g = xcd_block(); g < ngroup || !hooked; g += gridDim.x) {
         const bool live = g < ngroup;
         const int c0 = live ? toff[g] : 0;
         const int nc = live ? toff[g + 1] - c0 : 0;
-        const bool first = (int)threadIdx.x < nc;
-        typename XF::Raw q0{};
-        if (first) q0 = xf.raw(tcols[c0 + threadIdx.x]);
-        // the first column loads are in flight during the hook
+        int cc[SB];
+        typename XF::Raw qq[SB];
+#pragma unroll
+        for (int u = 0; u < SB; ++u) {
+            const int k = (int)threadIdx.x + u * kStreamBlock;
+            cc[u] = k < nc ? tcols[c0 + k] : 0;
+        }
+#pragma unroll
+        for (int u = 0; u < SB; ++u)
+            if ((int)threadIdx.x + u * kStreamBlock < nc) qq[u] = xf.raw(cc[u]);
+        // the first batch's loads are in flight during the hook
         if (!hooked) {
             if (!hook()) return;
             hooked = true;
         }
         if (!live) break;
-        if (first) lds[threadIdx.x] = xf.val(q0);
-        for (int k = threadIdx.x + kStreamBlock; k < nc; k += kStreamBlock) lds[k] = xf(tcols[c0 + k]);
+#pragma unroll
+        for (int u = 0; u < SB; ++u) {
+            const int k = (int)threadIdx.x + u * kStreamBlock;
+            if (k < nc) lds[k] = xf.val(qq[u]);
+        }
+        for (int k0 = (int)threadIdx.x + SB * kStreamBlock; k0 < nc; k0 += SB * kStreamBlock) {
+#pragma unroll
+            for (int u = 0; u < SB; ++u) {
+                const int k = k0 + u * kStreamBlock;
+                cc[u] = k < nc ? tcols[c0 + k] : 0;
+            }
+#pragma unroll
+            for (int u = 0; u < SB; ++u)
+                if (k0 + u * kStreamBlock < nc) qq[u] = xf.raw(cc[u]);
+#pragma unroll
+            for (int u = 0; u < SB; ++u) {
+                const int k = k0 + u * kStreamBlock;
+                if (k < nc) lds[k] = xf.val(qq[u]);
+            }
+        }
         __syncthreads();
         const int sl = 4 * g + (int)(threadIdx.x >> 6);
         if (sl < nslice) {
@@ -1061,6 +1091,32 @@ static size_t variant_lds(const DevMatrix &A, int vi, size_t vbytes)
     return kVariants[vi].engine == 2 ? (size_t)A.tile_max * vbytes : 0;
 }
 
+// The grid of a grid-stride row-engine launch, capped at the blocks of that
+// kernel (at its register and LDS use) that fit on the device at once: a
+// grid past that runs its last blocks after the others, so every block's
+// share of the rows would wait on a second round (occupancy per kernel and
+// dynamic LDS size, cached).
+static int resident_grid(const Context *ctx, const void *kernel, int grid, size_t lds)
+{
+    static std::mutex mu;
+    static std::map<std::pair<const void *, size_t>, int> cache;
+    int per_cu = 0;
+    {
+        std::lock_guard<std::mutex> lock(mu);
+        auto it = cache.find({kernel, lds});
+        if (it != cache.end()) {
+            per_cu = it->second;
+        } else {
+            if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, kStreamBlock, lds) !=
+                hipSuccess)
+                per_cu = 0;
+            cache[{kernel, lds}] = per_cu;
+        }
+    }
+    if (per_cu <= 0) return grid;
+    return std::max(1, std::min(grid, per_cu * ctx->cu_count));
+}
+
 // Gather arrays of a launch: the CSR (CSC read as CSR) arrays with the row
 // blocks of the stream engine, or the SELL-64 arrays with the slice count.
 template <typename MT>
@@ -1386,20 +1442,22 @@ static int ready_variant(Context *ctx, DevMatrix &A, int vi, bool f32)
 template <typename MT, typename VT>
 static void launch_spmv(Context *ctx, const DevMatrix &A, const VT *x, VT *y, int vi)
 {
-    const int grid = variant_grid(ctx, A, vi);
     const GArgs<MT> g = gargs<MT>(A, vi);
+    const size_t lds = variant_lds(A, vi, sizeof(VT));
     FFH_ENGINE(kVariants[vi].engine, {
-        hipLaunchKernelGGL((k_spmv<VV, MT, VT>), dim3(grid), dim3(kStreamBlock),
-                           variant_lds(A, vi, sizeof(VT)),
+        const int grid = resident_grid(ctx, (const void *)k_spmv<VV, MT, VT>,
+                                       variant_grid(ctx, A, vi), lds);
+        hipLaunchKernelGGL((k_spmv<VV, MT, VT>), dim3(grid), dim3(kStreamBlock), lds,
                            ctx->stream, A.n, g.ptr, g.idx, g.val, x, y, g.rb, g.nrb);
     });
     FFH_HIP(hipGetLastError());
 }
 
 // Autotuner: the fastest SpMV variant for this matrix and precision, timed
-// with events of its own (10 launches each after a warm-up) on the context
-// stream, into scratch vectors of its own, so that no solver vector or timing
-// event is touched.  FFH_SPMV_VARIANT=<index> forces a variant.
+// with events of its own (10 launches each after a warm-up of the solver's
+// SpMV launch) on the context stream, into scratch vectors, partials and
+// control block of its own, so that no solver vector or timing event is
+// touched.  FFH_SPMV_VARIANT=<index> forces a variant.
 // best_dir receives the fastest variant that gathers from L2 (not the tiled
 // engine): the CG direction launch forms each gathered column's direction on
 // the fly, which the tiled engine's staging serialises behind its start
@@ -1418,6 +1476,26 @@ static int tune_spmv(Context *ctx, DevMatrix &A, int &best_dir)
     tx.alloc(A.n);
     ty.alloc(A.n);
     FFH_HIP(hipMemsetAsync(tx.p, 0, sizeof(VT) * A.n, ctx->stream));
+    // the launch timed is the solver's own SpMV launch (k_pcg_apdot: the
+    // product, its p.Ap partials and its start hook), whose register use,
+    // and so occupancy, can differ from the plain product's
+    DBuf<double> part;
+    part.alloc((size_t)grid_cap(ctx) * 8);
+    DBuf<KrylovCtl> tctl;
+    tctl.alloc(1);
+    FFH_HIP(hipMemsetAsync(tctl.p, 0, sizeof(KrylovCtl), ctx->stream));
+    auto launch = [&](int vi) {
+        const GArgs<MT> g = gargs<MT>(A, vi);
+        const size_t lds = variant_lds(A, vi, sizeof(VT));
+        FFH_ENGINE(kVariants[vi].engine, {
+            const int grid = resident_grid(ctx, (const void *)k_pcg_apdot<VV, MT, VT>,
+                                           variant_grid(ctx, A, vi), lds);
+            hipLaunchKernelGGL((k_pcg_apdot<VV, MT, VT>), dim3(grid), dim3(kStreamBlock), lds,
+                               ctx->stream, A.n, g.ptr, g.idx, g.val, tx.p, ty.p, g.rb, g.nrb,
+                               part.p, tctl.p);
+        });
+        FFH_HIP(hipGetLastError());
+    };
     hipEvent_t e0 = nullptr, e1 = nullptr;
     FFH_HIP(hipEventCreate(&e0));
     int best = 0;
@@ -1431,9 +1509,9 @@ static int tune_spmv(Context *ctx, DevMatrix &A, int &best_dir)
             if ((kVariants[vi].engine == 1 || kVariants[vi].engine == 2) &&
                 ready_variant(ctx, A, vi, f32) != vi)
                 continue; // no SELL copy (or no column tiles) of this matrix
-            launch_spmv<MT, VT>(ctx, A, tx.p, ty.p, vi);
+            launch(vi);
             FFH_HIP(hipEventRecord(e0, ctx->stream));
-            for (int r = 0; r < 10; ++r) launch_spmv<MT, VT>(ctx, A, tx.p, ty.p, vi);
+            for (int r = 0; r < 10; ++r) launch(vi);
             FFH_HIP(hipEventRecord(e1, ctx->stream));
             FFH_HIP(hipEventSynchronize(e1));
             float ms = 0.0f;
@@ -1550,7 +1628,11 @@ static void launch_iterations(Context *ctx, const DevMatrix &A, double *partials
         const int gridv = grid_for(ctx, n, kBlock);
         const int gridd = (int)std::min<int64_t>(
             gridv, std::max<int64_t>(ctx->cu_count, ((int64_t)n + 4 * kBlock - 1) / (4 * kBlock)));
-        const int grids = variant_grid(ctx, A, vi_sp);
+        int grids = variant_grid(ctx, A, vi_sp);
+        FFH_ENGINE(kVariants[vi_sp].engine, {
+            grids = resident_grid(ctx, (const void *)k_pcg_apdot<VV, MT, VT>, grids,
+                                  variant_lds(A, vi_sp, sizeof(VT)));
+        });
         const int stride = partial_stride(ctx);
         const GArgs<MT> g = gargs<MT>(A, vi_sp);
         for (int it = 0; it < iters; ++it) {
@@ -1575,7 +1657,11 @@ static void launch_iterations(Context *ctx, const DevMatrix &A, double *partials
     const int gridv = (int)std::min<int64_t>(
         grid_for(ctx, n, kBlock),
         std::max<int64_t>(ctx->cu_count, ((int64_t)n + 4 * kBlock - 1) / (4 * kBlock)));
-    const int grids = variant_grid(ctx, A, vi);
+    int grids = variant_grid(ctx, A, vi);
+    FFH_ENGINE(kVariants[vi].engine, {
+        grids = resident_grid(ctx, (const void *)k_pcg_dirspmv<VV, MT, VT>, grids,
+                              variant_lds(A, vi, sizeof(VT)));
+    });
     const int stride = partial_stride(ctx);
     const GArgs<MT> g = gargs<MT>(A, vi);
     for (int it = 0; it < iters; ++it) {
@@ -1598,7 +1684,11 @@ static void launch_refresh(Context *ctx, Matrix &M, int vi64, int first)
 {
     const DevMatrix &A = M.m;
     Workspace &w = M.ws;
-    const int gr = variant_grid(ctx, A, vi64);
+    int gr = variant_grid(ctx, A, vi64);
+    FFH_ENGINE(kVariants[vi64].engine, {
+        gr = resident_grid(ctx, (const void *)k_mixed_refresh<VV>, gr,
+                           variant_lds(A, vi64, sizeof(double)));
+    });
     const int stride = partial_stride(ctx);
     const GArgs<double> g = gargs<double>(A, vi64);
     if (ctx->profiling) ctx->prof_mark(0, true);
@@ -1667,7 +1757,10 @@ static KrylovCtl run_chunks(Context *ctx, Matrix &M, int gslot, int64_t graph_ke
 static void launch_residual_on(Context *ctx, const DevMatrix &A, int vi, const double *x,
                                const double *b, double *r, double *partials, KrylovCtl *ctl)
 {
-    const int gr = variant_grid(ctx, A, vi);
+    int gr = variant_grid(ctx, A, vi);
+    FFH_ENGINE(kVariants[vi].engine, {
+        gr = resident_grid(ctx, (const void *)k_residual<VV>, gr, variant_lds(A, vi, sizeof(double)));
+    });
     const GArgs<double> g = gargs<double>(A, vi);
     FFH_ENGINE(kVariants[vi].engine, {
         hipLaunchKernelGGL((k_residual<VV>), dim3(gr), dim3(kStreamBlock),
@@ -1716,7 +1809,11 @@ static KrylovCtl pcg_fp64_on(Context *ctx, const DevMatrix &A, int vi, int vi_it
     init.tol_type = tol_type;
     init.tolerance = tolerance;
     hipLaunchKernelGGL(k_ctl_set, dim3(1), dim3(1), 0, s, w.ctl, init);
-    const int gi = variant_grid(ctx, A, vi);
+    int gi = variant_grid(ctx, A, vi);
+    FFH_ENGINE(kVariants[vi].engine, {
+        gi = resident_grid(ctx, (const void *)k_pcg_init<VV, double, double>, gi,
+                           variant_lds(A, vi, sizeof(double)));
+    });
     const int stride = partial_stride(ctx);
     const GArgs<double> g = gargs<double>(A, vi);
     FFH_ENGINE(kVariants[vi].engine, {
