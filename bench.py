@@ -60,12 +60,22 @@ def _pmc_pass(counter, cmd, tmp, kernel_regex=None, timeout=240):
             "-d", out, "-o", "p"]
     if kernel_regex:
         prof += ["--kernel-include-regex", kernel_regex]
+    import signal
+
     env = dict(os.environ, TMPDIR=tmp)
-    r = subprocess.run(prof + ["--", sys.executable, os.path.abspath(__file__)] + cmd, cwd=tmp,
-                       env=env, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL, timeout=timeout)
+    # its own process group, so that a time-out ends the profiled child too
+    proc = subprocess.Popen(prof + ["--", sys.executable, os.path.abspath(__file__)] + cmd, cwd=tmp,
+                            env=env, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL,
+                            start_new_session=True)
+    try:
+        rc = proc.wait(timeout=timeout)
+    except subprocess.TimeoutExpired:
+        os.killpg(proc.pid, signal.SIGKILL)
+        proc.wait()
+        raise RuntimeError(f"rocprofv3 --pmc {counter} did not finish in {timeout} s")
     files = glob.glob(os.path.join(out, "**", "*counter_collection.csv"), recursive=True)
-    if r.returncode != 0 or not files:
-        raise RuntimeError(f"rocprofv3 --pmc {counter} exited {r.returncode}")
+    if rc != 0 or not files:
+        raise RuntimeError(f"rocprofv3 --pmc {counter} exited {rc}")
     res = {}
     for row in csv.DictReader(open(files[0])):
         if row["Counter_Name"] != counter:

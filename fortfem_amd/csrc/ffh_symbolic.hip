@@ -691,6 +691,8 @@ bool device_colour(Context *ctx, int64_t T, int m, const int *d_inc, int base, i
         perm.clear();
         return true;
     }
+    FFH_REQUIRE(m >= 1 && m <= kMaxInc, FFH_ERR_INVALID, "colouring: 1 to 6 entities per cell");
+    FFH_REQUIRE(T * m < INT_MAX, FFH_ERR_INVALID, "colouring: incidence lists exceed int32");
     const int64_t n = T * m;
     DBuf<int> cnt, ptr, cur, list, col, stat;
     cnt.alloc(n_ent + 1);
@@ -719,7 +721,6 @@ bool device_colour(Context *ctx, int64_t T, int m, const int *d_inc, int base, i
                        ptr.p, cur.p, list.p);
     FFH_HIP(hipGetLastError());
     // rounds in batches of 8 between checks of the coloured count
-    FFH_REQUIRE(m >= 1 && m <= kMaxInc, FFH_ERR_INVALID, "colouring: 1 to 6 entities per cell");
     FFH_REQUIRE((T + kSymBlock - 1) / kSymBlock <= INT_MAX, FFH_ERR_INVALID, "colouring: too many cells");
     const int grid = (int)((T + kSymBlock - 1) / kSymBlock);
     int64_t coloured = 0;
