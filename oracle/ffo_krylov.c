@@ -26,12 +26,15 @@ static double dot(int n, const double *a, const double *b)
 }
 
 /* Breakdown test.  Mode 0 is the reference's absolute |pAp| < 1e-14
- * (:768, :897).  Mode 1 is the scale-invariant variant |pAp| < 1e-14 |rz|
- * (alpha = rz/pAp > 1e14), an option added because the absolute test fires
- * on small-h meshes long before a 1e-10 relative residual is reached. */
+ * (:768, :897).  Mode 1 is the scale-invariant variant: breakdown unless
+ * |pAp| > 1e-14 |rz| (alpha = rz/pAp < 1e14), an option added because the
+ * absolute test fires on small-h meshes long before a 1e-10 relative
+ * residual is reached.  Written as a negated comparison so that pAp = rz = 0
+ * (a zero residual: the reference's absolute test exits there too) and a NaN
+ * count as breakdown. */
 static int breakdown(double pap, double rz, int mode)
 {
-    if (mode == 1) return fabs(pap) < 1.0e-14 * fabs(rz);
+    if (mode == 1) return !(fabs(pap) > 1.0e-14 * fabs(rz));
     return fabs(pap) < 1.0e-14;
 }
 
