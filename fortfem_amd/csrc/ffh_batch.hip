@@ -312,31 +312,25 @@ __device__ __forceinline__ void spmv_item(const BatchSys &s, int slice0, double 
 }
 
 // Ap = A p of every item with an active right-hand side; partials of p.Ap
-// (every per-item launch walks the work table with a grid of at most the
-// blocks that are resident at once, so the last items do not wait for a
-// second round of blocks; an item's arithmetic does not depend on the grid)
-__global__ __launch_bounds__(256) void k_b_spmv(const int2 *__restrict__ work, int nwork,
+__global__ __launch_bounds__(256) void k_b_spmv(const int2 *__restrict__ work,
                                                 const BatchSys *__restrict__ sys,
                                                 const BatchScal *__restrict__ sc,
                                                 double *__restrict__ part)
 {
     __shared__ double sh[4 * kMaxRhs];
     __shared__ ScalView sv;
-    for (int item = blockIdx.x; item < nwork; item += gridDim.x) {
-        __syncthreads(); // the previous item's readers of sv and of the tile are done
-        const int2 wk = work[item];
-        const BatchSys s = sys[wk.x];
-        load_scal(sc, s.slot, s.R, sv);
-        if (!sv.any_active) continue;
-        double *out = part + (size_t)item * kMaxRhs;
-        const int slice0 = wk.y / 64;
-        switch (s.R) {
-        case 1: spmv_item<1>(s, slice0, sh, out); break;
-        case 2: spmv_item<2>(s, slice0, sh, out); break;
-        case 4: spmv_item<4>(s, slice0, sh, out); break;
-        case 6: spmv_item<6>(s, slice0, sh, out); break;
-        default: spmv_item<8>(s, slice0, sh, out); break;
-        }
+    const int2 wk = work[blockIdx.x];
+    const BatchSys s = sys[wk.x];
+    load_scal(sc, s.slot, s.R, sv);
+    if (!sv.any_active) return;
+    double *out = part + (size_t)blockIdx.x * kMaxRhs;
+    const int slice0 = wk.y / 64;
+    switch (s.R) {
+    case 1: spmv_item<1>(s, slice0, sh, out); break;
+    case 2: spmv_item<2>(s, slice0, sh, out); break;
+    case 4: spmv_item<4>(s, slice0, sh, out); break;
+    case 6: spmv_item<6>(s, slice0, sh, out); break;
+    default: spmv_item<8>(s, slice0, sh, out); break;
     }
 }
 
@@ -459,7 +453,7 @@ __device__ __forceinline__ void update_rows(const BatchSys &s, int row0, const S
     block_sums<R>(crz, sh, o_rz);
 }
 
-__global__ __launch_bounds__(256) void k_b_update(const int2 *__restrict__ work, int nwork,
+__global__ __launch_bounds__(256) void k_b_update(const int2 *__restrict__ work,
                                                   const BatchSys *__restrict__ sys,
                                                   const BatchScal *__restrict__ sc,
                                                   double *__restrict__ part_rr,
@@ -467,21 +461,18 @@ __global__ __launch_bounds__(256) void k_b_update(const int2 *__restrict__ work,
 {
     __shared__ double sh[4 * kMaxRhs];
     __shared__ ScalView sv;
-    for (int item = blockIdx.x; item < nwork; item += gridDim.x) {
-        __syncthreads();
-        const int2 wk = work[item];
-        const BatchSys s = sys[wk.x];
-        load_scal(sc, s.slot, s.R, sv);
-        if (!sv.any_active) continue;
-        double *o_rr = part_rr + (size_t)item * kMaxRhs;
-        double *o_rz = part_rz + (size_t)item * kMaxRhs;
-        switch (s.R) {
-        case 1: update_rows<1>(s, wk.y, sv, sh, o_rr, o_rz); break;
-        case 2: update_rows<2>(s, wk.y, sv, sh, o_rr, o_rz); break;
-        case 4: update_rows<4>(s, wk.y, sv, sh, o_rr, o_rz); break;
-        case 6: update_rows<6>(s, wk.y, sv, sh, o_rr, o_rz); break;
-        default: update_rows<8>(s, wk.y, sv, sh, o_rr, o_rz); break;
-        }
+    const int2 wk = work[blockIdx.x];
+    const BatchSys s = sys[wk.x];
+    load_scal(sc, s.slot, s.R, sv);
+    if (!sv.any_active) return;
+    double *o_rr = part_rr + (size_t)blockIdx.x * kMaxRhs;
+    double *o_rz = part_rz + (size_t)blockIdx.x * kMaxRhs;
+    switch (s.R) {
+    case 1: update_rows<1>(s, wk.y, sv, sh, o_rr, o_rz); break;
+    case 2: update_rows<2>(s, wk.y, sv, sh, o_rr, o_rz); break;
+    case 4: update_rows<4>(s, wk.y, sv, sh, o_rr, o_rz); break;
+    case 6: update_rows<6>(s, wk.y, sv, sh, o_rr, o_rz); break;
+    default: update_rows<8>(s, wk.y, sv, sh, o_rr, o_rz); break;
     }
 }
 
@@ -555,24 +546,21 @@ __device__ __forceinline__ void dir_rows(const BatchSys &s, int row0, const Scal
 }
 
 // (restart_only: p = D r where the refresh asked for a restart)
-__global__ __launch_bounds__(256) void k_b_dir(const int2 *__restrict__ work, int nwork,
+__global__ __launch_bounds__(256) void k_b_dir(const int2 *__restrict__ work,
                                                const BatchSys *__restrict__ sys,
                                                const BatchScal *__restrict__ sc, int restart_only)
 {
     __shared__ ScalView sv;
-    for (int item = blockIdx.x; item < nwork; item += gridDim.x) {
-        __syncthreads();
-        const int2 wk = work[item];
-        const BatchSys s = sys[wk.x];
-        load_scal(sc, s.slot, s.R, sv);
-        if (!sv.any_active) continue;
-        switch (s.R) {
-        case 1: dir_rows<1>(s, wk.y, sv, restart_only != 0); break;
-        case 2: dir_rows<2>(s, wk.y, sv, restart_only != 0); break;
-        case 4: dir_rows<4>(s, wk.y, sv, restart_only != 0); break;
-        case 6: dir_rows<6>(s, wk.y, sv, restart_only != 0); break;
-        default: dir_rows<8>(s, wk.y, sv, restart_only != 0); break;
-        }
+    const int2 wk = work[blockIdx.x];
+    const BatchSys s = sys[wk.x];
+    load_scal(sc, s.slot, s.R, sv);
+    if (!sv.any_active) return;
+    switch (s.R) {
+    case 1: dir_rows<1>(s, wk.y, sv, restart_only != 0); break;
+    case 2: dir_rows<2>(s, wk.y, sv, restart_only != 0); break;
+    case 4: dir_rows<4>(s, wk.y, sv, restart_only != 0); break;
+    case 6: dir_rows<6>(s, wk.y, sv, restart_only != 0); break;
+    default: dir_rows<8>(s, wk.y, sv, restart_only != 0); break;
     }
 }
 
@@ -660,7 +648,7 @@ __device__ __forceinline__ void resid_item(const BatchSys &s, int slice0, const 
     block_sums<R>(cdd, sh, o_dd);
 }
 
-__global__ __launch_bounds__(256) void k_b_resid(const int2 *__restrict__ work, int nwork,
+__global__ __launch_bounds__(256) void k_b_resid(const int2 *__restrict__ work,
                                                  const BatchSys *__restrict__ sys,
                                                  const BatchScal *__restrict__ sc,
                                                  double *__restrict__ part_rr,
@@ -670,27 +658,24 @@ __global__ __launch_bounds__(256) void k_b_resid(const int2 *__restrict__ work, 
 {
     __shared__ double sh[4 * kMaxRhs];
     __shared__ ScalView sv;
-    for (int item = blockIdx.x; item < nwork; item += gridDim.x) {
-        __syncthreads();
-        const int2 wk = work[item];
-        const BatchSys s = sys[wk.x];
-        load_scal(sc, s.slot, s.R, sv);
-        bool live = false, fresh = true;
-        for (int r = 0; r < s.R; ++r) {
-            const int stt = sv.state[r];
-            live = live || stt == kActive || stt == kCandidate || stt == kFresh;
-            fresh = fresh && (stt == kFresh || stt == kUnused);
-        }
-        if (!live) continue;
-        const size_t o = (size_t)item * kMaxRhs;
-        const int slice0 = wk.y / 64;
-        switch (s.R) {
-        case 1: resid_item<1>(s, slice0, sv, fresh, sh, part_rr + o, part_rz + o, part_bb + o, part_dd + o); break;
-        case 2: resid_item<2>(s, slice0, sv, fresh, sh, part_rr + o, part_rz + o, part_bb + o, part_dd + o); break;
-        case 4: resid_item<4>(s, slice0, sv, fresh, sh, part_rr + o, part_rz + o, part_bb + o, part_dd + o); break;
-        case 6: resid_item<6>(s, slice0, sv, fresh, sh, part_rr + o, part_rz + o, part_bb + o, part_dd + o); break;
-        default: resid_item<8>(s, slice0, sv, fresh, sh, part_rr + o, part_rz + o, part_bb + o, part_dd + o); break;
-        }
+    const int2 wk = work[blockIdx.x];
+    const BatchSys s = sys[wk.x];
+    load_scal(sc, s.slot, s.R, sv);
+    bool live = false, fresh = true;
+    for (int r = 0; r < s.R; ++r) {
+        const int stt = sv.state[r];
+        live = live || stt == kActive || stt == kCandidate || stt == kFresh;
+        fresh = fresh && (stt == kFresh || stt == kUnused);
+    }
+    if (!live) return;
+    const size_t o = (size_t)blockIdx.x * kMaxRhs;
+    const int slice0 = wk.y / 64;
+    switch (s.R) {
+    case 1: resid_item<1>(s, slice0, sv, fresh, sh, part_rr + o, part_rz + o, part_bb + o, part_dd + o); break;
+    case 2: resid_item<2>(s, slice0, sv, fresh, sh, part_rr + o, part_rz + o, part_bb + o, part_dd + o); break;
+    case 4: resid_item<4>(s, slice0, sv, fresh, sh, part_rr + o, part_rz + o, part_bb + o, part_dd + o); break;
+    case 6: resid_item<6>(s, slice0, sv, fresh, sh, part_rr + o, part_rz + o, part_bb + o, part_dd + o); break;
+    default: resid_item<8>(s, slice0, sv, fresh, sh, part_rr + o, part_rz + o, part_bb + o, part_dd + o); break;
     }
 }
 
@@ -812,13 +797,11 @@ struct BatchGroup {
     std::vector<int2> h_work;
     int nwork = 0, nsys = 0;
     size_t lds = 0;         // dynamic LDS of k_b_spmv: the largest tile of a tiled system
-    int g_spmv = 0, g_update = 0, g_dir = 0, g_resid = 0; // grids of the per-item launches
     int64_t generation = 0; // bumped whenever the tables change
     hipStream_t stream = nullptr;
     hipEvent_t done = nullptr;
 
-    void set(const Context *ctx, hipStream_t s, const std::vector<BatchSys> &sys,
-             const std::vector<int2> &work)
+    void set(hipStream_t s, const std::vector<BatchSys> &sys, const std::vector<int2> &work)
     {
         if (d_sys.p && sys.size() == h_sys.size() && work.size() == h_work.size() &&
             std::memcmp(sys.data(), h_sys.data(), sizeof(BatchSys) * sys.size()) == 0 &&
@@ -841,11 +824,6 @@ struct BatchGroup {
         lds = 0;
         for (const BatchSys &b : sys)
             if (b.tiled) lds = std::max(lds, tile_lds_bytes(b));
-        const int W = std::max(1, nwork);
-        g_spmv = resident_grid(ctx, (const void *)k_b_spmv, W, lds);
-        g_update = resident_grid(ctx, (const void *)k_b_update, W, 0);
-        g_dir = resident_grid(ctx, (const void *)k_b_dir, W, 0);
-        g_resid = resident_grid(ctx, (const void *)k_b_resid, W, 0);
     }
 
     // enqueue one PCG iteration of the group's systems
@@ -853,13 +831,12 @@ struct BatchGroup {
     {
         const int W = nwork, NS = nsys;
         hipStream_t s = stream;
-        hipLaunchKernelGGL(k_b_spmv, dim3(g_spmv), dim3(256), lds, s, d_work.p, W, d_sys.p, scal,
-                           d_p0.p);
+        hipLaunchKernelGGL(k_b_spmv, dim3(W), dim3(256), lds, s, d_work.p, d_sys.p, scal, d_p0.p);
         hipLaunchKernelGGL(k_b_alpha, dim3(NS), dim3(256), 0, s, d_sys.p, scal, d_p0.p);
-        hipLaunchKernelGGL(k_b_update, dim3(g_update), dim3(256), 0, s, d_work.p, W, d_sys.p,
-                           scal, d_p0.p, d_p1.p);
+        hipLaunchKernelGGL(k_b_update, dim3(W), dim3(256), 0, s, d_work.p, d_sys.p, scal, d_p0.p,
+                           d_p1.p);
         hipLaunchKernelGGL(k_b_beta, dim3(NS), dim3(256), 0, s, d_sys.p, scal, d_p0.p, d_p1.p);
-        hipLaunchKernelGGL(k_b_dir, dim3(g_dir), dim3(256), 0, s, d_work.p, W, d_sys.p, scal, 0);
+        hipLaunchKernelGGL(k_b_dir, dim3(W), dim3(256), 0, s, d_work.p, d_sys.p, scal, 0);
     }
 
     // enqueue the FP64 residual refresh (which also starts fresh systems)
@@ -867,11 +844,11 @@ struct BatchGroup {
     {
         const int W = nwork, NS = nsys;
         hipStream_t s = stream;
-        hipLaunchKernelGGL(k_b_resid, dim3(g_resid), dim3(256), 0, s, d_work.p, W, d_sys.p, scal,
-                           d_p0.p, d_p1.p, d_p2.p, d_p3.p);
+        hipLaunchKernelGGL(k_b_resid, dim3(W), dim3(256), 0, s, d_work.p, d_sys.p, scal, d_p0.p,
+                           d_p1.p, d_p2.p, d_p3.p);
         hipLaunchKernelGGL(k_b_refresh, dim3(NS), dim3(256), 0, s, d_sys.p, scal, d_p0.p, d_p1.p,
                            d_p2.p, d_p3.p);
-        hipLaunchKernelGGL(k_b_dir, dim3(g_dir), dim3(256), 0, s, d_work.p, W, d_sys.p, scal, 1);
+        hipLaunchKernelGGL(k_b_dir, dim3(W), dim3(256), 0, s, d_work.p, d_sys.p, scal, 1);
     }
 };
 
@@ -927,7 +904,7 @@ struct BatchEngine {
     {
         nwork = 0;
         for (int g = 0; g < kGroups; ++g) {
-            grp[g].set(ctx, ctx->stream, sys[g], work[g]);
+            grp[g].set(ctx->stream, sys[g], work[g]);
             nwork += grp[g].nwork;
         }
         FFH_HIP(hipStreamSynchronize(ctx->stream));
@@ -1239,12 +1216,10 @@ static int tuned_tiling(ffh_server *S, const ServedRequest *q)
                 sys[0].tiled = v ? A.tile_max : 0;
                 d_sys.upload(sys, s);
                 const size_t lds = tile_lds_bytes(sys[0]);
-                const int nw = (int)work.size();
-                const int grid = resident_grid(ctx, (const void *)k_b_spmv, nw, lds);
                 for (int k = 0; k < 7; ++k) {
                     if (k == 2) FFH_HIP(hipEventRecord(e0, s));
-                    hipLaunchKernelGGL(k_b_spmv, dim3(grid), dim3(256), lds, s, d_work.p, nw,
-                                       d_sys.p, d_sc.p, d_part.p);
+                    hipLaunchKernelGGL(k_b_spmv, dim3((unsigned)work.size()), dim3(256), lds, s,
+                                       d_work.p, d_sys.p, d_sc.p, d_part.p);
                 }
                 FFH_HIP(hipGetLastError());
                 FFH_HIP(hipEventRecord(e1, s));

@@ -374,9 +374,6 @@ void device_slots_cm(Context *ctx, int64_t T, int nl, const int *d_gdc, const DB
 // when a cell would need more than 256 colours
 bool device_colour(Context *ctx, int64_t T, int m, const int *d_inc, int base, int n_ent,
                    std::vector<int> &cstart, std::vector<int> &perm);
-// grid of a 256-thread grid-stride launch of kernel with lds bytes of
-// dynamic LDS, capped at the blocks of it that are resident at once
-int resident_grid(const Context *ctx, const void *kernel, int grid, size_t lds);
 void lagrange_boundary(int d, int ntet, const std::vector<unsigned char> &bf,
                        const std::vector<int> &gd, int n_global,
                        std::vector<int> &constrained);

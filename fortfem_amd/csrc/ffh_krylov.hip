@@ -1096,7 +1096,7 @@ static size_t variant_lds(const DevMatrix &A, int vi, size_t vbytes)
 // grid past that runs its last blocks after the others, so every block's
 // share of the rows would wait on a second round (occupancy per kernel and
 // dynamic LDS size, cached).
-int resident_grid(const Context *ctx, const void *kernel, int grid, size_t lds)
+static int resident_grid(const Context *ctx, const void *kernel, int grid, size_t lds)
 {
     static std::mutex mu;
     static std::map<std::pair<const void *, size_t>, int> cache;
