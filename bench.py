@@ -418,7 +418,7 @@ def served_stream(args):
         "chunk": args.chunk, "slots": args.slots, "precision_policy": args.precision,
         "hip_graph": bool(args.served_graph),
         "switched_to_fp64": srv.set_precision(-1),
-        "batched_spmv_autotune": dict(zip(("system_types", "tiled"), srv.tiling_stats())),
+        "batched_spmv_autotune": dict(zip(("system_types", "tiled", "split"), srv.product_forms())),
         "served_s": round(t_sat, 3), "fp64_s": round(t_base, 3),
     }
 

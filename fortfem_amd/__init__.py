@@ -1437,7 +1437,8 @@ class Server:
 
     def set_tiling(self, mode=-1):
         """Batched SpMV per system type: -1 autotune (default), 0 plain SELL-64,
-        1 LDS column tiles wherever they fit (ffh_server_set_tiling)."""
+        1 LDS column tiles wherever they fit, 2 / 4 / 8 row-split with that
+        many lanes per row wherever R allows it (ffh_server_set_tiling)."""
         self.ctx.check(self._lib.ffh_server_set_tiling(self.handle, mode))
 
     def tiling_stats(self):
@@ -1445,6 +1446,14 @@ class Server:
         n, t = C.c_int(0), C.c_int(0)
         self.ctx.check(self._lib.ffh_server_tiling_stats(self.handle, C.byref(n), C.byref(t)))
         return n.value, t.value
+
+    def product_forms(self):
+        """(system types chosen so far, how many run the tiled product, how
+        many a row-split product) (ffh_server_product_forms)."""
+        n, t, l = C.c_int(0), C.c_int(0), C.c_int(0)
+        self.ctx.check(self._lib.ffh_server_product_forms(self.handle, C.byref(n), C.byref(t),
+                                                          C.byref(l)))
+        return n.value, t.value, l.value
 
     def set_precision(self, policy=-1):
         """0 FP32 iterations (switch to FP64 on stagnation), 1 FP64, 2 FP64 for

@@ -57,7 +57,7 @@ EXPORTS = (
     "ffh_server_create", "ffh_server_free", "ffh_server_prepare", "ffh_server_submit", "ffh_server_step",
     "ffh_server_poll", "ffh_server_wait", "ffh_server_stats", "ffh_server_progress",
     "ffh_problem_solution", "ffh_server_set_precision", "ffh_server_use_graph",
-    "ffh_server_set_tiling", "ffh_server_tiling_stats",
+    "ffh_server_set_tiling", "ffh_server_tiling_stats", "ffh_server_product_forms",
     "ffh_solve_sparse_device_noalloc", "ffh_noalloc_spmv_variant", "ffh_problem_colouring",
 )
 
@@ -179,6 +179,7 @@ def load() -> C.CDLL:
         "ffh_server_use_graph": (C.c_int, [vp, C.c_int]),
         "ffh_server_set_tiling": (C.c_int, [vp, C.c_int]),
         "ffh_server_tiling_stats": (C.c_int, [vp, ip, ip]),
+        "ffh_server_product_forms": (C.c_int, [vp, ip, ip, ip]),
         "ffh_solve_sparse_device_noalloc": (C.c_int, [vp, C.c_int, C.c_int64, vp, vp, vp, vp, vp,
                                                       C.POINTER(SolverOptions), C.POINTER(SolverStats)]),
         "ffh_noalloc_spmv_variant": (C.c_int, [C.c_int, C.c_int64]),

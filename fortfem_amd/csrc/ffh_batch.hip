@@ -47,6 +47,8 @@ struct BatchSys {
     const int *tpack;
     const uint16_t *tlidx;
     int tiled;
+    // lanes per row of the row-split product (2, 4 or 8, dividing R), or 0
+    int split;
     const float *val32;    // SELL values of the request
     const double *val64;
     const float *dinv32;
@@ -293,11 +295,119 @@ __device__ __forceinline__ void item_spmv_tiled(const BatchSys &s, int slice0,
     }
 }
 
+// The same product with L lanes per row, each holding Q = R / L of the row's
+// right-hand sides: a wave takes 64 / L rows of its slice per pass (L passes),
+// the L lanes of a row load its index and value at one address and its
+// R-wide row of p as one contiguous run, so a gather touches 64 / L rows of
+// p instead of 64.  Each lane adds its products in the order of item_spmv
+// (the same A p to the bit), and the products p.Ap go through LDS back to
+// one lane per row, so the block sums see what item_spmv gives them.
+template <int R, int L, typename MT, typename VT>
+__device__ __forceinline__ void item_spmv_split(const BatchSys &s, int slice0,
+                                                const MT *__restrict__ val,
+                                                const VT *__restrict__ p, VT *__restrict__ ap,
+                                                double (&d)[R])
+{
+    static_assert(R % L == 0 && 64 % L == 0 && ((R / L) & (R / L - 1)) == 0, "row split");
+    extern __shared__ __align__(16) unsigned char b_dyn_lds[];
+    constexpr int Q = R / L, G = 64 / L, U = 8;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int sl = slice0 + wv;
+    // the wave's 64 rows x R products (4 x 64 x R doubles of dynamic LDS)
+    double *dw = reinterpret_cast<double *>(b_dyn_lds) + wv * 64 * R;
+    const int sub = lane % L, r0 = sub * Q;
+    if (sl < s.nslice) { // uniform in the wave
+        const int o = s.off[sl], w = (s.off[sl + 1] - o) >> 6;
+        for (int ps = 0; ps < L; ++ps) {
+            const int rl = ps * G + lane / L;
+            const int *ip = s.sidx + o + rl;
+            const MT *vp = val + o + rl;
+            VT acc[Q];
+#pragma unroll
+            for (int q = 0; q < Q; ++q) acc[q] = VT(0);
+            int j = 0;
+            for (; j + U <= w; j += U) {
+                int c[U];
+                VT v[U];
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    c[u] = ldm(ip + 64 * u);
+                    v[u] = (VT)ldm(vp + 64 * u);
+                }
+                Pack<VT, Q> pc[U];
+#pragma unroll
+                for (int u = 0; u < U; ++u)
+                    pc[u] = *reinterpret_cast<const Pack<VT, Q> *>(p + (size_t)c[u] * R + r0);
+#pragma unroll
+                for (int u = 0; u < U; ++u)
+#pragma unroll
+                    for (int q = 0; q < Q; ++q) acc[q] += v[u] * pc[u].v[q];
+                ip += 64 * U;
+                vp += 64 * U;
+            }
+            for (; j < w; ++j) {
+                const int c = ldm(ip);
+                const VT v = (VT)ldm(vp);
+                const Pack<VT, Q> pc = *reinterpret_cast<const Pack<VT, Q> *>(p + (size_t)c * R + r0);
+#pragma unroll
+                for (int q = 0; q < Q; ++q) acc[q] += v * pc.v[q];
+                ip += 64;
+                vp += 64;
+            }
+            const int row = s.srow[sl * 64 + rl];
+            if (row >= 0) {
+                const Pack<VT, Q> own =
+                    *reinterpret_cast<const Pack<VT, Q> *>(p + (size_t)row * R + r0);
+                Pack<VT, Q> out;
+#pragma unroll
+                for (int q = 0; q < Q; ++q) {
+                    out.v[q] = acc[q];
+                    dw[rl * R + r0 + q] = (double)own.v[q] * (double)acc[q];
+                }
+                *reinterpret_cast<Pack<VT, Q> *>(ap + (size_t)row * R + r0) = out;
+            } else {
+#pragma unroll
+                for (int q = 0; q < Q; ++q) dw[rl * R + r0 + q] = 0.0;
+            }
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < R; ++r) d[r] = sl < s.nslice ? dw[lane * R + r] : 0.0;
+}
+
+template <int R, int L>
+__device__ __forceinline__ void spmv_split(const BatchSys &s, int slice0, double (&d)[R])
+{
+    if (s.prec)
+        item_spmv_split<R, L, double, double>(s, slice0, s.val64, s.p64, s.ap64, d);
+    else
+        item_spmv_split<R, L, float, float>(s, slice0, s.val32, s.p, s.ap, d);
+}
+
+// lanes per row that the row-split form offers for R right-hand sides
+__host__ __device__ constexpr bool split_ok(int R, int L)
+{
+    return (L == 2 && (R == 2 || R == 4 || R == 8)) || (L == 4 && (R == 4 || R == 8)) ||
+           (L == 8 && R == 8);
+}
+
 template <int R>
 __device__ __forceinline__ void spmv_item(const BatchSys &s, int slice0, double *sh, double *out)
 {
     double d[R];
-    if (s.tiled) {
+    if (split_ok(R, 2) && s.split) {
+        if constexpr (R == 2) {
+            spmv_split<2, 2>(s, slice0, d);
+        } else if constexpr (R == 4) {
+            if (s.split == 4) spmv_split<4, 4>(s, slice0, d);
+            else spmv_split<4, 2>(s, slice0, d);
+        } else if constexpr (R == 8) {
+            if (s.split == 8) spmv_split<8, 8>(s, slice0, d);
+            else if (s.split == 4) spmv_split<8, 4>(s, slice0, d);
+            else spmv_split<8, 2>(s, slice0, d);
+        }
+    } else if (s.tiled) {
         if (s.prec)
             item_spmv_tiled<R, double, double>(s, slice0, s.val64, s.p64, s.ap64, d);
         else
@@ -779,11 +889,13 @@ __global__ void k_b_interleave(int n, int nrhs, int R, const double *__restrict_
 }
 
 // ----------------------------------------------------------------- engine --
-// Dynamic LDS of a tiled system's product: its largest column tile, R
-// values of its iteration precision per column.
+// Dynamic LDS of a system's product: for the tiled form its largest column
+// tile, R values of its iteration precision per column; for the row-split
+// form the four waves' 64 x R products p.Ap in FP64.
 constexpr size_t kTileLdsCap = 65536;
 size_t tile_lds_bytes(const BatchSys &b)
 {
+    if (b.split) return (size_t)4 * 64 * b.R * sizeof(double);
     return (size_t)b.tiled * (size_t)b.R * (b.prec ? sizeof(double) : sizeof(float));
 }
 
@@ -796,7 +908,7 @@ struct BatchGroup {
     std::vector<BatchSys> h_sys; // what d_sys / d_work hold
     std::vector<int2> h_work;
     int nwork = 0, nsys = 0;
-    size_t lds = 0;         // dynamic LDS of k_b_spmv: the largest tile of a tiled system
+    size_t lds = 0;         // dynamic LDS of k_b_spmv: the largest of its systems' needs
     int64_t generation = 0; // bumped whenever the tables change
     hipStream_t stream = nullptr;
     hipEvent_t done = nullptr;
@@ -822,8 +934,7 @@ struct BatchGroup {
         nwork = (int)work.size();
         nsys = (int)sys.size();
         lds = 0;
-        for (const BatchSys &b : sys)
-            if (b.tiled) lds = std::max(lds, tile_lds_bytes(b));
+        for (const BatchSys &b : sys) lds = std::max(lds, tile_lds_bytes(b));
     }
 
     // enqueue one PCG iteration of the group's systems
@@ -1027,8 +1138,12 @@ struct ffh_server {
     int64_t iterations_run = 0;
     // autotuned product per system type (pattern, R, iteration precision):
     // the largest tile when the tiled product won, else 0
-    std::map<std::tuple<const ffh::Problem *, int, int>, int> tiling;
-    int tiling_mode = -1; // -1 autotune, 0 plain, 1 tiled where the tiles fit
+    // product form per system type: x = tile columns of the tiled form (or
+    // 0), y = lanes per row of the row-split form (or 0)
+    std::map<std::tuple<const ffh::Problem *, int, int>, int2> tiling;
+    // -1 autotune, 0 plain, 1 tiled where the tiles fit, 2 / 4 / 8 row-split
+    // with that many lanes per row where R allows it
+    int tiling_mode = -1;
 };
 
 namespace ffh {
@@ -1174,24 +1289,31 @@ static BatchSys batch_sys(const ServedRequest *q)
 
 // Autotuner of the batched product: for each system type (pattern, R,
 // iteration precision), the first time it is batched, the plain SELL-64
-// product and the one with LDS column tiles (when the pattern's tiles fit
-// kTileLdsCap at R values per column) are timed on that system alone (HIP
-// events, 2 + 5 launches each, its p as it stands) and the faster is kept.
-// Both give the same bits, so the choice changes speed only.
-static int tuned_tiling(ffh_server *S, const ServedRequest *q)
+// product, the one with LDS column tiles (when the pattern's tiles fit
+// kTileLdsCap at R values per column) and the row-split forms that R allows
+// are timed on that system alone (HIP events, 2 + 5 launches each, its p as
+// it stands); the fastest is kept when it beats the plain form by 2 %.  All
+// forms give the same bits, so the choice changes speed only.
+static int2 tuned_form(ffh_server *S, const ServedRequest *q)
 {
     Context *ctx = S->ctx;
     DevMatrix &A = q->sym->reduced.m;
     const auto key = std::make_tuple((const Problem *)q->sym, q->R, q->prec);
     auto it = S->tiling.find(key);
     if (it != S->tiling.end()) return it->second;
-    int pick = 0;
     ensure_sell_tiles(ctx, A);
     const size_t sv = q->prec ? sizeof(double) : sizeof(float);
     const bool fits = A.tiles_ok && A.tile_max > 0 && (size_t)A.tile_max * q->R * sv <= kTileLdsCap;
-    if (fits && S->tiling_mode >= 0) {
-        pick = S->tiling_mode == 1 ? A.tile_max : 0;
-    } else if (fits) {
+    std::vector<int2> cand{make_int2(0, 0)};
+    if (fits) cand.push_back(make_int2(A.tile_max, 0));
+    for (int L = 2; L <= 8; L *= 2)
+        if (split_ok(q->R, L)) cand.push_back(make_int2(0, L));
+    int2 pick = make_int2(0, 0);
+    const int mode = S->tiling_mode;
+    if (mode >= 0) {
+        for (const int2 &c : cand)
+            if ((mode == 1 && c.x > 0) || (mode >= 2 && c.y == mode)) pick = c;
+    } else if (cand.size() > 1) {
         hipStream_t s = ctx->stream;
         BatchSys b = batch_sys(q);
         b.slot = 0;
@@ -1209,11 +1331,12 @@ static int tuned_tiling(ffh_server *S, const ServedRequest *q)
         d_part.alloc(work.size() * kMaxRhs);
         hipEvent_t e0 = nullptr, e1 = nullptr;
         FFH_HIP(hipEventCreate(&e0));
-        float ms[2] = {0.0f, 0.0f};
+        std::vector<float> ms(cand.size(), 0.0f);
         try {
             FFH_HIP(hipEventCreate(&e1));
-            for (int v = 0; v < 2; ++v) {
-                sys[0].tiled = v ? A.tile_max : 0;
+            for (size_t v = 0; v < cand.size(); ++v) {
+                sys[0].tiled = cand[v].x;
+                sys[0].split = cand[v].y;
                 d_sys.upload(sys, s);
                 const size_t lds = tile_lds_bytes(sys[0]);
                 for (int k = 0; k < 7; ++k) {
@@ -1233,7 +1356,10 @@ static int tuned_tiling(ffh_server *S, const ServedRequest *q)
         }
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);
-        if (ms[1] < 0.98f * ms[0]) pick = A.tile_max;
+        size_t best = 0;
+        for (size_t v = 1; v < cand.size(); ++v)
+            if (ms[v] < ms[best]) best = v;
+        if (ms[best] < 0.98f * ms[0]) pick = cand[best];
     }
     S->tiling[key] = pick;
     return pick;
@@ -1255,7 +1381,9 @@ static void rebuild_batch(ffh_server *S)
                      [&](const ServedRequest *a, const ServedRequest *b) { return cost(a) > cost(b); });
     for (ServedRequest *q : order) {
         BatchSys b = batch_sys(q);
-        b.tiled = tuned_tiling(S, q);
+        const int2 form = tuned_form(S, q);
+        b.tiled = form.x;
+        b.split = form.y;
         const int g = (int)(std::min_element(load, load + G) - load);
         load[g] += cost(q);
         BatchEngine::append(b, sys[g], work[g]);
@@ -1581,7 +1709,7 @@ int ffh_server_use_graph(ffh_server *S, int enable)
 int ffh_server_set_tiling(ffh_server *S, int mode)
 {
     if (!S) return FFH_ERR_HANDLE;
-    if (mode < -1 || mode > 1) return FFH_ERR_INVALID;
+    if (!(mode >= -1 && mode <= 2) && mode != 4 && mode != 8) return FFH_ERR_INVALID;
     S->tiling_mode = mode;
     S->tiling.clear();
     S->changed = true;
@@ -1591,13 +1719,21 @@ int ffh_server_set_tiling(ffh_server *S, int mode)
 int ffh_server_tiling_stats(const ffh_server *S, int *types, int *tiled)
 {
     if (!S) return FFH_ERR_HANDLE;
-    int n = 0, t = 0;
+    return ffh_server_product_forms(S, types, tiled, nullptr);
+}
+
+int ffh_server_product_forms(const ffh_server *S, int *types, int *tiled, int *split)
+{
+    if (!S) return FFH_ERR_HANDLE;
+    int n = 0, t = 0, l = 0;
     for (const auto &kv : S->tiling) {
         ++n;
-        t += kv.second > 0;
+        t += kv.second.x > 0;
+        l += kv.second.y > 0;
     }
     if (types) *types = n;
     if (tiled) *tiled = t;
+    if (split) *split = l;
     return FFH_OK;
 }
 

@@ -406,16 +406,22 @@ int ffh_server_set_precision(ffh_server *server, int policy, int64_t *switches);
 int ffh_server_use_graph(ffh_server *server, int enable);
 /* Batched SpMV of each system type (pattern, right-hand sides, iteration
  * precision): mode -1 (the default) times, the first time a type is batched,
- * the plain SELL-64 product against the one that stages each item's column
- * tile of p in LDS (offered when the pattern's tiles fit 64 KiB at R values
- * per column) on that system alone, and keeps the faster; 0 always plain; 1
- * tiled wherever the tiles fit.  Both products give the same bits.  Applies
- * to types chosen from now on.  No reference counterpart (a tuning
- * control). */
+ * the plain SELL-64 product (one lane per row), the one that stages each
+ * item's column tile of p in LDS (offered when the pattern's tiles fit
+ * 64 KiB at R values per column) and the row-split ones (2, 4 or 8 lanes per
+ * row, each lane a power-of-two share of the R right-hand sides) on that
+ * system alone, and keeps the fastest if it beats the plain product by 2 %;
+ * 0 always plain; 1 tiled wherever the tiles fit; 2, 4, 8 row-split with
+ * that many lanes per row wherever R allows it.  All products give the same
+ * bits.  Applies to types chosen from now on.  No reference counterpart (a
+ * tuning control). */
 int ffh_server_set_tiling(ffh_server *server, int mode);
 /* System types chosen so far (*types) and how many of them run the tiled
  * product (*tiled). */
 int ffh_server_tiling_stats(const ffh_server *server, int *types, int *tiled);
+/* The same, with how many of them run a row-split product (*split); any
+ * pointer may be NULL. */
+int ffh_server_product_forms(const ffh_server *server, int *types, int *tiled, int *split);
 
 /* ---- differentiable variants --------------------------------------------- */
 /* spmv_jvp (src/solvers/fortfem_sparse_matrix.f90:60-91): y_dot = A x_dot +

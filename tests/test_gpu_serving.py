@@ -131,29 +131,38 @@ def test_graph_replay_matches_direct_launches(ff):
         assert ca.all()
 
 
-def test_tiled_product_matches_plain(ff):
-    """The batched SpMV with each item's column tile of p staged in LDS adds
-    every row's products in the order of the plain SELL-64 product, so a
-    server forced to the tiled product (wherever the tiles fit), one forced
-    to the plain product and one left to the autotuner return bit-for-bit
-    the same solutions, iteration counts and residuals.  The autotuner
-    records one choice per system type (pattern, right-hand sides, iteration
-    precision)."""
+def test_product_forms_match_plain(ff):
+    """The batched SpMV with each item's column tile of p staged in LDS, and
+    the row-split ones (2, 4 or 8 lanes per row, each lane a share of the
+    right-hand sides, p.Ap products handed back to one lane per row through
+    LDS), add every row's products in the order of the plain SELL-64 product
+    and reduce them in its order, so servers forced to each form (wherever
+    it applies), one forced to the plain product and one left to the
+    autotuner return bit-for-bit the same solutions, iteration counts and
+    residuals.  The autotuner records one choice per system type (pattern,
+    right-hand sides, iteration precision).  CASES holds R = 1, 2, 4, 6 and
+    8 (7 padded), so every split width meets a system it applies to."""
     outs, stats = [], []
-    for mode in (0, 1, -1):
+    modes = (0, 1, 2, 4, 8, -1)
+    for mode in modes:
         srv = ff.Server(budget_bytes=1 << 30, max_systems=16, chunk=8)
         srv.set_tiling(mode)
         reqs = make_requests(ff, srv, seed=7)
         tickets = [srv.submit(r["key"], r["fam"], r["k"], r["coeff"], r["b"], mass=r["mass"]) for r in reqs]
         outs.append(srv.run_until_done(tickets, max_steps=20000))
-        stats.append(srv.tiling_stats())
+        stats.append(srv.product_forms())
+        assert srv.tiling_stats() == stats[-1][:2]
     for other in outs[1:]:
         assert outs[0].keys() == other.keys()
         for t in outs[0]:
             for u, w in zip(outs[0][t], other[t]):
                 assert np.array_equal(u, w)
-    assert stats[0][1] == 0 and stats[1][1] >= 1  # plain: none tiled; forced: tiles fit somewhere
-    assert stats[2][0] == len(CASES) and 0 <= stats[2][1] <= stats[2][0]
+    by = dict(zip(modes, stats))
+    assert by[0][1:] == (0, 0)  # plain: neither tiled nor split
+    assert by[1][1] >= 1 and by[1][2] == 0  # forced tiles fit somewhere
+    assert by[2][2] >= 3 and by[4][2] >= 2 and by[8][2] >= 1  # R = 2, 4, 8 / R = 4, 8 / R = 8
+    n, t, l = by[-1]
+    assert n == len(CASES) and t + l <= n
 
 
 def test_admission_by_memory(ff):
