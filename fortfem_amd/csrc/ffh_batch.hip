@@ -444,60 +444,63 @@ __global__ __launch_bounds__(256) void k_b_spmv(const int2 *__restrict__ work,
     }
 }
 
-// Totals of a system's per-item partials (items w0 .. w1 in order) for each
-// right-hand side; thread r holds total r after the call.
+// Totals of a system's per-item partials (items w0 .. w1) for each
+// right-hand side, in a block of kTotThreads: thread t adds right-hand side
+// t & 7 of items w0 + (t >> 3) + 128 j, K items per round with their loads
+// issued together (one load of a wave covers 8 items x 8 right-hand sides,
+// contiguous), then the 8 lanes of a right-hand side in each wave and the 16
+// waves are added in a fixed order.  Thread r < 8 holds total r after the
+// call; sh holds NQ x 16 x 8 doubles.
+constexpr int kTotThreads = 1024;
 template <int NQ>
 __device__ __forceinline__ void system_totals(const BatchSys &s, const double *const (&part)[NQ],
                                               double (&tot)[NQ], double *sh)
 {
-    double acc[NQ][kMaxRhs];
+    constexpr int K = NQ == 1 ? 8 : NQ == 2 ? 4 : 2;
+    const int r = threadIdx.x & 7, il = threadIdx.x >> 3;
+    double acc[NQ];
 #pragma unroll
-    for (int q = 0; q < NQ; ++q)
+    for (int q = 0; q < NQ; ++q) acc[q] = 0.0;
+    if (r < s.R) {
+        for (int w = s.w0 + il; w < s.w1; w += 128 * K) {
+            double v[K][NQ];
 #pragma unroll
-        for (int r = 0; r < kMaxRhs; ++r) acc[q][r] = 0.0;
-    // four items per thread and round, their loads issued together
-    for (int w = s.w0 + threadIdx.x; w < s.w1; w += 4 * 256) {
-        double v[4][NQ][kMaxRhs];
+            for (int k = 0; k < K; ++k)
 #pragma unroll
-        for (int k = 0; k < 4; ++k)
-#pragma unroll
-            for (int q = 0; q < NQ; ++q)
-#pragma unroll
-                for (int r = 0; r < kMaxRhs; ++r) {
-                    const int wk = w + k * 256;
-                    v[k][q][r] = (r < s.R && wk < s.w1) ? part[q][(size_t)wk * kMaxRhs + r] : 0.0;
+                for (int q = 0; q < NQ; ++q) {
+                    const int wk = w + 128 * k;
+                    v[k][q] = wk < s.w1 ? part[q][(size_t)wk * kMaxRhs + r] : 0.0;
                 }
 #pragma unroll
-        for (int k = 0; k < 4; ++k)
+            for (int k = 0; k < K; ++k)
 #pragma unroll
-            for (int q = 0; q < NQ; ++q)
-#pragma unroll
-                for (int r = 0; r < kMaxRhs; ++r) acc[q][r] += v[k][q][r];
+                for (int q = 0; q < NQ; ++q) acc[q] += v[k][q];
+        }
     }
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
 #pragma unroll
-    for (int q = 0; q < NQ; ++q)
-#pragma unroll
-        for (int r = 0; r < kMaxRhs; ++r) {
-            double v = acc[q][r];
-            for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
-            if (lane == 0) sh[(q * kMaxRhs + r) * 4 + wv] = v;
-        }
+    for (int q = 0; q < NQ; ++q) {
+        double v = acc[q];
+        for (int o = 32; o >= 8; o >>= 1) v += __shfl_down(v, o, 64);
+        if (lane < 8) sh[(q * 16 + wv) * 8 + lane] = v;
+    }
     __syncthreads();
 #pragma unroll
     for (int q = 0; q < NQ; ++q) {
-        const double *t = sh + (q * kMaxRhs + (threadIdx.x & (kMaxRhs - 1))) * 4;
-        tot[q] = t[0] + t[1] + t[2] + t[3];
+        double t = 0.0;
+        if (threadIdx.x < 8)
+            for (int j = 0; j < 16; ++j) t += sh[(q * 16 + j) * 8 + threadIdx.x];
+        tot[q] = t;
     }
     __syncthreads();
 }
 
 // per system: p.Ap -> alpha (one block per system)
-__global__ __launch_bounds__(256) void k_b_alpha(const BatchSys *__restrict__ sys,
-                                                 BatchScal *__restrict__ sc,
-                                                 const double *__restrict__ part)
+__global__ __launch_bounds__(kTotThreads) void k_b_alpha(const BatchSys *__restrict__ sys,
+                                                         BatchScal *__restrict__ sc,
+                                                         const double *__restrict__ part)
 {
-    __shared__ double sh[kMaxRhs * 4];
+    __shared__ double sh[1 * 16 * 8];
     const BatchSys s = sys[blockIdx.x];
     const double *const parts[1] = {part};
     double tot[1];
@@ -587,12 +590,12 @@ __global__ __launch_bounds__(256) void k_b_update(const int2 *__restrict__ work,
 }
 
 // per system: beta, stop tests
-__global__ __launch_bounds__(256) void k_b_beta(const BatchSys *__restrict__ sys,
-                                                BatchScal *__restrict__ sc,
-                                                const double *__restrict__ part_rr,
-                                                const double *__restrict__ part_rz)
+__global__ __launch_bounds__(kTotThreads) void k_b_beta(const BatchSys *__restrict__ sys,
+                                                        BatchScal *__restrict__ sc,
+                                                        const double *__restrict__ part_rr,
+                                                        const double *__restrict__ part_rz)
 {
-    __shared__ double sh[2 * kMaxRhs * 4];
+    __shared__ double sh[2 * 16 * 8];
     const BatchSys s = sys[blockIdx.x];
     const double *const parts[2] = {part_rr, part_rz};
     double tot[2];
@@ -791,14 +794,14 @@ __global__ __launch_bounds__(256) void k_b_resid(const int2 *__restrict__ work,
 
 // per system: true residual -> converged, or continue (reliable update keeps
 // p; a breakdown, or a fresh system, restarts p = D r)
-__global__ __launch_bounds__(256) void k_b_refresh(const BatchSys *__restrict__ sys,
-                                                   BatchScal *__restrict__ sc,
-                                                   const double *__restrict__ part_rr,
-                                                   const double *__restrict__ part_rz,
-                                                   const double *__restrict__ part_bb,
-                                                   const double *__restrict__ part_dd)
+__global__ __launch_bounds__(kTotThreads) void k_b_refresh(const BatchSys *__restrict__ sys,
+                                                           BatchScal *__restrict__ sc,
+                                                           const double *__restrict__ part_rr,
+                                                           const double *__restrict__ part_rz,
+                                                           const double *__restrict__ part_bb,
+                                                           const double *__restrict__ part_dd)
 {
-    __shared__ double sh[4 * kMaxRhs * 4];
+    __shared__ double sh[4 * 16 * 8];
     const BatchSys s = sys[blockIdx.x];
     const double *const parts[4] = {part_rr, part_rz, part_bb, part_dd};
     double tot[4];
@@ -943,10 +946,11 @@ struct BatchGroup {
         const int W = nwork, NS = nsys;
         hipStream_t s = stream;
         hipLaunchKernelGGL(k_b_spmv, dim3(W), dim3(256), lds, s, d_work.p, d_sys.p, scal, d_p0.p);
-        hipLaunchKernelGGL(k_b_alpha, dim3(NS), dim3(256), 0, s, d_sys.p, scal, d_p0.p);
+        hipLaunchKernelGGL(k_b_alpha, dim3(NS), dim3(kTotThreads), 0, s, d_sys.p, scal, d_p0.p);
         hipLaunchKernelGGL(k_b_update, dim3(W), dim3(256), 0, s, d_work.p, d_sys.p, scal, d_p0.p,
                            d_p1.p);
-        hipLaunchKernelGGL(k_b_beta, dim3(NS), dim3(256), 0, s, d_sys.p, scal, d_p0.p, d_p1.p);
+        hipLaunchKernelGGL(k_b_beta, dim3(NS), dim3(kTotThreads), 0, s, d_sys.p, scal, d_p0.p,
+                           d_p1.p);
         hipLaunchKernelGGL(k_b_dir, dim3(W), dim3(256), 0, s, d_work.p, d_sys.p, scal, 0);
     }
 
@@ -957,7 +961,7 @@ struct BatchGroup {
         hipStream_t s = stream;
         hipLaunchKernelGGL(k_b_resid, dim3(W), dim3(256), 0, s, d_work.p, d_sys.p, scal, d_p0.p,
                            d_p1.p, d_p2.p, d_p3.p);
-        hipLaunchKernelGGL(k_b_refresh, dim3(NS), dim3(256), 0, s, d_sys.p, scal, d_p0.p, d_p1.p,
+        hipLaunchKernelGGL(k_b_refresh, dim3(NS), dim3(kTotThreads), 0, s, d_sys.p, scal, d_p0.p, d_p1.p,
                            d_p2.p, d_p3.p);
         hipLaunchKernelGGL(k_b_dir, dim3(W), dim3(256), 0, s, d_work.p, d_sys.p, scal, 1);
     }
