@@ -57,18 +57,68 @@ __device__ __forceinline__ double block_sum(double v)
 }
 
 // Total of partials[0 .. nb) in block order, broadcast to every thread.
+// Thread t adds partials t, t + BS, t + 2 BS, ... in that order; the loads of
+// kSumLoads of them are issued together, so a sum over a few thousand
+// partials costs one or two load latencies, not one per partial.
+constexpr int kSumLoads = 4;
 template <int BS>
 __device__ __forceinline__ double sum_partials(const double *__restrict__ partials, int nb)
 {
     __shared__ double tot;
     double t = 0.0;
-    for (int i = threadIdx.x; i < nb; i += BS) t += partials[i];
+    for (int i0 = threadIdx.x; i0 < nb; i0 += kSumLoads * BS) {
+        double v[kSumLoads];
+#pragma unroll
+        for (int k = 0; k < kSumLoads; ++k) {
+            const int i = i0 + k * BS;
+            v[k] = i < nb ? partials[i] : 0.0;
+        }
+#pragma unroll
+        for (int k = 0; k < kSumLoads; ++k)
+            if (i0 + k * BS < nb) t += v[k];
+    }
     t = block_sum<BS>(t);
     if (threadIdx.x == 0) tot = t;
     __syncthreads();
     const double r = tot;
     __syncthreads();
     return r;
+}
+
+// Totals of two partial arrays (same order as sum_partials for each), with
+// the loads of both issued together.
+template <int BS>
+__device__ __forceinline__ void sum_partials2(const double *__restrict__ pa,
+                                              const double *__restrict__ pb, int nb, double &ta,
+                                              double &tb)
+{
+    __shared__ double tot[2];
+    double a = 0.0, b = 0.0;
+    for (int i0 = threadIdx.x; i0 < nb; i0 += kSumLoads * BS) {
+        double va[kSumLoads], vb[kSumLoads];
+#pragma unroll
+        for (int k = 0; k < kSumLoads; ++k) {
+            const int i = i0 + k * BS;
+            va[k] = i < nb ? pa[i] : 0.0;
+            vb[k] = i < nb ? pb[i] : 0.0;
+        }
+#pragma unroll
+        for (int k = 0; k < kSumLoads; ++k)
+            if (i0 + k * BS < nb) {
+                a += va[k];
+                b += vb[k];
+            }
+    }
+    a = block_sum<BS>(a);
+    b = block_sum<BS>(b);
+    if (threadIdx.x == 0) {
+        tot[0] = a;
+        tot[1] = b;
+    }
+    __syncthreads();
+    ta = tot[0];
+    tb = tot[1];
+    __syncthreads();
 }
 
 __device__ __forceinline__ bool is_breakdown(double pap, double rz, int mode)
@@ -593,8 +643,8 @@ __device__ __forceinline__ bool pcg_finish(const double *__restrict__ partials, 
                                            KrylovCtl *ctl, double &beta)
 {
     const int nb = ctl->nb_rz;
-    const double trr = sum_partials<BS>(partials + stride, nb);
-    const double trz = sum_partials<BS>(partials + 2 * stride, nb);
+    double trr, trz;
+    sum_partials2<BS>(partials + stride, partials + 2 * stride, nb, trr, trz);
     const int done_it = ctl->k2_done;
     const int par = done_it & 1;
     const bool restart = ctl->restart != 0;
@@ -954,8 +1004,8 @@ __global__ __launch_bounds__(kBlock) void k_mixed_refresh_finish(const double *_
                                                                  KrylovCtl *ctl)
 {
     if (ctl->converged) return;
-    const double trr = sum_partials<kBlock>(partials + stride, nb);
-    const double tdd = sum_partials<kBlock>(partials + 3 * stride, nb);
+    double trr, tdd;
+    sum_partials2<kBlock>(partials + stride, partials + 3 * stride, nb, trr, tdd);
     if (threadIdx.x == 0) {
         const double rn = sqrt(trr);
         if (first) {
