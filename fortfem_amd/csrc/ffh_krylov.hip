@@ -21,7 +21,6 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdlib>
-#include <map>
 #include <mutex>
 
 #include "ffh_internal.hpp"
@@ -1146,21 +1145,29 @@ static size_t variant_lds(const DevMatrix &A, int vi, size_t vbytes)
 // grid past that runs its last blocks after the others, so every block's
 // share of the rows would wait on a second round (occupancy per kernel and
 // dynamic LDS size, cached).
+// (a fixed table, so that no call allocates: the allocation-free solve
+// launches through here)
 static int resident_grid(const Context *ctx, const void *kernel, int grid, size_t lds)
 {
+    struct Entry {
+        const void *kernel;
+        size_t lds;
+        int per_cu;
+    };
+    constexpr int kEntries = 256;
     static std::mutex mu;
-    static std::map<std::pair<const void *, size_t>, int> cache;
-    int per_cu = 0;
+    static Entry table[kEntries];
+    static int used = 0;
+    int per_cu = -1;
     {
         std::lock_guard<std::mutex> lock(mu);
-        auto it = cache.find({kernel, lds});
-        if (it != cache.end()) {
-            per_cu = it->second;
-        } else {
+        for (int i = 0; i < used; ++i)
+            if (table[i].kernel == kernel && table[i].lds == lds) per_cu = table[i].per_cu;
+        if (per_cu < 0) {
             if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, kStreamBlock, lds) !=
                 hipSuccess)
                 per_cu = 0;
-            cache[{kernel, lds}] = per_cu;
+            if (used < kEntries) table[used++] = Entry{kernel, lds, per_cu};
         }
     }
     if (per_cu <= 0) return grid;
