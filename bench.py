@@ -611,7 +611,7 @@ def main():
     ap.add_argument("--arrival-factor", type=float, default=0.8,
                     help="served: Poisson arrival rate as a fraction of the saturated request rate")
     ap.add_argument("--slots", type=int, default=64, help="served: batch slots")
-    ap.add_argument("--chunk", type=int, default=16, help="served: iterations per server step")
+    ap.add_argument("--chunk", type=int, default=48, help="served: iterations per server step")
     ap.add_argument("--budget-gb", type=float, default=64.0, help="served: device memory for admitted requests")
     ap.add_argument("--max-iter", type=int, default=200000, help="served: iteration cap per right-hand side")
     ap.add_argument("--precision", type=int, default=0,

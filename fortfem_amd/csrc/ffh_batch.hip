@@ -1406,12 +1406,17 @@ static void run_chunk(ffh_server *S)
     S->iterations_run += S->chunk;
 }
 
-// An FP32-iterating request whose FP64 residual stalls (the stagnation rule
-// of kStallImprove / kStallRefreshes: the FP32 operator cannot resolve
-// further progress) continues in FP64 from its current x: its directions
-// restart after the next refresh.
+// An FP32-iterating request whose FP64 residual stalls (no refresh in a
+// window of kStallIterations iterations brings it below kStallImprove of its
+// best: the FP32 operator cannot resolve further progress) continues in FP64
+// from its current x: its directions restart after the next refresh.  The
+// window is counted in iterations, so that a longer chunk does not keep a
+// stalled request in FP32 longer (at least 4 refreshes; 32 at a chunk of 16,
+// the single solve's window of kStallRefreshes refreshes).
+constexpr int kStallIterations = 512;
 static void watch_stagnation(ffh_server *S, ServedRequest *q)
 {
+    const int window = std::max(4, (kStallIterations + S->chunk - 1) / S->chunk);
     bool stuck = false;
     for (int r = 0; r < q->nrhs; ++r) {
         const BatchScal &c = S->eng.h_scal[(size_t)q->slot * kMaxRhs + r];
@@ -1419,7 +1424,7 @@ static void watch_stagnation(ffh_server *S, ServedRequest *q)
         if (c.true_rr < kStallImprove * q->best_rr[r]) {
             q->best_rr[r] = c.true_rr;
             q->stall[r] = 0;
-        } else if (++q->stall[r] >= kStallRefreshes) {
+        } else if (++q->stall[r] >= window) {
             stuck = true;
         }
     }

@@ -393,8 +393,9 @@ int ffh_server_progress(const ffh_server *server, int64_t ticket, int *state, in
                         double *rel_residual);
 /* Iteration precision of requests admitted from now on: 0 (default) FP32
  * iterations with FP64 refreshes, switching a request to FP64 iterations when
- * its FP64 residual stalls (32 refreshes in a row without falling below 0.9
- * of its best); 1 FP64 iterations; 2 FP64 for Nedelec
+ * its FP64 residual stalls (no refresh in a window of 512 iterations, at
+ * least 4 refreshes, falls below 0.9 of its best); 1 FP64 iterations; 2 FP64
+ * for Nedelec
  * requests and 0 for H1.  policy = -1 only reads *switches (requests moved to
  * FP64 so far). */
 int ffh_server_set_precision(ffh_server *server, int policy, int64_t *switches);
