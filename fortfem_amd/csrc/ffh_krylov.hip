@@ -651,7 +651,13 @@ __device__ __forceinline__ bool pcg_finish(const double *__restrict__ partials, 
     bool stop = false, conv = false;
     if (!restart) {
         conv = !ctl->mixed && rn <= ctl->tol;
-        stop = conv || done_it >= ctl->max_iter;
+        // FP32 iterations of a mixed solve: the recurrence residual reaching
+        // the tolerance ends the chunk at once (the rest of its launches
+        // return); the FP64 refresh that follows decides convergence, and
+        // clears the stop if it does not agree.  The refresh then comes at
+        // this iteration instead of the chunk's end, nothing else changes.
+        const bool candidate = ctl->mixed && rn <= ctl->tol;
+        stop = conv || candidate || done_it >= ctl->max_iter;
     }
     beta = restart ? 0.0 : trz / ctl->rzs[par];
     if (blockIdx.x == 0 && threadIdx.x == 0) {
