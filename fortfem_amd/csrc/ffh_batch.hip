@@ -73,8 +73,11 @@ struct BatchScal {
 constexpr int kAbsTol = 1;
 // BatchScal::restart codes: 1 restart the direction, 2 breakdown (restart
 // after the refresh), 3 converged by the FP64 recurrence, kCapped stopped
-// at the iteration cap (the refresh records the true residual of x)
+// at the iteration cap (the refresh records the true residual of x),
+// kRecurrence32 the FP32 recurrence residual reached the tolerance (the
+// refresh decides on the FP64 residual)
 constexpr int kCapped = 4;
+constexpr int kRecurrence32 = 5;
 
 // R values of one row, moved with vector accesses (R = 6: three pairs)
 template <typename T, int R>
@@ -615,6 +618,12 @@ __global__ __launch_bounds__(kTotThreads) void k_b_beta(const BatchSys *__restri
             if (s.prec && trr <= q.tol2) {
                 q.state = kCandidate;
                 q.restart = 3;
+            } else if (trr <= q.tol2) {
+                // FP32 iteration: the right-hand side stops iterating until
+                // the chunk's FP64 refresh, which converges it or, if the
+                // FP64 residual disagrees, lets it go on from there
+                q.state = kCandidate;
+                q.restart = kRecurrence32;
             } else if (q.iters >= q.max_iter) {
                 // iteration cap: x stays as it is and the chunk's FP64
                 // refresh measures its true residual, which decides
@@ -811,6 +820,9 @@ __global__ __launch_bounds__(kTotThreads) void k_b_refresh(const BatchSys *__res
         const double trr = tot[0], trz = tot[1], tbb = tot[2], tdd = tot[3];
         const bool broke = q.restart == 2;
         const bool done_by_recurrence = q.state == kCandidate && q.restart == 3;
+        // stopped by the FP32 recurrence: its direction missed the last
+        // update, so if it goes on it restarts from D r
+        const bool stale_dir = q.state == kCandidate && q.restart == kRecurrence32;
         q.restart = 0;
         if ((q.state == kActive || q.state == kCandidate || q.state == kFresh) &&
             !(trr <= 1.0e300 && tbb <= 1.0e300)) {
@@ -854,7 +866,7 @@ __global__ __launch_bounds__(kTotThreads) void k_b_refresh(const BatchSys *__res
                 // residual.  The direction is kept while the recurrence
                 // tracked it (drift under 1 % of the residual); after a
                 // breakdown or a larger drift CG restarts from r.
-                q.restart = broke || tdd > 1.0e-4 * trr;
+                q.restart = broke || stale_dir || tdd > 1.0e-4 * trr;
                 q.state = kActive;
                 q.rz = trz;
             }
