@@ -971,6 +971,19 @@ __global__ __launch_bounds__(kBlock) void k_residual_finish(const double *__rest
     if (threadIdx.x == 0) ctl->true_rr = t;
 }
 
+// Mixed precision: the FP32 iterations of a chunk add their steps alpha p to
+// an FP32 correction dx (4 bytes read and written per row and iteration
+// instead of FP64 x's 8); before each refresh x += dx in FP64 and dx = 0, so
+// the refresh, and every result, sees x with the whole chunk's correction.
+__global__ __launch_bounds__(kBlock) void k_fold_dx(int n, double *__restrict__ x,
+                                                    float *__restrict__ dx)
+{
+    for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
+        x[i] += (double)dx[i];
+        dx[i] = 0.0f;
+    }
+}
+
 // Mixed precision, reliable update: the FP64 residual b - A x (FP64 matrix
 // and x) replaces the FP32 recurrence residual; partials of its r.r and
 // r.(D r) go where the next direction launch reads them, with the drift
@@ -2020,6 +2033,7 @@ void solve_mixed(Context *ctx, Matrix &M, const double *b, double *x,
     FFH_HIP(hipMemcpyAsync(w.b.p, b, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
     FFH_HIP(hipMemcpyAsync(w.x.p, x, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
     FFH_HIP(hipMemsetAsync(w.r32.p, 0, sizeof(float) * n, s));
+    FFH_HIP(hipMemsetAsync(w.x32.p, 0, sizeof(float) * n, s)); // dx
     const int use = o.method == 1 && o.preconditioner == 1;
     hipLaunchKernelGGL((k_jacobi<float>), dim3(grid_for(ctx, n, kBlock)), dim3(kBlock), 0, s, n,
                        A.gptr(), A.gidx(), A.gval(), use, w.dinv32.p, (float *)nullptr);
@@ -2035,10 +2049,13 @@ void solve_mixed(Context *ctx, Matrix &M, const double *b, double *x,
     FFH_HIP(hipGetLastError());
     const int chunk = o.check_interval > 0 ? o.check_interval : 16;
     const int prof = ctx->profiling ? 1 : -1;
+    const int gfold = grid_for(ctx, n, kBlock);
     auto enqueue = [&]() {
-        launch_iterations<float, float, double>(ctx, A, w.partials.p, w.ctl.p, chunk, w.x.p,
-                                                w.r32.p, w.p32.p, w.p32b.p, w.ap32.p, w.dinv32.p,
-                                                vi32, vi32sp, prof);
+        // the FP32 iterations accumulate their x steps in dx (w.x32)
+        launch_iterations<float, float, float>(ctx, A, w.partials.p, w.ctl.p, chunk, w.x32.p,
+                                               w.r32.p, w.p32.p, w.p32b.p, w.ap32.p, w.dinv32.p,
+                                               vi32, vi32sp, prof);
+        hipLaunchKernelGGL(k_fold_dx, dim3(gfold), dim3(kBlock), 0, s, n, w.x.p, w.x32.p);
         launch_refresh(ctx, M, vi64, 0);
     };
     double best = 1e300;
