@@ -366,12 +366,17 @@ def served_stream(args):
                     del pending[tid]
         return served, time.perf_counter() - t0
 
+    def progress(msg):
+        print(json.dumps({"progress": "served stream: " + msg}), file=sys.stderr, flush=True)
+
     served, t_sat = serve([0.0] * len(stream))
+    progress("saturated pass %.2f s" % t_sat)
     lat_sat = np.array([served[r["index"]][1] for r in stream]) * 1e3
     rate = args.arrival_factor * len(stream) / t_sat  # requests per second
     gaps = -np.log(1.0 - workloads.uniform(workloads.BASE_SEED + 5 + 77, len(stream))) / rate
     arrivals = np.concatenate([[0.0], np.cumsum(gaps)[:-1]])
     served_p, t_poi = serve(list(arrivals))
+    progress("Poisson pass %.2f s" % t_poi)
     lat = np.array([served_p[r["index"]][1] for r in stream]) * 1e3
 
     # FP64 baseline: one right-hand side at a time on the resident problem
@@ -393,6 +398,7 @@ def served_stream(args):
             iters64 += st.iterations
             x64 = p.solution()[free[(r["N"], r["family"], r["order"])]]
             errs.append(float(np.linalg.norm(x_srv[j] - x64) / max(np.linalg.norm(x64), 1e-300)))
+        progress("FP64 one at a time: request %d done, %.2f s so far" % (r["index"], t_base))
     iters_srv = int(sum(int(served[r["index"]][0][1].sum()) for r in stream))
     conv = all(bool(served[r["index"]][0][2].all()) and bool(served_p[r["index"]][0][2].all()) for r in stream)
     same = all(np.array_equal(served[r["index"]][0][0], served_p[r["index"]][0][0]) for r in stream)
