@@ -346,8 +346,13 @@ def served_stream(args):
         waiting); returns {index: (result, latency_s)} and the wall time."""
         t0 = time.perf_counter()
         pending, served, nxt = {}, {}, 0
+        t_said = 0.0
         while nxt < len(stream) or pending:
             now = time.perf_counter() - t0
+            if now - t_said >= 20.0:  # a line every 20 s on long streams
+                t_said = now
+                print(json.dumps({"progress": "served stream: %d of %d requests done at %.0f s"
+                                  % (len(served), len(stream), now)}), file=sys.stderr, flush=True)
             while nxt < len(stream) and arrivals[nxt] <= now:
                 r = stream[nxt]
                 tk = srv.submit(r["N"], r["family"], r["order"], r["coeff"], r["rhs"], mass=mass(r),
