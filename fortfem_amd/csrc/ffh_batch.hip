@@ -55,6 +55,9 @@ struct BatchSys {
     const double *dinv64;
     const double *b; // n x R
     double *x;       // n x R
+    // FP32 iteration: the steps alpha p of the chunk in FP32 (n x R), added
+    // to x and cleared by k_b_fold before each refresh
+    float *dx;
     float *r, *p, *ap;        // FP32 iteration
     double *r64, *p64, *ap64; // FP64 iteration (prec = 1)
     int n, R, w0, w1, slot, prec, nslice;
@@ -526,18 +529,20 @@ __global__ __launch_bounds__(kTotThreads) void k_b_alpha(const BatchSys *__restr
     }
 }
 
-// x += alpha p, r -= alpha Ap (active right-hand sides), partials of r.r and
-// r.(D r); thread = row, R values per thread
+// x += alpha p (FP64 iteration: x itself; FP32 iteration: the chunk's FP32
+// correction dx), r -= alpha Ap (active right-hand sides), partials of r.r
+// and r.(D r); thread = row, R values per thread
 template <int R, typename VT>
 __device__ __forceinline__ void update_item(const BatchSys &s, int row, const ScalView &sv,
-                                            VT *__restrict__ rv, const VT *__restrict__ pv,
+                                            VT *__restrict__ xv, VT *__restrict__ rv,
+                                            const VT *__restrict__ pv,
                                             const VT *__restrict__ apv, const VT *__restrict__ dinv,
                                             double (&crr)[R], double (&crz)[R])
 {
 #pragma unroll
     for (int r = 0; r < R; ++r) crr[r] = crz[r] = 0.0;
     if (row >= s.n) return;
-    Pack<double, R> x = ld<double, R>(s.x, row);
+    Pack<VT, R> x = ld<VT, R>(xv, row);
     Pack<VT, R> ri = ld<VT, R>(rv, row);
     const Pack<VT, R> pi = ld<VT, R>(pv, row), ai = ld<VT, R>(apv, row);
     const double di = (double)dinv[row];
@@ -545,13 +550,13 @@ __device__ __forceinline__ void update_item(const BatchSys &s, int row, const Sc
     for (int r = 0; r < R; ++r) {
         if (sv.state[r] != kActive) continue;
         const double a = sv.alpha[r];
-        x.v[r] += a * (double)pi.v[r];
+        x.v[r] += (VT)a * pi.v[r];
         ri.v[r] = ri.v[r] - (VT)a * ai.v[r];
         const double rr = (double)ri.v[r];
         crr[r] = rr * rr;
         crz[r] = rr * (di * rr);
     }
-    st<double, R>(s.x, row, x);
+    st<VT, R>(xv, row, x);
     st<VT, R>(rv, row, ri);
 }
 
@@ -562,9 +567,9 @@ __device__ __forceinline__ void update_rows(const BatchSys &s, int row0, const S
     double crr[R], crz[R];
     const int row = row0 + threadIdx.x;
     if (s.prec)
-        update_item<R, double>(s, row, sv, s.r64, s.p64, s.ap64, s.dinv64, crr, crz);
+        update_item<R, double>(s, row, sv, s.x, s.r64, s.p64, s.ap64, s.dinv64, crr, crz);
     else
-        update_item<R, float>(s, row, sv, s.r, s.p, s.ap, s.dinv32, crr, crz);
+        update_item<R, float>(s, row, sv, s.dx, s.r, s.p, s.ap, s.dinv32, crr, crz);
     block_sums<R>(crr, sh, o_rr);
     block_sums<R>(crz, sh, o_rz);
 }
@@ -770,6 +775,42 @@ __device__ __forceinline__ void resid_item(const BatchSys &s, int slice0, const 
     block_sums<R>(cdd, sh, o_dd);
 }
 
+// Before each refresh, FP32-iterating systems: x += dx, dx = 0 (the chunk's
+// steps, accumulated in FP32 by the update launch, folded into the FP64 x
+// that the residual and every result read).  One block per work item, a
+// thread per row, the R values of a row.
+template <int R>
+__device__ __forceinline__ void fold_rows(const BatchSys &s, int row)
+{
+    if (row >= s.n) return;
+    Pack<double, R> x = ld<double, R>(s.x, row);
+    const Pack<float, R> d = ld<float, R>(s.dx, row);
+    Pack<float, R> z;
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        x.v[r] += (double)d.v[r];
+        z.v[r] = 0.0f;
+    }
+    st<double, R>(s.x, row, x);
+    st<float, R>(s.dx, row, z);
+}
+
+__global__ __launch_bounds__(256) void k_b_fold(const int2 *__restrict__ work,
+                                                const BatchSys *__restrict__ sys)
+{
+    const int2 wk = work[blockIdx.x];
+    const BatchSys s = sys[wk.x];
+    if (s.prec || !s.dx) return;
+    const int row = wk.y + threadIdx.x;
+    switch (s.R) {
+    case 1: fold_rows<1>(s, row); break;
+    case 2: fold_rows<2>(s, row); break;
+    case 4: fold_rows<4>(s, row); break;
+    case 6: fold_rows<6>(s, row); break;
+    default: fold_rows<8>(s, row); break;
+    }
+}
+
 __global__ __launch_bounds__(256) void k_b_resid(const int2 *__restrict__ work,
                                                  const BatchSys *__restrict__ sys,
                                                  const BatchScal *__restrict__ sc,
@@ -971,6 +1012,7 @@ struct BatchGroup {
     {
         const int W = nwork, NS = nsys;
         hipStream_t s = stream;
+        hipLaunchKernelGGL(k_b_fold, dim3(W), dim3(256), 0, s, d_work.p, d_sys.p);
         hipLaunchKernelGGL(k_b_resid, dim3(W), dim3(256), 0, s, d_work.p, d_sys.p, scal, d_p0.p,
                            d_p1.p, d_p2.p, d_p3.p);
         hipLaunchKernelGGL(k_b_refresh, dim3(NS), dim3(kTotThreads), 0, s, d_sys.p, scal, d_p0.p, d_p1.p,
@@ -1113,7 +1155,7 @@ struct ServedRequest {
     double mass = 0.0, tol = 1e-10;
     int max_iter = 100000;
     size_t bytes = 0;
-    DBuf<float> val32, dinv32, r, p, ap; // SELL-64 values, FP32 iteration
+    DBuf<float> val32, dinv32, r, p, ap, dx; // SELL-64 values, FP32 iteration
     DBuf<double> val64, dinv, b, x, r64, p64, ap64;
     int family = 0;
     int prec = 0;                 // 0 FP32 iteration, 1 FP64 iteration
@@ -1169,7 +1211,7 @@ namespace ffh {
 // request may switch to FP64)
 static size_t request_bytes(int n, int64_t sell_total, int R)
 {
-    return (size_t)sell_total * 12 + (size_t)n * 12 + (size_t)n * R * (8 + 8 + 12 + 24) + 4096;
+    return (size_t)sell_total * 12 + (size_t)n * 12 + (size_t)n * R * (8 + 8 + 16 + 24) + 4096;
 }
 
 static void alloc_fp64_vectors(ServedRequest *q, hipStream_t s)
@@ -1250,8 +1292,10 @@ static void admit(ffh_server *S, ServedRequest *q)
         q->r.alloc(nr);
         q->p.alloc(nr);
         q->ap.alloc(nr);
+        q->dx.alloc(nr);
         FFH_HIP(hipMemsetAsync(q->r.p, 0, sizeof(float) * nr, s));
         FFH_HIP(hipMemsetAsync(q->p.p, 0, sizeof(float) * nr, s));
+        FFH_HIP(hipMemsetAsync(q->dx.p, 0, sizeof(float) * nr, s));
     }
     q->best_rr.assign(q->nrhs, 1e300);
     q->stall.assign(q->nrhs, 0);
@@ -1292,6 +1336,7 @@ static BatchSys batch_sys(const ServedRequest *q)
     b.r = q->r.p;
     b.p = q->p.p;
     b.ap = q->ap.p;
+    b.dx = q->dx.p;
     b.r64 = q->r64.p;
     b.p64 = q->p64.p;
     b.ap64 = q->ap64.p;
@@ -1453,7 +1498,7 @@ static void watch_stagnation(ffh_server *S, ServedRequest *q)
     }
     S->eng.push(S->ctx, q->slot);
     FFH_HIP(hipStreamSynchronize(s));
-    q->r.reset(), q->p.reset(), q->ap.reset();
+    q->r.reset(), q->p.reset(), q->ap.reset(), q->dx.reset();
     S->changed = true;
     ++S->switches;
 }
