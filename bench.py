@@ -623,6 +623,8 @@ def main():
                     help="served: Poisson arrival rate as a fraction of the saturated request rate")
     ap.add_argument("--slots", type=int, default=64, help="served: batch slots")
     ap.add_argument("--chunk", type=int, default=48, help="served: iterations per server step")
+    ap.add_argument("--check-interval", type=int, default=48,
+                    help="c2/c4: iterations per chunk (the mixed solve's FP64 refresh interval)")
     ap.add_argument("--budget-gb", type=float, default=64.0, help="served: device memory for admitted requests")
     ap.add_argument("--max-iter", type=int, default=200000, help="served: iteration cap per right-hand side")
     ap.add_argument("--precision", type=int, default=0,
@@ -653,7 +655,7 @@ def main():
     t_setup = time.perf_counter() - t_setup
     sizes = p.sizes()
     base = dict(method="pcg", preconditioner="jacobi", tolerance=1e-10, max_iterations=100000,
-                breakdown="relative", check_interval=32, use_graph=args.graph)
+                breakdown="relative", check_interval=args.check_interval, use_graph=args.graph)
     o64 = ff.solver_options(**base)
     omx = ff.solver_options(**base, precision="mixed")
 
