@@ -165,6 +165,34 @@ def test_product_forms_match_plain(ff):
     assert n == len(CASES) and t + l <= n
 
 
+def test_tight_tolerance_fp32_iteration(ff):
+    """At a relative tolerance of 1e-13 the FP32 iterations' recurrence
+    residual can reach the tolerance while the FP64 residual of x has not:
+    such a right-hand side stops until the chunk's FP64 refresh, which then
+    restarts it from D r (its direction missed the last update), and a
+    request whose FP64 residual stalls moves to FP64 iterations.  Every
+    right-hand side still ends converged on the FP64 residual, within 1e-10
+    of the oracle's FP64 solution."""
+    srv = ff.Server(budget_bytes=1 << 30, max_systems=16, chunk=8)
+    srv.set_precision(0)
+    reqs = make_requests(ff, srv, seed=3)
+    tickets = []
+    for r in reqs:
+        tk, nrhs, n = srv.submit(r["key"], r["fam"], r["k"], r["coeff"], r["b"], mass=r["mass"], tol=1e-13)
+        r["ticket"] = tk
+        tickets.append((tk, nrhs, n))
+    res = srv.run_until_done(tickets, max_steps=50000)
+    for r in reqs:
+        x, it, conv, rel = res[r["ticket"]]
+        assert conv.all() and (rel <= 1e-13).all(), (r["fam"], r["k"], rel)
+        _, _, red = oracle_system(ff, r["fam"], r["k"], r["N"], r["coeff"])
+        for j in range(r["b"].shape[0]):
+            xo, st = O.solve_sparse(red, r["b"][j], method="pcg", preconditioner="jacobi", tolerance=1e-14,
+                                    max_iterations=100000, breakdown="relative")
+            err = np.linalg.norm(x[j] - xo) / np.linalg.norm(xo)
+            assert err <= 1e-10, (r["fam"], r["k"], j, err)
+
+
 def test_admission_by_memory(ff):
     v, t = O.unit_box_mesh(4)
     srv = ff.Server(budget_bytes=1 << 30, max_systems=8, chunk=8)
